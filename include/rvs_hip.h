@@ -1,0 +1,187 @@
+/*
+ * rvs_hip.h -- C ABI of the MI355X (gfx950) per-frame view-synthesis path.
+ *
+ * This is the drop-in boundary that replaces the reference's CUDA library
+ * (the headers rvs_cuda_lib/src/<stage>.cuh).  Plain C types only.  Conventions, which differ from
+ * the reference on purpose:
+ *   - every device buffer is owned by the caller: no call allocates, frees or
+ *     consumes its inputs (the reference's wrappers cudaMalloc their outputs and
+ *     cudaFree their inputs, e.g. blend_img.cu:200-206);
+ *   - every call is stream-ordered on the hipStream_t it is given (NULL = the
+ *     default stream) and returns without synchronising, except where stated;
+ *   - parameters travel as kernel arguments: no __constant__ globals, so calls
+ *     on different streams may run concurrently (the reference's
+ *     cudaMemcpyToSymbol parameters make it non-reentrant);
+ *   - results follow the reference's CPU path (rvs/src), which is the parity
+ *     target, not the reference's CUDA kernels (which compute different results);
+ *   - 0 on success, a negative RVS_E_* code otherwise; rvs_last_error() gives a
+ *     message for the calling thread.  No C++ exception crosses this boundary.
+ */
+#ifndef RVS_HIP_H
+#define RVS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* rvs_stream_t;   /* hipStream_t */
+
+#define RVS_OK 0
+#define RVS_E_INVALID (-1)   /* bad argument */
+#define RVS_E_HIP (-2)       /* HIP runtime error */
+#define RVS_E_CAPACITY (-3)  /* a per-tile bin or workspace limit was exceeded; see rvs_last_error */
+#define RVS_E_UNSUPPORTED (-4)
+
+#define RVS_MAX_SOURCES 64
+
+/* Camera intrinsics (Parameters, rvs/src/Parameters.hpp:79-160). */
+typedef struct {
+    int proj;        /* 0 = Perspective, 1 = Equirectangular */
+    int w, h;        /* Parameters::getSize() */
+    float f[2];      /* Focal (perspective) */
+    float p[2];      /* Principle_point minus crop origin (perspective) */
+    float hor[2];    /* Hor_range in degrees (ERP) */
+    float ver[2];    /* Ver_range in degrees (ERP) */
+    int full_hor;    /* Parameters::isFullHorRange() */
+} rvs_cam;
+
+/* Relative pose source -> target (SpaceTransformer::get_rotation/get_translation,
+ * rvs/src/SpaceTransformer.cpp:77-92), row-major R. */
+typedef struct {
+    float R[9];
+    float t[3];
+} rvs_xform;
+
+const char* rvs_last_error(void);
+/* Library version string. */
+const char* rvs_version(void);
+
+/* ------------------------------------------------------------------ host-side
+ * Pure host functions (no GPU): the float semantics of the reference. */
+
+/* R, t as Parameters::getRotationMatrix + SpaceTransformer compute them in float
+ * (Parameters.cpp:67-97,151-155; SpaceTransformer.cpp:77-92). */
+int rvs_relative_pose(const float pos_from[3], const float rot_from[3], const float pos_to[3],
+                      const float rot_to[3], rvs_xform* out);
+/* Number of floats of the ERP unprojection table of a camera: 2*w + 2*h. */
+size_t rvs_erp_table_len(const rvs_cam* cam);
+/* Fill the table on the host: cos(phi_j), sin(phi_j) for every column and
+ * cos(theta_i), sin(theta_i) for every row, computed with the reference's float
+ * expressions (EquirectangularUnprojector.cpp:62-80, rows 0 and h-1 at v = 1e-3
+ * and h - 1e-3, :87-98).  The caller copies it to the device. */
+int rvs_erp_table_fill(const rvs_cam* cam, float* host_out);
+
+/* ---------------------------------------------------------- stage entry points
+ * One call per stage of the reference's rvs_cuda_lib, caller-owned buffers. */
+
+/* Replaces resizeImage + mergeFrame (process_input_frame.cuh:9-11): raw planar
+ * YUV420 (u8 when bitdepth <= 8, else u16) -> interleaved float3 Y,U,V scaled by
+ * color_scale (fl(1/max_level) as image_loading.cpp:134).  out: w*h*3 floats. */
+int rvs_decode_yuv420(const void* y, const void* u, const void* v, int bitdepth, int w, int h,
+                      float color_scale, float* out, rvs_stream_t stream);
+
+/* Replaces formatDepth (process_input_frame.cuh:13): raw depth plane -> metric
+ * depth; NaN where has_invalid and the result is 0. */
+int rvs_decode_depth(const void* raw, int bitdepth, int w, int h, float depth_scale, float znear,
+                     float zfar, int has_invalid, float* out, rvs_stream_t stream);
+
+/* Replaces unprojectERP_projectERP / unprojectERP_projectPerspective + scaleUV
+ * (unproject_project.cuh:12-20, scale_uv.cuh:12), for ERP and perspective
+ * sources and targets: source depth (src->w*src->h) -> target uv (x2) and
+ * target depth, on the source grid.  erp_table: device copy of
+ * rvs_erp_table_fill(src) when src is ERP, else NULL.  rescale = Precision. */
+int rvs_reproject(const rvs_cam* src, const rvs_cam* dst, const rvs_xform* xf, const float* depth,
+                  const float* erp_table, float rescale, float* uv, float* vdepth,
+                  rvs_stream_t stream);
+
+/* Replaces transform_trianglesMethod (transform_triangle.cuh:12-16) with the
+ * CPU semantics of transform.cpp:148-181: tile-binned rasterisation whose
+ * per-pixel winner equals the reference's sequential fold bit for bit.
+ * color: src w*h*3, depth/uv on the source grid (as produced by
+ * rvs_reproject).  Outputs at out_w*out_h: color*3, depth (+inf uncovered),
+ * shape (triangle validity, 0 uncovered).  workspace: rvs_raster_workspace_size. */
+size_t rvs_raster_workspace_size(int out_w, int out_h, uint32_t tile_cap);
+int rvs_raster(const float* color, const float* depth, const float* uv, int in_w, int in_h,
+               int out_w, int out_h, int wrap, float rescale, void* workspace, size_t ws_bytes,
+               uint32_t tile_cap, float* out_color, float* out_depth, float* out_shape,
+               rvs_stream_t stream);
+
+/* Replaces blendImages (blend_img.cuh:12-17) with the CPU semantics of
+ * BlendedViewSimple::blendBasicMode (BlendedView.cpp:180-212): blends one
+ * synthesized view (color, depth, shape) into the accumulator (acc_*), in place.
+ * first != 0 adopts the view (the reference's m_is_empty branch).
+ * acc_mask: u8 per pixel (m_depth_mask). */
+int rvs_blend_step(float* acc_color, float* acc_quality, float* acc_depth, float* acc_validity,
+                   uint8_t* acc_mask, const float* color, const float* depth, const float* shape,
+                   int w, int h, float blending_exp, int first, rvs_stream_t stream);
+
+/* Replaces inpaintImg (inpaint_img.cuh:12) with the CPU semantics of
+ * detail::inpaintBasicMode (inpainting.cpp:104-223): holes are pixels whose
+ * quality is not > 0; each takes the colour of the nearest known pixel chosen
+ * exactly as the reference's alternating raster sweeps choose it.  color is
+ * updated in place.  map_out (optional device int32 w*h*3): source x, source
+ * y, distance.  workspace: rvs_inpaint_workspace_size. */
+size_t rvs_inpaint_workspace_size(int w, int h);
+int rvs_inpaint(float* color, const float* quality, int w, int h, void* workspace,
+                size_t ws_bytes, int32_t* map_out, rvs_stream_t stream);
+
+/* Replaces quantization (process_input_frame.cuh:15) with the CPU semantics of
+ * write_color (image_writing.cpp:142-203): Y = (T)(v*max) for every pixel, U,V
+ * quantised at full resolution then 2x2 averaged as (a+b+c+d+2)>>2. */
+int rvs_encode_yuv420(const float* color, int w, int h, int bitdepth, void* y, void* u, void* v,
+                      rvs_stream_t stream);
+
+/* ------------------------------------------------------ fused per-view path */
+
+/* One input view resident on the device (InputView, rvs/src/View.hpp:156-174). */
+typedef struct {
+    rvs_cam cam;
+    int bitdepth_color, bitdepth_depth;
+    float znear, zfar;        /* Depth_range */
+    int has_invalid;          /* HasInvalidDepth */
+    float pos[3], rot[3];     /* Position, Rotation (yaw, pitch, roll in degrees) */
+    const void *y, *u, *v;    /* device colour planes (u8 or u16) */
+    const void* depth;        /* device depth plane (Y plane only, w*h) */
+    const float* erp_table;   /* device rvs_erp_table_fill(cam) for ERP sources, else NULL */
+} rvs_source;
+
+/* The virtual view to synthesize (VirtualCameraNames[0] after the pose trace). */
+typedef struct {
+    rvs_cam cam;
+    float pos[3], rot[3];
+    int bitdepth;             /* output BitDepthColor */
+    float blending_exp;       /* BlendingFactor */
+    float rescale;            /* Precision; only 1 is supported by this path */
+    float color_scale;        /* fl(1/max_level) of the first loaded view (image_loading.cpp:134) */
+    float depth_scale;        /* idem for depth (image_loading.cpp:193) */
+} rvs_target;
+
+/* Device outputs of one synthesized view; any pointer may be NULL except color. */
+typedef struct {
+    float* color;             /* w*h*3, inpainted (float3 Y,U,V) */
+    float* quality;           /* w*h, blended quality before inpainting */
+    float* depth;             /* w*h, blended depth */
+    float* validity;          /* w*h, blended validity */
+    void *y, *u, *v;          /* encoded YUV420 planes (u8/u16 by target bitdepth) */
+} rvs_view_out;
+
+/* Workspace bytes for rvs_synthesize with n_src sources of at most max_src_px
+ * pixels each. tile_cap = per (source, tile) bin capacity (0 = default). */
+size_t rvs_synth_workspace_size(int n_src, const rvs_target* tgt, uint32_t tile_cap);
+
+/* Pipeline::computeView for one virtual view (Pipeline.cpp:169-301, CPU
+ * semantics): for each source in order, reproject + rasterise + blend, then
+ * inpaint and encode.  Sources must already be resident.  Returns
+ * RVS_E_CAPACITY (and the needed tile_cap through needed_cap, if not NULL)
+ * when a per-tile bin overflowed; this check synchronises the stream. */
+int rvs_synthesize(const rvs_source* srcs, int n_src, const rvs_target* tgt, void* workspace,
+                   size_t ws_bytes, uint32_t tile_cap, const rvs_view_out* out,
+                   uint32_t* needed_cap, rvs_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

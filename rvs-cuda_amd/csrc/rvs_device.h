@@ -1,0 +1,363 @@
+// rvs_device.h -- device-side building blocks of the per-frame synthesis path.
+//
+// Vertex stage (decode depth -> unproject -> affine -> project -> uv scaling),
+// triangle setup, per-pixel barycentric test and the blend step.  Each
+// function restates one reference CPU function with identical float
+// operations (compiled with -ffp-contract=off):
+//   decode_depth      process_input_frame.cu:358-381 (nvcc contracts to one fma)
+//   vertex            EquirectangularUnprojector.cpp:55-85, PerspectiveUnprojector.cpp:59-86,
+//                     SynthesizedView.cpp:68-79, EquirectangularProjector.cpp:60-99,
+//                     PerspectiveProjector.cpp:59-87, SynthesizedView.cpp:326-331
+//   tri_setup         transform.cpp:62-96 (valid_tri, den cull, bbox)
+//   tri_pixel         transform.cpp:105-117
+//   fold_step         transform.cpp:119-142
+//   blend_step        BlendedView.cpp:180-212 + blending.cpp:293-369
+#ifndef RVS_DEVICE_H
+#define RVS_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rvs_math.h"
+
+namespace rvs_dev {
+
+using namespace rvs_math;
+
+// ---------------------------------------------------------------- parameters
+
+// One (source view, target view) pair, everything the vertex stage needs.
+// Constants are computed on the host with the reference's own float
+// expressions so that they carry the reference's bits.
+struct PairGeom {
+    // source grid
+    int sw, sh;
+    int src_erp;                 // 1: ERP source, 0: perspective
+    float sf0, sf1, sp0, sp1;    // perspective source intrinsics
+    const float* tab;            // ERP source: cos(phi)[sw], sin(phi)[sw], cos(theta)[sh], sin(theta)[sh]
+    // raw depth plane (Y plane of the depth file) and its decode parameters
+    const void* depth;
+    int depth_u8;
+    int depth_f32;               // depth already decoded (float plane): the reproject stage API
+    float dscale, znear, zfar;
+    int has_invalid;
+    // raw colour planes
+    const void *py, *pu, *pv;
+    int color_u8;
+    float cscale;
+    // relative pose (SpaceTransformer::get_rotation / get_translation)
+    float R[9];
+    float t[3];
+    // target projection
+    int dst_erp;
+    float df0n, df1n, dp0, dp1;  // perspective target: -f0, -f1, p0, p1
+    float u0, v0, du_dphi, dv_dtheta;  // ERP target constants
+    float sx, sy;                // cv::transform scale (output size / virtual size)
+    int wrap;                    // horizontal wrapping (ERP target with full range)
+    float rescale;               // g_rescale (Precision)
+};
+
+// Precomputed-array vertex source (the raster entry point on identical inputs).
+struct ArrayGeom {
+    int sw, sh;
+    const float* uv;     // sw*sh*2
+    const float* vd;     // sw*sh
+    const float* color;  // sw*sh*3
+    int wrap;
+    float rescale;
+};
+
+// ------------------------------------------------------------ vertex stage
+
+__device__ inline float decode_depth_raw(uint32_t raw, float scale, float znear, float zfar, int has_invalid) {
+    float depth = (float)raw * scale;
+    if (zfar >= 1000.f)
+        depth = znear / depth;
+    else
+        depth = zfar * znear / __builtin_fmaf(depth, (zfar - znear), znear);
+    if (has_invalid && depth == 0) depth = __builtin_nanf("");
+    return depth;
+}
+
+__device__ inline uint32_t load_plane(const void* p, int u8, size_t idx) {
+    return u8 ? (uint32_t)((const uint8_t*)p)[idx] : (uint32_t)((const uint16_t*)p)[idx];
+}
+
+// u', v', target depth of source pixel (i = row, j = column).
+__device__ inline float3 vertex(const PairGeom& g, int i, int j) {
+    const size_t idx = (size_t)i * g.sw + j;
+    const float d = g.depth_f32 ? ((const float*)g.depth)[idx]
+                                : decode_depth_raw(load_plane(g.depth, g.depth_u8, idx), g.dscale, g.znear,
+                                                   g.zfar, g.has_invalid);
+    float x, y, z;
+    if (g.src_erp) {
+        const float cp = g.tab[j], sp = g.tab[g.sw + j];
+        const float ct = g.tab[2 * g.sw + i], st = g.tab[2 * g.sw + g.sh + i];
+        const float a = ct * cp;
+        const float b = ct * sp;
+        x = a * d;
+        y = b * d;
+        z = st * d;
+    } else {
+        if (d > 0.f) {
+            const float uu = (float)j + 0.5f, vv = (float)i + 0.5f;
+            x = d;
+            y = -(d / g.sf0) * (uu - g.sp0);
+            z = -(d / g.sf1) * (vv - g.sp1);
+        } else {
+            x = y = z = __builtin_nanf("");
+        }
+    }
+    float X = 0.f, Y = 0.f, Z = 0.f;
+    X += g.R[0] * x; X += g.R[1] * y; X += g.R[2] * z;
+    Y += g.R[3] * x; Y += g.R[4] * y; Y += g.R[5] * z;
+    Z += g.R[6] * x; Z += g.R[7] * y; Z += g.R[8] * z;
+    X = X + g.t[0];
+    Y = Y + g.t[1];
+    Z = Z + g.t[2];
+    float u, v, vd;
+    if (g.dst_erp) {
+        const float radius = norm_sq_sum_d(X, Y, Z);
+        vd = radius;
+        const float phi = atan2f(Y, X);
+        const float theta = asinf(Z / radius);
+        u = g.u0 + g.du_dphi * phi;
+        v = g.v0 + g.dv_dtheta * theta;
+    } else {
+        if (X > 0.f) {
+            u = g.df0n * Y / X + g.dp0;
+            v = g.df1n * Z / X + g.dp1;
+            vd = X;
+        } else {
+            u = v = vd = __builtin_nanf("");
+        }
+    }
+    return make_float3(g.sx * u + 0.f, g.sy * v + 0.f, vd);
+}
+
+__device__ inline float3 vertex(const ArrayGeom& g, int i, int j) {
+    const size_t idx = (size_t)i * g.sw + j;
+    return make_float3(g.uv[2 * idx], g.uv[2 * idx + 1], g.vd[idx]);
+}
+
+// Decoded colour of a source pixel: Y, and U,V upsampled bilinearly with
+// src = x * (in/out) and no half-pixel offset (process_input_frame.cu:22-81,
+// 211-238).  Weights are 0, 1/2 or 1 for the even sizes used, so the products
+// are exact; the nvcc contraction is restated as fmaf for the general case.
+__device__ inline float3 color(const PairGeom& g, int i, int j) {
+    const int inW = g.sw / 2, inH = g.sh / 2;
+    const float scaleX = (float)inW / g.sw, scaleY = (float)inH / g.sh;
+    const float srcX = j * scaleX, srcY = i * scaleY;
+    const int x1 = (int)srcX, y1 = (int)srcY;
+    const int x2 = (int)fminf((float)(x1 + 1), (float)(inW - 1));
+    const int y2 = (int)fminf((float)(y1 + 1), (float)(inH - 1));
+    const float dx = srcX - x1, dy = srcY - y1;
+    float up[2];
+    const void* planes[2] = {g.pu, g.pv};
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        const float a = (float)load_plane(planes[p], g.color_u8, (size_t)y1 * inW + x1);
+        const float b = (float)load_plane(planes[p], g.color_u8, (size_t)y1 * inW + x2);
+        const float c = (float)load_plane(planes[p], g.color_u8, (size_t)y2 * inW + x1);
+        const float e = (float)load_plane(planes[p], g.color_u8, (size_t)y2 * inW + x2);
+        const float val1 = __builtin_fmaf(a, (1 - dx), b * dx);
+        const float val2 = __builtin_fmaf(c, (1 - dx), e * dx);
+        up[p] = __builtin_fmaf(val1, (1 - dy), val2 * dy);
+    }
+    const float yy = (float)load_plane(g.py, g.color_u8, (size_t)i * g.sw + j);
+    return make_float3(yy * g.cscale, up[0] * g.cscale, up[1] * g.cscale);
+}
+
+__device__ inline float3 color(const ArrayGeom& g, int i, int j) {
+    const size_t idx = (size_t)i * g.sw + j;
+    return make_float3(g.color[3 * idx], g.color[3 * idx + 1], g.color[3 * idx + 2]);
+}
+
+// ------------------------------------------------------------ triangles
+
+// Slots of one source quad (i, j) in the reference's traversal order
+// (transform.cpp:159-176): 0 = T1, 1 = T2, 2/3 = seam triangles (j == 0, wrap).
+// Vertex (row, col) for slot s, corner k.
+__device__ inline void slot_vertex(int slot, int k, int i, int j, int w, int& r, int& c) {
+    // T1: (j,i) (j+1,i) (j,i+1)      T2: (j+1,i+1) (j,i+1) (j+1,i)
+    // S1: (w-1,i) (0,i) (w-1,i+1)    S2: (0,i+1) (w-1,i+1) (0,i)
+    switch (slot) {
+    case 0: r = (k == 2) ? i + 1 : i;           c = (k == 1) ? j + 1 : j; break;
+    case 1: r = (k == 2) ? i : i + 1;           c = (k == 1) ? j : j + 1; break;
+    case 2: r = (k == 2) ? i + 1 : i;           c = (k == 1) ? 0 : w - 1; break;
+    default: r = (k == 0) ? i + 1 : (k == 1 ? i + 1 : i); c = (k == 1) ? w - 1 : 0; break;
+    }
+}
+
+struct Tri {
+    float A0, A1, B0, B1, C0, C1;
+    float dA, dB, dC;
+    float den, tv;
+    int xmin, xmax, ymin, ymax;
+};
+
+// Returns false when colorize_triangle returns before its pixel loop.
+__device__ inline bool tri_setup(Tri& T, float3 a, float3 b, float3 c, int out_w, int out_h, float rescale) {
+    T.A0 = a.x; T.A1 = a.y; T.B0 = b.x; T.B1 = b.y; T.C0 = c.x; T.C1 = c.y;
+    T.dA = a.z; T.dB = b.z; T.dC = c.z;
+    T.den = (T.B1 - T.C1) * (T.A0 - T.C0) + (T.C0 - T.B0) * (T.A1 - T.C1);
+    if (T.den <= 0.f) return false;
+    // valid_tri: stretch = (float) max of three double norms, / g_rescale
+    const float dab = norm_sq_sum_d(T.A0 - T.B0, T.A1 - T.B1, 0.f);
+    const float dac = norm_sq_sum_d(T.A0 - T.C0, T.A1 - T.C1, 0.f);
+    const float dbc = norm_sq_sum_d(T.B0 - T.C0, T.B1 - T.C1, 0.f);
+    float stretch = smax(dbc, smax(dab, dac));
+    stretch /= rescale;
+    float quality = 10000.f - 1000.f * stretch;
+    quality = smax(1.f, quality);
+    quality = smin(10000.f, quality);
+    T.tv = quality;
+    if (T.tv == 0.f) return false;
+    T.xmin = imax(0, cvt_i32(__builtin_floorf(smin(smin(T.A0, T.B0), T.C0))));
+    T.ymin = imax(0, cvt_i32(__builtin_floorf(smin(smin(T.A1, T.B1), T.C1))));
+    T.xmax = imin(out_w - 1, cvt_i32(__builtin_ceilf(smax(smax(T.A0, T.B0), T.C0))));
+    T.ymax = imin(out_h - 1, cvt_i32(__builtin_ceilf(smax(smax(T.A1, T.B1), T.C1))));
+    if (T.ymin > T.ymax || T.xmin > T.xmax) return false;
+    return true;
+}
+
+// Barycentric coordinates at pixel (x, y); returns coverage.
+__device__ inline bool tri_pixel(const Tri& T, int x, int y, float& l1, float& l2, float& l3) {
+    const float offset = 0.5f;
+    const float px = (float)x + offset, py = (float)y + offset;
+    l1 = ((T.B1 - T.C1) * (px - T.C0) + (T.C0 - T.B0) * (py - T.C1)) / T.den;
+    l2 = ((T.C1 - T.A1) * (px - T.C0) + (T.A0 - T.C0) * (py - T.C1)) / T.den;
+    l3 = 1.f - l1 - l2;
+    const float eps = 1e-6f;
+    return l1 >= -eps && l2 >= -eps && l3 >= -eps;
+}
+
+__device__ inline float tri_depth(const Tri& T, float l1, float l2, float l3) {
+    return T.dA * l1 + T.dB * l2 + T.dC * l3;
+}
+
+__device__ inline float3 tri_color(float3 cA, float3 cB, float3 cC, float l1, float l2, float l3) {
+    return make_float3(cA.x * l1 + cB.x * l2 + cC.x * l3,
+                       cA.y * l1 + cB.y * l2 + cC.y * l3,
+                       cA.z * l1 + cB.z * l2 + cC.z * l3);
+}
+
+// A candidate whose score v/d^3 is computed without overflow, underflow or
+// NaN: the fold (ratio^3 * shape < validity) then orders it like v/d^3, up to
+// a relative error far below kAmbiguity.
+__device__ inline bool depth_regular(float d) {
+    return d >= 9.5367431640625e-07f && d <= 1048576.0f;   // [2^-20, 2^20]
+}
+
+// Relative score band inside which two candidates are replayed in order.
+#define RVS_AMBIGUITY 3.814697265625e-06f   /* 2^-18 */
+
+// Per-pixel state of the sequential fold (transform.cpp:119-142).
+struct FoldState {
+    float new_d;
+    float shape;
+    int new_prol;
+    uint32_t seq;    // last accepted candidate (0xffffffff = none)
+};
+
+__device__ inline void fold_init(FoldState& s) {
+    s.new_d = __builtin_inff();
+    s.shape = 0.f;
+    s.new_prol = 1;
+    s.seq = 0xffffffffu;
+}
+
+__device__ inline void fold_step(FoldState& s, float d, float tv, bool prol, uint32_t seq) {
+    const float ratio = d / s.new_d;
+    const bool is_valid = ratio * ratio * ratio * s.shape < tv;
+    if ((is_valid || s.new_prol) && !prol) {
+        s.new_d = d; s.new_prol = 0; s.shape = tv; s.seq = seq;
+    } else if (is_valid && s.new_prol && prol) {
+        s.new_d = d; s.shape = tv; s.seq = seq;
+    }
+}
+
+// ---------------------------------------------------------------- blending
+
+struct BlendState {
+    float3 c;
+    float q, d, v;
+    int mask;   // m_depth_mask (0 or nonzero)
+};
+
+__device__ inline float rvs_pow(float x, float a) {
+    return a == 1.0f ? x : powf(x, a);
+}
+
+// First view: assign(color, depth, quality = shape/depth, validity = shape) and
+// m_depth_mask = depth > 0 (BlendedView.cpp:181-186, SynthesizedView.cpp:440).
+__device__ inline void blend_first(BlendState& s, float3 c, float d, float shape) {
+    s.c = c;
+    s.d = d;
+    s.q = shape / d;
+    s.v = shape;
+    s.mask = d > 0.f;
+}
+
+__device__ inline void blend_step(BlendState& s, float3 c, float d, float shape, float a) {
+    const float q_new = shape / d;
+    const float q_acc = s.q;
+    const bool m_new = d > 0.f;
+    float3 out;
+    float outQ;
+    int outMask;
+    if (a < 0) {
+        out = make_float3(0.f, 1.f, 0.f);
+        outQ = 0.f;
+        outMask = 0;
+        if (q_acc > outQ) { out = s.c; outQ = q_acc; outMask = s.mask; }
+        if (q_new > outQ) { out = c; outQ = q_new; outMask = m_new; }
+    } else {
+        float sumW = 0.0f, inpSum = 0.0f;
+        float3 fin = make_float3(0.f, 0.f, 0.f), inp = make_float3(0.f, 0.f, 0.f);
+        // view 0: the accumulator, view 1: the new view
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const float qi = i == 0 ? q_acc : q_new;
+            const bool mi = i == 0 ? (s.mask != 0) : m_new;
+            const float3 ci = i == 0 ? s.c : c;
+            float quality = qi;
+            const bool isValidDepth = qi > 0 && !mi;
+            if (quality > 0 && isValidDepth) {
+                quality = rvs_pow(quality, a);
+                sumW += quality;
+                fin.x += ci.x * quality; fin.y += ci.y * quality; fin.z += ci.z * quality;
+            } else if (quality > 0 && qi > 0) {
+                quality = rvs_pow(quality, a);
+                inpSum += quality;
+                inp.x += ci.x * quality; inp.y += ci.y * quality; inp.z += ci.z * quality;
+            }
+        }
+        if (sumW == 0) {
+            if (inpSum == 0.0f) {
+                out = make_float3(0.f, 1.f, 0.f);
+                outMask = 1;
+                outQ = 0.0f;
+            } else {
+                const float r = 1.f / inpSum;
+                out = make_float3(inp.x * r, inp.y * r, inp.z * r);
+                outMask = 1;
+                outQ = rvs_pow(inpSum, 1.0f / a);
+            }
+        } else {
+            const float r = 1.f / sumW;
+            out = make_float3(fin.x * r, fin.y * r, fin.z * r);
+            outMask = 0;
+            outQ = rvs_pow(sumW, 1.0f / a);
+        }
+    }
+    if (q_new > q_acc) s.d = d;
+    s.v = (s.v < shape) ? shape : s.v;
+    s.c = out;
+    s.q = outQ;
+    s.mask = outMask;
+}
+
+}  // namespace rvs_dev
+
+#endif

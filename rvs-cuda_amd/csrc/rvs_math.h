@@ -1,0 +1,293 @@
+// rvs_math.h -- float semantics shared by the host library and the gfx950 kernels.
+//
+// The reference's CPU path (an MSVC x64 build on OpenCV) fixes a few semantics
+// that IEEE float code alone does not; they are restated here once and used by
+// every kernel, so the kernels produce the reference's bits:
+//   * std::min / std::max with NaN: min(a,b) = (b < a) ? b : a, max(a,b) = (a < b) ? b : a.
+//   * float -> int is cvttss2si: truncation, INT_MIN when out of range or NaN
+//     (transform.cpp:90-93 feeds floor/ceil of projected coordinates through it).
+//   * cv::norm accumulates squares in double (transform.cpp:63-65,
+//     EquirectangularProjector.cpp:80).  norm_sq_sum_d() returns the exact float
+//     result of that double computation using float-float arithmetic, with an
+//     exact integer path for the rare inputs whose result lies too close to a
+//     float rounding midpoint to be decided in float-float.  No FP64 is used.
+// All code here must be compiled with -ffp-contract=off.
+#ifndef RVS_MATH_H
+#define RVS_MATH_H
+
+#include <stdint.h>
+#include <limits.h>
+
+#include <cmath>
+#include <cstring>
+#if defined(__HIPCC__) || defined(__HIP__)
+#include <hip/hip_runtime.h>
+#define RVS_HD __host__ __device__ inline
+#else
+#define RVS_HD inline
+#endif
+
+namespace rvs_math {
+
+RVS_HD float smin(float a, float b) { return (b < a) ? b : a; }
+RVS_HD float smax(float a, float b) { return (a < b) ? b : a; }
+RVS_HD int imin(int a, int b) { return (b < a) ? b : a; }
+RVS_HD int imax(int a, int b) { return (a < b) ? b : a; }
+
+RVS_HD uint32_t f2u(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __float_as_uint(f);
+#else
+    uint32_t u; std::memcpy(&u, &f, 4); return u;
+#endif
+}
+RVS_HD float u2f(uint32_t u) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __uint_as_float(u);
+#else
+    float f; std::memcpy(&f, &u, 4); return f;
+#endif
+}
+
+RVS_HD bool is_finite(float x) { return (f2u(x) & 0x7f800000u) != 0x7f800000u; }
+RVS_HD bool is_nan(float x) { return (f2u(x) & 0x7fffffffu) > 0x7f800000u; }
+
+// cvttss2si
+RVS_HD int cvt_i32(float x) {
+    if (x >= -2147483648.0f && x < 2147483648.0f) return (int)x;
+    return INT_MIN;
+}
+
+RVS_HD float ffloor(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_floorf(x);
+#else
+    return std::floor(x);
+#endif
+}
+RVS_HD float fceil(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_ceilf(x);
+#else
+    return std::ceil(x);
+#endif
+}
+RVS_HD float fsqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_sqrtf(x);   // correctly rounded (-fhip-fp32-correctly-rounded-divide-sqrt)
+#else
+    return std::sqrt(x);
+#endif
+}
+RVS_HD float ffma(float a, float b, float c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_fmaf(a, b, c);
+#else
+    return std::fma(a, b, c);
+#endif
+}
+RVS_HD float fabs_(float x) { return u2f(f2u(x) & 0x7fffffffu); }
+
+// float -> uint8/uint16 as MSVC emits it (32-bit cvttss2si then truncation)
+RVS_HD uint32_t quantize_u32(float v, unsigned maxv) { return (uint32_t)cvt_i32(v * (float)maxv); }
+
+// ---------------------------------------------------------------------------
+// Exact emulation of  (float) sqrt( ((0.0 + (double)a*a) + (double)b*b) + (double)c*c )
+// with IEEE double round-to-nearest-even at every step.
+// ---------------------------------------------------------------------------
+namespace detail {
+
+typedef unsigned __int128 u128;
+
+RVS_HD int clz64(uint64_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __clzll((long long)x);
+#else
+    return __builtin_clzll(x);
+#endif
+}
+RVS_HD int bitlen128(u128 x) {
+    uint64_t hi = (uint64_t)(x >> 64), lo = (uint64_t)x;
+    if (hi) return 128 - clz64(hi);
+    if (lo) return 64 - clz64(lo);
+    return 0;
+}
+
+// value = M * 2^E with M normalised to [2^63, 2^64), or M = 0
+struct XF { uint64_t M; int E; };
+
+RVS_HD XF xf_from_float_sq(float x) {
+    uint32_t b = f2u(x) & 0x7fffffffu;
+    uint32_t ex = b >> 23, mant = b & 0x7fffffu;
+    uint64_t m; int e;
+    if (ex == 0) { m = mant; e = -149; } else { m = mant | 0x800000u; e = (int)ex - 150; }
+    XF r;
+    if (m == 0) { r.M = 0; r.E = 0; return r; }
+    uint64_t sq = m * m;                 // < 2^48, exact
+    int s = clz64(sq);
+    r.M = sq << s;
+    r.E = 2 * e - s;
+    return r;
+}
+
+// Round the 128-bit value S * 2^E (plus a sticky bit for nonzero bits below) to
+// `prec` significant bits, round-to-nearest-even; returns normalised XF.
+RVS_HD XF xf_round(u128 S, int E, bool sticky, int prec) {
+    XF r;
+    int L = bitlen128(S);
+    if (L == 0) { r.M = 0; r.E = 0; return r; }
+    if (L > prec) {
+        int drop = L - prec;
+        u128 kept = S >> drop;
+        u128 rem = S - (kept << drop);
+        u128 half = (u128)1 << (drop - 1);
+        bool up = (rem > half) || (rem == half && (sticky || (kept & 1)));
+        if (rem == half && sticky) up = true;
+        if (up) kept += 1;
+        S = kept;
+        E += drop;
+        L = bitlen128(S);
+    }
+    // normalise to bit 63
+    r.M = (uint64_t)(S << (64 - L));
+    r.E = E - (64 - L);
+    return r;
+}
+
+// RN53(X + Y), X, Y >= 0
+RVS_HD XF xf_add_rn53(XF X, XF Y) {
+    if (X.M == 0) return Y;
+    if (Y.M == 0) return X;
+    if (X.E < Y.E) { XF t = X; X = Y; Y = t; }
+    int shift = X.E - Y.E;
+    u128 sx = (u128)X.M << 62;
+    u128 sy = (u128)Y.M << 62;
+    bool sticky = false;
+    if (shift >= 126) {
+        sticky = true; sy = 0;
+    } else if (shift > 0) {
+        u128 kept = sy >> shift;
+        sticky = (kept << shift) != sy;
+        sy = kept;
+    }
+    return xf_round(sx + sy, X.E - 62, sticky, 53);
+}
+
+// RN53(sqrt(X))
+RVS_HD XF xf_sqrt_rn53(XF X) {
+    if (X.M == 0) return X;
+    int k = (X.E & 1) ? 49 : 48;          // X.E - k even
+    u128 N = (u128)X.M << k;              // < 2^113
+    int Eh = (X.E - k) / 2;
+    // digit-by-digit integer square root
+    u128 op = N, res = 0, one = (u128)1 << 112;
+    while (one > op) one >>= 2;
+    while (one != 0) {
+        if (op >= res + one) { op -= res + one; res = (res >> 1) + one; }
+        else res >>= 1;
+        one >>= 2;
+    }
+    return xf_round(res, Eh, op != 0, 53);
+}
+
+// RN24(X) as a float (with subnormal and overflow handling)
+RVS_HD float xf_to_float(XF X) {
+    if (X.M == 0) return 0.0f;
+    // value = M * 2^E, M in [2^63, 2^64): exponent of leading bit = E + 63
+    int lead = X.E + 63;
+    int prec = 24;
+    if (lead < -126) prec = 24 - (-126 - lead);   // subnormal precision
+    if (prec <= 0) {
+        // below half the smallest subnormal (or exactly, handled by RNE on 1 bit)
+        if (prec == 0) {
+            // value in [2^-150, 2^-149): rounds to 2^-149 unless exactly 2^-150
+            bool exact_half = (X.M == 0x8000000000000000ull);
+            return exact_half ? 0.0f : u2f(1u);
+        }
+        return 0.0f;
+    }
+    XF r = xf_round((u128)X.M, X.E, false, prec);
+    int rlead = r.E + 63;
+    if (rlead > 127) return u2f(0x7f800000u);
+    // assemble float bits
+    uint32_t bits;
+    if (rlead >= -126) {
+        uint32_t mant = (uint32_t)(r.M >> 40) & 0x7fffffu;   // 24 bits incl. hidden
+        bits = ((uint32_t)(rlead + 127) << 23) | mant;
+    } else {
+        // subnormal: value = m * 2^-149
+        int sh = -149 - r.E;                  // r.M >> sh gives m
+        bits = (sh >= 64) ? 0u : (uint32_t)(r.M >> sh);
+    }
+    return u2f(bits);
+}
+
+RVS_HD float norm_slow(float a, float b, float c) {
+    XF A = xf_from_float_sq(a), B = xf_from_float_sq(b), Cc = xf_from_float_sq(c);
+    XF s = xf_add_rn53(xf_add_rn53(A, B), Cc);
+    return xf_to_float(xf_sqrt_rn53(s));
+}
+
+// error-free transforms
+RVS_HD void two_sum(float a, float b, float& s, float& e) {
+    s = a + b;
+    float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+}  // namespace detail
+
+// (float)sqrt(((0.0 + (double)a*a) + (double)b*b) + (double)c*c), exactly.
+RVS_HD float norm_sq_sum_d(float a, float b, float c) {
+    using namespace detail;
+    float aa = fabs_(a), ab = fabs_(b), ac = fabs_(c);
+    if (is_nan(a) || is_nan(b) || is_nan(c)) return a + b + c;  // NaN
+    if (!is_finite(aa) || !is_finite(ab) || !is_finite(ac)) return u2f(0x7f800000u);
+    float mx = smax(aa, smax(ab, ac));
+    if (mx == 0.0f) return 0.0f;
+    const float lo_lim = 9.094947e-13f;   // 2^-40
+    const float hi_lim = 1.0995116e12f;   // 2^40
+    bool ok = mx < hi_lim && (aa == 0.0f || aa >= lo_lim) && (ab == 0.0f || ab >= lo_lim) &&
+              (ac == 0.0f || ac >= lo_lim);
+    if (!ok) return norm_slow(a, b, c);
+    // exact squares
+    float pa = aa * aa, ea = ffma(aa, aa, -pa);
+    float pb = ab * ab, eb = ffma(ab, ab, -pb);
+    float pc = ac * ac, ec = ffma(ac, ac, -pc);
+    // float-float sum
+    float s1, t1;
+    two_sum(pa, pb, s1, t1);
+    float lo = t1 + (ea + eb);
+    float s2, t2;
+    two_sum(s1, pc, s2, t2);
+    lo = lo + (t2 + ec);
+    float shi = s2 + lo;
+    float slo = lo - (shi - s2);
+    // float-float sqrt
+    float r = fsqrt(shi);
+    float resid = ffma(-r, r, shi);       // exact
+    float rlo = (resid + slo) / (2.0f * r);
+    // decide RN24(r + rlo)
+    float tol = r * 5.684342e-14f;         // 2^-44 relative
+    uint32_t rb = f2u(r);
+    if (rlo > 0.0f) {
+        float up = u2f(rb + 1u);
+        float half = (up - r) * 0.5f;
+        float d = rlo - half;
+        if (d > tol) return up;
+        if (d < -tol) return r;
+    } else if (rlo < 0.0f) {
+        float dn = u2f(rb - 1u);
+        float half = (r - dn) * 0.5f;
+        float d = -rlo - half;
+        if (d > tol) return dn;
+        if (d < -tol) return r;
+    } else {
+        return r;
+    }
+    return norm_slow(a, b, c);
+}
+
+}  // namespace rvs_math
+
+#endif

@@ -1,0 +1,33 @@
+// rvs_stages.h -- launchers of the element-wise stage kernels.
+#ifndef RVS_STAGES_H
+#define RVS_STAGES_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rvs_device.h"
+
+namespace rvs_k {
+
+constexpr int kGeomBatch = 8;
+
+struct GeomBatch {
+    rvs_dev::PairGeom g[kGeomBatch];
+    int n, offset;
+};
+
+hipError_t launch_decode_color(const void* y, const void* u, const void* v, int u8, int w, int h, float cs,
+                               float* out, hipStream_t s);
+hipError_t launch_decode_depth(const void* raw, int u8, size_t n, float scale, float znear, float zfar,
+                               int has_invalid, float* out, hipStream_t s);
+hipError_t launch_reproject(const rvs_dev::PairGeom& g, float* uv, float* vd, hipStream_t s);
+hipError_t launch_blend_step(float* acc_c, float* acc_q, float* acc_d, float* acc_v, uint8_t* acc_m,
+                             const float* c, const float* d, const float* shape, size_t n, float a, int first,
+                             hipStream_t s);
+hipError_t launch_encode(const float* color, int w, int h, int bitdepth, void* y, void* u, void* v,
+                         hipStream_t s);
+hipError_t launch_store_geoms(const rvs_dev::PairGeom* host, int n, rvs_dev::PairGeom* dst, hipStream_t s);
+
+}  // namespace rvs_k
+
+#endif

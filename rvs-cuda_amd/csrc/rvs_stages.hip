@@ -1,0 +1,142 @@
+// rvs_stages.hip -- element-wise stages: decode, reproject, blend step, encode.
+#include <hip/hip_runtime.h>
+
+#include "rvs_stages.h"
+
+namespace rvs_k {
+
+using namespace rvs_dev;
+
+// read_color_YUV -> resizeImage + mergeFrame (image_loading.cpp:74-145).
+__global__ void k_decode_color(const void* y, const void* u, const void* v, int u8, int w, int h, float cs,
+                               float* __restrict__ out) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (x >= w || r >= h) return;
+    PairGeom g;
+    g.sw = w; g.sh = h; g.py = y; g.pu = u; g.pv = v; g.color_u8 = u8; g.cscale = cs;
+    const float3 c = color(g, r, x);
+    const size_t o = (size_t)r * w + x;
+    out[3 * o] = c.x; out[3 * o + 1] = c.y; out[3 * o + 2] = c.z;
+}
+
+// read_depth_YUV -> formatDepth (image_loading.cpp:147-213).
+__global__ void k_decode_depth(const void* raw, int u8, size_t n, float scale, float znear, float zfar,
+                               int has_invalid, float* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = decode_depth_raw(load_plane(raw, u8, i), scale, znear, zfar, has_invalid);
+}
+
+// SynthesizedView::computeBasicMode stage (a) (SynthesizedView.cpp:315-331).
+__global__ void k_reproject(PairGeom g, float* __restrict__ uv, float* __restrict__ vd) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j >= g.sw || i >= g.sh) return;
+    const float3 r = vertex(g, i, j);
+    const size_t o = (size_t)i * g.sw + j;
+    uv[2 * o] = r.x; uv[2 * o + 1] = r.y;
+    vd[o] = r.z;
+}
+
+// BlendedViewSimple::blendBasicMode on whole maps.
+__global__ void k_blend_step(float* acc_c, float* acc_q, float* acc_d, float* acc_v, uint8_t* acc_m,
+                             const float* __restrict__ c, const float* __restrict__ d,
+                             const float* __restrict__ shape, size_t n, float a, int first) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float3 cv = make_float3(c[3 * i], c[3 * i + 1], c[3 * i + 2]);
+    BlendState s;
+    if (first) {
+        blend_first(s, cv, d[i], shape[i]);
+        s.mask = s.mask ? 255 : 0;
+    } else {
+        s.c = make_float3(acc_c[3 * i], acc_c[3 * i + 1], acc_c[3 * i + 2]);
+        s.q = acc_q[i]; s.d = acc_d[i]; s.v = acc_v[i]; s.mask = acc_m[i];
+        blend_step(s, cv, d[i], shape[i], a);
+    }
+    acc_c[3 * i] = s.c.x; acc_c[3 * i + 1] = s.c.y; acc_c[3 * i + 2] = s.c.z;
+    acc_q[i] = s.q; acc_d[i] = s.d; acc_v[i] = s.v;
+    acc_m[i] = (uint8_t)(s.mask ? (s.mask == 1 ? 1 : 255) : 0);
+}
+
+// write_color (image_writing.cpp:142-203): one thread per 2x2 block.
+template <typename T>
+__global__ void k_encode(const float* __restrict__ color, int w, int h, unsigned maxv, T* __restrict__ Y,
+                         T* __restrict__ U, T* __restrict__ V) {
+    const int bx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int by = blockIdx.y;
+    const int hw = w / 2, hh = h / 2;
+    if (bx >= hw || by >= hh) return;
+    uint32_t su = 0, sv = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int x = 2 * bx + (k & 1), y = 2 * by + (k >> 1);
+        const size_t o = (size_t)y * w + x;
+        const float* c = color + 3 * o;
+        Y[o] = (T)quantize_u32(c[0], maxv);
+        su += (uint32_t)(T)quantize_u32(c[1], maxv);
+        sv += (uint32_t)(T)quantize_u32(c[2], maxv);
+    }
+    U[(size_t)by * hw + bx] = (T)((su + 2) >> 2);
+    V[(size_t)by * hw + bx] = (T)((sv + 2) >> 2);
+}
+
+hipError_t launch_decode_color(const void* y, const void* u, const void* v, int u8, int w, int h, float cs,
+                               float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_decode_color, dim3((w + 255) / 256, h), dim3(256), 0, s, y, u, v, u8, w, h, cs, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_decode_depth(const void* raw, int u8, size_t n, float scale, float znear, float zfar,
+                               int has_invalid, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_decode_depth, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, raw, u8, n, scale, znear,
+                       zfar, has_invalid, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_reproject(const PairGeom& g, float* uv, float* vd, hipStream_t s) {
+    hipLaunchKernelGGL(k_reproject, dim3((g.sw + 255) / 256, g.sh), dim3(256), 0, s, g, uv, vd);
+    return hipGetLastError();
+}
+
+hipError_t launch_blend_step(float* acc_c, float* acc_q, float* acc_d, float* acc_v, uint8_t* acc_m, const float* c,
+                             const float* d, const float* shape, size_t n, float a, int first, hipStream_t s) {
+    hipLaunchKernelGGL(k_blend_step, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, acc_c, acc_q, acc_d, acc_v,
+                       acc_m, c, d, shape, n, a, first);
+    return hipGetLastError();
+}
+
+hipError_t launch_encode(const float* color, int w, int h, int bitdepth, void* y, void* u, void* v, hipStream_t s) {
+    const unsigned maxv = (1u << bitdepth) - 1u;
+    const dim3 grid((w / 2 + 255) / 256, h / 2);
+    if (bitdepth <= 8)
+        hipLaunchKernelGGL(k_encode<uint8_t>, grid, dim3(256), 0, s, color, w, h, maxv, (uint8_t*)y, (uint8_t*)u,
+                           (uint8_t*)v);
+    else
+        hipLaunchKernelGGL(k_encode<uint16_t>, grid, dim3(256), 0, s, color, w, h, maxv, (uint16_t*)y, (uint16_t*)u,
+                           (uint16_t*)v);
+    return hipGetLastError();
+}
+
+// Geometry descriptors are written by a kernel from its arguments, so a view's
+// whole launch sequence is stream-ordered and capturable (no host memcpy).
+__global__ void k_store_geoms(GeomBatch b, PairGeom* dst) {
+    const int i = threadIdx.x;
+    if (i < b.n) dst[b.offset + i] = b.g[i];
+}
+
+hipError_t launch_store_geoms(const PairGeom* host, int n, PairGeom* dst, hipStream_t s) {
+    for (int off = 0; off < n; off += kGeomBatch) {
+        GeomBatch b;
+        b.n = (n - off) < kGeomBatch ? (n - off) : kGeomBatch;
+        b.offset = off;
+        for (int i = 0; i < b.n; ++i) b.g[i] = host[off + i];
+        hipLaunchKernelGGL(k_store_geoms, dim3(1), dim3(64), 0, s, b, dst);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace rvs_k

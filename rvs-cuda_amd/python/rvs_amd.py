@@ -1,0 +1,337 @@
+"""Python binding of the MI355X view-synthesis path (ctypes over the C ABI of
+include/rvs_hip.h and include/rvs_host.h).
+
+Device buffers are torch tensors on the GPU (torch is plumbing here: memory and
+streams); every computation happens in librvs_hip.so.  Loading fails loudly when
+the built libraries are missing: there is no fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_DIR = os.path.join(PKG, "lib")
+HIP_LIB = os.path.join(LIB_DIR, "librvs_hip.so")
+HOST_LIB = os.path.join(LIB_DIR, "librvs_host.so")
+
+RVS_OK = 0
+RVS_E_CAPACITY = -3
+
+
+class RvsCam(C.Structure):
+    _fields_ = [("proj", C.c_int), ("w", C.c_int), ("h", C.c_int),
+                ("f", C.c_float * 2), ("p", C.c_float * 2),
+                ("hor", C.c_float * 2), ("ver", C.c_float * 2), ("full_hor", C.c_int)]
+
+
+class RvsXform(C.Structure):
+    _fields_ = [("R", C.c_float * 9), ("t", C.c_float * 3)]
+
+
+class RvsSource(C.Structure):
+    _fields_ = [("cam", RvsCam), ("bitdepth_color", C.c_int), ("bitdepth_depth", C.c_int),
+                ("znear", C.c_float), ("zfar", C.c_float), ("has_invalid", C.c_int),
+                ("pos", C.c_float * 3), ("rot", C.c_float * 3),
+                ("y", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p), ("depth", C.c_void_p),
+                ("erp_table", C.c_void_p)]
+
+
+class RvsTarget(C.Structure):
+    _fields_ = [("cam", RvsCam), ("pos", C.c_float * 3), ("rot", C.c_float * 3),
+                ("bitdepth", C.c_int), ("blending_exp", C.c_float), ("rescale", C.c_float),
+                ("color_scale", C.c_float), ("depth_scale", C.c_float)]
+
+
+class RvsViewOut(C.Structure):
+    _fields_ = [("color", C.c_void_p), ("quality", C.c_void_p), ("depth", C.c_void_p),
+                ("validity", C.c_void_p), ("y", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p)]
+
+
+class RvsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("rvs error %d: %s" % (code, msg))
+        self.code = code
+
+
+_hip = None
+_host = None
+
+
+def _need(path):
+    if not os.path.exists(path):
+        raise RuntimeError("%s is not built; run __graft_entry__.build() (make -C rvs-cuda_amd/csrc)" % path)
+
+
+def hip():
+    """librvs_hip.so (loaded once)."""
+    global _hip
+    if _hip is None:
+        _need(HIP_LIB)
+        L = C.CDLL(HIP_LIB)
+        vp, i, f, sz, u32 = C.c_void_p, C.c_int, C.c_float, C.c_size_t, C.c_uint32
+        P = C.POINTER
+        sig = {
+            "rvs_last_error": ([], C.c_char_p),
+            "rvs_version": ([], C.c_char_p),
+            "rvs_relative_pose": ([vp, vp, vp, vp, P(RvsXform)], i),
+            "rvs_erp_table_len": ([P(RvsCam)], sz),
+            "rvs_erp_table_fill": ([P(RvsCam), vp], i),
+            "rvs_decode_yuv420": ([vp, vp, vp, i, i, i, f, vp, vp], i),
+            "rvs_decode_depth": ([vp, i, i, i, f, f, f, i, vp, vp], i),
+            "rvs_reproject": ([P(RvsCam), P(RvsCam), P(RvsXform), vp, vp, f, vp, vp, vp], i),
+            "rvs_raster_workspace_size": ([i, i, u32], sz),
+            "rvs_raster": ([vp, vp, vp, i, i, i, i, i, f, vp, sz, u32, vp, vp, vp, vp], i),
+            "rvs_blend_step": ([vp, vp, vp, vp, vp, vp, vp, vp, i, i, f, i, vp], i),
+            "rvs_inpaint_workspace_size": ([i, i], sz),
+            "rvs_inpaint": ([vp, vp, i, i, vp, sz, vp, vp], i),
+            "rvs_encode_yuv420": ([vp, i, i, i, vp, vp, vp, vp], i),
+            "rvs_synth_workspace_size": ([i, P(RvsTarget), u32], sz),
+            "rvs_synthesize": ([P(RvsSource), i, P(RvsTarget), vp, sz, u32, P(RvsViewOut), P(u32), vp], i),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _hip = L
+    return _hip
+
+
+def host():
+    """librvs_host.so (loaded once)."""
+    global _host
+    if _host is None:
+        _need(HOST_LIB)
+        L = C.CDLL(HOST_LIB)
+        L.rvs_scene_render.argtypes = [C.c_int, C.c_uint64, C.POINTER(RvsCam), C.c_void_p, C.c_void_p,
+                                       C.c_int, C.c_int, C.c_float, C.c_float, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_int]
+        L.rvs_scene_render.restype = C.c_int
+        _host = L
+    return _host
+
+
+def check(rc):
+    if rc != RVS_OK:
+        raise RvsError(rc, hip().rvs_last_error().decode())
+    return rc
+
+
+# ------------------------------------------------------------------ helpers
+
+def cam(proj, w, h, f=(0.0, 0.0), p=(0.0, 0.0), hor=(-180.0, 180.0), ver=(-90.0, 90.0), full_hor=None):
+    c = RvsCam()
+    c.proj = 1 if proj in (1, "erp", "Equirectangular") else 0
+    c.w, c.h = int(w), int(h)
+    c.f[0], c.f[1] = f
+    c.p[0], c.p[1] = p
+    c.hor[0], c.hor[1] = hor
+    c.ver[0], c.ver[1] = ver
+    if full_hor is None:
+        full_hor = c.proj == 1 and float(hor[0]) == -180.0 and float(hor[1]) == 180.0
+    c.full_hor = int(bool(full_hor))
+    return c
+
+
+def plane_dtype(bitdepth):
+    return np.uint8 if bitdepth <= 8 else np.uint16
+
+
+def max_level(bitdepth):
+    return (1 << bitdepth) - 1
+
+
+def scale_of(bitdepth):
+    """fl(1.0f / max_level(bitdepth)) as image_loading.cpp:134 computes it."""
+    return float(np.float32(1.0) / np.float32(max_level(bitdepth)))
+
+
+def relative_pose(pos_from, rot_from, pos_to, rot_to):
+    x = RvsXform()
+    a = [np.ascontiguousarray(v, np.float32) for v in (pos_from, rot_from, pos_to, rot_to)]
+    check(hip().rvs_relative_pose(*(v.ctypes.data for v in a), C.byref(x)))
+    return np.array(x.R, np.float32).reshape(3, 3), np.array(x.t, np.float32)
+
+
+def erp_table(c):
+    n = hip().rvs_erp_table_len(C.byref(c))
+    out = np.zeros(n, np.float32)
+    check(hip().rvs_erp_table_fill(C.byref(c), out.ctypes.data))
+    return out
+
+
+def render_scene(kind, seed, c, pos, rot, bd_color, bd_depth, znear, zfar, threads=0):
+    """Synthetic source view: returns raw planes (y, u, v, depth) as numpy arrays."""
+    dt_c, dt_d = plane_dtype(bd_color), plane_dtype(bd_depth)
+    y = np.zeros((c.h, c.w), dt_c)
+    u = np.zeros((c.h // 2, c.w // 2), dt_c)
+    v = np.zeros((c.h // 2, c.w // 2), dt_c)
+    d = np.zeros((c.h, c.w), dt_d)
+    p = np.ascontiguousarray(pos, np.float32)
+    r = np.ascontiguousarray(rot, np.float32)
+    rc = host().rvs_scene_render(kind, seed, C.byref(c), p.ctypes.data, r.ctypes.data, bd_color, bd_depth,
+                                 znear, zfar, y.ctypes.data, u.ctypes.data, v.ctypes.data, d.ctypes.data,
+                                 threads)
+    if rc != 0:
+        raise RuntimeError("rvs_scene_render failed")
+    return y, u, v, d
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return C.c_void_p(stream)
+
+
+# ---------------------------------------------------------- stage wrappers
+# (all tensors are torch CUDA tensors, contiguous)
+
+def decode_yuv420(y, u, v, bitdepth, color_scale, out, stream=None):
+    h, w = y.shape
+    check(hip().rvs_decode_yuv420(_ptr(y), _ptr(u), _ptr(v), bitdepth, w, h, color_scale, _ptr(out),
+                                  _stream(stream)))
+
+
+def decode_depth(raw, bitdepth, depth_scale, znear, zfar, has_invalid, out, stream=None):
+    h, w = raw.shape
+    check(hip().rvs_decode_depth(_ptr(raw), bitdepth, w, h, depth_scale, znear, zfar, int(has_invalid),
+                                 _ptr(out), _stream(stream)))
+
+
+def reproject(src_cam, dst_cam, R, t, depth, erp_tab, rescale, uv, vdepth, stream=None):
+    x = RvsXform()
+    for k in range(9):
+        x.R[k] = float(np.asarray(R, np.float32).ravel()[k])
+    for k in range(3):
+        x.t[k] = float(np.asarray(t, np.float32)[k])
+    check(hip().rvs_reproject(C.byref(src_cam), C.byref(dst_cam), C.byref(x), _ptr(depth), _ptr(erp_tab),
+                              rescale, _ptr(uv), _ptr(vdepth), _stream(stream)))
+
+
+def raster(color, depth, uv, out_w, out_h, wrap, rescale, out_color, out_depth, out_shape, workspace=None,
+           tile_cap=0, stream=None):
+    import torch
+    in_h, in_w = depth.shape
+    need = hip().rvs_raster_workspace_size(out_w, out_h, tile_cap)
+    if workspace is None or workspace.numel() < need:
+        workspace = torch.empty(need, dtype=torch.uint8, device=depth.device)
+    rc = hip().rvs_raster(_ptr(color), _ptr(depth), _ptr(uv), in_w, in_h, out_w, out_h, int(wrap), rescale,
+                          _ptr(workspace), need, tile_cap, _ptr(out_color), _ptr(out_depth), _ptr(out_shape),
+                          _stream(stream))
+    check(rc)
+    return workspace
+
+
+def blend_step(acc, color, depth, shape, exp, first, stream=None):
+    h, w = depth.shape
+    check(hip().rvs_blend_step(_ptr(acc["color"]), _ptr(acc["quality"]), _ptr(acc["depth"]),
+                               _ptr(acc["validity"]), _ptr(acc["mask"]), _ptr(color), _ptr(depth), _ptr(shape),
+                               w, h, exp, int(first), _stream(stream)))
+
+
+def inpaint(color, quality, map_out=None, workspace=None, stream=None):
+    import torch
+    h, w = quality.shape
+    need = hip().rvs_inpaint_workspace_size(w, h)
+    if workspace is None or workspace.numel() < need:
+        workspace = torch.empty(need, dtype=torch.uint8, device=quality.device)
+    check(hip().rvs_inpaint(_ptr(color), _ptr(quality), w, h, _ptr(workspace), need, _ptr(map_out),
+                            _stream(stream)))
+    return workspace
+
+
+def encode_yuv420(color, bitdepth, y, u, v, stream=None):
+    h, w = y.shape
+    check(hip().rvs_encode_yuv420(_ptr(color), w, h, bitdepth, _ptr(y), _ptr(u), _ptr(v), _stream(stream)))
+
+
+# ------------------------------------------------------------- fused path
+
+class DeviceSource:
+    """An input view resident in HBM: raw planes + ERP table (InputView)."""
+
+    def __init__(self, c, pos, rot, bd_color, bd_depth, znear, zfar, planes, has_invalid=True, device="cuda"):
+        import torch
+        self.cam = c
+        self.pos, self.rot = list(pos), list(rot)
+        self.bd_color, self.bd_depth = bd_color, bd_depth
+        self.znear, self.zfar, self.has_invalid = znear, zfar, has_invalid
+        self.planes = [torch.from_numpy(np.ascontiguousarray(p)).to(device) for p in planes]
+        self.table = torch.from_numpy(erp_table(c)).to(device) if c.proj == 1 else None
+
+    def struct(self):
+        s = RvsSource()
+        s.cam = self.cam
+        s.bitdepth_color, s.bitdepth_depth = self.bd_color, self.bd_depth
+        s.znear, s.zfar, s.has_invalid = self.znear, self.zfar, int(self.has_invalid)
+        for k in range(3):
+            s.pos[k], s.rot[k] = self.pos[k], self.rot[k]
+        s.y, s.u, s.v, s.depth = (p.data_ptr() for p in self.planes)
+        s.erp_table = self.table.data_ptr() if self.table is not None else None
+        return s
+
+
+def make_target(c, pos, rot, bitdepth, blending_exp, color_scale, depth_scale, rescale=1.0):
+    t = RvsTarget()
+    t.cam = c
+    for k in range(3):
+        t.pos[k], t.rot[k] = pos[k], rot[k]
+    t.bitdepth, t.blending_exp, t.rescale = bitdepth, blending_exp, rescale
+    t.color_scale, t.depth_scale = color_scale, depth_scale
+    return t
+
+
+class Synthesizer:
+    """One-view-at-a-time synthesis (the plain per-view path): owns a workspace
+    and output buffers for one target size."""
+
+    def __init__(self, sources, target, tile_cap=0, device="cuda", want_maps=True):
+        import torch
+        self.sources = sources
+        self.target = target
+        self.tile_cap = tile_cap
+        self.device = device
+        self.arr = (RvsSource * len(sources))(*[s.struct() for s in sources])
+        w, h = target.cam.w, target.cam.h
+        dt = torch.uint8 if target.bitdepth <= 8 else torch.int16
+        self.out = {
+            "color": torch.empty((h, w, 3), dtype=torch.float32, device=device),
+            "quality": torch.empty((h, w), dtype=torch.float32, device=device) if want_maps else None,
+            "depth": torch.empty((h, w), dtype=torch.float32, device=device) if want_maps else None,
+            "validity": torch.empty((h, w), dtype=torch.float32, device=device) if want_maps else None,
+            "y": torch.empty((h, w), dtype=dt, device=device),
+            "u": torch.empty((h // 2, w // 2), dtype=dt, device=device),
+            "v": torch.empty((h // 2, w // 2), dtype=dt, device=device),
+        }
+        self._alloc_ws()
+
+    def _alloc_ws(self):
+        import torch
+        self.ws_bytes = hip().rvs_synth_workspace_size(len(self.sources), C.byref(self.target), self.tile_cap)
+        self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=self.device)
+
+    def set_pose(self, pos, rot):
+        for k in range(3):
+            self.target.pos[k], self.target.rot[k] = pos[k], rot[k]
+
+    def run(self, stream=None):
+        vo = RvsViewOut()
+        for k in ("color", "quality", "depth", "validity", "y", "u", "v"):
+            setattr(vo, k, self.out[k].data_ptr() if self.out[k] is not None else None)
+        needed = C.c_uint32(0)
+        rc = hip().rvs_synthesize(self.arr, len(self.sources), C.byref(self.target), _ptr(self.ws),
+                                  self.ws_bytes, self.tile_cap, C.byref(vo), C.byref(needed), _stream(stream))
+        if rc == RVS_E_CAPACITY and needed.value > 0:
+            cap = 1
+            while cap < needed.value:
+                cap <<= 1
+            self.tile_cap = cap
+            self._alloc_ws()
+            return self.run(stream)
+        check(rc)
+        return self.out

@@ -1,0 +1,84 @@
+"""Camera rigs of the measured workloads, with synthetic sources.
+
+rig_rects_640:  two perspective 640x480 8-bit sources at lateral offsets +-0.05 m
+                (f = 500, principal point at the centre, 16-bit inverse depth in
+                [1, 10] m) and one 640x480 target between them; scene: rectangles
+                before a background plane (seed 1).
+rig_erp4_1080p: four equirectangular 4096x2048 10-bit sources on a 0.3 m square
+                (16-bit inverse depth in [0.5, 25] m) and a 1920x1080 perspective
+                target with a 90 degree horizontal field of view (f = 960) at the
+                square's centre; scene: spheres in a box (seed 2).
+"""
+import numpy as np
+
+import rvs_amd as R
+
+
+def rig(name, scale=1):
+    """Returns dict(sources=[...], target=dict(...)).  `scale` divides every
+    resolution (and focal length) for small parity cases."""
+    if name == "rects_640":
+        w, h, f = 640 // scale, 480 // scale, 500.0 / scale
+        srcs = []
+        for y in (0.05, -0.05):
+            srcs.append(dict(cam=R.cam(0, w, h, f=(f, f), p=(w / 2, h / 2)), pos=[0.0, y, 0.0], rot=[0.0, 0.0, 0.0],
+                             bd_color=8, bd_depth=16, znear=1.0, zfar=10.0))
+        tgt = dict(cam=R.cam(0, w, h, f=(f, f), p=(w / 2, h / 2)), pos=[0.0, 0.0, 0.0], rot=[0.0, 0.0, 0.0],
+                   bitdepth=8, blending_exp=1.0)
+        return dict(kind=0, seed=1, sources=srcs, target=tgt)
+    if name == "erp4_1080p":
+        sw, sh = 4096 // scale, 2048 // scale
+        tw, th, f = 1920 // scale, 1080 // scale, 960.0 / scale
+        srcs = []
+        for (x, y) in ((0.15, 0.15), (0.15, -0.15), (-0.15, 0.15), (-0.15, -0.15)):
+            srcs.append(dict(cam=R.cam(1, sw, sh), pos=[x, y, 0.0], rot=[0.0, 0.0, 0.0], bd_color=10, bd_depth=16,
+                             znear=0.5, zfar=25.0))
+        tgt = dict(cam=R.cam(0, tw, th, f=(f, f), p=(tw / 2, th / 2)), pos=[0.0, 0.0, 0.0], rot=[0.0, 0.0, 0.0],
+                   bitdepth=10, blending_exp=1.0)
+        return dict(kind=1, seed=2, sources=srcs, target=tgt)
+    raise KeyError(name)
+
+
+def render_sources(rg, threads=0):
+    """Raw planes (y, u, v, depth) for every source of a rig."""
+    out = []
+    for s in rg["sources"]:
+        out.append(R.render_scene(rg["kind"], rg["seed"], s["cam"], s["pos"], s["rot"], s["bd_color"],
+                                  s["bd_depth"], s["znear"], s["zfar"], threads=threads))
+    return out
+
+
+def oracle_sources(rg, planes, ob):
+    """The same sources in the oracle's description (ob = tests.oracle_bind)."""
+    res = []
+    for s, (y, u, v, d) in zip(rg["sources"], planes):
+        c = s["cam"]
+        oc = ob.make_cam(c.proj, c.w, c.h, f=tuple(c.f), p=tuple(c.p), hor=tuple(c.hor), ver=tuple(c.ver),
+                         full_hor=c.full_hor)
+        res.append(dict(cam=oc, pos=s["pos"], rot=s["rot"], bd_color=s["bd_color"], bd_depth=s["bd_depth"],
+                        znear=s["znear"], zfar=s["zfar"], has_invalid=True, y=y, u=u, v=v, depth=d))
+    return res
+
+
+def oracle_cam(c, ob):
+    return ob.make_cam(c.proj, c.w, c.h, f=tuple(c.f), p=tuple(c.p), hor=tuple(c.hor), ver=tuple(c.ver),
+                       full_hor=c.full_hor)
+
+
+def device_sources(rg, planes, device="cuda"):
+    return [R.DeviceSource(s["cam"], s["pos"], s["rot"], s["bd_color"], s["bd_depth"], s["znear"], s["zfar"],
+                           p, device=device) for s, p in zip(rg["sources"], planes)]
+
+
+def device_target(rg, pos=None, rot=None):
+    t = rg["target"]
+    s0 = rg["sources"][0]
+    return R.make_target(t["cam"], pos if pos is not None else t["pos"], rot if rot is not None else t["rot"],
+                         t["bitdepth"], t["blending_exp"], R.scale_of(s0["bd_color"]), R.scale_of(s0["bd_depth"]))
+
+
+def psnr(a, b, maxv):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    mse = np.mean((a - b) ** 2)
+    return float("inf") if mse == 0 else 10.0 * np.log10(maxv * maxv / mse)
