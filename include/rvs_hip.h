@@ -181,6 +181,14 @@ int rvs_synthesize(const rvs_source* srcs, int n_src, const rvs_target* tgt, voi
                    size_t ws_bytes, uint32_t tile_cap, const rvs_view_out* out,
                    uint32_t* needed_cap, rvs_stream_t stream);
 
+/* Per-stage device times of the last rvs_synthesize on the calling thread,
+ * measured with HIP events recorded on its stream between the stages when
+ * enabled: [0] binning, [1] tile raster + blend (incl. in-order replay),
+ * [2] inpainting, [3] encoding; milliseconds. */
+#define RVS_PROF_STAGES 4
+int rvs_profile_enable(int on);
+int rvs_profile_read(float ms[RVS_PROF_STAGES]);
+
 #ifdef __cplusplus
 }
 #endif

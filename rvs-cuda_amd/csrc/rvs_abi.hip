@@ -19,6 +19,15 @@ namespace {
 
 thread_local char g_err[512] = "";
 
+// stage events of rvs_synthesize (rvs_profile_enable)
+thread_local bool g_prof = false;
+thread_local bool g_prof_valid = false;
+thread_local hipEvent_t g_ev[RVS_PROF_STAGES + 1] = {};
+
+void prof_mark(int k, hipStream_t s) {
+    if (g_prof) hipEventRecord(g_ev[k], s);
+}
+
 int fail(int code, const char* fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
@@ -357,6 +366,7 @@ int rvs_synthesize(const rvs_source* srcs, int n_src, const rvs_target* tgt, voi
         max_sw = S.cam.w > max_sw ? S.cam.w : max_sw;
         max_sh = S.cam.h > max_sh ? S.cam.h : max_sh;
     }
+    g_prof_valid = false;
     PairGeom* dg = (PairGeom*)(base + og);
     HIP_TRY(rvs_k::launch_store_geoms(hg.data(), n_src, dg, s), "store geoms");
     const RasterLayout L = raster_layout(n_src, w, h, tile_cap);
@@ -364,19 +374,40 @@ int rvs_synthesize(const rvs_source* srcs, int n_src, const rvs_target* tgt, voi
     const rvs_k::RasterDims d = dims_of(w, h, tile_cap);
     HIP_TRY(rvs_k::launch_reset_u32(ws.status, 16, s), "reset");
     HIP_TRY(rvs_k::launch_reset_u32(ws.counts, (size_t)n_src * d.ntiles, s), "reset");
+    prof_mark(0, s);
     HIP_TRY(rvs_k::launch_bin<PairGeom>(dg, n_src, max_sw, max_sh, d, ws, s), "bin");
+    prof_mark(1, s);
     float* quality = out->quality ? out->quality : (float*)(base + oq);
     rvs_k::RasterOut ro;
     ro.color = out->color; ro.depth = out->depth; ro.quality = quality; ro.validity = out->validity;
     HIP_TRY((rvs_k::launch_raster<PairGeom, rvs_k::MODE_BLEND>(dg, n_src, d, ws, tgt->blending_exp, ro, s)),
             "raster");
+    prof_mark(2, s);
     const rvs_k::InpaintWS iws = rvs_k::inpaint_ws_carve(base + oi, w, h);
     HIP_TRY(rvs_k::launch_inpaint(out->color, quality, w, h, iws, nullptr, s), "inpaint");
+    prof_mark(3, s);
     if (out->y && out->u && out->v) {
         if ((w & 1) || (h & 1)) return fail(RVS_E_UNSUPPORTED, "rvs_synthesize: odd target size cannot be encoded");
         HIP_TRY(rvs_k::launch_encode(out->color, w, h, tgt->bitdepth, out->y, out->u, out->v, s), "encode");
     }
-    return check_raster_status(ws, d, n_src, needed_cap, s);
+    prof_mark(4, s);
+    const int rc = check_raster_status(ws, d, n_src, needed_cap, s);
+    g_prof_valid = g_prof && rc == RVS_OK;
+    return rc;
+}
+
+int rvs_profile_enable(int on) {
+    if (on && !g_ev[0])
+        for (int k = 0; k <= RVS_PROF_STAGES; ++k) HIP_TRY(hipEventCreate(&g_ev[k]), "event create");
+    g_prof = on != 0;
+    g_prof_valid = false;
+    return RVS_OK;
+}
+
+int rvs_profile_read(float ms[RVS_PROF_STAGES]) {
+    if (!g_prof_valid) return fail(RVS_E_INVALID, "no profiled rvs_synthesize on this thread");
+    for (int k = 0; k < RVS_PROF_STAGES; ++k) HIP_TRY(hipEventElapsedTime(&ms[k], g_ev[k], g_ev[k + 1]), "event time");
+    return RVS_OK;
 }
 
 }  // extern "C"

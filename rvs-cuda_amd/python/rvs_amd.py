@@ -88,6 +88,8 @@ def hip():
             "rvs_encode_yuv420": ([vp, i, i, i, vp, vp, vp, vp], i),
             "rvs_synth_workspace_size": ([i, P(RvsTarget), u32], sz),
             "rvs_synthesize": ([P(RvsSource), i, P(RvsTarget), vp, sz, u32, P(RvsViewOut), P(u32), vp], i),
+            "rvs_profile_enable": ([i], i),
+            "rvs_profile_read": ([vp], i),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -248,6 +250,19 @@ def inpaint(color, quality, map_out=None, workspace=None, stream=None):
 def encode_yuv420(color, bitdepth, y, u, v, stream=None):
     h, w = y.shape
     check(hip().rvs_encode_yuv420(_ptr(color), w, h, bitdepth, _ptr(y), _ptr(u), _ptr(v), _stream(stream)))
+
+
+PROF_STAGES = ("bin", "raster", "inpaint", "encode")
+
+
+def profile_enable(on=True):
+    check(hip().rvs_profile_enable(int(on)))
+
+
+def profile_read():
+    ms = np.zeros(len(PROF_STAGES), np.float32)
+    check(hip().rvs_profile_read(ms.ctypes.data))
+    return dict(zip(PROF_STAGES, (float(x) for x in ms)))
 
 
 # ------------------------------------------------------------- fused path

@@ -173,17 +173,30 @@ def test_raster_erp_to_erp_wrap(torch_cuda):
 
 
 def test_raster_identity_warp_all_ties(torch_cuda):
-    """Source camera == target camera: every pixel centre is a shared vertex of
-    six triangles, so every pixel goes through the in-order replay."""
+    """Source camera == target camera: every pixel centre is (up to float
+    rounding of the reprojection) a vertex shared by six triangles, so the
+    winners are near ties and every pixel goes through the in-order replay."""
     torch = torch_cuda
     rg = W.rig("rects_640", scale=4)
     planes = W.render_sources(rg)
     s = rg["sources"][0]
     col, vd, uv, wrap, dst = _stage_inputs(rg, 0, planes, s["pos"], s["rot"])
     _raster_case(torch, col, vd, uv, dst.w, dst.h, wrap, "identity")
-    # identity warp reproduces the source colour at every pixel centre (closed form)
-    oc, od, osh = ob.transform_triangles(col, vd, uv, dst.w, dst.h, wrap)
-    assert np.array_equal(oc, col)
+
+
+def test_raster_exact_grid_identity(torch_cuda):
+    """Closed form: vertices exactly on the pixel centres reproduce the source
+    colour and depth at every pixel (barycentric weight exactly 1)."""
+    torch = torch_cuda
+    rng = np.random.default_rng(9)
+    h, w = 48, 64
+    gy, gx = np.mgrid[0:h, 0:w]
+    uv = np.stack([gx + 0.5, gy + 0.5], -1).astype(np.float32)
+    depth = rng.uniform(1.0, 5.0, (h, w)).astype(np.float32)
+    col = rng.random((h, w, 3)).astype(np.float32)
+    _raster_case(torch, col, depth, uv, w, h, 0, "exact grid")
+    oc, od, osh = ob.transform_triangles(col, depth, uv, w, h, 0)
+    assert np.array_equal(oc, col) and np.array_equal(od, depth)
 
 
 def test_raster_random_triangles(torch_cuda):
