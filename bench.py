@@ -87,6 +87,7 @@ def main():
         syn.run()
         for k, v in R.profile_read().items():
             stage[k].append(v)
+    counters = R.profile_counters()
     R.profile_enable(False)
     stage_ms = {k: float(np.mean(v)) for k, v in stage.items()}
     dev_ms = sum(stage_ms.values())
@@ -115,6 +116,7 @@ def main():
             "sources_resident": True,
         },
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "counters": counters,
         "roofline": {
             "kernel": "one view's launch set (bin, raster+blend, inpaint, encode)",
             "bound": "hbm",

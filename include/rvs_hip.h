@@ -188,6 +188,10 @@ int rvs_synthesize(const rvs_source* srcs, int n_src, const rvs_target* tgt, voi
 #define RVS_PROF_STAGES 4
 int rvs_profile_enable(int on);
 int rvs_profile_read(float ms[RVS_PROF_STAGES]);
+/* Counters of the last profiled rvs_synthesize: [0] tiles with pixels replayed
+ * in order, [1] bin overflow flag, [2] largest (source, tile) bin, [3] total
+ * binned (quad, tile) entries over all sources.  Synchronises the device. */
+int rvs_profile_counters(uint32_t out[4]);
 
 #ifdef __cplusplus
 }

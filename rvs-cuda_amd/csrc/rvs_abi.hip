@@ -23,6 +23,9 @@ thread_local char g_err[512] = "";
 thread_local bool g_prof = false;
 thread_local bool g_prof_valid = false;
 thread_local hipEvent_t g_ev[RVS_PROF_STAGES + 1] = {};
+thread_local rvs_k::RasterWS g_prof_ws;
+thread_local rvs_k::RasterDims g_prof_dims;
+thread_local int g_prof_nsrc = 0;
 
 void prof_mark(int k, hipStream_t s) {
     if (g_prof) hipEventRecord(g_ev[k], s);
@@ -393,7 +396,21 @@ int rvs_synthesize(const rvs_source* srcs, int n_src, const rvs_target* tgt, voi
     prof_mark(4, s);
     const int rc = check_raster_status(ws, d, n_src, needed_cap, s);
     g_prof_valid = g_prof && rc == RVS_OK;
+    if (g_prof_valid) { g_prof_ws = ws; g_prof_dims = d; g_prof_nsrc = n_src; }
     return rc;
+}
+
+int rvs_profile_counters(uint32_t out[4]) {
+    if (!g_prof_valid) return fail(RVS_E_INVALID, "no profiled rvs_synthesize on this thread");
+    uint32_t st[4];
+    HIP_TRY(hipMemcpy(st, g_prof_ws.status, sizeof st, hipMemcpyDeviceToHost), "status copy");
+    std::vector<uint32_t> counts((size_t)g_prof_nsrc * g_prof_dims.ntiles);
+    HIP_TRY(hipMemcpy(counts.data(), g_prof_ws.counts, counts.size() * 4, hipMemcpyDeviceToHost), "counts copy");
+    uint32_t mx = 0;
+    uint64_t tot = 0;
+    for (uint32_t c : counts) { mx = c > mx ? c : mx; tot += c; }
+    out[0] = st[0]; out[1] = st[1]; out[2] = mx; out[3] = (uint32_t)(tot > 0xffffffffull ? 0xffffffffu : tot);
+    return RVS_OK;
 }
 
 int rvs_profile_enable(int on) {

@@ -56,13 +56,62 @@ __global__ void k_inp_init(const float* __restrict__ quality, int w, int h, Inpa
     if (p == 0) { ws.counters[0] = 0; ws.counters[1] = 0; }
 }
 
+constexpr int kCclTile = 32;
+
+__device__ inline int lfind(const int* lab, int p) {
+    while (true) {
+        const int q = lab[p];
+        if (q == p) return p;
+        p = q;
+    }
+}
+
+__device__ inline void lunite(int* lab, int a, int b) {
+    while (true) {
+        a = lfind(lab, a);
+        b = lfind(lab, b);
+        if (a == b) return;
+        if (a < b) { const int t = a; a = b; b = t; }
+        const int old = atomicMin(&lab[a], b);
+        if (old == a) return;
+        a = old;
+    }
+}
+
+// Labels inside one 32x32 tile in LDS; each hole pixel gets the global index of
+// its tile-local root (the smallest local index of its local component).
+__global__ void __launch_bounds__(256) k_ccl_local(int w, int h, InpaintWS ws) {
+    __shared__ int lab[kCclTile * kCclTile];
+    const int tx0 = blockIdx.x * kCclTile, ty0 = blockIdx.y * kCclTile;
+    const int tid = threadIdx.x;
+    for (int k = tid; k < kCclTile * kCclTile; k += 256) {
+        const int x = tx0 + (k % kCclTile), y = ty0 + (k / kCclTile);
+        lab[k] = (x < w && y < h && ws.label[y * w + x] >= 0) ? k : -1;
+    }
+    __syncthreads();
+    for (int k = tid; k < kCclTile * kCclTile; k += 256) {
+        if (lab[k] < 0) continue;
+        const int lx = k % kCclTile, ly = k / kCclTile;
+        if (lx + 1 < kCclTile && lab[k + 1] >= 0) lunite(lab, k, k + 1);
+        if (ly + 1 < kCclTile && lab[k + kCclTile] >= 0) lunite(lab, k, k + kCclTile);
+    }
+    __syncthreads();
+    for (int k = tid; k < kCclTile * kCclTile; k += 256) {
+        if (lab[k] < 0) continue;
+        const int r = lfind(lab, k);
+        const int x = tx0 + (k % kCclTile), y = ty0 + (k / kCclTile);
+        ws.label[y * w + x] = (ty0 + r / kCclTile) * w + (tx0 + r % kCclTile);
+    }
+}
+
+// Unions across tile borders on the tile-local roots.
 __global__ void k_ccl_merge(int w, int h, InpaintWS ws) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= w * h) return;
     if (ws.label[p] < 0) return;
     const int x = p % w, y = p / w;
-    if (x + 1 < w && ws.label[p + 1] >= 0) unite(ws.label, p, p + 1);
-    if (y + 1 < h && ws.label[p + w] >= 0) unite(ws.label, p, p + w);
+    if ((x % kCclTile) == kCclTile - 1 && x + 1 < w && ws.label[p + 1] >= 0) unite(ws.label, p, p + 1);
+    if ((y % kCclTile) == kCclTile - 1 && y + 1 < h && ws.label[p + w] >= 0) unite(ws.label, p, p + w);
 }
 
 __global__ void k_ccl_flatten(int w, int h, InpaintWS ws) {
@@ -113,106 +162,189 @@ __device__ inline void cand(int& bv, int& br, int& bs, int v, int r, int s) {
     if (v < bv || (v == bv && r < br)) { bv = v; br = r; bs = s; }
 }
 
-__global__ void __launch_bounds__(256) k_inp_solve(int w, int h, InpaintWS ws) {
-    __shared__ Fn wave_tot[4];
-    __shared__ int s_comp;
+// Component region access: LDS copy of the bounding box (small components) or
+// the global map (large ones).  Pixels outside the box that a member can see
+// are known pixels; non-member holes inside the box are never a member's
+// neighbour (4-connectivity), so they only pass through scans as constants.
+constexpr int kLdsCap = 6144;   // pixels of a component box held in LDS (48 KB)
+constexpr int kOther = -1;      // dist of a hole of another component (inside the box)
+
+struct Region {
+    int x0, y0, bw, bh, w, h, root;
+    int* ldist;   // LDS mode
+    int* lsrc;
+};
+
+template <bool LDS>
+__device__ inline void region_get(const Region& R, const InpaintWS& ws, int x, int y, int& v, int& s) {
+    const int p = y * R.w + x;
+    if (LDS) {
+        const int bx = x - R.x0, by = y - R.y0;
+        if (bx >= 0 && bx < R.bw && by >= 0 && by < R.bh) {
+            const int k = by * R.bw + bx;
+            v = R.ldist[k]; s = R.lsrc[k];
+        } else {
+            v = 0; s = p;                     // a known pixel next to the box
+        }
+    } else {
+        const int lab = ws.label[p];
+        if (lab < 0) { v = 0; s = p; }
+        else if (lab != R.root) { v = kOther; s = -1; }
+        else { v = ws.dist[p]; s = ws.src[p]; }
+    }
+}
+
+template <bool LDS>
+__device__ inline void region_put(const Region& R, const InpaintWS& ws, int x, int y, int v, int s) {
+    if (LDS) {
+        const int k = (y - R.y0) * R.bw + (x - R.x0);
+        R.ldist[k] = v; R.lsrc[k] = s;
+    } else {
+        const int p = y * R.w + x;
+        ws.dist[p] = v; ws.src[p] = s;
+    }
+}
+
+// One component: alternate forward / backward sweeps until one changes nothing.
+// Lines run along the box's shorter side and are processed in sweep order; each
+// line is a scan along the longer side.  Row lines scan x (forward: the scan
+// input is the left neighbour, rank 1; backward: the right one, rank 4); column
+// lines scan y (forward: up, rank 2; backward: down, rank 3).  Every other
+// neighbour is read from the map, which at that moment holds this sweep's value
+// for neighbours earlier in the sweep order and the previous sweep's otherwise.
+template <bool LDS>
+__device__ void solve_component(const Region& R, const InpaintWS& ws, Fn* wave_tot, int* s_carry) {
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
+    const bool cols = R.bh > R.bw;
+    const int nlines = cols ? R.bw : R.bh;
+    const int llen = cols ? R.bh : R.bw;
+    const int max_sweeps = 4 * (R.w + R.h) + 8;   // each changing sweep lowers a distance: a guard only
+    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+        const bool fwd = (sweep & 1) == 0;
+        const int rs = cols ? (fwd ? 2 : 3) : (fwd ? 1 : 4);
+        bool changed = false;
+        for (int li = 0; li < nlines; ++li) {
+            const int line = fwd ? li : nlines - 1 - li;
+            // carry in: the pixel before the first element in scan order (outside the box)
+            int Lv = kAbsent, Ls = -1;
+            {
+                const int pb = fwd ? -1 : llen;
+                const int bx = cols ? R.x0 + line : R.x0 + pb;
+                const int by = cols ? R.y0 + pb : R.y0 + line;
+                if (bx >= 0 && bx < R.w && by >= 0 && by < R.h) {
+                    int v, sr;
+                    region_get<LDS>(R, ws, bx, by, v, sr);
+                    if (v == 0) { Lv = 0; Ls = sr; }
+                }
+            }
+            for (int cb = 0; cb < llen; cb += 256) {
+                const int e = cb + tid;
+                const bool in = e < llen;
+                const int pos = fwd ? e : llen - 1 - e;
+                const int x = cols ? R.x0 + line : R.x0 + pos;
+                const int y = cols ? R.y0 + pos : R.y0 + line;
+                Fn f;
+                f.k = 1;
+                bool member = false;
+                int cur_v = 0;
+                f.cv = kAbsent; f.cs = -1; f.t = 0;
+                if (in) {
+                    int v, sr;
+                    region_get<LDS>(R, ws, x, y, v, sr);
+                    if (v == 0) {
+                        f.cv = 0; f.cs = sr;                       // known pixel: (0, itself)
+                    } else if (v != kOther) {
+                        member = true;
+                        cur_v = v;
+                        int bv = v, br = 0, bs = sr;
+                        // neighbours in the reference order, left(1) up(2) down(3) right(4),
+                        // except the scan input
+                        int nv, ns;
+                        if (rs != 1 && x > 0) { region_get<LDS>(R, ws, x - 1, y, nv, ns); if (nv != kOther) cand(bv, br, bs, nv + 1, 1, ns); }
+                        if (rs != 2 && y > 0) { region_get<LDS>(R, ws, x, y - 1, nv, ns); if (nv != kOther) cand(bv, br, bs, nv + 1, 2, ns); }
+                        if (rs != 3 && y + 1 < R.h) { region_get<LDS>(R, ws, x, y + 1, nv, ns); if (nv != kOther) cand(bv, br, bs, nv + 1, 3, ns); }
+                        if (rs != 4 && x + 1 < R.w) { region_get<LDS>(R, ws, x + 1, y, nv, ns); if (nv != kOther) cand(bv, br, bs, nv + 1, 4, ns); }
+                        f.cv = bv; f.cs = bs; f.t = (rs < br) ? 1 : 0;
+                    }
+                }
+                // inclusive scan of compositions in thread (= scan) order
+                for (int off = 1; off < 64; off <<= 1) {
+                    Fn g;
+                    g.cv = __shfl_up(f.cv, off);
+                    g.cs = __shfl_up(f.cs, off);
+                    g.t = __shfl_up(f.t, off);
+                    g.k = __shfl_up(f.k, off);
+                    if (lane >= off) f = compose(g, f);
+                }
+                if (lane == 63) wave_tot[wave] = f;
+                __syncthreads();   // all reads of this chunk are done before any write
+                Fn pre;
+                bool have = false;
+                for (int wv = 0; wv < wave; ++wv) {
+                    pre = have ? compose(pre, wave_tot[wv]) : wave_tot[wv];
+                    have = true;
+                }
+                if (have) f = compose(pre, f);
+                int rv, rsrc;
+                if (Lv + f.k < f.cv + f.t) { rv = Lv + f.k; rsrc = Ls; } else { rv = f.cv; rsrc = f.cs; }
+                if (member && rv != cur_v) {
+                    changed = true;
+                    region_put<LDS>(R, ws, x, y, rv, rsrc);
+                }
+                const int last = min(256, llen - cb) - 1;
+                if (tid == last) { s_carry[0] = rv; s_carry[1] = rsrc; }
+                __syncthreads();
+                Lv = s_carry[0]; Ls = s_carry[1];
+                __syncthreads();
+            }
+        }
+        if (!__syncthreads_or(changed)) break;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_inp_solve(int w, int h, InpaintWS ws) {
+    __shared__ Fn wave_tot[4];
+    __shared__ int s_carry[2];
+    __shared__ int s_comp;
+    __shared__ int ldist[kLdsCap];
+    __shared__ int lsrc[kLdsCap];
+    const int tid = threadIdx.x;
     while (true) {
         if (tid == 0) s_comp = atomicAdd(&ws.counters[1], 1);
         __syncthreads();
         const int c = s_comp;
         __syncthreads();
         if (c >= ws.counters[0]) return;
-        const int root = ws.root[c];
-        const int x0 = ws.comp[4 * c + 0], x1 = ws.comp[4 * c + 1];
-        const int y0 = ws.comp[4 * c + 2], y1 = ws.comp[4 * c + 3];
-        const int cw = x1 - x0 + 1;
-        // Every sweep that changes something strictly lowers a distance, so the loop
-        // ends; the bound only guards against a broken invariant.
-        const int max_sweeps = 4 * (w + h) + 8;
-        for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-            const bool fwd = (sweep & 1) == 0;
-            bool changed = false;
-            for (int yy = 0; yy <= y1 - y0; ++yy) {
-                const int y = fwd ? y0 + yy : y1 - yy;
-                // carry: the pixel before the first element in scan order
-                int Lv = kAbsent, Ls = -1;
-                {
-                    const int xb = fwd ? x0 - 1 : x1 + 1;
-                    if (xb >= 0 && xb < w) {
-                        const int pb = y * w + xb;
-                        if (ws.label[pb] < 0) { Lv = 0; Ls = pb; }
-                    }
-                }
-                for (int cb = 0; cb < cw; cb += 256) {
-                    const int e = cb + tid;
-                    const bool in = e < cw;
-                    const int x = fwd ? x0 + e : x1 - e;
-                    const int p = y * w + x;
-                    Fn f;
-                    f.k = 1;
-                    bool member = false;
-                    int cur_v = 0;
-                    if (!in) {
-                        f.cv = kAbsent; f.cs = -1; f.t = 0;
-                    } else {
-                        const int lab = ws.label[p];
-                        if (lab < 0) {
-                            f.cv = 0; f.cs = p; f.t = 0;           // known pixel: (0, itself)
-                        } else if (lab != root) {
-                            f.cv = kAbsent; f.cs = -1; f.t = 0;    // another component (never a neighbour)
-                        } else {
-                            member = true;
-                            cur_v = ws.dist[p];
-                            int bv = cur_v, br = 0, bs = ws.src[p];
-                            // neighbours in the reference order: left(1) up(2) down(3) right(4);
-                            // the scan input is left (forward) or right (backward)
-                            if (!fwd && x > 0) cand(bv, br, bs, ws.dist[p - 1] + 1, 1, ws.src[p - 1]);
-                            if (y > 0) cand(bv, br, bs, ws.dist[p - w] + 1, 2, ws.src[p - w]);
-                            if (y + 1 < h) cand(bv, br, bs, ws.dist[p + w] + 1, 3, ws.src[p + w]);
-                            if (fwd && x + 1 < w) cand(bv, br, bs, ws.dist[p + 1] + 1, 4, ws.src[p + 1]);
-                            const int rs = fwd ? 1 : 4;
-                            f.cv = bv; f.cs = bs; f.t = (rs < br) ? 1 : 0;
-                        }
-                    }
-                    // inclusive scan of compositions in thread (= scan) order
-                    for (int off = 1; off < 64; off <<= 1) {
-                        Fn g;
-                        g.cv = __shfl_up(f.cv, off);
-                        g.cs = __shfl_up(f.cs, off);
-                        g.t = __shfl_up(f.t, off);
-                        g.k = __shfl_up(f.k, off);
-                        if (lane >= off) f = compose(g, f);
-                    }
-                    if (lane == 63) wave_tot[wave] = f;
-                    __syncthreads();   // all reads of this row chunk are done before any write
-                    Fn pre; pre.cv = kAbsent; pre.cs = -1; pre.t = 0; pre.k = 0;
-                    bool have = false;
-                    for (int wv = 0; wv < wave; ++wv) {
-                        pre = have ? compose(pre, wave_tot[wv]) : wave_tot[wv];
-                        have = true;
-                    }
-                    if (have) f = compose(pre, f);
-                    // apply to the carry
-                    int rv, rsrc;
-                    if (Lv + f.k < f.cv + f.t) { rv = Lv + f.k; rsrc = Ls; } else { rv = f.cv; rsrc = f.cs; }
-                    if (member && rv != cur_v) {
-                        changed = true;
-                        ws.dist[p] = rv;
-                        ws.src[p] = rsrc;
-                    }
-                    // carry out: result at the last element of the chunk
-                    __shared__ int s_carry[2];
-                    const int last = min(256, cw - cb) - 1;
-                    if (tid == last) { s_carry[0] = rv; s_carry[1] = rsrc; }
-                    __syncthreads();
-                    Lv = s_carry[0]; Ls = s_carry[1];
-                    __syncthreads();
-                }
+        Region R;
+        R.root = ws.root[c];
+        R.x0 = ws.comp[4 * c + 0];
+        R.y0 = ws.comp[4 * c + 2];
+        R.bw = ws.comp[4 * c + 1] - R.x0 + 1;
+        R.bh = ws.comp[4 * c + 3] - R.y0 + 1;
+        R.w = w; R.h = h;
+        R.ldist = ldist; R.lsrc = lsrc;
+        const int area = R.bw * R.bh;
+        if (area <= kLdsCap) {
+            for (int k = tid; k < area; k += 256) {
+                const int x = R.x0 + k % R.bw, y = R.y0 + k / R.bw;
+                const int p = y * w + x;
+                const int lab = ws.label[p];
+                ldist[k] = lab < 0 ? 0 : (lab == R.root ? ws.dist[p] : kOther);
+                lsrc[k] = lab == R.root || lab < 0 ? ws.src[p] : -1;
             }
-            if (!__syncthreads_or(changed)) break;
+            __syncthreads();
+            solve_component<true>(R, ws, wave_tot, s_carry);
+            __syncthreads();
+            for (int k = tid; k < area; k += 256) {
+                const int x = R.x0 + k % R.bw, y = R.y0 + k / R.bw;
+                const int p = y * w + x;
+                if (ws.label[p] == R.root) { ws.dist[p] = ldist[k]; ws.src[p] = lsrc[k]; }
+            }
+        } else {
+            solve_component<false>(R, ws, wave_tot, s_carry);
         }
+        __syncthreads();
     }
 }
 
@@ -258,6 +390,8 @@ hipError_t launch_inpaint(float* color, const float* quality, int w, int h, cons
     const int n = w * h;
     const dim3 b(256), g((n + 255) / 256);
     hipLaunchKernelGGL(k_inp_init, g, b, 0, s, quality, w, h, ws);
+    hipLaunchKernelGGL(k_ccl_local, dim3((w + kCclTile - 1) / kCclTile, (h + kCclTile - 1) / kCclTile), b, 0, s, w, h,
+                       ws);
     hipLaunchKernelGGL(k_ccl_merge, g, b, 0, s, w, h, ws);
     hipLaunchKernelGGL(k_ccl_flatten, g, b, 0, s, w, h, ws);
     hipLaunchKernelGGL(k_comp_bbox, g, b, 0, s, w, h, ws);

@@ -90,6 +90,7 @@ def hip():
             "rvs_synthesize": ([P(RvsSource), i, P(RvsTarget), vp, sz, u32, P(RvsViewOut), P(u32), vp], i),
             "rvs_profile_enable": ([i], i),
             "rvs_profile_read": ([vp], i),
+            "rvs_profile_counters": ([vp], i),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -263,6 +264,12 @@ def profile_read():
     ms = np.zeros(len(PROF_STAGES), np.float32)
     check(hip().rvs_profile_read(ms.ctypes.data))
     return dict(zip(PROF_STAGES, (float(x) for x in ms)))
+
+
+def profile_counters():
+    c = np.zeros(4, np.uint32)
+    check(hip().rvs_profile_counters(c.ctypes.data))
+    return {"replay_tiles": int(c[0]), "bin_overflow": int(c[1]), "max_bin": int(c[2]), "bin_entries": int(c[3])}
 
 
 # ------------------------------------------------------------- fused path
