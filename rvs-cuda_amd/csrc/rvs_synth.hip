@@ -9,14 +9,16 @@
 // near-equal scores.  This file computes it in three steps:
 //   1. k_bin: every source quad is appended to the 16x16 target tiles its
 //      surviving triangles' (reference-exact) bounding boxes touch.
-//   2. k_raster: one 256-thread block per tile.  Pass 1 takes an LDS 64-bit
-//      atomicMax of (score bits << 32 | ~seq) per pixel; pass 2 marks pixels
-//      where another candidate's score lies within 2^-18 of the winner's, or
-//      where a candidate has non-finite or extreme values.  Unmarked pixels
-//      provably end the sequential fold on the argmax (the float test errs by
-//      less than 2^-21 relative), so their winner is re-evaluated and blended.
-//   3. k_fallback: marked pixels replay the reference fold literally, in
-//      traversal order, over the tile's bin sorted in LDS.
+//   2. k_raster: one 256-thread block per tile, up to four sources per phase.
+//      Every covered pixel takes two LDS 64-bit atomicMax: key1 = max of
+//      (score bits << 32 | ~seq), key2 = max over the candidates that are not
+//      the maximum (the loser of each key1 update).  A pixel whose runner-up
+//      scores within 2^-18 of the winner, or that saw a candidate with
+//      non-finite or extreme values, is deferred.  Every other pixel provably
+//      ends the sequential fold on the argmax (the float test errs by less than
+//      2^-21 relative), so its winner is re-evaluated and blended.
+//   3. k_fallback: deferred pixels replay the reference fold literally, in
+//      traversal order, over the tile's bins sorted in LDS.
 // Blending across sources (BlendedViewSimple::blendBasicMode) runs per pixel in
 // registers inside the same block, in InputCameraNames order.
 #include <hip/hip_runtime.h>
@@ -188,53 +190,77 @@ __device__ inline void resolve(const G& g, uint32_t seq, int x, int y, float3& c
     c = tri_color(color(g, r[0], cc[0]), color(g, r[1], cc[1]), color(g, r[2], cc[2]), l1, l2, l3);
 }
 
+// Tile index of block b: each XCD (blocks are dealt round-robin over 8 XCDs)
+// gets a contiguous band of tiles, so neighbouring tiles share that XCD's L2.
+__device__ inline int tile_of_block(int b, int ntiles) {
+    const int per = (ntiles + 7) / 8;
+    return (b % 8) * per + b / 8;
+}
+
+// Pass over one triangle of a binned quad for the tile at (tx0, ty0): per
+// covered pixel, push the candidate key into the top-2 keys of the pixel.
+// Candidates with non-finite inputs or extreme depths mark the pixel for the
+// in-order replay instead.
+__device__ inline void raster_tri(float3 a, float3 b, float3 c, uint32_t seq, bool prol_false_possible, int tx0,
+                                  int ty0, int out_w, int out_h, float rescale, unsigned long long* key1,
+                                  unsigned long long* key2, int* flag) {
+    Tri T;
+    if (!tri_setup(T, a, b, c, out_w, out_h, rescale)) return;
+    const bool prol = (a.z > 0.f) || (b.z > 0.f) || (c.z > 0.f);
+    const bool special = !finite3(a) || !finite3(b) || !finite3(c) || !is_finite(T.den) ||
+                         (prol_false_possible && !prol);
+    const int xa = max(T.xmin, tx0), xb = min(T.xmax, tx0 + kTile - 1);
+    const int ya = max(T.ymin, ty0), yb = min(T.ymax, ty0 + kTile - 1);
+    const unsigned long long lo = (unsigned long long)(~seq);
+    for (int y = ya; y <= yb; ++y)
+        for (int x = xa; x <= xb; ++x) {
+            float l1, l2, l3;
+            if (!tri_pixel(T, x, y, l1, l2, l3)) continue;
+            const int p = (y - ty0) * kTile + (x - tx0);
+            const float dd = tri_depth(T, l1, l2, l3);
+            if (special || !depth_regular(dd)) {
+                flag[p] = 1;
+                continue;
+            }
+            const float f = T.tv / (dd * dd * dd);
+            const unsigned long long k = ((unsigned long long)f2u(f) << 32) | lo;
+            const unsigned long long old = atomicMax(&key1[p], k);
+            // the candidate that is not (or no longer) the maximum goes to the runner-up slot
+            atomicMax(&key2[p], old < k ? old : k);
+        }
+}
+
 template <class G>
-__device__ inline void raster_quad(const G& g, uint32_t q, int pass, int tx0, int ty0, int out_w, int out_h,
-                                   unsigned long long* key, int* flag) {
+__device__ inline void raster_quad(const G& g, uint32_t q, int tx0, int ty0, int out_w, int out_h,
+                                   unsigned long long* key1, unsigned long long* key2, int* flag) {
     int i, j;
     quad_of(g, q, i, j);
-    Quad Q;
-    auto vtx = [&](int ii, int jj) -> float3 { return vertex(g, ii, jj); };
-    quad_build(Q, g, i, j, vtx, out_w, out_h);
-    for (int s = 0; s < 4; ++s) {
-        if (!(Q.alive & (1u << s))) continue;
-        const Tri& T = Q.t[s];
-        const bool special = (Q.special >> s) & 1u;
-        if (pass == 2 && special) continue;
-        const uint32_t seq = (q << 2) | (uint32_t)s;
-        const int xa = max(T.xmin, tx0), xb = min(T.xmax, tx0 + kTile - 1);
-        const int ya = max(T.ymin, ty0), yb = min(T.ymax, ty0 + kTile - 1);
-        for (int y = ya; y <= yb; ++y)
-            for (int x = xa; x <= xb; ++x) {
-                float l1, l2, l3;
-                if (!tri_pixel(T, x, y, l1, l2, l3)) continue;
-                const int p = (y - ty0) * kTile + (x - tx0);
-                const float dd = tri_depth(T, l1, l2, l3);
-                if (special || !depth_regular(dd)) {
-                    if (pass == 1) flag[p] = 1;
-                    continue;
-                }
-                const float f = T.tv / (dd * dd * dd);
-                if (pass == 1) {
-                    const unsigned long long k = ((unsigned long long)f2u(f) << 32) | (unsigned long long)(~seq);
-                    atomicMax(&key[p], k);
-                } else {
-                    const unsigned long long kw = key[p];
-                    if ((uint32_t)kw != ~seq) {
-                        const float fw = u2f((uint32_t)(kw >> 32));
-                        if (f >= fw - fw * RVS_AMBIGUITY) flag[p] = 1;
-                    }
-                }
-            }
+    const float3 v00 = vertex(g, i, j), v01 = vertex(g, i, j + 1), v10 = vertex(g, i + 1, j);
+    const float3 v11 = vertex(g, i + 1, j + 1);
+    if (!(v01.z > 0.f && v10.z > 0.f && v01.x > 0.f && v10.x > 0.f)) return;
+    const uint32_t s0 = q << 2;
+    if (v00.z > 0.f && v00.x > 0.f)
+        raster_tri(v00, v01, v10, s0, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
+    if (v11.z > 0.f && v11.x > 0.f)
+        raster_tri(v11, v10, v01, s0 | 1u, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
+    if (g.wrap && j == 0) {
+        const float3 l0 = vertex(g, i, g.sw - 1), l1 = vertex(g, i + 1, g.sw - 1);
+        raster_tri(l0, v00, l1, s0 | 2u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
+        raster_tri(v10, l1, v00, s0 | 3u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
     }
 }
+
+constexpr int kGroup = 4;   // sources rasterised together per block phase
 
 template <class G, int MODE>
 __global__ void __launch_bounds__(256) k_raster(const G* __restrict__ geoms, int nsrc, RasterDims d, RasterWS ws,
                                                 float blend_exp, RasterOut out) {
-    __shared__ unsigned long long key[kTile * kTile];
-    __shared__ int flag[kTile * kTile];
-    const int tile = blockIdx.x;
+    __shared__ unsigned long long key1[kGroup][kTile * kTile];
+    __shared__ unsigned long long key2[kGroup][kTile * kTile];
+    __shared__ int flag[kGroup][kTile * kTile];
+    __shared__ uint32_t pref[kGroup + 1];
+    const int tile = tile_of_block(blockIdx.x, d.ntiles);
+    if (tile >= d.ntiles) return;
     const int tid = threadIdx.x;
     const int tx0 = (tile % d.tiles_x) * kTile, ty0 = (tile / d.tiles_x) * kTile;
     const int px = tx0 + (tid % kTile), py = ty0 + (tid / kTile);
@@ -243,27 +269,47 @@ __global__ void __launch_bounds__(256) k_raster(const G* __restrict__ geoms, int
     BlendState st;
     float3 rc = make_float3(0.f, 0.f, 0.f);
     float rd = __builtin_inff(), rs = 0.f;
-    for (int s = 0; s < nsrc; ++s) {
-        const G& g = geoms[s];
-        key[tid] = 0ull;
-        flag[tid] = 0;
+    for (int s0 = 0; s0 < nsrc; s0 += kGroup) {
+        const int gs = min(kGroup, nsrc - s0);
+        for (int k = 0; k < gs; ++k) {
+            key1[k][tid] = 0ull;
+            key2[k][tid] = 0ull;
+            flag[k][tid] = 0;
+        }
+        if (tid == 0) {
+            uint32_t acc = 0;
+            for (int k = 0; k < gs; ++k) {
+                pref[k] = acc;
+                acc += min(ws.counts[(size_t)(s0 + k) * d.ntiles + tile], d.cap);
+            }
+            pref[gs] = acc;
+        }
         __syncthreads();
-        const uint32_t n = min(ws.counts[(size_t)s * d.ntiles + tile], d.cap);
-        const uint32_t* L = ws.lists + ((size_t)s * d.ntiles + tile) * d.cap;
-        for (uint32_t b = 0; b < n; b += 256)
-            if (b + tid < n) raster_quad(g, L[b + tid], 1, tx0, ty0, d.out_w, d.out_h, key, flag);
+        const uint32_t N = pref[gs];
+        for (uint32_t e = tid; e < N; e += 256) {
+            int k = 0;
+            while (k + 1 < gs && e >= pref[k + 1]) ++k;
+            const uint32_t q = ws.lists[((size_t)(s0 + k) * d.ntiles + tile) * d.cap + (e - pref[k])];
+            raster_quad(geoms[s0 + k], q, tx0, ty0, d.out_w, d.out_h, key1[k], key2[k], flag[k]);
+        }
         __syncthreads();
-        for (uint32_t b = 0; b < n; b += 256)
-            if (b + tid < n) raster_quad(g, L[b + tid], 2, tx0, ty0, d.out_w, d.out_h, key, flag);
-        __syncthreads();
-        if (inside && !deferred) {
-            if (flag[tid]) {
-                deferred = true;
-            } else {
-                const unsigned long long kw = key[tid];
+        if (inside) {
+            for (int k = 0; k < gs; ++k) {
+                const int s = s0 + k;
+                if (deferred) break;
+                const unsigned long long kw = key1[k][tid];
+                bool amb = flag[k][tid] != 0;
+                if (!amb && kw != 0ull) {
+                    const unsigned long long k2 = key2[k][tid];
+                    if (k2 != 0ull) {
+                        const float fw = u2f((uint32_t)(kw >> 32)), f2 = u2f((uint32_t)(k2 >> 32));
+                        amb = f2 >= fw - fw * RVS_AMBIGUITY;
+                    }
+                }
+                if (amb) { deferred = true; break; }
                 float3 c = make_float3(0.f, 0.f, 0.f);
                 float dep = __builtin_inff(), shape = 0.f;
-                if (kw != 0ull) resolve(g, ~(uint32_t)kw, px, py, c, dep, shape, d.out_w, d.out_h);
+                if (kw != 0ull) resolve(geoms[s], ~(uint32_t)kw, px, py, c, dep, shape, d.out_w, d.out_h);
                 if (MODE == MODE_RAW) {
                     rc = c; rd = dep; rs = shape;
                 } else if (s == 0) {
@@ -415,7 +461,8 @@ __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, i
 template <class G, int MODE>
 hipError_t launch_raster(const G* geoms, int nsrc, const RasterDims& d, const RasterWS& ws, float blend_exp,
                          const RasterOut& out, hipStream_t s) {
-    hipLaunchKernelGGL((k_raster<G, MODE>), dim3(d.ntiles), dim3(256), 0, s, geoms, nsrc, d, ws, blend_exp, out);
+    const int blocks = 8 * ((d.ntiles + 7) / 8);
+    hipLaunchKernelGGL((k_raster<G, MODE>), dim3(blocks), dim3(256), 0, s, geoms, nsrc, d, ws, blend_exp, out);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int fb_blocks = d.ntiles < 512 ? d.ntiles : 512;
