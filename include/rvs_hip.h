@@ -102,12 +102,14 @@ int rvs_reproject(const rvs_cam* src, const rvs_cam* dst, const rvs_xform* xf, c
  * per-pixel winner equals the reference's sequential fold bit for bit.
  * color: src w*h*3, depth/uv on the source grid (as produced by
  * rvs_reproject).  Outputs at out_w*out_h: color*3, depth (+inf uncovered),
- * shape (triangle validity, 0 uncovered).  workspace: rvs_raster_workspace_size. */
+ * shape (triangle validity, 0 uncovered).  workspace: rvs_raster_workspace_size.
+ * Synchronises the stream; RVS_E_CAPACITY reports the needed tile_cap through
+ * needed_cap (if not NULL) when a bin overflowed. */
 size_t rvs_raster_workspace_size(int out_w, int out_h, uint32_t tile_cap);
 int rvs_raster(const float* color, const float* depth, const float* uv, int in_w, int in_h,
                int out_w, int out_h, int wrap, float rescale, void* workspace, size_t ws_bytes,
                uint32_t tile_cap, float* out_color, float* out_depth, float* out_shape,
-               rvs_stream_t stream);
+               uint32_t* needed_cap, rvs_stream_t stream);
 
 /* Replaces blendImages (blend_img.cuh:12-17) with the CPU semantics of
  * BlendedViewSimple::blendBasicMode (BlendedView.cpp:180-212): blends one

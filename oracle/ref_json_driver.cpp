@@ -30,6 +30,10 @@ extern "C" int refjson_query(const char* text, const char* overrides_text, const
         std::string p(path ? path : "");
         std::string seg;
         std::istringstream ps(p);
+        if (overrides_text && *overrides_text && node.type() == json::Node::Type::object) {
+            std::istringstream os(overrides_text);
+            node.setOverrides(json::Node::readFrom(os));
+        }
         while (std::getline(ps, seg, '/')) {
             if (seg.empty()) continue;
             bool numeric = seg.find_first_not_of("0123456789") == std::string::npos;

@@ -255,7 +255,7 @@ size_t rvs_raster_workspace_size(int out_w, int out_h, uint32_t tile_cap) {
 
 int rvs_raster(const float* color, const float* depth, const float* uv, int in_w, int in_h, int out_w, int out_h,
                int wrap, float rescale, void* workspace, size_t ws_bytes, uint32_t tile_cap, float* out_color,
-               float* out_depth, float* out_shape, rvs_stream_t stream) {
+               float* out_depth, float* out_shape, uint32_t* needed_cap, rvs_stream_t stream) {
     if (!color || !depth || !uv || in_w < 2 || in_h < 2 || out_w < 1 || out_h < 1 || !workspace || !out_color ||
         !out_depth || !out_shape || !(rescale > 0.f))
         return fail(RVS_E_INVALID, "rvs_raster: bad argument");
@@ -278,7 +278,7 @@ int rvs_raster(const float* color, const float* depth, const float* uv, int in_w
     rvs_k::RasterOut o;
     o.color = out_color; o.depth = out_depth; o.quality = nullptr; o.validity = out_shape;
     HIP_TRY((rvs_k::launch_raster<ArrayGeom, rvs_k::MODE_RAW>(dg, 1, d, ws, 0.f, o, s)), "raster");
-    return check_raster_status(ws, d, 1, nullptr, s);
+    return check_raster_status(ws, d, 1, needed_cap, s);
 }
 
 int rvs_blend_step(float* acc_color, float* acc_quality, float* acc_depth, float* acc_validity, uint8_t* acc_mask,

@@ -81,7 +81,7 @@ def hip():
             "rvs_decode_depth": ([vp, i, i, i, f, f, f, i, vp, vp], i),
             "rvs_reproject": ([P(RvsCam), P(RvsCam), P(RvsXform), vp, vp, f, vp, vp, vp], i),
             "rvs_raster_workspace_size": ([i, i, u32], sz),
-            "rvs_raster": ([vp, vp, vp, i, i, i, i, i, f, vp, sz, u32, vp, vp, vp, vp], i),
+            "rvs_raster": ([vp, vp, vp, i, i, i, i, i, f, vp, sz, u32, vp, vp, vp, P(u32), vp], i),
             "rvs_blend_step": ([vp, vp, vp, vp, vp, vp, vp, vp, i, i, f, i, vp], i),
             "rvs_inpaint_workspace_size": ([i, i], sz),
             "rvs_inpaint": ([vp, vp, i, i, vp, sz, vp, vp], i),
@@ -110,6 +110,15 @@ def host():
                                        C.c_int, C.c_int, C.c_float, C.c_float, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.c_int]
         L.rvs_scene_render.restype = C.c_int
+        L.rvs_host_last_error.restype = C.c_char_p
+        L.rvs_json_query.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]
+        L.rvs_json_query.restype = C.c_int
+        L.rvs_pose_trace_parse.argtypes = [C.c_char_p, C.c_void_p, C.c_int]
+        L.rvs_pose_trace_parse.restype = C.c_int
+        L.rvs_config_describe.argtypes = [C.c_char_p, C.c_char_p, C.c_int]
+        L.rvs_config_describe.restype = C.c_int
+        L.rvs_app_run.argtypes = [C.c_char_p, C.c_char_p, C.c_int]
+        L.rvs_app_run.restype = C.c_int
         _host = L
     return _host
 
@@ -180,6 +189,36 @@ def render_scene(kind, seed, c, pos, rot, bd_color, bd_depth, znear, zfar, threa
     return y, u, v, d
 
 
+def json_query(text, path, conv, overrides=None):
+    buf = C.create_string_buffer(4096)
+    rc = host().rvs_json_query(text.encode(), (overrides or "").encode(), path.encode(), conv.encode(), buf, 4096)
+    return rc, buf.value.decode()
+
+
+def pose_trace_parse(text, max_rows=4096):
+    out = np.zeros((max_rows, 6), np.float32)
+    n = host().rvs_pose_trace_parse(text.encode(), out.ctypes.data, max_rows)
+    if n < 0:
+        raise ValueError(host().rvs_host_last_error().decode())
+    return out[:n]
+
+
+def config_describe(path):
+    import json
+    buf = C.create_string_buffer(1 << 16)
+    rc = host().rvs_config_describe(path.encode(), buf, 1 << 16)
+    if rc != 0:
+        raise ValueError(buf.value.decode())
+    return json.loads(buf.value.decode())
+
+
+def app_run(config_path, source_path="", stagewise=False):
+    n = host().rvs_app_run(config_path.encode(), source_path.encode(), int(stagewise))
+    if n < 0:
+        raise RuntimeError(host().rvs_host_last_error().decode())
+    return n
+
+
 def _ptr(t):
     return None if t is None else C.c_void_p(t.data_ptr())
 
@@ -220,14 +259,19 @@ def raster(color, depth, uv, out_w, out_h, wrap, rescale, out_color, out_depth, 
            tile_cap=0, stream=None):
     import torch
     in_h, in_w = depth.shape
-    need = hip().rvs_raster_workspace_size(out_w, out_h, tile_cap)
-    if workspace is None or workspace.numel() < need:
-        workspace = torch.empty(need, dtype=torch.uint8, device=depth.device)
-    rc = hip().rvs_raster(_ptr(color), _ptr(depth), _ptr(uv), in_w, in_h, out_w, out_h, int(wrap), rescale,
-                          _ptr(workspace), need, tile_cap, _ptr(out_color), _ptr(out_depth), _ptr(out_shape),
-                          _stream(stream))
-    check(rc)
-    return workspace
+    while True:
+        need = hip().rvs_raster_workspace_size(out_w, out_h, tile_cap)
+        if workspace is None or workspace.numel() < need:
+            workspace = torch.empty(need, dtype=torch.uint8, device=depth.device)
+        needed = C.c_uint32(0)
+        rc = hip().rvs_raster(_ptr(color), _ptr(depth), _ptr(uv), in_w, in_h, out_w, out_h, int(wrap), rescale,
+                              _ptr(workspace), need, tile_cap, _ptr(out_color), _ptr(out_depth), _ptr(out_shape),
+                              C.byref(needed), _stream(stream))
+        if rc == RVS_E_CAPACITY and needed.value > 0:
+            tile_cap = 1 << int(needed.value - 1).bit_length()
+            continue
+        check(rc)
+        return workspace
 
 
 def blend_step(acc, color, depth, shape, exp, first, stream=None):
