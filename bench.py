@@ -47,6 +47,8 @@ def main():
     ap.add_argument("--workload", default="erp4_1080p")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--batch", type=int, default=64, help="views per batch of the batched measurement (0: skip)")
+    ap.add_argument("--batch-reps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it (got --gpus %d)"
@@ -129,6 +131,33 @@ def main():
             "traffic": None,
         },
     }
+
+    if args.batch > 0:
+        # batched: `batch` 1080p views at random-walk poses, one launch set per batch
+        tgts = W.batch_targets(rg, args.batch, seed=3, sizes=((rg["target"]["cam"].w, rg["target"]["cam"].h),))
+        bs = R.BatchSynthesizer(srcs, tgts)
+        bs.run()
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(args.batch_reps):
+            a = time.perf_counter()
+            bs.run()
+            times.append(time.perf_counter() - a)
+        R.profile_enable(True)
+        bs.run()
+        bstage = R.profile_read()
+        bcount = R.profile_counters()
+        R.profile_enable(False)
+        bt = float(np.median(times))
+        out["batched"] = {
+            "views": args.batch,
+            "views_s": round(args.batch / bt, 2),
+            "batch_ms": round(1e3 * bt, 3),
+            "stage_ms": {k: round(v, 4) for k, v in bstage.items()},
+            "counters": bcount,
+            "vs_plain": round(args.batch / bt / views_s, 2),
+        }
+        del bs
 
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "tests"))

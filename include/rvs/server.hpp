@@ -1,0 +1,82 @@
+// server.hpp -- serving loop: resident source sets, memory-budget admission,
+// continuous batching of target-view requests onto the batched device path.
+//
+// Requests (scene, target pose and camera) are queued on arrival.  A scheduler
+// thread keeps up to `slots` batches in flight, each on its own HIP stream and
+// in its own share of the memory budget: whenever a slot is free it takes the
+// oldest pending request's scene and admits further pending requests of that
+// scene in arrival order while the batch's workspace plus outputs fit the
+// slot's share (and at most max_batch views).  A batch is one
+// rvs_synthesize_batch_async launch set followed by device-to-host copies of
+// the YUV planes; requests that arrive meanwhile wait for the next free slot,
+// so batches on different slots overlap on the device.  Each request's
+// latency runs from its arrival to the completion of its host copy.  Results
+// equal those of a lone rvs_synthesize call for the same target.
+#ifndef RVS_SERVER_HPP
+#define RVS_SERVER_HPP
+
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "rvs_hip.h"
+#include "rvs_host.h"
+
+namespace rvs {
+
+class Server {
+public:
+    Server(std::vector<std::vector<rvs_source>> scenes, const rvs_serve_opts& opts);
+    ~Server();
+    Server(const Server&) = delete;
+    Server& operator=(const Server&) = delete;
+
+    /** Open loop: request k arrives at the k-th event of a Poisson process of
+     * `rate` per second (seeded); blocks until every request is done. */
+    int run_open(const rvs_request* reqs, int n, double rate, uint64_t seed);
+    /** Closed loop: `concurrency` requests outstanding at all times. */
+    int run_closed(const rvs_request* reqs, int n, int concurrency);
+    const std::vector<rvs_request_stats>& stats() const { return m_stats; }
+    /** Host copy of request k's YUV planes (keep_outputs). */
+    const std::vector<uint8_t>& output(int k) const { return m_out[k]; }
+    /** Message of the last failure in the current run (empty when none). */
+    std::string error() const;
+
+private:
+    struct Slot;
+    void reset(const rvs_request* reqs, int n);
+    void submit_locked(int k, double now);
+    void loop();
+    bool launch(Slot& s);
+    void finish(Slot& s);
+    double now_ms() const;
+
+    std::vector<std::vector<rvs_source>> m_scenes;
+    rvs_serve_opts m_opt;
+    std::vector<Slot*> m_slots;
+    std::mutex m_mu;
+    std::condition_variable m_cv;
+    std::thread m_thread;
+    bool m_stop = false;
+    // current run
+    const rvs_request* m_reqs = nullptr;
+    int m_n = 0;
+    int m_done = 0;
+    int m_next_closed = 0;    // closed loop: next request to submit on completion
+    bool m_closed = false;
+    int m_batches = 0;
+    std::deque<int> m_pending;
+    std::vector<rvs_request_stats> m_stats;
+    std::vector<std::vector<uint8_t>> m_out;
+    int64_t m_t0 = 0;
+    int m_error = 0;
+    std::string m_errmsg;
+};
+
+}  // namespace rvs
+
+#endif

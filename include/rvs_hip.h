@@ -102,10 +102,11 @@ int rvs_reproject(const rvs_cam* src, const rvs_cam* dst, const rvs_xform* xf, c
  * per-pixel winner equals the reference's sequential fold bit for bit.
  * color: src w*h*3, depth/uv on the source grid (as produced by
  * rvs_reproject).  Outputs at out_w*out_h: color*3, depth (+inf uncovered),
- * shape (triangle validity, 0 uncovered).  workspace: rvs_raster_workspace_size.
+ * shape (triangle validity, 0 uncovered).  workspace: rvs_raster_workspace_size
+ * (0 for invalid sizes).
  * Synchronises the stream; RVS_E_CAPACITY reports the needed tile_cap through
  * needed_cap (if not NULL) when a bin overflowed. */
-size_t rvs_raster_workspace_size(int out_w, int out_h, uint32_t tile_cap);
+size_t rvs_raster_workspace_size(int in_w, int in_h, int out_w, int out_h, uint32_t tile_cap);
 int rvs_raster(const float* color, const float* depth, const float* uv, int in_w, int in_h,
                int out_w, int out_h, int wrap, float rescale, void* workspace, size_t ws_bytes,
                uint32_t tile_cap, float* out_color, float* out_depth, float* out_shape,
@@ -170,30 +171,59 @@ typedef struct {
     void *y, *u, *v;          /* encoded YUV420 planes (u8/u16 by target bitdepth) */
 } rvs_view_out;
 
-/* Workspace bytes for rvs_synthesize with n_src sources of at most max_src_px
- * pixels each. tile_cap = per (source, tile) bin capacity (0 = default). */
-size_t rvs_synth_workspace_size(int n_src, const rvs_target* tgt, uint32_t tile_cap);
+/* Workspace bytes for rvs_synthesize over these sources and target (0 when an
+ * argument is invalid).  tile_cap = per (source, tile) bin capacity (0 = default). */
+size_t rvs_synth_workspace_size(const rvs_source* srcs, int n_src, const rvs_target* tgt,
+                                uint32_t tile_cap);
 
 /* Pipeline::computeView for one virtual view (Pipeline.cpp:169-301, CPU
  * semantics): for each source in order, reproject + rasterise + blend, then
  * inpaint and encode.  Sources must already be resident.  Returns
  * RVS_E_CAPACITY (and the needed tile_cap through needed_cap, if not NULL)
- * when a per-tile bin overflowed; this check synchronises the stream. */
+ * when a per-tile bin overflowed; this check synchronises the stream.
+ * Equivalent to rvs_synthesize_batch with one view. */
 int rvs_synthesize(const rvs_source* srcs, int n_src, const rvs_target* tgt, void* workspace,
                    size_t ws_bytes, uint32_t tile_cap, const rvs_view_out* out,
                    uint32_t* needed_cap, rvs_stream_t stream);
 
-/* Per-stage device times of the last rvs_synthesize on the calling thread,
- * measured with HIP events recorded on its stream between the stages when
- * enabled: [0] binning, [1] tile raster + blend (incl. in-order replay),
- * [2] inpainting, [3] encoding; milliseconds. */
+/* Batches: n_views virtual views (e.g. the poses of a pose trace, or requests
+ * admitted together by a server) synthesized from the same resident sources by
+ * ONE launch set.  Every stage runs once over all views (tiles and pixels are
+ * numbered across the batch), so per-launch costs are paid once per batch and
+ * small views still fill the device.  Views may differ in camera and size.
+ * Results are those of n_views separate rvs_synthesize calls.  At most
+ * 4096 views and 2^31 output pixels per batch. */
+size_t rvs_synth_batch_workspace_size(const rvs_source* srcs, int n_src, const rvs_target* tgts,
+                                      int n_views, uint32_t tile_cap);
+int rvs_synthesize_batch(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views,
+                         void* workspace, size_t ws_bytes, uint32_t tile_cap,
+                         const rvs_view_out* outs, uint32_t* needed_cap, rvs_stream_t stream);
+/* Enqueue only: no synchronisation and no host-device copies, so the call can be
+ * captured into a HIP graph (hipStreamBeginCapture) and replayed.  Descriptors
+ * travel as kernel arguments: a replay re-runs the captured poses and buffers.
+ * Check for bin overflow afterwards with rvs_synth_check (same arguments); an
+ * overflowed batch must be re-run with the reported tile_cap. */
+int rvs_synthesize_batch_async(const rvs_source* srcs, int n_src, const rvs_target* tgts,
+                               int n_views, void* workspace, size_t ws_bytes, uint32_t tile_cap,
+                               const rvs_view_out* outs, rvs_stream_t stream);
+int rvs_synth_check(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views,
+                    const void* workspace, uint32_t tile_cap, uint32_t* needed_cap,
+                    rvs_stream_t stream);
+
+/* Per-stage device times of the last rvs_synthesize* call on the calling
+ * thread, measured with HIP events recorded on its stream between the stages
+ * when enabled: [0] binning (incl. source bounds and culling), [1] tile raster
+ * + blend (incl. in-order replay), [2] inpainting, [3] encoding; milliseconds.
+ * After an _async call, synchronise the stream first. */
 #define RVS_PROF_STAGES 4
 int rvs_profile_enable(int on);
 int rvs_profile_read(float ms[RVS_PROF_STAGES]);
-/* Counters of the last profiled rvs_synthesize: [0] tiles with pixels replayed
- * in order, [1] bin overflow flag, [2] largest (source, tile) bin, [3] total
- * binned (quad, tile) entries over all sources.  Synchronises the device. */
-int rvs_profile_counters(uint32_t out[4]);
+/* Counters of the last profiled call: [0] tiles with pixels replayed in order,
+ * [1] bin overflow flag, [2] largest (source, tile) bin, [3] total binned
+ * (quad, tile) entries over all sources, [4] binning work items left after
+ * culling.  Synchronises the device. */
+#define RVS_PROF_COUNTERS 5
+int rvs_profile_counters(uint32_t out[RVS_PROF_COUNTERS]);
 
 #ifdef __cplusplus
 }

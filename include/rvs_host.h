@@ -44,6 +44,49 @@ int rvs_config_describe(const char* path, char* out, int cap);
  * frames or -1 (rvs_host_last_error). */
 int rvs_app_run(const char* config_path, const char* source_path, int stagewise);
 
+/* ------------------------------------------------------------------ serving
+ * A request loop over resident source sets ("scenes"): target-view requests are
+ * admitted by memory budget and batched as they arrive (continuous batching),
+ * each batch one rvs_synthesize_batch_async launch set on one of `slots`
+ * streams, so batches overlap on the device.  Latency of a request = arrival to
+ * the completion of its YUV copy to host memory. */
+typedef struct {
+    size_t memory_budget;       /* device bytes for in-flight batches (workspaces + outputs), split over slots */
+    int max_batch;              /* views per batch */
+    int slots;                  /* batches in flight (one stream each) */
+    uint32_t tile_cap;          /* initial per (source, tile) bin capacity (0 = default); grows on overflow */
+    int copy_to_host;           /* copy each request's YUV planes to pinned host memory */
+    int keep_outputs;           /* keep a host copy per request (rvs_server_output) */
+    size_t host_staging_bytes;  /* pinned staging per slot (0 = a quarter of the slot's share) */
+} rvs_serve_opts;
+
+typedef struct {
+    int scene;                  /* index of the resident source set */
+    rvs_target target;
+} rvs_request;
+
+typedef struct {
+    double arrival_ms, launch_ms, done_ms;   /* since the start of the run */
+    int batch;                  /* batch id */
+    int batch_views;            /* views in that batch */
+    int rc;                     /* RVS_OK or an error code */
+} rvs_request_stats;
+
+/* scenes[k] points to n_src[k] resident sources (device pointers stay owned by
+ * the caller and must outlive the server).  Returns NULL on failure. */
+void* rvs_server_create(const rvs_source* const* scenes, const int* n_src, int n_scenes, const rvs_serve_opts* opts);
+void rvs_server_destroy(void* server);
+/* Open loop: request k arrives at the k-th event of a seeded Poisson process of
+ * `rate` per second.  Closed loop: `concurrency` requests outstanding.  Both
+ * block until every request is done, fill stats[n] and return 0, or the last
+ * error code of a failed request (its stats say which). */
+int rvs_server_run_open(void* server, const rvs_request* reqs, int n, double rate, uint64_t seed,
+                        rvs_request_stats* stats);
+int rvs_server_run_closed(void* server, const rvs_request* reqs, int n, int concurrency, rvs_request_stats* stats);
+/* Host copy of request k's YUV planes (Y, U, V concatenated) from the last run
+ * (keep_outputs); returns the byte count, or -1. */
+long long rvs_server_output(void* server, int k, void* out, size_t cap);
+
 #ifdef __cplusplus
 }
 #endif

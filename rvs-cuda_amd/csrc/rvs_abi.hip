@@ -19,16 +19,15 @@ namespace {
 
 thread_local char g_err[512] = "";
 
-// stage events of rvs_synthesize (rvs_profile_enable)
+// stage events of rvs_synthesize* (rvs_profile_enable)
 thread_local bool g_prof = false;
 thread_local bool g_prof_valid = false;
 thread_local hipEvent_t g_ev[RVS_PROF_STAGES + 1] = {};
 thread_local rvs_k::RasterWS g_prof_ws;
-thread_local rvs_k::RasterDims g_prof_dims;
-thread_local int g_prof_nsrc = 0;
+thread_local rvs_k::BatchDims g_prof_dims;
 
 void prof_mark(int k, hipStream_t s) {
-    if (g_prof) hipEventRecord(g_ev[k], s);
+    if (g_prof) (void)hipEventRecord(g_ev[k], s);
 }
 
 int fail(int code, const char* fmt, ...) {
@@ -102,53 +101,73 @@ void fill_source(PairGeom& g, const rvs_cam* src, const float* tab) {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-struct RasterLayout {
-    size_t counts, lists, deferred, dtiles, status, total;
+// Sequential sub-allocation of a workspace, 256-byte aligned.
+struct Layout {
+    size_t o = 0;
+    size_t take(size_t bytes) {
+        const size_t r = o;
+        o += align256(bytes);
+        return r;
+    }
 };
 
-RasterLayout raster_layout(int nsrc, int out_w, int out_h, uint32_t cap) {
-    const int tiles_x = (out_w + rvs_k::kTile - 1) / rvs_k::kTile;
-    const int tiles_y = (out_h + rvs_k::kTile - 1) / rvs_k::kTile;
-    const size_t nt = (size_t)tiles_x * tiles_y;
-    RasterLayout L;
-    size_t o = 0;
-    L.status = o; o += align256(16 * sizeof(uint32_t));
-    L.counts = o; o += align256(nsrc * nt * sizeof(uint32_t));
-    L.lists = o; o += align256((size_t)nsrc * nt * cap * sizeof(uint32_t));
-    L.deferred = o; o += align256((size_t)out_w * out_h);
-    L.dtiles = o; o += align256(nt * sizeof(uint32_t));
-    L.total = o;
-    return L;
+// Raster state of one launch set: status words, bins, deferred-pixel flags,
+// deferred-tile list and binning work list.
+struct RasterPlan {
+    rvs_k::BatchDims d;
+    size_t status, counts, lists, deferred, dtiles, work;
+};
+
+void plan_raster(Layout& L, RasterPlan& R, int nsrc, int nviews, uint32_t ntiles, uint32_t npx, uint32_t cap,
+                 int max_sw, int max_sh) {
+    rvs_k::BatchDims& d = R.d;
+    d.nviews = nviews; d.nsrc = nsrc; d.ntiles = ntiles; d.npx = npx; d.cap = cap;
+    d.max_sw = max_sw; d.max_sh = max_sh;
+    d.bins_x = (uint32_t)((max_sw - 1 + rvs_k::kBinQX - 1) / rvs_k::kBinQX);
+    d.bins_y = (uint32_t)((max_sh - 1 + rvs_k::kBinQY - 1) / rvs_k::kBinQY);
+    R.status = L.take(16 * sizeof(uint32_t));
+    R.counts = L.take((size_t)nsrc * ntiles * sizeof(uint32_t));
+    R.lists = L.take((size_t)nsrc * ntiles * cap * sizeof(uint32_t));
+    R.deferred = L.take(npx);
+    R.dtiles = L.take((size_t)ntiles * sizeof(uint32_t));
+    R.work = L.take((size_t)d.bins_x * d.bins_y * nsrc * nviews * sizeof(uint32_t));
 }
 
-rvs_k::RasterDims dims_of(int out_w, int out_h, uint32_t cap) {
-    rvs_k::RasterDims d;
-    d.out_w = out_w; d.out_h = out_h;
-    d.tiles_x = (out_w + rvs_k::kTile - 1) / rvs_k::kTile;
-    d.tiles_y = (out_h + rvs_k::kTile - 1) / rvs_k::kTile;
-    d.ntiles = d.tiles_x * d.tiles_y;
-    d.cap = cap;
-    return d;
-}
-
-rvs_k::RasterWS carve_raster(char* base, const RasterLayout& L) {
+rvs_k::RasterWS carve_raster(char* base, const RasterPlan& R) {
     rvs_k::RasterWS ws;
-    ws.status = (uint32_t*)(base + L.status);
-    ws.counts = (uint32_t*)(base + L.counts);
-    ws.lists = (uint32_t*)(base + L.lists);
-    ws.deferred = (uint8_t*)(base + L.deferred);
-    ws.deferred_tiles = (uint32_t*)(base + L.dtiles);
+    ws.status = (uint32_t*)(base + R.status);
+    ws.counts = (uint32_t*)(base + R.counts);
+    ws.lists = (uint32_t*)(base + R.lists);
+    ws.deferred = (uint8_t*)(base + R.deferred);
+    ws.deferred_tiles = (uint32_t*)(base + R.dtiles);
+    ws.work = (uint32_t*)(base + R.work);
     return ws;
 }
 
+hipError_t reset_raster(const rvs_k::RasterWS& ws, const rvs_k::BatchDims& d, hipStream_t s) {
+    hipError_t e = rvs_k::launch_reset_u32(ws.status, 16, s);
+    if (e != hipSuccess) return e;
+    return rvs_k::launch_reset_u32(ws.counts, (size_t)d.nsrc * d.ntiles, s);
+}
+
+// A view's tile grid and its place in the batch's global tile / pixel numbering.
+void view_grid(rvs_k::ViewDesc& v, int w, int h, uint32_t& tiles, uint32_t& px) {
+    v.w = w; v.h = h;
+    v.tiles_x = (w + rvs_k::kTile - 1) / rvs_k::kTile;
+    v.tiles_y = (h + rvs_k::kTile - 1) / rvs_k::kTile;
+    v.tile_base = tiles;
+    v.px_base = px;
+    tiles += (uint32_t)(v.tiles_x * v.tiles_y);
+    px += (uint32_t)(w * h);
+}
+
 // After a raster launch set: report bin overflow (synchronises the stream).
-int check_raster_status(const rvs_k::RasterWS& ws, const rvs_k::RasterDims& d, int nsrc, uint32_t* needed_cap,
-                        hipStream_t s) {
+int check_raster_status(const rvs_k::RasterWS& ws, const rvs_k::BatchDims& d, uint32_t* needed_cap, hipStream_t s) {
     uint32_t st[4];
     HIP_TRY(hipMemcpyAsync(st, ws.status, sizeof st, hipMemcpyDeviceToHost, s), "status copy");
     HIP_TRY(hipStreamSynchronize(s), "stream sync");
     if (st[1]) {
-        std::vector<uint32_t> counts((size_t)nsrc * d.ntiles);
+        std::vector<uint32_t> counts((size_t)d.nsrc * d.ntiles);
         HIP_TRY(hipMemcpy(counts.data(), ws.counts, counts.size() * 4, hipMemcpyDeviceToHost), "counts copy");
         uint32_t mx = 0;
         for (uint32_t c : counts) mx = c > mx ? c : mx;
@@ -157,6 +176,143 @@ int check_raster_status(const rvs_k::RasterWS& ws, const rvs_k::RasterDims& d, i
     }
     if (st[3]) return fail(RVS_E_CAPACITY, "in-order replay of a tile bin of %u quads exceeds %d", st[3],
                            rvs_k::kFallbackSortCap);
+    return RVS_OK;
+}
+
+// ------------------------------------------------------------- batches
+
+// Everything a fused launch set over n_views targets and n_src sources needs,
+// in workspace order.
+struct SynthPlan {
+    RasterPlan R;
+    size_t views, geoms, bounds, quality, inpaint, total;
+    int max_w, max_h;
+    bool any_persp;
+};
+
+int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views, uint32_t cap, SynthPlan& P,
+               std::vector<rvs_k::ViewDesc>* views) {
+    if (!srcs || n_src < 1 || n_src > RVS_MAX_SOURCES || !tgts || n_views < 1 || n_views > rvs_k::kMaxViews)
+        return fail(RVS_E_INVALID, "synthesize: bad source or view count");
+    int max_sw = 0, max_sh = 0;
+    for (int k = 0; k < n_src; ++k) {
+        if (!cam_ok(&srcs[k].cam)) return fail(RVS_E_INVALID, "synthesize: source %d has a bad camera", k);
+        max_sw = srcs[k].cam.w > max_sw ? srcs[k].cam.w : max_sw;
+        max_sh = srcs[k].cam.h > max_sh ? srcs[k].cam.h : max_sh;
+    }
+    uint64_t tiles = 0, px = 0;
+    P.max_w = 0; P.max_h = 0; P.any_persp = false;
+    for (int v = 0; v < n_views; ++v) {
+        const rvs_target& t = tgts[v];
+        if (!cam_ok(&t.cam)) return fail(RVS_E_INVALID, "synthesize: view %d has a bad camera", v);
+        if (t.rescale != 1.0f) return fail(RVS_E_UNSUPPORTED, "synthesize: Precision != 1 is not supported");
+        P.max_w = t.cam.w > P.max_w ? t.cam.w : P.max_w;
+        P.max_h = t.cam.h > P.max_h ? t.cam.h : P.max_h;
+        P.any_persp = P.any_persp || t.cam.proj == 0;
+        tiles += (uint64_t)((t.cam.w + rvs_k::kTile - 1) / rvs_k::kTile) * ((t.cam.h + rvs_k::kTile - 1) / rvs_k::kTile);
+        px += (uint64_t)t.cam.w * t.cam.h;
+    }
+    const uint64_t items = (uint64_t)((max_sw - 1 + rvs_k::kBinQX - 1) / rvs_k::kBinQX) *
+                           ((max_sh - 1 + rvs_k::kBinQY - 1) / rvs_k::kBinQY) * n_src * n_views;
+    if (px >= (1ull << 31) || tiles * n_src >= (1ull << 31) || items >= (1ull << 32) ||
+        (uint64_t)(max_sw - 1) * (max_sh - 1) >= (1ull << 30))
+        return fail(RVS_E_CAPACITY, "synthesize: batch too large (%llu pixels); split it",
+                    (unsigned long long)px);
+    Layout L;
+    P.views = L.take((size_t)n_views * sizeof(rvs_k::ViewDesc));
+    P.geoms = L.take((size_t)n_views * n_src * sizeof(PairGeom));
+    const size_t per = (size_t)((max_sw - 1 + rvs_k::kBinQX - 1) / rvs_k::kBinQX) *
+                       ((max_sh - 1 + rvs_k::kBinQY - 1) / rvs_k::kBinQY);
+    P.bounds = L.take(P.any_persp ? per * n_src * 6 * sizeof(float) : 0);
+    P.quality = L.take((size_t)px * sizeof(float));
+    plan_raster(L, P.R, n_src, n_views, (uint32_t)tiles, (uint32_t)px, cap, max_sw, max_sh);
+    P.inpaint = L.take(rvs_k::inpaint_ws_bytes((size_t)px, n_views));
+    P.total = L.o;
+    if (views) {
+        views->resize(n_views);
+        uint32_t tb = 0, pb = 0;
+        for (int v = 0; v < n_views; ++v) {
+            rvs_k::ViewDesc& d = (*views)[v];
+            std::memset(&d, 0, sizeof d);
+            view_grid(d, tgts[v].cam.w, tgts[v].cam.h, tb, pb);
+            d.bitdepth = tgts[v].bitdepth;
+            d.blend_exp = tgts[v].blending_exp;
+        }
+    }
+    return RVS_OK;
+}
+
+void fill_pair(PairGeom& g, const rvs_source& S, const rvs_target& T) {
+    std::memset(&g, 0, sizeof g);
+    fill_source(g, &S.cam, S.erp_table);
+    g.depth = S.depth;
+    g.depth_u8 = S.bitdepth_depth <= 8;
+    g.depth_f32 = 0;
+    g.dscale = T.depth_scale;
+    g.znear = S.znear; g.zfar = S.zfar; g.has_invalid = S.has_invalid;
+    g.py = S.y; g.pu = S.u; g.pv = S.v;
+    g.color_u8 = S.bitdepth_color <= 8;
+    g.cscale = T.color_scale;
+    rvs_xform xf;
+    rvs_relative_pose(S.pos, S.rot, T.pos, T.rot, &xf);
+    std::memcpy(g.R, xf.R, sizeof g.R);
+    std::memcpy(g.t, xf.t, sizeof g.t);
+    fill_target(g, &T.cam, T.rescale);
+}
+
+// Enqueues the fused launch set; returns after the last launch (no sync).
+int enqueue_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views, void* workspace,
+                  size_t ws_bytes, uint32_t cap, const rvs_view_out* outs, hipStream_t s, SynthPlan& P,
+                  rvs_k::RasterWS& ws) {
+    std::vector<rvs_k::ViewDesc> views;
+    const int prc = plan_synth(srcs, n_src, tgts, n_views, cap, P, &views);
+    if (prc != RVS_OK) return prc;
+    if (!workspace || !outs) return fail(RVS_E_INVALID, "synthesize: null workspace or outputs");
+    if (ws_bytes < P.total) return fail(RVS_E_INVALID, "synthesize: workspace %zu < %zu bytes", ws_bytes, P.total);
+    for (int k = 0; k < n_src; ++k) {
+        const rvs_source& S = srcs[k];
+        if (!S.y || !S.u || !S.v || !S.depth || (S.cam.proj == 1 && !S.erp_table))
+            return fail(RVS_E_INVALID, "synthesize: source %d incomplete", k);
+    }
+    char* base = (char*)workspace;
+    float* quality = (float*)(base + P.quality);
+    for (int v = 0; v < n_views; ++v) {
+        const rvs_view_out& o = outs[v];
+        rvs_k::ViewDesc& d = views[v];
+        if (!o.color) return fail(RVS_E_INVALID, "synthesize: view %d has no colour output", v);
+        const bool enc = o.y && o.u && o.v;
+        if (enc && ((d.w & 1) || (d.h & 1)))
+            return fail(RVS_E_UNSUPPORTED, "synthesize: odd target size cannot be encoded (view %d)", v);
+        if (enc && (d.bitdepth < 1 || d.bitdepth > 16))
+            return fail(RVS_E_INVALID, "synthesize: view %d output bit depth %d", v, d.bitdepth);
+        d.color = o.color;
+        d.quality = o.quality ? o.quality : quality + d.px_base;
+        d.depth = o.depth;
+        d.validity = o.validity;
+        d.y = enc ? o.y : nullptr; d.u = enc ? o.u : nullptr; d.v = enc ? o.v : nullptr;
+    }
+    std::vector<PairGeom> hg((size_t)n_views * n_src);
+    for (int v = 0; v < n_views; ++v)
+        for (int k = 0; k < n_src; ++k) fill_pair(hg[(size_t)v * n_src + k], srcs[k], tgts[v]);
+    rvs_k::ViewDesc* dviews = (rvs_k::ViewDesc*)(base + P.views);
+    PairGeom* dg = (PairGeom*)(base + P.geoms);
+    float* bounds = P.any_persp ? (float*)(base + P.bounds) : nullptr;
+    ws = carve_raster(base, P.R);
+    const rvs_k::BatchDims& d = P.R.d;
+    HIP_TRY(rvs_k::launch_upload(views.data(), views.size() * sizeof(rvs_k::ViewDesc), dviews, s), "view upload");
+    HIP_TRY(rvs_k::launch_upload(hg.data(), hg.size() * sizeof(PairGeom), dg, s), "geometry upload");
+    HIP_TRY(reset_raster(ws, d, s), "reset");
+    prof_mark(0, s);
+    if (bounds) HIP_TRY(rvs_k::launch_src_bounds(dg, n_src, d, bounds, s), "source bounds");
+    HIP_TRY(rvs_k::launch_bin<PairGeom>(dg, dviews, bounds, d, ws, s), "bin");
+    prof_mark(1, s);
+    HIP_TRY((rvs_k::launch_raster<PairGeom, rvs_k::MODE_BLEND>(dg, dviews, d, ws, s)), "raster");
+    prof_mark(2, s);
+    const rvs_k::InpaintWS iws = rvs_k::inpaint_ws_carve(base + P.inpaint, d.npx, n_views);
+    HIP_TRY(rvs_k::launch_inpaint(dviews, n_views, P.max_w, P.max_h, iws, nullptr, s), "inpaint");
+    prof_mark(3, s);
+    HIP_TRY(rvs_k::launch_encode_views(dviews, n_views, P.max_w, P.max_h, s), "encode");
+    prof_mark(4, s);
     return RVS_OK;
 }
 
@@ -248,9 +404,24 @@ int rvs_reproject(const rvs_cam* src, const rvs_cam* dst, const rvs_xform* xf, c
     return RVS_OK;
 }
 
-size_t rvs_raster_workspace_size(int out_w, int out_h, uint32_t tile_cap) {
+// Workspace of the raster stage entry point: one view, one precomputed source.
+static size_t raster_plan(int in_w, int in_h, int out_w, int out_h, uint32_t cap, RasterPlan& R, size_t& views,
+                          size_t& geom) {
+    Layout L;
+    views = L.take(sizeof(rvs_k::ViewDesc));
+    geom = L.take(sizeof(ArrayGeom));
+    const uint32_t ntiles = (uint32_t)(((out_w + rvs_k::kTile - 1) / rvs_k::kTile) *
+                                       ((out_h + rvs_k::kTile - 1) / rvs_k::kTile));
+    plan_raster(L, R, 1, 1, ntiles, (uint32_t)out_w * (uint32_t)out_h, cap, in_w, in_h);
+    return L.o;
+}
+
+size_t rvs_raster_workspace_size(int in_w, int in_h, int out_w, int out_h, uint32_t tile_cap) {
+    if (in_w < 2 || in_h < 2 || out_w < 1 || out_h < 1) return 0;
     if (tile_cap == 0) tile_cap = rvs_k::kDefaultTileCap;
-    return raster_layout(1, out_w, out_h, tile_cap).total + align256(sizeof(ArrayGeom));
+    RasterPlan R;
+    size_t a, b;
+    return raster_plan(in_w, in_h, out_w, out_h, tile_cap, R, a, b);
 }
 
 int rvs_raster(const float* color, const float* depth, const float* uv, int in_w, int in_h, int out_w, int out_h,
@@ -259,26 +430,32 @@ int rvs_raster(const float* color, const float* depth, const float* uv, int in_w
     if (!color || !depth || !uv || in_w < 2 || in_h < 2 || out_w < 1 || out_h < 1 || !workspace || !out_color ||
         !out_depth || !out_shape || !(rescale > 0.f))
         return fail(RVS_E_INVALID, "rvs_raster: bad argument");
+    if ((uint64_t)out_w * out_h >= (1ull << 31) || (uint64_t)(in_w - 1) * (in_h - 1) >= (1ull << 30))
+        return fail(RVS_E_CAPACITY, "rvs_raster: image too large");
     if (tile_cap == 0) tile_cap = rvs_k::kDefaultTileCap;
-    const size_t need = rvs_raster_workspace_size(out_w, out_h, tile_cap);
+    RasterPlan R;
+    size_t ov, og;
+    const size_t need = raster_plan(in_w, in_h, out_w, out_h, tile_cap, R, ov, og);
     if (ws_bytes < need) return fail(RVS_E_INVALID, "rvs_raster: workspace %zu < %zu bytes", ws_bytes, need);
     hipStream_t s = (hipStream_t)stream;
-    const RasterLayout L = raster_layout(1, out_w, out_h, tile_cap);
     char* base = (char*)workspace;
-    rvs_k::RasterWS ws = carve_raster(base, L);
-    const rvs_k::RasterDims d = dims_of(out_w, out_h, tile_cap);
-    ArrayGeom* dg = (ArrayGeom*)(base + L.total);
+    rvs_k::RasterWS ws = carve_raster(base, R);
+    rvs_k::ViewDesc vd;
+    std::memset(&vd, 0, sizeof vd);
+    uint32_t tb = 0, pb = 0;
+    view_grid(vd, out_w, out_h, tb, pb);
+    vd.color = out_color; vd.depth = out_depth; vd.validity = out_shape;
     ArrayGeom hg;
     hg.sw = in_w; hg.sh = in_h; hg.uv = uv; hg.vd = depth; hg.color = color; hg.wrap = wrap ? 1 : 0;
     hg.rescale = rescale;
-    HIP_TRY(hipMemcpyAsync(dg, &hg, sizeof hg, hipMemcpyHostToDevice, s), "geom upload");
-    HIP_TRY(rvs_k::launch_reset_u32(ws.status, 16, s), "reset");
-    HIP_TRY(rvs_k::launch_reset_u32(ws.counts, (size_t)d.ntiles, s), "reset");
-    HIP_TRY(rvs_k::launch_bin<ArrayGeom>(dg, 1, in_w, in_h, d, ws, s), "bin");
-    rvs_k::RasterOut o;
-    o.color = out_color; o.depth = out_depth; o.quality = nullptr; o.validity = out_shape;
-    HIP_TRY((rvs_k::launch_raster<ArrayGeom, rvs_k::MODE_RAW>(dg, 1, d, ws, 0.f, o, s)), "raster");
-    return check_raster_status(ws, d, 1, needed_cap, s);
+    rvs_k::ViewDesc* dv = (rvs_k::ViewDesc*)(base + ov);
+    ArrayGeom* dg = (ArrayGeom*)(base + og);
+    HIP_TRY(rvs_k::launch_upload(&vd, sizeof vd, dv, s), "view upload");
+    HIP_TRY(rvs_k::launch_upload(&hg, sizeof hg, dg, s), "geometry upload");
+    HIP_TRY(reset_raster(ws, R.d, s), "reset");
+    HIP_TRY(rvs_k::launch_bin<ArrayGeom>(dg, dv, nullptr, R.d, ws, s), "bin");
+    HIP_TRY((rvs_k::launch_raster<ArrayGeom, rvs_k::MODE_RAW>(dg, dv, R.d, ws, s)), "raster");
+    return check_raster_status(ws, R.d, needed_cap, s);
 }
 
 int rvs_blend_step(float* acc_color, float* acc_quality, float* acc_depth, float* acc_validity, uint8_t* acc_mask,
@@ -293,14 +470,28 @@ int rvs_blend_step(float* acc_color, float* acc_quality, float* acc_depth, float
     return RVS_OK;
 }
 
-size_t rvs_inpaint_workspace_size(int w, int h) { return rvs_k::inpaint_ws_bytes(w, h); }
+size_t rvs_inpaint_workspace_size(int w, int h) {
+    if (w < 1 || h < 1) return 0;
+    return align256(sizeof(rvs_k::ViewDesc)) + align256(rvs_k::inpaint_ws_bytes((size_t)w * h, 1));
+}
 
 int rvs_inpaint(float* color, const float* quality, int w, int h, void* workspace, size_t ws_bytes, int32_t* map_out,
                 rvs_stream_t stream) {
     if (!color || !quality || w < 1 || h < 1 || !workspace) return fail(RVS_E_INVALID, "rvs_inpaint: bad argument");
-    if (ws_bytes < rvs_k::inpaint_ws_bytes(w, h)) return fail(RVS_E_INVALID, "rvs_inpaint: workspace too small");
-    const rvs_k::InpaintWS ws = rvs_k::inpaint_ws_carve(workspace, w, h);
-    HIP_TRY(rvs_k::launch_inpaint(color, quality, w, h, ws, map_out, (hipStream_t)stream), "inpaint");
+    if ((uint64_t)w * h >= (1ull << 31)) return fail(RVS_E_CAPACITY, "rvs_inpaint: image too large");
+    if (ws_bytes < rvs_inpaint_workspace_size(w, h)) return fail(RVS_E_INVALID, "rvs_inpaint: workspace too small");
+    hipStream_t s = (hipStream_t)stream;
+    rvs_k::ViewDesc vd;
+    std::memset(&vd, 0, sizeof vd);
+    uint32_t tb = 0, pb = 0;
+    view_grid(vd, w, h, tb, pb);
+    vd.color = color;
+    vd.quality = (float*)quality;
+    rvs_k::ViewDesc* dv = (rvs_k::ViewDesc*)workspace;
+    const rvs_k::InpaintWS ws =
+        rvs_k::inpaint_ws_carve((char*)workspace + align256(sizeof(rvs_k::ViewDesc)), (size_t)w * h, 1);
+    HIP_TRY(rvs_k::launch_upload(&vd, sizeof vd, dv, s), "view upload");
+    HIP_TRY(rvs_k::launch_inpaint(dv, 1, w, h, ws, map_out, s), "inpaint");
     return RVS_OK;
 }
 
@@ -314,102 +505,72 @@ int rvs_encode_yuv420(const float* color, int w, int h, int bitdepth, void* y, v
 
 // ------------------------------------------------------------ fused path
 
-static size_t synth_layout(int n_src, int w, int h, uint32_t cap, size_t* off_geom, size_t* off_raster,
-                           size_t* off_inp, size_t* off_q) {
-    size_t o = 0;
-    *off_geom = o; o += align256((size_t)n_src * sizeof(PairGeom));
-    *off_q = o; o += align256((size_t)w * h * sizeof(float));
-    *off_raster = o; o += raster_layout(n_src, w, h, cap).total;
-    *off_inp = o; o += align256(rvs_k::inpaint_ws_bytes(w, h));
-    return o;
+size_t rvs_synth_batch_workspace_size(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views,
+                                      uint32_t tile_cap) {
+    if (tile_cap == 0) tile_cap = rvs_k::kDefaultTileCap;
+    SynthPlan P;
+    return plan_synth(srcs, n_src, tgts, n_views, tile_cap, P, nullptr) == RVS_OK ? P.total : 0;
 }
 
-size_t rvs_synth_workspace_size(int n_src, const rvs_target* tgt, uint32_t tile_cap) {
-    if (!tgt || n_src < 1) return 0;
+size_t rvs_synth_workspace_size(const rvs_source* srcs, int n_src, const rvs_target* tgt, uint32_t tile_cap) {
+    return rvs_synth_batch_workspace_size(srcs, n_src, tgt, 1, tile_cap);
+}
+
+int rvs_synthesize_batch_async(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views,
+                               void* workspace, size_t ws_bytes, uint32_t tile_cap, const rvs_view_out* outs,
+                               rvs_stream_t stream) {
     if (tile_cap == 0) tile_cap = rvs_k::kDefaultTileCap;
-    size_t a, b, c, d;
-    return synth_layout(n_src, tgt->cam.w, tgt->cam.h, tile_cap, &a, &b, &c, &d);
+    SynthPlan P;
+    rvs_k::RasterWS ws;
+    g_prof_valid = false;
+    const int rc = enqueue_synth(srcs, n_src, tgts, n_views, workspace, ws_bytes, tile_cap, outs,
+                                 (hipStream_t)stream, P, ws);
+    if (rc == RVS_OK && g_prof) { g_prof_valid = true; g_prof_ws = ws; g_prof_dims = P.R.d; }
+    return rc;
+}
+
+int rvs_synth_check(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views, const void* workspace,
+                    uint32_t tile_cap, uint32_t* needed_cap, rvs_stream_t stream) {
+    if (tile_cap == 0) tile_cap = rvs_k::kDefaultTileCap;
+    if (!workspace) return fail(RVS_E_INVALID, "rvs_synth_check: null workspace");
+    SynthPlan P;
+    const int prc = plan_synth(srcs, n_src, tgts, n_views, tile_cap, P, nullptr);
+    if (prc != RVS_OK) return prc;
+    const rvs_k::RasterWS ws = carve_raster((char*)workspace, P.R);
+    return check_raster_status(ws, P.R.d, needed_cap, (hipStream_t)stream);
+}
+
+int rvs_synthesize_batch(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views, void* workspace,
+                         size_t ws_bytes, uint32_t tile_cap, const rvs_view_out* outs, uint32_t* needed_cap,
+                         rvs_stream_t stream) {
+    if (tile_cap == 0) tile_cap = rvs_k::kDefaultTileCap;
+    SynthPlan P;
+    rvs_k::RasterWS ws;
+    g_prof_valid = false;
+    const int rc = enqueue_synth(srcs, n_src, tgts, n_views, workspace, ws_bytes, tile_cap, outs,
+                                 (hipStream_t)stream, P, ws);
+    if (rc != RVS_OK) return rc;
+    const int crc = check_raster_status(ws, P.R.d, needed_cap, (hipStream_t)stream);
+    if (crc == RVS_OK && g_prof) { g_prof_valid = true; g_prof_ws = ws; g_prof_dims = P.R.d; }
+    return crc;
 }
 
 int rvs_synthesize(const rvs_source* srcs, int n_src, const rvs_target* tgt, void* workspace, size_t ws_bytes,
                    uint32_t tile_cap, const rvs_view_out* out, uint32_t* needed_cap, rvs_stream_t stream) {
-    if (!srcs || n_src < 1 || n_src > RVS_MAX_SOURCES || !tgt || !cam_ok(&tgt->cam) || !workspace || !out ||
-        !out->color)
-        return fail(RVS_E_INVALID, "rvs_synthesize: bad argument");
-    if (tgt->rescale != 1.0f) return fail(RVS_E_UNSUPPORTED, "rvs_synthesize: Precision != 1 is not supported");
-    if (tile_cap == 0) tile_cap = rvs_k::kDefaultTileCap;
-    const int w = tgt->cam.w, h = tgt->cam.h;
-    size_t og, orr, oi, oq;
-    const size_t need = synth_layout(n_src, w, h, tile_cap, &og, &orr, &oi, &oq);
-    if (ws_bytes < need) return fail(RVS_E_INVALID, "rvs_synthesize: workspace %zu < %zu bytes", ws_bytes, need);
-    hipStream_t s = (hipStream_t)stream;
-    char* base = (char*)workspace;
-    std::vector<PairGeom> hg(n_src);
-    int max_sw = 0, max_sh = 0;
-    for (int k = 0; k < n_src; ++k) {
-        const rvs_source& S = srcs[k];
-        if (!cam_ok(&S.cam) || !S.y || !S.u || !S.v || !S.depth || (S.cam.proj == 1 && !S.erp_table))
-            return fail(RVS_E_INVALID, "rvs_synthesize: source %d incomplete", k);
-        PairGeom& g = hg[k];
-        std::memset(&g, 0, sizeof g);
-        fill_source(g, &S.cam, S.erp_table);
-        g.depth = S.depth;
-        g.depth_u8 = S.bitdepth_depth <= 8;
-        g.depth_f32 = 0;
-        g.dscale = tgt->depth_scale;
-        g.znear = S.znear; g.zfar = S.zfar; g.has_invalid = S.has_invalid;
-        g.py = S.y; g.pu = S.u; g.pv = S.v;
-        g.color_u8 = S.bitdepth_color <= 8;
-        g.cscale = tgt->color_scale;
-        rvs_xform xf;
-        rvs_relative_pose(S.pos, S.rot, tgt->pos, tgt->rot, &xf);
-        std::memcpy(g.R, xf.R, sizeof g.R);
-        std::memcpy(g.t, xf.t, sizeof g.t);
-        fill_target(g, &tgt->cam, tgt->rescale);
-        max_sw = S.cam.w > max_sw ? S.cam.w : max_sw;
-        max_sh = S.cam.h > max_sh ? S.cam.h : max_sh;
-    }
-    g_prof_valid = false;
-    PairGeom* dg = (PairGeom*)(base + og);
-    HIP_TRY(rvs_k::launch_store_geoms(hg.data(), n_src, dg, s), "store geoms");
-    const RasterLayout L = raster_layout(n_src, w, h, tile_cap);
-    rvs_k::RasterWS ws = carve_raster(base + orr, L);
-    const rvs_k::RasterDims d = dims_of(w, h, tile_cap);
-    HIP_TRY(rvs_k::launch_reset_u32(ws.status, 16, s), "reset");
-    HIP_TRY(rvs_k::launch_reset_u32(ws.counts, (size_t)n_src * d.ntiles, s), "reset");
-    prof_mark(0, s);
-    HIP_TRY(rvs_k::launch_bin<PairGeom>(dg, n_src, max_sw, max_sh, d, ws, s), "bin");
-    prof_mark(1, s);
-    float* quality = out->quality ? out->quality : (float*)(base + oq);
-    rvs_k::RasterOut ro;
-    ro.color = out->color; ro.depth = out->depth; ro.quality = quality; ro.validity = out->validity;
-    HIP_TRY((rvs_k::launch_raster<PairGeom, rvs_k::MODE_BLEND>(dg, n_src, d, ws, tgt->blending_exp, ro, s)),
-            "raster");
-    prof_mark(2, s);
-    const rvs_k::InpaintWS iws = rvs_k::inpaint_ws_carve(base + oi, w, h);
-    HIP_TRY(rvs_k::launch_inpaint(out->color, quality, w, h, iws, nullptr, s), "inpaint");
-    prof_mark(3, s);
-    if (out->y && out->u && out->v) {
-        if ((w & 1) || (h & 1)) return fail(RVS_E_UNSUPPORTED, "rvs_synthesize: odd target size cannot be encoded");
-        HIP_TRY(rvs_k::launch_encode(out->color, w, h, tgt->bitdepth, out->y, out->u, out->v, s), "encode");
-    }
-    prof_mark(4, s);
-    const int rc = check_raster_status(ws, d, n_src, needed_cap, s);
-    g_prof_valid = g_prof && rc == RVS_OK;
-    if (g_prof_valid) { g_prof_ws = ws; g_prof_dims = d; g_prof_nsrc = n_src; }
-    return rc;
+    return rvs_synthesize_batch(srcs, n_src, tgt, 1, workspace, ws_bytes, tile_cap, out, needed_cap, stream);
 }
 
-int rvs_profile_counters(uint32_t out[4]) {
+int rvs_profile_counters(uint32_t out[RVS_PROF_COUNTERS]) {
     if (!g_prof_valid) return fail(RVS_E_INVALID, "no profiled rvs_synthesize on this thread");
-    uint32_t st[4];
+    uint32_t st[8];
     HIP_TRY(hipMemcpy(st, g_prof_ws.status, sizeof st, hipMemcpyDeviceToHost), "status copy");
-    std::vector<uint32_t> counts((size_t)g_prof_nsrc * g_prof_dims.ntiles);
+    std::vector<uint32_t> counts((size_t)g_prof_dims.nsrc * g_prof_dims.ntiles);
     HIP_TRY(hipMemcpy(counts.data(), g_prof_ws.counts, counts.size() * 4, hipMemcpyDeviceToHost), "counts copy");
     uint32_t mx = 0;
     uint64_t tot = 0;
     for (uint32_t c : counts) { mx = c > mx ? c : mx; tot += c; }
     out[0] = st[0]; out[1] = st[1]; out[2] = mx; out[3] = (uint32_t)(tot > 0xffffffffull ? 0xffffffffu : tot);
+    out[4] = st[4];
     return RVS_OK;
 }
 

@@ -83,8 +83,9 @@ __device__ inline uint32_t load_plane(const void* p, int u8, size_t idx) {
     return u8 ? (uint32_t)((const uint8_t*)p)[idx] : (uint32_t)((const uint16_t*)p)[idx];
 }
 
-// u', v', target depth of source pixel (i = row, j = column).
-__device__ inline float3 vertex(const PairGeom& g, int i, int j) {
+// Source pixel (i = row, j = column) unprojected into the source camera frame
+// (decode depth, then EquirectangularUnprojector / PerspectiveUnprojector).
+__device__ inline float3 unproject_src(const PairGeom& g, int i, int j) {
     const size_t idx = (size_t)i * g.sw + j;
     const float d = g.depth_f32 ? ((const float*)g.depth)[idx]
                                 : decode_depth_raw(load_plane(g.depth, g.depth_u8, idx), g.dscale, g.znear,
@@ -108,10 +109,15 @@ __device__ inline float3 vertex(const PairGeom& g, int i, int j) {
             x = y = z = __builtin_nanf("");
         }
     }
+    return make_float3(x, y, z);
+}
+
+// Source-frame point -> target u', v' (after the uv rescale) and target depth.
+__device__ inline float3 project_tgt(const PairGeom& g, float3 p) {
     float X = 0.f, Y = 0.f, Z = 0.f;
-    X += g.R[0] * x; X += g.R[1] * y; X += g.R[2] * z;
-    Y += g.R[3] * x; Y += g.R[4] * y; Y += g.R[5] * z;
-    Z += g.R[6] * x; Z += g.R[7] * y; Z += g.R[8] * z;
+    X += g.R[0] * p.x; X += g.R[1] * p.y; X += g.R[2] * p.z;
+    Y += g.R[3] * p.x; Y += g.R[4] * p.y; Y += g.R[5] * p.z;
+    Z += g.R[6] * p.x; Z += g.R[7] * p.y; Z += g.R[8] * p.z;
     X = X + g.t[0];
     Y = Y + g.t[1];
     Z = Z + g.t[2];
@@ -134,6 +140,9 @@ __device__ inline float3 vertex(const PairGeom& g, int i, int j) {
     }
     return make_float3(g.sx * u + 0.f, g.sy * v + 0.f, vd);
 }
+
+// u', v', target depth of source pixel (i = row, j = column).
+__device__ inline float3 vertex(const PairGeom& g, int i, int j) { return project_tgt(g, unproject_src(g, i, j)); }
 
 __device__ inline float3 vertex(const ArrayGeom& g, int i, int j) {
     const size_t idx = (size_t)i * g.sw + j;

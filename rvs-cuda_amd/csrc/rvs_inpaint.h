@@ -1,4 +1,4 @@
-// rvs_inpaint.h -- workspace layout of the exact inpainting kernels.
+// rvs_inpaint.h -- workspace layout and launcher of the exact inpainting kernels.
 #ifndef RVS_INPAINT_H
 #define RVS_INPAINT_H
 
@@ -6,21 +6,27 @@
 #include <stdint.h>
 #include <stddef.h>
 
+#include "rvs_kernels.h"
+
 namespace rvs_k {
 
+constexpr int kInpaintBlocks = 1024;   // persistent component solvers
+
+// Indexed by global pixel (a view's pixels start at its px_base).
 struct InpaintWS {
-    int* label;     // w*h: -1 known pixel, else union-find parent (root after flattening)
-    int* dist;      // w*h: distance of the map (inpainting.cpp:182-188)
-    int* src;       // w*h: linear index of the chosen known pixel
-    int* cid;       // w*h: component id of a root
-    int* comp;      // 4 per component: x0, x1, y0, y1
+    int* label;     // -1 known pixel, else union-find parent (root after flattening)
+    int* dist;      // distance of the map (inpainting.cpp:182-188)
+    int* src;       // global index of the chosen known pixel
+    int* cid;       // component id of a root
+    int* comp;      // 4 per component: x0, x1, y0, y1 (view coordinates)
     int* root;      // root pixel of a component
     int* counters;  // [0] components, [1] work counter
 };
 
-size_t inpaint_ws_bytes(int w, int h);
-InpaintWS inpaint_ws_carve(void* base, int w, int h);
-hipError_t launch_inpaint(float* color, const float* quality, int w, int h, const InpaintWS& ws,
+size_t inpaint_ws_bytes(size_t npx, int nviews);
+InpaintWS inpaint_ws_carve(void* base, size_t npx, int nviews);
+// Inpaints every view of the batch in place (views[v].color from views[v].quality).
+hipError_t launch_inpaint(const ViewDesc* views, int nviews, int max_w, int max_h, const InpaintWS& ws,
                           int32_t* map_out, hipStream_t s);
 
 }  // namespace rvs_k

@@ -46,14 +46,19 @@ __device__ inline void unite(int* label, int a, int b) {
     }
 }
 
-__global__ void k_inp_init(const float* __restrict__ quality, int w, int h, InpaintWS ws) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= w * h) return;
-    const bool hole = !(quality[p] > 0.f);   // View::get_inpaint_mask (View.cpp:198-203)
-    ws.label[p] = hole ? p : -1;
-    ws.dist[p] = hole ? (w + h) : 0;
-    ws.src[p] = p;
-    if (p == 0) { ws.counters[0] = 0; ws.counters[1] = 0; }
+// Grid of the per-pixel kernels: x = pixel blocks of the largest view, y = view.
+#define RVS_VIEW_PIXEL(vd, p, gp)                                           \
+    const ViewDesc& vd = views[blockIdx.y];                                 \
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;                    \
+    if (p >= vd.w * vd.h) return;                                           \
+    const int gp = (int)vd.px_base + p;
+
+__global__ void k_inp_init(const ViewDesc* __restrict__ views, InpaintWS ws) {
+    RVS_VIEW_PIXEL(vd, p, gp)
+    const bool hole = !(vd.quality[p] > 0.f);   // View::get_inpaint_mask (View.cpp:198-203)
+    ws.label[gp] = hole ? gp : -1;
+    ws.dist[gp] = hole ? (vd.w + vd.h) : 0;
+    ws.src[gp] = gp;
 }
 
 constexpr int kCclTile = 32;
@@ -80,13 +85,16 @@ __device__ inline void lunite(int* lab, int a, int b) {
 
 // Labels inside one 32x32 tile in LDS; each hole pixel gets the global index of
 // its tile-local root (the smallest local index of its local component).
-__global__ void __launch_bounds__(256) k_ccl_local(int w, int h, InpaintWS ws) {
+__global__ void __launch_bounds__(256) k_ccl_local(const ViewDesc* __restrict__ views, InpaintWS ws) {
     __shared__ int lab[kCclTile * kCclTile];
+    const ViewDesc& vd = views[blockIdx.z];
+    const int w = vd.w, h = vd.h, base = (int)vd.px_base;
     const int tx0 = blockIdx.x * kCclTile, ty0 = blockIdx.y * kCclTile;
+    if (tx0 >= w || ty0 >= h) return;
     const int tid = threadIdx.x;
     for (int k = tid; k < kCclTile * kCclTile; k += 256) {
         const int x = tx0 + (k % kCclTile), y = ty0 + (k / kCclTile);
-        lab[k] = (x < w && y < h && ws.label[y * w + x] >= 0) ? k : -1;
+        lab[k] = (x < w && y < h && ws.label[base + y * w + x] >= 0) ? k : -1;
     }
     __syncthreads();
     for (int k = tid; k < kCclTile * kCclTile; k += 256) {
@@ -100,44 +108,41 @@ __global__ void __launch_bounds__(256) k_ccl_local(int w, int h, InpaintWS ws) {
         if (lab[k] < 0) continue;
         const int r = lfind(lab, k);
         const int x = tx0 + (k % kCclTile), y = ty0 + (k / kCclTile);
-        ws.label[y * w + x] = (ty0 + r / kCclTile) * w + (tx0 + r % kCclTile);
+        ws.label[base + y * w + x] = base + (ty0 + r / kCclTile) * w + (tx0 + r % kCclTile);
     }
 }
 
 // Unions across tile borders on the tile-local roots.
-__global__ void k_ccl_merge(int w, int h, InpaintWS ws) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= w * h) return;
-    if (ws.label[p] < 0) return;
-    const int x = p % w, y = p / w;
-    if ((x % kCclTile) == kCclTile - 1 && x + 1 < w && ws.label[p + 1] >= 0) unite(ws.label, p, p + 1);
-    if ((y % kCclTile) == kCclTile - 1 && y + 1 < h && ws.label[p + w] >= 0) unite(ws.label, p, p + w);
+__global__ void k_ccl_merge(const ViewDesc* __restrict__ views, InpaintWS ws) {
+    RVS_VIEW_PIXEL(vd, p, gp)
+    if (ws.label[gp] < 0) return;
+    const int x = p % vd.w, y = p / vd.w;
+    if ((x % kCclTile) == kCclTile - 1 && x + 1 < vd.w && ws.label[gp + 1] >= 0) unite(ws.label, gp, gp + 1);
+    if ((y % kCclTile) == kCclTile - 1 && y + 1 < vd.h && ws.label[gp + vd.w] >= 0) unite(ws.label, gp, gp + vd.w);
 }
 
-__global__ void k_ccl_flatten(int w, int h, InpaintWS ws) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= w * h) return;
-    if (ws.label[p] < 0) return;
-    const int r = find_root(ws.label, p);
-    ws.label[p] = r;
-    if (r == p) {
+__global__ void k_ccl_flatten(const ViewDesc* __restrict__ views, InpaintWS ws) {
+    RVS_VIEW_PIXEL(vd, p, gp)
+    if (ws.label[gp] < 0) return;
+    const int r = find_root(ws.label, gp);
+    ws.label[gp] = r;
+    if (r == gp) {
         const int id = atomicAdd(&ws.counters[0], 1);
-        ws.cid[p] = id;
+        ws.cid[gp] = id;
         ws.comp[4 * id + 0] = INT_MAX;
         ws.comp[4 * id + 1] = INT_MIN;
         ws.comp[4 * id + 2] = INT_MAX;
         ws.comp[4 * id + 3] = INT_MIN;
-        ws.root[id] = p;
+        ws.root[id] = gp;
     }
 }
 
-__global__ void k_comp_bbox(int w, int h, InpaintWS ws) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= w * h) return;
-    const int r = ws.label[p];
+__global__ void k_comp_bbox(const ViewDesc* __restrict__ views, InpaintWS ws) {
+    RVS_VIEW_PIXEL(vd, p, gp)
+    const int r = ws.label[gp];
     if (r < 0) return;
     const int id = ws.cid[r];
-    const int x = p % w, y = p / w;
+    const int x = p % vd.w, y = p / vd.w;
     atomicMin(&ws.comp[4 * id + 0], x);
     atomicMax(&ws.comp[4 * id + 1], x);
     atomicMin(&ws.comp[4 * id + 2], y);
@@ -171,13 +176,14 @@ constexpr int kOther = -1;      // dist of a hole of another component (inside t
 
 struct Region {
     int x0, y0, bw, bh, w, h, root;
+    int base;     // global index of the view's pixel (0, 0)
     int* ldist;   // LDS mode
     int* lsrc;
 };
 
 template <bool LDS>
 __device__ inline void region_get(const Region& R, const InpaintWS& ws, int x, int y, int& v, int& s) {
-    const int p = y * R.w + x;
+    const int p = R.base + y * R.w + x;
     if (LDS) {
         const int bx = x - R.x0, by = y - R.y0;
         if (bx >= 0 && bx < R.bw && by >= 0 && by < R.bh) {
@@ -200,7 +206,7 @@ __device__ inline void region_put(const Region& R, const InpaintWS& ws, int x, i
         const int k = (y - R.y0) * R.bw + (x - R.x0);
         R.ldist[k] = v; R.lsrc[k] = s;
     } else {
-        const int p = y * R.w + x;
+        const int p = R.base + y * R.w + x;
         ws.dist[p] = v; ws.src[p] = s;
     }
 }
@@ -303,32 +309,45 @@ __device__ void solve_component(const Region& R, const InpaintWS& ws, Fn* wave_t
     }
 }
 
-__global__ void __launch_bounds__(256) k_inp_solve(int w, int h, InpaintWS ws) {
+// View owning global pixel gp (views are sorted by px_base).
+__device__ inline int view_of_pixel(const ViewDesc* views, int nviews, int gp) {
+    int lo = 0, hi = nviews - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((int)views[mid].px_base <= gp) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) k_inp_solve(const ViewDesc* __restrict__ views, int nviews, InpaintWS ws) {
     __shared__ Fn wave_tot[4];
     __shared__ int s_carry[2];
     __shared__ int s_comp;
     __shared__ int ldist[kLdsCap];
     __shared__ int lsrc[kLdsCap];
     const int tid = threadIdx.x;
+    const int ncomp = ws.counters[0];
     while (true) {
         if (tid == 0) s_comp = atomicAdd(&ws.counters[1], 1);
         __syncthreads();
         const int c = s_comp;
         __syncthreads();
-        if (c >= ws.counters[0]) return;
+        if (c >= ncomp) return;
         Region R;
         R.root = ws.root[c];
+        const ViewDesc& vd = views[view_of_pixel(views, nviews, R.root)];
+        R.base = (int)vd.px_base;
         R.x0 = ws.comp[4 * c + 0];
         R.y0 = ws.comp[4 * c + 2];
         R.bw = ws.comp[4 * c + 1] - R.x0 + 1;
         R.bh = ws.comp[4 * c + 3] - R.y0 + 1;
-        R.w = w; R.h = h;
+        R.w = vd.w; R.h = vd.h;
         R.ldist = ldist; R.lsrc = lsrc;
         const int area = R.bw * R.bh;
         if (area <= kLdsCap) {
             for (int k = tid; k < area; k += 256) {
                 const int x = R.x0 + k % R.bw, y = R.y0 + k / R.bw;
-                const int p = y * w + x;
+                const int p = R.base + y * R.w + x;
                 const int lab = ws.label[p];
                 ldist[k] = lab < 0 ? 0 : (lab == R.root ? ws.dist[p] : kOther);
                 lsrc[k] = lab == R.root || lab < 0 ? ws.src[p] : -1;
@@ -338,7 +357,7 @@ __global__ void __launch_bounds__(256) k_inp_solve(int w, int h, InpaintWS ws) {
             __syncthreads();
             for (int k = tid; k < area; k += 256) {
                 const int x = R.x0 + k % R.bw, y = R.y0 + k / R.bw;
-                const int p = y * w + x;
+                const int p = R.base + y * R.w + x;
                 if (ws.label[p] == R.root) { ws.dist[p] = ldist[k]; ws.src[p] = lsrc[k]; }
             }
         } else {
@@ -348,55 +367,57 @@ __global__ void __launch_bounds__(256) k_inp_solve(int w, int h, InpaintWS ws) {
     }
 }
 
-__global__ void k_inp_apply(float* __restrict__ color, int w, int h, InpaintWS ws, int32_t* map_out) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= w * h) return;
-    const int s = ws.src[p];
+// Copies the chosen known pixel's colour into every hole; map_out (single-view
+// launches only) receives source x, source y and distance per pixel.
+__global__ void k_inp_apply(const ViewDesc* __restrict__ views, InpaintWS ws, int32_t* map_out) {
+    RVS_VIEW_PIXEL(vd, p, gp)
+    const int s = ws.src[gp] - (int)vd.px_base;
     if (map_out) {
-        map_out[3 * p] = s % w;
-        map_out[3 * p + 1] = s / w;
-        map_out[3 * p + 2] = ws.dist[p];
+        map_out[3 * p] = s % vd.w;
+        map_out[3 * p + 1] = s / vd.w;
+        map_out[3 * p + 2] = ws.dist[gp];
     }
-    if (ws.label[p] >= 0 && s != p) {
+    if (ws.label[gp] >= 0 && s != p) {
+        float* color = vd.color;
         color[3 * p] = color[3 * s];
         color[3 * p + 1] = color[3 * s + 1];
         color[3 * p + 2] = color[3 * s + 2];
     }
 }
 
-size_t inpaint_ws_bytes(int w, int h) {
-    const size_t n = (size_t)w * h;
-    const size_t maxc = n / 2 + 1;
-    return 4 * (4 * n) + 4 * (4 * maxc) + 4 * maxc + 64;
+size_t inpaint_ws_bytes(size_t npx, int nviews) {
+    const size_t maxc = npx / 2 + (size_t)nviews + 1;
+    return 4 * (4 * npx) + 4 * (4 * maxc) + 4 * maxc + 64;
 }
 
-InpaintWS inpaint_ws_carve(void* base, int w, int h) {
-    const size_t n = (size_t)w * h;
-    const size_t maxc = n / 2 + 1;
+InpaintWS inpaint_ws_carve(void* base, size_t npx, int nviews) {
+    const size_t maxc = npx / 2 + (size_t)nviews + 1;
     int* p = (int*)base;
     InpaintWS ws;
-    ws.label = p; p += n;
-    ws.dist = p; p += n;
-    ws.src = p; p += n;
-    ws.cid = p; p += n;
+    ws.label = p; p += npx;
+    ws.dist = p; p += npx;
+    ws.src = p; p += npx;
+    ws.cid = p; p += npx;
     ws.comp = p; p += 4 * maxc;
     ws.root = p; p += maxc;
     ws.counters = p;
     return ws;
 }
 
-hipError_t launch_inpaint(float* color, const float* quality, int w, int h, const InpaintWS& ws, int32_t* map_out,
-                          hipStream_t s) {
-    const int n = w * h;
-    const dim3 b(256), g((n + 255) / 256);
-    hipLaunchKernelGGL(k_inp_init, g, b, 0, s, quality, w, h, ws);
-    hipLaunchKernelGGL(k_ccl_local, dim3((w + kCclTile - 1) / kCclTile, (h + kCclTile - 1) / kCclTile), b, 0, s, w, h,
-                       ws);
-    hipLaunchKernelGGL(k_ccl_merge, g, b, 0, s, w, h, ws);
-    hipLaunchKernelGGL(k_ccl_flatten, g, b, 0, s, w, h, ws);
-    hipLaunchKernelGGL(k_comp_bbox, g, b, 0, s, w, h, ws);
-    hipLaunchKernelGGL(k_inp_solve, dim3(1024), b, 0, s, w, h, ws);
-    hipLaunchKernelGGL(k_inp_apply, g, b, 0, s, color, w, h, ws, map_out);
+hipError_t launch_inpaint(const ViewDesc* views, int nviews, int max_w, int max_h, const InpaintWS& ws,
+                          int32_t* map_out, hipStream_t s) {
+    const dim3 b(256), g((unsigned)(((size_t)max_w * max_h + 255) / 256), (unsigned)nviews);
+    hipError_t e = launch_reset_u32((uint32_t*)ws.counters, 2, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_inp_init, g, b, 0, s, views, ws);
+    hipLaunchKernelGGL(k_ccl_local,
+                       dim3((max_w + kCclTile - 1) / kCclTile, (max_h + kCclTile - 1) / kCclTile, nviews), b, 0, s,
+                       views, ws);
+    hipLaunchKernelGGL(k_ccl_merge, g, b, 0, s, views, ws);
+    hipLaunchKernelGGL(k_ccl_flatten, g, b, 0, s, views, ws);
+    hipLaunchKernelGGL(k_comp_bbox, g, b, 0, s, views, ws);
+    hipLaunchKernelGGL(k_inp_solve, dim3(kInpaintBlocks), b, 0, s, views, nviews, ws);
+    hipLaunchKernelGGL(k_inp_apply, g, b, 0, s, views, ws, map_out);
     return hipGetLastError();
 }
 

@@ -1,4 +1,12 @@
 // rvs_kernels.h -- launch-side declarations shared by the .hip translation units.
+//
+// A launch set synthesizes a batch of views (a single view is a batch of one)
+// from one set of resident sources.  Target tiles of all views are numbered
+// globally (view v owns tiles [tile_base, tile_base + tiles_x*tiles_y)), and so
+// are output pixels (view v owns [px_base, px_base + w*h)) for the inpainting
+// workspace.  Per (view, source) pair the vertex stage reads one descriptor G
+// (PairGeom for resident raw sources, ArrayGeom for precomputed inputs) at
+// geoms[view * nsrc + source].
 #ifndef RVS_KERNELS_H
 #define RVS_KERNELS_H
 
@@ -10,44 +18,66 @@
 namespace rvs_k {
 
 constexpr int kTile = 16;            // target tile edge (pixels); one 256-thread block per tile
-constexpr int kBinQX = 64;           // binning block: 64 x 4 source quads
+constexpr int kBinQX = 64;           // binning work item: 64 x 4 source quads
 constexpr int kBinQY = 4;
 constexpr uint32_t kDefaultTileCap = 1024;
 constexpr int kFallbackChunk = 128;  // list entries per LDS chunk in the in-order replay
 constexpr int kFallbackSortCap = 8192;  // longest (source, tile) list the replay sorts in LDS
+constexpr int kMaxViews = 4096;
+constexpr int kBinBlocks = 2048;    // persistent binning blocks
+constexpr int kFallbackBlocks = 1024;
 
-// Device-side state of one raster launch set.
+// One target view of a batch.
+struct ViewDesc {
+    int w, h, tiles_x, tiles_y;
+    uint32_t tile_base;      // first global tile
+    uint32_t px_base;        // first global pixel
+    int bitdepth;
+    float blend_exp;
+    float* color;            // w*h*3 (blend) / raster colour (raw)
+    float* quality;          // w*h (blend); may be null for raw
+    float* depth;            // w*h
+    float* validity;         // w*h (blend: validity, raw: triangle shape)
+    void *y, *u, *v;         // encoded planes, may be null
+};
+
+// Device-side state of one launch set.
 struct RasterWS {
-    uint32_t* counts;        // [nsrc][ntiles] bin fill counters
-    uint32_t* lists;         // [nsrc][ntiles][cap] quad indices
-    uint8_t* deferred;       // [out_h][out_w] pixel handled by the in-order replay
-    uint32_t* deferred_tiles;// [ntiles]
-    uint32_t* status;        // [0] n_deferred, [1] overflow flag, [2] max count, [3] replay overflow
+    uint32_t* counts;        // [global tile][source] bin fill counters
+    uint32_t* lists;         // [global tile][source][cap] quad indices
+    uint8_t* deferred;       // [global pixel] pixel handled by the in-order replay
+    uint32_t* deferred_tiles;// [global tile]
+    uint32_t* work;          // binning work items (view, source, block)
+    uint32_t* status;        // [0] n_deferred, [1] bin overflow, [2] -, [3] replay overflow, [4] work items,
+                             // [5] work cursor
 };
 
-struct RasterDims {
-    int out_w, out_h, tiles_x, tiles_y, ntiles;
-    uint32_t cap;
-};
-
-// Output maps of the raster (single source) or of the blend (all sources).
-struct RasterOut {
-    float* color;       // w*h*3
-    float* depth;       // w*h
-    float* quality;     // blend: quality; raw: unused
-    float* validity;    // blend: validity; raw: shape
+struct BatchDims {
+    int nviews, nsrc;
+    uint32_t ntiles;         // all views
+    uint32_t npx;            // all views
+    uint32_t cap;            // per (tile, source) bin capacity
+    int max_sw, max_sh;      // largest source grid
+    uint32_t bins_x, bins_y; // binning work items per source (for the largest source)
 };
 
 enum { MODE_BLEND = 0, MODE_RAW = 1 };
 
 hipError_t launch_reset_u32(uint32_t* p, size_t n, hipStream_t s);
+// Raw bytes into device memory through kernel arguments (stream-ordered, capturable).
+hipError_t launch_upload(const void* host, size_t bytes, void* dst, hipStream_t s);
+
+// Per binning work item of a perspective-target batch: bounding box of the
+// unprojected source points (source frame) for frustum culling.
+hipError_t launch_src_bounds(const rvs_dev::PairGeom* geoms, int nsrc, const BatchDims& d, float* bounds,
+                             hipStream_t s);
 
 template <class G>
-hipError_t launch_bin(const G* geoms, int nsrc, int max_sw, int max_sh, const RasterDims& d,
+hipError_t launch_bin(const G* geoms, const ViewDesc* views, const float* bounds, const BatchDims& d,
                       const RasterWS& ws, hipStream_t s);
 template <class G, int MODE>
-hipError_t launch_raster(const G* geoms, int nsrc, const RasterDims& d, const RasterWS& ws,
-                         float blend_exp, const RasterOut& out, hipStream_t s);
+hipError_t launch_raster(const G* geoms, const ViewDesc* views, const BatchDims& d, const RasterWS& ws,
+                         hipStream_t s);
 
 }  // namespace rvs_k
 

@@ -5,16 +5,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "rvs_device.h"
+#include "rvs_kernels.h"
 
 namespace rvs_k {
-
-constexpr int kGeomBatch = 8;
-
-struct GeomBatch {
-    rvs_dev::PairGeom g[kGeomBatch];
-    int n, offset;
-};
 
 hipError_t launch_decode_color(const void* y, const void* u, const void* v, int u8, int w, int h, float cs,
                                float* out, hipStream_t s);
@@ -26,7 +19,8 @@ hipError_t launch_blend_step(float* acc_c, float* acc_q, float* acc_d, float* ac
                              hipStream_t s);
 hipError_t launch_encode(const float* color, int w, int h, int bitdepth, void* y, void* u, void* v,
                          hipStream_t s);
-hipError_t launch_store_geoms(const rvs_dev::PairGeom* host, int n, rvs_dev::PairGeom* dst, hipStream_t s);
+// Encodes every view of a batch that has output planes (ViewDesc y, u, v).
+hipError_t launch_encode_views(const ViewDesc* views, int nviews, int max_w, int max_h, hipStream_t s);
 
 }  // namespace rvs_k
 

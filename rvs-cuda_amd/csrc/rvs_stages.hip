@@ -60,14 +60,11 @@ __global__ void k_blend_step(float* acc_c, float* acc_q, float* acc_d, float* ac
     acc_m[i] = (uint8_t)(s.mask ? (s.mask == 1 ? 1 : 255) : 0);
 }
 
-// write_color (image_writing.cpp:142-203): one thread per 2x2 block.
+// write_color (image_writing.cpp:142-203) for the 2x2 block (bx, by).
 template <typename T>
-__global__ void k_encode(const float* __restrict__ color, int w, int h, unsigned maxv, T* __restrict__ Y,
-                         T* __restrict__ U, T* __restrict__ V) {
-    const int bx = blockIdx.x * blockDim.x + threadIdx.x;
-    const int by = blockIdx.y;
-    const int hw = w / 2, hh = h / 2;
-    if (bx >= hw || by >= hh) return;
+__device__ inline void encode_block(const float* __restrict__ color, int w, int bx, int by, unsigned maxv,
+                                    T* __restrict__ Y, T* __restrict__ U, T* __restrict__ V) {
+    const int hw = w / 2;
     uint32_t su = 0, sv = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -80,6 +77,29 @@ __global__ void k_encode(const float* __restrict__ color, int w, int h, unsigned
     }
     U[(size_t)by * hw + bx] = (T)((su + 2) >> 2);
     V[(size_t)by * hw + bx] = (T)((sv + 2) >> 2);
+}
+
+// One thread per 2x2 block.
+template <typename T>
+__global__ void k_encode(const float* __restrict__ color, int w, int h, unsigned maxv, T* __restrict__ Y,
+                         T* __restrict__ U, T* __restrict__ V) {
+    const int bx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int by = blockIdx.y;
+    if (bx >= w / 2 || by >= h / 2) return;
+    encode_block(color, w, bx, by, maxv, Y, U, V);
+}
+
+// Batch of views (grid.z = view); views without output planes are skipped.
+__global__ void k_encode_views(const ViewDesc* __restrict__ views) {
+    const ViewDesc& vd = views[blockIdx.z];
+    const int bx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int by = blockIdx.y;
+    if (!vd.y || bx >= vd.w / 2 || by >= vd.h / 2) return;
+    const unsigned maxv = (1u << vd.bitdepth) - 1u;
+    if (vd.bitdepth <= 8)
+        encode_block(vd.color, vd.w, bx, by, maxv, (uint8_t*)vd.y, (uint8_t*)vd.u, (uint8_t*)vd.v);
+    else
+        encode_block(vd.color, vd.w, bx, by, maxv, (uint16_t*)vd.y, (uint16_t*)vd.u, (uint16_t*)vd.v);
 }
 
 hipError_t launch_decode_color(const void* y, const void* u, const void* v, int u8, int w, int h, float cs,
@@ -119,24 +139,10 @@ hipError_t launch_encode(const float* color, int w, int h, int bitdepth, void* y
     return hipGetLastError();
 }
 
-// Geometry descriptors are written by a kernel from its arguments, so a view's
-// whole launch sequence is stream-ordered and capturable (no host memcpy).
-__global__ void k_store_geoms(GeomBatch b, PairGeom* dst) {
-    const int i = threadIdx.x;
-    if (i < b.n) dst[b.offset + i] = b.g[i];
-}
-
-hipError_t launch_store_geoms(const PairGeom* host, int n, PairGeom* dst, hipStream_t s) {
-    for (int off = 0; off < n; off += kGeomBatch) {
-        GeomBatch b;
-        b.n = (n - off) < kGeomBatch ? (n - off) : kGeomBatch;
-        b.offset = off;
-        for (int i = 0; i < b.n; ++i) b.g[i] = host[off + i];
-        hipLaunchKernelGGL(k_store_geoms, dim3(1), dim3(64), 0, s, b, dst);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
+hipError_t launch_encode_views(const ViewDesc* views, int nviews, int max_w, int max_h, hipStream_t s) {
+    const dim3 grid((max_w / 2 + 255) / 256, max_h / 2, nviews);
+    hipLaunchKernelGGL(k_encode_views, grid, dim3(256), 0, s, views);
+    return hipGetLastError();
 }
 
 }  // namespace rvs_k

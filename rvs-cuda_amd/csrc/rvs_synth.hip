@@ -6,17 +6,24 @@
 // (D, V) when fl(fl(fl(r*r)*r)*V) < v with r = fl(d/D), strict, so ties keep the
 // earlier triangle.  Mathematically that is the argmax of v/d^3 with the lowest
 // traversal index on ties, but the float test is not transitive for
-// near-equal scores.  This file computes it in three steps:
-//   1. k_bin: every source quad is appended to the 16x16 target tiles its
-//      surviving triangles' (reference-exact) bounding boxes touch.
-//   2. k_raster: one 256-thread block per tile, up to four sources per phase.
-//      Every covered pixel takes two LDS 64-bit atomicMax: key1 = max of
-//      (score bits << 32 | ~seq), key2 = max over the candidates that are not
-//      the maximum (the loser of each key1 update).  A pixel whose runner-up
-//      scores within 2^-18 of the winner, or that saw a candidate with
-//      non-finite or extreme values, is deferred.  Every other pixel provably
-//      ends the sequential fold on the argmax (the float test errs by less than
-//      2^-21 relative), so its winner is re-evaluated and blended.
+// near-equal scores.  This file computes it for a batch of target views that
+// share one set of resident sources:
+//   0. k_bin_cull: binning work items (view, source, 64x4 source quads) whose
+//      source points all lie behind a perspective view or beyond one side of
+//      its image are dropped (bounding boxes of the unprojected points come
+//      from k_src_bounds and are tested with error margins); the rest form a
+//      work list.
+//   1. k_bin (persistent, pulls work items): every surviving source quad is
+//      appended to the 16x16 target tiles its triangles' (reference-exact)
+//      bounding boxes touch.
+//   2. k_raster: one 256-thread block per target tile, up to four sources per
+//      phase.  Every covered pixel takes two LDS 64-bit atomicMax: key1 = max
+//      of (score bits << 32 | ~seq), key2 = max over the candidates that are
+//      not the maximum (the loser of each key1 update).  A pixel whose
+//      runner-up scores within 2^-18 of the winner, or that saw a candidate
+//      with non-finite or extreme values, is deferred.  Every other pixel
+//      provably ends the sequential fold on the argmax (the float test errs by
+//      less than 2^-21 relative), so its winner is re-evaluated and blended.
 //   3. k_fallback: deferred pixels replay the reference fold literally, in
 //      traversal order, over the tile's bins sorted in LDS.
 // Blending across sources (BlendedViewSimple::blendBasicMode) runs per pixel in
@@ -41,6 +48,48 @@ hipError_t launch_reset_u32(uint32_t* p, size_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Descriptor uploads travel as kernel arguments, so a launch set stays
+// stream-ordered and capturable in a graph (no host-side memcpy).
+constexpr uint32_t kBlobBytes = 3584;
+struct Blob {
+    uint32_t n;
+    uint8_t bytes[kBlobBytes];
+};
+
+__global__ void k_upload(Blob b, uint8_t* dst) {
+    for (uint32_t i = threadIdx.x; i < b.n; i += blockDim.x) dst[i] = b.bytes[i];
+}
+
+hipError_t launch_upload(const void* host, size_t bytes, void* dst, hipStream_t s) {
+    const uint8_t* src = (const uint8_t*)host;
+    for (size_t off = 0; off < bytes; off += kBlobBytes) {
+        Blob b;
+        b.n = (uint32_t)((bytes - off) < (size_t)kBlobBytes ? (bytes - off) : (size_t)kBlobBytes);
+        for (uint32_t i = 0; i < b.n; ++i) b.bytes[i] = src[off + i];
+        hipLaunchKernelGGL(k_upload, dim3(1), dim3(256), 0, s, b, (uint8_t*)dst + off);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// View owning global tile T (views are sorted by tile_base).
+__device__ inline int find_view(const ViewDesc* views, int nviews, uint32_t T) {
+    int lo = 0, hi = nviews - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (views[mid].tile_base <= T) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Tile index of block b: each XCD (blocks are dealt round-robin over 8 XCDs)
+// gets a contiguous band of tiles, so neighbouring tiles share that XCD's L2.
+__device__ inline uint32_t tile_of_block(uint32_t b, uint32_t ntiles) {
+    const uint32_t per = (ntiles + 7) / 8;
+    return (b % 8) * per + b / 8;
+}
+
 // ------------------------------------------------------------------ quads
 
 // The (up to four) triangles of source quad (i, j), with the reference's
@@ -49,7 +98,6 @@ hipError_t launch_reset_u32(uint32_t* p, size_t n, hipStream_t s) {
 struct Quad {
     Tri t[4];
     uint32_t alive;
-    uint32_t special;   // bit k: slot k has non-finite inputs or a seam prol edge case
     uint32_t prol;      // bit k: prol flag of slot k (transform.cpp:121-123)
 };
 
@@ -61,15 +109,12 @@ __device__ inline void quad_add(Quad& Q, int slot, float3 a, float3 b, float3 c,
     Q.t[slot] = T;
     if (!ok) return;
     Q.alive |= 1u << slot;
-    const bool prol = (a.z > 0.f) || (b.z > 0.f) || (c.z > 0.f);
-    if (prol) Q.prol |= 1u << slot;
-    if (!finite3(a) || !finite3(b) || !finite3(c) || !is_finite(T.den) || !prol)
-        Q.special |= 1u << slot;
+    if ((a.z > 0.f) || (b.z > 0.f) || (c.z > 0.f)) Q.prol |= 1u << slot;
 }
 
 template <class G, class VF>
 __device__ inline void quad_build(Quad& Q, const G& g, int i, int j, VF vtx, int out_w, int out_h) {
-    Q.alive = 0; Q.special = 0; Q.prol = 0;
+    Q.alive = 0; Q.prol = 0;
     const float3 v00 = vtx(i, j), v01 = vtx(i, j + 1), v10 = vtx(i + 1, j), v11 = vtx(i + 1, j + 1);
     if (!(v01.z > 0.f && v10.z > 0.f && v01.x > 0.f && v10.x > 0.f)) return;
     if (v00.z > 0.f && v00.x > 0.f) quad_add(Q, 0, v00, v01, v10, out_w, out_h, g.rescale);
@@ -81,90 +126,232 @@ __device__ inline void quad_build(Quad& Q, const G& g, int i, int j, VF vtx, int
     }
 }
 
-// ------------------------------------------------------------------ binning
-
-template <class G>
-__global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, RasterDims d, RasterWS ws) {
-    const G& g = geoms[blockIdx.z];
-    const int i0 = blockIdx.y * kBinQY, j0 = blockIdx.x * kBinQX;
-    if (i0 >= g.sh - 1 || j0 >= g.sw - 1) return;
-    __shared__ float3 V[kBinQY + 1][kBinQX + 1];
-    const int tid = threadIdx.x;
-    for (int k = tid; k < (kBinQY + 1) * (kBinQX + 1); k += 256) {
-        const int r = k / (kBinQX + 1), c = k % (kBinQX + 1);
-        const int i = i0 + r, j = j0 + c;
-        if (i < g.sh && j < g.sw) V[r][c] = vertex(g, i, j);
-    }
-    __syncthreads();
-    const int qi = tid / kBinQX, qj = tid % kBinQX;
-    const int i = i0 + qi, j = j0 + qj;
-    bool more = false;
-    int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
-    uint32_t q = 0;
-    if (i < g.sh - 1 && j < g.sw - 1) {
-        Quad Q;
-        auto vtx = [&](int ii, int jj) -> float3 {
-            const int r = ii - i0, c = jj - j0;
-            if (r >= 0 && r <= kBinQY && c >= 0 && c <= kBinQX) return V[r][c];
-            return vertex(g, ii, jj);
-        };
-        quad_build(Q, g, i, j, vtx, d.out_w, d.out_h);
-        if (Q.alive) {
-            int xmin = INT_MAX, xmax = INT_MIN, ymin = INT_MAX, ymax = INT_MIN;
-            for (int s = 0; s < 4; ++s)
-                if (Q.alive & (1u << s)) {
-                    xmin = min(xmin, Q.t[s].xmin); xmax = max(xmax, Q.t[s].xmax);
-                    ymin = min(ymin, Q.t[s].ymin); ymax = max(ymax, Q.t[s].ymax);
-                }
-            tx0 = xmin / kTile; tx1 = xmax / kTile; ty0 = ymin / kTile; ty1 = ymax / kTile;
-            more = true;
-            q = (uint32_t)i * (uint32_t)(g.sw - 1) + (uint32_t)j;
-        }
-    }
-    const uint32_t src = blockIdx.z;
-    uint32_t* counts = ws.counts + (size_t)src * d.ntiles;
-    uint32_t* lists = ws.lists + (size_t)src * d.ntiles * d.cap;
-    const int lane = __lane_id();
-    const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-    int tx = tx0, ty = ty0;
-    // Wave-aggregated appends: one atomic per distinct tile per round.
-    while (__any(more)) {
-        const uint32_t t = more ? (uint32_t)(ty * d.tiles_x + tx) : 0xffffffffu;
-        const uint64_t act = __ballot(more);
-        const int leader = __ffsll((unsigned long long)act) - 1;
-        const uint32_t tl = __shfl(t, leader);
-        const bool mine = more && t == tl;
-        const uint64_t m = __ballot(mine);
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&counts[tl], (uint32_t)__popcll(m));
-        base = __shfl(base, leader);
-        if (mine) {
-            const uint32_t pos = base + (uint32_t)__popcll(m & lt_mask);
-            if (pos < d.cap)
-                lists[(size_t)tl * d.cap + pos] = q;
-            else
-                atomicMax(&ws.status[1], 1u);
-            if (++tx > tx1) { tx = tx0; if (++ty > ty1) more = false; }
-        }
-    }
-}
-
-template <class G>
-hipError_t launch_bin(const G* geoms, int nsrc, int max_sw, int max_sh, const RasterDims& d,
-                      const RasterWS& ws, hipStream_t s) {
-    dim3 grid((max_sw - 1 + kBinQX - 1) / kBinQX, (max_sh - 1 + kBinQY - 1) / kBinQY, nsrc);
-    hipLaunchKernelGGL(k_bin<G>, grid, dim3(256), 0, s, geoms, d, ws);
-    return hipGetLastError();
-}
-
-// ------------------------------------------------------------ tile raster
-
 template <class G>
 __device__ inline void quad_of(const G& g, uint32_t q, int& i, int& j) {
     const uint32_t w1 = (uint32_t)(g.sw - 1);
     i = (int)(q / w1);
     j = (int)(q - (uint32_t)i * w1);
 }
+
+// ------------------------------------------------------------ culling
+
+// Bounding box (source camera frame) of the finite unprojected points of one
+// binning work item's vertex window (kBinQY+1 rows x kBinQX+1 columns), as
+// min x, y, z, max x, y, z; no finite point -> min +inf, max -inf.  Only
+// source-side fields are read, so the first view's descriptors serve the batch.
+__global__ void __launch_bounds__(256) k_src_bounds(const PairGeom* __restrict__ geoms, BatchDims d, float* bounds) {
+    const PairGeom& g = geoms[blockIdx.z];
+    const int i0 = blockIdx.y * kBinQY, j0 = blockIdx.x * kBinQX;
+    float mn[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+    float mx[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+    if (i0 < g.sh - 1 && j0 < g.sw - 1) {
+        for (int k = threadIdx.x; k < (kBinQY + 1) * (kBinQX + 1); k += 256) {
+            const int i = i0 + k / (kBinQX + 1), j = j0 + k % (kBinQX + 1);
+            if (i >= g.sh || j >= g.sw) continue;
+            const float3 p = unproject_src(g, i, j);
+            if (!finite3(p)) continue;
+            mn[0] = fminf(mn[0], p.x); mx[0] = fmaxf(mx[0], p.x);
+            mn[1] = fminf(mn[1], p.y); mx[1] = fmaxf(mx[1], p.y);
+            mn[2] = fminf(mn[2], p.z); mx[2] = fmaxf(mx[2], p.z);
+        }
+    }
+    __shared__ float red[6][256];
+    for (int c = 0; c < 3; ++c) { red[c][threadIdx.x] = mn[c]; red[3 + c][threadIdx.x] = mx[c]; }
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int c = 0; c < 3; ++c) {
+                red[c][threadIdx.x] = fminf(red[c][threadIdx.x], red[c][threadIdx.x + st]);
+                red[3 + c][threadIdx.x] = fmaxf(red[3 + c][threadIdx.x], red[3 + c][threadIdx.x + st]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) {
+        const size_t item = ((size_t)blockIdx.z * d.bins_y + blockIdx.y) * d.bins_x + blockIdx.x;
+        bounds[item * 6 + threadIdx.x] = red[threadIdx.x][0];
+    }
+}
+
+hipError_t launch_src_bounds(const PairGeom* geoms, int nsrc, const BatchDims& d, float* bounds, hipStream_t s) {
+    hipLaunchKernelGGL(k_src_bounds, dim3(d.bins_x, d.bins_y, nsrc), dim3(256), 0, s, geoms, d, bounds);
+    return hipGetLastError();
+}
+
+// True when no triangle built from points inside box b can be alive for a
+// perspective target: every point is behind the camera (X < 0), or beyond one
+// image side by more than two pixels.  For X > 0 the side tests are linear in
+// the point, u < -m <=> f0*Y - (p0 + m)*X > 0 (u = -f0*Y/X + p0), so their
+// minimum over the box is at a corner.  A triangle is alive only if all its
+// vertices have X > 0 (vertex gating), and a vertex with u' < -1 everywhere
+// (or > W) gives xmin > xmax in tri_setup.  The margins cover the float error
+// of the corner evaluation here and of the vertex stage (both below 1e-6 of
+// the magnitudes involved); overflow or NaN never culls.
+__device__ inline bool box_outside(const PairGeom& g, const float* b, int out_w, int out_h) {
+    if (!(b[0] <= b[3])) return true;   // no finite point: every vertex is gated
+    float X[8], Y[8], Z[8];
+    float mag = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float x = (k & 1) ? b[3] : b[0], y = (k & 2) ? b[4] : b[1], z = (k & 4) ? b[5] : b[2];
+        X[k] = g.R[0] * x + g.R[1] * y + g.R[2] * z + g.t[0];
+        Y[k] = g.R[3] * x + g.R[4] * y + g.R[5] * z + g.t[1];
+        Z[k] = g.R[6] * x + g.R[7] * y + g.R[8] * z + g.t[2];
+        mag = fmaxf(mag, fabsf(x) + fabsf(y) + fabsf(z));
+    }
+    mag += fabsf(g.t[0]) + fabsf(g.t[1]) + fabsf(g.t[2]);
+    const float e = 1e-5f * mag;
+    const float f0 = -g.df0n, f1 = -g.df1n, p0 = g.dp0, p1 = g.dp1;
+    const float mu = 2.f / g.sx, mv = 2.f / g.sy;                       // two output pixels
+    const float ur = ((float)out_w + 2.f) / g.sx, vr = ((float)out_h + 2.f) / g.sy;
+    const float tl = e * (fabsf(f0) + fabsf(p0) + mu), tr = e * (fabsf(f0) + fabsf(p0) + ur);
+    const float tt = e * (fabsf(f1) + fabsf(p1) + mv), tb = e * (fabsf(f1) + fabsf(p1) + vr);
+    bool behind = true, left = true, right = true, top = true, bottom = true;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        behind = behind && (X[k] < -e);
+        left = left && (f0 * Y[k] - (p0 + mu) * X[k] > tl);
+        right = right && (-f0 * Y[k] - (ur - p0) * X[k] > tr);
+        top = top && (f1 * Z[k] - (p1 + mv) * X[k] > tt);
+        bottom = bottom && (-f1 * Z[k] - (vr - p1) * X[k] > tb);
+    }
+    return behind || left || right || top || bottom;
+}
+
+template <class G>
+__device__ inline bool item_culled(const G&, const float*, int, int) { return false; }
+template <>
+__device__ inline bool item_culled<PairGeom>(const PairGeom& g, const float* b, int out_w, int out_h) {
+    if (!b || g.dst_erp || g.wrap) return false;
+    return box_outside(g, b, out_w, out_h);
+}
+
+// One thread per (view, source, item): append the items that may cover the view
+// to the work list (code = (view * nsrc + source) * items + item).
+template <class G>
+__global__ void k_bin_cull(const G* __restrict__ geoms, const ViewDesc* __restrict__ views, const float* bounds,
+                           BatchDims d, RasterWS ws) {
+    const uint32_t per = d.bins_x * d.bins_y;
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t total = (size_t)per * d.nsrc * d.nviews;
+    bool keep = false;
+    if (gid < total) {
+        const uint32_t item = (uint32_t)(gid % per);
+        const uint32_t vs = (uint32_t)(gid / per);
+        const int s = (int)(vs % (uint32_t)d.nsrc), v = (int)(vs / (uint32_t)d.nsrc);
+        const G& g = geoms[vs];
+        const int bx = (int)(item % d.bins_x), by = (int)(item / d.bins_x);
+        if (by * kBinQY < g.sh - 1 && bx * kBinQX < g.sw - 1) {
+            const float* b = bounds ? bounds + ((size_t)s * per + item) * 6 : nullptr;
+            keep = !item_culled(g, b, views[v].w, views[v].h);
+        }
+    }
+    const int lane = __lane_id();
+    const uint64_t m = __ballot(keep);
+    if (m == 0ull) return;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&ws.status[4], (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (keep) {
+        const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+        ws.work[base + (uint32_t)__popcll(m & lt)] = (uint32_t)gid;
+    }
+}
+
+// ------------------------------------------------------------------ binning
+
+template <class G>
+__global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
+                                             BatchDims d, RasterWS ws) {
+    __shared__ float3 V[kBinQY + 1][kBinQX + 1];
+    __shared__ uint32_t s_item;
+    const int tid = threadIdx.x;
+    const int lane = __lane_id();
+    const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    const uint32_t per = d.bins_x * d.bins_y;
+    const uint32_t nwork = ws.status[4];
+    while (true) {
+        if (tid == 0) s_item = atomicAdd(&ws.status[5], 1u);
+        __syncthreads();
+        const uint32_t wi = s_item;
+        __syncthreads();
+        if (wi >= nwork) return;
+        const uint32_t code = ws.work[wi];
+        const uint32_t item = code % per, vs = code / per;
+        const int s = (int)(vs % (uint32_t)d.nsrc), v = (int)(vs / (uint32_t)d.nsrc);
+        const G& g = geoms[vs];
+        const ViewDesc& vd = views[v];
+        const int i0 = (int)(item / d.bins_x) * kBinQY, j0 = (int)(item % d.bins_x) * kBinQX;
+        for (int k = tid; k < (kBinQY + 1) * (kBinQX + 1); k += 256) {
+            const int r = k / (kBinQX + 1), c = k % (kBinQX + 1);
+            const int i = i0 + r, j = j0 + c;
+            if (i < g.sh && j < g.sw) V[r][c] = vertex(g, i, j);
+        }
+        __syncthreads();
+        const int i = i0 + tid / kBinQX, j = j0 + tid % kBinQX;
+        bool more = false;
+        int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
+        uint32_t q = 0;
+        if (i < g.sh - 1 && j < g.sw - 1) {
+            Quad Q;
+            auto vtx = [&](int ii, int jj) -> float3 {
+                const int r = ii - i0, c = jj - j0;
+                if (r >= 0 && r <= kBinQY && c >= 0 && c <= kBinQX) return V[r][c];
+                return vertex(g, ii, jj);
+            };
+            quad_build(Q, g, i, j, vtx, vd.w, vd.h);
+            if (Q.alive) {
+                int xmin = INT_MAX, xmax = INT_MIN, ymin = INT_MAX, ymax = INT_MIN;
+                for (int k = 0; k < 4; ++k)
+                    if (Q.alive & (1u << k)) {
+                        xmin = min(xmin, Q.t[k].xmin); xmax = max(xmax, Q.t[k].xmax);
+                        ymin = min(ymin, Q.t[k].ymin); ymax = max(ymax, Q.t[k].ymax);
+                    }
+                tx0 = xmin / kTile; tx1 = xmax / kTile; ty0 = ymin / kTile; ty1 = ymax / kTile;
+                more = true;
+                q = (uint32_t)i * (uint32_t)(g.sw - 1) + (uint32_t)j;
+            }
+        }
+        int tx = tx0, ty = ty0;
+        // Wave-aggregated appends: one atomic per distinct tile per round.
+        while (__any(more)) {
+            const uint32_t t = more ? vd.tile_base + (uint32_t)(ty * vd.tiles_x + tx) : 0xffffffffu;
+            const uint64_t act = __ballot(more);
+            const int leader = __ffsll((unsigned long long)act) - 1;
+            const uint32_t tl = __shfl(t, leader);
+            const bool mine = more && t == tl;
+            const uint64_t m = __ballot(mine);
+            const size_t bin = (size_t)tl * d.nsrc + s;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&ws.counts[bin], (uint32_t)__popcll(m));
+            base = __shfl(base, leader);
+            if (mine) {
+                const uint32_t pos = base + (uint32_t)__popcll(m & lt_mask);
+                if (pos < d.cap)
+                    ws.lists[bin * d.cap + pos] = q;
+                else
+                    atomicMax(&ws.status[1], 1u);
+                if (++tx > tx1) { tx = tx0; if (++ty > ty1) more = false; }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <class G>
+hipError_t launch_bin(const G* geoms, const ViewDesc* views, const float* bounds, const BatchDims& d,
+                      const RasterWS& ws, hipStream_t s) {
+    const size_t total = (size_t)d.bins_x * d.bins_y * d.nsrc * d.nviews;
+    hipLaunchKernelGGL(k_bin_cull<G>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, geoms, views, bounds,
+                       d, ws);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const size_t blocks = total < (size_t)kBinBlocks ? total : (size_t)kBinBlocks;
+    hipLaunchKernelGGL(k_bin<G>, dim3((unsigned)blocks), dim3(256), 0, s, geoms, views, d, ws);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ tile raster
 
 // Winner's colour, depth and validity at pixel (x, y) (the state the fold
 // holds after accepting candidate `seq`).
@@ -188,13 +375,6 @@ __device__ inline void resolve(const G& g, uint32_t seq, int x, int y, float3& c
     dep = tri_depth(T, l1, l2, l3);
     shape = T.tv;
     c = tri_color(color(g, r[0], cc[0]), color(g, r[1], cc[1]), color(g, r[2], cc[2]), l1, l2, l3);
-}
-
-// Tile index of block b: each XCD (blocks are dealt round-robin over 8 XCDs)
-// gets a contiguous band of tiles, so neighbouring tiles share that XCD's L2.
-__device__ inline int tile_of_block(int b, int ntiles) {
-    const int per = (ntiles + 7) / 8;
-    return (b % 8) * per + b / 8;
 }
 
 // Pass over one triangle of a binned quad for the tile at (tx0, ty0): per
@@ -250,27 +430,45 @@ __device__ inline void raster_quad(const G& g, uint32_t q, int tx0, int ty0, int
     }
 }
 
+template <int MODE>
+__device__ inline void write_pixel(const ViewDesc& vd, size_t o, const BlendState& st, float3 rc, float rd, float rs) {
+    if (MODE == MODE_RAW) {
+        vd.color[3 * o] = rc.x; vd.color[3 * o + 1] = rc.y; vd.color[3 * o + 2] = rc.z;
+        vd.depth[o] = rd;
+        vd.validity[o] = rs;
+    } else {
+        vd.color[3 * o] = st.c.x; vd.color[3 * o + 1] = st.c.y; vd.color[3 * o + 2] = st.c.z;
+        if (vd.depth) vd.depth[o] = st.d;
+        vd.quality[o] = st.q;
+        if (vd.validity) vd.validity[o] = st.v;
+    }
+}
+
 constexpr int kGroup = 4;   // sources rasterised together per block phase
 
 template <class G, int MODE>
-__global__ void __launch_bounds__(256) k_raster(const G* __restrict__ geoms, int nsrc, RasterDims d, RasterWS ws,
-                                                float blend_exp, RasterOut out) {
+__global__ void __launch_bounds__(256) k_raster(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
+                                                BatchDims d, RasterWS ws) {
     __shared__ unsigned long long key1[kGroup][kTile * kTile];
     __shared__ unsigned long long key2[kGroup][kTile * kTile];
     __shared__ int flag[kGroup][kTile * kTile];
     __shared__ uint32_t pref[kGroup + 1];
-    const int tile = tile_of_block(blockIdx.x, d.ntiles);
-    if (tile >= d.ntiles) return;
+    const uint32_t T = tile_of_block(blockIdx.x, d.ntiles);
+    if (T >= d.ntiles) return;
+    const int v = find_view(views, d.nviews, T);
+    const ViewDesc& vd = views[v];
+    const int tile = (int)(T - vd.tile_base);
     const int tid = threadIdx.x;
-    const int tx0 = (tile % d.tiles_x) * kTile, ty0 = (tile / d.tiles_x) * kTile;
+    const int tx0 = (tile % vd.tiles_x) * kTile, ty0 = (tile / vd.tiles_x) * kTile;
     const int px = tx0 + (tid % kTile), py = ty0 + (tid / kTile);
-    const bool inside = px < d.out_w && py < d.out_h;
+    const bool inside = px < vd.w && py < vd.h;
+    const G* gv = geoms + (size_t)v * d.nsrc;
     bool deferred = false;
     BlendState st;
     float3 rc = make_float3(0.f, 0.f, 0.f);
     float rd = __builtin_inff(), rs = 0.f;
-    for (int s0 = 0; s0 < nsrc; s0 += kGroup) {
-        const int gs = min(kGroup, nsrc - s0);
+    for (int s0 = 0; s0 < d.nsrc; s0 += kGroup) {
+        const int gs = min(kGroup, d.nsrc - s0);
         for (int k = 0; k < gs; ++k) {
             key1[k][tid] = 0ull;
             key2[k][tid] = 0ull;
@@ -280,7 +478,7 @@ __global__ void __launch_bounds__(256) k_raster(const G* __restrict__ geoms, int
             uint32_t acc = 0;
             for (int k = 0; k < gs; ++k) {
                 pref[k] = acc;
-                acc += min(ws.counts[(size_t)(s0 + k) * d.ntiles + tile], d.cap);
+                acc += min(ws.counts[(size_t)T * d.nsrc + s0 + k], d.cap);
             }
             pref[gs] = acc;
         }
@@ -289,8 +487,8 @@ __global__ void __launch_bounds__(256) k_raster(const G* __restrict__ geoms, int
         for (uint32_t e = tid; e < N; e += 256) {
             int k = 0;
             while (k + 1 < gs && e >= pref[k + 1]) ++k;
-            const uint32_t q = ws.lists[((size_t)(s0 + k) * d.ntiles + tile) * d.cap + (e - pref[k])];
-            raster_quad(geoms[s0 + k], q, tx0, ty0, d.out_w, d.out_h, key1[k], key2[k], flag[k]);
+            const uint32_t q = ws.lists[((size_t)T * d.nsrc + s0 + k) * d.cap + (e - pref[k])];
+            raster_quad(gv[s0 + k], q, tx0, ty0, vd.w, vd.h, key1[k], key2[k], flag[k]);
         }
         __syncthreads();
         if (inside) {
@@ -309,37 +507,26 @@ __global__ void __launch_bounds__(256) k_raster(const G* __restrict__ geoms, int
                 if (amb) { deferred = true; break; }
                 float3 c = make_float3(0.f, 0.f, 0.f);
                 float dep = __builtin_inff(), shape = 0.f;
-                if (kw != 0ull) resolve(geoms[s], ~(uint32_t)kw, px, py, c, dep, shape, d.out_w, d.out_h);
+                if (kw != 0ull) resolve(gv[s], ~(uint32_t)kw, px, py, c, dep, shape, vd.w, vd.h);
                 if (MODE == MODE_RAW) {
                     rc = c; rd = dep; rs = shape;
                 } else if (s == 0) {
                     blend_first(st, c, dep, shape);
                 } else {
-                    blend_step(st, c, dep, shape, blend_exp);
+                    blend_step(st, c, dep, shape, vd.blend_exp);
                 }
             }
         }
         __syncthreads();
     }
     if (inside) {
-        const size_t o = (size_t)py * d.out_w + px;
-        ws.deferred[o] = deferred ? 1 : 0;
-        if (!deferred) {
-            if (MODE == MODE_RAW) {
-                out.color[3 * o] = rc.x; out.color[3 * o + 1] = rc.y; out.color[3 * o + 2] = rc.z;
-                out.depth[o] = rd;
-                out.validity[o] = rs;
-            } else {
-                out.color[3 * o] = st.c.x; out.color[3 * o + 1] = st.c.y; out.color[3 * o + 2] = st.c.z;
-                if (out.depth) out.depth[o] = st.d;
-                if (out.quality) out.quality[o] = st.q;
-                if (out.validity) out.validity[o] = st.v;
-            }
-        }
+        const size_t o = (size_t)py * vd.w + px;
+        ws.deferred[vd.px_base + o] = deferred ? 1 : 0;
+        if (!deferred) write_pixel<MODE>(vd, o, st, rc, rd, rs);
     }
     if (__syncthreads_or(deferred) && tid == 0) {
         const uint32_t k = atomicAdd(&ws.status[0], 1u);
-        ws.deferred_tiles[k] = (uint32_t)tile;
+        ws.deferred_tiles[k] = T;
     }
 }
 
@@ -351,26 +538,31 @@ struct ReplayEntry {
 };
 
 template <class G, int MODE>
-__global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, int nsrc, RasterDims d, RasterWS ws,
-                                                  float blend_exp, RasterOut out) {
+__global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
+                                                  BatchDims d, RasterWS ws) {
     __shared__ uint32_t sorted[kFallbackSortCap];
     __shared__ ReplayEntry rec[kFallbackChunk];
     const int tid = threadIdx.x;
     const uint32_t ndef = ws.status[0];
     for (uint32_t ti = blockIdx.x; ti < ndef; ti += gridDim.x) {
-        const int tile = (int)ws.deferred_tiles[ti];
-        const int tx0 = (tile % d.tiles_x) * kTile, ty0 = (tile / d.tiles_x) * kTile;
+        const uint32_t T = ws.deferred_tiles[ti];
+        const int v = find_view(views, d.nviews, T);
+        const ViewDesc& vd = views[v];
+        const G* gv = geoms + (size_t)v * d.nsrc;
+        const int tile = (int)(T - vd.tile_base);
+        const int tx0 = (tile % vd.tiles_x) * kTile, ty0 = (tile / vd.tiles_x) * kTile;
         const int px = tx0 + (tid % kTile), py = ty0 + (tid / kTile);
-        const bool inside = px < d.out_w && py < d.out_h;
-        const size_t o = inside ? (size_t)py * d.out_w + px : 0;
-        const bool mine = inside && ws.deferred[o];
+        const bool inside = px < vd.w && py < vd.h;
+        const size_t o = inside ? (size_t)py * vd.w + px : 0;
+        const bool mine = inside && ws.deferred[vd.px_base + o];
         BlendState st;
         float3 rc = make_float3(0.f, 0.f, 0.f);
         float rd = __builtin_inff(), rs = 0.f;
-        for (int s = 0; s < nsrc; ++s) {
-            const G& g = geoms[s];
-            const uint32_t n = min(ws.counts[(size_t)s * d.ntiles + tile], d.cap);
-            const uint32_t* L = ws.lists + ((size_t)s * d.ntiles + tile) * d.cap;
+        for (int s = 0; s < d.nsrc; ++s) {
+            const G& g = gv[s];
+            const size_t bin = (size_t)T * d.nsrc + s;
+            const uint32_t n = min(ws.counts[bin], d.cap);
+            const uint32_t* L = ws.lists + bin * d.cap;
             if (n > (uint32_t)kFallbackSortCap) {
                 if (tid == 0) atomicMax(&ws.status[3], n);
                 __syncthreads();
@@ -402,7 +594,7 @@ __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, i
                     quad_of(g, q, i, j);
                     Quad Q;
                     auto vtx = [&](int ii, int jj2) -> float3 { return vertex(g, ii, jj2); };
-                    quad_build(Q, g, i, j, vtx, d.out_w, d.out_h);
+                    quad_build(Q, g, i, j, vtx, vd.w, vd.h);
                     ReplayEntry& R = rec[tid];
                     for (int k = 0; k < 4; ++k) R.t[k] = Q.t[k];
                     R.alive = Q.alive; R.prol = Q.prol; R.q = q;
@@ -414,11 +606,12 @@ __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, i
                         const ReplayEntry& R = rec[e];
                         for (int k = 0; k < 4; ++k) {
                             if (!(R.alive & (1u << k))) continue;
-                            const Tri& T = R.t[k];
-                            if (px < T.xmin || px > T.xmax || py < T.ymin || py > T.ymax) continue;
+                            const Tri& Tr = R.t[k];
+                            if (px < Tr.xmin || px > Tr.xmax || py < Tr.ymin || py > Tr.ymax) continue;
                             float l1, l2, l3;
-                            if (!tri_pixel(T, px, py, l1, l2, l3)) continue;
-                            fold_step(fs, tri_depth(T, l1, l2, l3), T.tv, (R.prol >> k) & 1u, (R.q << 2) | (uint32_t)k);
+                            if (!tri_pixel(Tr, px, py, l1, l2, l3)) continue;
+                            fold_step(fs, tri_depth(Tr, l1, l2, l3), Tr.tv, (R.prol >> k) & 1u,
+                                      (R.q << 2) | (uint32_t)k);
                         }
                     }
                 }
@@ -429,7 +622,7 @@ __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, i
                 float dep = __builtin_inff(), shape = 0.f;
                 if (fs.seq != 0xffffffffu) {
                     float dd, sh;
-                    resolve(g, fs.seq, px, py, c, dd, sh, d.out_w, d.out_h);
+                    resolve(g, fs.seq, px, py, c, dd, sh, vd.w, vd.h);
                     dep = fs.new_d;
                     shape = fs.shape;
                 }
@@ -438,43 +631,34 @@ __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, i
                 } else if (s == 0) {
                     blend_first(st, c, dep, shape);
                 } else {
-                    blend_step(st, c, dep, shape, blend_exp);
+                    blend_step(st, c, dep, shape, vd.blend_exp);
                 }
             }
         }
-        if (mine) {
-            if (MODE == MODE_RAW) {
-                out.color[3 * o] = rc.x; out.color[3 * o + 1] = rc.y; out.color[3 * o + 2] = rc.z;
-                out.depth[o] = rd;
-                out.validity[o] = rs;
-            } else {
-                out.color[3 * o] = st.c.x; out.color[3 * o + 1] = st.c.y; out.color[3 * o + 2] = st.c.z;
-                if (out.depth) out.depth[o] = st.d;
-                if (out.quality) out.quality[o] = st.q;
-                if (out.validity) out.validity[o] = st.v;
-            }
-        }
+        if (mine) write_pixel<MODE>(vd, o, st, rc, rd, rs);
         __syncthreads();
     }
 }
 
 template <class G, int MODE>
-hipError_t launch_raster(const G* geoms, int nsrc, const RasterDims& d, const RasterWS& ws, float blend_exp,
-                         const RasterOut& out, hipStream_t s) {
-    const int blocks = 8 * ((d.ntiles + 7) / 8);
-    hipLaunchKernelGGL((k_raster<G, MODE>), dim3(blocks), dim3(256), 0, s, geoms, nsrc, d, ws, blend_exp, out);
+hipError_t launch_raster(const G* geoms, const ViewDesc* views, const BatchDims& d, const RasterWS& ws,
+                         hipStream_t s) {
+    const unsigned blocks = 8u * ((d.ntiles + 7u) / 8u);
+    hipLaunchKernelGGL((k_raster<G, MODE>), dim3(blocks), dim3(256), 0, s, geoms, views, d, ws);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int fb_blocks = d.ntiles < 512 ? d.ntiles : 512;
-    hipLaunchKernelGGL((k_fallback<G, MODE>), dim3(fb_blocks), dim3(256), 0, s, geoms, nsrc, d, ws, blend_exp, out);
+    const unsigned fb_blocks = d.ntiles < (uint32_t)kFallbackBlocks ? d.ntiles : (uint32_t)kFallbackBlocks;
+    hipLaunchKernelGGL((k_fallback<G, MODE>), dim3(fb_blocks), dim3(256), 0, s, geoms, views, d, ws);
     return hipGetLastError();
 }
 
-template hipError_t launch_bin<PairGeom>(const PairGeom*, int, int, int, const RasterDims&, const RasterWS&, hipStream_t);
-template hipError_t launch_bin<ArrayGeom>(const ArrayGeom*, int, int, int, const RasterDims&, const RasterWS&, hipStream_t);
-template hipError_t launch_raster<PairGeom, MODE_BLEND>(const PairGeom*, int, const RasterDims&, const RasterWS&, float,
-                                                        const RasterOut&, hipStream_t);
-template hipError_t launch_raster<ArrayGeom, MODE_RAW>(const ArrayGeom*, int, const RasterDims&, const RasterWS&, float,
-                                                       const RasterOut&, hipStream_t);
+template hipError_t launch_bin<PairGeom>(const PairGeom*, const ViewDesc*, const float*, const BatchDims&,
+                                         const RasterWS&, hipStream_t);
+template hipError_t launch_bin<ArrayGeom>(const ArrayGeom*, const ViewDesc*, const float*, const BatchDims&,
+                                          const RasterWS&, hipStream_t);
+template hipError_t launch_raster<PairGeom, MODE_BLEND>(const PairGeom*, const ViewDesc*, const BatchDims&,
+                                                        const RasterWS&, hipStream_t);
+template hipError_t launch_raster<ArrayGeom, MODE_RAW>(const ArrayGeom*, const ViewDesc*, const BatchDims&,
+                                                       const RasterWS&, hipStream_t);
 
 }  // namespace rvs_k

@@ -10,6 +10,7 @@
 #include "rvs/config.hpp"
 #include "rvs/json.hpp"
 #include "rvs/pipeline.hpp"
+#include "rvs/server.hpp"
 #include "rvs_host.h"
 
 namespace {
@@ -124,6 +125,71 @@ int rvs_app_run(const char* config_path, const char* source_path, int stagewise)
         g_err = e.what();
         return -1;
     }
+}
+
+// ------------------------------------------------------------------ serving
+
+void* rvs_server_create(const rvs_source* const* scenes, const int* n_src, int n_scenes, const rvs_serve_opts* opts) {
+    try {
+        if (!scenes || !n_src || n_scenes < 1 || !opts) throw std::invalid_argument("rvs_server_create: bad argument");
+        std::vector<std::vector<rvs_source>> sc(n_scenes);
+        for (int k = 0; k < n_scenes; ++k) {
+            if (!scenes[k] || n_src[k] < 1 || n_src[k] > RVS_MAX_SOURCES)
+                throw std::invalid_argument("rvs_server_create: bad scene " + std::to_string(k));
+            sc[k].assign(scenes[k], scenes[k] + n_src[k]);
+        }
+        return new rvs::Server(std::move(sc), *opts);
+    } catch (std::exception const& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void rvs_server_destroy(void* server) { delete static_cast<rvs::Server*>(server); }
+
+static int copy_stats(rvs::Server* s, int n, rvs_request_stats* stats) {
+    if (stats)
+        for (int k = 0; k < n; ++k) stats[k] = s->stats()[k];
+    return 0;
+}
+
+int rvs_server_run_open(void* server, const rvs_request* reqs, int n, double rate, uint64_t seed,
+                        rvs_request_stats* stats) {
+    try {
+        if (!server || (!reqs && n > 0)) throw std::invalid_argument("rvs_server_run_open: bad argument");
+        rvs::Server* s = static_cast<rvs::Server*>(server);
+        const int rc = s->run_open(reqs, n, rate, seed);
+        copy_stats(s, n, stats);
+        if (rc != 0) g_err = "a request failed: " + s->error();
+        return rc;
+    } catch (std::exception const& e) {
+        g_err = e.what();
+        return RVS_E_INVALID;
+    }
+}
+
+int rvs_server_run_closed(void* server, const rvs_request* reqs, int n, int concurrency, rvs_request_stats* stats) {
+    try {
+        if (!server || (!reqs && n > 0)) throw std::invalid_argument("rvs_server_run_closed: bad argument");
+        rvs::Server* s = static_cast<rvs::Server*>(server);
+        const int rc = s->run_closed(reqs, n, concurrency);
+        copy_stats(s, n, stats);
+        if (rc != 0) g_err = "a request failed: " + s->error();
+        return rc;
+    } catch (std::exception const& e) {
+        g_err = e.what();
+        return RVS_E_INVALID;
+    }
+}
+
+long long rvs_server_output(void* server, int k, void* out, size_t cap) {
+    if (!server) return -1;
+    rvs::Server* s = static_cast<rvs::Server*>(server);
+    if (k < 0 || k >= (int)s->stats().size()) return -1;
+    const std::vector<uint8_t>& o = s->output(k);
+    if (o.empty()) return -1;
+    if (out) std::memcpy(out, o.data(), o.size() < cap ? o.size() : cap);
+    return (long long)o.size();
 }
 
 }  // extern "C"

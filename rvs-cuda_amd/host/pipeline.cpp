@@ -272,7 +272,7 @@ void SynthetisedViewTriangle::compute(InputView& input, rvs_stream_t stream) {
     const int wrap = (dst.proj == 1 && dst.full_hor) ? 1 : 0;
     uint32_t cap = 0;
     for (int attempt = 0; attempt < 8; ++attempt) {
-        const std::size_t need = rvs_raster_workspace_size(dst.w, dst.h, cap);
+        const std::size_t need = rvs_raster_workspace_size(src.w, src.h, dst.w, dst.h, cap);
         m_ws.resize(need);
         uint32_t needed = 0;
         const int rc = rvs_raster(input.getDevColor(), m_vdepth.as<float>(), m_uv.as<float>(), src.w, src.h, dst.w,
@@ -444,7 +444,8 @@ void Pipeline::computeView(int inputFrame, int virtualFrame, int virtualView) {
         o.color = color.as<float>(); o.quality = quality.as<float>(); o.depth = depth.as<float>();
         o.validity = validity.as<float>(); o.y = y.get(); o.u = u.get(); o.v = v.get();
         for (int attempt = 0;; ++attempt) {
-            const std::size_t need = rvs_synth_workspace_size(n, &t, m_tile_cap);
+            const std::size_t need = rvs_synth_workspace_size(srcs.data(), n, &t, m_tile_cap);
+            if (need == 0) check(RVS_E_INVALID, "rvs_synth_workspace_size");
             m_ws.resize(need);
             uint32_t needed = 0;
             const int rc = rvs_synthesize(srcs.data(), n, &t, m_ws.get(), need, m_tile_cap, &o, &needed, m_stream);

@@ -1,0 +1,352 @@
+// server.cpp -- continuous-batching request loop (see server.hpp).
+#include "rvs/server.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <stdexcept>
+#include <string>
+
+namespace rvs {
+
+namespace {
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t plane_bytes(const rvs_target& t) { return t.bitdepth <= 8 ? 1 : 2; }
+
+// Device bytes of one request's outputs: colour map + encoded planes.
+size_t output_bytes(const rvs_target& t) {
+    const size_t px = (size_t)t.cam.w * t.cam.h;
+    return align256(px * 3 * sizeof(float)) + align256(px * plane_bytes(t)) + 2 * align256(px / 4 * plane_bytes(t));
+}
+
+size_t yuv_bytes(const rvs_target& t) { return (size_t)t.cam.w * t.cam.h * 3 / 2 * plane_bytes(t); }
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace
+
+struct Server::Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    char* arena = nullptr;
+    size_t arena_bytes = 0;
+    uint8_t* pinned = nullptr;
+    size_t pinned_bytes = 0;
+    bool busy = false;
+    int scene = 0;
+    uint32_t cap = 0;
+    std::vector<int> reqs;
+    std::vector<rvs_target> tgts;
+    std::vector<rvs_view_out> outs;
+    std::vector<size_t> host_off;
+    size_t ws_bytes = 0;
+    size_t ws_off = 0;
+};
+
+Server::Server(std::vector<std::vector<rvs_source>> scenes, const rvs_serve_opts& opts)
+    : m_scenes(std::move(scenes)), m_opt(opts) {
+    if (m_scenes.empty()) throw std::invalid_argument("server: no scene");
+    if (m_opt.slots < 1) m_opt.slots = 1;
+    if (m_opt.max_batch < 1) m_opt.max_batch = 1;
+    const size_t share = m_opt.memory_budget / (size_t)m_opt.slots;
+    for (int k = 0; k < m_opt.slots; ++k) {
+        Slot* s = new Slot;
+        m_slots.push_back(s);
+        hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "stream create");
+        hip_check(hipEventCreateWithFlags(&s->done, hipEventDisableTiming), "event create");
+        hip_check(hipMalloc((void**)&s->arena, share), "server arena");
+        s->arena_bytes = share;
+        if (m_opt.copy_to_host) {
+            s->pinned_bytes = m_opt.host_staging_bytes ? m_opt.host_staging_bytes : share / 4;
+            hip_check(hipHostMalloc((void**)&s->pinned, s->pinned_bytes, 0), "pinned staging");
+        }
+        s->cap = m_opt.tile_cap;
+    }
+    m_thread = std::thread([this] { loop(); });
+}
+
+Server::~Server() {
+    {
+        std::lock_guard<std::mutex> lk(m_mu);
+        m_stop = true;
+    }
+    m_cv.notify_all();
+    if (m_thread.joinable()) m_thread.join();
+    for (Slot* s : m_slots) {
+        if (s->stream) (void)hipStreamSynchronize(s->stream);
+        if (s->pinned) (void)hipHostFree(s->pinned);
+        if (s->arena) (void)hipFree(s->arena);
+        if (s->done) (void)hipEventDestroy(s->done);
+        if (s->stream) (void)hipStreamDestroy(s->stream);
+        delete s;
+    }
+}
+
+double Server::now_ms() const {
+    const int64_t t = std::chrono::steady_clock::now().time_since_epoch().count();
+    return (double)(t - m_t0) * 1e-6;
+}
+
+std::string Server::error() const {
+    std::lock_guard<std::mutex> lk(const_cast<std::mutex&>(m_mu));
+    return m_errmsg;
+}
+
+void Server::reset(const rvs_request* reqs, int n) {
+    m_reqs = reqs;
+    m_n = n;
+    m_done = 0;
+    m_batches = 0;
+    m_error = 0;
+    m_errmsg.clear();
+    m_pending.clear();
+    m_stats.assign(n, rvs_request_stats{});
+    m_out.assign(m_opt.keep_outputs ? n : 0, std::vector<uint8_t>());
+    for (int k = 0; k < n; ++k) {
+        m_stats[k].rc = 1;   // not done
+        if (reqs[k].scene < 0 || reqs[k].scene >= (int)m_scenes.size())
+            throw std::invalid_argument("server: request with an unknown scene");
+    }
+    m_t0 = std::chrono::steady_clock::now().time_since_epoch().count();
+}
+
+void Server::submit_locked(int k, double now) {
+    m_stats[k].arrival_ms = now;
+    m_pending.push_back(k);
+}
+
+int Server::run_open(const rvs_request* reqs, int n, double rate, uint64_t seed) {
+    if (n <= 0) return 0;
+    if (!(rate > 0.0)) throw std::invalid_argument("server: arrival rate must be > 0");
+    {
+        std::lock_guard<std::mutex> lk(m_mu);
+        reset(reqs, n);
+        m_closed = false;
+    }
+    std::mt19937_64 rng(seed);
+    std::exponential_distribution<double> gap(rate);
+    const auto t0 = std::chrono::steady_clock::time_point(std::chrono::steady_clock::duration(m_t0));
+    double t = 0.0;
+    for (int k = 0; k < n; ++k) {
+        t += gap(rng);
+        std::this_thread::sleep_until(t0 + std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                                               std::chrono::duration<double>(t)));
+        {
+            std::lock_guard<std::mutex> lk(m_mu);
+            submit_locked(k, now_ms());
+        }
+        m_cv.notify_all();
+    }
+    std::unique_lock<std::mutex> lk(m_mu);
+    m_cv.wait(lk, [this] { return m_done == m_n; });
+    return m_error;
+}
+
+int Server::run_closed(const rvs_request* reqs, int n, int concurrency) {
+    if (n <= 0) return 0;
+    {
+        std::lock_guard<std::mutex> lk(m_mu);
+        reset(reqs, n);
+        m_closed = true;
+        const int c = std::max(1, std::min(concurrency, n));
+        const double t = now_ms();
+        for (int k = 0; k < c; ++k) submit_locked(k, t);
+        m_next_closed = c;
+    }
+    m_cv.notify_all();
+    std::unique_lock<std::mutex> lk(m_mu);
+    m_cv.wait(lk, [this] { return m_done == m_n; });
+    return m_error;
+}
+
+// Forms and enqueues the next batch on a free slot (lock held by the caller
+// only around the pending queue).
+bool Server::launch(Slot& s) {
+    std::vector<int> batch;
+    std::vector<rvs_target> tgts;
+    size_t ws = 0, outb = 0, hostb = 0;
+    {
+        std::lock_guard<std::mutex> lk(m_mu);
+        if (m_pending.empty()) return false;
+        const int scene = m_reqs[m_pending.front()].scene;
+        const std::vector<rvs_source>& S = m_scenes[scene];
+        for (auto it = m_pending.begin(); it != m_pending.end() && (int)batch.size() < m_opt.max_batch;) {
+            const rvs_request& r = m_reqs[*it];
+            if (r.scene != scene) { ++it; continue; }
+            tgts.push_back(r.target);
+            const size_t w2 = rvs_synth_batch_workspace_size(S.data(), (int)S.size(), tgts.data(), (int)tgts.size(),
+                                                             s.cap);
+            const size_t o2 = outb + output_bytes(r.target);
+            const size_t h2 = hostb + yuv_bytes(r.target);
+            const bool fits = w2 > 0 && o2 + w2 <= s.arena_bytes && (!s.pinned || h2 <= s.pinned_bytes);
+            if (!fits) {
+                tgts.pop_back();
+                if (batch.empty()) {
+                    // cannot be served within one slot's share of the budget
+                    m_stats[*it].rc = RVS_E_CAPACITY;
+                    m_stats[*it].done_ms = now_ms();
+                    m_error = RVS_E_CAPACITY;
+                    m_errmsg = "request " + std::to_string(*it) + " does not fit one slot's share of the memory budget";
+                    ++m_done;
+                    it = m_pending.erase(it);
+                    m_cv.notify_all();
+                    continue;
+                }
+                break;
+            }
+            ws = w2; outb = o2; hostb = h2;
+            batch.push_back(*it);
+            it = m_pending.erase(it);
+        }
+        if (batch.empty()) return false;
+        s.scene = scene;
+    }
+    const std::vector<rvs_source>& S = m_scenes[s.scene];
+    s.reqs = batch;
+    s.tgts = tgts;
+    s.ws_bytes = ws;
+    s.outs.assign(batch.size(), rvs_view_out{});
+    s.host_off.assign(batch.size(), 0);
+    // outputs first, workspace after them (so it can grow in place)
+    size_t o = 0, h = 0;
+    for (size_t k = 0; k < batch.size(); ++k) {
+        const rvs_target& t = tgts[k];
+        const size_t px = (size_t)t.cam.w * t.cam.h, b = plane_bytes(t);
+        rvs_view_out& vo = s.outs[k];
+        vo.color = (float*)(s.arena + o); o += align256(px * 3 * sizeof(float));
+        vo.y = s.arena + o; o += align256(px * b);
+        vo.u = s.arena + o; o += align256(px / 4 * b);
+        vo.v = s.arena + o; o += align256(px / 4 * b);
+        s.host_off[k] = h;
+        h += yuv_bytes(t);
+    }
+    s.ws_off = o;
+    const double t_launch = now_ms();
+    int rc = rvs_synthesize_batch_async(S.data(), (int)S.size(), tgts.data(), (int)tgts.size(), s.arena + s.ws_off,
+                                        ws, s.cap, s.outs.data(), s.stream);
+    if (rc == RVS_OK && s.pinned) {
+        for (size_t k = 0; k < batch.size() && rc == RVS_OK; ++k) {
+            const rvs_target& t = tgts[k];
+            const size_t px = (size_t)t.cam.w * t.cam.h, b = plane_bytes(t);
+            uint8_t* dst = s.pinned + s.host_off[k];
+            if (hipMemcpyAsync(dst, s.outs[k].y, px * b, hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
+                hipMemcpyAsync(dst + px * b, s.outs[k].u, px / 4 * b, hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
+                hipMemcpyAsync(dst + px * b + px / 4 * b, s.outs[k].v, px / 4 * b, hipMemcpyDeviceToHost, s.stream) !=
+                    hipSuccess)
+                rc = RVS_E_HIP;
+        }
+    }
+    if (rc == RVS_OK && hipEventRecord(s.done, s.stream) != hipSuccess) rc = RVS_E_HIP;
+    const std::string msg = rc == RVS_OK ? std::string() : std::string(rvs_last_error());
+    std::lock_guard<std::mutex> lk(m_mu);
+    const int id = m_batches++;
+    for (int k : batch) {
+        m_stats[k].launch_ms = t_launch;
+        m_stats[k].batch = id;
+        m_stats[k].batch_views = (int)batch.size();
+    }
+    if (rc != RVS_OK) {
+        (void)hipStreamSynchronize(s.stream);
+        for (int k : batch) {
+            m_stats[k].rc = rc;
+            m_stats[k].done_ms = now_ms();
+            ++m_done;
+        }
+        m_error = rc;
+        m_errmsg = msg.empty() ? "HIP error while enqueueing a batch" : msg;
+        m_cv.notify_all();
+        return true;
+    }
+    s.busy = true;
+    return true;
+}
+
+void Server::finish(Slot& s) {
+    const std::vector<rvs_source>& S = m_scenes[s.scene];
+    uint32_t needed = 0;
+    int rc = rvs_synth_check(S.data(), (int)S.size(), s.tgts.data(), (int)s.tgts.size(), s.arena + s.ws_off, s.cap,
+                             &needed, s.stream);
+    if (rc == RVS_E_CAPACITY && needed > 0) {
+        // A bin overflowed: raise this slot's capacity and run the batch again
+        // (the workspace sits after the outputs and grows in place).
+        uint32_t cap = 1;
+        while (cap < needed) cap <<= 1;
+        s.cap = cap;
+        const size_t w2 = rvs_synth_batch_workspace_size(S.data(), (int)S.size(), s.tgts.data(),
+                                                         (int)s.tgts.size(), cap);
+        if (w2 > 0 && s.ws_off + w2 <= s.arena_bytes) {
+            rc = rvs_synthesize_batch(S.data(), (int)S.size(), s.tgts.data(), (int)s.tgts.size(), s.arena + s.ws_off,
+                                      w2, cap, s.outs.data(), nullptr, s.stream);
+            if (rc == RVS_OK && s.pinned) {
+                for (size_t k = 0; k < s.reqs.size(); ++k) {
+                    const rvs_target& t = s.tgts[k];
+                    const size_t px = (size_t)t.cam.w * t.cam.h, b = plane_bytes(t);
+                    uint8_t* dst = s.pinned + s.host_off[k];
+                    if (hipMemcpyAsync(dst, s.outs[k].y, px * b, hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
+                        hipMemcpyAsync(dst + px * b, s.outs[k].u, px / 4 * b, hipMemcpyDeviceToHost, s.stream) !=
+                            hipSuccess ||
+                        hipMemcpyAsync(dst + px * b + px / 4 * b, s.outs[k].v, px / 4 * b, hipMemcpyDeviceToHost,
+                                       s.stream) != hipSuccess)
+                        rc = RVS_E_HIP;
+                }
+                if (rc == RVS_OK && hipStreamSynchronize(s.stream) != hipSuccess) rc = RVS_E_HIP;
+            }
+        }
+    }
+    const std::string msg = rc == RVS_OK ? std::string() : std::string(rvs_last_error());
+    const double t = now_ms();
+    {
+        std::lock_guard<std::mutex> lk(m_mu);
+        if (rc != RVS_OK) m_errmsg = msg.empty() ? "HIP error in a batch" : msg;
+        for (size_t k = 0; k < s.reqs.size(); ++k) {
+            const int r = s.reqs[k];
+            m_stats[r].rc = rc;
+            m_stats[r].done_ms = t;
+            if (rc != RVS_OK) m_error = rc;
+            if (rc == RVS_OK && m_opt.keep_outputs && s.pinned) {
+                const uint8_t* src = s.pinned + s.host_off[k];
+                m_out[r].assign(src, src + yuv_bytes(s.tgts[k]));
+            }
+            ++m_done;
+            if (m_closed && m_next_closed < m_n) submit_locked(m_next_closed++, t);
+        }
+        s.busy = false;
+    }
+    m_cv.notify_all();
+}
+
+void Server::loop() {
+    (void)hipSetDevice(0);
+    while (true) {
+        bool progressed = false;
+        for (Slot* s : m_slots) {
+            if (s->busy && hipEventQuery(s->done) == hipSuccess) {
+                finish(*s);
+                progressed = true;
+            }
+        }
+        for (Slot* s : m_slots) {
+            if (!s->busy && launch(*s)) progressed = true;
+        }
+        if (!progressed) {
+            std::unique_lock<std::mutex> lk(m_mu);
+            if (m_stop) break;
+            bool any_busy = false;
+            for (Slot* s : m_slots) any_busy = any_busy || s->busy;
+            if (any_busy)
+                m_cv.wait_for(lk, std::chrono::microseconds(20));
+            else
+                m_cv.wait_for(lk, std::chrono::milliseconds(2), [this] { return m_stop || !m_pending.empty(); });
+        }
+    }
+}
+
+}  // namespace rvs

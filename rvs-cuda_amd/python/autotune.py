@@ -1,0 +1,57 @@
+"""Autotuner of the batched and served paths.
+
+Picks, by measurement on the device it runs on:
+  * the bin capacity (tile_cap, quads per (source, tile) bin) from the fill the
+    binning pass reports for a sample batch, with headroom -- it sizes every
+    workspace and hence how many views fit a memory budget;
+  * the serving batch size (views per launch set) and the number of batches in
+    flight (slots, one stream each) from short closed-loop trials, maximizing
+    views/s subject to a p99 latency bound.
+"""
+import itertools
+import time
+
+import rvs_amd as R
+import serving as S
+
+
+def tune_tile_cap(sources, targets, headroom=1.25, floor=64):
+    """Bin capacity for batches like `targets`: next power of two above the
+    largest bin seen times `headroom`."""
+    bs = R.BatchSynthesizer(sources, targets)
+    bs.run()
+    R.profile_enable(True)
+    try:
+        bs.run()
+        c = R.profile_counters()
+    finally:
+        R.profile_enable(False)
+    return max(floor, R._next_pow2(int(c["max_bin"] * headroom) + 1)), c
+
+
+def tune_serving(scenes, requests, memory_budget, tile_cap, p99_bound_ms=50.0, batches=(4, 8, 16, 32, 64),
+                 slots=(1, 2, 3), trial_batches=4, log=None):
+    """Closed-loop trials over (max_batch, slots); each trial serves
+    trial_batches * max_batch * slots requests (cycling `requests`) at
+    concurrency max_batch * slots.  Returns (best, trials)."""
+    trials = []
+    for b, s in itertools.product(batches, slots):
+        n = trial_batches * b * s
+        reqs = [requests[k % len(requests)] for k in range(n)]
+        srv = S.Server(scenes, memory_budget=memory_budget, max_batch=b, slots=s, tile_cap=tile_cap)
+        try:
+            srv.run_closed(reqs[: b * s], concurrency=b * s)   # warm-up
+            t0 = time.perf_counter()
+            st = srv.run_closed(reqs, concurrency=b * s)
+            wall = time.perf_counter() - t0
+        finally:
+            srv.close()
+        summ = S.summarize(st, wall_s=wall)
+        summ.update(max_batch=b, slots=s)
+        trials.append(summ)
+        if log:
+            log(summ)
+    ok = [t for t in trials if t["p99_ms"] <= p99_bound_ms]
+    pool = ok if ok else trials
+    best = max(pool, key=lambda t: t["views_s"]) if ok else min(pool, key=lambda t: t["p99_ms"])
+    return best, trials

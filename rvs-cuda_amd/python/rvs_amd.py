@@ -80,14 +80,19 @@ def hip():
             "rvs_decode_yuv420": ([vp, vp, vp, i, i, i, f, vp, vp], i),
             "rvs_decode_depth": ([vp, i, i, i, f, f, f, i, vp, vp], i),
             "rvs_reproject": ([P(RvsCam), P(RvsCam), P(RvsXform), vp, vp, f, vp, vp, vp], i),
-            "rvs_raster_workspace_size": ([i, i, u32], sz),
+            "rvs_raster_workspace_size": ([i, i, i, i, u32], sz),
             "rvs_raster": ([vp, vp, vp, i, i, i, i, i, f, vp, sz, u32, vp, vp, vp, P(u32), vp], i),
             "rvs_blend_step": ([vp, vp, vp, vp, vp, vp, vp, vp, i, i, f, i, vp], i),
             "rvs_inpaint_workspace_size": ([i, i], sz),
             "rvs_inpaint": ([vp, vp, i, i, vp, sz, vp, vp], i),
             "rvs_encode_yuv420": ([vp, i, i, i, vp, vp, vp, vp], i),
-            "rvs_synth_workspace_size": ([i, P(RvsTarget), u32], sz),
+            "rvs_synth_workspace_size": ([P(RvsSource), i, P(RvsTarget), u32], sz),
             "rvs_synthesize": ([P(RvsSource), i, P(RvsTarget), vp, sz, u32, P(RvsViewOut), P(u32), vp], i),
+            "rvs_synth_batch_workspace_size": ([P(RvsSource), i, P(RvsTarget), i, u32], sz),
+            "rvs_synthesize_batch": ([P(RvsSource), i, P(RvsTarget), i, vp, sz, u32, P(RvsViewOut), P(u32), vp],
+                                     i),
+            "rvs_synthesize_batch_async": ([P(RvsSource), i, P(RvsTarget), i, vp, sz, u32, P(RvsViewOut), vp], i),
+            "rvs_synth_check": ([P(RvsSource), i, P(RvsTarget), i, vp, u32, P(u32), vp], i),
             "rvs_profile_enable": ([i], i),
             "rvs_profile_read": ([vp], i),
             "rvs_profile_counters": ([vp], i),
@@ -260,7 +265,7 @@ def raster(color, depth, uv, out_w, out_h, wrap, rescale, out_color, out_depth, 
     import torch
     in_h, in_w = depth.shape
     while True:
-        need = hip().rvs_raster_workspace_size(out_w, out_h, tile_cap)
+        need = hip().rvs_raster_workspace_size(in_w, in_h, out_w, out_h, tile_cap)
         if workspace is None or workspace.numel() < need:
             workspace = torch.empty(need, dtype=torch.uint8, device=depth.device)
         needed = C.c_uint32(0)
@@ -311,9 +316,10 @@ def profile_read():
 
 
 def profile_counters():
-    c = np.zeros(4, np.uint32)
+    c = np.zeros(5, np.uint32)
     check(hip().rvs_profile_counters(c.ctypes.data))
-    return {"replay_tiles": int(c[0]), "bin_overflow": int(c[1]), "max_bin": int(c[2]), "bin_entries": int(c[3])}
+    return {"replay_tiles": int(c[0]), "bin_overflow": int(c[1]), "max_bin": int(c[2]), "bin_entries": int(c[3]),
+            "bin_work_items": int(c[4])}
 
 
 # ------------------------------------------------------------- fused path
@@ -378,7 +384,10 @@ class Synthesizer:
 
     def _alloc_ws(self):
         import torch
-        self.ws_bytes = hip().rvs_synth_workspace_size(len(self.sources), C.byref(self.target), self.tile_cap)
+        self.ws_bytes = hip().rvs_synth_workspace_size(self.arr, len(self.sources), C.byref(self.target),
+                                                       self.tile_cap)
+        if self.ws_bytes == 0:
+            raise RvsError(-1, hip().rvs_last_error().decode())
         self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=self.device)
 
     def set_pose(self, pos, rot):
@@ -401,3 +410,82 @@ class Synthesizer:
             return self.run(stream)
         check(rc)
         return self.out
+
+
+def _next_pow2(n):
+    cap = 1
+    while cap < n:
+        cap <<= 1
+    return cap
+
+
+class BatchSynthesizer:
+    """Many target views from the same resident sources in one launch set
+    (rvs_synthesize_batch).  targets: list of RvsTarget (sizes may differ)."""
+
+    def __init__(self, sources, targets, tile_cap=0, device="cuda", want_maps=False):
+        import torch
+        self.sources = sources
+        self.arr = (RvsSource * len(sources))(*[s.struct() for s in sources])
+        self.tile_cap = tile_cap
+        self.device = device
+        self.set_targets(targets, want_maps)
+
+    def set_targets(self, targets, want_maps=False):
+        import torch
+        self.n = len(targets)
+        self.targets = (RvsTarget * self.n)(*targets)
+        self.outs = []
+        vo = (RvsViewOut * self.n)()
+        for k, t in enumerate(targets):
+            w, h = t.cam.w, t.cam.h
+            dt = torch.uint8 if t.bitdepth <= 8 else torch.int16
+            o = {"color": torch.empty((h, w, 3), dtype=torch.float32, device=self.device),
+                 "y": torch.empty((h, w), dtype=dt, device=self.device),
+                 "u": torch.empty((h // 2, w // 2), dtype=dt, device=self.device),
+                 "v": torch.empty((h // 2, w // 2), dtype=dt, device=self.device)}
+            for key in ("quality", "depth", "validity"):
+                o[key] = torch.empty((h, w), dtype=torch.float32, device=self.device) if want_maps else None
+            self.outs.append(o)
+            for key in ("color", "quality", "depth", "validity", "y", "u", "v"):
+                setattr(vo[k], key, o[key].data_ptr() if o[key] is not None else None)
+        self.vo = vo
+        self._alloc_ws()
+
+    def _alloc_ws(self):
+        import torch
+        self.ws_bytes = hip().rvs_synth_batch_workspace_size(self.arr, len(self.sources), self.targets, self.n,
+                                                             self.tile_cap)
+        if self.ws_bytes == 0:
+            raise RvsError(-1, hip().rvs_last_error().decode())
+        self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=self.device)
+
+    def set_pose(self, k, pos, rot):
+        for i in range(3):
+            self.targets[k].pos[i], self.targets[k].rot[i] = pos[i], rot[i]
+
+    def run(self, stream=None):
+        needed = C.c_uint32(0)
+        rc = hip().rvs_synthesize_batch(self.arr, len(self.sources), self.targets, self.n, _ptr(self.ws),
+                                        self.ws_bytes, self.tile_cap, self.vo, C.byref(needed), _stream(stream))
+        if rc == RVS_E_CAPACITY and needed.value > 0:
+            self.tile_cap = _next_pow2(needed.value)
+            self._alloc_ws()
+            return self.run(stream)
+        check(rc)
+        return self.outs
+
+    def enqueue(self, stream=None):
+        """Enqueue only (capturable in a HIP graph); call check() afterwards."""
+        check(hip().rvs_synthesize_batch_async(self.arr, len(self.sources), self.targets, self.n, _ptr(self.ws),
+                                               self.ws_bytes, self.tile_cap, self.vo, _stream(stream)))
+
+    def check(self, stream=None):
+        """Returns 0, or the tile capacity needed when a bin overflowed."""
+        needed = C.c_uint32(0)
+        rc = hip().rvs_synth_check(self.arr, len(self.sources), self.targets, self.n, _ptr(self.ws), self.tile_cap,
+                                   C.byref(needed), _stream(stream))
+        if rc == RVS_E_CAPACITY and needed.value > 0:
+            return int(needed.value)
+        check(rc)
+        return 0

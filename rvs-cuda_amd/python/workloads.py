@@ -82,3 +82,46 @@ def psnr(a, b, maxv):
     b = np.asarray(b, np.float64)
     mse = np.mean((a - b) ** 2)
     return float("inf") if mse == 0 else 10.0 * np.log10(maxv * maxv / mse)
+
+
+def perspective_cam(w, h, hfov_deg=90.0):
+    """Perspective camera of w x h pixels, principal point at the centre."""
+    f = (w / 2.0) / float(np.tan(np.radians(hfov_deg) / 2.0))
+    return R.cam(0, w, h, f=(f, f), p=(w / 2.0, h / 2.0))
+
+
+# 1280x720, 1920x1080 and 3840x2160 perspective targets (90 degree HFOV)
+BATCH_SIZES = ((1280, 720), (1920, 1080), (3840, 2160))
+
+
+def random_walk(n, seed, pos_lim=0.3, yaw_lim=30.0, pitch_lim=10.0):
+    """n poses (pos[3], rot[3] = yaw, pitch, roll in degrees) of a seeded smooth
+    random walk clamped to +-pos_lim metres, +-yaw_lim and +-pitch_lim degrees,
+    roll 0."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(3)
+    a = np.zeros(2)
+    vp = np.zeros(3)
+    va = np.zeros(2)
+    out = []
+    for _ in range(n):
+        vp = 0.8 * vp + rng.normal(0.0, 0.02, 3)
+        va = 0.8 * va + rng.normal(0.0, 2.0, 2)
+        p = np.clip(p + vp, -pos_lim, pos_lim)
+        a = np.clip(a + va, [-yaw_lim, -pitch_lim], [yaw_lim, pitch_lim])
+        out.append(([float(x) for x in np.float32(p)], [float(np.float32(a[0])), float(np.float32(a[1])), 0.0]))
+    return out
+
+
+def batch_targets(rg, n, seed=3, scale=1, sizes=BATCH_SIZES):
+    """n targets for the sources of rig rg: poses from random_walk(seed), sizes
+    cycling through `sizes` divided by `scale`."""
+    t = rg["target"]
+    s0 = rg["sources"][0]
+    res = []
+    for k, (pos, rot) in enumerate(random_walk(n, seed)):
+        w, h = sizes[k % len(sizes)]
+        c = perspective_cam(w // scale, h // scale)
+        res.append(R.make_target(c, pos, rot, t["bitdepth"], t["blending_exp"], R.scale_of(s0["bd_color"]),
+                                 R.scale_of(s0["bd_depth"])))
+    return res

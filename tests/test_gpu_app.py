@@ -88,7 +88,7 @@ def test_application_matches_oracle(tmp_path, torch_cuda, stagewise):
 
 def test_application_erp_sources(tmp_path, torch_cuda):
     print("case application erp", flush=True)
-    rg = W.rig("erp4_1080p", scale=8)
+    rg = W.rig("erp4_1080p", scale=4)
     planes = W.render_sources(rg)
     trace = [[0.0, 0.0, 0.0, 10.0, 0.0, 0.0], [0.05, 0.05, 0.0, -20.0, 5.0, 0.0]]
     cfg = _write_case(tmp_path, rg, planes, trace)

@@ -1,0 +1,162 @@
+"""GPU: batched synthesis (many target views of uneven size from the same
+resident sources in one launch set), frustum culling of the binning work, and
+HIP-graph capture of a batch.
+
+The bar: every view of a batch is bit-identical (all maps and YUV planes) to
+the same view synthesized alone, and to the CPU oracle; culled poses (targets
+looking away from most of the source data) match the oracle exactly; graph
+replays reproduce the eager result on every replay.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+import rvs_amd as R
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+MAPS = ("color", "y", "u", "v")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _alone(srcs, tgt):
+    syn = R.Synthesizer(srcs, tgt, want_maps=False)
+    return {k: v.clone() for k, v in syn.run().items() if v is not None}
+
+
+def _oracle(rg, planes, tgt):
+    t = rg["target"]
+    c = tgt.cam
+    oc = ob.make_cam(c.proj, c.w, c.h, f=tuple(c.f), p=tuple(c.p), hor=tuple(c.hor), ver=tuple(c.ver),
+                     full_hor=c.full_hor)
+    return ob.synthesize(W.oracle_sources(rg, planes, ob), oc, list(tgt.pos), list(tgt.rot), tgt.bitdepth,
+                         t["blending_exp"])
+
+
+def _check_oracle(out, ref, bitdepth, tag):
+    npdt = np.uint8 if bitdepth <= 8 else np.uint16
+    for k in ("y", "u", "v"):
+        a = out[k].cpu().numpy().view(npdt)
+        assert np.array_equal(a, ref[k]), "%s: plane %s differs in %d samples" % (tag, k, int((a != ref[k]).sum()))
+
+
+def test_batch_equals_alone_mixed_sizes(torch_cuda):
+    torch = torch_cuda
+    say("case batch erp4/4 mixed sizes x6")
+    rg = W.rig("erp4_1080p", scale=4)
+    planes = W.render_sources(rg)
+    srcs = W.device_sources(rg, planes)
+    tgts = W.batch_targets(rg, 6, seed=3, scale=4)
+    bs = R.BatchSynthesizer(srcs, tgts, want_maps=True)
+    outs = bs.run()
+    torch.cuda.synchronize()
+    for k, t in enumerate(tgts):
+        say("  view %d %dx%d" % (k, t.cam.w, t.cam.h))
+        syn = R.Synthesizer(srcs, t, want_maps=True)
+        alone = syn.run()
+        torch.cuda.synchronize()
+        for key in ("color", "quality", "depth", "validity", "y", "u", "v"):
+            assert torch.equal(outs[k][key], alone[key]), "view %d map %s" % (k, key)
+    for k in (0, 1):
+        _check_oracle(outs[k], _oracle(rg, planes, tgts[k]), tgts[k].bitdepth, "batch view %d" % k)
+
+
+def test_batch_rects_many_views(torch_cuda):
+    torch = torch_cuda
+    say("case batch rects/2 x12")
+    rg = W.rig("rects_640", scale=2)
+    planes = W.render_sources(rg)
+    srcs = W.device_sources(rg, planes)
+    tgts = []
+    for k, (pos, rot) in enumerate(W.random_walk(12, seed=7, pos_lim=0.05, yaw_lim=8.0, pitch_lim=4.0)):
+        tgts.append(W.device_target(rg, pos, rot))
+    bs = R.BatchSynthesizer(srcs, tgts)
+    outs = bs.run()
+    torch.cuda.synchronize()
+    for k in range(len(tgts)):
+        alone = _alone(srcs, tgts[k])
+        for key in MAPS:
+            assert torch.equal(outs[k][key], alone[key]), "view %d map %s" % (k, key)
+    for k in (0, 5, 11):
+        _check_oracle(outs[k], _oracle(rg, planes, tgts[k]), tgts[k].bitdepth, "rects view %d" % k)
+
+
+@pytest.mark.parametrize("rot", [[90.0, 0.0, 0.0], [180.0, 0.0, 0.0], [-60.0, 40.0, 0.0], [30.0, -80.0, 10.0]])
+def test_culled_poses_match_oracle_erp(torch_cuda, rot):
+    """Targets that see little of each ERP source: binning work items are culled."""
+    torch = torch_cuda
+    say("case culled erp rot=%s" % rot)
+    rg = W.rig("erp4_1080p", scale=4)
+    planes = W.render_sources(rg)
+    srcs = W.device_sources(rg, planes)
+    tgt = W.device_target(rg, [0.1, -0.05, 0.02], rot)
+    R.profile_enable(True)
+    try:
+        out = R.Synthesizer(srcs, tgt, want_maps=False).run()
+        torch.cuda.synchronize()
+        cnt = R.profile_counters()
+    finally:
+        R.profile_enable(False)
+    items = 4 * ((1024 - 1 + 63) // 64) * ((512 - 1 + 3) // 4)
+    say("  work items kept %d of %d" % (cnt["bin_work_items"], items))
+    assert cnt["bin_work_items"] < items
+    _check_oracle(out, _oracle(rg, planes, tgt), tgt.bitdepth, "culled %s" % rot)
+
+
+@pytest.mark.parametrize("yaw", [25.0, 70.0, 180.0])
+def test_culled_poses_match_oracle_perspective(torch_cuda, yaw):
+    torch = torch_cuda
+    say("case culled rects yaw=%s" % yaw)
+    rg = W.rig("rects_640", scale=2)
+    planes = W.render_sources(rg)
+    srcs = W.device_sources(rg, planes)
+    tgt = W.device_target(rg, [0.0, 0.0, 0.0], [yaw, 3.0, 0.0])
+    out = R.Synthesizer(srcs, tgt, want_maps=False).run()
+    torch.cuda.synchronize()
+    _check_oracle(out, _oracle(rg, planes, tgt), tgt.bitdepth, "rects yaw %s" % yaw)
+
+
+def test_batch_graph_replay(torch_cuda):
+    """A batch captured into a HIP graph replays to the eager result, replay
+    after replay, with unrelated eager work in between."""
+    torch = torch_cuda
+    say("case graph replay")
+    rg = W.rig("erp4_1080p", scale=4)
+    planes = W.render_sources(rg)
+    srcs = W.device_sources(rg, planes)
+    tgts = W.batch_targets(rg, 4, seed=5, scale=4)
+    bs = R.BatchSynthesizer(srcs, tgts)
+    eager = [{k: v.clone() for k, v in o.items() if v is not None} for o in bs.run()]
+    torch.cuda.synchronize()
+    for o in bs.outs:
+        for k in MAPS:
+            o[k].zero_()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        bs.enqueue()   # warm-up on the capture stream
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        bs.enqueue()
+    for rep in range(3):
+        for o in bs.outs:
+            for k in MAPS:
+                o[k].zero_()
+        junk = torch.randn(1 << 20, device="cuda").sum()   # unrelated eager work
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert bs.check() == 0
+        for v, o in enumerate(bs.outs):
+            for k in MAPS:
+                assert torch.equal(o[k], eager[v][k]), "replay %d view %d map %s" % (rep, v, k)
+        del junk
