@@ -1,17 +1,29 @@
-"""Benchmark of the per-frame view-synthesis path on one MI355X.
+"""Benchmark of the view-synthesis path on one MI355X.
 
-Workload (default): four equirectangular 4096x2048 10-bit sources with 16-bit
-inverse depth (0.5-25 m) on a 0.3 m square -> one 1920x1080 perspective view
-(90 degree horizontal FOV) at the square's centre; synthetic seeded scene.
-One step = one synthesized view, sources resident in HBM: reproject + bin +
-rasterise + blend all four sources, inpaint, encode YUV420 10-bit.
+Workload: four equirectangular 4096x2048 10-bit sources with 16-bit inverse
+depth (0.5-25 m) on a 0.3 m square, resident in HBM (with their culling
+tables), -> 1920x1080 perspective views (90 degree horizontal FOV) at poses of
+a seeded smooth random walk (+-0.3 m, yaw +-30, pitch +-10 degrees);
+synthetic seeded scene.  A view = reproject + bin + rasterise + blend all four
+sources, inpaint, encode YUV420 10-bit.
 
-Prints one JSON line.  `value` = views/s of the timed steps; p50/p99 are the
-per-view latencies (host call to completion, synchronous like the reference).
-roofline: algorithmic bytes of one view (each source sample once at its stored
-precision + each output sample once) / the view's device time, against 8 TB/s.
-cpu_baseline: the CPU oracle (single-threaded restatement of the reference CPU
-path) on the same view, on this host's cores.
+Prints one JSON line:
+  value         served views/s: the native request loop (continuous batching,
+                memory-budget admission, batches in flight on several streams,
+                YUV copied back to host memory per request) in a closed loop at
+                the autotuned batch size / slot count; `steps` requests timed
+                after `warmup` requests.  p50/p99: per-request latency
+                (arrival to host copy complete).
+  plain         the plain per-view path: one view per launch set, synchronous
+                (host call to completion), sources resident.
+  open_loop     the same server under Poisson arrivals at 80% of `value`.
+  batched       one launch set of `batch` views.
+  roofline      the dominant kernel (k_raster, tile raster + blend) in the
+                served configuration: algorithmic bytes of one launch set of
+                B views (every source sample once at its stored precision +
+                every output sample once) / k_raster's HIP-event duration, vs
+                8 TB/s.
+  cpu_baseline  the single-threaded CPU oracle on one view of the workload.
 """
 import argparse
 import json
@@ -25,30 +37,58 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "rvs-cuda_amd", "python"))
 
 HBM_PEAK_GBS = 8000.0
+P99_BOUND_MS = 50.0
 
 
-def b_alg(rg):
-    """Algorithmic bytes of one view: sum_s W*H*(1.5*b_color + b_depth) + W*H*1.5*b_out."""
-    b = lambda bd: 1 if bd <= 8 else 2  # noqa: E731
+def _b(bd):
+    return 1 if bd <= 8 else 2
+
+
+def b_alg(rg, targets):
+    """Algorithmic bytes of one launch set: sum_s W*H*(1.5*b_color + b_depth) + sum_v W*H*1.5*b_out."""
     tot = 0
     for s in rg["sources"]:
         c = s["cam"]
-        tot += c.w * c.h * (1.5 * b(s["bd_color"]) + b(s["bd_depth"]))
-    t = rg["target"]
-    tot += t["cam"].w * t["cam"].h * 1.5 * b(t["bitdepth"])
+        tot += c.w * c.h * (1.5 * _b(s["bd_color"]) + _b(s["bd_depth"]))
+    for t in targets:
+        tot += t.cam.w * t.cam.h * 1.5 * _b(t.bitdepth)
     return int(tot)
+
+
+def log(*a):
+    print("#", *a, file=sys.stderr, flush=True)
+
+
+def stage_profile(run, reps, R):
+    """Mean per-stage device ms of `run()` over `reps` profiled calls, plus counters."""
+    R.profile_enable(True)
+    stage = {k: [] for k in R.PROF_STAGES}
+    try:
+        for _ in range(reps):
+            run()
+            for k, v in R.profile_read().items():
+                stage[k].append(v)
+        counters = R.profile_counters()
+    finally:
+        R.profile_enable(False)
+    return {k: float(np.mean(v)) for k, v in stage.items()}, counters
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=512, help="served requests timed")
+    ap.add_argument("--warmup", type=int, default=64, help="served requests before the timed run")
+    ap.add_argument("--plain-steps", type=int, default=30)
     ap.add_argument("--workload", default="erp4_1080p")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--profile-steps", type=int, default=10)
-    ap.add_argument("--batch", type=int, default=64, help="views per batch of the batched measurement (0: skip)")
-    ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--profile-reps", type=int, default=5)
+    ap.add_argument("--batch", type=int, default=64, help="views of the single-batch measurement (0: skip)")
+    ap.add_argument("--budget-gib", type=float, default=32.0, help="serving memory budget (HBM)")
+    ap.add_argument("--tune-batches", default="4,8,16,32")
+    ap.add_argument("--tune-slots", default="1,2,3")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a plain device copy of this many bytes 5 times (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it (got --gpus %d)"
@@ -56,108 +96,146 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_probe > 0:
+        a = torch.empty(args.copy_probe, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe}), flush=True)
+        return
+    import autotune as A
     import rvs_amd as R
+    import serving as S
     import workloads as W
 
     rg = W.rig(args.workload)
     t0 = time.time()
     planes = W.render_sources(rg)
-    print("# scene rendered in %.1fs" % (time.time() - t0), file=sys.stderr, flush=True)
+    log("scene rendered in %.1fs" % (time.time() - t0))
     srcs = W.device_sources(rg, planes)
     tgt = W.device_target(rg)
+    (tw, th) = (rg["target"]["cam"].w, rg["target"]["cam"].h)
+    torch.cuda.synchronize()
+
+    # ---------------------------------------------------------- plain per-view path
     syn = R.Synthesizer(srcs, tgt, want_maps=False)
-    torch.cuda.synchronize()
-    for _ in range(args.warmup):
+    for _ in range(3):
         syn.run()
     torch.cuda.synchronize()
-
     lat = []
-    torch.cuda.synchronize()
     t_start = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(args.plain_steps):
         a = time.perf_counter()
-        syn.run()            # returns after the view is complete (synchronous plain path)
+        syn.run()            # returns after the view is complete (synchronous)
         lat.append(time.perf_counter() - a)
-    torch.cuda.synchronize()
-    t_total = time.perf_counter() - t_start
-    views_s = args.steps / t_total
+    plain_s = time.perf_counter() - t_start
+    plain_views_s = args.plain_steps / plain_s
+    plain_stage, plain_counters = stage_profile(syn.run, args.profile_reps, R)
+    log("plain %.1f views/s" % plain_views_s)
 
-    # device time per stage (HIP events on the synthesis stream), separate pass
-    R.profile_enable(True)
-    stage = {k: [] for k in R.PROF_STAGES}
-    for _ in range(args.profile_steps):
-        syn.run()
-        for k, v in R.profile_read().items():
-            stage[k].append(v)
-    counters = R.profile_counters()
-    R.profile_enable(False)
-    stage_ms = {k: float(np.mean(v)) for k, v in stage.items()}
-    dev_ms = sum(stage_ms.values())
-    balg = b_alg(rg)
-    achieved = balg / (dev_ms * 1e-3) / 1e9
+    # --------------------------------------------------------------- autotuning
+    walk = W.batch_targets(rg, max(args.steps, 256), seed=4, sizes=((tw, th),))
+    cap, cap_counters = A.tune_tile_cap(srcs, walk[:16])
+    log("tile_cap %d (max bin %d)" % (cap, cap_counters["max_bin"]))
+    budget = int(args.budget_gib * (1 << 30))
+    reqs = [(0, t) for t in walk]
+    best, trials = A.tune_serving([srcs], reqs, budget, cap, p99_bound_ms=P99_BOUND_MS,
+                                  batches=tuple(int(x) for x in args.tune_batches.split(",")),
+                                  slots=tuple(int(x) for x in args.tune_slots.split(",")), trial_batches=3,
+                                  log=lambda t: log("trial", t))
+    B, SL = best["max_batch"], best["slots"]
+    log("tuned: batch %d slots %d" % (B, SL))
+
+    # ------------------------------------------------------------------ serving
+    staging = B * A.yuv_bytes(walk[0])
+    srv = S.Server([srcs], memory_budget=budget, max_batch=B, slots=SL, tile_cap=cap, host_staging_bytes=staging)
+    try:
+        srv.run_closed(reqs[: args.warmup], concurrency=B * SL)
+        t_a = time.perf_counter()
+        st = srv.run_closed([reqs[k % len(reqs)] for k in range(args.steps)], concurrency=B * SL)
+        served_s = time.perf_counter() - t_a
+        served = S.summarize(st, wall_s=served_s)
+        rate = 0.8 * served["views_s"]
+        st_open = srv.run_open([reqs[k % len(reqs)] for k in range(args.steps)], rate=rate, seed=4)
+        open_loop = S.summarize(st_open)
+        open_loop["offered_rate"] = round(rate, 1)
+    finally:
+        srv.close()
+    log("served %s" % served)
+
+    # ------------------------------- one launch set of B views: dominant kernel
+    bs = R.BatchSynthesizer(srcs, walk[:B], tile_cap=cap)
+    bs.run()
+    bstage, bcount = stage_profile(bs.run, args.profile_reps, R)
+    balg = b_alg(rg, walk[:B])
+    achieved = balg / (bstage["raster"] * 1e-3) / 1e9
+    del bs
+
+    batched = None
+    if args.batch > 0:
+        bb = R.BatchSynthesizer(srcs, walk[: args.batch], tile_cap=cap)
+        bb.run()
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(3):
+            a = time.perf_counter()
+            bb.run()
+            times.append(time.perf_counter() - a)
+        bt = float(np.median(times))
+        batched = {"views": args.batch, "views_s": round(args.batch / bt, 2), "batch_ms": round(1e3 * bt, 3)}
+        del bb
 
     out = {
         "metric": "served views/s + p50/p99 ms vs plain per-view; 4x4096x2048 ERP->1080p",
-        "value": round(views_s, 3),
+        "value": served["views_s"],
         "unit": "views/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * t_total / args.steps, 3),
-        "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
-        "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+        "ms_per_step": round(1e3 * served_s / args.steps, 4),
+        "p50_ms": served["p50_ms"],
+        "p99_ms": served["p99_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
+        "vs_plain": round(served["views_s"] / plain_views_s, 2),
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "%s: 4x ERP 4096x2048 10-bit + 16-bit depth -> 1920x1080 perspective, blending 1.0"
-                        % args.workload,
-            "mode": "plain per-view path (one view per launch sequence, synchronous)",
-            "sources_resident": True,
+            "workload": "%s: 4x ERP 4096x2048 10-bit + 16-bit depth (resident) -> 1920x1080 perspective views at "
+                        "random-walk poses, blending 1.0, YUV420 10-bit out" % args.workload,
+            "mode": "served: closed loop, concurrency = batch x slots, continuous batching",
+            "max_batch": B, "slots": SL, "tile_cap": cap, "memory_budget_gib": args.budget_gib,
+            "p99_bound_ms": P99_BOUND_MS,
         },
-        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
-        "counters": counters,
+        "served": served,
+        "open_loop": open_loop,
+        "plain": {
+            "views_s": round(plain_views_s, 2),
+            "ms_per_view": round(1e3 * plain_s / args.plain_steps, 4),
+            "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+            "stage_ms": {k: round(v, 4) for k, v in plain_stage.items()},
+            "counters": plain_counters,
+        },
+        "batched": batched,
+        "autotune": {"trials": trials},
         "roofline": {
-            "kernel": "one view's launch set (bin, raster+blend, inpaint, encode)",
+            "kernel": "k_raster (tile raster + blend), one launch over %d views" % B,
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "alg_bytes": balg,
-            "device_ms": round(dev_ms, 4),
+            "kernel_ms": round(bstage["raster"], 4),
+            "launch_set_stage_ms": {k: round(v, 4) for k, v in bstage.items()},
+            "counters": bcount,
             "traffic": None,
         },
     }
-
-    if args.batch > 0:
-        # batched: `batch` 1080p views at random-walk poses, one launch set per batch
-        tgts = W.batch_targets(rg, args.batch, seed=3, sizes=((rg["target"]["cam"].w, rg["target"]["cam"].h),))
-        bs = R.BatchSynthesizer(srcs, tgts)
-        bs.run()
-        torch.cuda.synchronize()
-        times = []
-        for _ in range(args.batch_reps):
-            a = time.perf_counter()
-            bs.run()
-            times.append(time.perf_counter() - a)
-        R.profile_enable(True)
-        bs.run()
-        bstage = R.profile_read()
-        bcount = R.profile_counters()
-        R.profile_enable(False)
-        bt = float(np.median(times))
-        out["batched"] = {
-            "views": args.batch,
-            "views_s": round(args.batch / bt, 2),
-            "batch_ms": round(1e3 * bt, 3),
-            "stage_ms": {k: round(v, 4) for k, v in bstage.items()},
-            "counters": bcount,
-            "vs_plain": round(args.batch / bt / views_s, 2),
-        }
-        del bs
 
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "tests"))

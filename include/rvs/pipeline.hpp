@@ -82,6 +82,8 @@ private:
     Parameters m_params;
     float m_color_scale, m_depth_scale;
     DeviceBuffer m_y, m_u, m_v, m_raw_depth, m_table;
+    DeviceBuffer m_prep;              // resident culling table (rvs_source_prep)
+    float m_prep_scale = 0.f;
     bool m_decoded = false;
 };
 

@@ -149,7 +149,21 @@ typedef struct {
     const void *y, *u, *v;    /* device colour planes (u8 or u16) */
     const void* depth;        /* device depth plane (Y plane only, w*h) */
     const float* erp_table;   /* device rvs_erp_table_fill(cam) for ERP sources, else NULL */
+    const float* prep;        /* device rvs_source_prep table, or NULL (then built per call) */
+    float prep_depth_scale;   /* depth_scale the prep table was built with */
 } rvs_source;
+
+/* Resident per-source binning metadata: for every block of 64 x 4 source quads,
+ * the bounding box of its unprojected points (source camera frame).  A
+ * perspective target skips the blocks that lie behind it or beyond an image
+ * side (with error margins; the result is unchanged).  Built once per loaded
+ * source frame and reused by every view while the depth plane is unchanged;
+ * valid for targets with the same depth_scale.  rvs_source_prep fills `out`
+ * (device, rvs_source_prep_size bytes) and sets src->prep and
+ * src->prep_depth_scale.  rvs_synthesize* build the table in their workspace
+ * when src->prep is NULL or its depth_scale differs from the target's. */
+size_t rvs_source_prep_size(const rvs_cam* cam);
+int rvs_source_prep(rvs_source* src, float depth_scale, void* out, size_t bytes, rvs_stream_t stream);
 
 /* The virtual view to synthesize (VirtualCameraNames[0] after the pose trace). */
 typedef struct {
@@ -212,10 +226,11 @@ int rvs_synth_check(const rvs_source* srcs, int n_src, const rvs_target* tgts, i
 
 /* Per-stage device times of the last rvs_synthesize* call on the calling
  * thread, measured with HIP events recorded on its stream between the stages
- * when enabled: [0] binning (incl. source bounds and culling), [1] tile raster
- * + blend (incl. in-order replay), [2] inpainting, [3] encoding; milliseconds.
- * After an _async call, synchronise the stream first. */
-#define RVS_PROF_STAGES 4
+ * when enabled: [0] binning (incl. culling tables built per call, culling),
+ * [1] tile raster + blend (k_raster), [2] in-order replay of deferred pixels,
+ * [3] inpainting, [4] encoding; milliseconds.  After an _async call,
+ * synchronise the stream first. */
+#define RVS_PROF_STAGES 5
 int rvs_profile_enable(int on);
 int rvs_profile_read(float ms[RVS_PROF_STAGES]);
 /* Counters of the last profiled call: [0] tiles with pixels replayed in order,

@@ -57,7 +57,7 @@ typedef struct {
     uint32_t tile_cap;          /* initial per (source, tile) bin capacity (0 = default); grows on overflow */
     int copy_to_host;           /* copy each request's YUV planes to pinned host memory */
     int keep_outputs;           /* keep a host copy per request (rvs_server_output) */
-    size_t host_staging_bytes;  /* pinned staging per slot (0 = a quarter of the slot's share) */
+    size_t host_staging_bytes;  /* pinned staging per slot (0 = a quarter of the slot's share, at most 1 GiB) */
 } rvs_serve_opts;
 
 typedef struct {

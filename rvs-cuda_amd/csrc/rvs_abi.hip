@@ -221,9 +221,9 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
     Layout L;
     P.views = L.take((size_t)n_views * sizeof(rvs_k::ViewDesc));
     P.geoms = L.take((size_t)n_views * n_src * sizeof(PairGeom));
-    const size_t per = (size_t)((max_sw - 1 + rvs_k::kBinQX - 1) / rvs_k::kBinQX) *
-                       ((max_sh - 1 + rvs_k::kBinQY - 1) / rvs_k::kBinQY);
-    P.bounds = L.take(P.any_persp ? per * n_src * 6 * sizeof(float) : 0);
+    size_t nb = 0;
+    for (int k = 0; k < n_src; ++k) nb += align256(rvs_k::src_bounds_floats(srcs[k].cam.w, srcs[k].cam.h) * 4);
+    P.bounds = L.take(P.any_persp ? nb : 0);
     P.quality = L.take((size_t)px * sizeof(float));
     plan_raster(L, P.R, n_src, n_views, (uint32_t)tiles, (uint32_t)px, cap, max_sw, max_sh);
     P.inpaint = L.take(rvs_k::inpaint_ws_bytes((size_t)px, n_views));
@@ -296,23 +296,47 @@ int enqueue_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int
         for (int k = 0; k < n_src; ++k) fill_pair(hg[(size_t)v * n_src + k], srcs[k], tgts[v]);
     rvs_k::ViewDesc* dviews = (rvs_k::ViewDesc*)(base + P.views);
     PairGeom* dg = (PairGeom*)(base + P.geoms);
-    float* bounds = P.any_persp ? (float*)(base + P.bounds) : nullptr;
     ws = carve_raster(base, P.R);
     const rvs_k::BatchDims& d = P.R.d;
-    HIP_TRY(rvs_k::launch_upload(views.data(), views.size() * sizeof(rvs_k::ViewDesc), dviews, s), "view upload");
-    HIP_TRY(rvs_k::launch_upload(hg.data(), hg.size() * sizeof(PairGeom), dg, s), "geometry upload");
     HIP_TRY(reset_raster(ws, d, s), "reset");
     prof_mark(0, s);
-    if (bounds) HIP_TRY(rvs_k::launch_src_bounds(dg, n_src, d, bounds, s), "source bounds");
-    HIP_TRY(rvs_k::launch_bin<PairGeom>(dg, dviews, bounds, d, ws, s), "bin");
+    if (P.any_persp) {
+        // culling tables: the source's resident one when it matches the views'
+        // depth scale, else one built here; views with another depth scale do not cull
+        char* nb = base + P.bounds;
+        for (int k = 0; k < n_src; ++k) {
+            const rvs_source& S = srcs[k];
+            const float ds = tgts[0].depth_scale;
+            const float* tab = nullptr;
+            if (S.prep && S.prep_depth_scale == ds) {
+                tab = S.prep;
+            } else {
+                float* own = (float*)nb;
+                HIP_TRY(rvs_k::launch_src_bounds(hg[k], own, s), "source bounds");
+                tab = own;
+            }
+            nb += align256(rvs_k::src_bounds_floats(S.cam.w, S.cam.h) * 4);
+            for (int v = 0; v < n_views; ++v) {
+                PairGeom& g = hg[(size_t)v * n_src + k];
+                const bool same = tgts[v].depth_scale == ds;
+                g.bounds = same ? tab : nullptr;
+                g.bounds_bx = rvs_k::bin_blocks_x(S.cam.w);
+            }
+        }
+    }
+    HIP_TRY(rvs_k::launch_upload(views.data(), views.size() * sizeof(rvs_k::ViewDesc), dviews, s), "view upload");
+    HIP_TRY(rvs_k::launch_upload(hg.data(), hg.size() * sizeof(PairGeom), dg, s), "geometry upload");
+    HIP_TRY(rvs_k::launch_bin<PairGeom>(dg, dviews, d, ws, s), "bin");
     prof_mark(1, s);
     HIP_TRY((rvs_k::launch_raster<PairGeom, rvs_k::MODE_BLEND>(dg, dviews, d, ws, s)), "raster");
     prof_mark(2, s);
+    HIP_TRY((rvs_k::launch_replay<PairGeom, rvs_k::MODE_BLEND>(dg, dviews, d, ws, s)), "replay");
+    prof_mark(3, s);
     const rvs_k::InpaintWS iws = rvs_k::inpaint_ws_carve(base + P.inpaint, d.npx, n_views);
     HIP_TRY(rvs_k::launch_inpaint(dviews, n_views, P.max_w, P.max_h, iws, nullptr, s), "inpaint");
-    prof_mark(3, s);
-    HIP_TRY(rvs_k::launch_encode_views(dviews, n_views, P.max_w, P.max_h, s), "encode");
     prof_mark(4, s);
+    HIP_TRY(rvs_k::launch_encode_views(dviews, n_views, P.max_w, P.max_h, s), "encode");
+    prof_mark(5, s);
     return RVS_OK;
 }
 
@@ -365,6 +389,30 @@ int rvs_erp_table_fill(const rvs_cam* cam, float* out) {
         out[2 * cam->w + i] = std::cos(theta);
         out[2 * cam->w + cam->h + i] = std::sin(theta);
     }
+    return RVS_OK;
+}
+
+size_t rvs_source_prep_size(const rvs_cam* cam) {
+    if (!cam_ok(cam)) return 0;
+    return rvs_k::src_bounds_floats(cam->w, cam->h) * sizeof(float);
+}
+
+int rvs_source_prep(rvs_source* src, float depth_scale, void* out, size_t bytes, rvs_stream_t stream) {
+    if (!src || !cam_ok(&src->cam) || !out || !src->depth || (src->cam.proj == 1 && !src->erp_table))
+        return fail(RVS_E_INVALID, "rvs_source_prep: bad argument");
+    if ((uint64_t)(src->cam.w - 1) * (src->cam.h - 1) >= (1ull << 30))
+        return fail(RVS_E_CAPACITY, "rvs_source_prep: source too large");
+    if (bytes < rvs_source_prep_size(&src->cam)) return fail(RVS_E_INVALID, "rvs_source_prep: buffer too small");
+    PairGeom g;
+    std::memset(&g, 0, sizeof g);
+    fill_source(g, &src->cam, src->erp_table);
+    g.depth = src->depth;
+    g.depth_u8 = src->bitdepth_depth <= 8;
+    g.dscale = depth_scale;
+    g.znear = src->znear; g.zfar = src->zfar; g.has_invalid = src->has_invalid;
+    HIP_TRY(rvs_k::launch_src_bounds(g, (float*)out, (hipStream_t)stream), "source bounds");
+    src->prep = (const float*)out;
+    src->prep_depth_scale = depth_scale;
     return RVS_OK;
 }
 
@@ -453,8 +501,9 @@ int rvs_raster(const float* color, const float* depth, const float* uv, int in_w
     HIP_TRY(rvs_k::launch_upload(&vd, sizeof vd, dv, s), "view upload");
     HIP_TRY(rvs_k::launch_upload(&hg, sizeof hg, dg, s), "geometry upload");
     HIP_TRY(reset_raster(ws, R.d, s), "reset");
-    HIP_TRY(rvs_k::launch_bin<ArrayGeom>(dg, dv, nullptr, R.d, ws, s), "bin");
+    HIP_TRY(rvs_k::launch_bin<ArrayGeom>(dg, dv, R.d, ws, s), "bin");
     HIP_TRY((rvs_k::launch_raster<ArrayGeom, rvs_k::MODE_RAW>(dg, dv, R.d, ws, s)), "raster");
+    HIP_TRY((rvs_k::launch_replay<ArrayGeom, rvs_k::MODE_RAW>(dg, dv, R.d, ws, s)), "replay");
     return check_raster_status(ws, R.d, needed_cap, s);
 }
 

@@ -55,6 +55,10 @@ struct PairGeom {
     float sx, sy;                // cv::transform scale (output size / virtual size)
     int wrap;                    // horizontal wrapping (ERP target with full range)
     float rescale;               // g_rescale (Precision)
+    // bounding boxes of the unprojected source points per binning block
+    // (6 floats each, bounds_bx blocks per row) for frustum culling; null = no culling
+    const float* bounds;
+    int bounds_bx;
 };
 
 // Precomputed-array vertex source (the raster entry point on identical inputs).

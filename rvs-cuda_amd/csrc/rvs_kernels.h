@@ -67,16 +67,24 @@ hipError_t launch_reset_u32(uint32_t* p, size_t n, hipStream_t s);
 // Raw bytes into device memory through kernel arguments (stream-ordered, capturable).
 hipError_t launch_upload(const void* host, size_t bytes, void* dst, hipStream_t s);
 
-// Per binning work item of a perspective-target batch: bounding box of the
-// unprojected source points (source frame) for frustum culling.
-hipError_t launch_src_bounds(const rvs_dev::PairGeom* geoms, int nsrc, const BatchDims& d, float* bounds,
-                             hipStream_t s);
+// Binning blocks per row / column of a source grid.
+inline int bin_blocks_x(int sw) { return (sw - 1 + kBinQX - 1) / kBinQX; }
+inline int bin_blocks_y(int sh) { return (sh - 1 + kBinQY - 1) / kBinQY; }
+// Floats of a source's culling table: one box (min xyz, max xyz) per binning block.
+inline size_t src_bounds_floats(int sw, int sh) { return (size_t)bin_blocks_x(sw) * bin_blocks_y(sh) * 6; }
+
+// Fills the culling table of one source (only its source-side fields are read).
+hipError_t launch_src_bounds(const rvs_dev::PairGeom& g, float* out, hipStream_t s);
 
 template <class G>
-hipError_t launch_bin(const G* geoms, const ViewDesc* views, const float* bounds, const BatchDims& d,
-                      const RasterWS& ws, hipStream_t s);
+hipError_t launch_bin(const G* geoms, const ViewDesc* views, const BatchDims& d, const RasterWS& ws, hipStream_t s);
+// Tile rasterisation + blending (k_raster), then the in-order replay of the
+// deferred pixels (k_fallback); the replay must follow the raster launch.
 template <class G, int MODE>
 hipError_t launch_raster(const G* geoms, const ViewDesc* views, const BatchDims& d, const RasterWS& ws,
+                         hipStream_t s);
+template <class G, int MODE>
+hipError_t launch_replay(const G* geoms, const ViewDesc* views, const BatchDims& d, const RasterWS& ws,
                          hipStream_t s);
 
 }  // namespace rvs_k

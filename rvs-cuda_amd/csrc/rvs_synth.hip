@@ -135,45 +135,40 @@ __device__ inline void quad_of(const G& g, uint32_t q, int& i, int& j) {
 
 // ------------------------------------------------------------ culling
 
-// Bounding box (source camera frame) of the finite unprojected points of one
-// binning work item's vertex window (kBinQY+1 rows x kBinQX+1 columns), as
-// min x, y, z, max x, y, z; no finite point -> min +inf, max -inf.  Only
-// source-side fields are read, so the first view's descriptors serve the batch.
-__global__ void __launch_bounds__(256) k_src_bounds(const PairGeom* __restrict__ geoms, BatchDims d, float* bounds) {
-    const PairGeom& g = geoms[blockIdx.z];
-    const int i0 = blockIdx.y * kBinQY, j0 = blockIdx.x * kBinQX;
-    float mn[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
-    float mx[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
-    if (i0 < g.sh - 1 && j0 < g.sw - 1) {
-        for (int k = threadIdx.x; k < (kBinQY + 1) * (kBinQX + 1); k += 256) {
-            const int i = i0 + k / (kBinQX + 1), j = j0 + k % (kBinQX + 1);
-            if (i >= g.sh || j >= g.sw) continue;
-            const float3 p = unproject_src(g, i, j);
-            if (!finite3(p)) continue;
-            mn[0] = fminf(mn[0], p.x); mx[0] = fmaxf(mx[0], p.x);
-            mn[1] = fminf(mn[1], p.y); mx[1] = fmaxf(mx[1], p.y);
-            mn[2] = fminf(mn[2], p.z); mx[2] = fmaxf(mx[2], p.z);
-        }
+// Culling table of one source: per binning block (kBinQY x kBinQX quads), the
+// bounding box (source camera frame) of the finite unprojected points of its
+// vertex window (kBinQY+1 rows x kBinQX+1 columns) as min x, y, z, max x, y, z;
+// no finite point -> min +inf, max -inf.  One wave per block.
+__global__ void __launch_bounds__(256) k_src_bounds(PairGeom g, int bins_x, int bins_y, float* out) {
+    const int lane = threadIdx.x & 63;
+    const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (item >= bins_x * bins_y) return;   // whole wave
+    const int i0 = (item / bins_x) * kBinQY, j0 = (item % bins_x) * kBinQX;
+    float v[6] = {__builtin_inff(), __builtin_inff(), __builtin_inff(),
+                  -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+    for (int k = lane; k < (kBinQY + 1) * (kBinQX + 1); k += 64) {
+        const int i = i0 + k / (kBinQX + 1), j = j0 + k % (kBinQX + 1);
+        if (i >= g.sh || j >= g.sw) continue;
+        const float3 p = unproject_src(g, i, j);
+        if (!finite3(p)) continue;
+        v[0] = fminf(v[0], p.x); v[1] = fminf(v[1], p.y); v[2] = fminf(v[2], p.z);
+        v[3] = fmaxf(v[3], p.x); v[4] = fmaxf(v[4], p.y); v[5] = fmaxf(v[5], p.z);
     }
-    __shared__ float red[6][256];
-    for (int c = 0; c < 3; ++c) { red[c][threadIdx.x] = mn[c]; red[3 + c][threadIdx.x] = mx[c]; }
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st)
-            for (int c = 0; c < 3; ++c) {
-                red[c][threadIdx.x] = fminf(red[c][threadIdx.x], red[c][threadIdx.x + st]);
-                red[3 + c][threadIdx.x] = fmaxf(red[3 + c][threadIdx.x], red[3 + c][threadIdx.x + st]);
-            }
-        __syncthreads();
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[c] = fminf(v[c], __shfl_xor(v[c], off));
+#pragma unroll
+        for (int c = 3; c < 6; ++c) v[c] = fmaxf(v[c], __shfl_xor(v[c], off));
     }
-    if (threadIdx.x < 6) {
-        const size_t item = ((size_t)blockIdx.z * d.bins_y + blockIdx.y) * d.bins_x + blockIdx.x;
-        bounds[item * 6 + threadIdx.x] = red[threadIdx.x][0];
-    }
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) out[(size_t)item * 6 + c] = v[c];
 }
 
-hipError_t launch_src_bounds(const PairGeom* geoms, int nsrc, const BatchDims& d, float* bounds, hipStream_t s) {
-    hipLaunchKernelGGL(k_src_bounds, dim3(d.bins_x, d.bins_y, nsrc), dim3(256), 0, s, geoms, d, bounds);
+hipError_t launch_src_bounds(const PairGeom& g, float* out, hipStream_t s) {
+    const int bx = bin_blocks_x(g.sw), by = bin_blocks_y(g.sh);
+    hipLaunchKernelGGL(k_src_bounds, dim3((unsigned)((bx * by + 3) / 4)), dim3(256), 0, s, g, bx, by, out);
     return hipGetLastError();
 }
 
@@ -218,18 +213,18 @@ __device__ inline bool box_outside(const PairGeom& g, const float* b, int out_w,
 }
 
 template <class G>
-__device__ inline bool item_culled(const G&, const float*, int, int) { return false; }
+__device__ inline bool item_culled(const G&, int, int, int, int) { return false; }
 template <>
-__device__ inline bool item_culled<PairGeom>(const PairGeom& g, const float* b, int out_w, int out_h) {
-    if (!b || g.dst_erp || g.wrap) return false;
-    return box_outside(g, b, out_w, out_h);
+__device__ inline bool item_culled<PairGeom>(const PairGeom& g, int bx, int by, int out_w, int out_h) {
+    if (!g.bounds || g.dst_erp || g.wrap) return false;
+    return box_outside(g, g.bounds + ((size_t)by * g.bounds_bx + bx) * 6, out_w, out_h);
 }
 
 // One thread per (view, source, item): append the items that may cover the view
 // to the work list (code = (view * nsrc + source) * items + item).
 template <class G>
-__global__ void k_bin_cull(const G* __restrict__ geoms, const ViewDesc* __restrict__ views, const float* bounds,
-                           BatchDims d, RasterWS ws) {
+__global__ void k_bin_cull(const G* __restrict__ geoms, const ViewDesc* __restrict__ views, BatchDims d,
+                           RasterWS ws) {
     const uint32_t per = d.bins_x * d.bins_y;
     const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t total = (size_t)per * d.nsrc * d.nviews;
@@ -237,13 +232,10 @@ __global__ void k_bin_cull(const G* __restrict__ geoms, const ViewDesc* __restri
     if (gid < total) {
         const uint32_t item = (uint32_t)(gid % per);
         const uint32_t vs = (uint32_t)(gid / per);
-        const int s = (int)(vs % (uint32_t)d.nsrc), v = (int)(vs / (uint32_t)d.nsrc);
+        const int v = (int)(vs / (uint32_t)d.nsrc);
         const G& g = geoms[vs];
         const int bx = (int)(item % d.bins_x), by = (int)(item / d.bins_x);
-        if (by * kBinQY < g.sh - 1 && bx * kBinQX < g.sw - 1) {
-            const float* b = bounds ? bounds + ((size_t)s * per + item) * 6 : nullptr;
-            keep = !item_culled(g, b, views[v].w, views[v].h);
-        }
+        if (by * kBinQY < g.sh - 1 && bx * kBinQX < g.sw - 1) keep = !item_culled(g, bx, by, views[v].w, views[v].h);
     }
     const int lane = __lane_id();
     const uint64_t m = __ballot(keep);
@@ -339,11 +331,9 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
 }
 
 template <class G>
-hipError_t launch_bin(const G* geoms, const ViewDesc* views, const float* bounds, const BatchDims& d,
-                      const RasterWS& ws, hipStream_t s) {
+hipError_t launch_bin(const G* geoms, const ViewDesc* views, const BatchDims& d, const RasterWS& ws, hipStream_t s) {
     const size_t total = (size_t)d.bins_x * d.bins_y * d.nsrc * d.nviews;
-    hipLaunchKernelGGL(k_bin_cull<G>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, geoms, views, bounds,
-                       d, ws);
+    hipLaunchKernelGGL(k_bin_cull<G>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, geoms, views, d, ws);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t blocks = total < (size_t)kBinBlocks ? total : (size_t)kBinBlocks;
@@ -645,20 +635,28 @@ hipError_t launch_raster(const G* geoms, const ViewDesc* views, const BatchDims&
                          hipStream_t s) {
     const unsigned blocks = 8u * ((d.ntiles + 7u) / 8u);
     hipLaunchKernelGGL((k_raster<G, MODE>), dim3(blocks), dim3(256), 0, s, geoms, views, d, ws);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+template <class G, int MODE>
+hipError_t launch_replay(const G* geoms, const ViewDesc* views, const BatchDims& d, const RasterWS& ws,
+                         hipStream_t s) {
     const unsigned fb_blocks = d.ntiles < (uint32_t)kFallbackBlocks ? d.ntiles : (uint32_t)kFallbackBlocks;
     hipLaunchKernelGGL((k_fallback<G, MODE>), dim3(fb_blocks), dim3(256), 0, s, geoms, views, d, ws);
     return hipGetLastError();
 }
 
-template hipError_t launch_bin<PairGeom>(const PairGeom*, const ViewDesc*, const float*, const BatchDims&,
-                                         const RasterWS&, hipStream_t);
-template hipError_t launch_bin<ArrayGeom>(const ArrayGeom*, const ViewDesc*, const float*, const BatchDims&,
-                                          const RasterWS&, hipStream_t);
+template hipError_t launch_bin<PairGeom>(const PairGeom*, const ViewDesc*, const BatchDims&, const RasterWS&,
+                                         hipStream_t);
+template hipError_t launch_bin<ArrayGeom>(const ArrayGeom*, const ViewDesc*, const BatchDims&, const RasterWS&,
+                                          hipStream_t);
 template hipError_t launch_raster<PairGeom, MODE_BLEND>(const PairGeom*, const ViewDesc*, const BatchDims&,
                                                         const RasterWS&, hipStream_t);
 template hipError_t launch_raster<ArrayGeom, MODE_RAW>(const ArrayGeom*, const ViewDesc*, const BatchDims&,
+                                                       const RasterWS&, hipStream_t);
+template hipError_t launch_replay<PairGeom, MODE_BLEND>(const PairGeom*, const ViewDesc*, const BatchDims&,
+                                                        const RasterWS&, hipStream_t);
+template hipError_t launch_replay<ArrayGeom, MODE_RAW>(const ArrayGeom*, const ViewDesc*, const BatchDims&,
                                                        const RasterWS&, hipStream_t);
 
 }  // namespace rvs_k

@@ -198,6 +198,11 @@ InputView::InputView(std::string const& color_path, std::string const& depth_pat
         check(rvs_erp_table_fill(&c, tab.data()), "rvs_erp_table_fill");
         m_table.upload(tab.data(), tab.size() * sizeof(float), stream);
     }
+    // culling table of this frame's depth, kept with the view
+    rvs_source s = source();
+    m_prep.resize(rvs_source_prep_size(&s.cam));
+    check(rvs_source_prep(&s, m_depth_scale, m_prep.get(), m_prep.bytes(), stream), "rvs_source_prep");
+    m_prep_scale = m_depth_scale;
     hip_check(hipStreamSynchronize((hipStream_t)stream), "input upload");
 }
 
@@ -213,6 +218,8 @@ rvs_source InputView::source() const {
     for (int k = 0; k < 3; ++k) { s.pos[k] = m_params.getPosition()[k]; s.rot[k] = m_params.getRotation()[k]; }
     s.y = m_y.get(); s.u = m_u.get(); s.v = m_v.get(); s.depth = m_raw_depth.get();
     s.erp_table = s.cam.proj == 1 ? m_table.as<float>() : nullptr;
+    s.prep = m_prep.bytes() ? m_prep.as<float>() : nullptr;
+    s.prep_depth_scale = m_prep_scale;
     return s;
 }
 

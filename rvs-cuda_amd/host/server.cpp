@@ -65,7 +65,8 @@ Server::Server(std::vector<std::vector<rvs_source>> scenes, const rvs_serve_opts
         hip_check(hipMalloc((void**)&s->arena, share), "server arena");
         s->arena_bytes = share;
         if (m_opt.copy_to_host) {
-            s->pinned_bytes = m_opt.host_staging_bytes ? m_opt.host_staging_bytes : share / 4;
+            s->pinned_bytes = m_opt.host_staging_bytes ? m_opt.host_staging_bytes
+                                                       : std::min(share / 4, (size_t)1 << 30);
             hip_check(hipHostMalloc((void**)&s->pinned, s->pinned_bytes, 0), "pinned staging");
         }
         s->cap = m_opt.tile_cap;

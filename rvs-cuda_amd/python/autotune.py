@@ -15,6 +15,11 @@ import rvs_amd as R
 import serving as S
 
 
+def yuv_bytes(t):
+    """Bytes of a target's encoded YUV420 planes."""
+    return t.cam.w * t.cam.h * 3 // 2 * (1 if t.bitdepth <= 8 else 2)
+
+
 def tune_tile_cap(sources, targets, headroom=1.25, floor=64):
     """Bin capacity for batches like `targets`: next power of two above the
     largest bin seen times `headroom`."""
@@ -38,7 +43,8 @@ def tune_serving(scenes, requests, memory_budget, tile_cap, p99_bound_ms=50.0, b
     for b, s in itertools.product(batches, slots):
         n = trial_batches * b * s
         reqs = [requests[k % len(requests)] for k in range(n)]
-        srv = S.Server(scenes, memory_budget=memory_budget, max_batch=b, slots=s, tile_cap=tile_cap)
+        srv = S.Server(scenes, memory_budget=memory_budget, max_batch=b, slots=s, tile_cap=tile_cap,
+                       host_staging_bytes=b * max(yuv_bytes(t) for _, t in requests))
         try:
             srv.run_closed(reqs[: b * s], concurrency=b * s)   # warm-up
             t0 = time.perf_counter()

@@ -34,7 +34,7 @@ class RvsSource(C.Structure):
                 ("znear", C.c_float), ("zfar", C.c_float), ("has_invalid", C.c_int),
                 ("pos", C.c_float * 3), ("rot", C.c_float * 3),
                 ("y", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p), ("depth", C.c_void_p),
-                ("erp_table", C.c_void_p)]
+                ("erp_table", C.c_void_p), ("prep", C.c_void_p), ("prep_depth_scale", C.c_float)]
 
 
 class RvsTarget(C.Structure):
@@ -77,6 +77,8 @@ def hip():
             "rvs_relative_pose": ([vp, vp, vp, vp, P(RvsXform)], i),
             "rvs_erp_table_len": ([P(RvsCam)], sz),
             "rvs_erp_table_fill": ([P(RvsCam), vp], i),
+            "rvs_source_prep_size": ([P(RvsCam)], sz),
+            "rvs_source_prep": ([P(RvsSource), f, vp, sz, vp], i),
             "rvs_decode_yuv420": ([vp, vp, vp, i, i, i, f, vp, vp], i),
             "rvs_decode_depth": ([vp, i, i, i, f, f, f, i, vp, vp], i),
             "rvs_reproject": ([P(RvsCam), P(RvsCam), P(RvsXform), vp, vp, f, vp, vp, vp], i),
@@ -302,7 +304,7 @@ def encode_yuv420(color, bitdepth, y, u, v, stream=None):
     check(hip().rvs_encode_yuv420(_ptr(color), w, h, bitdepth, _ptr(y), _ptr(u), _ptr(v), _stream(stream)))
 
 
-PROF_STAGES = ("bin", "raster", "inpaint", "encode")
+PROF_STAGES = ("bin", "raster", "replay", "inpaint", "encode")
 
 
 def profile_enable(on=True):
@@ -335,6 +337,19 @@ class DeviceSource:
         self.znear, self.zfar, self.has_invalid = znear, zfar, has_invalid
         self.planes = [torch.from_numpy(np.ascontiguousarray(p)).to(device) for p in planes]
         self.table = torch.from_numpy(erp_table(c)).to(device) if c.proj == 1 else None
+        self.prep = None
+        self.prep_scale = 0.0
+        self.device = device
+
+    def prepare(self, depth_scale, stream=None):
+        """Builds the resident culling table (rvs_source_prep) for targets with this depth_scale."""
+        import torch
+        n = hip().rvs_source_prep_size(C.byref(self.cam))
+        self.prep = torch.empty(n, dtype=torch.uint8, device=self.device)
+        s = self.struct()
+        check(hip().rvs_source_prep(C.byref(s), depth_scale, _ptr(self.prep), n, _stream(stream)))
+        self.prep_scale = s.prep_depth_scale
+        return self
 
     def struct(self):
         s = RvsSource()
@@ -345,6 +360,8 @@ class DeviceSource:
             s.pos[k], s.rot[k] = self.pos[k], self.rot[k]
         s.y, s.u, s.v, s.depth = (p.data_ptr() for p in self.planes)
         s.erp_table = self.table.data_ptr() if self.table is not None else None
+        s.prep = self.prep.data_ptr() if self.prep is not None else None
+        s.prep_depth_scale = self.prep_scale
         return s
 
 

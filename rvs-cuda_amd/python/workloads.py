@@ -65,9 +65,16 @@ def oracle_cam(c, ob):
                        full_hor=c.full_hor)
 
 
-def device_sources(rg, planes, device="cuda"):
-    return [R.DeviceSource(s["cam"], s["pos"], s["rot"], s["bd_color"], s["bd_depth"], s["znear"], s["zfar"],
+def device_sources(rg, planes, device="cuda", prep=True):
+    """Resident sources; prep=True also builds their culling tables for the
+    rig's depth scale (fl(1/max_level) of the first source's depth bit depth)."""
+    srcs = [R.DeviceSource(s["cam"], s["pos"], s["rot"], s["bd_color"], s["bd_depth"], s["znear"], s["zfar"],
                            p, device=device) for s, p in zip(rg["sources"], planes)]
+    if prep:
+        ds = R.scale_of(rg["sources"][0]["bd_depth"])
+        for s in srcs:
+            s.prepare(ds)
+    return srcs
 
 
 def device_target(rg, pos=None, rot=None):
