@@ -212,6 +212,8 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
         tiles += (uint64_t)((t.cam.w + rvs_k::kTile - 1) / rvs_k::kTile) * ((t.cam.h + rvs_k::kTile - 1) / rvs_k::kTile);
         px += (uint64_t)t.cam.w * t.cam.h;
     }
+    if (max_sw > rvs_k::kMaxSourceSide || max_sh > rvs_k::kMaxSourceSide)
+        return fail(RVS_E_UNSUPPORTED, "synthesize: source side above %d", rvs_k::kMaxSourceSide);
     const uint64_t items = (uint64_t)((max_sw - 1 + rvs_k::kBinQX - 1) / rvs_k::kBinQX) *
                            ((max_sh - 1 + rvs_k::kBinQY - 1) / rvs_k::kBinQY) * n_src * n_views;
     if (px >= (1ull << 31) || tiles * n_src >= (1ull << 31) || items >= (1ull << 32) ||
@@ -478,7 +480,8 @@ int rvs_raster(const float* color, const float* depth, const float* uv, int in_w
     if (!color || !depth || !uv || in_w < 2 || in_h < 2 || out_w < 1 || out_h < 1 || !workspace || !out_color ||
         !out_depth || !out_shape || !(rescale > 0.f))
         return fail(RVS_E_INVALID, "rvs_raster: bad argument");
-    if ((uint64_t)out_w * out_h >= (1ull << 31) || (uint64_t)(in_w - 1) * (in_h - 1) >= (1ull << 30))
+    if ((uint64_t)out_w * out_h >= (1ull << 31) || (uint64_t)(in_w - 1) * (in_h - 1) >= (1ull << 30) ||
+        in_w > rvs_k::kMaxSourceSide || in_h > rvs_k::kMaxSourceSide)
         return fail(RVS_E_CAPACITY, "rvs_raster: image too large");
     if (tile_cap == 0) tile_cap = rvs_k::kDefaultTileCap;
     RasterPlan R;

@@ -159,23 +159,46 @@ __device__ inline float3 vertex(const ArrayGeom& g, int i, int j) {
 // are exact; the nvcc contraction is restated as fmaf for the general case.
 __device__ inline float3 color(const PairGeom& g, int i, int j) {
     const int inW = g.sw / 2, inH = g.sh / 2;
-    const float scaleX = (float)inW / g.sw, scaleY = (float)inH / g.sh;
-    const float srcX = j * scaleX, srcY = i * scaleY;
-    const int x1 = (int)srcX, y1 = (int)srcY;
-    const int x2 = (int)fminf((float)(x1 + 1), (float)(inW - 1));
-    const int y2 = (int)fminf((float)(y1 + 1), (float)(inH - 1));
-    const float dx = srcX - x1, dy = srcY - y1;
     float up[2];
     const void* planes[2] = {g.pu, g.pv};
+    if (((g.sw | g.sh) & 1) == 0) {
+        // Even sizes: scale = 1/2 exactly, so srcX = j/2, x1 = j >> 1, dx in {0, 1/2}.
+        // fmaf(a, 1 - dx, b * dx) is then a (dx = 0) or (a + b) / 2, exact for
+        // integer samples < 2^16 (and likewise for the vertical pass): the
+        // same bits as the general formula below, without its divisions.
+        const int x1 = j >> 1, y1 = i >> 1;
+        const int x2 = min(x1 + 1, inW - 1), y2 = min(y1 + 1, inH - 1);
+        const bool hx = (j & 1) != 0, hy = (i & 1) != 0;
 #pragma unroll
-    for (int p = 0; p < 2; ++p) {
-        const float a = (float)load_plane(planes[p], g.color_u8, (size_t)y1 * inW + x1);
-        const float b = (float)load_plane(planes[p], g.color_u8, (size_t)y1 * inW + x2);
-        const float c = (float)load_plane(planes[p], g.color_u8, (size_t)y2 * inW + x1);
-        const float e = (float)load_plane(planes[p], g.color_u8, (size_t)y2 * inW + x2);
-        const float val1 = __builtin_fmaf(a, (1 - dx), b * dx);
-        const float val2 = __builtin_fmaf(c, (1 - dx), e * dx);
-        up[p] = __builtin_fmaf(val1, (1 - dy), val2 * dy);
+        for (int p = 0; p < 2; ++p) {
+            const float a = (float)load_plane(planes[p], g.color_u8, (size_t)y1 * inW + x1);
+            float val1 = a, val2 = 0.f;
+            if (hx) val1 = (a + (float)load_plane(planes[p], g.color_u8, (size_t)y1 * inW + x2)) * 0.5f;
+            if (hy) {
+                const float c = (float)load_plane(planes[p], g.color_u8, (size_t)y2 * inW + x1);
+                val2 = hx ? (c + (float)load_plane(planes[p], g.color_u8, (size_t)y2 * inW + x2)) * 0.5f : c;
+                up[p] = (val1 + val2) * 0.5f;
+            } else {
+                up[p] = val1;
+            }
+        }
+    } else {
+        const float scaleX = (float)inW / g.sw, scaleY = (float)inH / g.sh;
+        const float srcX = j * scaleX, srcY = i * scaleY;
+        const int x1 = (int)srcX, y1 = (int)srcY;
+        const int x2 = (int)fminf((float)(x1 + 1), (float)(inW - 1));
+        const int y2 = (int)fminf((float)(y1 + 1), (float)(inH - 1));
+        const float dx = srcX - x1, dy = srcY - y1;
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const float a = (float)load_plane(planes[p], g.color_u8, (size_t)y1 * inW + x1);
+            const float b = (float)load_plane(planes[p], g.color_u8, (size_t)y1 * inW + x2);
+            const float c = (float)load_plane(planes[p], g.color_u8, (size_t)y2 * inW + x1);
+            const float e = (float)load_plane(planes[p], g.color_u8, (size_t)y2 * inW + x2);
+            const float val1 = __builtin_fmaf(a, (1 - dx), b * dx);
+            const float val2 = __builtin_fmaf(c, (1 - dx), e * dx);
+            up[p] = __builtin_fmaf(val1, (1 - dy), val2 * dy);
+        }
     }
     const float yy = (float)load_plane(g.py, g.color_u8, (size_t)i * g.sw + j);
     return make_float3(yy * g.cscale, up[0] * g.cscale, up[1] * g.cscale);
@@ -209,11 +232,16 @@ struct Tri {
     int xmin, xmax, ymin, ymax;
 };
 
-// Returns false when colorize_triangle returns before its pixel loop.
-__device__ inline bool tri_setup(Tri& T, float3 a, float3 b, float3 c, int out_w, int out_h, float rescale) {
+// Corners, vertex depths and the barycentric denominator (transform.cpp:80-85).
+__device__ inline void tri_load(Tri& T, float3 a, float3 b, float3 c) {
     T.A0 = a.x; T.A1 = a.y; T.B0 = b.x; T.B1 = b.y; T.C0 = c.x; T.C1 = c.y;
     T.dA = a.z; T.dB = b.z; T.dC = c.z;
     T.den = (T.B1 - T.C1) * (T.A0 - T.C0) + (T.C0 - T.B0) * (T.A1 - T.C1);
+}
+
+// Returns false when colorize_triangle returns before its pixel loop.
+__device__ inline bool tri_setup(Tri& T, float3 a, float3 b, float3 c, int out_w, int out_h, float rescale) {
+    tri_load(T, a, b, c);
     if (T.den <= 0.f) return false;
     // valid_tri: stretch = (float) max of three double norms, / g_rescale
     const float dab = norm_sq_sum_d(T.A0 - T.B0, T.A1 - T.B1, 0.f);
@@ -234,15 +262,40 @@ __device__ inline bool tri_setup(Tri& T, float3 a, float3 b, float3 c, int out_w
     return true;
 }
 
-// Barycentric coordinates at pixel (x, y); returns coverage.
-__device__ inline bool tri_pixel(const Tri& T, int x, int y, float& l1, float& l2, float& l3) {
+// Numerators of the barycentric coordinates l1, l2 at pixel (x, y).
+__device__ inline void tri_num(const Tri& T, int x, int y, float& n1, float& n2) {
     const float offset = 0.5f;
     const float px = (float)x + offset, py = (float)y + offset;
-    l1 = ((T.B1 - T.C1) * (px - T.C0) + (T.C0 - T.B0) * (py - T.C1)) / T.den;
-    l2 = ((T.C1 - T.A1) * (px - T.C0) + (T.A0 - T.C0) * (py - T.C1)) / T.den;
+    n1 = (T.B1 - T.C1) * (px - T.C0) + (T.C0 - T.B0) * (py - T.C1);
+    n2 = (T.C1 - T.A1) * (px - T.C0) + (T.A0 - T.C0) * (py - T.C1);
+}
+
+// Coverage from the numerators (transform.cpp:105-117).
+__device__ inline bool tri_cover(const Tri& T, float n1, float n2, float& l1, float& l2, float& l3) {
+    l1 = n1 / T.den;
+    l2 = n2 / T.den;
     l3 = 1.f - l1 - l2;
     const float eps = 1e-6f;
     return l1 >= -eps && l2 >= -eps && l3 >= -eps;
+}
+
+// Barycentric coordinates at pixel (x, y); returns coverage.
+__device__ inline bool tri_pixel(const Tri& T, int x, int y, float& l1, float& l2, float& l3) {
+    float n1, n2;
+    tri_num(T, x, y, n1, n2);
+    return tri_cover(T, n1, n2, l1, l2, l3);
+}
+
+// Division-free rejection of pixels that tri_cover cannot accept (den > 0):
+// n < fl(-2e-6 * den) gives n / den < -1e-6 after rounding (also when the
+// bound is subnormal or 0: then n <= -2^-149 and den < 2^-130), and with
+// n1, n2 >= 0 and a normal den, n1 + n2 > fl(1.00001 * den) gives
+// l1 + l2 > 1 + 9e-6 after rounding, so l3 < -1e-6.  A NaN or infinite den
+// never rejects.
+__device__ inline bool tri_surely_outside(const Tri& T, float n1, float n2) {
+    const float t = -2e-6f * T.den;
+    if (n1 < t || n2 < t) return true;
+    return n1 >= 0.f && n2 >= 0.f && T.den >= 1e-30f && n1 + n2 > T.den * 1.00001f;
 }
 
 __device__ inline float tri_depth(const Tri& T, float l1, float l2, float l3) {
@@ -270,23 +323,25 @@ struct FoldState {
     float new_d;
     float shape;
     int new_prol;
-    uint32_t seq;    // last accepted candidate (0xffffffff = none)
+    uint32_t ij;     // last accepted candidate: packed source quad (row << 16 | column)
+    int slot;        // and its slot (-1 = none accepted)
 };
 
 __device__ inline void fold_init(FoldState& s) {
     s.new_d = __builtin_inff();
     s.shape = 0.f;
     s.new_prol = 1;
-    s.seq = 0xffffffffu;
+    s.ij = 0u;
+    s.slot = -1;
 }
 
-__device__ inline void fold_step(FoldState& s, float d, float tv, bool prol, uint32_t seq) {
+__device__ inline void fold_step(FoldState& s, float d, float tv, bool prol, uint32_t ij, int slot) {
     const float ratio = d / s.new_d;
     const bool is_valid = ratio * ratio * ratio * s.shape < tv;
     if ((is_valid || s.new_prol) && !prol) {
-        s.new_d = d; s.new_prol = 0; s.shape = tv; s.seq = seq;
+        s.new_d = d; s.new_prol = 0; s.shape = tv; s.ij = ij; s.slot = slot;
     } else if (is_valid && s.new_prol && prol) {
-        s.new_d = d; s.shape = tv; s.seq = seq;
+        s.new_d = d; s.shape = tv; s.ij = ij; s.slot = slot;
     }
 }
 

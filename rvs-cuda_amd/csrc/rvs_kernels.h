@@ -25,6 +25,9 @@ constexpr int kFallbackChunk = 128;  // list entries per LDS chunk in the in-ord
 constexpr int kFallbackSortCap = 8192;  // longest (source, tile) list the replay sorts in LDS
 constexpr int kMaxViews = 4096;
 constexpr int kBinBlocks = 2048;    // persistent binning blocks
+constexpr int kRasterGroup = 2;     // sources rasterised together per k_raster phase
+constexpr int kRasterCache = 448;   // binned quads per phase whose coverage-pass results stay in LDS
+constexpr int kMaxSourceSide = 65536;  // bins hold quads as (row << 16) | column
 constexpr int kFallbackBlocks = 1024;
 
 // One target view of a batch.

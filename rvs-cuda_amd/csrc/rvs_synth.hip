@@ -16,14 +16,16 @@
 //   1. k_bin (persistent, pulls work items): every surviving source quad is
 //      appended to the 16x16 target tiles its triangles' (reference-exact)
 //      bounding boxes touch.
-//   2. k_raster: one 256-thread block per target tile, up to four sources per
+//   2. k_raster: one 256-thread block per target tile, two sources per
 //      phase.  Every covered pixel takes two LDS 64-bit atomicMax: key1 = max
-//      of (score bits << 32 | ~seq), key2 = max over the candidates that are
-//      not the maximum (the loser of each key1 update).  A pixel whose
-//      runner-up scores within 2^-18 of the winner, or that saw a candidate
-//      with non-finite or extreme values, is deferred.  Every other pixel
-//      provably ends the sequential fold on the argmax (the float test errs by
-//      less than 2^-21 relative), so its winner is re-evaluated and blended.
+//      of (score bits << 32 | candidate id), key2 = max over the candidates
+//      that are not the maximum (the loser of each key1 update).  A pixel
+//      whose runner-up scores within 2^-18 of the winner (exact ties
+//      included), or that saw a candidate with non-finite or extreme values,
+//      is deferred.  Every other pixel provably ends the sequential fold on
+//      the argmax (the float test errs by less than 2^-21 relative), so its
+//      winner is re-evaluated -- from the corner vertices and validity the
+//      coverage pass left in LDS -- and blended.
 //   3. k_fallback: deferred pixels replay the reference fold literally, in
 //      traversal order, over the tile's bins sorted in LDS.
 // Blending across sources (BlendedViewSimple::blendBasicMode) runs per pixel in
@@ -126,11 +128,11 @@ __device__ inline void quad_build(Quad& Q, const G& g, int i, int j, VF vtx, int
     }
 }
 
-template <class G>
-__device__ inline void quad_of(const G& g, uint32_t q, int& i, int& j) {
-    const uint32_t w1 = (uint32_t)(g.sw - 1);
-    i = (int)(q / w1);
-    j = (int)(q - (uint32_t)i * w1);
+// Bins hold source quads as (row << 16) | column: ascending order of these
+// codes is the reference's traversal order (rows, then columns).
+__device__ inline void unpack_ij(uint32_t ij, int& i, int& j) {
+    i = (int)(ij >> 16);
+    j = (int)(ij & 0xffffu);
 }
 
 // ------------------------------------------------------------ culling
@@ -301,7 +303,7 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
                     }
                 tx0 = xmin / kTile; tx1 = xmax / kTile; ty0 = ymin / kTile; ty1 = ymax / kTile;
                 more = true;
-                q = (uint32_t)i * (uint32_t)(g.sw - 1) + (uint32_t)j;
+                q = ((uint32_t)i << 16) | (uint32_t)j;
             }
         }
         int tx = tx0, ty = ty0;
@@ -343,14 +345,11 @@ hipError_t launch_bin(const G* geoms, const ViewDesc* views, const BatchDims& d,
 
 // ------------------------------------------------------------ tile raster
 
-// Winner's colour, depth and validity at pixel (x, y) (the state the fold
-// holds after accepting candidate `seq`).
+// Winner's colour, depth and validity at pixel (x, y) for slot `slot` of
+// source quad (i, j): the state the fold holds after accepting that candidate.
 template <class G>
-__device__ inline void resolve(const G& g, uint32_t seq, int x, int y, float3& c, float& dep, float& shape,
-                               int out_w, int out_h) {
-    int i, j;
-    quad_of(g, seq >> 2, i, j);
-    const int slot = (int)(seq & 3u);
+__device__ inline void resolve(const G& g, int i, int j, int slot, int x, int y, float3& c, float& dep,
+                               float& shape, int out_w, int out_h) {
     int r[3], cc[3];
     float3 v[3];
 #pragma unroll
@@ -367,25 +366,46 @@ __device__ inline void resolve(const G& g, uint32_t seq, int x, int y, float3& c
     c = tri_color(color(g, r[0], cc[0]), color(g, r[1], cc[1]), color(g, r[2], cc[2]), l1, l2, l3);
 }
 
-// Pass over one triangle of a binned quad for the tile at (tx0, ty0): per
-// covered pixel, push the candidate key into the top-2 keys of the pixel.
-// Candidates with non-finite inputs or extreme depths mark the pixel for the
-// in-order replay instead.
-__device__ inline void raster_tri(float3 a, float3 b, float3 c, uint32_t seq, bool prol_false_possible, int tx0,
-                                  int ty0, int out_w, int out_h, float rescale, unsigned long long* key1,
-                                  unsigned long long* key2, int* flag) {
+// The same from the corner vertices and validity the coverage pass cached
+// (identical float operations, without re-evaluating vertices or norms).
+template <class G>
+__device__ inline void resolve_cached(const G& g, int i, int j, int slot, float3 a, float3 b, float3 cv, float tv,
+                                      int x, int y, float3& c, float& dep, float& shape) {
     Tri T;
-    if (!tri_setup(T, a, b, c, out_w, out_h, rescale)) return;
+    tri_load(T, a, b, cv);
+    T.tv = tv;
+    float l1, l2, l3;
+    tri_pixel(T, x, y, l1, l2, l3);
+    dep = tri_depth(T, l1, l2, l3);
+    shape = tv;
+    int r[3], cc[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) slot_vertex(slot, k, i, j, g.sw, r[k], cc[k]);
+    c = tri_color(color(g, r[0], cc[0]), color(g, r[1], cc[1]), color(g, r[2], cc[2]), l1, l2, l3);
+}
+
+// Pass over one triangle of a binned quad for the tile at (tx0, ty0): per
+// covered pixel, push the candidate key (score bits << 32 | candidate id)
+// into the top-2 keys of the pixel.  Candidates with non-finite inputs or
+// extreme depths mark the pixel for the in-order replay instead.  Returns the
+// triangle's validity (0 when colorize_triangle returns early).
+__device__ inline float raster_tri(float3 a, float3 b, float3 c, uint32_t id, bool prol_false_possible, int tx0,
+                                   int ty0, int out_w, int out_h, float rescale, unsigned long long* key1,
+                                   unsigned long long* key2, int* flag) {
+    Tri T;
+    if (!tri_setup(T, a, b, c, out_w, out_h, rescale)) return 0.f;
     const bool prol = (a.z > 0.f) || (b.z > 0.f) || (c.z > 0.f);
     const bool special = !finite3(a) || !finite3(b) || !finite3(c) || !is_finite(T.den) ||
                          (prol_false_possible && !prol);
     const int xa = max(T.xmin, tx0), xb = min(T.xmax, tx0 + kTile - 1);
     const int ya = max(T.ymin, ty0), yb = min(T.ymax, ty0 + kTile - 1);
-    const unsigned long long lo = (unsigned long long)(~seq);
     for (int y = ya; y <= yb; ++y)
         for (int x = xa; x <= xb; ++x) {
+            float n1, n2;
+            tri_num(T, x, y, n1, n2);
+            if (tri_surely_outside(T, n1, n2)) continue;
             float l1, l2, l3;
-            if (!tri_pixel(T, x, y, l1, l2, l3)) continue;
+            if (!tri_cover(T, n1, n2, l1, l2, l3)) continue;
             const int p = (y - ty0) * kTile + (x - tx0);
             const float dd = tri_depth(T, l1, l2, l3);
             if (special || !depth_regular(dd)) {
@@ -393,30 +413,58 @@ __device__ inline void raster_tri(float3 a, float3 b, float3 c, uint32_t seq, bo
                 continue;
             }
             const float f = T.tv / (dd * dd * dd);
-            const unsigned long long k = ((unsigned long long)f2u(f) << 32) | lo;
+            const unsigned long long k = ((unsigned long long)f2u(f) << 32) | id;
             const unsigned long long old = atomicMax(&key1[p], k);
             // the candidate that is not (or no longer) the maximum goes to the runner-up slot
             atomicMax(&key2[p], old < k ? old : k);
         }
+    return T.tv;
 }
 
+// Per-phase LDS copy of what the coverage pass computed for each binned quad:
+// the four corner vertices (v00, v01, v10, v11; u', v', depth), the validity
+// of triangles T1 and T2 and the quad code.
+struct RasterCache {
+    float v[12][kRasterCache];
+    float tv[2][kRasterCache];
+    uint32_t ij[kRasterCache];
+};
+
+// Candidate ids: (entry << 2) | slot, entry = index of the quad in the phase's
+// concatenated bins.  A non-deferred pixel's winner beats every other
+// candidate by more than the float test's error, so any unique id works; equal
+// scores are deferred to the in-order replay.
 template <class G>
-__device__ inline void raster_quad(const G& g, uint32_t q, int tx0, int ty0, int out_w, int out_h,
-                                   unsigned long long* key1, unsigned long long* key2, int* flag) {
+__device__ inline void raster_quad(const G& g, uint32_t ij, uint32_t e, int tx0, int ty0, int out_w, int out_h,
+                                   unsigned long long* key1, unsigned long long* key2, int* flag, RasterCache* C) {
     int i, j;
-    quad_of(g, q, i, j);
+    unpack_ij(ij, i, j);
     const float3 v00 = vertex(g, i, j), v01 = vertex(g, i, j + 1), v10 = vertex(g, i + 1, j);
     const float3 v11 = vertex(g, i + 1, j + 1);
-    if (!(v01.z > 0.f && v10.z > 0.f && v01.x > 0.f && v10.x > 0.f)) return;
-    const uint32_t s0 = q << 2;
-    if (v00.z > 0.f && v00.x > 0.f)
-        raster_tri(v00, v01, v10, s0, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
-    if (v11.z > 0.f && v11.x > 0.f)
-        raster_tri(v11, v10, v01, s0 | 1u, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
-    if (g.wrap && j == 0) {
-        const float3 l0 = vertex(g, i, g.sw - 1), l1 = vertex(g, i + 1, g.sw - 1);
-        raster_tri(l0, v00, l1, s0 | 2u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
-        raster_tri(v10, l1, v00, s0 | 3u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
+    float tv0 = 0.f, tv1 = 0.f;
+    const uint32_t s0 = e << 2;
+    if (v01.z > 0.f && v10.z > 0.f && v01.x > 0.f && v10.x > 0.f) {
+        if (v00.z > 0.f && v00.x > 0.f)
+            tv0 = raster_tri(v00, v01, v10, s0, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
+        if (v11.z > 0.f && v11.x > 0.f)
+            tv1 = raster_tri(v11, v10, v01, s0 | 1u, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
+        if (g.wrap && j == 0) {
+            const float3 l0 = vertex(g, i, g.sw - 1), l1 = vertex(g, i + 1, g.sw - 1);
+            raster_tri(l0, v00, l1, s0 | 2u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
+            raster_tri(v10, l1, v00, s0 | 3u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
+        }
+    }
+    if (C) {
+        const float3 vs[4] = {v00, v01, v10, v11};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            C->v[3 * k][e] = vs[k].x;
+            C->v[3 * k + 1][e] = vs[k].y;
+            C->v[3 * k + 2][e] = vs[k].z;
+        }
+        C->tv[0][e] = tv0;
+        C->tv[1][e] = tv1;
+        C->ij[e] = ij;
     }
 }
 
@@ -434,15 +482,14 @@ __device__ inline void write_pixel(const ViewDesc& vd, size_t o, const BlendStat
     }
 }
 
-constexpr int kGroup = 4;   // sources rasterised together per block phase
-
 template <class G, int MODE>
 __global__ void __launch_bounds__(256) k_raster(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
                                                 BatchDims d, RasterWS ws) {
-    __shared__ unsigned long long key1[kGroup][kTile * kTile];
-    __shared__ unsigned long long key2[kGroup][kTile * kTile];
-    __shared__ int flag[kGroup][kTile * kTile];
-    __shared__ uint32_t pref[kGroup + 1];
+    __shared__ unsigned long long key1[kRasterGroup][kTile * kTile];
+    __shared__ unsigned long long key2[kRasterGroup][kTile * kTile];
+    __shared__ int flag[kRasterGroup][kTile * kTile];
+    __shared__ uint32_t pref[kRasterGroup + 1];
+    __shared__ RasterCache cache;
     const uint32_t T = tile_of_block(blockIdx.x, d.ntiles);
     if (T >= d.ntiles) return;
     const int v = find_view(views, d.nviews, T);
@@ -457,8 +504,8 @@ __global__ void __launch_bounds__(256) k_raster(const G* __restrict__ geoms, con
     BlendState st;
     float3 rc = make_float3(0.f, 0.f, 0.f);
     float rd = __builtin_inff(), rs = 0.f;
-    for (int s0 = 0; s0 < d.nsrc; s0 += kGroup) {
-        const int gs = min(kGroup, d.nsrc - s0);
+    for (int s0 = 0; s0 < d.nsrc; s0 += kRasterGroup) {
+        const int gs = min(kRasterGroup, d.nsrc - s0);
         for (int k = 0; k < gs; ++k) {
             key1[k][tid] = 0ull;
             key2[k][tid] = 0ull;
@@ -474,11 +521,12 @@ __global__ void __launch_bounds__(256) k_raster(const G* __restrict__ geoms, con
         }
         __syncthreads();
         const uint32_t N = pref[gs];
+        const bool cached = N <= (uint32_t)kRasterCache;
         for (uint32_t e = tid; e < N; e += 256) {
             int k = 0;
             while (k + 1 < gs && e >= pref[k + 1]) ++k;
-            const uint32_t q = ws.lists[((size_t)T * d.nsrc + s0 + k) * d.cap + (e - pref[k])];
-            raster_quad(gv[s0 + k], q, tx0, ty0, vd.w, vd.h, key1[k], key2[k], flag[k]);
+            const uint32_t ij = ws.lists[((size_t)T * d.nsrc + s0 + k) * d.cap + (e - pref[k])];
+            raster_quad(gv[s0 + k], ij, e, tx0, ty0, vd.w, vd.h, key1[k], key2[k], flag[k], cached ? &cache : nullptr);
         }
         __syncthreads();
         if (inside) {
@@ -497,7 +545,26 @@ __global__ void __launch_bounds__(256) k_raster(const G* __restrict__ geoms, con
                 if (amb) { deferred = true; break; }
                 float3 c = make_float3(0.f, 0.f, 0.f);
                 float dep = __builtin_inff(), shape = 0.f;
-                if (kw != 0ull) resolve(gv[s], ~(uint32_t)kw, px, py, c, dep, shape, vd.w, vd.h);
+                if (kw != 0ull) {
+                    const uint32_t id = (uint32_t)kw, e = id >> 2;
+                    const int slot = (int)(id & 3u);
+                    const G& g = gv[s];
+                    if (cached && slot < 2) {
+                        const int ka = slot ? 3 : 0, kb = slot ? 2 : 1, kc = slot ? 1 : 2;
+                        const float3 a = make_float3(cache.v[3 * ka][e], cache.v[3 * ka + 1][e], cache.v[3 * ka + 2][e]);
+                        const float3 b = make_float3(cache.v[3 * kb][e], cache.v[3 * kb + 1][e], cache.v[3 * kb + 2][e]);
+                        const float3 cc = make_float3(cache.v[3 * kc][e], cache.v[3 * kc + 1][e], cache.v[3 * kc + 2][e]);
+                        int i, j;
+                        unpack_ij(cache.ij[e], i, j);
+                        resolve_cached(g, i, j, slot, a, b, cc, cache.tv[slot][e], px, py, c, dep, shape);
+                    } else {
+                        const uint32_t ij = cached ? cache.ij[e]
+                                                   : ws.lists[((size_t)T * d.nsrc + s) * d.cap + (e - pref[k])];
+                        int i, j;
+                        unpack_ij(ij, i, j);
+                        resolve(g, i, j, slot, px, py, c, dep, shape, vd.w, vd.h);
+                    }
+                }
                 if (MODE == MODE_RAW) {
                     rc = c; rd = dep; rs = shape;
                 } else if (s == 0) {
@@ -524,7 +591,7 @@ __global__ void __launch_bounds__(256) k_raster(const G* __restrict__ geoms, con
 
 struct ReplayEntry {
     Tri t[4];
-    uint32_t alive, prol, q;
+    uint32_t alive, prol, q;   // q: quad code (row << 16 | column)
 };
 
 template <class G, int MODE>
@@ -581,7 +648,7 @@ __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, c
                 if (tid < kFallbackChunk && b + tid < n) {
                     const uint32_t q = sorted[b + tid];
                     int i, j;
-                    quad_of(g, q, i, j);
+                    unpack_ij(q, i, j);
                     Quad Q;
                     auto vtx = [&](int ii, int jj2) -> float3 { return vertex(g, ii, jj2); };
                     quad_build(Q, g, i, j, vtx, vd.w, vd.h);
@@ -600,8 +667,7 @@ __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, c
                             if (px < Tr.xmin || px > Tr.xmax || py < Tr.ymin || py > Tr.ymax) continue;
                             float l1, l2, l3;
                             if (!tri_pixel(Tr, px, py, l1, l2, l3)) continue;
-                            fold_step(fs, tri_depth(Tr, l1, l2, l3), Tr.tv, (R.prol >> k) & 1u,
-                                      (R.q << 2) | (uint32_t)k);
+                            fold_step(fs, tri_depth(Tr, l1, l2, l3), Tr.tv, (R.prol >> k) & 1u, R.q, k);
                         }
                     }
                 }
@@ -610,9 +676,11 @@ __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, c
             if (mine) {
                 float3 c = make_float3(0.f, 0.f, 0.f);
                 float dep = __builtin_inff(), shape = 0.f;
-                if (fs.seq != 0xffffffffu) {
+                if (fs.slot >= 0) {
                     float dd, sh;
-                    resolve(g, fs.seq, px, py, c, dd, sh, vd.w, vd.h);
+                    int i, j;
+                    unpack_ij(fs.ij, i, j);
+                    resolve(g, i, j, fs.slot, px, py, c, dd, sh, vd.w, vd.h);
                     dep = fs.new_d;
                     shape = fs.shape;
                 }
