@@ -128,6 +128,27 @@ __device__ inline void quad_build(Quad& Q, const G& g, int i, int j, VF vtx, int
     }
 }
 
+// Calls fn(slot, triangle, prol) for every triangle of source quad (i, j)
+// that gets past the vertex gating and colorize_triangle's early returns, in
+// slot order (the quad_build gating, one triangle at a time).
+template <class G, class F>
+__device__ inline void quad_tris(const G& g, int i, int j, int out_w, int out_h, F fn) {
+    const float3 v00 = vertex(g, i, j), v01 = vertex(g, i, j + 1), v10 = vertex(g, i + 1, j);
+    const float3 v11 = vertex(g, i + 1, j + 1);
+    if (!(v01.z > 0.f && v10.z > 0.f && v01.x > 0.f && v10.x > 0.f)) return;
+    auto one = [&](int slot, float3 a, float3 b, float3 c) {
+        Tri T;
+        if (tri_setup(T, a, b, c, out_w, out_h, g.rescale)) fn(slot, T, (a.z > 0.f) || (b.z > 0.f) || (c.z > 0.f));
+    };
+    if (v00.z > 0.f && v00.x > 0.f) one(0, v00, v01, v10);
+    if (v11.z > 0.f && v11.x > 0.f) one(1, v11, v10, v01);
+    if (g.wrap && j == 0) {
+        const float3 l0 = vertex(g, i, g.sw - 1), l1 = vertex(g, i + 1, g.sw - 1);
+        one(2, l0, v00, l1);
+        one(3, v10, l1, v00);
+    }
+}
+
 // Bins hold source quads as (row << 16) | column: ascending order of these
 // codes is the reference's traversal order (rows, then columns).
 __device__ inline void unpack_ij(uint32_t ij, int& i, int& j) {
@@ -594,11 +615,140 @@ struct ReplayEntry {
     uint32_t alive, prol, q;   // q: quad code (row << 16 | column)
 };
 
+// A candidate (triangle covering a deferred pixel) recorded for the in-order
+// fold: quad code, slot | prol | next-in-list, depth, validity.
+struct Cand {
+    uint32_t ij;
+    uint32_t meta;   // bits 0-15: next record (0xffff = end), 16-17: slot, 18: prol
+    float d, tv;
+};
+constexpr int kCandPool = kFallbackSortCap * 4 / (int)sizeof(Cand);   // records sharing the sort buffer
+static_assert(kCandPool < 0xffff, "candidate links are 16-bit");
+
+// Deferred pixels of one tile and source, in the reference's traversal order.
+// Fast path: the block rasterises the source's bin in parallel and appends
+// every candidate covering a deferred pixel to that pixel's list in LDS; each
+// pixel then folds its own candidates in ascending (quad, slot) order.  When
+// the lists overflow the pool, the bin is sorted in LDS and every deferred
+// pixel scans it in order instead.
+template <class G>
+__device__ void replay_source(const G& g, const uint32_t* L, uint32_t n, int tx0, int ty0, int out_w, int out_h,
+                              bool mine, int px, int py, const uint8_t* dmask, uint32_t* sorted, ReplayEntry* rec,
+                              uint32_t* pool_n, FoldState& fs) {
+    const int tid = threadIdx.x;
+    Cand* pool = reinterpret_cast<Cand*>(sorted);
+    int* head = reinterpret_cast<int*>(rec);
+    head[tid] = -1;
+    if (tid == 0) *pool_n = 0u;
+    __syncthreads();
+    for (uint32_t e = tid; e < n; e += 256) {
+        const uint32_t q = L[e];
+        int i, j;
+        unpack_ij(q, i, j);
+        quad_tris(g, i, j, out_w, out_h, [&](int k, const Tri& Tr, bool prol) {
+            const int xa = max(Tr.xmin, tx0), xb = min(Tr.xmax, tx0 + kTile - 1);
+            const int ya = max(Tr.ymin, ty0), yb = min(Tr.ymax, ty0 + kTile - 1);
+            for (int y = ya; y <= yb; ++y)
+                for (int x = xa; x <= xb; ++x) {
+                    const int p = (y - ty0) * kTile + (x - tx0);
+                    if (!dmask[p]) continue;
+                    float n1, n2;
+                    tri_num(Tr, x, y, n1, n2);
+                    if (tri_surely_outside(Tr, n1, n2)) continue;
+                    float l1, l2, l3;
+                    if (!tri_cover(Tr, n1, n2, l1, l2, l3)) continue;
+                    const uint32_t r = atomicAdd(pool_n, 1u);
+                    if (r >= (uint32_t)kCandPool) continue;
+                    const int nx = atomicExch(&head[p], (int)r);
+                    Cand& C = pool[r];
+                    C.ij = q;
+                    C.meta = (uint32_t)(nx < 0 ? 0xffff : nx) | ((uint32_t)k << 16) | ((prol ? 1u : 0u) << 18);
+                    C.d = tri_depth(Tr, l1, l2, l3);
+                    C.tv = Tr.tv;
+                }
+        });
+    }
+    __syncthreads();
+    fold_init(fs);
+    if (*pool_n <= (uint32_t)kCandPool) {
+        if (mine) {
+            const int p = (py - ty0) * kTile + (px - tx0);
+            bool first = true;
+            uint64_t last = 0;
+            while (true) {
+                uint64_t best = ~0ull;
+                int bi = -1;
+                for (int r = head[p]; r >= 0;) {
+                    const Cand& C = pool[r];
+                    const uint64_t key = ((uint64_t)C.ij << 2) | ((C.meta >> 16) & 3u);
+                    if ((first || key > last) && key < best) { best = key; bi = r; }
+                    const uint32_t nx = C.meta & 0xffffu;
+                    r = nx == 0xffffu ? -1 : (int)nx;
+                }
+                if (bi < 0) break;
+                const Cand& C = pool[bi];
+                fold_step(fs, C.d, C.tv, (C.meta >> 18) & 1u, C.ij, (int)((C.meta >> 16) & 3u));
+                last = best;
+                first = false;
+            }
+        }
+        return;
+    }
+    // overflow: sort the bin (bitonic, ascending quad code = traversal order) and scan it
+    uint32_t np2 = 1;
+    while (np2 < n) np2 <<= 1;
+    for (uint32_t e = tid; e < np2; e += 256) sorted[e] = e < n ? L[e] : 0xffffffffu;
+    __syncthreads();
+    for (uint32_t k = 2; k <= np2; k <<= 1)
+        for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+            for (uint32_t e = tid; e < np2; e += 256) {
+                const uint32_t p = e ^ jj;
+                if (p > e) {
+                    const uint32_t a = sorted[e], b = sorted[p];
+                    const bool asc = (e & k) == 0;
+                    if ((a > b) == asc) { sorted[e] = b; sorted[p] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    for (uint32_t b = 0; b < n; b += kFallbackChunk) {
+        if (tid < kFallbackChunk && b + tid < n) {
+            const uint32_t q = sorted[b + tid];
+            int i, j;
+            unpack_ij(q, i, j);
+            Quad Q;
+            auto vtx = [&](int ii, int jj2) -> float3 { return vertex(g, ii, jj2); };
+            quad_build(Q, g, i, j, vtx, out_w, out_h);
+            ReplayEntry& R = rec[tid];
+            for (int k = 0; k < 4; ++k) R.t[k] = Q.t[k];
+            R.alive = Q.alive; R.prol = Q.prol; R.q = q;
+        }
+        __syncthreads();
+        if (mine) {
+            const uint32_t m = min((uint32_t)kFallbackChunk, n - b);
+            for (uint32_t e = 0; e < m; ++e) {
+                const ReplayEntry& R = rec[e];
+                for (int k = 0; k < 4; ++k) {
+                    if (!(R.alive & (1u << k))) continue;
+                    const Tri& Tr = R.t[k];
+                    if (px < Tr.xmin || px > Tr.xmax || py < Tr.ymin || py > Tr.ymax) continue;
+                    float l1, l2, l3;
+                    if (!tri_pixel(Tr, px, py, l1, l2, l3)) continue;
+                    fold_step(fs, tri_depth(Tr, l1, l2, l3), Tr.tv, (R.prol >> k) & 1u, R.q, k);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 template <class G, int MODE>
 __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
                                                   BatchDims d, RasterWS ws) {
     __shared__ uint32_t sorted[kFallbackSortCap];
     __shared__ ReplayEntry rec[kFallbackChunk];
+    __shared__ uint8_t dmask[kTile * kTile];
+    __shared__ uint32_t pool_n;
     const int tid = threadIdx.x;
     const uint32_t ndef = ws.status[0];
     for (uint32_t ti = blockIdx.x; ti < ndef; ti += gridDim.x) {
@@ -612,6 +762,7 @@ __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, c
         const bool inside = px < vd.w && py < vd.h;
         const size_t o = inside ? (size_t)py * vd.w + px : 0;
         const bool mine = inside && ws.deferred[vd.px_base + o];
+        dmask[tid] = mine ? 1 : 0;
         BlendState st;
         float3 rc = make_float3(0.f, 0.f, 0.f);
         float rd = __builtin_inff(), rs = 0.f;
@@ -619,60 +770,15 @@ __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, c
             const G& g = gv[s];
             const size_t bin = (size_t)T * d.nsrc + s;
             const uint32_t n = min(ws.counts[bin], d.cap);
-            const uint32_t* L = ws.lists + bin * d.cap;
             if (n > (uint32_t)kFallbackSortCap) {
                 if (tid == 0) atomicMax(&ws.status[3], n);
                 __syncthreads();
                 continue;
             }
-            uint32_t np2 = 1;
-            while (np2 < n) np2 <<= 1;
-            for (uint32_t e = tid; e < np2; e += 256) sorted[e] = e < n ? L[e] : 0xffffffffu;
-            __syncthreads();
-            // bitonic sort ascending (quad index order = traversal order)
-            for (uint32_t k = 2; k <= np2; k <<= 1)
-                for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
-                    for (uint32_t e = tid; e < np2; e += 256) {
-                        const uint32_t p = e ^ jj;
-                        if (p > e) {
-                            const uint32_t a = sorted[e], b = sorted[p];
-                            const bool asc = (e & k) == 0;
-                            if ((a > b) == asc) { sorted[e] = b; sorted[p] = a; }
-                        }
-                    }
-                    __syncthreads();
-                }
             FoldState fs;
-            fold_init(fs);
-            for (uint32_t b = 0; b < n; b += kFallbackChunk) {
-                if (tid < kFallbackChunk && b + tid < n) {
-                    const uint32_t q = sorted[b + tid];
-                    int i, j;
-                    unpack_ij(q, i, j);
-                    Quad Q;
-                    auto vtx = [&](int ii, int jj2) -> float3 { return vertex(g, ii, jj2); };
-                    quad_build(Q, g, i, j, vtx, vd.w, vd.h);
-                    ReplayEntry& R = rec[tid];
-                    for (int k = 0; k < 4; ++k) R.t[k] = Q.t[k];
-                    R.alive = Q.alive; R.prol = Q.prol; R.q = q;
-                }
-                __syncthreads();
-                if (mine) {
-                    const uint32_t m = min((uint32_t)kFallbackChunk, n - b);
-                    for (uint32_t e = 0; e < m; ++e) {
-                        const ReplayEntry& R = rec[e];
-                        for (int k = 0; k < 4; ++k) {
-                            if (!(R.alive & (1u << k))) continue;
-                            const Tri& Tr = R.t[k];
-                            if (px < Tr.xmin || px > Tr.xmax || py < Tr.ymin || py > Tr.ymax) continue;
-                            float l1, l2, l3;
-                            if (!tri_pixel(Tr, px, py, l1, l2, l3)) continue;
-                            fold_step(fs, tri_depth(Tr, l1, l2, l3), Tr.tv, (R.prol >> k) & 1u, R.q, k);
-                        }
-                    }
-                }
-                __syncthreads();
-            }
+            replay_source(g, ws.lists + bin * d.cap, n, tx0, ty0, vd.w, vd.h, mine, px, py, dmask, sorted, rec,
+                          &pool_n, fs);
+            __syncthreads();
             if (mine) {
                 float3 c = make_float3(0.f, 0.f, 0.f);
                 float dep = __builtin_inff(), shape = 0.f;
