@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--budget-gib", type=float, default=32.0, help="serving memory budget (HBM)")
     ap.add_argument("--tune-batches", default="4,8,16,32")
     ap.add_argument("--tune-slots", default="1,2,3")
+    ap.add_argument("--roofline-only", type=int, default=0,
+                    help="only run the roofline launch set (this many views, tile_cap 512) 5 times (counter passes)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes 5 times (counter calibration)")
     args = ap.parse_args()
@@ -118,6 +120,15 @@ def main():
     tgt = W.device_target(rg)
     (tw, th) = (rg["target"]["cam"].w, rg["target"]["cam"].h)
     torch.cuda.synchronize()
+    if args.roofline_only > 0:
+        walk = W.batch_targets(rg, args.roofline_only, seed=4, sizes=((tw, th),))
+        bs = R.BatchSynthesizer(srcs, walk, tile_cap=512)
+        for _ in range(5):
+            bs.run()
+        torch.cuda.synchronize()
+        print(json.dumps({"roofline_only_views": args.roofline_only,
+                          "alg_bytes": b_alg(rg, walk)}), flush=True)
+        return
 
     # ---------------------------------------------------------- plain per-view path
     syn = R.Synthesizer(srcs, tgt, want_maps=False)

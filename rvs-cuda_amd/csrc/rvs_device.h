@@ -239,11 +239,11 @@ __device__ inline void tri_load(Tri& T, float3 a, float3 b, float3 c) {
     T.den = (T.B1 - T.C1) * (T.A0 - T.C0) + (T.C0 - T.B0) * (T.A1 - T.C1);
 }
 
-// Returns false when colorize_triangle returns before its pixel loop.
-__device__ inline bool tri_setup(Tri& T, float3 a, float3 b, float3 c, int out_w, int out_h, float rescale) {
-    tri_load(T, a, b, c);
-    if (T.den <= 0.f) return false;
-    // valid_tri: stretch = (float) max of three double norms, / g_rescale
+// valid_tri (transform.cpp:62-75): stretch = (float) max of three double
+// norms, / g_rescale; validity = clamp(10000 - 1000 * stretch, 1, 10000).  The
+// clamp's max(1, q) maps NaN to 1, so the result is never 0 and
+// colorize_triangle's `validity == 0` return is never taken.
+__device__ inline float tri_validity(const Tri& T, float rescale) {
     const float dab = norm_sq_sum_d(T.A0 - T.B0, T.A1 - T.B1, 0.f);
     const float dac = norm_sq_sum_d(T.A0 - T.C0, T.A1 - T.C1, 0.f);
     const float dbc = norm_sq_sum_d(T.B0 - T.C0, T.B1 - T.C1, 0.f);
@@ -252,13 +252,26 @@ __device__ inline bool tri_setup(Tri& T, float3 a, float3 b, float3 c, int out_w
     float quality = 10000.f - 1000.f * stretch;
     quality = smax(1.f, quality);
     quality = smin(10000.f, quality);
-    T.tv = quality;
-    if (T.tv == 0.f) return false;
+    return quality;
+}
+
+// colorize_triangle's early returns (transform.cpp:80-96) without the
+// validity: den <= 0, empty clipped bounding box.  T.tv is left 0.
+__device__ inline bool tri_setup_geom(Tri& T, float3 a, float3 b, float3 c, int out_w, int out_h) {
+    tri_load(T, a, b, c);
+    T.tv = 0.f;
+    if (T.den <= 0.f) return false;
     T.xmin = imax(0, cvt_i32(__builtin_floorf(smin(smin(T.A0, T.B0), T.C0))));
     T.ymin = imax(0, cvt_i32(__builtin_floorf(smin(smin(T.A1, T.B1), T.C1))));
     T.xmax = imin(out_w - 1, cvt_i32(__builtin_ceilf(smax(smax(T.A0, T.B0), T.C0))));
     T.ymax = imin(out_h - 1, cvt_i32(__builtin_ceilf(smax(smax(T.A1, T.B1), T.C1))));
-    if (T.ymin > T.ymax || T.xmin > T.xmax) return false;
+    return !(T.ymin > T.ymax || T.xmin > T.xmax);
+}
+
+// Returns false when colorize_triangle returns before its pixel loop.
+__device__ inline bool tri_setup(Tri& T, float3 a, float3 b, float3 c, int out_w, int out_h, float rescale) {
+    if (!tri_setup_geom(T, a, b, c, out_w, out_h)) return false;
+    T.tv = tri_validity(T, rescale);
     return true;
 }
 

@@ -105,26 +105,30 @@ struct Quad {
 
 __device__ inline bool finite3(float3 v) { return is_finite(v.x) && is_finite(v.y) && is_finite(v.z); }
 
-__device__ inline void quad_add(Quad& Q, int slot, float3 a, float3 b, float3 c, int out_w, int out_h, float rescale) {
+__device__ inline void quad_add(Quad& Q, int slot, float3 a, float3 b, float3 c, int out_w, int out_h, float rescale,
+                                bool validity) {
     Tri T;
-    const bool ok = tri_setup(T, a, b, c, out_w, out_h, rescale);
+    const bool ok = validity ? tri_setup(T, a, b, c, out_w, out_h, rescale) : tri_setup_geom(T, a, b, c, out_w, out_h);
     Q.t[slot] = T;
     if (!ok) return;
     Q.alive |= 1u << slot;
     if ((a.z > 0.f) || (b.z > 0.f) || (c.z > 0.f)) Q.prol |= 1u << slot;
 }
 
+// validity = false leaves t[k].tv at 0 (binning needs only which triangles
+// survive and their bounding boxes).
 template <class G, class VF>
-__device__ inline void quad_build(Quad& Q, const G& g, int i, int j, VF vtx, int out_w, int out_h) {
+__device__ inline void quad_build(Quad& Q, const G& g, int i, int j, VF vtx, int out_w, int out_h,
+                                  bool validity = true) {
     Q.alive = 0; Q.prol = 0;
     const float3 v00 = vtx(i, j), v01 = vtx(i, j + 1), v10 = vtx(i + 1, j), v11 = vtx(i + 1, j + 1);
     if (!(v01.z > 0.f && v10.z > 0.f && v01.x > 0.f && v10.x > 0.f)) return;
-    if (v00.z > 0.f && v00.x > 0.f) quad_add(Q, 0, v00, v01, v10, out_w, out_h, g.rescale);
-    if (v11.z > 0.f && v11.x > 0.f) quad_add(Q, 1, v11, v10, v01, out_w, out_h, g.rescale);
+    if (v00.z > 0.f && v00.x > 0.f) quad_add(Q, 0, v00, v01, v10, out_w, out_h, g.rescale, validity);
+    if (v11.z > 0.f && v11.x > 0.f) quad_add(Q, 1, v11, v10, v01, out_w, out_h, g.rescale, validity);
     if (g.wrap && j == 0) {
         const float3 l0 = vtx(i, g.sw - 1), l1 = vtx(i + 1, g.sw - 1);
-        quad_add(Q, 2, l0, v00, l1, out_w, out_h, g.rescale);
-        quad_add(Q, 3, v10, l1, v00, out_w, out_h, g.rescale);
+        quad_add(Q, 2, l0, v00, l1, out_w, out_h, g.rescale, validity);
+        quad_add(Q, 3, v10, l1, v00, out_w, out_h, g.rescale, validity);
     }
 }
 
@@ -314,7 +318,7 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
                 if (r >= 0 && r <= kBinQY && c >= 0 && c <= kBinQX) return V[r][c];
                 return vertex(g, ii, jj);
             };
-            quad_build(Q, g, i, j, vtx, vd.w, vd.h);
+            quad_build(Q, g, i, j, vtx, vd.w, vd.h, false);
             if (Q.alive) {
                 int xmin = INT_MAX, xmax = INT_MIN, ymin = INT_MAX, ymax = INT_MIN;
                 for (int k = 0; k < 4; ++k)
@@ -409,17 +413,21 @@ __device__ inline void resolve_cached(const G& g, int i, int j, int slot, float3
 // covered pixel, push the candidate key (score bits << 32 | candidate id)
 // into the top-2 keys of the pixel.  Candidates with non-finite inputs or
 // extreme depths mark the pixel for the in-order replay instead.  Returns the
-// triangle's validity (0 when colorize_triangle returns early).
+// triangle's validity, or 0 when the triangle does not reach the tile (then it
+// is not evaluated; the reference computes it before the bounding box test,
+// and it never causes a return, so skipping it changes nothing).
 __device__ inline float raster_tri(float3 a, float3 b, float3 c, uint32_t id, bool prol_false_possible, int tx0,
                                    int ty0, int out_w, int out_h, float rescale, unsigned long long* key1,
                                    unsigned long long* key2, int* flag) {
     Tri T;
-    if (!tri_setup(T, a, b, c, out_w, out_h, rescale)) return 0.f;
+    if (!tri_setup_geom(T, a, b, c, out_w, out_h)) return 0.f;
+    const int xa = max(T.xmin, tx0), xb = min(T.xmax, tx0 + kTile - 1);
+    const int ya = max(T.ymin, ty0), yb = min(T.ymax, ty0 + kTile - 1);
+    if (xa > xb || ya > yb) return 0.f;   // misses this tile: its validity is not needed here
+    T.tv = tri_validity(T, rescale);
     const bool prol = (a.z > 0.f) || (b.z > 0.f) || (c.z > 0.f);
     const bool special = !finite3(a) || !finite3(b) || !finite3(c) || !is_finite(T.den) ||
                          (prol_false_possible && !prol);
-    const int xa = max(T.xmin, tx0), xb = min(T.xmax, tx0 + kTile - 1);
-    const int ya = max(T.ymin, ty0), yb = min(T.ymax, ty0 + kTile - 1);
     for (int y = ya; y <= yb; ++y)
         for (int x = xa; x <= xb; ++x) {
             float n1, n2;
