@@ -16,6 +16,8 @@ Prints one JSON line:
                 (arrival to host copy complete).
   plain         the plain per-view path: one view per launch set, synchronous
                 (host call to completion), sources resident.
+  plain_stagewise  the same view through the per-stage entry points, in the
+                reference's per-stage call order (intermediate maps in HBM).
   open_loop     the same server under Poisson arrivals at 80% of `value`.
   batched       one launch set of `batch` views.
   roofline      the dominant kernel (k_raster, tile raster + blend) in the
@@ -80,6 +82,7 @@ def main():
     ap.add_argument("--steps", type=int, default=512, help="served requests timed")
     ap.add_argument("--warmup", type=int, default=64, help="served requests before the timed run")
     ap.add_argument("--plain-steps", type=int, default=30)
+    ap.add_argument("--stagewise-steps", type=int, default=10)
     ap.add_argument("--workload", default="erp4_1080p")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=5)
@@ -126,8 +129,9 @@ def main():
         for _ in range(5):
             bs.run()
         torch.cuda.synchronize()
-        print(json.dumps({"roofline_only_views": args.roofline_only,
-                          "alg_bytes": b_alg(rg, walk)}), flush=True)
+        st, cnt = stage_profile(bs.run, 3, R)
+        print(json.dumps({"roofline_only_views": args.roofline_only, "alg_bytes": b_alg(rg, walk),
+                          "stage_ms": st, "counters": cnt}), flush=True)
         return
 
     # ---------------------------------------------------------- plain per-view path
@@ -145,6 +149,17 @@ def main():
     plain_views_s = args.plain_steps / plain_s
     plain_stage, plain_counters = stage_profile(syn.run, args.profile_reps, R)
     log("plain %.1f views/s" % plain_views_s)
+    # the same per-view path with the reference's per-stage call structure
+    sw = R.StagewiseSynthesizer(srcs, tgt)
+    sw.run()
+    torch.cuda.synchronize()
+    t_a = time.perf_counter()
+    for _ in range(args.stagewise_steps):
+        sw.run()
+    torch.cuda.synchronize()
+    stagewise_views_s = args.stagewise_steps / (time.perf_counter() - t_a)
+    del sw
+    log("plain stagewise %.1f views/s" % stagewise_views_s)
 
     # --------------------------------------------------------------- autotuning
     walk = W.batch_targets(rg, max(args.steps, 256), seed=4, sizes=((tw, th),))
@@ -212,6 +227,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "vs_plain": round(served["views_s"] / plain_views_s, 2),
+        "vs_plain_stagewise": round(served["views_s"] / stagewise_views_s, 2),
         "dtype": "f32",
         "data": "synthetic",
         "config": {
@@ -230,6 +246,11 @@ def main():
             "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
             "stage_ms": {k: round(v, 4) for k, v in plain_stage.items()},
             "counters": plain_counters,
+        },
+        "plain_stagewise": {
+            "views_s": round(stagewise_views_s, 2),
+            "what": "per view and source: rvs_reproject, rvs_raster (synchronous), rvs_blend_step; then "
+                    "rvs_inpaint, rvs_encode_yuv420 -- the reference's per-stage call structure",
         },
         "batched": batched,
         "autotune": {"trials": trials},

@@ -428,7 +428,7 @@ __device__ inline float raster_tri(float3 a, float3 b, float3 c, uint32_t id, bo
     const bool prol = (a.z > 0.f) || (b.z > 0.f) || (c.z > 0.f);
     const bool special = !finite3(a) || !finite3(b) || !finite3(c) || !is_finite(T.den) ||
                          (prol_false_possible && !prol);
-    for (int y = ya; y <= yb; ++y)
+    for (int y = ya; y <= yb; ++y) {
         for (int x = xa; x <= xb; ++x) {
             float n1, n2;
             tri_num(T, x, y, n1, n2);
@@ -447,6 +447,7 @@ __device__ inline float raster_tri(float3 a, float3 b, float3 c, uint32_t id, bo
             // the candidate that is not (or no longer) the maximum goes to the runner-up slot
             atomicMax(&key2[p], old < k ? old : k);
         }
+    }
     return T.tv;
 }
 
@@ -468,29 +469,36 @@ __device__ inline void raster_quad(const G& g, uint32_t ij, uint32_t e, int tx0,
                                    unsigned long long* key1, unsigned long long* key2, int* flag, RasterCache* C) {
     int i, j;
     unpack_ij(ij, i, j);
-    const float3 v00 = vertex(g, i, j), v01 = vertex(g, i, j + 1), v10 = vertex(g, i + 1, j);
-    const float3 v11 = vertex(g, i + 1, j + 1);
-    float tv0 = 0.f, tv1 = 0.f;
+    // vertices are evaluated (and cached) just before their triangles need
+    // them; keys commute, so only the set of candidates matters here
+    auto keep = [&](int k, float3 v) {
+        if (C) { C->v[3 * k][e] = v.x; C->v[3 * k + 1][e] = v.y; C->v[3 * k + 2][e] = v.z; }
+    };
+    const float3 v01 = vertex(g, i, j + 1), v10 = vertex(g, i + 1, j);
+    keep(1, v01);
+    keep(2, v10);
+    const bool gate = v01.z > 0.f && v10.z > 0.f && v01.x > 0.f && v10.x > 0.f;
     const uint32_t s0 = e << 2;
-    if (v01.z > 0.f && v10.z > 0.f && v01.x > 0.f && v10.x > 0.f) {
-        if (v00.z > 0.f && v00.x > 0.f)
+    float tv0 = 0.f, tv1 = 0.f;
+    {
+        const float3 v00 = vertex(g, i, j);
+        keep(0, v00);
+        if (gate && v00.z > 0.f && v00.x > 0.f)
             tv0 = raster_tri(v00, v01, v10, s0, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
-        if (v11.z > 0.f && v11.x > 0.f)
+    }
+    {
+        const float3 v11 = vertex(g, i + 1, j + 1);
+        keep(3, v11);
+        if (gate && v11.z > 0.f && v11.x > 0.f)
             tv1 = raster_tri(v11, v10, v01, s0 | 1u, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
-        if (g.wrap && j == 0) {
-            const float3 l0 = vertex(g, i, g.sw - 1), l1 = vertex(g, i + 1, g.sw - 1);
-            raster_tri(l0, v00, l1, s0 | 2u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
-            raster_tri(v10, l1, v00, s0 | 3u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
-        }
+    }
+    if (gate && g.wrap && j == 0) {
+        const float3 v00 = vertex(g, i, j);
+        const float3 l0 = vertex(g, i, g.sw - 1), l1 = vertex(g, i + 1, g.sw - 1);
+        raster_tri(l0, v00, l1, s0 | 2u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
+        raster_tri(v10, l1, v00, s0 | 3u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
     }
     if (C) {
-        const float3 vs[4] = {v00, v01, v10, v11};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            C->v[3 * k][e] = vs[k].x;
-            C->v[3 * k + 1][e] = vs[k].y;
-            C->v[3 * k + 2][e] = vs[k].z;
-        }
         C->tv[0][e] = tv0;
         C->tv[1][e] = tv1;
         C->ij[e] = ij;

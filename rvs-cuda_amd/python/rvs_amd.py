@@ -506,3 +506,58 @@ class BatchSynthesizer:
             return int(needed.value)
         check(rc)
         return 0
+
+
+class StagewiseSynthesizer:
+    """The plain per-view path with the reference's per-stage call structure
+    (Pipeline::computeView over rvs_cuda_lib's entry points, Pipeline.cpp:169-301):
+    input views decoded once at load; per view and source: reproject, raster
+    (synchronous status check), blend; then inpaint and encode.  Same results
+    as Synthesizer."""
+
+    def __init__(self, sources, target, device="cuda"):
+        import torch
+        self.sources = sources
+        self.target = target
+        self.device = device
+        self.dec = []
+        for s in sources:
+            c = s.cam
+            col = torch.empty((c.h, c.w, 3), dtype=torch.float32, device=device)
+            dep = torch.empty((c.h, c.w), dtype=torch.float32, device=device)
+            y, u, v, d = s.planes
+            decode_yuv420(y, u, v, s.bd_color, target.color_scale, col)
+            decode_depth(d, s.bd_depth, target.depth_scale, s.znear, s.zfar, s.has_invalid, dep)
+            uv = torch.empty((c.h, c.w, 2), dtype=torch.float32, device=device)
+            vd = torch.empty((c.h, c.w), dtype=torch.float32, device=device)
+            self.dec.append((col, dep, uv, vd))
+        w, h = target.cam.w, target.cam.h
+        f32 = dict(dtype=torch.float32, device=device)
+        self.rc = torch.empty((h, w, 3), **f32)
+        self.rd = torch.empty((h, w), **f32)
+        self.rs = torch.empty((h, w), **f32)
+        self.acc = {"color": torch.empty((h, w, 3), **f32), "quality": torch.empty((h, w), **f32),
+                    "depth": torch.empty((h, w), **f32), "validity": torch.empty((h, w), **f32),
+                    "mask": torch.empty((h, w), dtype=torch.uint8, device=device)}
+        dt = torch.uint8 if target.bitdepth <= 8 else torch.int16
+        self.out = {"y": torch.empty((h, w), dtype=dt, device=device),
+                    "u": torch.empty((h // 2, w // 2), dtype=dt, device=device),
+                    "v": torch.empty((h // 2, w // 2), dtype=dt, device=device)}
+        self.rws = None
+        self.iws = None
+
+    def run(self, stream=None):
+        t = self.target
+        w, h = t.cam.w, t.cam.h
+        wrap = t.cam.proj == 1 and t.cam.full_hor
+        for k, s in enumerate(self.sources):
+            col, dep, uv, vd = self.dec[k]
+            Rm, tv = relative_pose(s.pos, s.rot, list(t.pos), list(t.rot))
+            reproject(s.cam, t.cam, Rm, tv, dep, s.table, t.rescale, uv, vd, stream)
+            self.rws = raster(col, vd, uv, w, h, wrap, t.rescale, self.rc, self.rd, self.rs, workspace=self.rws,
+                              stream=stream)
+            blend_step(self.acc, self.rc, self.rd, self.rs, t.blending_exp, k == 0, stream)
+        self.iws = inpaint(self.acc["color"], self.acc["quality"], workspace=self.iws, stream=stream)
+        encode_yuv420(self.acc["color"], t.bitdepth, self.out["y"], self.out["u"], self.out["v"], stream)
+        return dict(self.out, color=self.acc["color"], quality=self.acc["quality"], depth=self.acc["depth"],
+                    validity=self.acc["validity"])

@@ -36,6 +36,21 @@ def rig(name, scale=1):
         tgt = dict(cam=R.cam(0, tw, th, f=(f, f), p=(tw / 2, th / 2)), pos=[0.0, 0.0, 0.0], rot=[0.0, 0.0, 0.0],
                    bitdepth=10, blending_exp=1.0)
         return dict(kind=1, seed=2, sources=srcs, target=tgt)
+    if name == "persp11_2048":
+        # 11 perspective 2048x2048 10-bit views on a 3 x 4 grid (one slot unused),
+        # 0.1 m pitch, f = 1200 px, principal point at the centre, facing +x
+        sz, f = 2048 // scale, 1200.0 / scale
+        srcs = []
+        for r in range(3):
+            for c in range(4):
+                if r == 2 and c == 3:
+                    continue
+                srcs.append(dict(cam=R.cam(0, sz, sz, f=(f, f), p=(sz / 2, sz / 2)),
+                                 pos=[0.0, (c - 1.5) * 0.1, (1 - r) * 0.1], rot=[0.0, 0.0, 0.0], bd_color=10,
+                                 bd_depth=16, znear=0.5, zfar=25.0))
+        tgt = dict(cam=perspective_cam(1920 // scale, 1080 // scale), pos=[0.0, 0.0, 0.0], rot=[0.0, 0.0, 0.0],
+                   bitdepth=10, blending_exp=1.0)
+        return dict(kind=1, seed=4, sources=srcs, target=tgt)
     raise KeyError(name)
 
 
@@ -132,3 +147,28 @@ def batch_targets(rg, n, seed=3, scale=1, sizes=BATCH_SIZES):
         res.append(R.make_target(c, pos, rot, t["bitdepth"], t["blending_exp"], R.scale_of(s0["bd_color"]),
                                  R.scale_of(s0["bd_depth"])))
     return res
+
+
+# Serving mix: 720p, 1080p, 1440p and 4K perspective (90 degree HFOV) and
+# 2048x1024 equirectangular targets.
+SERVE_SIZES = ((1280, 720, 0), (1920, 1080, 0), (2560, 1440, 0), (3840, 2160, 0), (2048, 1024, 1))
+
+
+def serve_requests(rigs, n, seed=4, scale=1, sizes=SERVE_SIZES):
+    """n requests (scene index, RvsTarget) over the given rigs: scene, size and
+    pose drawn from a seeded generator; poses stay inside each rig's bounds
+    (+-0.1 m and yaw +-20 degrees, pitch +-8 degrees around the rig centre)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        k = int(rng.integers(len(rigs)))
+        rg = rigs[k]
+        t = rg["target"]
+        s0 = rg["sources"][0]
+        w, h, proj = sizes[int(rng.integers(len(sizes)))]
+        c = R.cam(1, w // scale, h // scale) if proj == 1 else perspective_cam(w // scale, h // scale)
+        pos = [float(np.float32(x)) for x in rng.uniform(-0.1, 0.1, 3)]
+        rot = [float(np.float32(rng.uniform(-20.0, 20.0))), float(np.float32(rng.uniform(-8.0, 8.0))), 0.0]
+        out.append((k, R.make_target(c, pos, rot, t["bitdepth"], t["blending_exp"], R.scale_of(s0["bd_color"]),
+                                     R.scale_of(s0["bd_depth"]))))
+    return out

@@ -160,3 +160,71 @@ def test_batch_graph_replay(torch_cuda):
             for k in MAPS:
                 assert torch.equal(o[k], eager[v][k]), "replay %d view %d map %s" % (rep, v, k)
         del junk
+
+
+@pytest.mark.parametrize("name,scale", [("erp4_1080p", 4), ("rects_640", 2)])
+def test_stagewise_equals_fused(torch_cuda, name, scale):
+    """The per-stage call structure (reproject, raster, blend per source, then
+    inpaint, encode) gives the fused path's bits."""
+    torch = torch_cuda
+    say("case stagewise vs fused %s/%d" % (name, scale))
+    rg = W.rig(name, scale=scale)
+    planes = W.render_sources(rg)
+    srcs = W.device_sources(rg, planes)
+    tgt = W.device_target(rg, [0.02, -0.01, 0.0], [4.0, -2.0, 0.0])
+    fused = R.Synthesizer(srcs, tgt, want_maps=True).run()
+    torch.cuda.synchronize()
+    fused = {k: v.clone() for k, v in fused.items() if v is not None}
+    stage = R.StagewiseSynthesizer(srcs, tgt).run()
+    torch.cuda.synchronize()
+    for key in ("color", "quality", "depth", "validity", "y", "u", "v"):
+        assert torch.equal(stage[key], fused[key]), key
+
+
+def _planes_within(out, ref, bitdepth, tag, frac=0.995, db=48.0):
+    """North-star colour tolerance: within 1 code on >= 99.5% of samples and
+    >= 48 dB PSNR, per plane."""
+    npdt = np.uint8 if bitdepth <= 8 else np.uint16
+    maxv = (1 << bitdepth) - 1
+    for k in ("y", "u", "v"):
+        a = out[k].cpu().numpy().view(npdt).astype(np.int64)
+        b = ref[k].astype(np.int64)
+        close = float(np.mean(np.abs(a - b) <= 1))
+        p = W.psnr(a, b, maxv)
+        say("  %s plane %s: within 1 code %.5f, PSNR %.1f dB" % (tag, k, close, p))
+        assert close >= frac and p >= db, "%s plane %s" % (tag, k)
+
+
+def test_erp_target_within_tolerance(torch_cuda):
+    """ERP targets project with the device atan2f/asinf (the host oracle uses
+    libm): colour planes within the stated tolerance, and a batch mixing ERP
+    and perspective targets equals each view synthesized alone."""
+    torch = torch_cuda
+    say("case erp target")
+    rg = W.rig("erp4_1080p", scale=4)
+    planes = W.render_sources(rg)
+    srcs = W.device_sources(rg, planes)
+    erp = R.make_target(R.cam(1, 512, 256), [0.05, 0.02, 0.0], [20.0, 0.0, 0.0], 10, 1.0,
+                        R.scale_of(10), R.scale_of(16))
+    persp = W.batch_targets(rg, 2, seed=9, scale=4)
+    outs = R.BatchSynthesizer(srcs, [erp] + persp).run()
+    torch.cuda.synchronize()
+    ref = _oracle(rg, planes, erp)
+    _planes_within(outs[0], ref, 10, "erp 512x256")
+    for k, t in enumerate([erp] + persp):
+        alone = _alone(srcs, t)
+        for key in MAPS:
+            assert torch.equal(outs[k][key], alone[key]), "view %d map %s" % (k, key)
+
+
+def test_perspective_grid_sources(torch_cuda):
+    """Eleven perspective sources (the serving rig's second scene) -> one view, exact."""
+    torch = torch_cuda
+    say("case persp11 / 4")
+    rg = W.rig("persp11_2048", scale=4)
+    planes = W.render_sources(rg)
+    srcs = W.device_sources(rg, planes)
+    tgt = W.device_target(rg, [0.01, 0.03, -0.02], [5.0, 2.0, 0.0])
+    out = R.Synthesizer(srcs, tgt, want_maps=False).run()
+    torch.cuda.synchronize()
+    _check_oracle(out, _oracle(rg, planes, tgt), tgt.bitdepth, "persp11")
