@@ -413,7 +413,7 @@ __device__ inline void resolve_cached(const G& g, int i, int j, int slot, float3
 // covered pixel, push the candidate key (score bits << 32 | candidate id)
 // into the top-2 keys of the pixel.  Candidates with non-finite inputs or
 // extreme depths mark the pixel for the in-order replay instead.  Returns the
-// triangle's validity, or 0 when the triangle does not reach the tile (then it
+// triangle's validity, or 0 when no pixel of the tile can be covered (then it
 // is not evaluated; the reference computes it before the bounding box test,
 // and it never causes a return, so skipping it changes nothing).
 __device__ inline float raster_tri(float3 a, float3 b, float3 c, uint32_t id, bool prol_false_possible, int tx0,
@@ -421,13 +421,14 @@ __device__ inline float raster_tri(float3 a, float3 b, float3 c, uint32_t id, bo
                                    unsigned long long* key2, int* flag) {
     Tri T;
     if (!tri_setup_geom(T, a, b, c, out_w, out_h)) return 0.f;
-    const int xa = max(T.xmin, tx0), xb = min(T.xmax, tx0 + kTile - 1);
-    const int ya = max(T.ymin, ty0), yb = min(T.ymax, ty0 + kTile - 1);
-    if (xa > xb || ya > yb) return 0.f;   // misses this tile: its validity is not needed here
-    T.tv = tri_validity(T, rescale);
     const bool prol = (a.z > 0.f) || (b.z > 0.f) || (c.z > 0.f);
     const bool special = !finite3(a) || !finite3(b) || !finite3(c) || !is_finite(T.den) ||
                          (prol_false_possible && !prol);
+    int xa = max(T.xmin, tx0), xb = min(T.xmax, tx0 + kTile - 1);
+    int ya = max(T.ymin, ty0), yb = min(T.ymax, ty0 + kTile - 1);
+    if (!special) tri_tight(T, xa, xb, ya, yb);
+    if (xa > xb || ya > yb) return 0.f;   // no pixel of this tile can be covered
+    T.tv = tri_validity(T, rescale);
     for (int y = ya; y <= yb; ++y) {
         for (int x = xa; x <= xb; ++x) {
             float n1, n2;
@@ -662,8 +663,11 @@ __device__ void replay_source(const G& g, const uint32_t* L, uint32_t n, int tx0
         int i, j;
         unpack_ij(q, i, j);
         quad_tris(g, i, j, out_w, out_h, [&](int k, const Tri& Tr, bool prol) {
-            const int xa = max(Tr.xmin, tx0), xb = min(Tr.xmax, tx0 + kTile - 1);
-            const int ya = max(Tr.ymin, ty0), yb = min(Tr.ymax, ty0 + kTile - 1);
+            int xa = max(Tr.xmin, tx0), xb = min(Tr.xmax, tx0 + kTile - 1);
+            int ya = max(Tr.ymin, ty0), yb = min(Tr.ymax, ty0 + kTile - 1);
+            if (is_finite(Tr.A0) && is_finite(Tr.A1) && is_finite(Tr.B0) && is_finite(Tr.B1) &&
+                is_finite(Tr.C0) && is_finite(Tr.C1) && is_finite(Tr.den))
+                tri_tight(Tr, xa, xb, ya, yb);
             for (int y = ya; y <= yb; ++y)
                 for (int x = xa; x <= xb; ++x) {
                     const int p = (y - ty0) * kTile + (x - tx0);
