@@ -20,6 +20,11 @@ Prints one JSON line:
                 reference's per-stage call order (intermediate maps in HBM).
   open_loop     the same server under Poisson arrivals at 80% of `value`.
   batched       one launch set of `batch` views.
+  batched_mixed_graph  64 random-walk poses at 720p / 1080p / 4K in one
+                launch set captured in a HIP graph (bin capacity autotuned).
+  serving_mix   two resident scenes (the ERP set + 11 perspective 2048x2048
+                views), requests from 720p to 4K perspective and 2048x1024
+                ERP: closed-loop capacity and Poisson arrivals at 70% of it.
   roofline      the dominant kernel (k_raster, tile raster + blend) in the
                 served configuration: algorithmic bytes of one launch set of
                 B views (every source sample once at its stored precision +
@@ -90,6 +95,9 @@ def main():
     ap.add_argument("--budget-gib", type=float, default=32.0, help="serving memory budget (HBM)")
     ap.add_argument("--tune-batches", default="4,8,16,32")
     ap.add_argument("--tune-slots", default="1,2,3")
+    ap.add_argument("--no-c3", action="store_true", help="skip the mixed-size 64-pose graph batch")
+    ap.add_argument("--no-c4", action="store_true", help="skip the two-scene serving mix")
+    ap.add_argument("--c4-requests", type=int, default=256)
     ap.add_argument("--roofline-only", type=int, default=0,
                     help="only run the roofline launch set (this many views, tile_cap 512) 5 times (counter passes)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -213,6 +221,74 @@ def main():
         batched = {"views": args.batch, "views_s": round(args.batch / bt, 2), "batch_ms": round(1e3 * bt, 3)}
         del bb
 
+    c3 = None
+    if not args.no_c3:
+        # 64 random-walk poses, sizes cycling 1280x720 / 1920x1080 / 3840x2160,
+        # one launch set captured in a HIP graph and replayed
+        tg = W.batch_targets(rg, 64, seed=3)
+        cap3, _ = A.tune_tile_cap(srcs, tg[:15])
+        bs3 = R.BatchSynthesizer(srcs, tg, tile_cap=cap3)
+        bs3.run()
+        cap3 = bs3.tile_cap
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        bs3.run()
+        eager_s = time.perf_counter() - a
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            bs3.enqueue()
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            bs3.enqueue()
+        graph.replay()
+        torch.cuda.synchronize()
+        reps = 5
+        a = time.perf_counter()
+        for _ in range(reps):
+            graph.replay()
+        torch.cuda.synchronize()
+        rep_s = (time.perf_counter() - a) / reps
+        ok = bs3.check() == 0
+        c3 = {"views": 64, "sizes": "1280x720, 1920x1080, 3840x2160 cycling", "tile_cap": cap3,
+              "graph_views_s": round(64 / rep_s, 2), "graph_batch_ms": round(1e3 * rep_s, 3),
+              "eager_batch_ms": round(1e3 * eager_s, 3), "bins_ok": bool(ok),
+              "alg_bytes": b_alg(rg, tg), "achieved_GBs": round(b_alg(rg, tg) / rep_s / 1e9, 1)}
+        del graph, bs3
+        torch.cuda.empty_cache()
+        log("c3 %s" % c3)
+
+    c4 = None
+    if not args.no_c4:
+        # two resident scenes (config-2 ERP set + 11 perspective 2048x2048 views),
+        # targets 720p..4K perspective and 2048x1024 ERP, Poisson arrivals
+        rg_b = W.rig("persp11_2048")
+        srcs_b = W.device_sources(rg_b, W.render_sources(rg_b))
+        mix = W.serve_requests([rg, rg_b], args.c4_requests, seed=4)
+        staging4 = B * max(A.yuv_bytes(t) for _, t in mix)
+        budget4 = max(budget, 96 << 30)   # 4K views over 11 sources need larger bins
+        srv = S.Server([srcs, srcs_b], memory_budget=budget4, max_batch=B, slots=SL, tile_cap=cap,
+                       host_staging_bytes=staging4)
+        try:
+            srv.run_closed(mix[:B * SL], concurrency=B * SL)
+            a = time.perf_counter()
+            stc = srv.run_closed(mix, concurrency=B * SL)
+            cap_s = time.perf_counter() - a
+            closed4 = S.summarize(stc, wall_s=cap_s)
+            rate4 = 0.7 * closed4["views_s"]
+            sto = srv.run_open(mix, rate=rate4, seed=4)
+            open4 = S.summarize(sto)
+            open4["offered_rate"] = round(rate4, 1)
+        finally:
+            srv.close()
+        c4 = {"scenes": ["4x ERP 4096x2048", "11x perspective 2048x2048"],
+              "targets": "1280x720, 1920x1080, 2560x1440, 3840x2160 perspective, 2048x1024 ERP",
+              "requests": len(mix), "closed_loop": closed4, "open_loop_70pct": open4}
+        del srcs_b
+        log("c4 %s" % c4)
+
     out = {
         "metric": "served views/s + p50/p99 ms vs plain per-view; 4x4096x2048 ERP->1080p",
         "value": served["views_s"],
@@ -253,6 +329,8 @@ def main():
                     "rvs_inpaint, rvs_encode_yuv420 -- the reference's per-stage call structure",
         },
         "batched": batched,
+        "batched_mixed_graph": c3,
+        "serving_mix": c4,
         "autotune": {"trials": trials},
         "roofline": {
             "kernel": "k_raster (tile raster + blend), one launch over %d views" % B,
