@@ -21,7 +21,7 @@ Prints one JSON line:
   open_loop     the same server under Poisson arrivals at 80% of `value`.
   batched       one launch set of `batch` views.
   batched_mixed_graph  64 random-walk poses at 720p / 1080p / 4K in one
-                launch set captured in a HIP graph (bin capacity autotuned).
+                launch set captured in a HIP graph (entry capacity autotuned).
   serving_mix   two resident scenes (the ERP set + 11 perspective 2048x2048
                 views), requests from 720p to 4K perspective and 2048x1024
                 ERP: closed-loop capacity and Poisson arrivals at 70% of it.
@@ -99,7 +99,7 @@ def main():
     ap.add_argument("--no-c4", action="store_true", help="skip the two-scene serving mix")
     ap.add_argument("--c4-requests", type=int, default=256)
     ap.add_argument("--roofline-only", type=int, default=0,
-                    help="only run the roofline launch set (this many views, tile_cap 512) 5 times (counter passes)")
+                    help="only run the roofline launch set (this many views) 5 times (counter passes)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes 5 times (counter calibration)")
     args = ap.parse_args()
@@ -133,7 +133,7 @@ def main():
     torch.cuda.synchronize()
     if args.roofline_only > 0:
         walk = W.batch_targets(rg, args.roofline_only, seed=4, sizes=((tw, th),))
-        bs = R.BatchSynthesizer(srcs, walk, tile_cap=512)
+        bs = R.BatchSynthesizer(srcs, walk)
         for _ in range(5):
             bs.run()
         torch.cuda.synchronize()
@@ -171,8 +171,9 @@ def main():
 
     # --------------------------------------------------------------- autotuning
     walk = W.batch_targets(rg, max(args.steps, 256), seed=4, sizes=((tw, th),))
-    cap, cap_counters = A.tune_tile_cap(srcs, walk[:16])
-    log("tile_cap %d (max bin %d)" % (cap, cap_counters["max_bin"]))
+    cap, cap_counters = A.tune_entry_cap(srcs, walk[:16])
+    log("entries per view %d (sample: %d entries, largest bin %d)" % (cap, cap_counters["bin_entries"],
+                                                                     cap_counters["max_bin"]))
     budget = int(args.budget_gib * (1 << 30))
     reqs = [(0, t) for t in walk]
     best, trials = A.tune_serving([srcs], reqs, budget, cap, p99_bound_ms=P99_BOUND_MS,
@@ -184,7 +185,7 @@ def main():
 
     # ------------------------------------------------------------------ serving
     staging = B * A.yuv_bytes(walk[0])
-    srv = S.Server([srcs], memory_budget=budget, max_batch=B, slots=SL, tile_cap=cap, host_staging_bytes=staging)
+    srv = S.Server([srcs], memory_budget=budget, max_batch=B, slots=SL, entry_cap=cap, host_staging_bytes=staging)
     try:
         srv.run_closed(reqs[: args.warmup], concurrency=B * SL)
         t_a = time.perf_counter()
@@ -200,7 +201,7 @@ def main():
     log("served %s" % served)
 
     # ------------------------------- one launch set of B views: dominant kernel
-    bs = R.BatchSynthesizer(srcs, walk[:B], tile_cap=cap)
+    bs = R.BatchSynthesizer(srcs, walk[:B], entry_cap=cap * B)
     bs.run()
     bstage, bcount = stage_profile(bs.run, args.profile_reps, R)
     balg = b_alg(rg, walk[:B])
@@ -209,7 +210,7 @@ def main():
 
     batched = None
     if args.batch > 0:
-        bb = R.BatchSynthesizer(srcs, walk[: args.batch], tile_cap=cap)
+        bb = R.BatchSynthesizer(srcs, walk[: args.batch], entry_cap=cap * args.batch)
         bb.run()
         torch.cuda.synchronize()
         times = []
@@ -226,10 +227,10 @@ def main():
         # 64 random-walk poses, sizes cycling 1280x720 / 1920x1080 / 3840x2160,
         # one launch set captured in a HIP graph and replayed
         tg = W.batch_targets(rg, 64, seed=3)
-        cap3, _ = A.tune_tile_cap(srcs, tg[:15])
-        bs3 = R.BatchSynthesizer(srcs, tg, tile_cap=cap3)
+        cap3, _ = A.tune_entry_cap(srcs, tg[:15])
+        bs3 = R.BatchSynthesizer(srcs, tg, entry_cap=min(cap3 * len(tg), 0xFFFFFFFF))
         bs3.run()
-        cap3 = bs3.tile_cap
+        cap3 = bs3.entry_cap
         torch.cuda.synchronize()
         a = time.perf_counter()
         bs3.run()
@@ -252,7 +253,7 @@ def main():
         torch.cuda.synchronize()
         rep_s = (time.perf_counter() - a) / reps
         ok = bs3.check() == 0
-        c3 = {"views": 64, "sizes": "1280x720, 1920x1080, 3840x2160 cycling", "tile_cap": cap3,
+        c3 = {"views": 64, "sizes": "1280x720, 1920x1080, 3840x2160 cycling", "entry_cap": cap3,
               "graph_views_s": round(64 / rep_s, 2), "graph_batch_ms": round(1e3 * rep_s, 3),
               "eager_batch_ms": round(1e3 * eager_s, 3), "bins_ok": bool(ok),
               "alg_bytes": b_alg(rg, tg), "achieved_GBs": round(b_alg(rg, tg) / rep_s / 1e9, 1)}
@@ -268,8 +269,8 @@ def main():
         srcs_b = W.device_sources(rg_b, W.render_sources(rg_b))
         mix = W.serve_requests([rg, rg_b], args.c4_requests, seed=4)
         staging4 = B * max(A.yuv_bytes(t) for _, t in mix)
-        budget4 = max(budget, 96 << 30)   # 4K views over 11 sources need larger bins
-        srv = S.Server([srcs, srcs_b], memory_budget=budget4, max_batch=B, slots=SL, tile_cap=cap,
+        budget4 = max(budget, 96 << 30)   # 4K views over 11 sources need larger workspaces
+        srv = S.Server([srcs, srcs_b], memory_budget=budget4, max_batch=B, slots=SL, entry_cap=0,
                        host_staging_bytes=staging4)
         try:
             srv.run_closed(mix[:B * SL], concurrency=B * SL)
@@ -310,7 +311,7 @@ def main():
             "workload": "%s: 4x ERP 4096x2048 10-bit + 16-bit depth (resident) -> 1920x1080 perspective views at "
                         "random-walk poses, blending 1.0, YUV420 10-bit out" % args.workload,
             "mode": "served: closed loop, concurrency = batch x slots, continuous batching",
-            "max_batch": B, "slots": SL, "tile_cap": cap, "memory_budget_gib": args.budget_gib,
+            "max_batch": B, "slots": SL, "entries_per_view": cap, "memory_budget_gib": args.budget_gib,
             "p99_bound_ms": P99_BOUND_MS,
         },
         "served": served,

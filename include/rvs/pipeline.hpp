@@ -190,7 +190,7 @@ private:
     bool m_stagewise = false;
     float m_color_scale = -1.f, m_depth_scale = -1.f;
     DeviceBuffer m_ws;
-    uint32_t m_tile_cap = 0;
+    uint32_t m_entry_cap = 0;   // binned-entry list capacity of the fused path (0: default)
     std::future<void> m_save;
 };
 

@@ -32,7 +32,7 @@ typedef struct ihipStream_t* rvs_stream_t;   /* hipStream_t */
 #define RVS_OK 0
 #define RVS_E_INVALID (-1)   /* bad argument */
 #define RVS_E_HIP (-2)       /* HIP runtime error */
-#define RVS_E_CAPACITY (-3)  /* a per-tile bin or workspace limit was exceeded; see rvs_last_error */
+#define RVS_E_CAPACITY (-3)  /* the binned-entry list or a workspace limit was exceeded; see rvs_last_error */
 #define RVS_E_UNSUPPORTED (-4)
 
 #define RVS_MAX_SOURCES 64
@@ -104,12 +104,14 @@ int rvs_reproject(const rvs_cam* src, const rvs_cam* dst, const rvs_xform* xf, c
  * rvs_reproject).  Outputs at out_w*out_h: color*3, depth (+inf uncovered),
  * shape (triangle validity, 0 uncovered).  workspace: rvs_raster_workspace_size
  * (0 for invalid sizes).
- * Synchronises the stream; RVS_E_CAPACITY reports the needed tile_cap through
- * needed_cap (if not NULL) when a bin overflowed. */
-size_t rvs_raster_workspace_size(int in_w, int in_h, int out_w, int out_h, uint32_t tile_cap);
+ * entry_cap: capacity of the binned (quad, tile) entry list, 0 = one entry per
+ * output pixel (at least 2^20).  Synchronises the stream; RVS_E_CAPACITY
+ * reports the needed entry_cap through needed_cap (if not NULL) when the
+ * entries exceeded it (the outputs are then not written). */
+size_t rvs_raster_workspace_size(int in_w, int in_h, int out_w, int out_h, uint32_t entry_cap);
 int rvs_raster(const float* color, const float* depth, const float* uv, int in_w, int in_h,
                int out_w, int out_h, int wrap, float rescale, void* workspace, size_t ws_bytes,
-               uint32_t tile_cap, float* out_color, float* out_depth, float* out_shape,
+               uint32_t entry_cap, float* out_color, float* out_depth, float* out_shape,
                uint32_t* needed_cap, rvs_stream_t stream);
 
 /* Replaces blendImages (blend_img.cuh:12-17) with the CPU semantics of
@@ -186,18 +188,21 @@ typedef struct {
 } rvs_view_out;
 
 /* Workspace bytes for rvs_synthesize over these sources and target (0 when an
- * argument is invalid).  tile_cap = per (source, tile) bin capacity (0 = default). */
+ * argument is invalid).  entry_cap = capacity of the binned (quad, tile) entry
+ * list of the launch set (0 = one entry per output pixel and source, at least
+ * 2^20); bins are compact, so only the total matters. */
 size_t rvs_synth_workspace_size(const rvs_source* srcs, int n_src, const rvs_target* tgt,
-                                uint32_t tile_cap);
+                                uint32_t entry_cap);
 
 /* Pipeline::computeView for one virtual view (Pipeline.cpp:169-301, CPU
  * semantics): for each source in order, reproject + rasterise + blend, then
  * inpaint and encode.  Sources must already be resident.  Returns
- * RVS_E_CAPACITY (and the needed tile_cap through needed_cap, if not NULL)
- * when a per-tile bin overflowed; this check synchronises the stream.
+ * RVS_E_CAPACITY (and the needed entry_cap through needed_cap, if not NULL)
+ * when the binned entries exceeded entry_cap; this check synchronises the
+ * stream.
  * Equivalent to rvs_synthesize_batch with one view. */
 int rvs_synthesize(const rvs_source* srcs, int n_src, const rvs_target* tgt, void* workspace,
-                   size_t ws_bytes, uint32_t tile_cap, const rvs_view_out* out,
+                   size_t ws_bytes, uint32_t entry_cap, const rvs_view_out* out,
                    uint32_t* needed_cap, rvs_stream_t stream);
 
 /* Batches: n_views virtual views (e.g. the poses of a pose trace, or requests
@@ -208,20 +213,20 @@ int rvs_synthesize(const rvs_source* srcs, int n_src, const rvs_target* tgt, voi
  * Results are those of n_views separate rvs_synthesize calls.  At most
  * 4096 views and 2^31 output pixels per batch. */
 size_t rvs_synth_batch_workspace_size(const rvs_source* srcs, int n_src, const rvs_target* tgts,
-                                      int n_views, uint32_t tile_cap);
+                                      int n_views, uint32_t entry_cap);
 int rvs_synthesize_batch(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views,
-                         void* workspace, size_t ws_bytes, uint32_t tile_cap,
+                         void* workspace, size_t ws_bytes, uint32_t entry_cap,
                          const rvs_view_out* outs, uint32_t* needed_cap, rvs_stream_t stream);
 /* Enqueue only: no synchronisation and no host-device copies, so the call can be
  * captured into a HIP graph (hipStreamBeginCapture) and replayed.  Descriptors
  * travel as kernel arguments: a replay re-runs the captured poses and buffers.
- * Check for bin overflow afterwards with rvs_synth_check (same arguments); an
- * overflowed batch must be re-run with the reported tile_cap. */
+ * Check for list overflow afterwards with rvs_synth_check (same arguments);
+ * an overflowed batch must be re-run with the reported entry_cap. */
 int rvs_synthesize_batch_async(const rvs_source* srcs, int n_src, const rvs_target* tgts,
-                               int n_views, void* workspace, size_t ws_bytes, uint32_t tile_cap,
+                               int n_views, void* workspace, size_t ws_bytes, uint32_t entry_cap,
                                const rvs_view_out* outs, rvs_stream_t stream);
 int rvs_synth_check(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views,
-                    const void* workspace, uint32_t tile_cap, uint32_t* needed_cap,
+                    const void* workspace, uint32_t entry_cap, uint32_t* needed_cap,
                     rvs_stream_t stream);
 
 /* Per-stage device times of the last rvs_synthesize* call on the calling
@@ -234,7 +239,7 @@ int rvs_synth_check(const rvs_source* srcs, int n_src, const rvs_target* tgts, i
 int rvs_profile_enable(int on);
 int rvs_profile_read(float ms[RVS_PROF_STAGES]);
 /* Counters of the last profiled call: [0] tiles with pixels replayed in order,
- * [1] bin overflow flag, [2] largest (source, tile) bin, [3] total binned
+ * [1] list overflow flag, [2] largest (source, tile) bin, [3] total binned
  * (quad, tile) entries over all sources, [4] binning work items left after
  * culling.  Synchronises the device. */
 #define RVS_PROF_COUNTERS 5

@@ -54,7 +54,7 @@ typedef struct {
     size_t memory_budget;       /* device bytes for in-flight batches (workspaces + outputs), split over slots */
     int max_batch;              /* views per batch */
     int slots;                  /* batches in flight (one stream each) */
-    uint32_t tile_cap;          /* initial per (source, tile) bin capacity (0 = default); grows on overflow */
+    uint32_t entry_cap;         /* binned entries per view of a batch (0 = library default); grows on overflow */
     int copy_to_host;           /* copy each request's YUV planes to pinned host memory */
     int keep_outputs;           /* keep a host copy per request (rvs_server_output) */
     size_t host_staging_bytes;  /* pinned staging per slot (0 = a quarter of the slot's share, at most 1 GiB) */

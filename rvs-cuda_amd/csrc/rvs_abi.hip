@@ -111,33 +111,55 @@ struct Layout {
     }
 };
 
-// Raster state of one launch set: status words, bins, deferred-pixel flags,
-// deferred-tile list and binning work list.
+// Raster state of one launch set: status words, compact bins (counts,
+// offsets, fill cursors, entry list, scan sums, kept tile ranges),
+// deferred-pixel flags, deferred-tile list and binning work list.
 struct RasterPlan {
     rvs_k::BatchDims d;
-    size_t status, counts, lists, deferred, dtiles, work;
+    size_t status, counts, offsets, fill, lists, ranges, bsum, deferred, dtiles, work;
 };
+
+// List capacity: the caller's, or one entry per output pixel and source
+// (at least 2^20); the kernels report the exact need when it is exceeded.
+uint32_t entry_capacity(uint32_t requested, uint64_t npx, int nsrc) {
+    if (requested) return requested;
+    const uint64_t c = npx * (uint64_t)nsrc;
+    return (uint32_t)(c < (1ull << 20) ? (1ull << 20) : (c > 0xffffffffull ? 0xffffffffull : c));
+}
 
 void plan_raster(Layout& L, RasterPlan& R, int nsrc, int nviews, uint32_t ntiles, uint32_t npx, uint32_t cap,
                  int max_sw, int max_sh) {
     rvs_k::BatchDims& d = R.d;
-    d.nviews = nviews; d.nsrc = nsrc; d.ntiles = ntiles; d.npx = npx; d.cap = cap;
+    d.nviews = nviews; d.nsrc = nsrc; d.ntiles = ntiles; d.npx = npx;
+    d.cap = entry_capacity(cap, npx, nsrc);
     d.max_sw = max_sw; d.max_sh = max_sh;
     d.bins_x = (uint32_t)((max_sw - 1 + rvs_k::kBinQX - 1) / rvs_k::kBinQX);
     d.bins_y = (uint32_t)((max_sh - 1 + rvs_k::kBinQY - 1) / rvs_k::kBinQY);
+    const uint64_t items = (uint64_t)d.bins_x * d.bins_y * nsrc * nviews;
+    const uint64_t keep = items / 4 > 4096 ? items / 4 : 4096;
+    d.range_cap = (uint32_t)(keep < items ? keep : items);
+    const size_t nbins = (size_t)nsrc * ntiles;
     R.status = L.take(16 * sizeof(uint32_t));
-    R.counts = L.take((size_t)nsrc * ntiles * sizeof(uint32_t));
-    R.lists = L.take((size_t)nsrc * ntiles * cap * sizeof(uint32_t));
+    R.counts = L.take(nbins * sizeof(uint32_t));
+    R.offsets = L.take(nbins * sizeof(uint32_t));
+    R.fill = L.take(nbins * sizeof(uint32_t));
+    R.bsum = L.take(rvs_k::scan_blocks(nbins) * sizeof(unsigned long long));
+    R.lists = L.take((size_t)d.cap * sizeof(uint32_t));
+    R.ranges = L.take((size_t)d.range_cap * 256 * sizeof(uint32_t));
     R.deferred = L.take(npx);
     R.dtiles = L.take((size_t)ntiles * sizeof(uint32_t));
-    R.work = L.take((size_t)d.bins_x * d.bins_y * nsrc * nviews * sizeof(uint32_t));
+    R.work = L.take((size_t)items * sizeof(uint32_t));
 }
 
 rvs_k::RasterWS carve_raster(char* base, const RasterPlan& R) {
     rvs_k::RasterWS ws;
     ws.status = (uint32_t*)(base + R.status);
     ws.counts = (uint32_t*)(base + R.counts);
+    ws.offsets = (uint32_t*)(base + R.offsets);
+    ws.fill = (uint32_t*)(base + R.fill);
+    ws.bsum = (unsigned long long*)(base + R.bsum);
     ws.lists = (uint32_t*)(base + R.lists);
+    ws.ranges = (uint32_t*)(base + R.ranges);
     ws.deferred = (uint8_t*)(base + R.deferred);
     ws.deferred_tiles = (uint32_t*)(base + R.dtiles);
     ws.work = (uint32_t*)(base + R.work);
@@ -146,8 +168,9 @@ rvs_k::RasterWS carve_raster(char* base, const RasterPlan& R) {
 
 hipError_t reset_raster(const rvs_k::RasterWS& ws, const rvs_k::BatchDims& d, hipStream_t s) {
     hipError_t e = rvs_k::launch_reset_u32(ws.status, 16, s);
-    if (e != hipSuccess) return e;
-    return rvs_k::launch_reset_u32(ws.counts, (size_t)d.nsrc * d.ntiles, s);
+    if (e == hipSuccess) e = rvs_k::launch_reset_u32(ws.counts, (size_t)d.nsrc * d.ntiles, s);
+    if (e == hipSuccess) e = rvs_k::launch_reset_u32(ws.fill, (size_t)d.nsrc * d.ntiles, s);
+    return e;
 }
 
 // A view's tile grid and its place in the batch's global tile / pixel numbering.
@@ -161,18 +184,16 @@ void view_grid(rvs_k::ViewDesc& v, int w, int h, uint32_t& tiles, uint32_t& px) 
     px += (uint32_t)(w * h);
 }
 
-// After a raster launch set: report bin overflow (synchronises the stream).
+// After a raster launch set: report list overflow (synchronises the stream).
 int check_raster_status(const rvs_k::RasterWS& ws, const rvs_k::BatchDims& d, uint32_t* needed_cap, hipStream_t s) {
     uint32_t st[4];
     HIP_TRY(hipMemcpyAsync(st, ws.status, sizeof st, hipMemcpyDeviceToHost, s), "status copy");
     HIP_TRY(hipStreamSynchronize(s), "stream sync");
     if (st[1]) {
-        std::vector<uint32_t> counts((size_t)d.nsrc * d.ntiles);
-        HIP_TRY(hipMemcpy(counts.data(), ws.counts, counts.size() * 4, hipMemcpyDeviceToHost), "counts copy");
-        uint32_t mx = 0;
-        for (uint32_t c : counts) mx = c > mx ? c : mx;
-        if (needed_cap) *needed_cap = mx;
-        return fail(RVS_E_CAPACITY, "tile bin overflow: a (source, tile) bin holds %u quads, capacity %u", mx, d.cap);
+        if (needed_cap) *needed_cap = st[2];
+        if (st[2] == 0xffffffffu)
+            return fail(RVS_E_CAPACITY, "binned entries exceed 2^32 - 1: split the batch");
+        return fail(RVS_E_CAPACITY, "binned (quad, tile) entries %u exceed the list capacity %u", st[2], d.cap);
     }
     if (st[3]) return fail(RVS_E_CAPACITY, "in-order replay of a tile bin of %u quads exceeds %d", st[3],
                            rvs_k::kFallbackSortCap);
@@ -466,16 +487,15 @@ static size_t raster_plan(int in_w, int in_h, int out_w, int out_h, uint32_t cap
     return L.o;
 }
 
-size_t rvs_raster_workspace_size(int in_w, int in_h, int out_w, int out_h, uint32_t tile_cap) {
+size_t rvs_raster_workspace_size(int in_w, int in_h, int out_w, int out_h, uint32_t entry_cap) {
     if (in_w < 2 || in_h < 2 || out_w < 1 || out_h < 1) return 0;
-    if (tile_cap == 0) tile_cap = rvs_k::kDefaultTileCap;
     RasterPlan R;
     size_t a, b;
-    return raster_plan(in_w, in_h, out_w, out_h, tile_cap, R, a, b);
+    return raster_plan(in_w, in_h, out_w, out_h, entry_cap, R, a, b);
 }
 
 int rvs_raster(const float* color, const float* depth, const float* uv, int in_w, int in_h, int out_w, int out_h,
-               int wrap, float rescale, void* workspace, size_t ws_bytes, uint32_t tile_cap, float* out_color,
+               int wrap, float rescale, void* workspace, size_t ws_bytes, uint32_t entry_cap, float* out_color,
                float* out_depth, float* out_shape, uint32_t* needed_cap, rvs_stream_t stream) {
     if (!color || !depth || !uv || in_w < 2 || in_h < 2 || out_w < 1 || out_h < 1 || !workspace || !out_color ||
         !out_depth || !out_shape || !(rescale > 0.f))
@@ -483,10 +503,9 @@ int rvs_raster(const float* color, const float* depth, const float* uv, int in_w
     if ((uint64_t)out_w * out_h >= (1ull << 31) || (uint64_t)(in_w - 1) * (in_h - 1) >= (1ull << 30) ||
         in_w > rvs_k::kMaxSourceSide || in_h > rvs_k::kMaxSourceSide)
         return fail(RVS_E_CAPACITY, "rvs_raster: image too large");
-    if (tile_cap == 0) tile_cap = rvs_k::kDefaultTileCap;
     RasterPlan R;
     size_t ov, og;
-    const size_t need = raster_plan(in_w, in_h, out_w, out_h, tile_cap, R, ov, og);
+    const size_t need = raster_plan(in_w, in_h, out_w, out_h, entry_cap, R, ov, og);
     if (ws_bytes < need) return fail(RVS_E_INVALID, "rvs_raster: workspace %zu < %zu bytes", ws_bytes, need);
     hipStream_t s = (hipStream_t)stream;
     char* base = (char*)workspace;
@@ -558,48 +577,44 @@ int rvs_encode_yuv420(const float* color, int w, int h, int bitdepth, void* y, v
 // ------------------------------------------------------------ fused path
 
 size_t rvs_synth_batch_workspace_size(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views,
-                                      uint32_t tile_cap) {
-    if (tile_cap == 0) tile_cap = rvs_k::kDefaultTileCap;
+                                      uint32_t entry_cap) {
     SynthPlan P;
-    return plan_synth(srcs, n_src, tgts, n_views, tile_cap, P, nullptr) == RVS_OK ? P.total : 0;
+    return plan_synth(srcs, n_src, tgts, n_views, entry_cap, P, nullptr) == RVS_OK ? P.total : 0;
 }
 
-size_t rvs_synth_workspace_size(const rvs_source* srcs, int n_src, const rvs_target* tgt, uint32_t tile_cap) {
-    return rvs_synth_batch_workspace_size(srcs, n_src, tgt, 1, tile_cap);
+size_t rvs_synth_workspace_size(const rvs_source* srcs, int n_src, const rvs_target* tgt, uint32_t entry_cap) {
+    return rvs_synth_batch_workspace_size(srcs, n_src, tgt, 1, entry_cap);
 }
 
 int rvs_synthesize_batch_async(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views,
-                               void* workspace, size_t ws_bytes, uint32_t tile_cap, const rvs_view_out* outs,
+                               void* workspace, size_t ws_bytes, uint32_t entry_cap, const rvs_view_out* outs,
                                rvs_stream_t stream) {
-    if (tile_cap == 0) tile_cap = rvs_k::kDefaultTileCap;
     SynthPlan P;
     rvs_k::RasterWS ws;
     g_prof_valid = false;
-    const int rc = enqueue_synth(srcs, n_src, tgts, n_views, workspace, ws_bytes, tile_cap, outs,
+    const int rc = enqueue_synth(srcs, n_src, tgts, n_views, workspace, ws_bytes, entry_cap, outs,
                                  (hipStream_t)stream, P, ws);
     if (rc == RVS_OK && g_prof) { g_prof_valid = true; g_prof_ws = ws; g_prof_dims = P.R.d; }
     return rc;
 }
 
 int rvs_synth_check(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views, const void* workspace,
-                    uint32_t tile_cap, uint32_t* needed_cap, rvs_stream_t stream) {
-    if (tile_cap == 0) tile_cap = rvs_k::kDefaultTileCap;
+                    uint32_t entry_cap, uint32_t* needed_cap, rvs_stream_t stream) {
     if (!workspace) return fail(RVS_E_INVALID, "rvs_synth_check: null workspace");
     SynthPlan P;
-    const int prc = plan_synth(srcs, n_src, tgts, n_views, tile_cap, P, nullptr);
+    const int prc = plan_synth(srcs, n_src, tgts, n_views, entry_cap, P, nullptr);
     if (prc != RVS_OK) return prc;
     const rvs_k::RasterWS ws = carve_raster((char*)workspace, P.R);
     return check_raster_status(ws, P.R.d, needed_cap, (hipStream_t)stream);
 }
 
 int rvs_synthesize_batch(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views, void* workspace,
-                         size_t ws_bytes, uint32_t tile_cap, const rvs_view_out* outs, uint32_t* needed_cap,
+                         size_t ws_bytes, uint32_t entry_cap, const rvs_view_out* outs, uint32_t* needed_cap,
                          rvs_stream_t stream) {
-    if (tile_cap == 0) tile_cap = rvs_k::kDefaultTileCap;
     SynthPlan P;
     rvs_k::RasterWS ws;
     g_prof_valid = false;
-    const int rc = enqueue_synth(srcs, n_src, tgts, n_views, workspace, ws_bytes, tile_cap, outs,
+    const int rc = enqueue_synth(srcs, n_src, tgts, n_views, workspace, ws_bytes, entry_cap, outs,
                                  (hipStream_t)stream, P, ws);
     if (rc != RVS_OK) return rc;
     const int crc = check_raster_status(ws, P.R.d, needed_cap, (hipStream_t)stream);
@@ -608,8 +623,8 @@ int rvs_synthesize_batch(const rvs_source* srcs, int n_src, const rvs_target* tg
 }
 
 int rvs_synthesize(const rvs_source* srcs, int n_src, const rvs_target* tgt, void* workspace, size_t ws_bytes,
-                   uint32_t tile_cap, const rvs_view_out* out, uint32_t* needed_cap, rvs_stream_t stream) {
-    return rvs_synthesize_batch(srcs, n_src, tgt, 1, workspace, ws_bytes, tile_cap, out, needed_cap, stream);
+                   uint32_t entry_cap, const rvs_view_out* out, uint32_t* needed_cap, rvs_stream_t stream) {
+    return rvs_synthesize_batch(srcs, n_src, tgt, 1, workspace, ws_bytes, entry_cap, out, needed_cap, stream);
 }
 
 int rvs_profile_counters(uint32_t out[RVS_PROF_COUNTERS]) {

@@ -20,7 +20,8 @@ namespace rvs_k {
 constexpr int kTile = 16;            // target tile edge (pixels); one 256-thread block per tile
 constexpr int kBinQX = 64;           // binning work item: 64 x 4 source quads
 constexpr int kBinQY = 4;
-constexpr uint32_t kDefaultTileCap = 1024;
+// default list capacity: entries per output pixel per source (x 1/4)
+constexpr uint32_t kDefaultEntriesPerPxSrcQ = 4;
 constexpr int kFallbackChunk = 128;  // list entries per LDS chunk in the in-order replay
 constexpr int kFallbackSortCap = 8192;  // longest (source, tile) list the replay sorts in LDS
 constexpr int kMaxViews = 4096;
@@ -46,20 +47,25 @@ struct ViewDesc {
 
 // Device-side state of one launch set.
 struct RasterWS {
-    uint32_t* counts;        // [global tile][source] bin fill counters
-    uint32_t* lists;         // [global tile][source][cap] quad indices
+    uint32_t* counts;        // [global tile][source] entries per bin
+    uint32_t* offsets;       // [global tile][source] first entry of each bin in lists
+    uint32_t* fill;          // [global tile][source] fill cursors
+    uint32_t* lists;         // [cap] quad codes (row << 16 | column), bins back to back
+    uint32_t* ranges;        // [range_cap work items][256] packed tile range per quad
+    unsigned long long* bsum;// scan block sums
     uint8_t* deferred;       // [global pixel] pixel handled by the in-order replay
     uint32_t* deferred_tiles;// [global tile]
     uint32_t* work;          // binning work items (view, source, block)
-    uint32_t* status;        // [0] n_deferred, [1] bin overflow, [2] -, [3] replay overflow, [4] work items,
-                             // [5] work cursor
+    uint32_t* status;        // [0] n_deferred, [1] list overflow, [2] entries (saturated), [3] replay
+                             // overflow, [4] work items, [5] count cursor, [6] fill cursor
 };
 
 struct BatchDims {
     int nviews, nsrc;
     uint32_t ntiles;         // all views
     uint32_t npx;            // all views
-    uint32_t cap;            // per (tile, source) bin capacity
+    uint32_t cap;            // list capacity (binned (quad, tile) entries of the launch set)
+    uint32_t range_cap;      // work items whose quad tile ranges are kept between the passes
     int max_sw, max_sh;      // largest source grid
     uint32_t bins_x, bins_y; // binning work items per source (for the largest source)
 };
@@ -67,6 +73,8 @@ struct BatchDims {
 enum { MODE_BLEND = 0, MODE_RAW = 1 };
 
 hipError_t launch_reset_u32(uint32_t* p, size_t n, hipStream_t s);
+// Scan blocks needed for nbins bins (bsum entries).
+size_t scan_blocks(size_t nbins);
 // Raw bytes into device memory through kernel arguments (stream-ordered, capturable).
 hipError_t launch_upload(const void* host, size_t bytes, void* dst, hipStream_t s);
 

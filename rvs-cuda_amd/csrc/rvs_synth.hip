@@ -278,8 +278,33 @@ __global__ void k_bin_cull(const G* __restrict__ geoms, const ViewDesc* __restri
 }
 
 // ------------------------------------------------------------------ binning
+//
+// Bins are compact: a counting pass (k_bin<COUNT>) finds every surviving
+// quad's tile range and counts the (quad, tile) entries per (tile, source)
+// bin; an exclusive scan turns the counts into bin offsets; a fill pass
+// (k_bin<FILL>) writes the quad codes.  The counting pass stores the tile
+// range of each quad of the first range_cap work items (4 bytes per quad);
+// later items re-evaluate their vertices in the fill pass.  When the entries
+// exceed the list capacity the fill pass and the raster are skipped and the
+// needed capacity is reported.
 
-template <class G>
+constexpr uint32_t kRangeNone = 0xffffffffu;   // no surviving triangle
+constexpr uint32_t kRangeBig = 0xfffffffeu;    // range too wide to pack: re-evaluated
+
+__device__ inline uint32_t pack_range(int tx0, int tx1, int ty0, int ty1) {
+    const int ntx = tx1 - tx0, nty = ty1 - ty0;
+    if (ntx > 14 || nty > 14) return kRangeBig;
+    return (uint32_t)tx0 | ((uint32_t)ty0 << 12) | ((uint32_t)ntx << 24) | ((uint32_t)nty << 28);
+}
+
+__device__ inline void unpack_range(uint32_t r, int& tx0, int& tx1, int& ty0, int& ty1) {
+    tx0 = (int)(r & 0xfffu);
+    ty0 = (int)((r >> 12) & 0xfffu);
+    tx1 = tx0 + (int)((r >> 24) & 0xfu);
+    ty1 = ty0 + (int)(r >> 28);
+}
+
+template <class G, bool FILL>
 __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
                                              BatchDims d, RasterWS ws) {
     __shared__ float3 V[kBinQY + 1][kBinQX + 1];
@@ -288,9 +313,10 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
     const int lane = __lane_id();
     const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     const uint32_t per = d.bins_x * d.bins_y;
+    if (FILL && ws.status[1]) return;   // list capacity exceeded: nothing to fill
     const uint32_t nwork = ws.status[4];
     while (true) {
-        if (tid == 0) s_item = atomicAdd(&ws.status[5], 1u);
+        if (tid == 0) s_item = atomicAdd(&ws.status[FILL ? 6 : 5], 1u);
         __syncthreads();
         const uint32_t wi = s_item;
         __syncthreads();
@@ -301,38 +327,46 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
         const G& g = geoms[vs];
         const ViewDesc& vd = views[v];
         const int i0 = (int)(item / d.bins_x) * kBinQY, j0 = (int)(item % d.bins_x) * kBinQX;
-        for (int k = tid; k < (kBinQY + 1) * (kBinQX + 1); k += 256) {
-            const int r = k / (kBinQX + 1), c = k % (kBinQX + 1);
-            const int i = i0 + r, j = j0 + c;
-            if (i < g.sh && j < g.sw) V[r][c] = vertex(g, i, j);
-        }
-        __syncthreads();
         const int i = i0 + tid / kBinQX, j = j0 + tid % kBinQX;
+        const bool stored = wi < d.range_cap;
+        uint32_t r = kRangeNone;
+        if (FILL && stored) r = ws.ranges[(size_t)wi * 256 + tid];
         bool more = false;
         int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
-        uint32_t q = 0;
-        if (i < g.sh - 1 && j < g.sw - 1) {
-            Quad Q;
-            auto vtx = [&](int ii, int jj) -> float3 {
-                const int r = ii - i0, c = jj - j0;
-                if (r >= 0 && r <= kBinQY && c >= 0 && c <= kBinQX) return V[r][c];
-                return vertex(g, ii, jj);
-            };
-            quad_build(Q, g, i, j, vtx, vd.w, vd.h, false);
-            if (Q.alive) {
-                int xmin = INT_MAX, xmax = INT_MIN, ymin = INT_MAX, ymax = INT_MIN;
-                for (int k = 0; k < 4; ++k)
-                    if (Q.alive & (1u << k)) {
-                        xmin = min(xmin, Q.t[k].xmin); xmax = max(xmax, Q.t[k].xmax);
-                        ymin = min(ymin, Q.t[k].ymin); ymax = max(ymax, Q.t[k].ymax);
-                    }
-                tx0 = xmin / kTile; tx1 = xmax / kTile; ty0 = ymin / kTile; ty1 = ymax / kTile;
-                more = true;
-                q = ((uint32_t)i << 16) | (uint32_t)j;
+        if (!FILL || __syncthreads_or(!stored || r == kRangeBig)) {
+            for (int k = tid; k < (kBinQY + 1) * (kBinQX + 1); k += 256) {
+                const int rr = k / (kBinQX + 1), c = k % (kBinQX + 1);
+                const int ii = i0 + rr, jj = j0 + c;
+                if (ii < g.sh && jj < g.sw) V[rr][c] = vertex(g, ii, jj);
             }
+            __syncthreads();
+            if (i < g.sh - 1 && j < g.sw - 1) {
+                Quad Q;
+                auto vtx = [&](int ii, int jj) -> float3 {
+                    const int rr = ii - i0, c = jj - j0;
+                    if (rr >= 0 && rr <= kBinQY && c >= 0 && c <= kBinQX) return V[rr][c];
+                    return vertex(g, ii, jj);
+                };
+                quad_build(Q, g, i, j, vtx, vd.w, vd.h, false);
+                if (Q.alive) {
+                    int xmin = INT_MAX, xmax = INT_MIN, ymin = INT_MAX, ymax = INT_MIN;
+                    for (int k = 0; k < 4; ++k)
+                        if (Q.alive & (1u << k)) {
+                            xmin = min(xmin, Q.t[k].xmin); xmax = max(xmax, Q.t[k].xmax);
+                            ymin = min(ymin, Q.t[k].ymin); ymax = max(ymax, Q.t[k].ymax);
+                        }
+                    tx0 = xmin / kTile; tx1 = xmax / kTile; ty0 = ymin / kTile; ty1 = ymax / kTile;
+                    more = true;
+                }
+            }
+            if (!FILL && stored) ws.ranges[(size_t)wi * 256 + tid] = more ? pack_range(tx0, tx1, ty0, ty1) : kRangeNone;
+        } else if (r != kRangeNone) {
+            unpack_range(r, tx0, tx1, ty0, ty1);
+            more = true;
         }
+        const uint32_t q = ((uint32_t)i << 16) | (uint32_t)j;
         int tx = tx0, ty = ty0;
-        // Wave-aggregated appends: one atomic per distinct tile per round.
+        // Wave-aggregated: one atomic per distinct tile per round.
         while (__any(more)) {
             const uint32_t t = more ? vd.tile_base + (uint32_t)(ty * vd.tiles_x + tx) : 0xffffffffu;
             const uint64_t act = __ballot(more);
@@ -342,20 +376,84 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
             const uint64_t m = __ballot(mine);
             const size_t bin = (size_t)tl * d.nsrc + s;
             uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&ws.counts[bin], (uint32_t)__popcll(m));
+            if (lane == leader) base = atomicAdd(FILL ? &ws.fill[bin] : &ws.counts[bin], (uint32_t)__popcll(m));
             base = __shfl(base, leader);
             if (mine) {
-                const uint32_t pos = base + (uint32_t)__popcll(m & lt_mask);
-                if (pos < d.cap)
-                    ws.lists[bin * d.cap + pos] = q;
-                else
-                    atomicMax(&ws.status[1], 1u);
+                if (FILL) ws.lists[(size_t)ws.offsets[bin] + base + (uint32_t)__popcll(m & lt_mask)] = q;
                 if (++tx > tx1) { tx = tx0; if (++ty > ty1) more = false; }
             }
         }
         __syncthreads();
     }
 }
+
+// Exclusive scan of the bin counts into bin offsets (64-bit block sums; the
+// total, saturated to 32 bits, goes to status[2], and status[1] is set when
+// it exceeds the list capacity).
+constexpr int kScanPer = 2048;   // bins per scan block (256 threads x 8)
+
+__global__ void __launch_bounds__(256) k_scan_blocks(const uint32_t* __restrict__ counts, uint32_t* offsets,
+                                                     size_t n, unsigned long long* bsum) {
+    __shared__ unsigned long long part[256];
+    const size_t base = (size_t)blockIdx.x * kScanPer + (size_t)threadIdx.x * 8;
+    uint32_t c[8];
+    unsigned long long sum = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        c[k] = base + k < n ? counts[base + k] : 0u;
+        sum += c[k];
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {   // inclusive Hillis-Steele scan
+        const unsigned long long add = (int)threadIdx.x >= off ? part[threadIdx.x - off] : 0ull;
+        __syncthreads();
+        part[threadIdx.x] += add;
+        __syncthreads();
+    }
+    unsigned long long run = part[threadIdx.x] - sum;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (base + k < n) offsets[base + k] = (uint32_t)run;
+        run += c[k];
+    }
+    if (threadIdx.x == 255) bsum[blockIdx.x] = part[255];
+}
+
+__global__ void __launch_bounds__(1024) k_scan_top(unsigned long long* bsum, uint32_t nb, uint32_t cap,
+                                                   uint32_t* status) {
+    __shared__ unsigned long long part[1024];
+    __shared__ unsigned long long carry;
+    if (threadIdx.x == 0) carry = 0ull;
+    __syncthreads();
+    for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
+        const uint32_t b = b0 + threadIdx.x;
+        const unsigned long long x = b < nb ? bsum[b] : 0ull;
+        part[threadIdx.x] = x;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const unsigned long long add = (int)threadIdx.x >= off ? part[threadIdx.x - off] : 0ull;
+            __syncthreads();
+            part[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (b < nb) bsum[b] = carry + part[threadIdx.x] - x;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += part[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        status[2] = carry > 0xffffffffull ? 0xffffffffu : (uint32_t)carry;
+        if (carry > (unsigned long long)cap) status[1] = 1u;
+    }
+}
+
+__global__ void k_scan_add(uint32_t* offsets, size_t n, const unsigned long long* __restrict__ bsum) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) offsets[i] += (uint32_t)bsum[i / kScanPer];
+}
+
+size_t scan_blocks(size_t nbins) { return (nbins + kScanPer - 1) / kScanPer; }
 
 template <class G>
 hipError_t launch_bin(const G* geoms, const ViewDesc* views, const BatchDims& d, const RasterWS& ws, hipStream_t s) {
@@ -364,7 +462,14 @@ hipError_t launch_bin(const G* geoms, const ViewDesc* views, const BatchDims& d,
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t blocks = total < (size_t)kBinBlocks ? total : (size_t)kBinBlocks;
-    hipLaunchKernelGGL(k_bin<G>, dim3((unsigned)blocks), dim3(256), 0, s, geoms, views, d, ws);
+    hipLaunchKernelGGL((k_bin<G, false>), dim3((unsigned)blocks), dim3(256), 0, s, geoms, views, d, ws);
+    const size_t nbins = (size_t)d.ntiles * d.nsrc;
+    const size_t nb = scan_blocks(nbins);
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(256), 0, s, ws.counts, ws.offsets, nbins, ws.bsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, ws.bsum, (uint32_t)nb, d.cap, ws.status);
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((nbins + 255) / 256)), dim3(256), 0, s, ws.offsets, nbins,
+                       ws.bsum);
+    hipLaunchKernelGGL((k_bin<G, true>), dim3((unsigned)blocks), dim3(256), 0, s, geoms, views, d, ws);
     return hipGetLastError();
 }
 
@@ -529,7 +634,7 @@ __global__ void __launch_bounds__(256) k_raster(const G* __restrict__ geoms, con
     __shared__ uint32_t pref[kRasterGroup + 1];
     __shared__ RasterCache cache;
     const uint32_t T = tile_of_block(blockIdx.x, d.ntiles);
-    if (T >= d.ntiles) return;
+    if (T >= d.ntiles || ws.status[1]) return;   // bins not filled: capacity reported to the host
     const int v = find_view(views, d.nviews, T);
     const ViewDesc& vd = views[v];
     const int tile = (int)(T - vd.tile_base);
@@ -553,7 +658,7 @@ __global__ void __launch_bounds__(256) k_raster(const G* __restrict__ geoms, con
             uint32_t acc = 0;
             for (int k = 0; k < gs; ++k) {
                 pref[k] = acc;
-                acc += min(ws.counts[(size_t)T * d.nsrc + s0 + k], d.cap);
+                acc += ws.counts[(size_t)T * d.nsrc + s0 + k];
             }
             pref[gs] = acc;
         }
@@ -563,7 +668,7 @@ __global__ void __launch_bounds__(256) k_raster(const G* __restrict__ geoms, con
         for (uint32_t e = tid; e < N; e += 256) {
             int k = 0;
             while (k + 1 < gs && e >= pref[k + 1]) ++k;
-            const uint32_t ij = ws.lists[((size_t)T * d.nsrc + s0 + k) * d.cap + (e - pref[k])];
+            const uint32_t ij = ws.lists[(size_t)ws.offsets[(size_t)T * d.nsrc + s0 + k] + (e - pref[k])];
             raster_quad(gv[s0 + k], ij, e, tx0, ty0, vd.w, vd.h, key1[k], key2[k], flag[k], cached ? &cache : nullptr);
         }
         __syncthreads();
@@ -597,7 +702,8 @@ __global__ void __launch_bounds__(256) k_raster(const G* __restrict__ geoms, con
                         resolve_cached(g, i, j, slot, a, b, cc, cache.tv[slot][e], px, py, c, dep, shape);
                     } else {
                         const uint32_t ij = cached ? cache.ij[e]
-                                                   : ws.lists[((size_t)T * d.nsrc + s) * d.cap + (e - pref[k])];
+                                                   : ws.lists[(size_t)ws.offsets[(size_t)T * d.nsrc + s] +
+                                                              (e - pref[k])];
                         int i, j;
                         unpack_ij(ij, i, j);
                         resolve(g, i, j, slot, px, py, c, dep, shape, vd.w, vd.h);
@@ -644,10 +750,12 @@ static_assert(kCandPool < 0xffff, "candidate links are 16-bit");
 
 // Deferred pixels of one tile and source, in the reference's traversal order.
 // Fast path: the block rasterises the source's bin in parallel and appends
-// every candidate covering a deferred pixel to that pixel's list in LDS; each
-// pixel then folds its own candidates in ascending (quad, slot) order.  When
-// the lists overflow the pool, the bin is sorted in LDS and every deferred
-// pixel scans it in order instead.
+// every candidate covering a deferred pixel of the current pixel range to
+// that pixel's list in LDS; each pixel then folds its own candidates in
+// ascending (quad, slot) order.  When a range's lists overflow the pool, the
+// range is split (down to a single pixel); only a single pixel with more
+// candidates than the pool falls back to sorting the bin in LDS and scanning
+// it in order.
 template <class G>
 __device__ void replay_source(const G& g, const uint32_t* L, uint32_t n, int tx0, int ty0, int out_w, int out_h,
                               bool mine, int px, int py, const uint8_t* dmask, uint32_t* sorted, ReplayEntry* rec,
@@ -655,50 +763,63 @@ __device__ void replay_source(const G& g, const uint32_t* L, uint32_t n, int tx0
     const int tid = threadIdx.x;
     Cand* pool = reinterpret_cast<Cand*>(sorted);
     int* head = reinterpret_cast<int*>(rec);
-    head[tid] = -1;
-    if (tid == 0) *pool_n = 0u;
-    __syncthreads();
-    for (uint32_t e = tid; e < n; e += 256) {
-        const uint32_t q = L[e];
-        int i, j;
-        unpack_ij(q, i, j);
-        quad_tris(g, i, j, out_w, out_h, [&](int k, const Tri& Tr, bool prol) {
-            int xa = max(Tr.xmin, tx0), xb = min(Tr.xmax, tx0 + kTile - 1);
-            int ya = max(Tr.ymin, ty0), yb = min(Tr.ymax, ty0 + kTile - 1);
-            if (is_finite(Tr.A0) && is_finite(Tr.A1) && is_finite(Tr.B0) && is_finite(Tr.B1) &&
-                is_finite(Tr.C0) && is_finite(Tr.C1) && is_finite(Tr.den))
-                tri_tight(Tr, xa, xb, ya, yb);
-            for (int y = ya; y <= yb; ++y)
-                for (int x = xa; x <= xb; ++x) {
-                    const int p = (y - ty0) * kTile + (x - tx0);
-                    if (!dmask[p]) continue;
-                    float n1, n2;
-                    tri_num(Tr, x, y, n1, n2);
-                    if (tri_surely_outside(Tr, n1, n2)) continue;
-                    float l1, l2, l3;
-                    if (!tri_cover(Tr, n1, n2, l1, l2, l3)) continue;
-                    const uint32_t r = atomicAdd(pool_n, 1u);
-                    if (r >= (uint32_t)kCandPool) continue;
-                    const int nx = atomicExch(&head[p], (int)r);
-                    Cand& C = pool[r];
-                    C.ij = q;
-                    C.meta = (uint32_t)(nx < 0 ? 0xffff : nx) | ((uint32_t)k << 16) | ((prol ? 1u : 0u) << 18);
-                    C.d = tri_depth(Tr, l1, l2, l3);
-                    C.tv = Tr.tv;
-                }
-        });
-    }
-    __syncthreads();
+    const int pme = (py - ty0) * kTile + (px - tx0);
     fold_init(fs);
-    if (*pool_n <= (uint32_t)kCandPool) {
-        if (mine) {
-            const int p = (py - ty0) * kTile + (px - tx0);
+    int lo = 0, span = kTile * kTile;
+    bool scan_all = false;
+    while (lo < kTile * kTile) {
+        const int hi = min(lo + span, kTile * kTile);
+        head[tid] = -1;
+        if (tid == 0) *pool_n = 0u;
+        __syncthreads();
+        for (uint32_t e = tid; e < n; e += 256) {
+            const uint32_t q = L[e];
+            int i, j;
+            unpack_ij(q, i, j);
+            quad_tris(g, i, j, out_w, out_h, [&](int k, const Tri& Tr, bool prol) {
+                int xa = max(Tr.xmin, tx0), xb = min(Tr.xmax, tx0 + kTile - 1);
+                int ya = max(Tr.ymin, ty0), yb = min(Tr.ymax, ty0 + kTile - 1);
+                if (is_finite(Tr.A0) && is_finite(Tr.A1) && is_finite(Tr.B0) && is_finite(Tr.B1) &&
+                    is_finite(Tr.C0) && is_finite(Tr.C1) && is_finite(Tr.den))
+                    tri_tight(Tr, xa, xb, ya, yb);
+                for (int y = ya; y <= yb; ++y)
+                    for (int x = xa; x <= xb; ++x) {
+                        const int p = (y - ty0) * kTile + (x - tx0);
+                        if (p < lo || p >= hi || !dmask[p]) continue;
+                        float n1, n2;
+                        tri_num(Tr, x, y, n1, n2);
+                        if (tri_surely_outside(Tr, n1, n2)) continue;
+                        float l1, l2, l3;
+                        if (!tri_cover(Tr, n1, n2, l1, l2, l3)) continue;
+                        const uint32_t r = atomicAdd(pool_n, 1u);
+                        if (r >= (uint32_t)kCandPool) continue;
+                        const int nx = atomicExch(&head[p], (int)r);
+                        Cand& C = pool[r];
+                        C.ij = q;
+                        C.meta = (uint32_t)(nx < 0 ? 0xffff : nx) | ((uint32_t)k << 16) | ((prol ? 1u : 0u) << 18);
+                        C.d = tri_depth(Tr, l1, l2, l3);
+                        C.tv = Tr.tv;
+                    }
+            });
+        }
+        __syncthreads();
+        const bool overflow = *pool_n > (uint32_t)kCandPool;
+        __syncthreads();
+        if (overflow) {
+            if (span > 1) {
+                span = max(1, span / 4);   // retry this range in smaller pieces
+                continue;
+            }
+            scan_all = true;               // one pixel with more candidates than the pool
+            break;
+        }
+        if (mine && pme >= lo && pme < hi) {
             bool first = true;
             uint64_t last = 0;
             while (true) {
                 uint64_t best = ~0ull;
                 int bi = -1;
-                for (int r = head[p]; r >= 0;) {
+                for (int r = head[pme]; r >= 0;) {
                     const Cand& C = pool[r];
                     const uint64_t key = ((uint64_t)C.ij << 2) | ((C.meta >> 16) & 3u);
                     if ((first || key > last) && key < best) { best = key; bi = r; }
@@ -712,8 +833,18 @@ __device__ void replay_source(const G& g, const uint32_t* L, uint32_t n, int tx0
                 first = false;
             }
         }
+        __syncthreads();
+        lo = hi;
+        span = min(span * 2, kTile * kTile);
+    }
+    if (!scan_all) return;
+    if (n > (uint32_t)kFallbackSortCap) {   // cannot be sorted in LDS: reported as a capacity error
+        if (tid == 0) atomicMax(pool_n + 1, n);
         return;
     }
+    // the remaining deferred pixels (from lo on) scan the whole bin in order
+    mine = mine && pme >= lo;
+    if (mine) fold_init(fs);
     // overflow: sort the bin (bitonic, ascending quad code = traversal order) and scan it
     uint32_t np2 = 1;
     while (np2 < n) np2 <<= 1;
@@ -768,9 +899,9 @@ __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, c
     __shared__ uint32_t sorted[kFallbackSortCap];
     __shared__ ReplayEntry rec[kFallbackChunk];
     __shared__ uint8_t dmask[kTile * kTile];
-    __shared__ uint32_t pool_n;
+    __shared__ uint32_t pool_n[2];   // [0] candidate records, [1] bin too large for the sorted scan
     const int tid = threadIdx.x;
-    const uint32_t ndef = ws.status[0];
+    const uint32_t ndef = ws.status[1] ? 0u : ws.status[0];
     for (uint32_t ti = blockIdx.x; ti < ndef; ti += gridDim.x) {
         const uint32_t T = ws.deferred_tiles[ti];
         const int v = find_view(views, d.nviews, T);
@@ -789,16 +920,18 @@ __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, c
         for (int s = 0; s < d.nsrc; ++s) {
             const G& g = gv[s];
             const size_t bin = (size_t)T * d.nsrc + s;
-            const uint32_t n = min(ws.counts[bin], d.cap);
-            if (n > (uint32_t)kFallbackSortCap) {
-                if (tid == 0) atomicMax(&ws.status[3], n);
-                __syncthreads();
+            const uint32_t n = ws.counts[bin];
+            FoldState fs;
+            if (tid == 0) pool_n[1] = 0u;
+            replay_source(g, ws.lists + ws.offsets[bin], n, tx0, ty0, vd.w, vd.h, mine, px, py, dmask, sorted, rec,
+                          pool_n, fs);
+            __syncthreads();
+            const uint32_t too_big = pool_n[1];
+            __syncthreads();   // read by every thread before thread 0 clears it again
+            if (too_big) {
+                if (tid == 0) atomicMax(&ws.status[3], too_big);
                 continue;
             }
-            FoldState fs;
-            replay_source(g, ws.lists + bin * d.cap, n, tx0, ty0, vd.w, vd.h, mine, px, py, dmask, sorted, rec,
-                          &pool_n, fs);
-            __syncthreads();
             if (mine) {
                 float3 c = make_float3(0.f, 0.f, 0.f);
                 float dep = __builtin_inff(), shape = 0.f;

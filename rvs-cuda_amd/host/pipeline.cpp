@@ -25,6 +25,12 @@ void check(int rc, const char* what) {
     }
 }
 
+// Entry-list capacity for a re-run after RVS_E_CAPACITY (needed + 1/8, saturated).
+uint32_t grow_entry_cap(uint32_t needed) {
+    const uint64_t c = (uint64_t)needed + needed / 8 + 1024;
+    return c > 0xffffffffull ? 0xffffffffu : (uint32_t)c;
+}
+
 void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess) {
         std::ostringstream m;
@@ -286,14 +292,13 @@ void SynthetisedViewTriangle::compute(InputView& input, rvs_stream_t stream) {
                                   dst.h, wrap, 1.0f, m_ws.get(), need, cap, getDevColor(), getDevDepth(),
                                   getDevValidity(), &needed, stream);
         if (rc == RVS_E_CAPACITY && needed > 0) {
-            cap = 1;
-            while (cap < needed) cap <<= 1;
+            cap = grow_entry_cap(needed);
             continue;
         }
         check(rc, "rvs_raster");
         return;
     }
-    throw std::runtime_error("rvs_raster: bin capacity could not be satisfied");
+    throw std::runtime_error("rvs_raster: entry capacity could not be satisfied");
 }
 
 // ------------------------------------------------------ BlendedViewSimple
@@ -451,14 +456,13 @@ void Pipeline::computeView(int inputFrame, int virtualFrame, int virtualView) {
         o.color = color.as<float>(); o.quality = quality.as<float>(); o.depth = depth.as<float>();
         o.validity = validity.as<float>(); o.y = y.get(); o.u = u.get(); o.v = v.get();
         for (int attempt = 0;; ++attempt) {
-            const std::size_t need = rvs_synth_workspace_size(srcs.data(), n, &t, m_tile_cap);
+            const std::size_t need = rvs_synth_workspace_size(srcs.data(), n, &t, m_entry_cap);
             if (need == 0) check(RVS_E_INVALID, "rvs_synth_workspace_size");
             m_ws.resize(need);
             uint32_t needed = 0;
-            const int rc = rvs_synthesize(srcs.data(), n, &t, m_ws.get(), need, m_tile_cap, &o, &needed, m_stream);
+            const int rc = rvs_synthesize(srcs.data(), n, &t, m_ws.get(), need, m_entry_cap, &o, &needed, m_stream);
             if (rc == RVS_E_CAPACITY && needed > 0 && attempt < 8) {
-                m_tile_cap = 1;
-                while (m_tile_cap < needed) m_tile_cap <<= 1;
+                m_entry_cap = grow_entry_cap(needed);
                 continue;
             }
             check(rc, "rvs_synthesize");

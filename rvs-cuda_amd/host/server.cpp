@@ -27,6 +27,13 @@ size_t output_bytes(const rvs_target& t) {
 
 size_t yuv_bytes(const rvs_target& t) { return (size_t)t.cam.w * t.cam.h * 3 / 2 * plane_bytes(t); }
 
+// Entry-list capacity of a batch of n views (0 = the library's default).
+uint32_t batch_cap(uint32_t per_view, size_t n) {
+    if (!per_view) return 0u;
+    const uint64_t c = (uint64_t)per_view * n;
+    return c > 0xffffffffull ? 0xffffffffu : (uint32_t)c;
+}
+
 void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
@@ -42,13 +49,14 @@ struct Server::Slot {
     size_t pinned_bytes = 0;
     bool busy = false;
     int scene = 0;
-    uint32_t cap = 0;
+    uint32_t per_view = 0;   // binned entries per view (0: library default)
     std::vector<int> reqs;
     std::vector<rvs_target> tgts;
     std::vector<rvs_view_out> outs;
     std::vector<size_t> host_off;
     size_t ws_bytes = 0;
     size_t ws_off = 0;
+    uint32_t cap = 0;        // entry capacity the current batch was launched with
 };
 
 Server::Server(std::vector<std::vector<rvs_source>> scenes, const rvs_serve_opts& opts)
@@ -69,7 +77,7 @@ Server::Server(std::vector<std::vector<rvs_source>> scenes, const rvs_serve_opts
                                                        : std::min(share / 4, (size_t)1 << 30);
             hip_check(hipHostMalloc((void**)&s->pinned, s->pinned_bytes, 0), "pinned staging");
         }
-        s->cap = m_opt.tile_cap;
+        s->per_view = m_opt.entry_cap;
     }
     m_thread = std::thread([this] { loop(); });
 }
@@ -184,7 +192,7 @@ bool Server::launch(Slot& s) {
             if (r.scene != scene) { ++it; continue; }
             tgts.push_back(r.target);
             const size_t w2 = rvs_synth_batch_workspace_size(S.data(), (int)S.size(), tgts.data(), (int)tgts.size(),
-                                                             s.cap);
+                                                             batch_cap(s.per_view, tgts.size()));
             const size_t o2 = outb + output_bytes(r.target);
             const size_t h2 = hostb + yuv_bytes(r.target);
             const bool fits = w2 > 0 && o2 + w2 <= s.arena_bytes && (!s.pinned || h2 <= s.pinned_bytes);
@@ -231,6 +239,7 @@ bool Server::launch(Slot& s) {
     }
     s.ws_off = o;
     const double t_launch = now_ms();
+    s.cap = batch_cap(s.per_view, tgts.size());
     int rc = rvs_synthesize_batch_async(S.data(), (int)S.size(), tgts.data(), (int)tgts.size(), s.arena + s.ws_off,
                                         ws, s.cap, s.outs.data(), s.stream);
     if (rc == RVS_OK && s.pinned) {
@@ -276,11 +285,12 @@ void Server::finish(Slot& s) {
     int rc = rvs_synth_check(S.data(), (int)S.size(), s.tgts.data(), (int)s.tgts.size(), s.arena + s.ws_off, s.cap,
                              &needed, s.stream);
     if (rc == RVS_E_CAPACITY && needed > 0) {
-        // A bin overflowed: raise this slot's capacity and run the batch again
-        // (the workspace sits after the outputs and grows in place).
-        uint32_t cap = 1;
-        while (cap < needed) cap <<= 1;
-        s.cap = cap;
+        // The entry list overflowed: raise this slot's per-view capacity and run
+        // the batch again (the workspace sits after the outputs and grows in place).
+        const uint64_t nv = s.tgts.size();
+        const uint64_t want = ((uint64_t)needed + needed / 8 + 1024 + nv - 1) / nv;
+        s.per_view = (uint32_t)(want > 0xffffffffull ? 0xffffffffull : want);
+        const uint32_t cap = batch_cap(s.per_view, s.tgts.size());
         const size_t w2 = rvs_synth_batch_workspace_size(S.data(), (int)S.size(), s.tgts.data(),
                                                          (int)s.tgts.size(), cap);
         if (w2 > 0 && s.ws_off + w2 <= s.arena_bytes) {

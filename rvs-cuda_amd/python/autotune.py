@@ -1,9 +1,9 @@
 """Autotuner of the batched and served paths.
 
 Picks, by measurement on the device it runs on:
-  * the bin capacity (tile_cap, quads per (source, tile) bin) from the fill the
-    binning pass reports for a sample batch, with headroom -- it sizes every
-    workspace and hence how many views fit a memory budget;
+  * the binned-entry list capacity per view (entries of (quad, tile) pairs)
+    from what the binning pass reports for a sample batch, with headroom -- it
+    sizes every workspace and hence how many views fit a memory budget;
   * the serving batch size (views per launch set) and the number of batches in
     flight (slots, one stream each) from short closed-loop trials, maximizing
     views/s subject to a p99 latency bound.
@@ -20,9 +20,9 @@ def yuv_bytes(t):
     return t.cam.w * t.cam.h * 3 // 2 * (1 if t.bitdepth <= 8 else 2)
 
 
-def tune_tile_cap(sources, targets, headroom=1.25, floor=64):
-    """Bin capacity for batches like `targets`: next power of two above the
-    largest bin seen times `headroom`."""
+def tune_entry_cap(sources, targets, headroom=1.25):
+    """Binned entries per view for batches like `targets`: the sample's
+    entries per view times `headroom` (rounded up to a multiple of 2^16)."""
     bs = R.BatchSynthesizer(sources, targets)
     bs.run()
     R.profile_enable(True)
@@ -31,10 +31,11 @@ def tune_tile_cap(sources, targets, headroom=1.25, floor=64):
         c = R.profile_counters()
     finally:
         R.profile_enable(False)
-    return max(floor, R._next_pow2(int(c["max_bin"] * headroom) + 1)), c
+    per_view = int(c["bin_entries"] * headroom / len(targets)) + 1
+    return ((per_view + 0xFFFF) >> 16) << 16, c
 
 
-def tune_serving(scenes, requests, memory_budget, tile_cap, p99_bound_ms=50.0, batches=(4, 8, 16, 32, 64),
+def tune_serving(scenes, requests, memory_budget, entry_cap, p99_bound_ms=50.0, batches=(4, 8, 16, 32, 64),
                  slots=(1, 2, 3), trial_batches=4, log=None):
     """Closed-loop trials over (max_batch, slots); each trial serves
     trial_batches * max_batch * slots requests (cycling `requests`) at
@@ -43,7 +44,7 @@ def tune_serving(scenes, requests, memory_budget, tile_cap, p99_bound_ms=50.0, b
     for b, s in itertools.product(batches, slots):
         n = trial_batches * b * s
         reqs = [requests[k % len(requests)] for k in range(n)]
-        srv = S.Server(scenes, memory_budget=memory_budget, max_batch=b, slots=s, tile_cap=tile_cap,
+        srv = S.Server(scenes, memory_budget=memory_budget, max_batch=b, slots=s, entry_cap=entry_cap,
                        host_staging_bytes=b * max(yuv_bytes(t) for _, t in requests))
         try:
             srv.run_closed(reqs[: b * s], concurrency=b * s)   # warm-up

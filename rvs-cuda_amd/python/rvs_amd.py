@@ -263,19 +263,19 @@ def reproject(src_cam, dst_cam, R, t, depth, erp_tab, rescale, uv, vdepth, strea
 
 
 def raster(color, depth, uv, out_w, out_h, wrap, rescale, out_color, out_depth, out_shape, workspace=None,
-           tile_cap=0, stream=None):
+           entry_cap=0, stream=None):
     import torch
     in_h, in_w = depth.shape
     while True:
-        need = hip().rvs_raster_workspace_size(in_w, in_h, out_w, out_h, tile_cap)
+        need = hip().rvs_raster_workspace_size(in_w, in_h, out_w, out_h, entry_cap)
         if workspace is None or workspace.numel() < need:
             workspace = torch.empty(need, dtype=torch.uint8, device=depth.device)
         needed = C.c_uint32(0)
         rc = hip().rvs_raster(_ptr(color), _ptr(depth), _ptr(uv), in_w, in_h, out_w, out_h, int(wrap), rescale,
-                              _ptr(workspace), need, tile_cap, _ptr(out_color), _ptr(out_depth), _ptr(out_shape),
+                              _ptr(workspace), need, entry_cap, _ptr(out_color), _ptr(out_depth), _ptr(out_shape),
                               C.byref(needed), _stream(stream))
         if rc == RVS_E_CAPACITY and needed.value > 0:
-            tile_cap = 1 << int(needed.value - 1).bit_length()
+            entry_cap = grow_cap(needed.value)
             continue
         check(rc)
         return workspace
@@ -379,11 +379,11 @@ class Synthesizer:
     """One-view-at-a-time synthesis (the plain per-view path): owns a workspace
     and output buffers for one target size."""
 
-    def __init__(self, sources, target, tile_cap=0, device="cuda", want_maps=True):
+    def __init__(self, sources, target, entry_cap=0, device="cuda", want_maps=True):
         import torch
         self.sources = sources
         self.target = target
-        self.tile_cap = tile_cap
+        self.entry_cap = entry_cap
         self.device = device
         self.arr = (RvsSource * len(sources))(*[s.struct() for s in sources])
         w, h = target.cam.w, target.cam.h
@@ -402,7 +402,7 @@ class Synthesizer:
     def _alloc_ws(self):
         import torch
         self.ws_bytes = hip().rvs_synth_workspace_size(self.arr, len(self.sources), C.byref(self.target),
-                                                       self.tile_cap)
+                                                       self.entry_cap)
         if self.ws_bytes == 0:
             raise RvsError(-1, hip().rvs_last_error().decode())
         self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=self.device)
@@ -417,16 +417,18 @@ class Synthesizer:
             setattr(vo, k, self.out[k].data_ptr() if self.out[k] is not None else None)
         needed = C.c_uint32(0)
         rc = hip().rvs_synthesize(self.arr, len(self.sources), C.byref(self.target), _ptr(self.ws),
-                                  self.ws_bytes, self.tile_cap, C.byref(vo), C.byref(needed), _stream(stream))
+                                  self.ws_bytes, self.entry_cap, C.byref(vo), C.byref(needed), _stream(stream))
         if rc == RVS_E_CAPACITY and needed.value > 0:
-            cap = 1
-            while cap < needed.value:
-                cap <<= 1
-            self.tile_cap = cap
+            self.entry_cap = grow_cap(needed.value)
             self._alloc_ws()
             return self.run(stream)
         check(rc)
         return self.out
+
+
+def grow_cap(needed):
+    """List capacity for a re-run after RVS_E_CAPACITY: needed entries + 1/8, as uint32."""
+    return min(int(needed) + int(needed) // 8 + 1024, 0xFFFFFFFF)
 
 
 def _next_pow2(n):
@@ -440,11 +442,11 @@ class BatchSynthesizer:
     """Many target views from the same resident sources in one launch set
     (rvs_synthesize_batch).  targets: list of RvsTarget (sizes may differ)."""
 
-    def __init__(self, sources, targets, tile_cap=0, device="cuda", want_maps=False):
+    def __init__(self, sources, targets, entry_cap=0, device="cuda", want_maps=False):
         import torch
         self.sources = sources
         self.arr = (RvsSource * len(sources))(*[s.struct() for s in sources])
-        self.tile_cap = tile_cap
+        self.entry_cap = entry_cap
         self.device = device
         self.set_targets(targets, want_maps)
 
@@ -472,7 +474,7 @@ class BatchSynthesizer:
     def _alloc_ws(self):
         import torch
         self.ws_bytes = hip().rvs_synth_batch_workspace_size(self.arr, len(self.sources), self.targets, self.n,
-                                                             self.tile_cap)
+                                                             self.entry_cap)
         if self.ws_bytes == 0:
             raise RvsError(-1, hip().rvs_last_error().decode())
         self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=self.device)
@@ -484,9 +486,9 @@ class BatchSynthesizer:
     def run(self, stream=None):
         needed = C.c_uint32(0)
         rc = hip().rvs_synthesize_batch(self.arr, len(self.sources), self.targets, self.n, _ptr(self.ws),
-                                        self.ws_bytes, self.tile_cap, self.vo, C.byref(needed), _stream(stream))
+                                        self.ws_bytes, self.entry_cap, self.vo, C.byref(needed), _stream(stream))
         if rc == RVS_E_CAPACITY and needed.value > 0:
-            self.tile_cap = _next_pow2(needed.value)
+            self.entry_cap = grow_cap(needed.value)
             self._alloc_ws()
             return self.run(stream)
         check(rc)
@@ -495,12 +497,12 @@ class BatchSynthesizer:
     def enqueue(self, stream=None):
         """Enqueue only (capturable in a HIP graph); call check() afterwards."""
         check(hip().rvs_synthesize_batch_async(self.arr, len(self.sources), self.targets, self.n, _ptr(self.ws),
-                                               self.ws_bytes, self.tile_cap, self.vo, _stream(stream)))
+                                               self.ws_bytes, self.entry_cap, self.vo, _stream(stream)))
 
     def check(self, stream=None):
-        """Returns 0, or the tile capacity needed when a bin overflowed."""
+        """Returns 0, or the entry capacity needed when the binned entries overflowed."""
         needed = C.c_uint32(0)
-        rc = hip().rvs_synth_check(self.arr, len(self.sources), self.targets, self.n, _ptr(self.ws), self.tile_cap,
+        rc = hip().rvs_synth_check(self.arr, len(self.sources), self.targets, self.n, _ptr(self.ws), self.entry_cap,
                                    C.byref(needed), _stream(stream))
         if rc == RVS_E_CAPACITY and needed.value > 0:
             return int(needed.value)

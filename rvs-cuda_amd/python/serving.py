@@ -10,7 +10,7 @@ import rvs_amd as R
 
 
 class RvsServeOpts(C.Structure):
-    _fields_ = [("memory_budget", C.c_size_t), ("max_batch", C.c_int), ("slots", C.c_int), ("tile_cap", C.c_uint32),
+    _fields_ = [("memory_budget", C.c_size_t), ("max_batch", C.c_int), ("slots", C.c_int), ("entry_cap", C.c_uint32),
                 ("copy_to_host", C.c_int), ("keep_outputs", C.c_int), ("host_staging_bytes", C.c_size_t)]
 
 
@@ -52,14 +52,14 @@ def _stats_dict(st):
 class Server:
     """scenes: list of lists of rvs_amd.DeviceSource (kept alive by the server)."""
 
-    def __init__(self, scenes, memory_budget, max_batch=64, slots=2, tile_cap=0, copy_to_host=True,
+    def __init__(self, scenes, memory_budget, max_batch=64, slots=2, entry_cap=0, copy_to_host=True,
                  keep_outputs=False, host_staging_bytes=0):
         self._scenes = scenes
         self._arrs = [(R.RvsSource * len(sc))(*[s.struct() for s in sc]) for sc in scenes]
         n = len(scenes)
         ptrs = (C.POINTER(R.RvsSource) * n)(*[C.cast(a, C.POINTER(R.RvsSource)) for a in self._arrs])
         counts = (C.c_int * n)(*[len(sc) for sc in scenes])
-        self.opts = RvsServeOpts(int(memory_budget), int(max_batch), int(slots), int(tile_cap), int(copy_to_host),
+        self.opts = RvsServeOpts(int(memory_budget), int(max_batch), int(slots), int(entry_cap), int(copy_to_host),
                                  int(keep_outputs), int(host_staging_bytes))
         self.h = _lib().rvs_server_create(ptrs, counts, n, C.byref(self.opts))
         if not self.h:

@@ -228,3 +228,28 @@ def test_perspective_grid_sources(torch_cuda):
     out = R.Synthesizer(srcs, tgt, want_maps=False).run()
     torch.cuda.synchronize()
     _check_oracle(out, _oracle(rg, planes, tgt), tgt.bitdepth, "persp11")
+
+
+def test_dense_sources_compact_bins(torch_cuda):
+    """Sources far denser than the target (eleven 512x512 views into 128x72):
+    thousands of quads per (tile, source) bin, exact against the oracle; the
+    entry list grows on demand from a deliberately small capacity."""
+    torch = torch_cuda
+    say("case dense bins")
+    rg = W.rig("persp11_2048", scale=4)
+    planes = W.render_sources(rg)
+    srcs = W.device_sources(rg, planes)
+    t = rg["target"]
+    tgt = R.make_target(W.perspective_cam(128, 72), [0.01, 0.02, 0.0], [3.0, 1.0, 0.0], t["bitdepth"],
+                        t["blending_exp"], R.scale_of(10), R.scale_of(16))
+    R.profile_enable(True)
+    try:
+        syn = R.Synthesizer(srcs, tgt, entry_cap=4096, want_maps=False)
+        out = syn.run()
+        torch.cuda.synchronize()
+        cnt = R.profile_counters()
+    finally:
+        R.profile_enable(False)
+    say("  largest bin %d, entries %d, capacity grown to %d" % (cnt["max_bin"], cnt["bin_entries"], syn.entry_cap))
+    assert cnt["max_bin"] > 2048 and syn.entry_cap >= cnt["bin_entries"]
+    _check_oracle(out, _oracle(rg, planes, tgt), tgt.bitdepth, "dense")
