@@ -328,41 +328,45 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
         const ViewDesc& vd = views[v];
         const int i0 = (int)(item / d.bins_x) * kBinQY, j0 = (int)(item % d.bins_x) * kBinQX;
         const int i = i0 + tid / kBinQX, j = j0 + tid % kBinQX;
-        const bool stored = wi < d.range_cap;
-        uint32_t r = kRangeNone;
-        if (FILL && stored) r = ws.ranges[(size_t)wi * 256 + tid];
+        const bool stored = wi < d.range_cap;   // same for the whole block
         bool more = false;
         int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
-        if (!FILL || __syncthreads_or(!stored || r == kRangeBig)) {
+        auto quad_range = [&](auto vtx) {
+            Quad Q;
+            quad_build(Q, g, i, j, vtx, vd.w, vd.h, false);
+            if (!Q.alive) return;
+            int xmin = INT_MAX, xmax = INT_MIN, ymin = INT_MAX, ymax = INT_MIN;
+            for (int k = 0; k < 4; ++k)
+                if (Q.alive & (1u << k)) {
+                    xmin = min(xmin, Q.t[k].xmin); xmax = max(xmax, Q.t[k].xmax);
+                    ymin = min(ymin, Q.t[k].ymin); ymax = max(ymax, Q.t[k].ymax);
+                }
+            tx0 = xmin / kTile; tx1 = xmax / kTile; ty0 = ymin / kTile; ty1 = ymax / kTile;
+            more = true;
+        };
+        if (FILL && stored) {
+            const uint32_t r = ws.ranges[(size_t)wi * 256 + tid];
+            if (r == kRangeBig) {
+                // a range too wide to pack: this quad's vertices again
+                quad_range([&](int ii, int jj) -> float3 { return vertex(g, ii, jj); });
+            } else if (r != kRangeNone) {
+                unpack_range(r, tx0, tx1, ty0, ty1);
+                more = true;
+            }
+        } else {
             for (int k = tid; k < (kBinQY + 1) * (kBinQX + 1); k += 256) {
                 const int rr = k / (kBinQX + 1), c = k % (kBinQX + 1);
                 const int ii = i0 + rr, jj = j0 + c;
                 if (ii < g.sh && jj < g.sw) V[rr][c] = vertex(g, ii, jj);
             }
             __syncthreads();
-            if (i < g.sh - 1 && j < g.sw - 1) {
-                Quad Q;
-                auto vtx = [&](int ii, int jj) -> float3 {
+            if (i < g.sh - 1 && j < g.sw - 1)
+                quad_range([&](int ii, int jj) -> float3 {
                     const int rr = ii - i0, c = jj - j0;
                     if (rr >= 0 && rr <= kBinQY && c >= 0 && c <= kBinQX) return V[rr][c];
                     return vertex(g, ii, jj);
-                };
-                quad_build(Q, g, i, j, vtx, vd.w, vd.h, false);
-                if (Q.alive) {
-                    int xmin = INT_MAX, xmax = INT_MIN, ymin = INT_MAX, ymax = INT_MIN;
-                    for (int k = 0; k < 4; ++k)
-                        if (Q.alive & (1u << k)) {
-                            xmin = min(xmin, Q.t[k].xmin); xmax = max(xmax, Q.t[k].xmax);
-                            ymin = min(ymin, Q.t[k].ymin); ymax = max(ymax, Q.t[k].ymax);
-                        }
-                    tx0 = xmin / kTile; tx1 = xmax / kTile; ty0 = ymin / kTile; ty1 = ymax / kTile;
-                    more = true;
-                }
-            }
+                });
             if (!FILL && stored) ws.ranges[(size_t)wi * 256 + tid] = more ? pack_range(tx0, tx1, ty0, ty1) : kRangeNone;
-        } else if (r != kRangeNone) {
-            unpack_range(r, tx0, tx1, ty0, ty1);
-            more = true;
         }
         const uint32_t q = ((uint32_t)i << 16) | (uint32_t)j;
         int tx = tx0, ty = ty0;
