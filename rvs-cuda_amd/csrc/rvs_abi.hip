@@ -116,7 +116,7 @@ struct Layout {
 // deferred-pixel flags, deferred-tile list and binning work list.
 struct RasterPlan {
     rvs_k::BatchDims d;
-    size_t status, counts, offsets, fill, lists, ranges, bsum, deferred, dtiles, work;
+    size_t status, counts, single, multi, offsets, fill, lists, ranges, bsum, deferred, dtiles, work;
 };
 
 // List capacity: the caller's, or one entry per output pixel and source
@@ -136,16 +136,18 @@ void plan_raster(Layout& L, RasterPlan& R, int nsrc, int nviews, uint32_t ntiles
     d.bins_x = (uint32_t)((max_sw - 1 + rvs_k::kBinQX - 1) / rvs_k::kBinQX);
     d.bins_y = (uint32_t)((max_sh - 1 + rvs_k::kBinQY - 1) / rvs_k::kBinQY);
     const uint64_t items = (uint64_t)d.bins_x * d.bins_y * nsrc * nviews;
-    const uint64_t keep = items / 4 > 4096 ? items / 4 : 4096;
+    const uint64_t keep = items / 8 > 4096 ? items / 8 : 4096;
     d.range_cap = (uint32_t)(keep < items ? keep : items);
     const size_t nbins = (size_t)nsrc * ntiles;
     R.status = L.take(16 * sizeof(uint32_t));
     R.counts = L.take(nbins * sizeof(uint32_t));
+    R.single = L.take(nbins * sizeof(uint32_t));
+    R.multi = L.take(nbins * sizeof(uint32_t));
     R.offsets = L.take(nbins * sizeof(uint32_t));
     R.fill = L.take(nbins * sizeof(uint32_t));
     R.bsum = L.take(rvs_k::scan_blocks(nbins) * sizeof(unsigned long long));
     R.lists = L.take((size_t)d.cap * sizeof(uint32_t));
-    R.ranges = L.take((size_t)d.range_cap * 256 * sizeof(uint32_t));
+    R.ranges = L.take((size_t)d.range_cap * 256 * sizeof(uint2));
     R.deferred = L.take(npx);
     R.dtiles = L.take((size_t)ntiles * sizeof(uint32_t));
     R.work = L.take((size_t)items * sizeof(uint32_t));
@@ -155,11 +157,13 @@ rvs_k::RasterWS carve_raster(char* base, const RasterPlan& R) {
     rvs_k::RasterWS ws;
     ws.status = (uint32_t*)(base + R.status);
     ws.counts = (uint32_t*)(base + R.counts);
+    ws.single = (uint32_t*)(base + R.single);
+    ws.multi = (uint32_t*)(base + R.multi);
     ws.offsets = (uint32_t*)(base + R.offsets);
     ws.fill = (uint32_t*)(base + R.fill);
     ws.bsum = (unsigned long long*)(base + R.bsum);
     ws.lists = (uint32_t*)(base + R.lists);
-    ws.ranges = (uint32_t*)(base + R.ranges);
+    ws.ranges = (uint2*)(base + R.ranges);
     ws.deferred = (uint8_t*)(base + R.deferred);
     ws.deferred_tiles = (uint32_t*)(base + R.dtiles);
     ws.work = (uint32_t*)(base + R.work);
@@ -168,7 +172,8 @@ rvs_k::RasterWS carve_raster(char* base, const RasterPlan& R) {
 
 hipError_t reset_raster(const rvs_k::RasterWS& ws, const rvs_k::BatchDims& d, hipStream_t s) {
     hipError_t e = rvs_k::launch_reset_u32(ws.status, 16, s);
-    if (e == hipSuccess) e = rvs_k::launch_reset_u32(ws.counts, (size_t)d.nsrc * d.ntiles, s);
+    if (e == hipSuccess) e = rvs_k::launch_reset_u32(ws.single, (size_t)d.nsrc * d.ntiles, s);
+    if (e == hipSuccess) e = rvs_k::launch_reset_u32(ws.multi, (size_t)d.nsrc * d.ntiles, s);
     if (e == hipSuccess) e = rvs_k::launch_reset_u32(ws.fill, (size_t)d.nsrc * d.ntiles, s);
     return e;
 }

@@ -47,17 +47,19 @@ struct ViewDesc {
 
 // Device-side state of one launch set.
 struct RasterWS {
-    uint32_t* counts;        // [global tile][source] entries per bin
+    uint32_t* counts;        // [global tile][source] entries per bin (written by the scan)
+    uint32_t* single;        // [global tile][source] single-tile quads of kept ranges (counting pass)
+    uint32_t* multi;         // [global tile][source] all other entries (counting pass)
     uint32_t* offsets;       // [global tile][source] first entry of each bin in lists
-    uint32_t* fill;          // [global tile][source] fill cursors
+    uint32_t* fill;          // [global tile][source] fill cursors of the multi-tile region
     uint32_t* lists;         // [cap] quad codes (row << 16 | column), bins back to back
-    uint32_t* ranges;        // [range_cap work items][256] packed tile range per quad
+    uint2* ranges;           // [range_cap work items][256] packed tile range + single-tile slot
     unsigned long long* bsum;// scan block sums
     uint8_t* deferred;       // [global pixel] pixel handled by the in-order replay
     uint32_t* deferred_tiles;// [global tile]
     uint32_t* work;          // binning work items (view, source, block)
     uint32_t* status;        // [0] n_deferred, [1] list overflow, [2] entries (saturated), [3] replay
-                             // overflow, [4] work items, [5] count cursor, [6] fill cursor
+                             // overflow, [4] work items
 };
 
 struct BatchDims {

@@ -282,11 +282,18 @@ __global__ void k_bin_cull(const G* __restrict__ geoms, const ViewDesc* __restri
 // Bins are compact: a counting pass (k_bin<COUNT>) finds every surviving
 // quad's tile range and counts the (quad, tile) entries per (tile, source)
 // bin; an exclusive scan turns the counts into bin offsets; a fill pass
-// (k_bin<FILL>) writes the quad codes.  The counting pass stores the tile
-// range of each quad of the first range_cap work items (4 bytes per quad);
-// later items re-evaluate their vertices in the fill pass.  When the entries
-// exceed the list capacity the fill pass and the raster are skipped and the
-// needed capacity is reported.
+// (k_bin<FILL>) writes the quad codes.  For the first range_cap work items the
+// counting pass keeps each quad's tile range and, for a quad touching a single
+// tile (most of them), its slot in that bin, so the fill pass writes it
+// without atomics; quads spanning several tiles take slots after the
+// single-tile ones through fill cursors.  Later items re-evaluate their
+// vertices in the fill pass.  When the entries exceed the list capacity the
+// fill pass and the raster are skipped and the needed capacity is reported.
+//
+// Each block first gathers its entries per bin in an LDS table (a few to a
+// few dozen distinct tiles per 64x4 quad block), then takes one global atomic
+// per (bin, kind): device-scope atomics are performed at the memory side, so
+// their number and the chains of dependent ones set this kernel's time.
 
 constexpr uint32_t kRangeNone = 0xffffffffu;   // no surviving triangle
 constexpr uint32_t kRangeBig = 0xfffffffeu;    // range too wide to pack: re-evaluated
@@ -304,23 +311,51 @@ __device__ inline void unpack_range(uint32_t r, int& tx0, int& tx1, int& ty0, in
     ty1 = ty0 + (int)(r >> 28);
 }
 
+constexpr int kBinTable = 128;                 // LDS bin table entries per block (power of two)
+constexpr uint32_t kTableEmpty = 0xffffffffu;
+
+// Slot of global tile t in the block's LDS table, inserting it; -1 when the
+// table is full of other tiles (keys are never removed while an item runs,
+// so a tile refused once stays absent and find_slot agrees).
+__device__ inline int table_slot(uint32_t* keys, uint32_t t) {
+    uint32_t h = (t * 2654435761u) >> 25;
+    for (int probe = 0; probe < kBinTable; ++probe) {
+        const uint32_t k = keys[h];
+        if (k == t) return (int)h;
+        if (k == kTableEmpty) {
+            const uint32_t old = atomicCAS(&keys[h], kTableEmpty, t);
+            if (old == kTableEmpty || old == t) return (int)h;
+        }
+        h = (h + 1) & (kBinTable - 1);
+    }
+    return -1;
+}
+
+__device__ inline int find_slot(const uint32_t* keys, uint32_t t) {
+    uint32_t h = (t * 2654435761u) >> 25;
+    for (int probe = 0; probe < kBinTable; ++probe) {
+        const uint32_t k = keys[h];
+        if (k == t) return (int)h;
+        if (k == kTableEmpty) return -1;
+        h = (h + 1) & (kBinTable - 1);
+    }
+    return -1;
+}
+
 template <class G, bool FILL>
 __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
                                              BatchDims d, RasterWS ws) {
     __shared__ float3 V[kBinQY + 1][kBinQX + 1];
-    __shared__ uint32_t s_item;
+    __shared__ uint32_t t_key[kBinTable];
+    __shared__ uint32_t t_n[2][kBinTable];      // entries per bin: [0] multi / fill, [1] single-tile
+    __shared__ uint32_t t_base[2][kBinTable];   // [0] fill write cursor, [1] first single-tile slot
     const int tid = threadIdx.x;
-    const int lane = __lane_id();
-    const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     const uint32_t per = d.bins_x * d.bins_y;
     if (FILL && ws.status[1]) return;   // list capacity exceeded: nothing to fill
     const uint32_t nwork = ws.status[4];
-    while (true) {
-        if (tid == 0) s_item = atomicAdd(&ws.status[FILL ? 6 : 5], 1u);
-        __syncthreads();
-        const uint32_t wi = s_item;
-        __syncthreads();
-        if (wi >= nwork) return;
+    // Work items are alike in cost: a static stride over them keeps every
+    // block busy without a shared work counter.
+    for (uint32_t wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
         const uint32_t code = ws.work[wi];
         const uint32_t item = code % per, vs = code / per;
         const int s = (int)(vs % (uint32_t)d.nsrc), v = (int)(vs / (uint32_t)d.nsrc);
@@ -329,6 +364,11 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
         const int i0 = (int)(item / d.bins_x) * kBinQY, j0 = (int)(item % d.bins_x) * kBinQX;
         const int i = i0 + tid / kBinQX, j = j0 + tid % kBinQX;
         const bool stored = wi < d.range_cap;   // same for the whole block
+        const uint32_t q = ((uint32_t)i << 16) | (uint32_t)j;
+        if (tid < kBinTable) {
+            t_key[tid] = kTableEmpty;
+            t_n[0][tid] = 0; t_n[1][tid] = 0;
+        }
         bool more = false;
         int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
         auto quad_range = [&](auto vtx) {
@@ -345,13 +385,20 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
             more = true;
         };
         if (FILL && stored) {
-            const uint32_t r = ws.ranges[(size_t)wi * 256 + tid];
-            if (r == kRangeBig) {
+            __syncthreads();   // table cleared
+            const uint2 r = ws.ranges[(size_t)wi * 256 + tid];
+            if (r.x == kRangeBig) {
                 // a range too wide to pack: this quad's vertices again
                 quad_range([&](int ii, int jj) -> float3 { return vertex(g, ii, jj); });
-            } else if (r != kRangeNone) {
-                unpack_range(r, tx0, tx1, ty0, ty1);
-                more = true;
+            } else if (r.x != kRangeNone) {
+                unpack_range(r.x, tx0, tx1, ty0, ty1);
+                if (tx0 == tx1 && ty0 == ty1) {
+                    // single tile: its slot was taken in the counting pass
+                    const size_t bin = (size_t)(vd.tile_base + (uint32_t)(ty0 * vd.tiles_x + tx0)) * d.nsrc + s;
+                    ws.lists[(size_t)ws.offsets[bin] + r.y] = q;
+                } else {
+                    more = true;
+                }
             }
         } else {
             for (int k = tid; k < (kBinQY + 1) * (kBinQX + 1); k += 256) {
@@ -359,35 +406,67 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
                 const int ii = i0 + rr, jj = j0 + c;
                 if (ii < g.sh && jj < g.sw) V[rr][c] = vertex(g, ii, jj);
             }
-            __syncthreads();
+            __syncthreads();   // vertices loaded, table cleared
             if (i < g.sh - 1 && j < g.sw - 1)
                 quad_range([&](int ii, int jj) -> float3 {
                     const int rr = ii - i0, c = jj - j0;
                     if (rr >= 0 && rr <= kBinQY && c >= 0 && c <= kBinQX) return V[rr][c];
                     return vertex(g, ii, jj);
                 });
-            if (!FILL && stored) ws.ranges[(size_t)wi * 256 + tid] = more ? pack_range(tx0, tx1, ty0, ty1) : kRangeNone;
         }
-        const uint32_t q = ((uint32_t)i << 16) | (uint32_t)j;
-        int tx = tx0, ty = ty0;
-        // Wave-aggregated: one atomic per distinct tile per round.
-        while (__any(more)) {
-            const uint32_t t = more ? vd.tile_base + (uint32_t)(ty * vd.tiles_x + tx) : 0xffffffffu;
-            const uint64_t act = __ballot(more);
-            const int leader = __ffsll((unsigned long long)act) - 1;
-            const uint32_t tl = __shfl(t, leader);
-            const bool mine = more && t == tl;
-            const uint64_t m = __ballot(mine);
-            const size_t bin = (size_t)tl * d.nsrc + s;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(FILL ? &ws.fill[bin] : &ws.counts[bin], (uint32_t)__popcll(m));
-            base = __shfl(base, leader);
-            if (mine) {
-                if (FILL) ws.lists[(size_t)ws.offsets[bin] + base + (uint32_t)__popcll(m & lt_mask)] = q;
-                if (++tx > tx1) { tx = tx0; if (++ty > ty1) more = false; }
+        auto tile_of = [&](int tx, int ty) { return vd.tile_base + (uint32_t)(ty * vd.tiles_x + tx); };
+        auto bin_of = [&](uint32_t t) { return (size_t)t * d.nsrc + s; };
+        // A stored single-tile quad takes a slot of its bin's single-tile
+        // region in the counting pass; every other entry is a multi entry.
+        const bool single = !FILL && stored && more && tx0 == tx1 && ty0 == ty1;
+        int my_slot = -1;
+        uint32_t rank = 0;
+        if (single) {
+            const uint32_t t = tile_of(tx0, ty0);
+            my_slot = table_slot(t_key, t);
+            if (my_slot >= 0) rank = atomicAdd(&t_n[1][my_slot], 1u);
+            else rank = atomicAdd(&ws.single[bin_of(t)], 1u);   // table full: final slot now
+        } else if (more) {
+            for (int ty = ty0; ty <= ty1; ++ty)
+                for (int tx = tx0; tx <= tx1; ++tx) {
+                    const uint32_t t = tile_of(tx, ty);
+                    const int k = table_slot(t_key, t);
+                    if (k >= 0) {
+                        atomicAdd(&t_n[0][k], 1u);
+                    } else if (FILL) {
+                        const size_t bin = bin_of(t);
+                        ws.lists[(size_t)ws.offsets[bin] + ws.single[bin] + atomicAdd(&ws.fill[bin], 1u)] = q;
+                    } else {
+                        atomicAdd(&ws.multi[bin_of(t)], 1u);
+                    }
+                }
+        }
+        __syncthreads();   // table complete
+        if (tid < kBinTable && t_key[tid] != kTableEmpty) {
+            const size_t bin = bin_of(t_key[tid]);
+            if (FILL) {
+                const uint32_t n = t_n[0][tid];
+                if (n) t_base[0][tid] = ws.offsets[bin] + ws.single[bin] + atomicAdd(&ws.fill[bin], n);
+            } else {
+                if (t_n[1][tid]) t_base[1][tid] = atomicAdd(&ws.single[bin], t_n[1][tid]);
+                if (t_n[0][tid]) atomicAdd(&ws.multi[bin], t_n[0][tid]);
             }
         }
-        __syncthreads();
+        __syncthreads();   // bases known
+        if (FILL && more && !single) {
+            for (int ty = ty0; ty <= ty1; ++ty)
+                for (int tx = tx0; tx <= tx1; ++tx) {
+                    const int k = find_slot(t_key, tile_of(tx, ty));
+                    if (k >= 0) ws.lists[atomicAdd(&t_base[0][k], 1u)] = q;
+                }
+        }
+        if (!FILL && stored) {
+            uint2 r;
+            r.x = (tx1 >= tx0) ? pack_range(tx0, tx1, ty0, ty1) : kRangeNone;
+            r.y = (my_slot >= 0) ? t_base[1][my_slot] + rank : rank;
+            ws.ranges[(size_t)wi * 256 + tid] = r;
+        }
+        __syncthreads();   // LDS free for the next item
     }
 }
 
@@ -396,15 +475,17 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
 // it exceeds the list capacity).
 constexpr int kScanPer = 2048;   // bins per scan block (256 threads x 8)
 
-__global__ void __launch_bounds__(256) k_scan_blocks(const uint32_t* __restrict__ counts, uint32_t* offsets,
-                                                     size_t n, unsigned long long* bsum) {
+__global__ void __launch_bounds__(256) k_scan_blocks(const uint32_t* __restrict__ single,
+                                                     const uint32_t* __restrict__ multi, uint32_t* counts,
+                                                     uint32_t* offsets, size_t n, unsigned long long* bsum) {
     __shared__ unsigned long long part[256];
     const size_t base = (size_t)blockIdx.x * kScanPer + (size_t)threadIdx.x * 8;
     uint32_t c[8];
     unsigned long long sum = 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        c[k] = base + k < n ? counts[base + k] : 0u;
+        c[k] = base + k < n ? single[base + k] + multi[base + k] : 0u;
+        if (base + k < n) counts[base + k] = c[k];
         sum += c[k];
     }
     part[threadIdx.x] = sum;
@@ -469,7 +550,8 @@ hipError_t launch_bin(const G* geoms, const ViewDesc* views, const BatchDims& d,
     hipLaunchKernelGGL((k_bin<G, false>), dim3((unsigned)blocks), dim3(256), 0, s, geoms, views, d, ws);
     const size_t nbins = (size_t)d.ntiles * d.nsrc;
     const size_t nb = scan_blocks(nbins);
-    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(256), 0, s, ws.counts, ws.offsets, nbins, ws.bsum);
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(256), 0, s, ws.single, ws.multi, ws.counts,
+                       ws.offsets, nbins, ws.bsum);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, ws.bsum, (uint32_t)nb, d.cap, ws.status);
     hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((nbins + 255) / 256)), dim3(256), 0, s, ws.offsets, nbins,
                        ws.bsum);
