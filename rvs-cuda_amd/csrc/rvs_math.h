@@ -7,10 +7,12 @@
 //   * float -> int is cvttss2si: truncation, INT_MIN when out of range or NaN
 //     (transform.cpp:90-93 feeds floor/ceil of projected coordinates through it).
 //   * cv::norm accumulates squares in double (transform.cpp:63-65,
-//     EquirectangularProjector.cpp:80).  norm_sq_sum_d() returns the exact float
-//     result of that double computation using float-float arithmetic, with an
+//     EquirectangularProjector.cpp:80).  norm_sq_sum_d() returns the float
+//     result of that double computation: on the GPU with FP64 itself (gfx950
+//     issues FP64 adds and FMAs at the unpacked FP32 rate, and llvm.sqrt.f64
+//     is correctly rounded), on the host with float-float arithmetic plus an
 //     exact integer path for the rare inputs whose result lies too close to a
-//     float rounding midpoint to be decided in float-float.  No FP64 is used.
+//     float rounding midpoint (norm_sq_sum_ff, checked against double on the CPU).
 // All code here must be compiled with -ffp-contract=off.
 #ifndef RVS_MATH_H
 #define RVS_MATH_H
@@ -237,8 +239,9 @@ RVS_HD void two_sum(float a, float b, float& s, float& e) {
 
 }  // namespace detail
 
-// (float)sqrt(((0.0 + (double)a*a) + (double)b*b) + (double)c*c), exactly.
-RVS_HD float norm_sq_sum_d(float a, float b, float c) {
+// (float)sqrt(((0.0 + (double)a*a) + (double)b*b) + (double)c*c), exactly,
+// in float-float arithmetic.
+RVS_HD float norm_sq_sum_ff(float a, float b, float c) {
     using namespace detail;
     float aa = fabs_(a), ab = fabs_(b), ac = fabs_(c);
     if (is_nan(a) || is_nan(b) || is_nan(c)) return a + b + c;  // NaN
@@ -286,6 +289,21 @@ RVS_HD float norm_sq_sum_d(float a, float b, float c) {
         return r;
     }
     return norm_slow(a, b, c);
+}
+
+// (float)sqrt(((0.0 + (double)a*a) + (double)b*b) + (double)c*c).  The
+// squares of floats are exact in double, so only the two sums, the square
+// root and the final conversion round, as in the reference.
+RVS_HD float norm_sq_sum_d(float a, float b, float c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double s = 0.0;
+    s += (double)a * (double)a;
+    s += (double)b * (double)b;
+    s += (double)c * (double)c;
+    return (float)__builtin_sqrt(s);
+#else
+    return norm_sq_sum_ff(a, b, c);
+#endif
 }
 
 }  // namespace rvs_math
