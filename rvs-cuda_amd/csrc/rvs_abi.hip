@@ -112,11 +112,12 @@ struct Layout {
 };
 
 // Raster state of one launch set: status words, compact bins (counts,
-// offsets, fill cursors, entry list, scan sums, kept tile ranges),
+// offsets, fill cursors, entry list, scan sums, kept tile ranges and vertex
+// blocks with their item map),
 // deferred-pixel flags, deferred-tile list and binning work list.
 struct RasterPlan {
     rvs_k::BatchDims d;
-    size_t status, counts, single, multi, offsets, fill, lists, ranges, bsum, deferred, dtiles, work;
+    size_t status, counts, single, multi, offsets, fill, lists, ranges, vbuf, imap, bsum, deferred, dtiles, work;
 };
 
 // List capacity: the caller's, or one entry per output pixel and source
@@ -148,6 +149,8 @@ void plan_raster(Layout& L, RasterPlan& R, int nsrc, int nviews, uint32_t ntiles
     R.bsum = L.take(rvs_k::scan_blocks(nbins) * sizeof(unsigned long long));
     R.lists = L.take((size_t)d.cap * sizeof(uint32_t));
     R.ranges = L.take((size_t)d.range_cap * 256 * sizeof(uint2));
+    R.vbuf = L.take((size_t)d.range_cap * 3 * rvs_k::kBinVerts * sizeof(float));
+    R.imap = L.take((size_t)items * sizeof(uint32_t));
     R.deferred = L.take(npx);
     R.dtiles = L.take((size_t)ntiles * sizeof(uint32_t));
     R.work = L.take((size_t)items * sizeof(uint32_t));
@@ -164,6 +167,8 @@ rvs_k::RasterWS carve_raster(char* base, const RasterPlan& R) {
     ws.bsum = (unsigned long long*)(base + R.bsum);
     ws.lists = (uint32_t*)(base + R.lists);
     ws.ranges = (uint2*)(base + R.ranges);
+    ws.vbuf = (float*)(base + R.vbuf);
+    ws.imap = (uint32_t*)(base + R.imap);
     ws.deferred = (uint8_t*)(base + R.deferred);
     ws.deferred_tiles = (uint32_t*)(base + R.dtiles);
     ws.work = (uint32_t*)(base + R.work);

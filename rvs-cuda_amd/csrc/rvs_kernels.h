@@ -25,6 +25,7 @@ constexpr uint32_t kDefaultEntriesPerPxSrcQ = 4;
 constexpr int kFallbackChunk = 128;  // list entries per LDS chunk in the in-order replay
 constexpr int kFallbackSortCap = 8192;  // longest (source, tile) list the replay sorts in LDS
 constexpr int kMaxViews = 4096;
+constexpr int kBinVerts = (kBinQY + 1) * (kBinQX + 1);   // vertices of one binning work item
 constexpr int kBinBlocks = 2048;    // persistent binning blocks
 constexpr int kRasterGroup = 2;     // sources rasterised together per k_raster phase
 constexpr int kRasterCache = 448;   // binned quads per phase whose coverage-pass results stay in LDS
@@ -54,6 +55,8 @@ struct RasterWS {
     uint32_t* fill;          // [global tile][source] fill cursors of the multi-tile region
     uint32_t* lists;         // [cap] quad codes (row << 16 | column), bins back to back
     uint2* ranges;           // [range_cap work items][256] packed tile range + single-tile slot
+    float* vbuf;             // [range_cap work items][kBinQY + 1][kBinQX + 1][3] vertices (u', v', depth)
+    uint32_t* imap;          // [view][source][item] work index of a kept item, ~0 otherwise
     unsigned long long* bsum;// scan block sums
     uint8_t* deferred;       // [global pixel] pixel handled by the in-order replay
     uint32_t* deferred_tiles;// [global tile]
@@ -67,7 +70,7 @@ struct BatchDims {
     uint32_t ntiles;         // all views
     uint32_t npx;            // all views
     uint32_t cap;            // list capacity (binned (quad, tile) entries of the launch set)
-    uint32_t range_cap;      // work items whose quad tile ranges are kept between the passes
+    uint32_t range_cap;      // work items whose tile ranges and vertices are kept between the passes
     int max_sw, max_sh;      // largest source grid
     uint32_t bins_x, bins_y; // binning work items per source (for the largest source)
 };

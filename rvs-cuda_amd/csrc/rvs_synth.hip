@@ -266,15 +266,20 @@ __global__ void k_bin_cull(const G* __restrict__ geoms, const ViewDesc* __restri
     }
     const int lane = __lane_id();
     const uint64_t m = __ballot(keep);
-    if (m == 0ull) return;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&ws.status[4], (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
-    if (keep) {
-        const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-        ws.work[base + (uint32_t)__popcll(m & lt)] = (uint32_t)gid;
+    uint32_t wi = 0xffffffffu;
+    if (m != 0ull) {
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&ws.status[4], (uint32_t)__popcll(m));
+        base = __shfl(base, leader);
+        if (keep) {
+            const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+            const uint32_t pos = base + (uint32_t)__popcll(m & lt);
+            ws.work[pos] = (uint32_t)gid;
+            if (pos < d.range_cap) wi = pos;
+        }
     }
+    if (gid < total) ws.imap[gid] = wi;
 }
 
 // ------------------------------------------------------------------ binning
@@ -401,10 +406,15 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
                 }
             }
         } else {
-            for (int k = tid; k < (kBinQY + 1) * (kBinQX + 1); k += 256) {
+            float* vb = (!FILL && stored) ? ws.vbuf + (size_t)wi * (3 * kBinVerts) : nullptr;
+            for (int k = tid; k < kBinVerts; k += 256) {
                 const int rr = k / (kBinQX + 1), c = k % (kBinQX + 1);
                 const int ii = i0 + rr, jj = j0 + c;
-                if (ii < g.sh && jj < g.sw) V[rr][c] = vertex(g, ii, jj);
+                if (ii < g.sh && jj < g.sw) {
+                    const float3 x = vertex(g, ii, jj);
+                    V[rr][c] = x;
+                    if (vb) { vb[3 * k] = x.x; vb[3 * k + 1] = x.y; vb[3 * k + 2] = x.z; }
+                }
             }
             __syncthreads();   // vertices loaded, table cleared
             if (i < g.sh - 1 && j < g.sw - 1)
@@ -561,17 +571,32 @@ hipError_t launch_bin(const G* geoms, const ViewDesc* views, const BatchDims& d,
 
 // ------------------------------------------------------------ tile raster
 
+// Vertex (row lr, column lc) of a kept work item's vertex block (the counting
+// pass evaluated it with vertex(); the bits are the same).
+__device__ inline float3 kept_vertex(const float* vb, int lr, int lc) {
+    const float* p = vb + 3 * (lr * (kBinQX + 1) + lc);
+    return make_float3(p[0], p[1], p[2]);
+}
+
+// Vertex block of quad (i, j) of pair vs when its work item was kept, else null.
+__device__ inline const float* kept_block(const RasterWS& ws, const BatchDims& d, uint32_t vs, int i, int j) {
+    const uint32_t item = (uint32_t)(i / kBinQY) * d.bins_x + (uint32_t)(j / kBinQX);
+    const uint32_t wi = ws.imap[(size_t)vs * (d.bins_x * d.bins_y) + item];
+    return wi == 0xffffffffu ? nullptr : ws.vbuf + (size_t)wi * (3 * kBinVerts);
+}
+
 // Winner's colour, depth and validity at pixel (x, y) for slot `slot` of
 // source quad (i, j): the state the fold holds after accepting that candidate.
 template <class G>
 __device__ inline void resolve(const G& g, int i, int j, int slot, int x, int y, float3& c, float& dep,
-                               float& shape, int out_w, int out_h) {
+                               float& shape, int out_w, int out_h, const float* vb) {
     int r[3], cc[3];
     float3 v[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         slot_vertex(slot, k, i, j, g.sw, r[k], cc[k]);
-        v[k] = vertex(g, r[k], cc[k]);
+        v[k] = (vb && slot < 2) ? kept_vertex(vb, i % kBinQY + (r[k] - i), j % kBinQX + (cc[k] - j))
+                               : vertex(g, r[k], cc[k]);
     }
     Tri T;
     tri_setup(T, v[0], v[1], v[2], out_w, out_h, g.rescale);
@@ -658,34 +683,40 @@ struct RasterCache {
 // scores are deferred to the in-order replay.
 template <class G>
 __device__ inline void raster_quad(const G& g, uint32_t ij, uint32_t e, int tx0, int ty0, int out_w, int out_h,
-                                   unsigned long long* key1, unsigned long long* key2, int* flag, RasterCache* C) {
+                                   unsigned long long* key1, unsigned long long* key2, int* flag, RasterCache* C,
+                                   const RasterWS& ws, const BatchDims& d, uint32_t vs) {
     int i, j;
     unpack_ij(ij, i, j);
+    const float* vb = kept_block(ws, d, vs, i, j);
+    const int li = i % kBinQY, lj = j % kBinQX;
+    auto vtx = [&](int di, int dj) -> float3 {
+        return vb ? kept_vertex(vb, li + di, lj + dj) : vertex(g, i + di, j + dj);
+    };
     // vertices are evaluated (and cached) just before their triangles need
     // them; keys commute, so only the set of candidates matters here
     auto keep = [&](int k, float3 v) {
         if (C) { C->v[3 * k][e] = v.x; C->v[3 * k + 1][e] = v.y; C->v[3 * k + 2][e] = v.z; }
     };
-    const float3 v01 = vertex(g, i, j + 1), v10 = vertex(g, i + 1, j);
+    const float3 v01 = vtx(0, 1), v10 = vtx(1, 0);
     keep(1, v01);
     keep(2, v10);
     const bool gate = v01.z > 0.f && v10.z > 0.f && v01.x > 0.f && v10.x > 0.f;
     const uint32_t s0 = e << 2;
     float tv0 = 0.f, tv1 = 0.f;
     {
-        const float3 v00 = vertex(g, i, j);
+        const float3 v00 = vtx(0, 0);
         keep(0, v00);
         if (gate && v00.z > 0.f && v00.x > 0.f)
             tv0 = raster_tri(v00, v01, v10, s0, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
     }
     {
-        const float3 v11 = vertex(g, i + 1, j + 1);
+        const float3 v11 = vtx(1, 1);
         keep(3, v11);
         if (gate && v11.z > 0.f && v11.x > 0.f)
             tv1 = raster_tri(v11, v10, v01, s0 | 1u, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
     }
     if (gate && g.wrap && j == 0) {
-        const float3 v00 = vertex(g, i, j);
+        const float3 v00 = vtx(0, 0);
         const float3 l0 = vertex(g, i, g.sw - 1), l1 = vertex(g, i + 1, g.sw - 1);
         raster_tri(l0, v00, l1, s0 | 2u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
         raster_tri(v10, l1, v00, s0 | 3u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
@@ -755,7 +786,8 @@ __global__ void __launch_bounds__(256) k_raster(const G* __restrict__ geoms, con
             int k = 0;
             while (k + 1 < gs && e >= pref[k + 1]) ++k;
             const uint32_t ij = ws.lists[(size_t)ws.offsets[(size_t)T * d.nsrc + s0 + k] + (e - pref[k])];
-            raster_quad(gv[s0 + k], ij, e, tx0, ty0, vd.w, vd.h, key1[k], key2[k], flag[k], cached ? &cache : nullptr);
+            raster_quad(gv[s0 + k], ij, e, tx0, ty0, vd.w, vd.h, key1[k], key2[k], flag[k], cached ? &cache : nullptr,
+                        ws, d, (uint32_t)(v * d.nsrc + s0 + k));
         }
         __syncthreads();
         if (inside) {
@@ -792,7 +824,8 @@ __global__ void __launch_bounds__(256) k_raster(const G* __restrict__ geoms, con
                                                               (e - pref[k])];
                         int i, j;
                         unpack_ij(ij, i, j);
-                        resolve(g, i, j, slot, px, py, c, dep, shape, vd.w, vd.h);
+                        resolve(g, i, j, slot, px, py, c, dep, shape, vd.w, vd.h,
+                                kept_block(ws, d, (uint32_t)(v * d.nsrc + s), i, j));
                     }
                 }
                 if (MODE == MODE_RAW) {
@@ -1025,7 +1058,7 @@ __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, c
                     float dd, sh;
                     int i, j;
                     unpack_ij(fs.ij, i, j);
-                    resolve(g, i, j, fs.slot, px, py, c, dd, sh, vd.w, vd.h);
+                    resolve(g, i, j, fs.slot, px, py, c, dd, sh, vd.w, vd.h, nullptr);
                     dep = fs.new_d;
                     shape = fs.shape;
                 }
