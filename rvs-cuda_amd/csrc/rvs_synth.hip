@@ -669,10 +669,9 @@ __device__ inline float raster_tri(float3 a, float3 b, float3 c, uint32_t id, bo
 }
 
 // Per-phase LDS copy of what the coverage pass computed for each binned quad:
-// the four corner vertices (v00, v01, v10, v11; u', v', depth), the validity
-// of triangles T1 and T2 and the quad code.
+// the validity of triangles T1 and T2 and the quad code (the corner vertices
+// are read again from the kept vertex blocks).
 struct RasterCache {
-    float v[12][kRasterCache];
     float tv[2][kRasterCache];
     uint32_t ij[kRasterCache];
 };
@@ -692,26 +691,19 @@ __device__ inline void raster_quad(const G& g, uint32_t ij, uint32_t e, int tx0,
     auto vtx = [&](int di, int dj) -> float3 {
         return vb ? kept_vertex(vb, li + di, lj + dj) : vertex(g, i + di, j + dj);
     };
-    // vertices are evaluated (and cached) just before their triangles need
-    // them; keys commute, so only the set of candidates matters here
-    auto keep = [&](int k, float3 v) {
-        if (C) { C->v[3 * k][e] = v.x; C->v[3 * k + 1][e] = v.y; C->v[3 * k + 2][e] = v.z; }
-    };
+    // vertices are fetched just before their triangles need them; keys
+    // commute, so only the set of candidates matters here
     const float3 v01 = vtx(0, 1), v10 = vtx(1, 0);
-    keep(1, v01);
-    keep(2, v10);
     const bool gate = v01.z > 0.f && v10.z > 0.f && v01.x > 0.f && v10.x > 0.f;
     const uint32_t s0 = e << 2;
     float tv0 = 0.f, tv1 = 0.f;
     {
         const float3 v00 = vtx(0, 0);
-        keep(0, v00);
         if (gate && v00.z > 0.f && v00.x > 0.f)
             tv0 = raster_tri(v00, v01, v10, s0, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
     }
     {
         const float3 v11 = vtx(1, 1);
-        keep(3, v11);
         if (gate && v11.z > 0.f && v11.x > 0.f)
             tv1 = raster_tri(v11, v10, v01, s0 | 1u, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
     }
@@ -810,19 +802,23 @@ __global__ void __launch_bounds__(256) k_raster(const G* __restrict__ geoms, con
                     const uint32_t id = (uint32_t)kw, e = id >> 2;
                     const int slot = (int)(id & 3u);
                     const G& g = gv[s];
-                    if (cached && slot < 2) {
-                        const int ka = slot ? 3 : 0, kb = slot ? 2 : 1, kc = slot ? 1 : 2;
-                        const float3 a = make_float3(cache.v[3 * ka][e], cache.v[3 * ka + 1][e], cache.v[3 * ka + 2][e]);
-                        const float3 b = make_float3(cache.v[3 * kb][e], cache.v[3 * kb + 1][e], cache.v[3 * kb + 2][e]);
-                        const float3 cc = make_float3(cache.v[3 * kc][e], cache.v[3 * kc + 1][e], cache.v[3 * kc + 2][e]);
-                        int i, j;
+                    int i, j;
+                    const float* vb = nullptr;
+                    if (cached) {
                         unpack_ij(cache.ij[e], i, j);
+                        if (slot < 2) vb = kept_block(ws, d, (uint32_t)(v * d.nsrc + s), i, j);
+                    }
+                    if (vb) {
+                        // T1: v00 v01 v10; T2: v11 v10 v01 (corner offsets of slot_vertex)
+                        const int li = i % kBinQY, lj = j % kBinQX;
+                        const float3 a = kept_vertex(vb, li + slot, lj + slot);
+                        const float3 b = kept_vertex(vb, li + slot, lj + 1 - slot);
+                        const float3 cc = kept_vertex(vb, li + 1 - slot, lj + slot);
                         resolve_cached(g, i, j, slot, a, b, cc, cache.tv[slot][e], px, py, c, dep, shape);
                     } else {
                         const uint32_t ij = cached ? cache.ij[e]
                                                    : ws.lists[(size_t)ws.offsets[(size_t)T * d.nsrc + s] +
                                                               (e - pref[k])];
-                        int i, j;
                         unpack_ij(ij, i, j);
                         resolve(g, i, j, slot, px, py, c, dep, shape, vd.w, vd.h,
                                 kept_block(ws, d, (uint32_t)(v * d.nsrc + s), i, j));
