@@ -735,7 +735,7 @@ __device__ inline void write_pixel(const ViewDesc& vd, size_t o, const BlendStat
 }
 
 template <class G, int MODE>
-__global__ void __launch_bounds__(256) k_raster(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_raster(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
                                                 BatchDims d, RasterWS ws) {
     __shared__ unsigned long long key1[kRasterGroup][kTile * kTile];
     __shared__ unsigned long long key2[kRasterGroup][kTile * kTile];
