@@ -27,7 +27,7 @@ constexpr int kFallbackSortCap = 8192;  // longest (source, tile) list the repla
 constexpr int kMaxViews = 4096;
 constexpr int kBinVerts = (kBinQY + 1) * (kBinQX + 1);   // vertices of one binning work item
 constexpr int kBinBlocks = 2048;    // persistent binning blocks
-constexpr int kRasterGroup = 2;     // sources rasterised together per k_raster phase
+constexpr int kRasterGroup = 4;     // sources rasterised together per k_raster phase
 constexpr int kRasterCache = 1024;  // binned quads per phase whose coverage-pass results stay in LDS
 constexpr int kMaxSourceSide = 65536;  // bins hold quads as (row << 16) | column
 constexpr int kFallbackBlocks = 1024;

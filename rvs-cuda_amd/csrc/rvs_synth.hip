@@ -13,19 +13,21 @@
 //      its image are dropped (bounding boxes of the unprojected points come
 //      from k_src_bounds and are tested with error margins); the rest form a
 //      work list.
-//   1. k_bin (persistent, pulls work items): every surviving source quad is
-//      appended to the 16x16 target tiles its triangles' (reference-exact)
-//      bounding boxes touch.
-//   2. k_raster: one 256-thread block per target tile, two sources per
-//      phase.  Every covered pixel takes two LDS 64-bit atomicMax: key1 = max
-//      of (score bits << 32 | candidate id), key2 = max over the candidates
-//      that are not the maximum (the loser of each key1 update).  A pixel
-//      whose runner-up scores within 2^-18 of the winner (exact ties
-//      included), or that saw a candidate with non-finite or extreme values,
-//      is deferred.  Every other pixel provably ends the sequential fold on
-//      the argmax (the float test errs by less than 2^-21 relative), so its
-//      winner is re-evaluated -- from the corner vertices and validity the
-//      coverage pass left in LDS -- and blended.
+//   1. k_bin (counting pass, scan, fill pass): every surviving source quad
+//      is appended to the 16x16 target tiles its triangles' (reference-exact)
+//      bounding boxes touch; the counting pass keeps each work item's vertex
+//      block for the raster.
+//   2. k_raster: one 256-thread block per target tile, four sources per
+//      phase.  Every covered pixel takes an LDS 64-bit atomicMax, key1 = max
+//      of (score bits << 32 | candidate id), and a 32-bit one, key2 = max
+//      score over the candidates that are not the maximum (the loser of each
+//      key1 update).  A pixel whose runner-up scores within 2^-18 of the
+//      winner (exact ties included), or that saw a candidate with non-finite
+//      or extreme values (key2 = all ones), is deferred.  Every other pixel
+//      provably ends the sequential fold on the argmax (the float test errs
+//      by less than 2^-21 relative), so its winner is re-evaluated -- from
+//      the kept vertices and the validity the coverage pass left in LDS --
+//      and blended.
 //   3. k_fallback: deferred pixels replay the reference fold literally, in
 //      traversal order, over the tile's bins sorted in LDS.
 // Blending across sources (BlendedViewSimple::blendBasicMode) runs per pixel in
@@ -625,16 +627,20 @@ __device__ inline void resolve_cached(const G& g, int i, int j, int slot, float3
     c = tri_color(color(g, r[0], cc[0]), color(g, r[1], cc[1]), color(g, r[2], cc[2]), l1, l2, l3);
 }
 
+// key2 value of a pixel that must take the in-order replay (above every
+// score: scores are finite positive floats).
+constexpr uint32_t kDeferKey = 0xffffffffu;
+
 // Pass over one triangle of a binned quad for the tile at (tx0, ty0): per
 // covered pixel, push the candidate key (score bits << 32 | candidate id)
-// into the top-2 keys of the pixel.  Candidates with non-finite inputs or
-// extreme depths mark the pixel for the in-order replay instead.  Returns the
+// into key1 and the displaced score into key2.  Candidates with non-finite
+// inputs or extreme depths mark the pixel for the in-order replay instead.  Returns the
 // triangle's validity, or 0 when no pixel of the tile can be covered (then it
 // is not evaluated; the reference computes it before the bounding box test,
 // and it never causes a return, so skipping it changes nothing).
 __device__ inline float raster_tri(float3 a, float3 b, float3 c, uint32_t id, bool prol_false_possible, int tx0,
                                    int ty0, int out_w, int out_h, float rescale, unsigned long long* key1,
-                                   unsigned long long* key2, int* flag) {
+                                   uint32_t* key2) {
     Tri T;
     if (!tri_setup_geom(T, a, b, c, out_w, out_h)) return 0.f;
     const bool prol = (a.z > 0.f) || (b.z > 0.f) || (c.z > 0.f);
@@ -655,14 +661,14 @@ __device__ inline float raster_tri(float3 a, float3 b, float3 c, uint32_t id, bo
             const int p = (y - ty0) * kTile + (x - tx0);
             const float dd = tri_depth(T, l1, l2, l3);
             if (special || !depth_regular(dd)) {
-                flag[p] = 1;
+                atomicMax(&key2[p], kDeferKey);
                 continue;
             }
             const float f = T.tv / (dd * dd * dd);
             const unsigned long long k = ((unsigned long long)f2u(f) << 32) | id;
             const unsigned long long old = atomicMax(&key1[p], k);
             // the candidate that is not (or no longer) the maximum goes to the runner-up slot
-            atomicMax(&key2[p], old < k ? old : k);
+            atomicMax(&key2[p], (uint32_t)((old < k ? old : k) >> 32));
         }
     }
     return T.tv;
@@ -682,7 +688,7 @@ struct RasterCache {
 // scores are deferred to the in-order replay.
 template <class G>
 __device__ inline void raster_quad(const G& g, uint32_t ij, uint32_t e, int tx0, int ty0, int out_w, int out_h,
-                                   unsigned long long* key1, unsigned long long* key2, int* flag, RasterCache* C,
+                                   unsigned long long* key1, uint32_t* key2, RasterCache* C,
                                    const RasterWS& ws, const BatchDims& d, uint32_t vs) {
     int i, j;
     unpack_ij(ij, i, j);
@@ -700,18 +706,18 @@ __device__ inline void raster_quad(const G& g, uint32_t ij, uint32_t e, int tx0,
     {
         const float3 v00 = vtx(0, 0);
         if (gate && v00.z > 0.f && v00.x > 0.f)
-            tv0 = raster_tri(v00, v01, v10, s0, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
+            tv0 = raster_tri(v00, v01, v10, s0, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2);
     }
     {
         const float3 v11 = vtx(1, 1);
         if (gate && v11.z > 0.f && v11.x > 0.f)
-            tv1 = raster_tri(v11, v10, v01, s0 | 1u, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
+            tv1 = raster_tri(v11, v10, v01, s0 | 1u, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2);
     }
     if (gate && g.wrap && j == 0) {
         const float3 v00 = vtx(0, 0);
         const float3 l0 = vertex(g, i, g.sw - 1), l1 = vertex(g, i + 1, g.sw - 1);
-        raster_tri(l0, v00, l1, s0 | 2u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
-        raster_tri(v10, l1, v00, s0 | 3u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, flag);
+        raster_tri(l0, v00, l1, s0 | 2u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2);
+        raster_tri(v10, l1, v00, s0 | 3u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2);
     }
     if (C) {
         C->tv[0][e] = tv0;
@@ -738,8 +744,7 @@ template <class G, int MODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_raster(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
                                                 BatchDims d, RasterWS ws) {
     __shared__ unsigned long long key1[kRasterGroup][kTile * kTile];
-    __shared__ unsigned long long key2[kRasterGroup][kTile * kTile];
-    __shared__ int flag[kRasterGroup][kTile * kTile];
+    __shared__ uint32_t key2[kRasterGroup][kTile * kTile];
     __shared__ uint32_t pref[kRasterGroup + 1];
     __shared__ RasterCache cache;
     const uint32_t T = tile_of_block(blockIdx.x, d.ntiles);
@@ -760,8 +765,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
         const int gs = min(kRasterGroup, d.nsrc - s0);
         for (int k = 0; k < gs; ++k) {
             key1[k][tid] = 0ull;
-            key2[k][tid] = 0ull;
-            flag[k][tid] = 0;
+            key2[k][tid] = 0u;
         }
         if (tid == 0) {
             uint32_t acc = 0;
@@ -778,7 +782,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
             int k = 0;
             while (k + 1 < gs && e >= pref[k + 1]) ++k;
             const uint32_t ij = ws.lists[(size_t)ws.offsets[(size_t)T * d.nsrc + s0 + k] + (e - pref[k])];
-            raster_quad(gv[s0 + k], ij, e, tx0, ty0, vd.w, vd.h, key1[k], key2[k], flag[k], cached ? &cache : nullptr,
+            raster_quad(gv[s0 + k], ij, e, tx0, ty0, vd.w, vd.h, key1[k], key2[k], cached ? &cache : nullptr,
                         ws, d, (uint32_t)(v * d.nsrc + s0 + k));
         }
         __syncthreads();
@@ -787,13 +791,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
                 const int s = s0 + k;
                 if (deferred) break;
                 const unsigned long long kw = key1[k][tid];
-                bool amb = flag[k][tid] != 0;
-                if (!amb && kw != 0ull) {
-                    const unsigned long long k2 = key2[k][tid];
-                    if (k2 != 0ull) {
-                        const float fw = u2f((uint32_t)(kw >> 32)), f2 = u2f((uint32_t)(k2 >> 32));
-                        amb = f2 >= fw - fw * RVS_AMBIGUITY;
-                    }
+                const uint32_t k2 = key2[k][tid];
+                bool amb = k2 == kDeferKey;
+                if (!amb && kw != 0ull && k2 != 0u) {
+                    const float fw = u2f((uint32_t)(kw >> 32)), f2 = u2f(k2);
+                    amb = f2 >= fw - fw * RVS_AMBIGUITY;
                 }
                 if (amb) { deferred = true; break; }
                 float3 c = make_float3(0.f, 0.f, 0.f);
