@@ -288,17 +288,18 @@ __global__ void k_bin_cull(const G* __restrict__ geoms, const ViewDesc* __restri
 //
 // Bins are compact: a counting pass (k_bin<COUNT>) finds every surviving
 // quad's tile range and counts the (quad, tile) entries per (tile, source)
-// bin; an exclusive scan turns the counts into bin offsets; a fill pass
-// (k_bin<FILL>) writes the quad codes.  For the first range_cap work items the
-// counting pass keeps each quad's tile range and, for a quad touching a single
-// tile (most of them), its slot in that bin, so the fill pass writes it
+// bin; an exclusive scan turns the counts into bin offsets; fill passes write
+// the quad codes.  For the first range_cap work items the counting pass keeps
+// each quad's tile range and, for a quad touching a single tile (most of
+// them), its slot in that bin, so their fill pass (k_fill_kept) writes it
 // without atomics; quads spanning several tiles take slots after the
 // single-tile ones through fill cursors.  Later items re-evaluate their
-// vertices in the fill pass.  When the entries exceed the list capacity the
-// fill pass and the raster are skipped and the needed capacity is reported.
+// vertices in their fill pass (k_bin<FILL>).  When the entries exceed the list
+// capacity the fill passes and the raster are skipped and the needed capacity
+// is reported.
 //
-// Each block first gathers its entries per bin in an LDS table (a few to a
-// few dozen distinct tiles per 64x4 quad block), then takes one global atomic
+// k_bin blocks first gather their entries per bin in an LDS table (a few to a
+// few dozen distinct tiles per 64x4 quad block), then take one global atomic
 // per (bin, kind): device-scope atomics are performed at the memory side, so
 // their number and the chains of dependent ones set this kernel's time.
 
@@ -316,6 +317,20 @@ __device__ inline void unpack_range(uint32_t r, int& tx0, int& tx1, int& ty0, in
     ty0 = (int)((r >> 12) & 0xfffu);
     tx1 = tx0 + (int)((r >> 24) & 0xfu);
     ty1 = ty0 + (int)(r >> 28);
+}
+
+// Vertex (row lr, column lc) of a kept work item's vertex block (the counting
+// pass evaluated it with vertex(); the bits are the same).
+__device__ inline float3 kept_vertex(const float* vb, int lr, int lc) {
+    const float* p = vb + 3 * (lr * (kBinQX + 1) + lc);
+    return make_float3(p[0], p[1], p[2]);
+}
+
+// Vertex block of quad (i, j) of pair vs when its work item was kept, else null.
+__device__ inline const float* kept_block(const RasterWS& ws, const BatchDims& d, uint32_t vs, int i, int j) {
+    const uint32_t item = (uint32_t)(i / kBinQY) * d.bins_x + (uint32_t)(j / kBinQX);
+    const uint32_t wi = ws.imap[(size_t)vs * (d.bins_x * d.bins_y) + item];
+    return wi == 0xffffffffu ? nullptr : ws.vbuf + (size_t)wi * (3 * kBinVerts);
 }
 
 constexpr int kBinTable = 128;                 // LDS bin table entries per block (power of two)
@@ -349,6 +364,20 @@ __device__ inline int find_slot(const uint32_t* keys, uint32_t t) {
     return -1;
 }
 
+// Tile range [tx0, tx1] x [ty0, ty1] touched by the bounding boxes of a
+// quad's surviving triangles; false when none survives.
+__device__ inline bool quad_tiles(const Quad& Q, int& tx0, int& tx1, int& ty0, int& ty1) {
+    if (!Q.alive) return false;
+    int xmin = INT_MAX, xmax = INT_MIN, ymin = INT_MAX, ymax = INT_MIN;
+    for (int k = 0; k < 4; ++k)
+        if (Q.alive & (1u << k)) {
+            xmin = min(xmin, Q.t[k].xmin); xmax = max(xmax, Q.t[k].xmax);
+            ymin = min(ymin, Q.t[k].ymin); ymax = max(ymax, Q.t[k].ymax);
+        }
+    tx0 = xmin / kTile; tx1 = xmax / kTile; ty0 = ymin / kTile; ty1 = ymax / kTile;
+    return true;
+}
+
 template <class G, bool FILL>
 __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
                                              BatchDims d, RasterWS ws) {
@@ -361,8 +390,9 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
     if (FILL && ws.status[1]) return;   // list capacity exceeded: nothing to fill
     const uint32_t nwork = ws.status[4];
     // Work items are alike in cost: a static stride over them keeps every
-    // block busy without a shared work counter.
-    for (uint32_t wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+    // block busy without a shared work counter.  The fill pass of the kept
+    // items is k_fill_kept; this one fills the others.
+    for (uint32_t wi = (FILL ? d.range_cap : 0u) + blockIdx.x; wi < nwork; wi += gridDim.x) {
         const uint32_t code = ws.work[wi];
         const uint32_t item = code % per, vs = code / per;
         const int s = (int)(vs % (uint32_t)d.nsrc), v = (int)(vs / (uint32_t)d.nsrc);
@@ -370,7 +400,7 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
         const ViewDesc& vd = views[v];
         const int i0 = (int)(item / d.bins_x) * kBinQY, j0 = (int)(item % d.bins_x) * kBinQX;
         const int i = i0 + tid / kBinQX, j = j0 + tid % kBinQX;
-        const bool stored = wi < d.range_cap;   // same for the whole block
+        const bool stored = !FILL && wi < d.range_cap;   // same for the whole block
         const uint32_t q = ((uint32_t)i << 16) | (uint32_t)j;
         if (tid < kBinTable) {
             t_key[tid] = kTableEmpty;
@@ -381,34 +411,10 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
         auto quad_range = [&](auto vtx) {
             Quad Q;
             quad_build(Q, g, i, j, vtx, vd.w, vd.h, false);
-            if (!Q.alive) return;
-            int xmin = INT_MAX, xmax = INT_MIN, ymin = INT_MAX, ymax = INT_MIN;
-            for (int k = 0; k < 4; ++k)
-                if (Q.alive & (1u << k)) {
-                    xmin = min(xmin, Q.t[k].xmin); xmax = max(xmax, Q.t[k].xmax);
-                    ymin = min(ymin, Q.t[k].ymin); ymax = max(ymax, Q.t[k].ymax);
-                }
-            tx0 = xmin / kTile; tx1 = xmax / kTile; ty0 = ymin / kTile; ty1 = ymax / kTile;
-            more = true;
+            more = quad_tiles(Q, tx0, tx1, ty0, ty1);
         };
-        if (FILL && stored) {
-            __syncthreads();   // table cleared
-            const uint2 r = ws.ranges[(size_t)wi * 256 + tid];
-            if (r.x == kRangeBig) {
-                // a range too wide to pack: this quad's vertices again
-                quad_range([&](int ii, int jj) -> float3 { return vertex(g, ii, jj); });
-            } else if (r.x != kRangeNone) {
-                unpack_range(r.x, tx0, tx1, ty0, ty1);
-                if (tx0 == tx1 && ty0 == ty1) {
-                    // single tile: its slot was taken in the counting pass
-                    const size_t bin = (size_t)(vd.tile_base + (uint32_t)(ty0 * vd.tiles_x + tx0)) * d.nsrc + s;
-                    ws.lists[(size_t)ws.offsets[bin] + r.y] = q;
-                } else {
-                    more = true;
-                }
-            }
-        } else {
-            float* vb = (!FILL && stored) ? ws.vbuf + (size_t)wi * (3 * kBinVerts) : nullptr;
+        {
+            float* vb = stored ? ws.vbuf + (size_t)wi * (3 * kBinVerts) : nullptr;
             for (int k = tid; k < kBinVerts; k += 256) {
                 const int rr = k / (kBinQX + 1), c = k % (kBinQX + 1);
                 const int ii = i0 + rr, jj = j0 + c;
@@ -479,6 +485,87 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
             ws.ranges[(size_t)wi * 256 + tid] = r;
         }
         __syncthreads();   // LDS free for the next item
+    }
+}
+
+// Fill pass over the kept work items, one block per item and one thread per
+// quad, without item loops or barriers (the many independent loads overlap):
+// single-tile quads write their counted slot, the others take slots after the
+// bin's single-tile region through an LDS bin table and the fill cursors.  A quad whose
+// range was too wide to pack is re-binned from the kept vertices.
+template <class G>
+__global__ void __launch_bounds__(256) k_fill_kept(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
+                                                   BatchDims d, RasterWS ws) {
+    __shared__ uint32_t t_key[kBinTable];
+    __shared__ uint32_t t_n[kBinTable];
+    __shared__ uint32_t t_base[kBinTable];
+    const uint32_t wi = blockIdx.x;
+    if (ws.status[1] || wi >= ws.status[4]) return;   // uniform per block
+    const int tid = threadIdx.x;
+    const uint32_t per = d.bins_x * d.bins_y;
+    if (tid < kBinTable) {
+        t_key[tid] = kTableEmpty;
+        t_n[tid] = 0;
+    }
+    const uint32_t code = ws.work[wi];
+    const uint32_t item = code % per, vs = code / per;
+    const int s = (int)(vs % (uint32_t)d.nsrc), v = (int)(vs / (uint32_t)d.nsrc);
+    const ViewDesc& vd = views[v];
+    const int i0 = (int)(item / d.bins_x) * kBinQY, j0 = (int)(item % d.bins_x) * kBinQX;
+    const int i = i0 + tid / kBinQX, j = j0 + tid % kBinQX;
+    const uint32_t q = ((uint32_t)i << 16) | (uint32_t)j;
+    const uint2 r = ws.ranges[(size_t)wi * 256 + tid];
+    bool more = false;
+    int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
+    if (r.x == kRangeBig) {
+        const G& g = geoms[vs];
+        const float* vb = ws.vbuf + (size_t)wi * (3 * kBinVerts);
+        Quad Q;
+        quad_build(Q, g, i, j,
+                   [&](int ii, int jj) -> float3 {
+                       const int rr = ii - i0, c = jj - j0;
+                       if (rr >= 0 && rr <= kBinQY && c >= 0 && c <= kBinQX) return kept_vertex(vb, rr, c);
+                       return vertex(g, ii, jj);
+                   },
+                   vd.w, vd.h, false);
+        more = quad_tiles(Q, tx0, tx1, ty0, ty1);
+    } else if (r.x != kRangeNone) {
+        unpack_range(r.x, tx0, tx1, ty0, ty1);
+        if (tx0 == tx1 && ty0 == ty1) {
+            const size_t bin = (size_t)(vd.tile_base + (uint32_t)(ty0 * vd.tiles_x + tx0)) * d.nsrc + s;
+            ws.lists[(size_t)ws.offsets[bin] + r.y] = q;
+        } else {
+            more = true;
+        }
+    }
+    // multi-tile entries: LDS bin table, one fill-cursor atomic per bin
+    auto tile_of = [&](int tx, int ty) { return vd.tile_base + (uint32_t)(ty * vd.tiles_x + tx); };
+    __syncthreads();   // table cleared
+    if (more) {
+        for (int ty = ty0; ty <= ty1; ++ty)
+            for (int tx = tx0; tx <= tx1; ++tx) {
+                const uint32_t t = tile_of(tx, ty);
+                const int k = table_slot(t_key, t);
+                if (k >= 0) {
+                    atomicAdd(&t_n[k], 1u);
+                } else {
+                    const size_t bin = (size_t)t * d.nsrc + s;
+                    ws.lists[(size_t)ws.offsets[bin] + ws.single[bin] + atomicAdd(&ws.fill[bin], 1u)] = q;
+                }
+            }
+    }
+    __syncthreads();   // table complete
+    if (tid < kBinTable && t_n[tid]) {
+        const size_t bin = (size_t)t_key[tid] * d.nsrc + s;
+        t_base[tid] = ws.offsets[bin] + ws.single[bin] + atomicAdd(&ws.fill[bin], t_n[tid]);
+    }
+    __syncthreads();   // bases known
+    if (more) {
+        for (int ty = ty0; ty <= ty1; ++ty)
+            for (int tx = tx0; tx <= tx1; ++tx) {
+                const int k = find_slot(t_key, tile_of(tx, ty));
+                if (k >= 0) ws.lists[atomicAdd(&t_base[k], 1u)] = q;
+            }
     }
 }
 
@@ -567,25 +654,14 @@ hipError_t launch_bin(const G* geoms, const ViewDesc* views, const BatchDims& d,
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, ws.bsum, (uint32_t)nb, d.cap, ws.status);
     hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((nbins + 255) / 256)), dim3(256), 0, s, ws.offsets, nbins,
                        ws.bsum);
-    hipLaunchKernelGGL((k_bin<G, true>), dim3((unsigned)blocks), dim3(256), 0, s, geoms, views, d, ws);
+    if (d.range_cap)
+        hipLaunchKernelGGL(k_fill_kept<G>, dim3(d.range_cap), dim3(256), 0, s, geoms, views, d, ws);
+    if (total > d.range_cap)
+        hipLaunchKernelGGL((k_bin<G, true>), dim3((unsigned)blocks), dim3(256), 0, s, geoms, views, d, ws);
     return hipGetLastError();
 }
 
 // ------------------------------------------------------------ tile raster
-
-// Vertex (row lr, column lc) of a kept work item's vertex block (the counting
-// pass evaluated it with vertex(); the bits are the same).
-__device__ inline float3 kept_vertex(const float* vb, int lr, int lc) {
-    const float* p = vb + 3 * (lr * (kBinQX + 1) + lc);
-    return make_float3(p[0], p[1], p[2]);
-}
-
-// Vertex block of quad (i, j) of pair vs when its work item was kept, else null.
-__device__ inline const float* kept_block(const RasterWS& ws, const BatchDims& d, uint32_t vs, int i, int j) {
-    const uint32_t item = (uint32_t)(i / kBinQY) * d.bins_x + (uint32_t)(j / kBinQX);
-    const uint32_t wi = ws.imap[(size_t)vs * (d.bins_x * d.bins_y) + item];
-    return wi == 0xffffffffu ? nullptr : ws.vbuf + (size_t)wi * (3 * kBinVerts);
-}
 
 // Winner's colour, depth and validity at pixel (x, y) for slot `slot` of
 // source quad (i, j): the state the fold holds after accepting that candidate.
