@@ -378,7 +378,9 @@ __device__ inline bool quad_tiles(const Quad& Q, int& tx0, int& tx1, int& ty0, i
     return true;
 }
 
-template <class G, bool FILL>
+// FLAT: one block per kept work item (counting pass only); otherwise
+// persistent blocks stride over the items after the kept ones.
+template <class G, bool FILL, bool FLAT>
 __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
                                              BatchDims d, RasterWS ws) {
     __shared__ float3 V[kBinQY + 1][kBinQX + 1];
@@ -389,10 +391,14 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
     const uint32_t per = d.bins_x * d.bins_y;
     if (FILL && ws.status[1]) return;   // list capacity exceeded: nothing to fill
     const uint32_t nwork = ws.status[4];
-    // Work items are alike in cost: a static stride over them keeps every
-    // block busy without a shared work counter.  The fill pass of the kept
-    // items is k_fill_kept; this one fills the others.
-    for (uint32_t wi = (FILL ? d.range_cap : 0u) + blockIdx.x; wi < nwork; wi += gridDim.x) {
+    // Work items after the kept ones are alike in cost: a static stride over
+    // them keeps every block busy without a shared work counter.  The fill
+    // pass of the kept items is k_fill_kept.
+    static_assert(!(FILL && FLAT), "kept items are filled by k_fill_kept");
+    const uint64_t lim = FLAT ? min(nwork, d.range_cap) : nwork;
+    const uint64_t step = FLAT ? lim : gridDim.x;
+    for (uint64_t wi64 = (FLAT ? 0u : d.range_cap) + blockIdx.x; wi64 < lim; wi64 += step) {
+        const uint32_t wi = (uint32_t)wi64;
         const uint32_t code = ws.work[wi];
         const uint32_t item = code % per, vs = code / per;
         const int s = (int)(vs % (uint32_t)d.nsrc), v = (int)(vs / (uint32_t)d.nsrc);
@@ -646,7 +652,10 @@ hipError_t launch_bin(const G* geoms, const ViewDesc* views, const BatchDims& d,
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t blocks = total < (size_t)kBinBlocks ? total : (size_t)kBinBlocks;
-    hipLaunchKernelGGL((k_bin<G, false>), dim3((unsigned)blocks), dim3(256), 0, s, geoms, views, d, ws);
+    if (d.range_cap)
+        hipLaunchKernelGGL((k_bin<G, false, true>), dim3(d.range_cap), dim3(256), 0, s, geoms, views, d, ws);
+    if (total > d.range_cap)
+        hipLaunchKernelGGL((k_bin<G, false, false>), dim3((unsigned)blocks), dim3(256), 0, s, geoms, views, d, ws);
     const size_t nbins = (size_t)d.ntiles * d.nsrc;
     const size_t nb = scan_blocks(nbins);
     hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(256), 0, s, ws.single, ws.multi, ws.counts,
@@ -657,7 +666,7 @@ hipError_t launch_bin(const G* geoms, const ViewDesc* views, const BatchDims& d,
     if (d.range_cap)
         hipLaunchKernelGGL(k_fill_kept<G>, dim3(d.range_cap), dim3(256), 0, s, geoms, views, d, ws);
     if (total > d.range_cap)
-        hipLaunchKernelGGL((k_bin<G, true>), dim3((unsigned)blocks), dim3(256), 0, s, geoms, views, d, ws);
+        hipLaunchKernelGGL((k_bin<G, true, false>), dim3((unsigned)blocks), dim3(256), 0, s, geoms, views, d, ws);
     return hipGetLastError();
 }
 
