@@ -155,15 +155,21 @@ typedef struct {
     float prep_depth_scale;   /* depth_scale the prep table was built with */
 } rvs_source;
 
-/* Resident per-source binning metadata: for every block of 64 x 4 source quads,
- * the bounding box of its unprojected points (source camera frame).  A
- * perspective target skips the blocks that lie behind it or beyond an image
- * side (with error margins; the result is unchanged).  Built once per loaded
- * source frame and reused by every view while the depth plane is unchanged;
- * valid for targets with the same depth_scale.  rvs_source_prep fills `out`
- * (device, rvs_source_prep_size bytes) and sets src->prep and
- * src->prep_depth_scale.  rvs_synthesize* build the table in their workspace
- * when src->prep is NULL or its depth_scale differs from the target's. */
+/* Resident per-source preparation, built once per loaded source frame and
+ * reused by every view while the frame's planes are unchanged:
+ *   - binning metadata: for every block of 64 x 4 source quads, the bounding
+ *     box of its unprojected points (source camera frame).  A perspective
+ *     target skips the blocks that lie behind it or beyond an image side (with
+ *     error margins; the result is unchanged).  Valid for targets with the
+ *     same depth_scale;
+ *   - the decoded colour plane (Y and upsampled U, V before the colour scale,
+ *     16 bytes per source pixel), read by the winner resolve instead of the
+ *     raw YUV420 planes (the same values).
+ * rvs_source_prep fills `out` (device, rvs_source_prep_size bytes) and sets
+ * src->prep and src->prep_depth_scale.  rvs_synthesize* build the binning
+ * table in their workspace when src->prep is NULL or its depth_scale differs
+ * from the target's, and decode colours from the raw planes when src->prep is
+ * NULL. */
 size_t rvs_source_prep_size(const rvs_cam* cam);
 int rvs_source_prep(rvs_source* src, float depth_scale, void* out, size_t bytes, rvs_stream_t stream);
 

@@ -101,6 +101,12 @@ void fill_source(PairGeom& g, const rvs_cam* src, const float* tab) {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Source preparation buffer: binning-block bounds, then (256-byte aligned)
+// the decoded colour plane (float4 per source pixel).
+size_t prep_color_offset(const rvs_cam* cam) {
+    return align256(rvs_k::src_bounds_floats(cam->w, cam->h) * sizeof(float));
+}
+
 // Sequential sub-allocation of a workspace, 256-byte aligned.
 struct Layout {
     size_t o = 0;
@@ -286,6 +292,7 @@ void fill_pair(PairGeom& g, const rvs_source& S, const rvs_target& T) {
     g.py = S.y; g.pu = S.u; g.pv = S.v;
     g.color_u8 = S.bitdepth_color <= 8;
     g.cscale = T.color_scale;
+    g.col = S.prep ? (const float4*)((const char*)S.prep + prep_color_offset(&S.cam)) : nullptr;
     rvs_xform xf;
     rvs_relative_pose(S.pos, S.rot, T.pos, T.rot, &xf);
     std::memcpy(g.R, xf.R, sizeof g.R);
@@ -427,11 +434,12 @@ int rvs_erp_table_fill(const rvs_cam* cam, float* out) {
 
 size_t rvs_source_prep_size(const rvs_cam* cam) {
     if (!cam_ok(cam)) return 0;
-    return rvs_k::src_bounds_floats(cam->w, cam->h) * sizeof(float);
+    return prep_color_offset(cam) + (size_t)cam->w * cam->h * sizeof(float4);
 }
 
 int rvs_source_prep(rvs_source* src, float depth_scale, void* out, size_t bytes, rvs_stream_t stream) {
-    if (!src || !cam_ok(&src->cam) || !out || !src->depth || (src->cam.proj == 1 && !src->erp_table))
+    if (!src || !cam_ok(&src->cam) || !out || !src->depth || !src->y || !src->u || !src->v ||
+        (src->cam.proj == 1 && !src->erp_table))
         return fail(RVS_E_INVALID, "rvs_source_prep: bad argument");
     if ((uint64_t)(src->cam.w - 1) * (src->cam.h - 1) >= (1ull << 30))
         return fail(RVS_E_CAPACITY, "rvs_source_prep: source too large");
@@ -444,6 +452,9 @@ int rvs_source_prep(rvs_source* src, float depth_scale, void* out, size_t bytes,
     g.dscale = depth_scale;
     g.znear = src->znear; g.zfar = src->zfar; g.has_invalid = src->has_invalid;
     HIP_TRY(rvs_k::launch_src_bounds(g, (float*)out, (hipStream_t)stream), "source bounds");
+    HIP_TRY(rvs_k::launch_prep_color(src->y, src->u, src->v, src->bitdepth_color <= 8, src->cam.w, src->cam.h,
+                                     (float4*)((char*)out + prep_color_offset(&src->cam)), (hipStream_t)stream),
+            "source colour");
     src->prep = (const float*)out;
     src->prep_depth_scale = depth_scale;
     return RVS_OK;

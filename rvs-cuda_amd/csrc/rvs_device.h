@@ -59,6 +59,8 @@ struct PairGeom {
     // (6 floats each, bounds_bx blocks per row) for frustum culling; null = no culling
     const float* bounds;
     int bounds_bx;
+    // resident decoded colour per source pixel (color_raw: Y, U, V, 0), or null
+    const float4* col;
 };
 
 // Precomputed-array vertex source (the raster entry point on identical inputs).
@@ -153,11 +155,12 @@ __device__ inline float3 vertex(const ArrayGeom& g, int i, int j) {
     return make_float3(g.uv[2 * idx], g.uv[2 * idx + 1], g.vd[idx]);
 }
 
-// Decoded colour of a source pixel: Y, and U,V upsampled bilinearly with
-// src = x * (in/out) and no half-pixel offset (process_input_frame.cu:22-81,
-// 211-238).  Weights are 0, 1/2 or 1 for the even sizes used, so the products
-// are exact; the nvcc contraction is restated as fmaf for the general case.
-__device__ inline float3 color(const PairGeom& g, int i, int j) {
+// Decoded colour of a source pixel before the colour scale: Y, and U,V
+// upsampled bilinearly with src = x * (in/out) and no half-pixel offset
+// (process_input_frame.cu:22-81, 211-238).  Weights are 0, 1/2 or 1 for the
+// even sizes used, so the products are exact; the nvcc contraction is
+// restated as fmaf for the general case.
+__device__ inline float3 color_raw(const PairGeom& g, int i, int j) {
     const int inW = g.sw / 2, inH = g.sh / 2;
     float up[2];
     const void* planes[2] = {g.pu, g.pv};
@@ -201,7 +204,18 @@ __device__ inline float3 color(const PairGeom& g, int i, int j) {
         }
     }
     const float yy = (float)load_plane(g.py, g.color_u8, (size_t)i * g.sw + j);
-    return make_float3(yy * g.cscale, up[0] * g.cscale, up[1] * g.cscale);
+    return make_float3(yy, up[0], up[1]);
+}
+
+// Decoded colour times the colour scale (the same product either way, from
+// the resident decoded plane when the source has one).
+__device__ inline float3 color(const PairGeom& g, int i, int j) {
+    if (g.col) {
+        const float4 r = g.col[(size_t)i * g.sw + j];
+        return make_float3(r.x * g.cscale, r.y * g.cscale, r.z * g.cscale);
+    }
+    const float3 r = color_raw(g, i, j);
+    return make_float3(r.x * g.cscale, r.y * g.cscale, r.z * g.cscale);
 }
 
 __device__ inline float3 color(const ArrayGeom& g, int i, int j) {
@@ -306,6 +320,32 @@ __device__ inline void tri_num(const Tri& T, int x, int y, float& n1, float& n2)
 __device__ inline bool tri_cover(const Tri& T, float n1, float n2, float& l1, float& l2, float& l3) {
     l1 = n1 / T.den;
     l2 = n2 / T.den;
+    l3 = 1.f - l1 - l2;
+    const float eps = 1e-6f;
+    return l1 >= -eps && l2 >= -eps && l3 >= -eps;
+}
+
+// n / den rounded to float, from rden = 1 / (double)den rounded to double.
+// The double product lies within 2^-52 relative of n / den; a quotient of two
+// floats whose float result is normal lies at least 2^-49 relative from every
+// float rounding midpoint (and never on one: the midpoint's odd 25-bit
+// significand times den's would have to fit n's 24 bits), so the conversion
+// rounds it as the division does.  Results that are subnormal or zero with
+// n != 0 (midpoints can be hit there) take the division itself.
+__device__ inline float div_rcp(float n, float den, double rden) {
+    float q = (float)((double)n * rden);
+    if (!(__builtin_fabsf(q) >= 1.17549435e-38f || n == 0.f)) {
+        asm volatile("");   // keeps the rare division behind a branch (no if-conversion)
+        q = n / den;
+    }
+    return q;
+}
+
+// tri_cover with the divisions by den done through rden = 1 / (double)den.
+__device__ inline bool tri_cover_r(const Tri& T, float n1, float n2, double rden, float& l1, float& l2,
+                                   float& l3) {
+    l1 = div_rcp(n1, T.den, rden);
+    l2 = div_rcp(n2, T.den, rden);
     l3 = 1.f - l1 - l2;
     const float eps = 1e-6f;
     return l1 >= -eps && l2 >= -eps && l3 >= -eps;

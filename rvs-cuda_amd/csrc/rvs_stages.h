@@ -9,6 +9,8 @@
 
 namespace rvs_k {
 
+hipError_t launch_prep_color(const void* y, const void* u, const void* v, int u8, int w, int h, float4* out,
+                             hipStream_t s);
 hipError_t launch_decode_color(const void* y, const void* u, const void* v, int u8, int w, int h, float cs,
                                float* out, hipStream_t s);
 hipError_t launch_decode_depth(const void* raw, int u8, size_t n, float scale, float znear, float zfar,

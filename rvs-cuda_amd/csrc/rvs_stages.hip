@@ -14,10 +14,22 @@ __global__ void k_decode_color(const void* y, const void* u, const void* v, int 
     const int r = blockIdx.y;
     if (x >= w || r >= h) return;
     PairGeom g;
-    g.sw = w; g.sh = h; g.py = y; g.pu = u; g.pv = v; g.color_u8 = u8; g.cscale = cs;
+    g.sw = w; g.sh = h; g.py = y; g.pu = u; g.pv = v; g.color_u8 = u8; g.cscale = cs; g.col = nullptr;
     const float3 c = color(g, r, x);
     const size_t o = (size_t)r * w + x;
     out[3 * o] = c.x; out[3 * o + 1] = c.y; out[3 * o + 2] = c.z;
+}
+
+// Resident decoded colour plane of a source (color_raw per pixel).
+__global__ void k_prep_color(const void* y, const void* u, const void* v, int u8, int w, int h,
+                             float4* __restrict__ out) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (x >= w || r >= h) return;
+    PairGeom g;
+    g.sw = w; g.sh = h; g.py = y; g.pu = u; g.pv = v; g.color_u8 = u8; g.col = nullptr;
+    const float3 c = color_raw(g, r, x);
+    out[(size_t)r * w + x] = make_float4(c.x, c.y, c.z, 0.f);
 }
 
 // read_depth_YUV -> formatDepth (image_loading.cpp:147-213).
@@ -105,6 +117,12 @@ __global__ void k_encode_views(const ViewDesc* __restrict__ views) {
 hipError_t launch_decode_color(const void* y, const void* u, const void* v, int u8, int w, int h, float cs,
                                float* out, hipStream_t s) {
     hipLaunchKernelGGL(k_decode_color, dim3((w + 255) / 256, h), dim3(256), 0, s, y, u, v, u8, w, h, cs, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_prep_color(const void* y, const void* u, const void* v, int u8, int w, int h, float4* out,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_prep_color, dim3((w + 255) / 256, h), dim3(256), 0, s, y, u, v, u8, w, h, out);
     return hipGetLastError();
 }
 
