@@ -737,27 +737,49 @@ __device__ inline float raster_tri(float3 a, float3 b, float3 c, uint32_t id, bo
     if (xa > xb || ya > yb) return 0.f;   // no pixel of this tile can be covered
     T.tv = tri_validity(T, rescale);
     const double rden = 1.0 / (double)T.den;
-    for (int y = ya; y <= yb; ++y) {
-        for (int x = xa; x <= xb; ++x) {
-            float n1, n2;
-            tri_num(T, x, y, n1, n2);
-            if (tri_surely_outside(T, n1, n2)) continue;
-            float l1, l2, l3;
-            if (!(special ? tri_cover(T, n1, n2, l1, l2, l3) : tri_cover_r(T, n1, n2, rden, l1, l2, l3))) continue;
-            const int p = (y - ty0) * kTile + (x - tx0);
-            const float dd = tri_depth(T, l1, l2, l3);
-            if (special || !depth_regular(dd)) {
-                atomicMax(&key2[p], kDeferKey);
-                continue;
-            }
-            // the score only orders candidates outside the ambiguity band:
-            // v_rcp_f32 (1 ulp) keeps its error within 5 * 2^-24 relative
-            const float f = T.tv * __builtin_amdgcn_rcpf(dd * dd * dd);
-            const unsigned long long k = ((unsigned long long)f2u(f) << 32) | id;
-            const unsigned long long old = atomicMax(&key1[p], k);
-            // the candidate that is not (or no longer) the maximum goes to the runner-up slot
-            atomicMax(&key2[p], (uint32_t)((old < k ? old : k) >> 32));
+    auto push = [&](int x, int y) {
+        float n1, n2;
+        tri_num(T, x, y, n1, n2);
+        float l1, l2, l3;
+        if (!(special ? tri_cover(T, n1, n2, l1, l2, l3) : tri_cover_r(T, n1, n2, rden, l1, l2, l3))) return;
+        const int p = (y - ty0) * kTile + (x - tx0);
+        const float dd = tri_depth(T, l1, l2, l3);
+        if (special || !depth_regular(dd)) {
+            atomicMax(&key2[p], kDeferKey);
+            return;
         }
+        // the score only orders candidates outside the ambiguity band:
+        // v_rcp_f32 (1 ulp) keeps its error within 5 * 2^-24 relative
+        const float f = T.tv * __builtin_amdgcn_rcpf(dd * dd * dd);
+        const unsigned long long k = ((unsigned long long)f2u(f) << 32) | id;
+        const unsigned long long old = atomicMax(&key1[p], k);
+        // the candidate that is not (or no longer) the maximum goes to the runner-up slot
+        atomicMax(&key2[p], (uint32_t)((old < k ? old : k) >> 32));
+    };
+    if (xb - xa < 8 && yb - ya < 8) {
+        // Two steps, so that lanes with few candidates do not wait through the
+        // exact test of lanes with large boxes: the division-free rejection
+        // marks the candidate pixels of the box (8 x 8 bit mask), then only
+        // those take the exact coverage test and the key updates.
+        uint64_t mask = 0ull;
+        for (int y = ya; y <= yb; ++y)
+            for (int x = xa; x <= xb; ++x) {
+                float n1, n2;
+                tri_num(T, x, y, n1, n2);
+                if (!tri_surely_outside(T, n1, n2)) mask |= 1ull << ((y - ya) * 8 + (x - xa));
+            }
+        while (mask) {
+            const int bit = __builtin_ctzll(mask);
+            mask &= mask - 1ull;
+            push(xa + (bit & 7), ya + (bit >> 3));
+        }
+    } else {
+        for (int y = ya; y <= yb; ++y)
+            for (int x = xa; x <= xb; ++x) {
+                float n1, n2;
+                tri_num(T, x, y, n1, n2);
+                if (!tri_surely_outside(T, n1, n2)) push(x, y);
+            }
     }
     return T.tv;
 }
