@@ -675,7 +675,7 @@ hipError_t launch_bin(const G* geoms, const ViewDesc* views, const BatchDims& d,
 // Winner's colour, depth and validity at pixel (x, y) for slot `slot` of
 // source quad (i, j): the state the fold holds after accepting that candidate.
 template <class G>
-__device__ inline void resolve(const G& g, int i, int j, int slot, int x, int y, float3& c, float& dep,
+__device__ __forceinline__ void resolve(const G& g, int i, int j, int slot, int x, int y, float3& c, float& dep,
                                float& shape, int out_w, int out_h, const float* vb) {
     int r[3], cc[3];
     float3 v[3];
@@ -697,7 +697,7 @@ __device__ inline void resolve(const G& g, int i, int j, int slot, int x, int y,
 // The same from the corner vertices and validity the coverage pass cached
 // (identical float operations, without re-evaluating vertices or norms).
 template <class G>
-__device__ inline void resolve_cached(const G& g, int i, int j, int slot, float3 a, float3 b, float3 cv, float tv,
+__device__ __forceinline__ void resolve_cached(const G& g, int i, int j, int slot, float3 a, float3 b, float3 cv, float tv,
                                       int x, int y, float3& c, float& dep, float& shape) {
     Tri T;
     tri_load(T, a, b, cv);
@@ -723,7 +723,7 @@ constexpr uint32_t kDeferKey = 0xffffffffu;
 // triangle's validity, or 0 when no pixel of the tile can be covered (then it
 // is not evaluated; the reference computes it before the bounding box test,
 // and it never causes a return, so skipping it changes nothing).
-__device__ inline float raster_tri(float3 a, float3 b, float3 c, uint32_t id, bool prol_false_possible, int tx0,
+__device__ __forceinline__ float raster_tri(float3 a, float3 b, float3 c, uint32_t id, bool prol_false_possible, int tx0,
                                    int ty0, int out_w, int out_h, float rescale, unsigned long long* key1,
                                    uint32_t* key2) {
     Tri T;
@@ -797,7 +797,7 @@ struct RasterCache {
 // candidate by more than the float test's error, so any unique id works; equal
 // scores are deferred to the in-order replay.
 template <class G>
-__device__ inline void raster_quad(const G& g, uint32_t ij, uint32_t e, int tx0, int ty0, int out_w, int out_h,
+__device__ __forceinline__ void raster_quad(const G& g, uint32_t ij, uint32_t e, int tx0, int ty0, int out_w, int out_h,
                                    unsigned long long* key1, uint32_t* key2, RasterCache* C,
                                    const RasterWS& ws, const BatchDims& d, uint32_t vs) {
     int i, j;
@@ -850,7 +850,10 @@ __device__ inline void write_pixel(const ViewDesc& vd, size_t o, const BlendStat
     }
 }
 
-template <class G, int MODE>
+// MULTI: more sources than one phase holds (the phases loop, and the blend
+// state lives across them); otherwise one phase, which keeps the coverage pass
+// free of the blend state's registers.
+template <class G, int MODE, bool MULTI>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_raster(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
                                                 BatchDims d, RasterWS ws) {
     __shared__ unsigned long long key1[kRasterGroup][kTile * kTile];
@@ -871,7 +874,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
     BlendState st;
     float3 rc = make_float3(0.f, 0.f, 0.f);
     float rd = __builtin_inff(), rs = 0.f;
-    for (int s0 = 0; s0 < d.nsrc; s0 += kRasterGroup) {
+    for (int s0 = 0; s0 < (MULTI ? d.nsrc : 1); s0 += kRasterGroup) {
         const int gs = min(kRasterGroup, d.nsrc - s0);
         for (int k = 0; k < gs; ++k) {
             key1[k][tid] = 0ull;
@@ -1188,7 +1191,10 @@ template <class G, int MODE>
 hipError_t launch_raster(const G* geoms, const ViewDesc* views, const BatchDims& d, const RasterWS& ws,
                          hipStream_t s) {
     const unsigned blocks = 8u * ((d.ntiles + 7u) / 8u);
-    hipLaunchKernelGGL((k_raster<G, MODE>), dim3(blocks), dim3(256), 0, s, geoms, views, d, ws);
+    if (d.nsrc > kRasterGroup)
+        hipLaunchKernelGGL((k_raster<G, MODE, true>), dim3(blocks), dim3(256), 0, s, geoms, views, d, ws);
+    else
+        hipLaunchKernelGGL((k_raster<G, MODE, false>), dim3(blocks), dim3(256), 0, s, geoms, views, d, ws);
     return hipGetLastError();
 }
 
