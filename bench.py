@@ -25,11 +25,11 @@ Prints one JSON line:
   serving_mix   two resident scenes (the ERP set + 11 perspective 2048x2048
                 views), requests from 720p to 4K perspective and 2048x1024
                 ERP: closed-loop capacity and Poisson arrivals at 70% of it.
-  roofline      the dominant kernel (k_raster, tile raster + blend) in the
-                served configuration: algorithmic bytes of one launch set of
-                B views (every source sample once at its stored precision +
-                every output sample once) / k_raster's HIP-event duration, vs
-                8 TB/s.
+  roofline      the dominant kernel (k_raster, tile raster + blend) on one
+                launch set of 8 views: algorithmic bytes (every source sample
+                once at its stored precision + every output sample once) /
+                k_raster's HIP-event duration, vs 8 TB/s; `traffic` = its HBM
+                bytes per launch from the committed counter summary.
   cpu_baseline  the single-threaded CPU oracle on one view of the workload.
 """
 import argparse
@@ -64,6 +64,30 @@ def b_alg(rg, targets):
 
 def log(*a):
     print("#", *a, file=sys.stderr, flush=True)
+
+
+ROOFLINE_VIEWS = 8
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_raster_erp4_1080p_8views.md")
+
+
+def roofline_walk(W, rg, n=ROOFLINE_VIEWS):
+    """The launch set the dominant kernel is measured on (and profiled with
+    `bench.py --roofline-only 8`): n random-walk poses at the target size."""
+    t = rg["target"]["cam"]
+    return W.batch_targets(rg, n, seed=4, sizes=((t.w, t.h),))
+
+
+def committed_traffic():
+    """HBM bytes per k_raster launch of the roofline launch set, from the
+    committed counter summary (rocprofv3 FETCH_SIZE and WRITE_SIZE passes,
+    scaled by the same counters on a copy of known size), or None."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def stage_profile(run, reps, R):
@@ -132,7 +156,7 @@ def main():
     (tw, th) = (rg["target"]["cam"].w, rg["target"]["cam"].h)
     torch.cuda.synchronize()
     if args.roofline_only > 0:
-        walk = W.batch_targets(rg, args.roofline_only, seed=4, sizes=((tw, th),))
+        walk = roofline_walk(W, rg, args.roofline_only)
         bs = R.BatchSynthesizer(srcs, walk)
         for _ in range(5):
             bs.run()
@@ -200,12 +224,16 @@ def main():
         srv.close()
     log("served %s" % served)
 
-    # ------------------------------- one launch set of B views: dominant kernel
-    bs = R.BatchSynthesizer(srcs, walk[:B], entry_cap=cap * B)
-    bs.run()
+    # ------------------- dominant kernel: one launch set of ROOFLINE_VIEWS views
+    rwalk = roofline_walk(W, rg)
+    bs = R.BatchSynthesizer(srcs, rwalk)
+    for _ in range(5):
+        bs.run()
+    torch.cuda.synchronize()
     bstage, bcount = stage_profile(bs.run, args.profile_reps, R)
-    balg = b_alg(rg, walk[:B])
+    balg = b_alg(rg, rwalk)
     achieved = balg / (bstage["raster"] * 1e-3) / 1e9
+    traffic = committed_traffic()
     del bs
 
     batched = None
@@ -334,7 +362,7 @@ def main():
         "serving_mix": c4,
         "autotune": {"trials": trials},
         "roofline": {
-            "kernel": "k_raster (tile raster + blend), one launch over %d views" % B,
+            "kernel": "k_raster (tile raster + blend), one launch over %d views" % ROOFLINE_VIEWS,
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
@@ -344,7 +372,8 @@ def main():
             "kernel_ms": round(bstage["raster"], 4),
             "launch_set_stage_ms": {k: round(v, 4) for k, v in bstage.items()},
             "counters": bcount,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic is not None else None,
         },
     }
 
