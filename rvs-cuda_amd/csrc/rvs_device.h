@@ -325,32 +325,6 @@ __device__ inline bool tri_cover(const Tri& T, float n1, float n2, float& l1, fl
     return l1 >= -eps && l2 >= -eps && l3 >= -eps;
 }
 
-// n / den rounded to float, from rden = 1 / (double)den rounded to double.
-// The double product lies within 2^-52 relative of n / den; a quotient of two
-// floats whose float result is normal lies at least 2^-49 relative from every
-// float rounding midpoint (and never on one: the midpoint's odd 25-bit
-// significand times den's would have to fit n's 24 bits), so the conversion
-// rounds it as the division does.  Results that are subnormal or zero with
-// n != 0 (midpoints can be hit there) take the division itself.
-__device__ inline float div_rcp(float n, float den, double rden) {
-    float q = (float)((double)n * rden);
-    if (!(__builtin_fabsf(q) >= 1.17549435e-38f || n == 0.f)) {
-        asm volatile("");   // keeps the rare division behind a branch (no if-conversion)
-        q = n / den;
-    }
-    return q;
-}
-
-// tri_cover with the divisions by den done through rden = 1 / (double)den.
-__device__ inline bool tri_cover_r(const Tri& T, float n1, float n2, double rden, float& l1, float& l2,
-                                   float& l3) {
-    l1 = div_rcp(n1, T.den, rden);
-    l2 = div_rcp(n2, T.den, rden);
-    l3 = 1.f - l1 - l2;
-    const float eps = 1e-6f;
-    return l1 >= -eps && l2 >= -eps && l3 >= -eps;
-}
-
 // Barycentric coordinates at pixel (x, y); returns coverage.
 __device__ inline bool tri_pixel(const Tri& T, int x, int y, float& l1, float& l2, float& l3) {
     float n1, n2;

@@ -736,12 +736,11 @@ __device__ __forceinline__ float raster_tri(float3 a, float3 b, float3 c, uint32
     if (!special) tri_tight(T, xa, xb, ya, yb);
     if (xa > xb || ya > yb) return 0.f;   // no pixel of this tile can be covered
     T.tv = tri_validity(T, rescale);
-    const double rden = 1.0 / (double)T.den;
     auto push = [&](int x, int y) {
         float n1, n2;
         tri_num(T, x, y, n1, n2);
         float l1, l2, l3;
-        if (!(special ? tri_cover(T, n1, n2, l1, l2, l3) : tri_cover_r(T, n1, n2, rden, l1, l2, l3))) return;
+        if (!tri_cover(T, n1, n2, l1, l2, l3)) return;
         const int p = (y - ty0) * kTile + (x - tx0);
         const float dd = tri_depth(T, l1, l2, l3);
         if (special || !depth_regular(dd)) {

@@ -99,43 +99,3 @@ def test_slivers_keep_the_full_box():
     guard = _tight(A, B, C, den)[0]
     assert not guard.any()
 
-
-def _div_rcp(n, den):
-    """div_rcp in rvs_device.h: (float)((double)n * RN_double(1 / den)), the
-    division itself for subnormal or zero results of a nonzero n."""
-    q = (n.astype(np.float64) * (1.0 / den.astype(np.float64))).astype(f)
-    slow = (np.abs(q) < f(1.17549435e-38)) & (n != 0)
-    return np.where(slow, n / den, q)
-
-
-def test_reciprocal_division_rounds_like_division():
-    """Random quotients, and quotients placed next to float rounding midpoints
-    (n = RN(den * m) for a midpoint m), over the whole exponent range."""
-    rng = np.random.default_rng(7)
-    N = 2_000_000
-    with np.errstate(over="ignore", under="ignore", invalid="ignore", divide="ignore"):
-        for rep in range(4):
-            den = (rng.uniform(1, 2, N) * 2.0 ** rng.integers(-60, 60, N)).astype(f)
-            n = (rng.uniform(-2, 2, N) * 2.0 ** rng.integers(-60, 60, N)).astype(f)
-            # midpoints: odd 25-bit significand M * 2^e
-            M = (rng.integers(1 << 23, 1 << 24, N) * 2 + 1).astype(np.float64)
-            mid = M * 2.0 ** rng.integers(-100, 60, N).astype(np.float64) / 2.0 ** 24
-            n2 = (mid * den.astype(np.float64)).astype(f)
-            # and the neighbours of those numerators
-            n3 = np.nextafter(n2, f(np.inf))
-            n4 = np.nextafter(n2, f(-np.inf))
-            for num in (n, n2, n3, n4, -n2):
-                ok = den > 0
-                a, b = num[ok], den[ok]
-                exact = a / b
-                fast = _div_rcp(a, b)
-                same = (exact.view(np.uint32) == fast.view(np.uint32)) | (np.isnan(exact) & np.isnan(fast))
-                assert same.all(), (a[~same][:4], b[~same][:4], exact[~same][:4], fast[~same][:4])
-
-
-def test_reciprocal_division_subnormal_results():
-    """Quotients in the subnormal range (exact midpoints occur there) go
-    through the division."""
-    den = np.array([3.0, 2.0, 7.0, 1.5], f)
-    n = np.array([3 * 1.5e-45, 3e-45, 7 * 2.1e-44, 1e-40], f)
-    assert (_div_rcp(n, den).view(np.uint32) == (n / den).view(np.uint32)).all()
