@@ -137,16 +137,40 @@ __global__ void k_ccl_flatten(const ViewDesc* __restrict__ views, InpaintWS ws) 
     }
 }
 
+// Component bounding boxes.  The lanes of a wave that belong to one component
+// are reduced first (usually the whole wave: a large hole covers it), so each
+// component takes four atomics per wave instead of four per pixel --
+// same-address device atomics are serialised at the memory side.
 __global__ void k_comp_bbox(const ViewDesc* __restrict__ views, InpaintWS ws) {
-    RVS_VIEW_PIXEL(vd, p, gp)
-    const int r = ws.label[gp];
-    if (r < 0) return;
-    const int id = ws.cid[r];
-    const int x = p % vd.w, y = p / vd.w;
-    atomicMin(&ws.comp[4 * id + 0], x);
-    atomicMax(&ws.comp[4 * id + 1], x);
-    atomicMin(&ws.comp[4 * id + 2], y);
-    atomicMax(&ws.comp[4 * id + 3], y);
+    const ViewDesc& vd = views[blockIdx.y];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = p < vd.w * vd.h;
+    const int gp = (int)vd.px_base + (in ? p : 0);
+    const int r = in ? ws.label[gp] : -1;
+    const int id = r >= 0 ? ws.cid[r] : -1;
+    const int x = in ? p % vd.w : 0, y = in ? p / vd.w : 0;
+    const int lane = __lane_id();
+    bool pending = id >= 0;
+    while (__any(pending)) {
+        const int leader = __ffsll((unsigned long long)__ballot(pending)) - 1;
+        const int lid = __shfl(id, leader);
+        const bool mine = pending && id == lid;
+        int x0 = mine ? x : INT_MAX, x1 = mine ? x : INT_MIN;
+        int y0 = mine ? y : INT_MAX, y1 = mine ? y : INT_MIN;
+        for (int off = 32; off > 0; off >>= 1) {
+            x0 = min(x0, __shfl_xor(x0, off));
+            x1 = max(x1, __shfl_xor(x1, off));
+            y0 = min(y0, __shfl_xor(y0, off));
+            y1 = max(y1, __shfl_xor(y1, off));
+        }
+        if (lane == leader) {
+            atomicMin(&ws.comp[4 * lid + 0], x0);
+            atomicMax(&ws.comp[4 * lid + 1], x1);
+            atomicMin(&ws.comp[4 * lid + 2], y0);
+            atomicMax(&ws.comp[4 * lid + 3], y1);
+        }
+        pending = pending && !mine;
+    }
 }
 
 // S -> (S.v + k < C.v + t) ? (S.v + k, S.src) : C
@@ -309,6 +333,152 @@ __device__ void solve_component(const Region& R, const InpaintWS& ws, Fn* wave_t
     }
 }
 
+constexpr int kBigPx = 4;   // consecutive pixels per thread in a line step of a large component
+
+__device__ inline void apply_fn(const Fn& f, int& v, int& s) {
+    if (v + f.k < f.cv + f.t) { v = v + f.k; } else { v = f.cv; s = f.cs; }
+}
+
+// Map value of view pixel (x, y) for component R from its three words, loaded
+// unconditionally so that a line step issues all its loads at once.
+__device__ inline void map_value(const Region& R, int lab, int d, int sr, int p, int& v, int& s) {
+    if (lab < 0) { v = 0; s = p; }
+    else if (lab != R.root) { v = kOther; s = -1; }
+    else { v = d; s = sr; }
+}
+
+// A component whose box exceeds the LDS copy, on the global map: the same
+// sweeps and line order as solve_component, but each thread takes kBigPx
+// consecutive pixels of a line (composed in registers before the block-wide
+// scan) and loads every word it needs (the pixels and their neighbours)
+// before using any, so a line step costs about one memory latency.
+__device__ void solve_big(const Region& R, const InpaintWS& ws, Fn* wave_tot, int* s_carry) {
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const bool cols = R.bh > R.bw;
+    const int nlines = cols ? R.bw : R.bh;
+    const int llen = cols ? R.bh : R.bw;
+    const int span = 256 * kBigPx;
+    const int max_sweeps = 4 * (R.w + R.h) + 8;   // each changing sweep lowers a distance: a guard only
+    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+        const bool fwd = (sweep & 1) == 0;
+        const int rs = cols ? (fwd ? 2 : 3) : (fwd ? 1 : 4);
+        // neighbour offsets in the reference order left(1) up(2) down(3) right(4)
+        const int ndx[4] = {-1, 0, 0, 1}, ndy[4] = {0, -1, 1, 0};
+        bool changed = false;
+        for (int li = 0; li < nlines; ++li) {
+            const int line = fwd ? li : nlines - 1 - li;
+            int Lv = kAbsent, Ls = -1;
+            {
+                const int pb = fwd ? -1 : llen;
+                const int bx = cols ? R.x0 + line : R.x0 + pb;
+                const int by = cols ? R.y0 + pb : R.y0 + line;
+                if (bx >= 0 && bx < R.w && by >= 0 && by < R.h) {
+                    int v, sr;
+                    region_get<false>(R, ws, bx, by, v, sr);
+                    if (v == 0) { Lv = 0; Ls = sr; }
+                }
+            }
+            for (int cb = 0; cb < llen; cb += span) {
+                // loads: kBigPx pixels x (itself + 4 neighbours) x (label, dist, src)
+                int lab[kBigPx][5], dd[kBigPx][5], ss[kBigPx][5], pp[kBigPx][5];
+                bool ok[kBigPx][5];
+#pragma unroll
+                for (int k = 0; k < kBigPx; ++k) {
+                    const int e = min(cb + tid * kBigPx + k, llen - 1);
+                    const int pos = fwd ? e : llen - 1 - e;
+                    const int x = cols ? R.x0 + line : R.x0 + pos;
+                    const int y = cols ? R.y0 + pos : R.y0 + line;
+#pragma unroll
+                    for (int n = 0; n < 5; ++n) {
+                        const int nx = n ? x + ndx[n - 1] : x, ny = n ? y + ndy[n - 1] : y;
+                        ok[k][n] = nx >= 0 && nx < R.w && ny >= 0 && ny < R.h;
+                        const int q = R.base + (ok[k][n] ? ny * R.w + nx : y * R.w + x);
+                        pp[k][n] = q;
+                        lab[k][n] = ws.label[q];
+                        dd[k][n] = ws.dist[q];
+                        ss[k][n] = ws.src[q];
+                    }
+                }
+                Fn f[kBigPx];
+                int cur[kBigPx];   // > 0: a hole of this component with that distance
+#pragma unroll
+                for (int k = 0; k < kBigPx; ++k) {
+                    const int e = cb + tid * kBigPx + k;
+                    f[k].k = 1; f[k].cv = kAbsent; f[k].cs = -1; f[k].t = 0;
+                    cur[k] = 0;
+                    int v, sr;
+                    map_value(R, lab[k][0], dd[k][0], ss[k][0], pp[k][0], v, sr);
+                    if (e < llen) {
+                        if (v == 0) {
+                            f[k].cv = 0; f[k].cs = sr;
+                        } else if (v != kOther) {
+                            cur[k] = v;
+                            int bv = v, br = 0, bs = sr;
+#pragma unroll
+                            for (int n = 1; n < 5; ++n) {
+                                if (n == rs || !ok[k][n]) continue;
+                                int nv, ns;
+                                map_value(R, lab[k][n], dd[k][n], ss[k][n], pp[k][n], nv, ns);
+                                if (nv != kOther) cand(bv, br, bs, nv + 1, n, ns);
+                            }
+                            f[k].cv = bv; f[k].cs = bs; f[k].t = (rs < br) ? 1 : 0;
+                        }
+                    }
+                }
+                Fn A = f[0];
+#pragma unroll
+                for (int k = 1; k < kBigPx; ++k) A = compose(A, f[k]);
+                for (int off = 1; off < 64; off <<= 1) {
+                    Fn g;
+                    g.cv = __shfl_up(A.cv, off);
+                    g.cs = __shfl_up(A.cs, off);
+                    g.t = __shfl_up(A.t, off);
+                    g.k = __shfl_up(A.k, off);
+                    if (lane >= off) A = compose(g, A);
+                }
+                Fn ex;
+                ex.cv = __shfl_up(A.cv, 1);
+                ex.cs = __shfl_up(A.cs, 1);
+                ex.t = __shfl_up(A.t, 1);
+                ex.k = __shfl_up(A.k, 1);
+                if (lane == 63) wave_tot[wave] = A;
+                __syncthreads();   // all reads of this span are done before any write
+                Fn pre;
+                bool have = false;
+                for (int wv = 0; wv < wave; ++wv) {
+                    pre = have ? compose(pre, wave_tot[wv]) : wave_tot[wv];
+                    have = true;
+                }
+                if (lane > 0) {
+                    pre = have ? compose(pre, ex) : ex;
+                    have = true;
+                }
+                int Sv = Lv, Ss = Ls;
+                if (have) apply_fn(pre, Sv, Ss);
+                const int last = min(cb + span, llen) - 1;
+#pragma unroll
+                for (int k = 0; k < kBigPx; ++k) {
+                    apply_fn(f[k], Sv, Ss);
+                    const int e = cb + tid * kBigPx + k;
+                    if (e <= last) {
+                        if (cur[k] > 0 && Sv != cur[k]) {
+                            changed = true;
+                            ws.dist[pp[k][0]] = Sv;
+                            ws.src[pp[k][0]] = Ss;
+                        }
+                        if (e == last) { s_carry[0] = Sv; s_carry[1] = Ss; }
+                    }
+                }
+                __syncthreads();
+                Lv = s_carry[0]; Ls = s_carry[1];
+                __syncthreads();
+            }
+        }
+        if (!__syncthreads_or(changed)) break;
+    }
+}
+
 // View owning global pixel gp (views are sorted by px_base).
 __device__ inline int view_of_pixel(const ViewDesc* views, int nviews, int gp) {
     int lo = 0, hi = nviews - 1;
@@ -327,12 +497,18 @@ __global__ void __launch_bounds__(256) k_inp_solve(const ViewDesc* __restrict__ 
     __shared__ int lsrc[kLdsCap];
     const int tid = threadIdx.x;
     const int ncomp = ws.counters[0];
+    // two passes over the components: the large ones (they take longest) are
+    // all started before any small one
+    for (int pass = 0; pass < 2; ++pass) {
     while (true) {
-        if (tid == 0) s_comp = atomicAdd(&ws.counters[1], 1);
+        if (tid == 0) s_comp = atomicAdd(&ws.counters[1 + pass], 1);
         __syncthreads();
         const int c = s_comp;
         __syncthreads();
-        if (c >= ncomp) return;
+        if (c >= ncomp) break;
+        if ((ws.comp[4 * c + 1] - ws.comp[4 * c + 0] + 1) * (ws.comp[4 * c + 3] - ws.comp[4 * c + 2] + 1) <= kLdsCap
+            ? pass == 0 : pass == 1)
+            continue;   // the same decision in every thread of the block
         Region R;
         R.root = ws.root[c];
         const ViewDesc& vd = views[view_of_pixel(views, nviews, R.root)];
@@ -361,9 +537,10 @@ __global__ void __launch_bounds__(256) k_inp_solve(const ViewDesc* __restrict__ 
                 if (ws.label[p] == R.root) { ws.dist[p] = ldist[k]; ws.src[p] = lsrc[k]; }
             }
         } else {
-            solve_component<false>(R, ws, wave_tot, s_carry);
+            solve_big(R, ws, wave_tot, s_carry);
         }
         __syncthreads();
+    }
     }
 }
 
@@ -407,7 +584,7 @@ InpaintWS inpaint_ws_carve(void* base, size_t npx, int nviews) {
 hipError_t launch_inpaint(const ViewDesc* views, int nviews, int max_w, int max_h, const InpaintWS& ws,
                           int32_t* map_out, hipStream_t s) {
     const dim3 b(256), g((unsigned)(((size_t)max_w * max_h + 255) / 256), (unsigned)nviews);
-    hipError_t e = launch_reset_u32((uint32_t*)ws.counters, 2, s);
+    hipError_t e = launch_reset_u32((uint32_t*)ws.counters, 3, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_inp_init, g, b, 0, s, views, ws);
     hipLaunchKernelGGL(k_ccl_local,

@@ -292,6 +292,23 @@ def test_inpaint_patterns(torch_cuda):
     _inpaint_case(torch, q, "lattice")
 
 
+def test_inpaint_large_components(torch_cuda):
+    """Components far beyond the LDS box: lines longer than one block step
+    (2048 pixels), along rows (wide box) and columns (tall box), and a wedge."""
+    torch = torch_cuda
+    rng = np.random.default_rng(8)
+    for tag, (h, w) in (("wide", (70, 2600)), ("tall", (2600, 70))):
+        q = np.ones((h, w), np.float32)
+        q[5:h - 9, 3:w - 11] = 0.0
+        q[rng.random((h, w)) < 0.002] = 1.0   # a few known islands inside
+        _inpaint_case(torch, q, tag)
+    h, w = 360, 640
+    yy, xx = np.mgrid[0:h, 0:w]
+    q = np.where(xx * 0.6 + yy > 300, 0.0, 1.0).astype(np.float32)   # an uncovered wedge
+    q[rng.random((h, w)) < 0.01] = 0.0
+    _inpaint_case(torch, q, "wedge")
+
+
 # ------------------------------------------------------------------ encode
 
 @pytest.mark.parametrize("bd", [8, 10])
