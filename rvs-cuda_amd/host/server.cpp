@@ -19,11 +19,7 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 size_t plane_bytes(const rvs_target& t) { return t.bitdepth <= 8 ? 1 : 2; }
 
-// Device bytes of one request's outputs: colour map + encoded planes.
-size_t output_bytes(const rvs_target& t) {
-    const size_t px = (size_t)t.cam.w * t.cam.h;
-    return align256(px * 3 * sizeof(float)) + align256(px * plane_bytes(t)) + 2 * align256(px / 4 * plane_bytes(t));
-}
+size_t color_bytes(const rvs_target& t) { return align256((size_t)t.cam.w * t.cam.h * 3 * sizeof(float)); }
 
 size_t yuv_bytes(const rvs_target& t) { return (size_t)t.cam.w * t.cam.h * 3 / 2 * plane_bytes(t); }
 
@@ -40,9 +36,16 @@ void hip_check(hipError_t e, const char* what) {
 
 }  // namespace
 
+// A slot holds one batch from launch until its outputs are on the host: its
+// launch set runs on the shared compute stream (batches of all slots in
+// order, so the device is never split between them), and its YUV planes
+// (contiguous at the start of the arena, laid out as in the host staging
+// buffer) are copied back on the slot's own stream, overlapping the next
+// batch's compute.
 struct Server::Slot {
-    hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
+    hipStream_t stream = nullptr;      // host copies (and capacity re-runs)
+    hipEvent_t computed = nullptr;     // the batch's launch set has finished
+    hipEvent_t done = nullptr;         // ... and its outputs are on the host
     char* arena = nullptr;
     size_t arena_bytes = 0;
     uint8_t* pinned = nullptr;
@@ -55,6 +58,7 @@ struct Server::Slot {
     std::vector<rvs_view_out> outs;
     std::vector<size_t> host_off;
     size_t ws_bytes = 0;
+    size_t yuv_bytes = 0;    // the batch's encoded planes at the start of the arena
     size_t ws_off = 0;
     uint32_t cap = 0;        // entry capacity the current batch was launched with
 };
@@ -65,10 +69,14 @@ Server::Server(std::vector<std::vector<rvs_source>> scenes, const rvs_serve_opts
     if (m_opt.slots < 1) m_opt.slots = 1;
     if (m_opt.max_batch < 1) m_opt.max_batch = 1;
     const size_t share = m_opt.memory_budget / (size_t)m_opt.slots;
+    hipStream_t cs = nullptr;
+    hip_check(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "stream create");
+    m_compute = cs;
     for (int k = 0; k < m_opt.slots; ++k) {
         Slot* s = new Slot;
         m_slots.push_back(s);
         hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "stream create");
+        hip_check(hipEventCreateWithFlags(&s->computed, hipEventDisableTiming), "event create");
         hip_check(hipEventCreateWithFlags(&s->done, hipEventDisableTiming), "event create");
         hip_check(hipMalloc((void**)&s->arena, share), "server arena");
         s->arena_bytes = share;
@@ -89,14 +97,17 @@ Server::~Server() {
     }
     m_cv.notify_all();
     if (m_thread.joinable()) m_thread.join();
+    if (m_compute) (void)hipStreamSynchronize((hipStream_t)m_compute);
     for (Slot* s : m_slots) {
         if (s->stream) (void)hipStreamSynchronize(s->stream);
         if (s->pinned) (void)hipHostFree(s->pinned);
         if (s->arena) (void)hipFree(s->arena);
         if (s->done) (void)hipEventDestroy(s->done);
+        if (s->computed) (void)hipEventDestroy(s->computed);
         if (s->stream) (void)hipStreamDestroy(s->stream);
         delete s;
     }
+    if (m_compute) (void)hipStreamDestroy((hipStream_t)m_compute);
 }
 
 double Server::now_ms() const {
@@ -181,7 +192,7 @@ int Server::run_closed(const rvs_request* reqs, int n, int concurrency) {
 bool Server::launch(Slot& s) {
     std::vector<int> batch;
     std::vector<rvs_target> tgts;
-    size_t ws = 0, outb = 0, hostb = 0;
+    size_t ws = 0, colb = 0, hostb = 0;
     {
         std::lock_guard<std::mutex> lk(m_mu);
         if (m_pending.empty()) return false;
@@ -193,9 +204,9 @@ bool Server::launch(Slot& s) {
             tgts.push_back(r.target);
             const size_t w2 = rvs_synth_batch_workspace_size(S.data(), (int)S.size(), tgts.data(), (int)tgts.size(),
                                                              batch_cap(s.per_view, tgts.size()));
-            const size_t o2 = outb + output_bytes(r.target);
+            const size_t c2 = colb + color_bytes(r.target);
             const size_t h2 = hostb + yuv_bytes(r.target);
-            const bool fits = w2 > 0 && o2 + w2 <= s.arena_bytes && (!s.pinned || h2 <= s.pinned_bytes);
+            const bool fits = w2 > 0 && align256(h2) + c2 + w2 <= s.arena_bytes && (!s.pinned || h2 <= s.pinned_bytes);
             if (!fits) {
                 tgts.pop_back();
                 if (batch.empty()) {
@@ -211,7 +222,7 @@ bool Server::launch(Slot& s) {
                 }
                 break;
             }
-            ws = w2; outb = o2; hostb = h2;
+            ws = w2; colb = c2; hostb = h2;
             batch.push_back(*it);
             it = m_pending.erase(it);
         }
@@ -224,36 +235,36 @@ bool Server::launch(Slot& s) {
     s.ws_bytes = ws;
     s.outs.assign(batch.size(), rvs_view_out{});
     s.host_off.assign(batch.size(), 0);
-    // outputs first, workspace after them (so it can grow in place)
-    size_t o = 0, h = 0;
+    // the batch's YUV planes first, contiguous as in the host staging buffer;
+    // then the colour maps; the workspace after them (so it can grow in place)
+    size_t h = 0;
     for (size_t k = 0; k < batch.size(); ++k) {
         const rvs_target& t = tgts[k];
         const size_t px = (size_t)t.cam.w * t.cam.h, b = plane_bytes(t);
         rvs_view_out& vo = s.outs[k];
-        vo.color = (float*)(s.arena + o); o += align256(px * 3 * sizeof(float));
-        vo.y = s.arena + o; o += align256(px * b);
-        vo.u = s.arena + o; o += align256(px / 4 * b);
-        vo.v = s.arena + o; o += align256(px / 4 * b);
+        vo.y = s.arena + h;
+        vo.u = s.arena + h + px * b;
+        vo.v = s.arena + h + px * b + px / 4 * b;
         s.host_off[k] = h;
         h += yuv_bytes(t);
+    }
+    s.yuv_bytes = h;
+    size_t o = align256(h);
+    for (size_t k = 0; k < batch.size(); ++k) {
+        s.outs[k].color = (float*)(s.arena + o);
+        o += color_bytes(tgts[k]);
     }
     s.ws_off = o;
     const double t_launch = now_ms();
     s.cap = batch_cap(s.per_view, tgts.size());
+    hipStream_t cs = (hipStream_t)m_compute;
     int rc = rvs_synthesize_batch_async(S.data(), (int)S.size(), tgts.data(), (int)tgts.size(), s.arena + s.ws_off,
-                                        ws, s.cap, s.outs.data(), s.stream);
-    if (rc == RVS_OK && s.pinned) {
-        for (size_t k = 0; k < batch.size() && rc == RVS_OK; ++k) {
-            const rvs_target& t = tgts[k];
-            const size_t px = (size_t)t.cam.w * t.cam.h, b = plane_bytes(t);
-            uint8_t* dst = s.pinned + s.host_off[k];
-            if (hipMemcpyAsync(dst, s.outs[k].y, px * b, hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
-                hipMemcpyAsync(dst + px * b, s.outs[k].u, px / 4 * b, hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
-                hipMemcpyAsync(dst + px * b + px / 4 * b, s.outs[k].v, px / 4 * b, hipMemcpyDeviceToHost, s.stream) !=
-                    hipSuccess)
-                rc = RVS_E_HIP;
-        }
-    }
+                                        ws, s.cap, s.outs.data(), cs);
+    if (rc == RVS_OK && hipEventRecord(s.computed, cs) != hipSuccess) rc = RVS_E_HIP;
+    if (rc == RVS_OK && hipStreamWaitEvent(s.stream, s.computed, 0) != hipSuccess) rc = RVS_E_HIP;
+    if (rc == RVS_OK && s.pinned &&
+        hipMemcpyAsync(s.pinned, s.arena, s.yuv_bytes, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
+        rc = RVS_E_HIP;
     if (rc == RVS_OK && hipEventRecord(s.done, s.stream) != hipSuccess) rc = RVS_E_HIP;
     const std::string msg = rc == RVS_OK ? std::string() : std::string(rvs_last_error());
     std::lock_guard<std::mutex> lk(m_mu);
@@ -264,6 +275,7 @@ bool Server::launch(Slot& s) {
         m_stats[k].batch_views = (int)batch.size();
     }
     if (rc != RVS_OK) {
+        (void)hipStreamSynchronize(cs);
         (void)hipStreamSynchronize(s.stream);
         for (int k : batch) {
             m_stats[k].rc = rc;
@@ -297,18 +309,9 @@ void Server::finish(Slot& s) {
             rc = rvs_synthesize_batch(S.data(), (int)S.size(), s.tgts.data(), (int)s.tgts.size(), s.arena + s.ws_off,
                                       w2, cap, s.outs.data(), nullptr, s.stream);
             if (rc == RVS_OK && s.pinned) {
-                for (size_t k = 0; k < s.reqs.size(); ++k) {
-                    const rvs_target& t = s.tgts[k];
-                    const size_t px = (size_t)t.cam.w * t.cam.h, b = plane_bytes(t);
-                    uint8_t* dst = s.pinned + s.host_off[k];
-                    if (hipMemcpyAsync(dst, s.outs[k].y, px * b, hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
-                        hipMemcpyAsync(dst + px * b, s.outs[k].u, px / 4 * b, hipMemcpyDeviceToHost, s.stream) !=
-                            hipSuccess ||
-                        hipMemcpyAsync(dst + px * b + px / 4 * b, s.outs[k].v, px / 4 * b, hipMemcpyDeviceToHost,
-                                       s.stream) != hipSuccess)
-                        rc = RVS_E_HIP;
-                }
-                if (rc == RVS_OK && hipStreamSynchronize(s.stream) != hipSuccess) rc = RVS_E_HIP;
+                if (hipMemcpyAsync(s.pinned, s.arena, s.yuv_bytes, hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
+                    hipStreamSynchronize(s.stream) != hipSuccess)
+                    rc = RVS_E_HIP;
             }
         }
     }
