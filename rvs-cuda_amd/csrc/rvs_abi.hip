@@ -143,7 +143,13 @@ void plan_raster(Layout& L, RasterPlan& R, int nsrc, int nviews, uint32_t ntiles
     d.bins_x = (uint32_t)((max_sw - 1 + rvs_k::kBinQX - 1) / rvs_k::kBinQX);
     d.bins_y = (uint32_t)((max_sh - 1 + rvs_k::kBinQY - 1) / rvs_k::kBinQY);
     const uint64_t items = (uint64_t)d.bins_x * d.bins_y * nsrc * nviews;
-    const uint64_t keep = items / 8 > 4096 ? items / 8 : 4096;
+    // Work items that keep their tile ranges and vertex window (5.9 KB each):
+    // at least an eighth of the items before culling, and enough for the entry
+    // capacity (a surviving item bins ~300 entries in the measured scenes, so
+    // capacity / 240 items cover a launch set that fills it); the rest
+    // re-evaluate vertices.
+    uint64_t keep = items / 8 > 4096 ? items / 8 : 4096;
+    if ((uint64_t)d.cap / 240 > keep) keep = (uint64_t)d.cap / 240;
     d.range_cap = (uint32_t)(keep < items ? keep : items);
     const size_t nbins = (size_t)nsrc * ntiles;
     R.status = L.take(16 * sizeof(uint32_t));

@@ -20,7 +20,7 @@ struct InpaintWS {
     int* cid;       // component id of a root
     int* comp;      // 4 per component: x0, x1, y0, y1 (view coordinates)
     int* root;      // root pixel of a component
-    int* counters;  // [0] components, [1] [2] work counters of the large / small component passes
+    int* counters;  // [0] components, [2] work counter of the small-component pass
 };
 
 size_t inpaint_ws_bytes(size_t npx, int nviews);

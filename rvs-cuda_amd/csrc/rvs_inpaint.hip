@@ -137,10 +137,11 @@ __global__ void k_ccl_flatten(const ViewDesc* __restrict__ views, InpaintWS ws) 
     }
 }
 
-// Component bounding boxes.  The lanes of a wave that belong to one component
-// are reduced first (usually the whole wave: a large hole covers it), so each
-// component takes four atomics per wave instead of four per pixel --
-// same-address device atomics are serialised at the memory side.
+// Component bounding boxes.  The lanes of a wave that belong to the component
+// of its first hole are reduced first (usually the whole wave when a large
+// hole covers it), so a large component takes four atomics per wave instead of
+// four per pixel -- same-address device atomics are serialised at the memory
+// side; holes of other components in the wave use plain atomics.
 __global__ void k_comp_bbox(const ViewDesc* __restrict__ views, InpaintWS ws) {
     const ViewDesc& vd = views[blockIdx.y];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -150,26 +151,32 @@ __global__ void k_comp_bbox(const ViewDesc* __restrict__ views, InpaintWS ws) {
     const int id = r >= 0 ? ws.cid[r] : -1;
     const int x = in ? p % vd.w : 0, y = in ? p / vd.w : 0;
     const int lane = __lane_id();
-    bool pending = id >= 0;
-    while (__any(pending)) {
-        const int leader = __ffsll((unsigned long long)__ballot(pending)) - 1;
-        const int lid = __shfl(id, leader);
-        const bool mine = pending && id == lid;
-        int x0 = mine ? x : INT_MAX, x1 = mine ? x : INT_MIN;
-        int y0 = mine ? y : INT_MAX, y1 = mine ? y : INT_MIN;
-        for (int off = 32; off > 0; off >>= 1) {
-            x0 = min(x0, __shfl_xor(x0, off));
-            x1 = max(x1, __shfl_xor(x1, off));
-            y0 = min(y0, __shfl_xor(y0, off));
-            y1 = max(y1, __shfl_xor(y1, off));
-        }
-        if (lane == leader) {
-            atomicMin(&ws.comp[4 * lid + 0], x0);
-            atomicMax(&ws.comp[4 * lid + 1], x1);
-            atomicMin(&ws.comp[4 * lid + 2], y0);
-            atomicMax(&ws.comp[4 * lid + 3], y1);
-        }
-        pending = pending && !mine;
+    const uint64_t act = __ballot(id >= 0);
+    if (act == 0ull) return;
+    // one reduced round for the component of the wave's first hole ...
+    const int leader = __ffsll((unsigned long long)act) - 1;
+    const int lid = __shfl(id, leader);
+    const bool mine = id >= 0 && id == lid;
+    int x0 = mine ? x : INT_MAX, x1 = mine ? x : INT_MIN;
+    int y0 = mine ? y : INT_MAX, y1 = mine ? y : INT_MIN;
+    for (int off = 32; off > 0; off >>= 1) {
+        x0 = min(x0, __shfl_xor(x0, off));
+        x1 = max(x1, __shfl_xor(x1, off));
+        y0 = min(y0, __shfl_xor(y0, off));
+        y1 = max(y1, __shfl_xor(y1, off));
+    }
+    if (lane == leader) {
+        atomicMin(&ws.comp[4 * lid + 0], x0);
+        atomicMax(&ws.comp[4 * lid + 1], x1);
+        atomicMin(&ws.comp[4 * lid + 2], y0);
+        atomicMax(&ws.comp[4 * lid + 3], y1);
+    }
+    // ... and plain atomics for holes of other (small) components
+    if (id >= 0 && !mine) {
+        atomicMin(&ws.comp[4 * id + 0], x);
+        atomicMax(&ws.comp[4 * id + 1], x);
+        atomicMin(&ws.comp[4 * id + 2], y);
+        atomicMax(&ws.comp[4 * id + 3], y);
     }
 }
 
@@ -498,13 +505,21 @@ __global__ void __launch_bounds__(256) k_inp_solve(const ViewDesc* __restrict__ 
     const int tid = threadIdx.x;
     const int ncomp = ws.counters[0];
     // two passes over the components: the large ones (they take longest) are
-    // all started before any small one
+    // all started first -- found by a static stride, so the small ones cost no
+    // queue atomics here -- then the small ones are pulled from a queue
     for (int pass = 0; pass < 2; ++pass) {
+    int next = blockIdx.x;
     while (true) {
-        if (tid == 0) s_comp = atomicAdd(&ws.counters[1 + pass], 1);
-        __syncthreads();
-        const int c = s_comp;
-        __syncthreads();
+        int c;
+        if (pass == 0) {
+            c = next;
+            next += gridDim.x;
+        } else {
+            if (tid == 0) s_comp = atomicAdd(&ws.counters[2], 1);
+            __syncthreads();
+            c = s_comp;
+            __syncthreads();
+        }
         if (c >= ncomp) break;
         if ((ws.comp[4 * c + 1] - ws.comp[4 * c + 0] + 1) * (ws.comp[4 * c + 3] - ws.comp[4 * c + 2] + 1) <= kLdsCap
             ? pass == 0 : pass == 1)

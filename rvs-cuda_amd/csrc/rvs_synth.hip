@@ -378,8 +378,8 @@ __device__ inline bool quad_tiles(const Quad& Q, int& tx0, int& tx1, int& ty0, i
     return true;
 }
 
-// FLAT: one block per kept work item (counting pass only); otherwise
-// persistent blocks stride over the items after the kept ones.
+// FLAT: blocks stride over the kept work items (counting pass only);
+// otherwise persistent blocks stride over the items after the kept ones.
 template <class G, bool FILL, bool FLAT>
 __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
                                              BatchDims d, RasterWS ws) {
@@ -396,7 +396,7 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
     // pass of the kept items is k_fill_kept.
     static_assert(!(FILL && FLAT), "kept items are filled by k_fill_kept");
     const uint64_t lim = FLAT ? min(nwork, d.range_cap) : nwork;
-    const uint64_t step = FLAT ? lim : gridDim.x;
+    const uint64_t step = gridDim.x;
     for (uint64_t wi64 = (FLAT ? 0u : d.range_cap) + blockIdx.x; wi64 < lim; wi64 += step) {
         const uint32_t wi = (uint32_t)wi64;
         const uint32_t code = ws.work[wi];
@@ -494,8 +494,8 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
     }
 }
 
-// Fill pass over the kept work items, one block per item and one thread per
-// quad, without item loops or barriers (the many independent loads overlap):
+// Fill pass over the kept work items, a block per item (striding when there
+// are more items than blocks) and a thread per quad:
 // single-tile quads write their counted slot, the others take slots after the
 // bin's single-tile region through an LDS bin table and the fill cursors.  A quad whose
 // range was too wide to pack is re-binned from the kept vertices.
@@ -505,73 +505,76 @@ __global__ void __launch_bounds__(256) k_fill_kept(const G* __restrict__ geoms, 
     __shared__ uint32_t t_key[kBinTable];
     __shared__ uint32_t t_n[kBinTable];
     __shared__ uint32_t t_base[kBinTable];
-    const uint32_t wi = blockIdx.x;
-    if (ws.status[1] || wi >= ws.status[4]) return;   // uniform per block
+    if (ws.status[1]) return;   // list capacity exceeded: nothing to fill
+    const uint32_t lim = min(ws.status[4], d.range_cap);
     const int tid = threadIdx.x;
     const uint32_t per = d.bins_x * d.bins_y;
-    if (tid < kBinTable) {
-        t_key[tid] = kTableEmpty;
-        t_n[tid] = 0;
-    }
-    const uint32_t code = ws.work[wi];
-    const uint32_t item = code % per, vs = code / per;
-    const int s = (int)(vs % (uint32_t)d.nsrc), v = (int)(vs / (uint32_t)d.nsrc);
-    const ViewDesc& vd = views[v];
-    const int i0 = (int)(item / d.bins_x) * kBinQY, j0 = (int)(item % d.bins_x) * kBinQX;
-    const int i = i0 + tid / kBinQX, j = j0 + tid % kBinQX;
-    const uint32_t q = ((uint32_t)i << 16) | (uint32_t)j;
-    const uint2 r = ws.ranges[(size_t)wi * 256 + tid];
-    bool more = false;
-    int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
-    if (r.x == kRangeBig) {
-        const G& g = geoms[vs];
-        const float* vb = ws.vbuf + (size_t)wi * (3 * kBinVerts);
-        Quad Q;
-        quad_build(Q, g, i, j,
-                   [&](int ii, int jj) -> float3 {
-                       const int rr = ii - i0, c = jj - j0;
-                       if (rr >= 0 && rr <= kBinQY && c >= 0 && c <= kBinQX) return kept_vertex(vb, rr, c);
-                       return vertex(g, ii, jj);
-                   },
-                   vd.w, vd.h, false);
-        more = quad_tiles(Q, tx0, tx1, ty0, ty1);
-    } else if (r.x != kRangeNone) {
-        unpack_range(r.x, tx0, tx1, ty0, ty1);
-        if (tx0 == tx1 && ty0 == ty1) {
-            const size_t bin = (size_t)(vd.tile_base + (uint32_t)(ty0 * vd.tiles_x + tx0)) * d.nsrc + s;
-            ws.lists[(size_t)ws.offsets[bin] + r.y] = q;
-        } else {
-            more = true;
+    for (uint32_t wi = blockIdx.x; wi < lim; wi += gridDim.x) {
+        if (tid < kBinTable) {
+            t_key[tid] = kTableEmpty;
+            t_n[tid] = 0;
         }
-    }
-    // multi-tile entries: LDS bin table, one fill-cursor atomic per bin
-    auto tile_of = [&](int tx, int ty) { return vd.tile_base + (uint32_t)(ty * vd.tiles_x + tx); };
-    __syncthreads();   // table cleared
-    if (more) {
-        for (int ty = ty0; ty <= ty1; ++ty)
-            for (int tx = tx0; tx <= tx1; ++tx) {
-                const uint32_t t = tile_of(tx, ty);
-                const int k = table_slot(t_key, t);
-                if (k >= 0) {
-                    atomicAdd(&t_n[k], 1u);
-                } else {
-                    const size_t bin = (size_t)t * d.nsrc + s;
-                    ws.lists[(size_t)ws.offsets[bin] + ws.single[bin] + atomicAdd(&ws.fill[bin], 1u)] = q;
+        const uint32_t code = ws.work[wi];
+        const uint32_t item = code % per, vs = code / per;
+        const int s = (int)(vs % (uint32_t)d.nsrc), v = (int)(vs / (uint32_t)d.nsrc);
+        const ViewDesc& vd = views[v];
+        const int i0 = (int)(item / d.bins_x) * kBinQY, j0 = (int)(item % d.bins_x) * kBinQX;
+        const int i = i0 + tid / kBinQX, j = j0 + tid % kBinQX;
+        const uint32_t q = ((uint32_t)i << 16) | (uint32_t)j;
+        const uint2 r = ws.ranges[(size_t)wi * 256 + tid];
+        bool more = false;
+        int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
+        if (r.x == kRangeBig) {
+            const G& g = geoms[vs];
+            const float* vb = ws.vbuf + (size_t)wi * (3 * kBinVerts);
+            Quad Q;
+            quad_build(Q, g, i, j,
+                       [&](int ii, int jj) -> float3 {
+                           const int rr = ii - i0, c = jj - j0;
+                           if (rr >= 0 && rr <= kBinQY && c >= 0 && c <= kBinQX) return kept_vertex(vb, rr, c);
+                           return vertex(g, ii, jj);
+                       },
+                       vd.w, vd.h, false);
+            more = quad_tiles(Q, tx0, tx1, ty0, ty1);
+        } else if (r.x != kRangeNone) {
+            unpack_range(r.x, tx0, tx1, ty0, ty1);
+            if (tx0 == tx1 && ty0 == ty1) {
+                const size_t bin = (size_t)(vd.tile_base + (uint32_t)(ty0 * vd.tiles_x + tx0)) * d.nsrc + s;
+                ws.lists[(size_t)ws.offsets[bin] + r.y] = q;
+            } else {
+                more = true;
+            }
+        }
+        // multi-tile entries: LDS bin table, one fill-cursor atomic per bin
+        auto tile_of = [&](int tx, int ty) { return vd.tile_base + (uint32_t)(ty * vd.tiles_x + tx); };
+        __syncthreads();   // table cleared
+        if (more) {
+            for (int ty = ty0; ty <= ty1; ++ty)
+                for (int tx = tx0; tx <= tx1; ++tx) {
+                    const uint32_t t = tile_of(tx, ty);
+                    const int k = table_slot(t_key, t);
+                    if (k >= 0) {
+                        atomicAdd(&t_n[k], 1u);
+                    } else {
+                        const size_t bin = (size_t)t * d.nsrc + s;
+                        ws.lists[(size_t)ws.offsets[bin] + ws.single[bin] + atomicAdd(&ws.fill[bin], 1u)] = q;
+                    }
                 }
-            }
-    }
-    __syncthreads();   // table complete
-    if (tid < kBinTable && t_n[tid]) {
-        const size_t bin = (size_t)t_key[tid] * d.nsrc + s;
-        t_base[tid] = ws.offsets[bin] + ws.single[bin] + atomicAdd(&ws.fill[bin], t_n[tid]);
-    }
-    __syncthreads();   // bases known
-    if (more) {
-        for (int ty = ty0; ty <= ty1; ++ty)
-            for (int tx = tx0; tx <= tx1; ++tx) {
-                const int k = find_slot(t_key, tile_of(tx, ty));
-                if (k >= 0) ws.lists[atomicAdd(&t_base[k], 1u)] = q;
-            }
+        }
+        __syncthreads();   // table complete
+        if (tid < kBinTable && t_n[tid]) {
+            const size_t bin = (size_t)t_key[tid] * d.nsrc + s;
+            t_base[tid] = ws.offsets[bin] + ws.single[bin] + atomicAdd(&ws.fill[bin], t_n[tid]);
+        }
+        __syncthreads();   // bases known
+        if (more) {
+            for (int ty = ty0; ty <= ty1; ++ty)
+                for (int tx = tx0; tx <= tx1; ++tx) {
+                    const int k = find_slot(t_key, tile_of(tx, ty));
+                    if (k >= 0) ws.lists[atomicAdd(&t_base[k], 1u)] = q;
+                }
+        }
+        __syncthreads();   // table free for the next item
     }
 }
 
@@ -652,8 +655,13 @@ hipError_t launch_bin(const G* geoms, const ViewDesc* views, const BatchDims& d,
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t blocks = total < (size_t)kBinBlocks ? total : (size_t)kBinBlocks;
+    // kept items: up to one block each, at most max(total / 8, 65536) blocks
+    // (blocks past the surviving items return at once; a few per item are
+    // cheaper than many idle blocks when most items are culled)
+    const size_t fb = total / 8 > 65536 ? total / 8 : 65536;
+    const unsigned flat = (unsigned)(d.range_cap < fb ? d.range_cap : fb);
     if (d.range_cap)
-        hipLaunchKernelGGL((k_bin<G, false, true>), dim3(d.range_cap), dim3(256), 0, s, geoms, views, d, ws);
+        hipLaunchKernelGGL((k_bin<G, false, true>), dim3(flat), dim3(256), 0, s, geoms, views, d, ws);
     if (total > d.range_cap)
         hipLaunchKernelGGL((k_bin<G, false, false>), dim3((unsigned)blocks), dim3(256), 0, s, geoms, views, d, ws);
     const size_t nbins = (size_t)d.ntiles * d.nsrc;
@@ -664,7 +672,7 @@ hipError_t launch_bin(const G* geoms, const ViewDesc* views, const BatchDims& d,
     hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((nbins + 255) / 256)), dim3(256), 0, s, ws.offsets, nbins,
                        ws.bsum);
     if (d.range_cap)
-        hipLaunchKernelGGL(k_fill_kept<G>, dim3(d.range_cap), dim3(256), 0, s, geoms, views, d, ws);
+        hipLaunchKernelGGL(k_fill_kept<G>, dim3(flat), dim3(256), 0, s, geoms, views, d, ws);
     if (total > d.range_cap)
         hipLaunchKernelGGL((k_bin<G, true, false>), dim3((unsigned)blocks), dim3(256), 0, s, geoms, views, d, ws);
     return hipGetLastError();
