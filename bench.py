@@ -296,16 +296,32 @@ def main():
         rg_b = W.rig("persp11_2048")
         srcs_b = W.device_sources(rg_b, W.render_sources(rg_b))
         mix = W.serve_requests([rg, rg_b], args.c4_requests, seed=4)
-        staging4 = B * max(A.yuv_bytes(t) for _, t in mix)
         budget4 = max(budget, 96 << 30)   # 4K views over 11 sources need larger workspaces
-        srv = S.Server([srcs, srcs_b], memory_budget=budget4, max_batch=B, slots=SL, entry_cap=0,
-                       host_staging_bytes=staging4)
+        # requests here cost 0.2-15 ms of device time each: the batch size is
+        # swept separately (2 slots; every trial is reported); closed-loop
+        # capacity is the best of them, and the open loop runs at 70 % of it
+        trials4 = []
+        for b4 in (4, 8, 16, 32):
+            staging4 = b4 * max(A.yuv_bytes(t) for _, t in mix)
+            srv = S.Server([srcs, srcs_b], memory_budget=budget4, max_batch=b4, slots=2, entry_cap=0,
+                           host_staging_bytes=staging4)
+            try:
+                srv.run_closed(mix[:2 * b4], concurrency=2 * b4)
+                a = time.perf_counter()
+                stc = srv.run_closed(mix, concurrency=2 * b4)
+                cap_s = time.perf_counter() - a
+                closed = S.summarize(stc, wall_s=cap_s)
+                closed["max_batch"] = b4
+                trials4.append(closed)
+                log("c4 trial %s" % closed)
+            finally:
+                srv.close()
+        closed4 = max(trials4, key=lambda t: t["views_s"])
+        b4 = closed4["max_batch"]
+        srv = S.Server([srcs, srcs_b], memory_budget=budget4, max_batch=b4, slots=2, entry_cap=0,
+                       host_staging_bytes=b4 * max(A.yuv_bytes(t) for _, t in mix))
         try:
-            srv.run_closed(mix[:B * SL], concurrency=B * SL)
-            a = time.perf_counter()
-            stc = srv.run_closed(mix, concurrency=B * SL)
-            cap_s = time.perf_counter() - a
-            closed4 = S.summarize(stc, wall_s=cap_s)
+            srv.run_closed(mix[:2 * b4], concurrency=2 * b4)
             rate4 = 0.7 * closed4["views_s"]
             sto = srv.run_open(mix, rate=rate4, seed=4)
             open4 = S.summarize(sto)
@@ -314,7 +330,8 @@ def main():
             srv.close()
         c4 = {"scenes": ["4x ERP 4096x2048", "11x perspective 2048x2048"],
               "targets": "1280x720, 1920x1080, 2560x1440, 3840x2160 perspective, 2048x1024 ERP",
-              "requests": len(mix), "closed_loop": closed4, "open_loop_70pct": open4}
+              "requests": len(mix), "closed_loop": closed4, "open_loop_70pct": open4,
+              "batch_trials": trials4}
         del srcs_b
         log("c4 %s" % c4)
 
