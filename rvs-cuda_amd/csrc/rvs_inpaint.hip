@@ -486,6 +486,168 @@ __device__ void solve_big(const Region& R, const InpaintWS& ws, Fn* wave_tot, in
     }
 }
 
+constexpr int kLineMax = 256 * kBigPx;   // longest line of solve_lines (one block step)
+
+// Large components whose lines fit one block step.  Each thread owns the same
+// kBigPx positions on every line, so the previous line's values from this
+// sweep stay in registers; a line step loads only its own pixels, the next
+// line and one along-line neighbour (all independent loads) and takes one
+// barrier.  cols: lines are columns.  Same sweeps, candidate order and line
+// order semantics as solve_component.
+__device__ void solve_lines(const Region& R, const InpaintWS& ws, bool cols, Fn* wave_tot) {
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int nlines = cols ? R.bw : R.bh;
+    const int llen = cols ? R.bh : R.bw;
+    const int max_sweeps = 4 * (R.w + R.h) + 8;   // each changing sweep lowers a distance: a guard only
+    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+        const bool fwd = (sweep & 1) == 0;
+        // ranks (left 1, up 2, down 3, right 4) of the scan input, of the
+        // across-line neighbour already visited in this sweep, of the one not yet
+        // visited, and of the along-line neighbour after the pixel in scan order
+        const int rs = cols ? (fwd ? 2 : 3) : (fwd ? 1 : 4);
+        const int r_prev = cols ? (fwd ? 1 : 4) : (fwd ? 2 : 3);
+        const int r_next = cols ? (fwd ? 4 : 1) : (fwd ? 3 : 2);
+        const int r_along = cols ? (fwd ? 3 : 2) : (fwd ? 4 : 1);
+        // view pixel of line coordinate lc (box-relative) and scan index e
+        auto at = [&](int lc, int e, int& x, int& y) {
+            const int pos = fwd ? e : llen - 1 - e;
+            x = cols ? R.x0 + lc : R.x0 + pos;
+            y = cols ? R.y0 + pos : R.y0 + lc;
+        };
+        auto inside = [&](int x, int y) { return x >= 0 && x < R.w && y >= 0 && y < R.h; };
+        auto load = [&](int x, int y, int& v, int& sr) {
+            const int q = R.base + (inside(x, y) ? y * R.w + x : 0);
+            map_value(R, ws.label[q], ws.dist[q], ws.src[q], q, v, sr);
+        };
+        bool changed = false;
+        // line coordinate of processing step li (-1 / nlines: outside the box)
+        auto lcoord = [&](int li) { return fwd ? li : nlines - 1 - li; };
+        // words of line lc at this thread's positions, the along-line
+        // neighbour after them and the pixel before the line's first one
+        auto load_line = [&](int lc, int* v, int* sr, bool* in) {
+#pragma unroll
+            for (int k = 0; k < kBigPx; ++k) {
+                int x, y;
+                at(lc, tid * kBigPx + k, x, y);
+                in[k] = inside(x, y);
+                load(x, y, v[k], sr[k]);
+            }
+        };
+        auto load_ends = [&](int lc, int& av, int& as_, bool& ain, int& Lv, int& Ls) {
+            int x, y;
+            at(lc, tid * kBigPx + kBigPx, x, y);
+            ain = inside(x, y);
+            load(x, y, av, as_);
+            at(lc, -1, x, y);
+            int v, sr;
+            load(x, y, v, sr);
+            const bool known = inside(x, y) && v == 0;
+            Lv = known ? 0 : kAbsent;
+            Ls = known ? sr : -1;
+        };
+        // the line before the first one (outside the box, constant), the first
+        // line and the line after it; later lines are loaded one step ahead
+        int pv[kBigPx], ps[kBigPx], sv[kBigPx], ss[kBigPx], nv[kBigPx], ns[kBigPx];
+        bool pin[kBigPx], sin[kBigPx], nin[kBigPx];
+        int av, as_, Lv, Ls;
+        bool ain;
+        load_line(lcoord(-1), pv, ps, pin);
+        load_line(lcoord(0), sv, ss, sin);
+        load_line(lcoord(1), nv, ns, nin);
+        load_ends(lcoord(0), av, as_, ain, Lv, Ls);
+        for (int li = 0; li < nlines; ++li) {
+            const int lc = lcoord(li);
+            // prefetch for the next step: the line after its next one and its ends
+            int fv[kBigPx], fs[kBigPx], fav, fas, fLv, fLs;
+            bool fin[kBigPx], fain;
+            load_line(lcoord(li + 2), fv, fs, fin);
+            load_ends(lcoord(li + 1), fav, fas, fain, fLv, fLs);
+            Fn f[kBigPx];
+#pragma unroll
+            for (int k = 0; k < kBigPx; ++k) {
+                const int e = tid * kBigPx + k;
+                f[k].k = 1; f[k].cv = kAbsent; f[k].cs = -1; f[k].t = 0;
+                if (e >= llen) continue;
+                const int v = sv[k];
+                if (v == 0) {
+                    f[k].cv = 0; f[k].cs = ss[k];
+                } else if (v != kOther) {
+                    int bv = v, br = 0, bs = ss[k];
+                    const int alv = k + 1 < kBigPx ? sv[k + 1] : av, als = k + 1 < kBigPx ? ss[k + 1] : as_;
+                    const bool alin = k + 1 < kBigPx ? sin[k + 1] : ain;
+#pragma unroll
+                    for (int n = 1; n <= 4; ++n) {
+                        if (n == rs) continue;
+                        int cvv, css;
+                        bool ok;
+                        if (n == r_prev) { cvv = pv[k]; css = ps[k]; ok = pin[k]; }
+                        else if (n == r_next) { cvv = nv[k]; css = ns[k]; ok = nin[k]; }
+                        else { cvv = alv; css = als; ok = alin && n == r_along; }   // the remaining rank
+                        if (ok && cvv != kOther) cand(bv, br, bs, cvv + 1, n, css);
+                    }
+                    f[k].cv = bv; f[k].cs = bs; f[k].t = (rs < br) ? 1 : 0;
+                }
+            }
+            Fn A = f[0];
+#pragma unroll
+            for (int k = 1; k < kBigPx; ++k) A = compose(A, f[k]);
+            for (int off = 1; off < 64; off <<= 1) {
+                Fn g;
+                g.cv = __shfl_up(A.cv, off);
+                g.cs = __shfl_up(A.cs, off);
+                g.t = __shfl_up(A.t, off);
+                g.k = __shfl_up(A.k, off);
+                if (lane >= off) A = compose(g, A);
+            }
+            Fn ex;
+            ex.cv = __shfl_up(A.cv, 1);
+            ex.cs = __shfl_up(A.cs, 1);
+            ex.t = __shfl_up(A.t, 1);
+            ex.k = __shfl_up(A.k, 1);
+            Fn* wt = wave_tot + 4 * (li & 1);   // alternate buffers: one barrier per line
+            if (lane == 63) wt[wave] = A;
+            __syncthreads();   // every load of this line is done before any write
+            Fn pre;
+            bool have = false;
+            for (int wv = 0; wv < wave; ++wv) {
+                pre = have ? compose(pre, wt[wv]) : wt[wv];
+                have = true;
+            }
+            if (lane > 0) {
+                pre = have ? compose(pre, ex) : ex;
+                have = true;
+            }
+            int Sv = Lv, Ss = Ls;
+            if (have) apply_fn(pre, Sv, Ss);
+#pragma unroll
+            for (int k = 0; k < kBigPx; ++k) {
+                apply_fn(f[k], Sv, Ss);
+                const int e = tid * kBigPx + k;
+                const bool member = e < llen && sv[k] != 0 && sv[k] != kOther;
+                if (member && Sv != sv[k]) {
+                    int x, y;
+                    at(lc, e, x, y);
+                    const int q = R.base + y * R.w + x;
+                    changed = true;
+                    ws.dist[q] = Sv;
+                    ws.src[q] = Ss;
+                }
+                pv[k] = member ? Sv : sv[k];
+                ps[k] = member ? Ss : ss[k];
+                pin[k] = sin[k];
+            }
+#pragma unroll
+            for (int k = 0; k < kBigPx; ++k) {
+                sv[k] = nv[k]; ss[k] = ns[k]; sin[k] = nin[k];
+                nv[k] = fv[k]; ns[k] = fs[k]; nin[k] = fin[k];
+            }
+            av = fav; as_ = fas; ain = fain; Lv = fLv; Ls = fLs;
+        }
+        if (!__syncthreads_or(changed)) break;
+    }
+}
+
 // View owning global pixel gp (views are sorted by px_base).
 __device__ inline int view_of_pixel(const ViewDesc* views, int nviews, int gp) {
     int lo = 0, hi = nviews - 1;
@@ -497,7 +659,7 @@ __device__ inline int view_of_pixel(const ViewDesc* views, int nviews, int gp) {
 }
 
 __global__ void __launch_bounds__(256) k_inp_solve(const ViewDesc* __restrict__ views, int nviews, InpaintWS ws) {
-    __shared__ Fn wave_tot[4];
+    __shared__ Fn wave_tot[8];
     __shared__ int s_carry[2];
     __shared__ int s_comp;
     __shared__ int ldist[kLdsCap];
@@ -551,6 +713,10 @@ __global__ void __launch_bounds__(256) k_inp_solve(const ViewDesc* __restrict__ 
                 const int p = R.base + y * R.w + x;
                 if (ws.label[p] == R.root) { ws.dist[p] = ldist[k]; ws.src[p] = lsrc[k]; }
             }
+        } else if (R.bw <= kLineMax || R.bh <= kLineMax) {
+            // lines that fit one step; fewer lines when both orientations fit
+            const bool cols = R.bh <= kLineMax && (R.bw > kLineMax || R.bh > R.bw);
+            solve_lines(R, ws, cols, wave_tot);
         } else {
             solve_big(R, ws, wave_tot, s_carry);
         }
