@@ -58,7 +58,7 @@ private:
     std::vector<std::vector<rvs_source>> m_scenes;
     rvs_serve_opts m_opt;
     std::vector<Slot*> m_slots;
-    void* m_compute = nullptr;   // hipStream_t shared by every slot's launch set (in order)
+    std::vector<void*> m_compute;   // hipStream_t: slot k's launch sets run on m_compute[k % size]
     std::mutex m_mu;
     std::condition_variable m_cv;
     std::thread m_thread;

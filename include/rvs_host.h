@@ -46,18 +46,21 @@ int rvs_app_run(const char* config_path, const char* source_path, int stagewise)
 
 /* ------------------------------------------------------------------ serving
  * A request loop over resident source sets ("scenes"): target-view requests are
- * admitted by memory budget and batched as they arrive (continuous batching),
- * each batch one rvs_synthesize_batch_async launch set on one of `slots`
- * streams, so batches overlap on the device.  Latency of a request = arrival to
- * the completion of its YUV copy to host memory. */
+ * admitted by memory budget and batched as they arrive (continuous batching).
+ * Each batch is one rvs_synthesize_batch_async launch set on one of
+ * `compute_streams` streams shared by the `slots` in flight (in order on one
+ * stream; overlapping when there are several), and its YUV planes are copied
+ * back on the slot's own stream, overlapping later batches.  Latency of a
+ * request = arrival to the completion of its YUV copy to host memory. */
 typedef struct {
     size_t memory_budget;       /* device bytes for in-flight batches (workspaces + outputs), split over slots */
     int max_batch;              /* views per batch */
-    int slots;                  /* batches in flight (one stream each) */
+    int slots;                  /* batches in flight */
     uint32_t entry_cap;         /* binned entries per view of a batch (0 = library default); grows on overflow */
     int copy_to_host;           /* copy each request's YUV planes to pinned host memory */
     int keep_outputs;           /* keep a host copy per request (rvs_server_output) */
     size_t host_staging_bytes;  /* pinned staging per slot (0 = a quarter of the slot's share, at most 1 GiB) */
+    int compute_streams;        /* streams the slots' launch sets run on (0 or 1: one, in order) */
 } rvs_serve_opts;
 
 typedef struct {

@@ -37,11 +37,12 @@ void hip_check(hipError_t e, const char* what) {
 }  // namespace
 
 // A slot holds one batch from launch until its outputs are on the host: its
-// launch set runs on the shared compute stream (batches of all slots in
-// order, so the device is never split between them), and its YUV planes
+// launch set runs on a compute stream shared with other slots (with one such
+// stream the batches of all slots run in order, so the device is never split
+// between them; several let long tails overlap), and its YUV planes
 // (contiguous at the start of the arena, laid out as in the host staging
-// buffer) are copied back on the slot's own stream, overlapping the next
-// batch's compute.
+// buffer) are copied back on the slot's own stream, overlapping later
+// batches.
 struct Server::Slot {
     hipStream_t stream = nullptr;      // host copies (and capacity re-runs)
     hipEvent_t computed = nullptr;     // the batch's launch set has finished
@@ -51,6 +52,7 @@ struct Server::Slot {
     uint8_t* pinned = nullptr;
     size_t pinned_bytes = 0;
     bool busy = false;
+    int index = 0;           // slot number (picks the compute stream)
     int scene = 0;
     uint32_t per_view = 0;   // binned entries per view (0: library default)
     std::vector<int> reqs;
@@ -69,11 +71,15 @@ Server::Server(std::vector<std::vector<rvs_source>> scenes, const rvs_serve_opts
     if (m_opt.slots < 1) m_opt.slots = 1;
     if (m_opt.max_batch < 1) m_opt.max_batch = 1;
     const size_t share = m_opt.memory_budget / (size_t)m_opt.slots;
-    hipStream_t cs = nullptr;
-    hip_check(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "stream create");
-    m_compute = cs;
+    const int ncs = std::max(1, std::min(m_opt.compute_streams, m_opt.slots));
+    for (int k = 0; k < ncs; ++k) {
+        hipStream_t cs = nullptr;
+        hip_check(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "stream create");
+        m_compute.push_back(cs);
+    }
     for (int k = 0; k < m_opt.slots; ++k) {
         Slot* s = new Slot;
+        s->index = k;
         m_slots.push_back(s);
         hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "stream create");
         hip_check(hipEventCreateWithFlags(&s->computed, hipEventDisableTiming), "event create");
@@ -97,7 +103,7 @@ Server::~Server() {
     }
     m_cv.notify_all();
     if (m_thread.joinable()) m_thread.join();
-    if (m_compute) (void)hipStreamSynchronize((hipStream_t)m_compute);
+    for (void* cs : m_compute) (void)hipStreamSynchronize((hipStream_t)cs);
     for (Slot* s : m_slots) {
         if (s->stream) (void)hipStreamSynchronize(s->stream);
         if (s->pinned) (void)hipHostFree(s->pinned);
@@ -107,7 +113,7 @@ Server::~Server() {
         if (s->stream) (void)hipStreamDestroy(s->stream);
         delete s;
     }
-    if (m_compute) (void)hipStreamDestroy((hipStream_t)m_compute);
+    for (void* cs : m_compute) (void)hipStreamDestroy((hipStream_t)cs);
 }
 
 double Server::now_ms() const {
@@ -257,7 +263,7 @@ bool Server::launch(Slot& s) {
     s.ws_off = o;
     const double t_launch = now_ms();
     s.cap = batch_cap(s.per_view, tgts.size());
-    hipStream_t cs = (hipStream_t)m_compute;
+    hipStream_t cs = (hipStream_t)m_compute[s.index % m_compute.size()];
     int rc = rvs_synthesize_batch_async(S.data(), (int)S.size(), tgts.data(), (int)tgts.size(), s.arena + s.ws_off,
                                         ws, s.cap, s.outs.data(), cs);
     if (rc == RVS_OK && hipEventRecord(s.computed, cs) != hipSuccess) rc = RVS_E_HIP;
