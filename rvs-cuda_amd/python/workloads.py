@@ -51,6 +51,21 @@ def rig(name, scale=1):
         tgt = dict(cam=perspective_cam(1920 // scale, 1080 // scale), pos=[0.0, 0.0, 0.0], rot=[0.0, 0.0, 0.0],
                    bitdepth=10, blending_exp=1.0)
         return dict(kind=1, seed=4, sources=srcs, target=tgt)
+    if name.startswith("erp11_4096"):
+        # 11 ERP 4096x2048 10-bit views with 16-bit depth on a 3 x 4 grid (one
+        # slot unused, 0.1 m pitch); "erp11_4096:<seed>" picks the scene
+        seed = int(name.split(":")[1]) if ":" in name else 5
+        sw, sh = 4096 // scale, 2048 // scale
+        srcs = []
+        for r in range(3):
+            for c in range(4):
+                if r == 2 and c == 3:
+                    continue
+                srcs.append(dict(cam=R.cam(1, sw, sh), pos=[0.0, (c - 1.5) * 0.1, (1 - r) * 0.1],
+                                 rot=[0.0, 0.0, 0.0], bd_color=10, bd_depth=16, znear=0.5, zfar=25.0))
+        tgt = dict(cam=R.cam(1, 8192 // scale, 4096 // scale), pos=[0.0, 0.0, 0.0], rot=[0.0, 0.0, 0.0],
+                   bitdepth=10, blending_exp=1.0)
+        return dict(kind=1, seed=seed, sources=srcs, target=tgt)
     raise KeyError(name)
 
 

@@ -2,7 +2,10 @@
 
 Requests for two resident scenes at mixed target sizes arrive while others are
 on the device; each result must be bit-identical to the same view synthesized
-alone.  Memory-budget admission rejects a request that cannot fit.
+alone.  Memory-budget admission rejects a request that cannot fit.  A
+large-serving case (config 5's scene shape at reduced resolution: scenes of
+11 ERP sources, ERP and perspective targets, closed loop at full concurrency)
+holds to the same rule.
 """
 import sys
 
@@ -74,5 +77,34 @@ def test_budget_rejects_request_that_cannot_fit(torch_cuda):
     try:
         with pytest.raises(R.RvsError):
             srv.run_closed([(1, small), (0, big), (1, small)], concurrency=3)
+    finally:
+        srv.close()
+
+
+def test_served_erp11_scenes_equal_alone(torch_cuda):
+    say("case serve 2 scenes x 11 ERP sources, ERP + perspective targets, closed loop")
+    scale = 16
+    rigs = [W.rig("erp11_4096:%d" % seed, scale=scale) for seed in (5, 6)]
+    srcs = [W.device_sources(rg, W.render_sources(rg)) for rg in rigs]
+    rng = np.random.default_rng(9)
+    reqs = []
+    for k in range(12):
+        sc = int(rng.integers(2))
+        rg = rigs[sc]
+        pos = [float(np.float32(x)) for x in rng.uniform(-0.1, 0.1, 3)]
+        rot = [float(np.float32(rng.uniform(-20, 20))), float(np.float32(rng.uniform(-8, 8))), 0.0]
+        # ERP 8192x4096 / 16, 4K perspective / 8 (an even height)
+        cam = rg["target"]["cam"] if k % 2 == 0 else W.perspective_cam(3840 // 8, 2160 // 8)
+        s0 = rg["sources"][0]
+        reqs.append((sc, R.make_target(cam, pos, rot, 10, 1.0, R.scale_of(s0["bd_color"]),
+                                       R.scale_of(s0["bd_depth"]))))
+    srv = S.Server(srcs, memory_budget=8 << 30, max_batch=4, slots=2, keep_outputs=True)
+    try:
+        st = srv.run_closed(reqs, concurrency=len(reqs))
+        say("  summary", S.summarize(st))
+        assert (st["rc"] == 0).all()
+        assert st["batch_views"].max() > 1
+        for k, (scene, tgt) in enumerate(reqs):
+            assert srv.output(k) == _yuv_alone(srcs[scene], tgt), "request %d (scene %d)" % (k, scene)
     finally:
         srv.close()
