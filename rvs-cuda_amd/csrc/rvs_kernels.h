@@ -20,13 +20,11 @@ namespace rvs_k {
 constexpr int kTile = 16;            // target tile edge (pixels); one 256-thread block per tile
 constexpr int kBinQX = 64;           // binning work item: 64 x 4 source quads
 constexpr int kBinQY = 4;
-// default list capacity: entries per output pixel per source (x 1/4)
-constexpr uint32_t kDefaultEntriesPerPxSrcQ = 4;
 constexpr int kFallbackChunk = 128;  // list entries per LDS chunk in the in-order replay
 constexpr int kFallbackSortCap = 8192;  // longest (source, tile) list the replay sorts in LDS
 constexpr int kMaxViews = 4096;
 constexpr int kBinVerts = (kBinQY + 1) * (kBinQX + 1);   // vertices of one binning work item
-constexpr int kBinBlocks = 2048;    // persistent binning blocks
+constexpr int kBinBlocks = 2048;    // persistent binning blocks (work items past the kept range)
 constexpr int kRasterGroup = 4;     // sources rasterised together per k_raster phase
 constexpr int kRasterCache = 1024;  // binned quads per phase whose coverage-pass results stay in LDS
 constexpr int kMaxSourceSide = 65536;  // bins hold quads as (row << 16) | column
