@@ -794,18 +794,20 @@ __device__ __forceinline__ float raster_tri(float3 a, float3 b, float3 c, uint32
 // Per-phase LDS copy of what the coverage pass computed for each binned quad:
 // the validity of triangles T1 and T2 and the quad code (the corner vertices
 // are read again from the kept vertex blocks).
+template <int CAP>
 struct RasterCache {
-    float tv[2][kRasterCache];
-    uint32_t ij[kRasterCache];
+    static constexpr int cap = CAP;
+    float tv[2][CAP];
+    uint32_t ij[CAP];
 };
 
 // Candidate ids: (entry << 2) | slot, entry = index of the quad in the phase's
 // concatenated bins.  A non-deferred pixel's winner beats every other
 // candidate by more than the float test's error, so any unique id works; equal
 // scores are deferred to the in-order replay.
-template <class G>
+template <class G, class Cache>
 __device__ __forceinline__ void raster_quad(const G& g, uint32_t ij, uint32_t e, int tx0, int ty0, int out_w, int out_h,
-                                   unsigned long long* key1, uint32_t* key2, RasterCache* C,
+                                   unsigned long long* key1, uint32_t* key2, Cache* C,
                                    const RasterWS& ws, const BatchDims& d, uint32_t vs) {
     int i, j;
     unpack_ij(ij, i, j);
@@ -861,12 +863,13 @@ __device__ inline void write_pixel(const ViewDesc& vd, size_t o, const BlendStat
 // state lives across them); otherwise one phase, which keeps the coverage pass
 // free of the blend state's registers.
 template <class G, int MODE, bool MULTI>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_raster(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 6 : 8))) k_raster(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
                                                 BatchDims d, RasterWS ws) {
     __shared__ unsigned long long key1[kRasterGroup][kTile * kTile];
     __shared__ uint32_t key2[kRasterGroup][kTile * kTile];
     __shared__ uint32_t pref[kRasterGroup + 1];
-    __shared__ RasterCache cache;
+    // one phase: a smaller cache keeps the block within 20 KB of LDS (8 blocks per CU)
+    __shared__ RasterCache<MULTI ? kRasterCache : kRasterCacheOne> cache;
     const uint32_t T = tile_of_block(blockIdx.x, d.ntiles);
     if (T >= d.ntiles || ws.status[1]) return;   // bins not filled: capacity reported to the host
     const int v = find_view(views, d.nviews, T);
@@ -897,7 +900,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
         }
         __syncthreads();
         const uint32_t N = pref[gs];
-        const bool cached = N <= (uint32_t)kRasterCache;
+        const bool cached = N <= (uint32_t)cache.cap;
         for (uint32_t e = tid; e < N; e += 256) {
             int k = 0;
             while (k + 1 < gs && e >= pref[k + 1]) ++k;
