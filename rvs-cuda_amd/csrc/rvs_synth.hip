@@ -680,6 +680,14 @@ hipError_t launch_bin(const G* geoms, const ViewDesc* views, const BatchDims& d,
 
 // ------------------------------------------------------------ tile raster
 
+// The lane's index in its wave, computed where it is called (not merged with
+// other reads of it, so it need not stay live in a VGPR in between).
+__device__ inline int lane_now() {
+    int l;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+    return l;
+}
+
 // Winner's colour, depth and validity at pixel (x, y) for slot `slot` of
 // source quad (i, j): the state the fold holds after accepting that candidate.
 template <class G>
@@ -868,6 +876,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI 
     __shared__ unsigned long long key1[kRasterGroup][kTile * kTile];
     __shared__ uint32_t key2[kRasterGroup][kTile * kTile];
     __shared__ uint32_t pref[kRasterGroup + 1];
+    __shared__ int s_deferred;   // any pixel of the tile deferred
     // one phase: a smaller cache keeps the block within 20 KB of LDS (8 blocks per CU)
     __shared__ RasterCache<MULTI ? kRasterCache : kRasterCacheOne> cache;
     const uint32_t T = tile_of_block(blockIdx.x, d.ntiles);
@@ -877,8 +886,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI 
     const int tile = (int)(T - vd.tile_base);
     const int tid = threadIdx.x;
     const int tx0 = (tile % vd.tiles_x) * kTile, ty0 = (tile / vd.tiles_x) * kTile;
-    const int px = tx0 + (tid % kTile), py = ty0 + (tid / kTile);
-    const bool inside = px < vd.w && py < vd.h;
+    // The thread's index is re-derived after the coverage pass (wave base in
+    // an SGPR + lane id) instead of being held in a VGPR through it.
+    const int wave_base = __builtin_amdgcn_readfirstlane(tid & ~63);
     const G* gv = geoms + (size_t)v * d.nsrc;
     bool deferred = false;
     BlendState st;
@@ -891,6 +901,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI 
             key2[k][tid] = 0u;
         }
         if (tid == 0) {
+            if (s0 == 0) s_deferred = 0;
             uint32_t acc = 0;
             for (int k = 0; k < gs; ++k) {
                 pref[k] = acc;
@@ -901,7 +912,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI 
         __syncthreads();
         const uint32_t N = pref[gs];
         const bool cached = N <= (uint32_t)cache.cap;
-        for (uint32_t e = tid; e < N; e += 256) {
+        // a wave-uniform trip count; the entry index is re-derived each trip
+        const uint32_t trips = (N + 255u) / 256u;
+        for (uint32_t it = 0; it < trips; ++it) {
+            const uint32_t e = it * 256u + (uint32_t)(wave_base + lane_now());
+            if (e >= N) continue;
             int k = 0;
             while (k + 1 < gs && e >= pref[k + 1]) ++k;
             const uint32_t ij = ws.lists[(size_t)ws.offsets[(size_t)T * d.nsrc + s0 + k] + (e - pref[k])];
@@ -909,12 +924,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI 
                         ws, d, (uint32_t)(v * d.nsrc + s0 + k));
         }
         __syncthreads();
+        const int t2 = wave_base + lane_now();
+        const int px = tx0 + (t2 % kTile), py = ty0 + (t2 / kTile);
+        const bool inside = px < vd.w && py < vd.h;
         if (inside) {
             for (int k = 0; k < gs; ++k) {
                 const int s = s0 + k;
                 if (deferred) break;
-                const unsigned long long kw = key1[k][tid];
-                const uint32_t k2 = key2[k][tid];
+                const unsigned long long kw = key1[k][t2];
+                const uint32_t k2 = key2[k][t2];
                 bool amb = k2 == kDeferKey;
                 if (!amb && kw != 0ull && k2 != 0u) {
                     const float fw = u2f((uint32_t)(kw >> 32)), f2 = u2f(k2);
@@ -960,12 +978,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI 
         }
         __syncthreads();
     }
+    const int t3 = wave_base + lane_now();
+    const int px = tx0 + (t3 % kTile), py = ty0 + (t3 / kTile);
+    const bool inside = px < vd.w && py < vd.h;
     if (inside) {
         const size_t o = (size_t)py * vd.w + px;
         ws.deferred[vd.px_base + o] = deferred ? 1 : 0;
         if (!deferred) write_pixel<MODE>(vd, o, st, rc, rd, rs);
     }
-    if (__syncthreads_or(deferred) && tid == 0) {
+    if (deferred) s_deferred = 1;
+    __syncthreads();
+    if (s_deferred && t3 == 0) {
         const uint32_t k = atomicAdd(&ws.status[0], 1u);
         ws.deferred_tiles[k] = T;
     }
