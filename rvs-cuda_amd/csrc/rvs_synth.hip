@@ -910,7 +910,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI 
             pref[gs] = acc;
         }
         __syncthreads();
-        const uint32_t N = pref[gs];
+        const uint32_t N = __builtin_amdgcn_readfirstlane(pref[gs]);   // uniform: kept in an SGPR
         const bool cached = N <= (uint32_t)cache.cap;
         // a wave-uniform trip count; the entry index is re-derived each trip
         const uint32_t trips = (N + 255u) / 256u;
