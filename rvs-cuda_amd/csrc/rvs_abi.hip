@@ -282,6 +282,7 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
             view_grid(d, tgts[v].cam.w, tgts[v].cam.h, tb, pb);
             d.bitdepth = tgts[v].bitdepth;
             d.blend_exp = tgts[v].blending_exp;
+            d.blend_inv = 1.0f / tgts[v].blending_exp;
         }
     }
     return RVS_OK;

@@ -413,7 +413,8 @@ __device__ inline void blend_first(BlendState& s, float3 c, float d, float shape
     s.mask = d > 0.f;
 }
 
-__device__ inline void blend_step(BlendState& s, float3 c, float d, float shape, float a) {
+// a_inv = 1.0f / a, computed once per view on the host (the same float division).
+__device__ inline void blend_step(BlendState& s, float3 c, float d, float shape, float a, float a_inv) {
     const float q_new = shape / d;
     const float q_acc = s.q;
     const bool m_new = d > 0.f;
@@ -456,13 +457,13 @@ __device__ inline void blend_step(BlendState& s, float3 c, float d, float shape,
                 const float r = 1.f / inpSum;
                 out = make_float3(inp.x * r, inp.y * r, inp.z * r);
                 outMask = 1;
-                outQ = rvs_pow(inpSum, 1.0f / a);
+                outQ = rvs_pow(inpSum, a_inv);
             }
         } else {
             const float r = 1.f / sumW;
             out = make_float3(fin.x * r, fin.y * r, fin.z * r);
             outMask = 0;
-            outQ = rvs_pow(sumW, 1.0f / a);
+            outQ = rvs_pow(sumW, a_inv);
         }
     }
     if (q_new > q_acc) s.d = d;

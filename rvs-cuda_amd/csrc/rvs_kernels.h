@@ -38,6 +38,7 @@ struct ViewDesc {
     uint32_t px_base;        // first global pixel
     int bitdepth;
     float blend_exp;
+    float blend_inv;         // 1.0f / blend_exp
     float* color;            // w*h*3 (blend) / raster colour (raw)
     float* quality;          // w*h (blend); may be null for raw
     float* depth;            // w*h

@@ -54,7 +54,7 @@ __global__ void k_reproject(PairGeom g, float* __restrict__ uv, float* __restric
 // BlendedViewSimple::blendBasicMode on whole maps.
 __global__ void k_blend_step(float* acc_c, float* acc_q, float* acc_d, float* acc_v, uint8_t* acc_m,
                              const float* __restrict__ c, const float* __restrict__ d,
-                             const float* __restrict__ shape, size_t n, float a, int first) {
+                             const float* __restrict__ shape, size_t n, float a, float a_inv, int first) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float3 cv = make_float3(c[3 * i], c[3 * i + 1], c[3 * i + 2]);
@@ -65,7 +65,7 @@ __global__ void k_blend_step(float* acc_c, float* acc_q, float* acc_d, float* ac
     } else {
         s.c = make_float3(acc_c[3 * i], acc_c[3 * i + 1], acc_c[3 * i + 2]);
         s.q = acc_q[i]; s.d = acc_d[i]; s.v = acc_v[i]; s.mask = acc_m[i];
-        blend_step(s, cv, d[i], shape[i], a);
+        blend_step(s, cv, d[i], shape[i], a, a_inv);
     }
     acc_c[3 * i] = s.c.x; acc_c[3 * i + 1] = s.c.y; acc_c[3 * i + 2] = s.c.z;
     acc_q[i] = s.q; acc_d[i] = s.d; acc_v[i] = s.v;
@@ -140,8 +140,9 @@ hipError_t launch_reproject(const PairGeom& g, float* uv, float* vd, hipStream_t
 
 hipError_t launch_blend_step(float* acc_c, float* acc_q, float* acc_d, float* acc_v, uint8_t* acc_m, const float* c,
                              const float* d, const float* shape, size_t n, float a, int first, hipStream_t s) {
+    const float a_inv = 1.0f / a;
     hipLaunchKernelGGL(k_blend_step, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, acc_c, acc_q, acc_d, acc_v,
-                       acc_m, c, d, shape, n, a, first);
+                       acc_m, c, d, shape, n, a, a_inv, first);
     return hipGetLastError();
 }
 

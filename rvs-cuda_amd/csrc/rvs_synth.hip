@@ -972,7 +972,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI 
                 } else if (s == 0) {
                     blend_first(st, c, dep, shape);
                 } else {
-                    blend_step(st, c, dep, shape, vd.blend_exp);
+                    blend_step(st, c, dep, shape, vd.blend_exp, vd.blend_inv);
                 }
             }
         }
@@ -1211,7 +1211,7 @@ __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, c
                 } else if (s == 0) {
                     blend_first(st, c, dep, shape);
                 } else {
-                    blend_step(st, c, dep, shape, vd.blend_exp);
+                    blend_step(st, c, dep, shape, vd.blend_exp, vd.blend_inv);
                 }
             }
         }
