@@ -735,10 +735,11 @@ constexpr uint32_t kDeferKey = 0xffffffffu;
 // Pass over one triangle of a binned quad for the tile at (tx0, ty0): per
 // covered pixel, push the candidate key (score bits << 32 | candidate id)
 // into key1 and the displaced score into key2.  Candidates with non-finite
-// inputs or extreme depths mark the pixel for the in-order replay instead.  Returns the
-// triangle's validity, or 0 when no pixel of the tile can be covered (then it
-// is not evaluated; the reference computes it before the bounding box test,
-// and it never causes a return, so skipping it changes nothing).
+// inputs or extreme depths mark the pixel for the in-order replay instead.
+// Returns the triangle's validity, or 0 when no pixel of the tile can be
+// covered (then it is not evaluated: the reference computes it before the
+// bounding box test, but it never causes a return, and a triangle that covers
+// no pixel is never a winner, so skipping it changes nothing).
 __device__ __forceinline__ float raster_tri(float3 a, float3 b, float3 c, uint32_t id, bool prol_false_possible, int tx0,
                                    int ty0, int out_w, int out_h, float rescale, unsigned long long* key1,
                                    uint32_t* key2) {
@@ -751,7 +752,6 @@ __device__ __forceinline__ float raster_tri(float3 a, float3 b, float3 c, uint32
     int ya = max(T.ymin, ty0), yb = min(T.ymax, ty0 + kTile - 1);
     if (!special) tri_tight(T, xa, xb, ya, yb);
     if (xa > xb || ya > yb) return 0.f;   // no pixel of this tile can be covered
-    T.tv = tri_validity(T, rescale);
     auto push = [&](int x, int y) {
         float n1, n2;
         tri_num(T, x, y, n1, n2);
@@ -783,12 +783,15 @@ __device__ __forceinline__ float raster_tri(float3 a, float3 b, float3 c, uint32
                 tri_num(T, x, y, n1, n2);
                 if (!tri_surely_outside(T, n1, n2)) mask |= 1ull << ((y - ya) * 8 + (x - xa));
             }
+        if (!mask) return 0.f;   // no candidate pixel: the validity is never needed
+        T.tv = tri_validity(T, rescale);
         while (mask) {
             const int bit = __builtin_ctzll(mask);
             mask &= mask - 1ull;
             push(xa + (bit & 7), ya + (bit >> 3));
         }
     } else {
+        T.tv = tri_validity(T, rescale);
         for (int y = ya; y <= yb; ++y)
             for (int x = xa; x <= xb; ++x) {
                 float n1, n2;
