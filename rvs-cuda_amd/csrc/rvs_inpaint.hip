@@ -14,7 +14,12 @@
 // closed under composition.  Hole pixels only ever read known pixels or holes of
 // their own 4-connected component, so components are solved independently
 // (labelled with a lock-free union-find) and each sweeps until it stops
-// changing, which is what the global loop gives every component.
+// changing, which is what the global loop gives every component.  One block
+// solves one component: in LDS when its bounding box fits (solve_component),
+// otherwise on the global map -- with the previous line in registers and the
+// next lines prefetched when its lines fit one block step (solve_lines), else
+// with several pixels per thread per step (solve_big).  Large components are
+// started first.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
