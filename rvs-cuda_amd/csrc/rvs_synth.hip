@@ -18,7 +18,8 @@
 //      bounding boxes touch; the counting pass keeps each work item's vertex
 //      block for the raster.
 //   2. k_raster: one 256-thread block per target tile, four sources per
-//      phase.  Every covered pixel takes an LDS 64-bit atomicMax, key1 = max
+//      phase (one phase, 8 waves per SIMD, for launch sets of up to four
+//      sources).  Every covered pixel takes an LDS 64-bit atomicMax, key1 = max
 //      of (score bits << 32 | candidate id), and a 32-bit one, key2 = max
 //      score over the candidates that are not the maximum (the loser of each
 //      key1 update).  A pixel whose runner-up scores within 2^-18 of the
@@ -29,7 +30,8 @@
 //      the kept vertices and the validity the coverage pass left in LDS --
 //      and blended.
 //   3. k_fallback: deferred pixels replay the reference fold literally, in
-//      traversal order, over the tile's bins sorted in LDS.
+//      traversal order (per-pixel candidate lists gathered in LDS, or a sort
+//      of the tile's bins when a list overflows).
 // Blending across sources (BlendedViewSimple::blendBasicMode) runs per pixel in
 // registers inside the same block, in InputCameraNames order.
 #include <hip/hip_runtime.h>
