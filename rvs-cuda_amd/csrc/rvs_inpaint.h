@@ -11,6 +11,7 @@
 namespace rvs_k {
 
 constexpr int kInpaintBlocks = 1024;   // persistent component solvers
+constexpr int kInpaintBigBlocks = 256;  // solvers of large components (one 1024-thread block each)
 
 // Indexed by global pixel (a view's pixels start at its px_base).
 struct InpaintWS {

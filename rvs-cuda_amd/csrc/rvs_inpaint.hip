@@ -19,7 +19,8 @@
 // otherwise on the global map -- with the previous line in registers and the
 // next lines prefetched when its lines fit one block step (solve_lines), else
 // with several pixels per thread per step (solve_big).  Large components are
-// started first.
+// started first; those whose lines fit a 1024-thread step have their own
+// kernel (k_inp_big).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -491,7 +492,8 @@ __device__ void solve_big(const Region& R, const InpaintWS& ws, Fn* wave_tot, in
     }
 }
 
-constexpr int kLineMax = 256 * kBigPx;   // longest line of solve_lines (one block step)
+constexpr int kBigThreads = 1024;                // block of k_inp_big
+constexpr int kLineMax = kBigThreads * kBigPx;   // longest line of solve_lines (one block step)
 
 // Large components whose lines fit one block step.  Each thread owns the same
 // kBigPx positions on every line, so the previous line's values from this
@@ -499,7 +501,9 @@ constexpr int kLineMax = 256 * kBigPx;   // longest line of solve_lines (one blo
 // line and one along-line neighbour (all independent loads) and takes one
 // barrier.  cols: lines are columns.  Same sweeps, candidate order and line
 // order semantics as solve_component.
+template <int NT>
 __device__ void solve_lines(const Region& R, const InpaintWS& ws, bool cols, Fn* wave_tot) {
+    constexpr int NW = NT / 64;   // waves per block
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int nlines = cols ? R.bw : R.bh;
@@ -610,7 +614,7 @@ __device__ void solve_lines(const Region& R, const InpaintWS& ws, bool cols, Fn*
             ex.cs = __shfl_up(A.cs, 1);
             ex.t = __shfl_up(A.t, 1);
             ex.k = __shfl_up(A.k, 1);
-            Fn* wt = wave_tot + 4 * (li & 1);   // alternate buffers: one barrier per line
+            Fn* wt = wave_tot + NW * (li & 1);   // alternate buffers: one barrier per line
             if (lane == 63) wt[wave] = A;
             __syncthreads();   // every load of this line is done before any write
             Fn pre;
@@ -671,8 +675,9 @@ __global__ void __launch_bounds__(256) k_inp_solve(const ViewDesc* __restrict__ 
     __shared__ int lsrc[kLdsCap];
     const int tid = threadIdx.x;
     const int ncomp = ws.counters[0];
-    // two passes over the components: the large ones (they take longest) are
-    // all started first -- found by a static stride, so the small ones cost no
+    // two passes over the components: the large ones that k_inp_big does not
+    // take (both box sides beyond its line length; they take longest) are all
+    // started first -- found by a static stride, so the small ones cost no
     // queue atomics here -- then the small ones are pulled from a queue
     for (int pass = 0; pass < 2; ++pass) {
     int next = blockIdx.x;
@@ -688,9 +693,12 @@ __global__ void __launch_bounds__(256) k_inp_solve(const ViewDesc* __restrict__ 
             __syncthreads();
         }
         if (c >= ncomp) break;
-        if ((ws.comp[4 * c + 1] - ws.comp[4 * c + 0] + 1) * (ws.comp[4 * c + 3] - ws.comp[4 * c + 2] + 1) <= kLdsCap
-            ? pass == 0 : pass == 1)
-            continue;   // the same decision in every thread of the block
+        {
+            const int bw = ws.comp[4 * c + 1] - ws.comp[4 * c + 0] + 1, bh = ws.comp[4 * c + 3] - ws.comp[4 * c + 2] + 1;
+            const bool small = bw * bh <= kLdsCap;
+            if (!small && (bw <= kLineMax || bh <= kLineMax)) continue;   // k_inp_big's
+            if (small ? pass == 0 : pass == 1) continue;   // the same decision in every thread of the block
+        }
         Region R;
         R.root = ws.root[c];
         const ViewDesc& vd = views[view_of_pixel(views, nviews, R.root)];
@@ -718,15 +726,36 @@ __global__ void __launch_bounds__(256) k_inp_solve(const ViewDesc* __restrict__ 
                 const int p = R.base + y * R.w + x;
                 if (ws.label[p] == R.root) { ws.dist[p] = ldist[k]; ws.src[p] = lsrc[k]; }
             }
-        } else if (R.bw <= kLineMax || R.bh <= kLineMax) {
-            // lines that fit one step; fewer lines when both orientations fit
-            const bool cols = R.bh <= kLineMax && (R.bw > kLineMax || R.bh > R.bw);
-            solve_lines(R, ws, cols, wave_tot);
         } else {
             solve_big(R, ws, wave_tot, s_carry);
         }
         __syncthreads();
     }
+    }
+}
+
+// Components too large for the LDS copy whose lines (along one side of the
+// box) fit one step of a 1024-thread block: one block each, static stride.
+__global__ void __launch_bounds__(kBigThreads) k_inp_big(const ViewDesc* __restrict__ views, int nviews,
+                                                         InpaintWS ws) {
+    __shared__ Fn wave_tot[2 * kBigThreads / 64];
+    const int ncomp = ws.counters[0];
+    for (int c = blockIdx.x; c < ncomp; c += gridDim.x) {
+        const int bw = ws.comp[4 * c + 1] - ws.comp[4 * c + 0] + 1, bh = ws.comp[4 * c + 3] - ws.comp[4 * c + 2] + 1;
+        if (bw * bh <= kLdsCap || (bw > kLineMax && bh > kLineMax)) continue;
+        Region R;
+        R.root = ws.root[c];
+        const ViewDesc& vd = views[view_of_pixel(views, nviews, R.root)];
+        R.base = (int)vd.px_base;
+        R.x0 = ws.comp[4 * c + 0];
+        R.y0 = ws.comp[4 * c + 2];
+        R.bw = bw; R.bh = bh;
+        R.w = vd.w; R.h = vd.h;
+        R.ldist = nullptr; R.lsrc = nullptr;
+        // fewer lines when both orientations fit
+        const bool cols = bh <= kLineMax && (bw > kLineMax || bh > bw);
+        solve_lines<kBigThreads>(R, ws, cols, wave_tot);
+        __syncthreads();
     }
 }
 
@@ -779,6 +808,7 @@ hipError_t launch_inpaint(const ViewDesc* views, int nviews, int max_w, int max_
     hipLaunchKernelGGL(k_ccl_merge, g, b, 0, s, views, ws);
     hipLaunchKernelGGL(k_ccl_flatten, g, b, 0, s, views, ws);
     hipLaunchKernelGGL(k_comp_bbox, g, b, 0, s, views, ws);
+    hipLaunchKernelGGL(k_inp_big, dim3(kInpaintBigBlocks), dim3(kBigThreads), 0, s, views, nviews, ws);
     hipLaunchKernelGGL(k_inp_solve, dim3(kInpaintBlocks), b, 0, s, views, nviews, ws);
     hipLaunchKernelGGL(k_inp_apply, g, b, 0, s, views, ws, map_out);
     return hipGetLastError();
