@@ -16,7 +16,10 @@ HIP_LIB = os.path.join(LIB_DIR, "librvs_hip.so")
 HOST_LIB = os.path.join(LIB_DIR, "librvs_host.so")
 
 RVS_OK = 0
+RVS_E_INVALID = -1
+RVS_E_HIP = -2
 RVS_E_CAPACITY = -3
+RVS_E_UNSUPPORTED = -4
 
 
 class RvsCam(C.Structure):
