@@ -14,7 +14,15 @@
 // closed under composition.  Hole pixels only ever read known pixels or holes of
 // their own 4-connected component, so components are solved independently
 // (labelled with a lock-free union-find) and each sweeps until it stops
-// changing, which is what the global loop gives every component.  One block
+// changing, which is what the global loop gives every component.  Two sweeps
+// always suffice: after the forward and the backward sweep every hole holds its
+// exact L1 distance to the nearest known pixel (the two-pass city-block
+// distance transform of Rosenfeld and Pfaltz: a nearest known pixel in any
+// quadrant is reached by a monotone path whose pixels are all holes, one leg
+// per sweep), so the reference's third sweep improves nothing -- the distances,
+// and the sources, which only change with a strictly smaller distance, are
+// final.  The solvers stop after the backward sweep (or after the forward one
+// if it changed nothing: then no hole touches a known pixel).  One block
 // solves one component: in LDS when its bounding box fits (solve_component),
 // otherwise on the global map -- with the previous line in registers and the
 // next lines prefetched when its lines fit one block step (solve_lines), else
@@ -29,6 +37,7 @@
 namespace rvs_k {
 
 constexpr int kAbsent = 1 << 29;
+constexpr int kSweeps = 2;   // forward, backward: the fixed point of the alternating sweeps
 
 __device__ inline int ld(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
@@ -143,46 +152,72 @@ __global__ void k_ccl_flatten(const ViewDesc* __restrict__ views, InpaintWS ws) 
     }
 }
 
-// Component bounding boxes.  The lanes of a wave that belong to the component
-// of its first hole are reduced first (usually the whole wave when a large
-// hole covers it), so a large component takes four atomics per wave instead of
-// four per pixel -- same-address device atomics are serialised at the memory
-// side; holes of other components in the wave use plain atomics.
-__global__ void k_comp_bbox(const ViewDesc* __restrict__ views, InpaintWS ws) {
+// Component bounding boxes.  Same-address device atomics are serialised at
+// the memory side, so a large hole must not cost atomics per pixel: each wave
+// reduces, over its kBboxPx pixels per lane, the box of the component of its
+// first hole (usually the whole wave when a large hole covers it), the block
+// merges the waves' boxes per component, and one thread takes four atomics per
+// distinct component of the block -- a few per 4096 pixels.  Holes of other
+// components use plain atomics.
+constexpr int kBboxThreads = 1024;
+constexpr int kBboxPx = 4;   // pixels per thread, kBboxThreads apart
+
+__global__ void __launch_bounds__(kBboxThreads) k_comp_bbox(const ViewDesc* __restrict__ views, InpaintWS ws) {
+    __shared__ int s_lid[kBboxThreads / 64];
+    __shared__ int s_box[kBboxThreads / 64][4];
     const ViewDesc& vd = views[blockIdx.y];
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in = p < vd.w * vd.h;
-    const int gp = (int)vd.px_base + (in ? p : 0);
-    const int r = in ? ws.label[gp] : -1;
-    const int id = r >= 0 ? ws.cid[r] : -1;
-    const int x = in ? p % vd.w : 0, y = in ? p / vd.w : 0;
-    const int lane = __lane_id();
-    const uint64_t act = __ballot(id >= 0);
-    if (act == 0ull) return;
-    // one reduced round for the component of the wave's first hole ...
-    const int leader = __ffsll((unsigned long long)act) - 1;
-    const int lid = __shfl(id, leader);
-    const bool mine = id >= 0 && id == lid;
-    int x0 = mine ? x : INT_MAX, x1 = mine ? x : INT_MIN;
-    int y0 = mine ? y : INT_MAX, y1 = mine ? y : INT_MIN;
+    const int n = vd.w * vd.h;
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    int lid = -1;   // wave-uniform: component of the wave's first hole
+    int x0 = INT_MAX, x1 = INT_MIN, y0 = INT_MAX, y1 = INT_MIN;
+    for (int k = 0; k < kBboxPx; ++k) {
+        const int p = blockIdx.x * (kBboxThreads * kBboxPx) + k * kBboxThreads + (int)threadIdx.x;
+        const bool in = p < n;
+        const int gp = (int)vd.px_base + (in ? p : 0);
+        const int r = in ? ws.label[gp] : -1;
+        const int id = r >= 0 ? ws.cid[r] : -1;
+        const uint64_t act = __ballot(id >= 0);
+        if (act == 0ull) continue;
+        if (lid < 0) lid = __shfl(id, __ffsll((unsigned long long)act) - 1);
+        const int x = p % vd.w, y = p / vd.w;
+        if (id == lid) {
+            x0 = min(x0, x); x1 = max(x1, x);
+            y0 = min(y0, y); y1 = max(y1, y);
+        } else if (id >= 0) {
+            atomicMin(&ws.comp[4 * id + 0], x);
+            atomicMax(&ws.comp[4 * id + 1], x);
+            atomicMin(&ws.comp[4 * id + 2], y);
+            atomicMax(&ws.comp[4 * id + 3], y);
+        }
+    }
     for (int off = 32; off > 0; off >>= 1) {
         x0 = min(x0, __shfl_xor(x0, off));
         x1 = max(x1, __shfl_xor(x1, off));
         y0 = min(y0, __shfl_xor(y0, off));
         y1 = max(y1, __shfl_xor(y1, off));
     }
-    if (lane == leader) {
-        atomicMin(&ws.comp[4 * lid + 0], x0);
-        atomicMax(&ws.comp[4 * lid + 1], x1);
-        atomicMin(&ws.comp[4 * lid + 2], y0);
-        atomicMax(&ws.comp[4 * lid + 3], y1);
+    if (lane == 0) {
+        s_lid[wave] = lid;
+        s_box[wave][0] = x0; s_box[wave][1] = x1; s_box[wave][2] = y0; s_box[wave][3] = y1;
     }
-    // ... and plain atomics for holes of other (small) components
-    if (id >= 0 && !mine) {
-        atomicMin(&ws.comp[4 * id + 0], x);
-        atomicMax(&ws.comp[4 * id + 1], x);
-        atomicMin(&ws.comp[4 * id + 2], y);
-        atomicMax(&ws.comp[4 * id + 3], y);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        constexpr int NW = kBboxThreads / 64;
+        for (int w = 0; w < NW; ++w) {
+            const int id = s_lid[w];
+            if (id < 0) continue;
+            int b0 = s_box[w][0], b1 = s_box[w][1], b2 = s_box[w][2], b3 = s_box[w][3];
+            for (int u = w + 1; u < NW; ++u)
+                if (s_lid[u] == id) {
+                    b0 = min(b0, s_box[u][0]); b1 = max(b1, s_box[u][1]);
+                    b2 = min(b2, s_box[u][2]); b3 = max(b3, s_box[u][3]);
+                    s_lid[u] = -1;
+                }
+            atomicMin(&ws.comp[4 * id + 0], b0);
+            atomicMax(&ws.comp[4 * id + 1], b1);
+            atomicMin(&ws.comp[4 * id + 2], b2);
+            atomicMax(&ws.comp[4 * id + 3], b3);
+        }
     }
 }
 
@@ -248,7 +283,7 @@ __device__ inline void region_put(const Region& R, const InpaintWS& ws, int x, i
     }
 }
 
-// One component: alternate forward / backward sweeps until one changes nothing.
+// One component: a forward sweep, then a backward one if the first changed anything.
 // Lines run along the box's shorter side and are processed in sweep order; each
 // line is a scan along the longer side.  Row lines scan x (forward: the scan
 // input is the left neighbour, rank 1; backward: the right one, rank 4); column
@@ -262,8 +297,7 @@ __device__ void solve_component(const Region& R, const InpaintWS& ws, Fn* wave_t
     const bool cols = R.bh > R.bw;
     const int nlines = cols ? R.bw : R.bh;
     const int llen = cols ? R.bh : R.bw;
-    const int max_sweeps = 4 * (R.w + R.h) + 8;   // each changing sweep lowers a distance: a guard only
-    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    for (int sweep = 0; sweep < kSweeps; ++sweep) {
         const bool fwd = (sweep & 1) == 0;
         const int rs = cols ? (fwd ? 2 : 3) : (fwd ? 1 : 4);
         bool changed = false;
@@ -372,8 +406,7 @@ __device__ void solve_big(const Region& R, const InpaintWS& ws, Fn* wave_tot, in
     const int nlines = cols ? R.bw : R.bh;
     const int llen = cols ? R.bh : R.bw;
     const int span = 256 * kBigPx;
-    const int max_sweeps = 4 * (R.w + R.h) + 8;   // each changing sweep lowers a distance: a guard only
-    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    for (int sweep = 0; sweep < kSweeps; ++sweep) {
         const bool fwd = (sweep & 1) == 0;
         const int rs = cols ? (fwd ? 2 : 3) : (fwd ? 1 : 4);
         // neighbour offsets in the reference order left(1) up(2) down(3) right(4)
@@ -508,8 +541,7 @@ __device__ void solve_lines(const Region& R, const InpaintWS& ws, bool cols, Fn*
     const int lane = tid & 63, wave = tid >> 6;
     const int nlines = cols ? R.bw : R.bh;
     const int llen = cols ? R.bh : R.bw;
-    const int max_sweeps = 4 * (R.w + R.h) + 8;   // each changing sweep lowers a distance: a guard only
-    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    for (int sweep = 0; sweep < kSweeps; ++sweep) {
         const bool fwd = (sweep & 1) == 0;
         // ranks (left 1, up 2, down 3, right 4) of the scan input, of the
         // across-line neighbour already visited in this sweep, of the one not yet
@@ -585,16 +617,10 @@ __device__ void solve_lines(const Region& R, const InpaintWS& ws, bool cols, Fn*
                     int bv = v, br = 0, bs = ss[k];
                     const int alv = k + 1 < kBigPx ? sv[k + 1] : av, als = k + 1 < kBigPx ? ss[k + 1] : as_;
                     const bool alin = k + 1 < kBigPx ? sin[k + 1] : ain;
-#pragma unroll
-                    for (int n = 1; n <= 4; ++n) {
-                        if (n == rs) continue;
-                        int cvv, css;
-                        bool ok;
-                        if (n == r_prev) { cvv = pv[k]; css = ps[k]; ok = pin[k]; }
-                        else if (n == r_next) { cvv = nv[k]; css = ns[k]; ok = nin[k]; }
-                        else { cvv = alv; css = als; ok = alin && n == r_along; }   // the remaining rank
-                        if (ok && cvv != kOther) cand(bv, br, bs, cvv + 1, n, css);
-                    }
+                    // cand keeps the (value, rank) minimum, so the calls' order is free
+                    if (pin[k] && pv[k] != kOther) cand(bv, br, bs, pv[k] + 1, r_prev, ps[k]);
+                    if (nin[k] && nv[k] != kOther) cand(bv, br, bs, nv[k] + 1, r_next, ns[k]);
+                    if (alin && alv != kOther) cand(bv, br, bs, alv + 1, r_along, als);
                     f[k].cv = bv; f[k].cs = bs; f[k].t = (rs < br) ? 1 : 0;
                 }
             }
@@ -807,7 +833,9 @@ hipError_t launch_inpaint(const ViewDesc* views, int nviews, int max_w, int max_
                        views, ws);
     hipLaunchKernelGGL(k_ccl_merge, g, b, 0, s, views, ws);
     hipLaunchKernelGGL(k_ccl_flatten, g, b, 0, s, views, ws);
-    hipLaunchKernelGGL(k_comp_bbox, g, b, 0, s, views, ws);
+    const size_t per_block = (size_t)kBboxThreads * kBboxPx;
+    hipLaunchKernelGGL(k_comp_bbox, dim3((unsigned)(((size_t)max_w * max_h + per_block - 1) / per_block), (unsigned)nviews),
+                       dim3(kBboxThreads), 0, s, views, ws);
     hipLaunchKernelGGL(k_inp_big, dim3(kInpaintBigBlocks), dim3(kBigThreads), 0, s, views, nviews, ws);
     hipLaunchKernelGGL(k_inp_solve, dim3(kInpaintBlocks), b, 0, s, views, nviews, ws);
     hipLaunchKernelGGL(k_inp_apply, g, b, 0, s, views, ws, map_out);

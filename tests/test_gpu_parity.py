@@ -307,6 +307,32 @@ def test_inpaint_large_components(torch_cuda):
     q = np.where(xx * 0.6 + yy > 300, 0.0, 1.0).astype(np.float32)   # an uncovered wedge
     q[rng.random((h, w)) < 0.01] = 0.0
     _inpaint_case(torch, q, "wedge")
+    # a corridor winding between known walls (one long component)
+    h, w = 300, 2400
+    q = np.zeros((h, w), np.float32)
+    y0, x0, y1, x1 = 1, 1, h - 2, w - 2
+    while y0 <= y1 and x0 <= x1:
+        q[y0, x0:x1 + 1] = 1.0
+        q[y0:y1 + 1, x1] = 1.0
+        if y1 - y0 >= 2:
+            q[y1, x0 + 2:x1 + 1] = 1.0
+        if x1 - x0 >= 2:
+            q[y0 + 2:y1 + 1, x0 + 2] = 1.0
+        y0, x0, y1, x1 = y0 + 6, x0 + 6, y1 - 6, x1 - 6
+    _inpaint_case(torch, q, "spiral")
+
+
+def test_inpaint_component_beyond_line_length(torch_cuda):
+    """A component whose box exceeds 4096 pixels in both directions (several
+    block spans per line)."""
+    torch = torch_cuda
+    rng = np.random.default_rng(9)
+    h = w = 4200
+    q = np.ones((h, w), np.float32)
+    q[2:h - 3, 4:w - 2] = 0.0
+    q[rng.random((h, w)) < 0.0005] = 1.0   # known islands inside
+    q[h // 2, : w // 3] = 1.0              # and a wall from the left border
+    _inpaint_case(torch, q, "4200x4200")
 
 
 # ------------------------------------------------------------------ encode

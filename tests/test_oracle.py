@@ -450,6 +450,57 @@ def test_inpaint_tie_choices_by_hand():
     assert m[1, 1].tolist() == [0, 1, 1]
 
 
+def _spiral(h, w):
+    """A one-pixel-wide known wall winding inwards: holes form a long corridor."""
+    hole = np.ones((h, w), bool)
+    y0, x0, y1, x1 = 1, 1, h - 2, w - 2
+    while y0 <= y1 and x0 <= x1:
+        hole[y0, x0:x1 + 1] = False
+        hole[y0:y1 + 1, x1] = False
+        if y1 - y0 >= 2:
+            hole[y1, x0 + 2:x1 + 1] = False
+        if x1 - x0 >= 2:
+            hole[y0 + 2:y1 + 1, x0 + 2] = False
+        y0, x0, y1, x1 = y0 + 4, x0 + 4, y1 - 4, x1 - 4
+    return hole
+
+
+def test_inpaint_two_sweeps_reach_the_fixed_point():
+    """The alternating sweeps of compute_nearest stop after at most three
+    sweeps, the third changing nothing: forward + backward already give every
+    hole its exact L1 distance (two-pass city-block distance transform).  The
+    device solvers stop after two sweeps on that ground.  Shapes that would
+    need more sweeps if paths had to bend through holes: spirals, combs, thin
+    diagonals, sparse and dense random holes, single known pixels."""
+    rng = np.random.default_rng(11)
+    shapes = []
+    for h, w in ((23, 31), (40, 17), (64, 64)):
+        shapes.append(_spiral(h, w))
+        comb = np.ones((h, w), bool)
+        comb[::3, : w - 2] = False            # teeth from the left
+        comb[1::6, 2:] = False                # and from the right
+        shapes.append(comb)
+        diag = np.ones((h, w), bool)
+        for k in range(0, min(h, w), 2):
+            diag[k, k] = False
+        shapes.append(diag)
+        one = np.ones((h, w), bool)
+        one[rng.integers(h), rng.integers(w)] = False
+        shapes.append(one)
+        for frac in (0.3, 0.7, 0.95, 0.995):
+            shapes.append(rng.random((h, w)) < frac)
+        blob = np.ones((h, w), bool)
+        blob[rng.random((h, w)) < 0.02] = False
+        shapes.append(blob)
+    for hole in shapes:
+        q = np.where(hole, 0.0, 1.0).astype(np.float32)
+        col = rng.random(hole.shape + (3,)).astype(np.float32)
+        out, m, sweeps = ob.inpaint(col, q, want_map=True)
+        assert sweeps <= 3, sweeps
+        if (~hole).any():
+            assert np.array_equal(m[..., 2], np.where(hole, bfs_dist(hole), 0))
+
+
 def test_inpaint_no_known_pixel_keeps_colour():
     col = np.random.default_rng(7).random((4, 5, 3)).astype(np.float32)
     out, m, sweeps = ob.inpaint(col, np.zeros((4, 5), np.float32), want_map=True)
