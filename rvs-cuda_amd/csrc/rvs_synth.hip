@@ -380,15 +380,66 @@ __device__ inline bool quad_tiles(const Quad& Q, int& tx0, int& tx1, int& ty0, i
     return true;
 }
 
+// Block-cooperative walk over the tiles of the threads' quad ranges: entry e
+// of the block's concatenated ranges goes to thread e % 256, which finds the
+// owning quad by a binary search of the block-wide inclusive scan of the
+// ranges' tile counts.  One quad with a wide range (a stretched triangle
+// across a depth edge, a quad near an equirectangular pole) then costs every
+// lane of the block a few iterations instead of holding its wave.  Ranges of
+// at most kLaneTiles tiles (nearly all) are walked by their own lane, and a
+// block without a wider one skips the scan.
+struct RangeWalk {
+    uint32_t incl[256];   // inclusive prefix of the tile counts
+    uint32_t org[256];    // tx0 | ty0 << 16
+    uint32_t wid[256];    // tx1 - tx0 + 1
+    uint32_t wsum[4];     // wave totals
+};
+
+constexpr uint32_t kLaneTiles = 4;   // ranges up to this many tiles stay with their own lane
+
+// Publishes this thread's range (n = 0: none); all 256 threads call it, and
+// the walk may start after the __syncthreads() that follows.
+__device__ inline void range_publish(RangeWalk& W, int tid, bool on, int tx0, int tx1, int ty0, int ty1) {
+    const uint32_t n = on ? (uint32_t)(tx1 - tx0 + 1) * (uint32_t)(ty1 - ty0 + 1) : 0u;
+    const int lane = tid & 63, wave = tid >> 6;
+    uint32_t x = n;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) W.wsum[wave] = x;
+    W.org[tid] = on ? ((uint32_t)tx0 | ((uint32_t)ty0 << 16)) : 0u;
+    W.wid[tid] = on ? (uint32_t)(tx1 - tx0 + 1) : 1u;
+    __syncthreads();
+    for (int w = 0; w < wave; ++w) x += W.wsum[w];
+    W.incl[tid] = x;
+}
+
+template <class F>
+__device__ inline void range_walk(const RangeWalk& W, int tid, F fn) {
+    const uint32_t total = W.incl[255];
+    for (uint32_t e = (uint32_t)tid; e < total; e += 256u) {
+        int lo = 0, hi = 255;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (W.incl[mid] > e) hi = mid; else lo = mid + 1;
+        }
+        const uint32_t l = e - (lo ? W.incl[lo - 1] : 0u);
+        const uint32_t o = W.org[lo], w = W.wid[lo];
+        fn(lo, (int)(o & 0xffffu) + (int)(l % w), (int)(o >> 16) + (int)(l / w));
+    }
+}
+
 // FLAT: blocks stride over the kept work items (counting pass only);
 // otherwise persistent blocks stride over the items after the kept ones.
 template <class G, bool FILL, bool FLAT>
-__global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_bin(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
                                              BatchDims d, RasterWS ws) {
     __shared__ float3 V[kBinQY + 1][kBinQX + 1];
     __shared__ uint32_t t_key[kBinTable];
     __shared__ uint32_t t_n[2][kBinTable];      // entries per bin: [0] multi / fill, [1] single-tile
     __shared__ uint32_t t_base[2][kBinTable];   // [0] fill write cursor, [1] first single-tile slot
+    __shared__ RangeWalk walk;
     const int tid = threadIdx.x;
     const uint32_t per = d.bins_x * d.bins_y;
     if (FILL && ws.status[1]) return;   // list capacity exceeded: nothing to fill
@@ -409,7 +460,6 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
         const int i0 = (int)(item / d.bins_x) * kBinQY, j0 = (int)(item % d.bins_x) * kBinQX;
         const int i = i0 + tid / kBinQX, j = j0 + tid % kBinQX;
         const bool stored = !FILL && wi < d.range_cap;   // same for the whole block
-        const uint32_t q = ((uint32_t)i << 16) | (uint32_t)j;
         if (tid < kBinTable) {
             t_key[tid] = kTableEmpty;
             t_n[0][tid] = 0; t_n[1][tid] = 0;
@@ -452,21 +502,36 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
             my_slot = table_slot(t_key, t);
             if (my_slot >= 0) rank = atomicAdd(&t_n[1][my_slot], 1u);
             else rank = atomicAdd(&ws.single[bin_of(t)], 1u);   // table full: final slot now
-        } else if (more) {
-            for (int ty = ty0; ty <= ty1; ++ty)
-                for (int tx = tx0; tx <= tx1; ++tx) {
-                    const uint32_t t = tile_of(tx, ty);
-                    const int k = table_slot(t_key, t);
-                    if (k >= 0) {
-                        atomicAdd(&t_n[0][k], 1u);
-                    } else if (FILL) {
-                        const size_t bin = bin_of(t);
-                        ws.lists[(size_t)ws.offsets[bin] + ws.single[bin] + atomicAdd(&ws.fill[bin], 1u)] = q;
-                    } else {
-                        atomicAdd(&ws.multi[bin_of(t)], 1u);
-                    }
-                }
         }
+        // multi entries: narrow ranges by their own lane, wide ones dealt to the block
+        auto quad_of = [&](int lane) {
+            return ((uint32_t)(i0 + lane / kBinQX) << 16) | (uint32_t)(j0 + lane % kBinQX);
+        };
+        const bool multi = more && !single;
+        const bool wide = multi && (uint32_t)(tx1 - tx0 + 1) * (uint32_t)(ty1 - ty0 + 1) > kLaneTiles;
+        const bool any_wide = __syncthreads_or(wide);
+        if (any_wide) {
+            range_publish(walk, tid, wide, tx0, tx1, ty0, ty1);
+            __syncthreads();   // ranges scanned
+        }
+        auto each_tile = [&](auto fn) {
+            if (multi && !wide)
+                for (int ty = ty0; ty <= ty1; ++ty)
+                    for (int tx = tx0; tx <= tx1; ++tx) fn(tid, tx, ty);
+            if (any_wide) range_walk(walk, tid, fn);
+        };
+        each_tile([&](int lane, int tx, int ty) {
+            const uint32_t t = tile_of(tx, ty);
+            const int k = table_slot(t_key, t);
+            if (k >= 0) {
+                atomicAdd(&t_n[0][k], 1u);
+            } else if (FILL) {
+                const size_t bin = bin_of(t);
+                ws.lists[(size_t)ws.offsets[bin] + ws.single[bin] + atomicAdd(&ws.fill[bin], 1u)] = quad_of(lane);
+            } else {
+                atomicAdd(&ws.multi[bin_of(t)], 1u);
+            }
+        });
         __syncthreads();   // table complete
         if (tid < kBinTable && t_key[tid] != kTableEmpty) {
             const size_t bin = bin_of(t_key[tid]);
@@ -479,13 +544,11 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
             }
         }
         __syncthreads();   // bases known
-        if (FILL && more && !single) {
-            for (int ty = ty0; ty <= ty1; ++ty)
-                for (int tx = tx0; tx <= tx1; ++tx) {
-                    const int k = find_slot(t_key, tile_of(tx, ty));
-                    if (k >= 0) ws.lists[atomicAdd(&t_base[0][k], 1u)] = q;
-                }
-        }
+        if (FILL)
+            each_tile([&](int lane, int tx, int ty) {
+                const int k = find_slot(t_key, tile_of(tx, ty));
+                if (k >= 0) ws.lists[atomicAdd(&t_base[0][k], 1u)] = quad_of(lane);
+            });
         if (!FILL && stored) {
             uint2 r;
             r.x = (tx1 >= tx0) ? pack_range(tx0, tx1, ty0, ty1) : kRangeNone;
@@ -502,11 +565,12 @@ __global__ void __launch_bounds__(256) k_bin(const G* __restrict__ geoms, const 
 // bin's single-tile region through an LDS bin table and the fill cursors.  A quad whose
 // range was too wide to pack is re-binned from the kept vertices.
 template <class G>
-__global__ void __launch_bounds__(256) k_fill_kept(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_fill_kept(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
                                                    BatchDims d, RasterWS ws) {
     __shared__ uint32_t t_key[kBinTable];
     __shared__ uint32_t t_n[kBinTable];
     __shared__ uint32_t t_base[kBinTable];
+    __shared__ RangeWalk walk;
     if (ws.status[1]) return;   // list capacity exceeded: nothing to fill
     const uint32_t lim = min(ws.status[4], d.range_cap);
     const int tid = threadIdx.x;
@@ -547,35 +611,44 @@ __global__ void __launch_bounds__(256) k_fill_kept(const G* __restrict__ geoms, 
                 more = true;
             }
         }
-        // multi-tile entries: LDS bin table, one fill-cursor atomic per bin
+        // multi-tile entries (narrow ranges by their own lane, wide ones dealt
+        // to the block): LDS bin table, one fill-cursor atomic per bin
         auto tile_of = [&](int tx, int ty) { return vd.tile_base + (uint32_t)(ty * vd.tiles_x + tx); };
-        __syncthreads();   // table cleared
-        if (more) {
-            for (int ty = ty0; ty <= ty1; ++ty)
-                for (int tx = tx0; tx <= tx1; ++tx) {
-                    const uint32_t t = tile_of(tx, ty);
-                    const int k = table_slot(t_key, t);
-                    if (k >= 0) {
-                        atomicAdd(&t_n[k], 1u);
-                    } else {
-                        const size_t bin = (size_t)t * d.nsrc + s;
-                        ws.lists[(size_t)ws.offsets[bin] + ws.single[bin] + atomicAdd(&ws.fill[bin], 1u)] = q;
-                    }
-                }
+        auto quad_of = [&](int lane) {
+            return ((uint32_t)(i0 + lane / kBinQX) << 16) | (uint32_t)(j0 + lane % kBinQX);
+        };
+        const bool wide = more && (uint32_t)(tx1 - tx0 + 1) * (uint32_t)(ty1 - ty0 + 1) > kLaneTiles;
+        const bool any_wide = __syncthreads_or(wide);   // also orders the table clear
+        if (any_wide) {
+            range_publish(walk, tid, wide, tx0, tx1, ty0, ty1);
+            __syncthreads();   // ranges scanned
         }
+        auto each_tile = [&](auto fn) {
+            if (more && !wide)
+                for (int ty = ty0; ty <= ty1; ++ty)
+                    for (int tx = tx0; tx <= tx1; ++tx) fn(tid, tx, ty);
+            if (any_wide) range_walk(walk, tid, fn);
+        };
+        each_tile([&](int lane, int tx, int ty) {
+            const uint32_t t = tile_of(tx, ty);
+            const int k = table_slot(t_key, t);
+            if (k >= 0) {
+                atomicAdd(&t_n[k], 1u);
+            } else {
+                const size_t bin = (size_t)t * d.nsrc + s;
+                ws.lists[(size_t)ws.offsets[bin] + ws.single[bin] + atomicAdd(&ws.fill[bin], 1u)] = quad_of(lane);
+            }
+        });
         __syncthreads();   // table complete
         if (tid < kBinTable && t_n[tid]) {
             const size_t bin = (size_t)t_key[tid] * d.nsrc + s;
             t_base[tid] = ws.offsets[bin] + ws.single[bin] + atomicAdd(&ws.fill[bin], t_n[tid]);
         }
         __syncthreads();   // bases known
-        if (more) {
-            for (int ty = ty0; ty <= ty1; ++ty)
-                for (int tx = tx0; tx <= tx1; ++tx) {
-                    const int k = find_slot(t_key, tile_of(tx, ty));
-                    if (k >= 0) ws.lists[atomicAdd(&t_base[k], 1u)] = q;
-                }
-        }
+        each_tile([&](int lane, int tx, int ty) {
+            const int k = find_slot(t_key, tile_of(tx, ty));
+            if (k >= 0) ws.lists[atomicAdd(&t_base[k], 1u)] = quad_of(lane);
+        });
         __syncthreads();   // table free for the next item
     }
 }
