@@ -25,6 +25,9 @@ Prints one JSON line:
   serving_mix   two resident scenes (the ERP set + 11 perspective 2048x2048
                 views), requests from 720p to 4K perspective and 2048x1024
                 ERP: closed-loop capacity and Poisson arrivals at 70% of it.
+  serving_scenes_8k  eight resident scenes of 11 ERP 4096x2048 sources,
+                requests for 8192x4096 ERP and 3840x2160 perspective views:
+                closed-loop capacity (best of batch 1, 2, 4).
   roofline      the dominant kernel (k_raster, tile raster + blend) on one
                 launch set of 8 views: algorithmic bytes (every source sample
                 once at its stored precision + every output sample once) /
@@ -122,6 +125,9 @@ def main():
     ap.add_argument("--no-c3", action="store_true", help="skip the mixed-size 64-pose graph batch")
     ap.add_argument("--no-c4", action="store_true", help="skip the two-scene serving mix")
     ap.add_argument("--c4-requests", type=int, default=256)
+    ap.add_argument("--no-c5", action="store_true", help="skip the eight-scene 8K ERP / 4K serving run")
+    ap.add_argument("--c5-scenes", type=int, default=8)
+    ap.add_argument("--c5-requests", type=int, default=64)
     ap.add_argument("--roofline-only", type=int, default=0,
                     help="only run the roofline launch set (this many views) 5 times (counter passes)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -339,6 +345,40 @@ def main():
         del srcs_b
         log("c4 %s" % c4)
 
+    c5 = None
+    if not args.no_c5:
+        # eight resident scenes of 11 ERP 4096x2048 sources each (3 x 4 grid,
+        # 0.1 m pitch); requests for 8192x4096 ERP and 3840x2160 perspective
+        # views over all scenes, served closed loop at batch 1, 2 and 4 (2
+        # slots); capacity is the best of them
+        a = time.perf_counter()
+        rigs5 = [W.rig("erp11_4096:%d" % sd) for sd in range(5, 5 + args.c5_scenes)]
+        srcs5 = [W.device_sources(r5, W.render_sources(r5)) for r5 in rigs5]
+        log("c5 %d scenes rendered in %.1fs" % (len(rigs5), time.perf_counter() - a))
+        mix5 = W.serve_requests(rigs5, args.c5_requests, seed=5, sizes=((8192, 4096, 1), (3840, 2160, 0)))
+        budget5 = max(budget, 128 << 30)
+        trials5 = []
+        for b5 in (1, 2, 4):
+            srv = S.Server(srcs5, memory_budget=budget5, max_batch=b5, slots=2, entry_cap=0,
+                           host_staging_bytes=b5 * max(A.yuv_bytes(t) for _, t in mix5))
+            try:
+                srv.run_closed(mix5[:4 * b5], concurrency=2 * b5)
+                a = time.perf_counter()
+                st5 = srv.run_closed(mix5, concurrency=2 * b5)
+                closed = S.summarize(st5, wall_s=time.perf_counter() - a)
+                closed["max_batch"] = b5
+                trials5.append(closed)
+                log("c5 trial %s" % closed)
+            finally:
+                srv.close()
+        c5 = {"scenes": "%d x 11 ERP 4096x2048, resident" % len(rigs5),
+              "targets": "8192x4096 ERP, 3840x2160 perspective", "requests": len(mix5),
+              "resident_source_bytes": int(sum(b_alg(r5, []) for r5 in rigs5)),
+              "closed_loop": max(trials5, key=lambda t: t["views_s"]), "batch_trials": trials5}
+        del srcs5
+        torch.cuda.empty_cache()
+        log("c5 %s" % c5)
+
     out = {
         "metric": "served views/s + p50/p99 ms vs plain per-view; 4x4096x2048 ERP->1080p",
         "value": served["views_s"],
@@ -381,6 +421,7 @@ def main():
         "batched": batched,
         "batched_mixed_graph": c3,
         "serving_mix": c4,
+        "serving_scenes_8k": c5,
         "autotune": {"trials": trials},
         "roofline": {
             "kernel": "k_raster (tile raster + blend), one launch over %d views" % ROOFLINE_VIEWS,
