@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -150,6 +151,12 @@ void plan_raster(Layout& L, RasterPlan& R, int nsrc, int nviews, uint32_t ntiles
     // re-evaluate vertices.
     uint64_t keep = items / 8 > 4096 ? items / 8 : 4096;
     if ((uint64_t)d.cap / 240 > keep) keep = (uint64_t)d.cap / 240;
+    // RVS_KEEP_ITEMS=<n> lowers the kept range (less workspace; the items
+    // beyond it take the re-evaluating passes, with the same results)
+    if (const char* env = std::getenv("RVS_KEEP_ITEMS")) {
+        const long long n = std::strtoll(env, nullptr, 10);
+        if (n > 0 && (uint64_t)n < keep) keep = (uint64_t)n;
+    }
     d.range_cap = (uint32_t)(keep < items ? keep : items);
     const size_t nbins = (size_t)nsrc * ntiles;
     R.status = L.take(16 * sizeof(uint32_t));

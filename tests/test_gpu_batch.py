@@ -253,3 +253,39 @@ def test_dense_sources_compact_bins(torch_cuda):
     say("  largest bin %d, entries %d, capacity grown to %d" % (cnt["max_bin"], cnt["bin_entries"], syn.entry_cap))
     assert cnt["max_bin"] > 2048 and syn.entry_cap >= cnt["bin_entries"]
     _check_oracle(out, _oracle(rg, planes, tgt), tgt.bitdepth, "dense")
+
+
+def test_binning_beyond_the_kept_range(torch_cuda, monkeypatch):
+    """Work items beyond the kept range (here all but one: RVS_KEEP_ITEMS=1)
+    are counted and filled by the re-evaluating binning passes, wide tile
+    ranges included: every map equals the default run's (all items kept)."""
+    torch = torch_cuda
+    say("case binning beyond the kept range")
+    cases = []
+    rg = W.rig("erp4_1080p", scale=4)
+    srcs = W.device_sources(rg, W.render_sources(rg))
+    erp = R.make_target(R.cam(1, 512, 256), [0.05, 0.02, 0.0], [20.0, 0.0, 0.0], 10, 1.0,
+                        R.scale_of(10), R.scale_of(16))
+    cases.append(("erp4/4", srcs, [erp] + W.batch_targets(rg, 3, seed=3, scale=4)))
+    rg_b = W.rig("persp11_2048", scale=4)
+    srcs_b = W.device_sources(rg_b, W.render_sources(rg_b))
+    cases.append(("persp11/4", srcs_b, [W.device_target(rg_b, [0.01, 0.03, -0.02], [5.0, 2.0, 0.0])]))
+    for tag, s, tg in cases:
+        say("  %s, %d views" % (tag, len(tg)))
+        monkeypatch.delenv("RVS_KEEP_ITEMS", raising=False)
+        ref = [{k: v.clone() for k, v in o.items() if v is not None}
+               for o in R.BatchSynthesizer(s, tg, want_maps=True).run()]
+        torch.cuda.synchronize()
+        monkeypatch.setenv("RVS_KEEP_ITEMS", "1")
+        R.profile_enable(True)
+        try:
+            outs = R.BatchSynthesizer(s, tg, want_maps=True).run()
+            torch.cuda.synchronize()
+            cnt = R.profile_counters()
+        finally:
+            R.profile_enable(False)
+            monkeypatch.delenv("RVS_KEEP_ITEMS", raising=False)
+        assert cnt["bin_work_items"] > 1
+        for k in range(len(tg)):
+            for key in ("color", "quality", "depth", "validity", "y", "u", "v"):
+                assert torch.equal(outs[k][key], ref[k][key]), "%s view %d map %s" % (tag, k, key)
