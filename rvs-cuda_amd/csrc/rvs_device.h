@@ -257,11 +257,25 @@ __device__ inline void tri_load(Tri& T, float3 a, float3 b, float3 c) {
 // norms, / g_rescale; validity = clamp(10000 - 1000 * stretch, 1, 10000).  The
 // clamp's max(1, q) maps NaN to 1, so the result is never 0 and
 // colorize_triangle's `validity == 0` return is never taken.
+//
+// The three norms share one square root: (float) sqrt(s) of a double s is
+// monotone non-decreasing (correctly rounded sqrt, then rounding to float) and
+// maps NaN to NaN, so applying it after the same (a < b) ? b : a maxima on the
+// squared lengths gives the same bits as the maximum of the three norms; the
+// zero third component adds +0.0 to a non-negative sum, which changes nothing.
+__device__ inline double edge_sq_d(float dx, float dy) {
+    double s = (double)dx * (double)dx;
+    s += (double)dy * (double)dy;
+    return s;
+}
+
+__device__ inline double dmax_d(double a, double b) { return (a < b) ? b : a; }
+
 __device__ inline float tri_validity(const Tri& T, float rescale) {
-    const float dab = norm_sq_sum_d(T.A0 - T.B0, T.A1 - T.B1, 0.f);
-    const float dac = norm_sq_sum_d(T.A0 - T.C0, T.A1 - T.C1, 0.f);
-    const float dbc = norm_sq_sum_d(T.B0 - T.C0, T.B1 - T.C1, 0.f);
-    float stretch = smax(dbc, smax(dab, dac));
+    const double sab = edge_sq_d(T.A0 - T.B0, T.A1 - T.B1);
+    const double sac = edge_sq_d(T.A0 - T.C0, T.A1 - T.C1);
+    const double sbc = edge_sq_d(T.B0 - T.C0, T.B1 - T.C1);
+    float stretch = (float)__builtin_sqrt(dmax_d(sbc, dmax_d(sab, sac)));
     stretch /= rescale;
     float quality = 10000.f - 1000.f * stretch;
     quality = smax(1.f, quality);
