@@ -253,37 +253,53 @@ __device__ inline bool item_culled<PairGeom>(const PairGeom& g, int bx, int by, 
 
 // One thread per (view, source, item): append the items that may cover the view
 // to the work list (code = (view * nsrc + source) * items + item).
+// One block row per (view, source) pair (blockIdx.y, striding): the pair's
+// descriptor is then uniform and read with scalar loads.  Survivors are
+// appended to the work list with one atomic per 1024-thread block (the waves'
+// counts are combined in LDS): the list's counter is a single address, and
+// same-address device atomics are serialised at the memory side.
+constexpr int kCullThreads = 1024;
+
 template <class G>
-__global__ void k_bin_cull(const G* __restrict__ geoms, const ViewDesc* __restrict__ views, BatchDims d,
-                           RasterWS ws) {
+__global__ void __launch_bounds__(kCullThreads) k_bin_cull(const G* __restrict__ geoms,
+                                                           const ViewDesc* __restrict__ views, BatchDims d,
+                                                           RasterWS ws) {
+    __shared__ uint32_t s_cnt[kCullThreads / 64];
+    __shared__ uint32_t s_base;
     const uint32_t per = d.bins_x * d.bins_y;
-    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t total = (size_t)per * d.nsrc * d.nviews;
-    bool keep = false;
-    if (gid < total) {
-        const uint32_t item = (uint32_t)(gid % per);
-        const uint32_t vs = (uint32_t)(gid / per);
-        const int v = (int)(vs / (uint32_t)d.nsrc);
-        const G& g = geoms[vs];
-        const int bx = (int)(item % d.bins_x), by = (int)(item / d.bins_x);
-        if (by * kBinQY < g.sh - 1 && bx * kBinQX < g.sw - 1) keep = !item_culled(g, bx, by, views[v].w, views[v].h);
-    }
-    const int lane = __lane_id();
-    const uint64_t m = __ballot(keep);
-    uint32_t wi = 0xffffffffu;
-    if (m != 0ull) {
-        const int leader = __ffsll((unsigned long long)m) - 1;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&ws.status[4], (uint32_t)__popcll(m));
-        base = __shfl(base, leader);
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t npairs = (uint32_t)d.nsrc * (uint32_t)d.nviews;
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    for (uint32_t vs = blockIdx.y; vs < npairs; vs += gridDim.y) {
+        const size_t gid = (size_t)vs * per + item;
+        bool keep = false;
+        if (item < per) {
+            const int v = (int)(vs / (uint32_t)d.nsrc);
+            const G& g = geoms[vs];
+            const int bx = (int)(item % d.bins_x), by = (int)(item / d.bins_x);
+            if (by * kBinQY < g.sh - 1 && bx * kBinQX < g.sw - 1) keep = !item_culled(g, bx, by, views[v].w, views[v].h);
+        }
+        const uint64_t m = __ballot(keep);
+        if (lane == 0) s_cnt[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tot = 0;
+            for (int w = 0; w < kCullThreads / 64; ++w) tot += s_cnt[w];
+            s_base = tot ? atomicAdd(&ws.status[4], tot) : 0u;
+        }
+        __syncthreads();
+        uint32_t base = s_base;
+        for (int w = 0; w < wave; ++w) base += s_cnt[w];
+        uint32_t wi = 0xffffffffu;
         if (keep) {
             const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
             const uint32_t pos = base + (uint32_t)__popcll(m & lt);
             ws.work[pos] = (uint32_t)gid;
             if (pos < d.range_cap) wi = pos;
         }
+        if (item < per) ws.imap[gid] = wi;
+        __syncthreads();   // s_cnt / s_base free for the next pair
     }
-    if (gid < total) ws.imap[gid] = wi;
 }
 
 // ------------------------------------------------------------------ binning
@@ -726,7 +742,10 @@ size_t scan_blocks(size_t nbins) { return (nbins + kScanPer - 1) / kScanPer; }
 template <class G>
 hipError_t launch_bin(const G* geoms, const ViewDesc* views, const BatchDims& d, const RasterWS& ws, hipStream_t s) {
     const size_t total = (size_t)d.bins_x * d.bins_y * d.nsrc * d.nviews;
-    hipLaunchKernelGGL(k_bin_cull<G>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, geoms, views, d, ws);
+    const uint32_t npairs = (uint32_t)d.nsrc * (uint32_t)d.nviews;
+    hipLaunchKernelGGL(k_bin_cull<G>,
+                       dim3((d.bins_x * d.bins_y + kCullThreads - 1) / kCullThreads, npairs < 65535u ? npairs : 65535u),
+                       dim3(kCullThreads), 0, s, geoms, views, d, ws);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t blocks = total < (size_t)kBinBlocks ? total : (size_t)kBinBlocks;
