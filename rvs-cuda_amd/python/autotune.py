@@ -36,10 +36,13 @@ def tune_entry_cap(sources, targets, headroom=1.25):
 
 
 def tune_serving(scenes, requests, memory_budget, entry_cap, p99_bound_ms=50.0, batches=(4, 8, 16, 32, 64),
-                 slots=(1, 2, 3), trial_batches=4, log=None):
+                 slots=(1, 2, 3), trial_batches=4, log=None, tie=0.02):
     """Closed-loop trials over (max_batch, slots); each trial serves
     trial_batches * max_batch * slots requests (cycling `requests`) at
-    concurrency max_batch * slots.  Returns (best, trials)."""
+    concurrency max_batch * slots.  Among the trials within the p99 bound,
+    those within `tie` (relative) of the best rate are equal within the short
+    trials' noise, and the one with the lowest p99 is chosen; with none within
+    the bound, the lowest p99.  Returns (best, trials)."""
     trials = []
     for b, s in itertools.product(batches, slots):
         n = trial_batches * b * s
@@ -58,7 +61,14 @@ def tune_serving(scenes, requests, memory_budget, entry_cap, p99_bound_ms=50.0, 
         trials.append(summ)
         if log:
             log(summ)
+    return pick_serving(trials, p99_bound_ms, tie), trials
+
+
+def pick_serving(trials, p99_bound_ms=50.0, tie=0.02):
+    """The chosen trial: lowest p99 among those within the bound whose rate is
+    within `tie` of the best such rate; the lowest p99 overall if none is."""
     ok = [t for t in trials if t["p99_ms"] <= p99_bound_ms]
-    pool = ok if ok else trials
-    best = max(pool, key=lambda t: t["views_s"]) if ok else min(pool, key=lambda t: t["p99_ms"])
-    return best, trials
+    if not ok:
+        return min(trials, key=lambda t: t["p99_ms"])
+    top = max(t["views_s"] for t in ok)
+    return min((t for t in ok if t["views_s"] >= (1.0 - tie) * top), key=lambda t: t["p99_ms"])

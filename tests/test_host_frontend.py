@@ -287,3 +287,16 @@ def test_host_functions_of_the_abi_match_the_oracle():
     assert np.array_equal(xyz[..., 0], (ct[:, None] * cp[None, :]).astype(np.float32))
     assert np.array_equal(xyz[..., 1], (ct[:, None] * sp[None, :]).astype(np.float32))
     assert np.array_equal(xyz[..., 2], np.broadcast_to(st[:, None], (32, 64)))
+
+
+def test_serving_autotune_choice():
+    """The serving autotuner's choice: within the p99 bound, rates within 2 %
+    of the best count as equal and the lowest p99 wins; beyond the bound only
+    if nothing meets it (then the lowest p99)."""
+    import autotune as A
+    t = lambda b, s, v, p: {"max_batch": b, "slots": s, "views_s": v, "p99_ms": p}
+    trials = [t(16, 2, 3071.0, 12.9), t(32, 2, 3101.0, 25.1), t(32, 3, 3110.0, 49.0), t(64, 2, 3300.0, 60.0)]
+    assert A.pick_serving(trials)["max_batch"] == 16            # 3071 >= 0.98 * 3110, lowest p99
+    assert A.pick_serving(trials, tie=0.0)["slots"] == 3        # strict: the best rate within the bound
+    assert A.pick_serving(trials, p99_bound_ms=20.0)["max_batch"] == 16
+    assert A.pick_serving([t(8, 1, 100.0, 80.0), t(4, 1, 90.0, 70.0)])["max_batch"] == 4
