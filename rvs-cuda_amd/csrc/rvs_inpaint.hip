@@ -226,17 +226,32 @@ struct Fn {
     int cv, cs, t, k;
 };
 
+// Selects rather than branches throughout (the solvers evaluate these per
+// pixel and line step; divergent branches would cost exec-mask juggling).
 __device__ inline Fn compose(const Fn& f1, const Fn& f2) {   // f2 after f1
     Fn r;
-    if (f1.cv + f2.k < f2.cv + f2.t) { r.cv = f1.cv + f2.k; r.cs = f1.cs; r.t = f1.t; }
-    else { r.cv = f2.cv; r.cs = f2.cs; r.t = f2.t; }
+    const bool a = f1.cv + f2.k < f2.cv + f2.t;
+    r.cv = a ? f1.cv + f2.k : f2.cv;
+    r.cs = a ? f1.cs : f2.cs;
+    r.t = a ? f1.t : f2.t;
     r.k = f1.k + f2.k;
     return r;
 }
 
 // lexicographic (value, rank) minimum, first argument wins ties of both
 __device__ inline void cand(int& bv, int& br, int& bs, int v, int r, int s) {
-    if (v < bv || (v == bv && r < br)) { bv = v; br = r; bs = s; }
+    const bool take = (v < bv) | ((v == bv) & (r < br));
+    bv = take ? v : bv;
+    br = take ? r : br;
+    bs = take ? s : bs;
+}
+
+// the same, offered only when `ok`
+__device__ inline void cand_if(bool ok, int& bv, int& br, int& bs, int v, int r, int s) {
+    const bool take = ok & ((v < bv) | ((v == bv) & (r < br)));
+    bv = take ? v : bv;
+    br = take ? r : br;
+    bs = take ? s : bs;
 }
 
 // Component region access: LDS copy of the bounding box (small components) or
@@ -383,7 +398,9 @@ __device__ void solve_component(const Region& R, const InpaintWS& ws, Fn* wave_t
 constexpr int kBigPx = 4;   // consecutive pixels per thread in a line step of a large component
 
 __device__ inline void apply_fn(const Fn& f, int& v, int& s) {
-    if (v + f.k < f.cv + f.t) { v = v + f.k; } else { v = f.cv; s = f.cs; }
+    const bool a = v + f.k < f.cv + f.t;
+    s = a ? s : f.cs;
+    v = a ? v + f.k : f.cv;
 }
 
 // Map value of view pixel (x, y) for component R from its three words, loaded
@@ -525,16 +542,17 @@ __device__ void solve_big(const Region& R, const InpaintWS& ws, Fn* wave_tot, in
     }
 }
 
-constexpr int kBigThreads = 1024;                // block of k_inp_big
-constexpr int kLineMax = kBigThreads * kBigPx;   // longest line of solve_lines (one block step)
+constexpr int kBigThreads = 1024;                 // block of k_inp_big
+constexpr int kLinePx = 4;                        // consecutive pixels per thread in its line steps
+constexpr int kLineMax = kBigThreads * kLinePx;   // longest line of solve_lines (one block step)
 
 // Large components whose lines fit one block step.  Each thread owns the same
-// kBigPx positions on every line, so the previous line's values from this
+// PX positions on every line, so the previous line's values from this
 // sweep stay in registers; a line step loads only its own pixels, the next
 // line and one along-line neighbour (all independent loads) and takes one
 // barrier.  cols: lines are columns.  Same sweeps, candidate order and line
 // order semantics as solve_component.
-template <int NT>
+template <int NT, int PX>
 __device__ void solve_lines(const Region& R, const InpaintWS& ws, bool cols, Fn* wave_tot) {
     constexpr int NW = NT / 64;   // waves per block
     const int tid = threadIdx.x;
@@ -568,16 +586,16 @@ __device__ void solve_lines(const Region& R, const InpaintWS& ws, bool cols, Fn*
         // neighbour after them and the pixel before the line's first one
         auto load_line = [&](int lc, int* v, int* sr, bool* in) {
 #pragma unroll
-            for (int k = 0; k < kBigPx; ++k) {
+            for (int k = 0; k < PX; ++k) {
                 int x, y;
-                at(lc, tid * kBigPx + k, x, y);
+                at(lc, tid * PX + k, x, y);
                 in[k] = inside(x, y);
                 load(x, y, v[k], sr[k]);
             }
         };
         auto load_ends = [&](int lc, int& av, int& as_, bool& ain, int& Lv, int& Ls) {
             int x, y;
-            at(lc, tid * kBigPx + kBigPx, x, y);
+            at(lc, tid * PX + PX, x, y);
             ain = inside(x, y);
             load(x, y, av, as_);
             at(lc, -1, x, y);
@@ -589,8 +607,8 @@ __device__ void solve_lines(const Region& R, const InpaintWS& ws, bool cols, Fn*
         };
         // the line before the first one (outside the box, constant), the first
         // line and the line after it; later lines are loaded one step ahead
-        int pv[kBigPx], ps[kBigPx], sv[kBigPx], ss[kBigPx], nv[kBigPx], ns[kBigPx];
-        bool pin[kBigPx], sin[kBigPx], nin[kBigPx];
+        int pv[PX], ps[PX], sv[PX], ss[PX], nv[PX], ns[PX];
+        bool pin[PX], sin[PX], nin[PX];
         int av, as_, Lv, Ls;
         bool ain;
         load_line(lcoord(-1), pv, ps, pin);
@@ -600,33 +618,35 @@ __device__ void solve_lines(const Region& R, const InpaintWS& ws, bool cols, Fn*
         for (int li = 0; li < nlines; ++li) {
             const int lc = lcoord(li);
             // prefetch for the next step: the line after its next one and its ends
-            int fv[kBigPx], fs[kBigPx], fav, fas, fLv, fLs;
-            bool fin[kBigPx], fain;
+            int fv[PX], fs[PX], fav, fas, fLv, fLs;
+            bool fin[PX], fain;
             load_line(lcoord(li + 2), fv, fs, fin);
             load_ends(lcoord(li + 1), fav, fas, fain, fLv, fLs);
-            Fn f[kBigPx];
+            Fn f[PX];
 #pragma unroll
-            for (int k = 0; k < kBigPx; ++k) {
-                const int e = tid * kBigPx + k;
-                f[k].k = 1; f[k].cv = kAbsent; f[k].cs = -1; f[k].t = 0;
-                if (e >= llen) continue;
+            for (int k = 0; k < PX; ++k) {
+                // past the line: (kAbsent, -1, 0); known pixel: (0, itself); a
+                // hole of another component: (kAbsent, -1, 0); a member: its
+                // best candidate
+                const bool valid = tid * PX + k < llen;
                 const int v = sv[k];
-                if (v == 0) {
-                    f[k].cv = 0; f[k].cs = ss[k];
-                } else if (v != kOther) {
-                    int bv = v, br = 0, bs = ss[k];
-                    const int alv = k + 1 < kBigPx ? sv[k + 1] : av, als = k + 1 < kBigPx ? ss[k + 1] : as_;
-                    const bool alin = k + 1 < kBigPx ? sin[k + 1] : ain;
-                    // cand keeps the (value, rank) minimum, so the calls' order is free
-                    if (pin[k] && pv[k] != kOther) cand(bv, br, bs, pv[k] + 1, r_prev, ps[k]);
-                    if (nin[k] && nv[k] != kOther) cand(bv, br, bs, nv[k] + 1, r_next, ns[k]);
-                    if (alin && alv != kOther) cand(bv, br, bs, alv + 1, r_along, als);
-                    f[k].cv = bv; f[k].cs = bs; f[k].t = (rs < br) ? 1 : 0;
-                }
+                const bool known = valid & (v == 0);
+                const bool member = valid & (v != 0) & (v != kOther);
+                int bv = v, br = 0, bs = ss[k];
+                const int alv = k + 1 < PX ? sv[k + 1] : av, als = k + 1 < PX ? ss[k + 1] : as_;
+                const bool alin = k + 1 < PX ? sin[k + 1] : ain;
+                // cand keeps the (value, rank) minimum, so the calls' order is free
+                cand_if(pin[k] & (pv[k] != kOther), bv, br, bs, pv[k] + 1, r_prev, ps[k]);
+                cand_if(nin[k] & (nv[k] != kOther), bv, br, bs, nv[k] + 1, r_next, ns[k]);
+                cand_if(alin & (alv != kOther), bv, br, bs, alv + 1, r_along, als);
+                f[k].k = 1;
+                f[k].cv = known ? 0 : (member ? bv : kAbsent);
+                f[k].cs = known ? ss[k] : (member ? bs : -1);
+                f[k].t = (member & (rs < br)) ? 1 : 0;
             }
             Fn A = f[0];
 #pragma unroll
-            for (int k = 1; k < kBigPx; ++k) A = compose(A, f[k]);
+            for (int k = 1; k < PX; ++k) A = compose(A, f[k]);
             for (int off = 1; off < 64; off <<= 1) {
                 Fn g;
                 g.cv = __shfl_up(A.cv, off);
@@ -656,9 +676,9 @@ __device__ void solve_lines(const Region& R, const InpaintWS& ws, bool cols, Fn*
             int Sv = Lv, Ss = Ls;
             if (have) apply_fn(pre, Sv, Ss);
 #pragma unroll
-            for (int k = 0; k < kBigPx; ++k) {
+            for (int k = 0; k < PX; ++k) {
                 apply_fn(f[k], Sv, Ss);
-                const int e = tid * kBigPx + k;
+                const int e = tid * PX + k;
                 const bool member = e < llen && sv[k] != 0 && sv[k] != kOther;
                 if (member && Sv != sv[k]) {
                     int x, y;
@@ -673,7 +693,7 @@ __device__ void solve_lines(const Region& R, const InpaintWS& ws, bool cols, Fn*
                 pin[k] = sin[k];
             }
 #pragma unroll
-            for (int k = 0; k < kBigPx; ++k) {
+            for (int k = 0; k < PX; ++k) {
                 sv[k] = nv[k]; ss[k] = ns[k]; sin[k] = nin[k];
                 nv[k] = fv[k]; ns[k] = fs[k]; nin[k] = fin[k];
             }
@@ -780,7 +800,7 @@ __global__ void __launch_bounds__(kBigThreads) k_inp_big(const ViewDesc* __restr
         R.ldist = nullptr; R.lsrc = nullptr;
         // fewer lines when both orientations fit
         const bool cols = bh <= kLineMax && (bw > kLineMax || bh > bw);
-        solve_lines<kBigThreads>(R, ws, cols, wave_tot);
+        solve_lines<kBigThreads, kLinePx>(R, ws, cols, wave_tot);
         __syncthreads();
     }
 }
