@@ -256,6 +256,8 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
         const rvs_target& t = tgts[v];
         if (!cam_ok(&t.cam)) return fail(RVS_E_INVALID, "synthesize: view %d has a bad camera", v);
         if (t.rescale != 1.0f) return fail(RVS_E_UNSUPPORTED, "synthesize: Precision != 1 is not supported");
+        if (t.cam.w > rvs_k::kMaxSourceSide || t.cam.h > rvs_k::kMaxSourceSide)
+            return fail(RVS_E_UNSUPPORTED, "synthesize: view %d: side above %d", v, rvs_k::kMaxSourceSide);
         P.max_w = t.cam.w > P.max_w ? t.cam.w : P.max_w;
         P.max_h = t.cam.h > P.max_h ? t.cam.h : P.max_h;
         P.any_persp = P.any_persp || t.cam.proj == 0;

@@ -28,7 +28,8 @@ constexpr int kBinBlocks = 2048;    // persistent binning blocks (work items pas
 constexpr int kRasterGroup = 4;     // sources rasterised together per k_raster phase
 constexpr int kRasterCache = 1024;  // binned quads per phase whose coverage-pass results stay in LDS
 constexpr int kRasterCacheOne = 640; // the same for launch sets of up to kRasterGroup sources (one phase)
-constexpr int kMaxSourceSide = 65536;  // bins hold quads as (row << 16) | column
+constexpr int kMaxSourceSide = 65536;  // bins hold quads as (row << 16) | column; also the
+                                       // largest target side (tile coordinates fit 16 bits)
 constexpr int kFallbackBlocks = 1024;
 
 // One target view of a batch.

@@ -18,6 +18,7 @@ import pytest
 
 import oracle_bind as ob
 import rvs_amd as R
+import workloads as W
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden", "reference_config")
@@ -287,6 +288,27 @@ def test_host_functions_of_the_abi_match_the_oracle():
     assert np.array_equal(xyz[..., 0], (ct[:, None] * cp[None, :]).astype(np.float32))
     assert np.array_equal(xyz[..., 1], (ct[:, None] * sp[None, :]).astype(np.float32))
     assert np.array_equal(xyz[..., 2], np.broadcast_to(st[:, None], (32, 64)))
+
+
+def test_workspace_planning_limits():
+    """Workspace planning is host code (no GPU): a valid batch has a size, and
+    views with a side above 65536 (tile coordinates are 16-bit) or batches
+    beyond 2^31 pixels are refused with a size of 0 and a message."""
+    rg = W.rig("erp4_1080p")
+    srcs = []
+    for s in rg["sources"]:
+        st = R.RvsSource()
+        st.cam = s["cam"]
+        srcs.append(st)
+    arr = (R.RvsSource * len(srcs))(*srcs)
+    ok = W.batch_targets(rg, 2, seed=4, sizes=((1920, 1080),))
+    assert R.hip().rvs_synth_batch_workspace_size(arr, len(srcs), (R.RvsTarget * 2)(*ok), 2, 0) > 0
+    wide = R.make_target(R.cam(0, 70000, 64), [0.0] * 3, [0.0] * 3, 10, 1.0, R.scale_of(10), R.scale_of(16))
+    assert R.hip().rvs_synth_batch_workspace_size(arr, len(srcs), (R.RvsTarget * 1)(wide), 1, 0) == 0
+    assert b"side above" in R.hip().rvs_last_error()
+    big = R.make_target(R.cam(0, 65536, 40000), [0.0] * 3, [0.0] * 3, 10, 1.0, R.scale_of(10), R.scale_of(16))
+    assert R.hip().rvs_synth_batch_workspace_size(arr, len(srcs), (R.RvsTarget * 1)(big), 1, 0) == 0
+    assert b"too large" in R.hip().rvs_last_error()
 
 
 def test_serving_autotune_choice():
