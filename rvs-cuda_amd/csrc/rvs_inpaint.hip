@@ -68,14 +68,6 @@ __device__ inline void unite(int* label, int a, int b) {
     if (p >= vd.w * vd.h) return;                                           \
     const int gp = (int)vd.px_base + p;
 
-__global__ void k_inp_init(const ViewDesc* __restrict__ views, InpaintWS ws) {
-    RVS_VIEW_PIXEL(vd, p, gp)
-    const bool hole = !(vd.quality[p] > 0.f);   // View::get_inpaint_mask (View.cpp:198-203)
-    ws.label[gp] = hole ? gp : -1;
-    ws.dist[gp] = hole ? (vd.w + vd.h) : 0;
-    ws.src[gp] = gp;
-}
-
 constexpr int kCclTile = 32;
 
 __device__ inline int lfind(const int* lab, int p) {
@@ -98,8 +90,11 @@ __device__ inline void lunite(int* lab, int a, int b) {
     }
 }
 
-// Labels inside one 32x32 tile in LDS; each hole pixel gets the global index of
-// its tile-local root (the smallest local index of its local component).
+// Inpainting state of one 32x32 tile and its labels in LDS: every pixel gets
+// the map's initial (source, distance) -- (itself, 0) when known, (itself,
+// w + h) for a hole (inpaint_color, inpainting.cpp:185-191) -- known pixels
+// the label -1, and each hole pixel the global index of its tile-local root
+// (the smallest local index of its local component).
 __global__ void __launch_bounds__(256) k_ccl_local(const ViewDesc* __restrict__ views, InpaintWS ws) {
     __shared__ int lab[kCclTile * kCclTile];
     const ViewDesc& vd = views[blockIdx.z];
@@ -109,7 +104,15 @@ __global__ void __launch_bounds__(256) k_ccl_local(const ViewDesc* __restrict__ 
     const int tid = threadIdx.x;
     for (int k = tid; k < kCclTile * kCclTile; k += 256) {
         const int x = tx0 + (k % kCclTile), y = ty0 + (k / kCclTile);
-        lab[k] = (x < w && y < h && ws.label[base + y * w + x] >= 0) ? k : -1;
+        bool hole = false;
+        if (x < w && y < h) {
+            const int p = y * w + x, gp = base + p;
+            hole = !(vd.quality[p] > 0.f);   // View::get_inpaint_mask (View.cpp:198-203)
+            ws.dist[gp] = hole ? (w + h) : 0;
+            ws.src[gp] = gp;
+            if (!hole) ws.label[gp] = -1;
+        }
+        lab[k] = hole ? k : -1;
     }
     __syncthreads();
     for (int k = tid; k < kCclTile * kCclTile; k += 256) {
@@ -847,7 +850,6 @@ hipError_t launch_inpaint(const ViewDesc* views, int nviews, int max_w, int max_
     const dim3 b(256), g((unsigned)(((size_t)max_w * max_h + 255) / 256), (unsigned)nviews);
     hipError_t e = launch_reset_u32((uint32_t*)ws.counters, 3, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_inp_init, g, b, 0, s, views, ws);
     hipLaunchKernelGGL(k_ccl_local,
                        dim3((max_w + kCclTile - 1) / kCclTile, (max_h + kCclTile - 1) / kCclTile, nviews), b, 0, s,
                        views, ws);
