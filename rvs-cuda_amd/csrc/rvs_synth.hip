@@ -1078,16 +1078,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI 
     const int t3 = wave_base + lane_now();
     const int px = tx0 + (t3 % kTile), py = ty0 + (t3 / kTile);
     const bool inside = px < vd.w && py < vd.h;
-    if (inside) {
-        const size_t o = (size_t)py * vd.w + px;
-        ws.deferred[vd.px_base + o] = deferred ? 1 : 0;
-        if (!deferred) write_pixel<MODE>(vd, o, st, rc, rd, rs);
-    }
+    const size_t o = (size_t)py * vd.w + px;
+    if (inside && !deferred) write_pixel<MODE>(vd, o, st, rc, rd, rs);
     if (deferred) s_deferred = 1;
     __syncthreads();
-    if (s_deferred && t3 == 0) {
-        const uint32_t k = atomicAdd(&ws.status[0], 1u);
-        ws.deferred_tiles[k] = T;
+    // the replay reads the per-pixel flags of the listed tiles only
+    if (s_deferred) {
+        if (inside) ws.deferred[vd.px_base + o] = deferred ? 1 : 0;
+        if (t3 == 0) {
+            const uint32_t k = atomicAdd(&ws.status[0], 1u);
+            ws.deferred_tiles[k] = T;
+        }
     }
 }
 
