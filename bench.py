@@ -130,6 +130,9 @@ def main():
     ap.add_argument("--c5-requests", type=int, default=64)
     ap.add_argument("--roofline-only", type=int, default=0,
                     help="only run the roofline launch set (this many views) 5 times (counter passes)")
+    ap.add_argument("--launch-set", default="",
+                    help="only run one launch set 5 times: RIG,WxH[e],N -- N serving poses of rig RIG at that size "
+                         "(suffix e: equirectangular target), e.g. persp11_2048,3840x2160,4 (profiling)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes 5 times (counter calibration)")
     args = ap.parse_args()
@@ -152,6 +155,22 @@ def main():
     import rvs_amd as R
     import serving as S
     import workloads as W
+
+    if args.launch_set:
+        name, size, n = args.launch_set.split(",")
+        erp = size.endswith("e")
+        w, h = (int(x) for x in size.rstrip("e").split("x"))
+        rg_ls = W.rig(name)
+        srcs_ls = W.device_sources(rg_ls, W.render_sources(rg_ls))
+        tg = [t for _, t in W.serve_requests([rg_ls], int(n), seed=4, sizes=((w, h, 1 if erp else 0),))]
+        bs = R.BatchSynthesizer(srcs_ls, tg)
+        for _ in range(5):
+            bs.run()
+        torch.cuda.synchronize()
+        st, cnt = stage_profile(bs.run, 3, R)
+        print(json.dumps({"launch_set": args.launch_set, "alg_bytes": b_alg(rg_ls, tg), "stage_ms": st,
+                          "counters": cnt}), flush=True)
+        return
 
     rg = W.rig(args.workload)
     t0 = time.time()
