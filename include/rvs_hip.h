@@ -231,6 +231,17 @@ int rvs_synthesize_batch(const rvs_source* srcs, int n_src, const rvs_target* tg
 int rvs_synthesize_batch_async(const rvs_source* srcs, int n_src, const rvs_target* tgts,
                                int n_views, void* workspace, size_t ws_bytes, uint32_t entry_cap,
                                const rvs_view_out* outs, rvs_stream_t stream);
+/* The same launch set in two stream-ordered parts: binning, rasterisation and
+ * the in-order replay on `stream`, then (after an event recorded there)
+ * inpainting and encoding on `finish_stream`.  No counterpart in the
+ * reference (its stages run back to back, Pipeline.cpp:169-301): for request
+ * loops whose batches use separate workspaces, so that one batch's inpainting
+ * and encoding overlap the next batch's binning and rasterisation on
+ * `stream`.  Check overflow with rvs_synth_check on `finish_stream`. */
+int rvs_synthesize_batch_async_split(const rvs_source* srcs, int n_src, const rvs_target* tgts,
+                                     int n_views, void* workspace, size_t ws_bytes, uint32_t entry_cap,
+                                     const rvs_view_out* outs, rvs_stream_t stream,
+                                     rvs_stream_t finish_stream);
 int rvs_synth_check(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views,
                     const void* workspace, uint32_t entry_cap, uint32_t* needed_cap,
                     rvs_stream_t stream);

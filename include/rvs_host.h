@@ -50,7 +50,8 @@ int rvs_app_run(const char* config_path, const char* source_path, int stagewise)
  * Each batch is one rvs_synthesize_batch_async launch set on one of
  * `compute_streams` streams shared by the `slots` in flight (in order on one
  * stream; overlapping when there are several), and its YUV planes are copied
- * back on the slot's own stream, overlapping later batches.  Latency of a
+ * back on the slot's own stream, overlapping later batches (with
+ * `split_finish`, the batch's inpainting and encoding run there too).  Latency of a
  * request = arrival to the completion of its YUV copy to host memory. */
 typedef struct {
     size_t memory_budget;       /* device bytes for in-flight batches (workspaces + outputs), split over slots */
@@ -61,6 +62,8 @@ typedef struct {
     int keep_outputs;           /* keep a host copy per request (rvs_server_output) */
     size_t host_staging_bytes;  /* pinned staging per slot (0 = a quarter of the slot's share, at most 1 GiB) */
     int compute_streams;        /* streams the slots' launch sets run on (0 or 1: one, in order) */
+    int split_finish;           /* 1: inpainting + encoding of a batch on its slot's stream, overlapping the
+                                   next batch's binning and raster (rvs_synthesize_batch_async_split) */
 } rvs_serve_opts;
 
 typedef struct {

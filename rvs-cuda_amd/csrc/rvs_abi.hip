@@ -317,9 +317,11 @@ void fill_pair(PairGeom& g, const rvs_source& S, const rvs_target& T) {
 }
 
 // Enqueues the fused launch set; returns after the last launch (no sync).
+// With a finish stream sf other than s, inpainting and encoding go to sf
+// after an event recorded on s behind the in-order replay.
 int enqueue_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views, void* workspace,
                   size_t ws_bytes, uint32_t cap, const rvs_view_out* outs, hipStream_t s, SynthPlan& P,
-                  rvs_k::RasterWS& ws) {
+                  rvs_k::RasterWS& ws, hipStream_t sf = nullptr) {
     std::vector<rvs_k::ViewDesc> views;
     const int prc = plan_synth(srcs, n_src, tgts, n_views, cap, P, &views);
     if (prc != RVS_OK) return prc;
@@ -388,6 +390,15 @@ int enqueue_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int
     prof_mark(2, s);
     HIP_TRY((rvs_k::launch_replay<PairGeom, rvs_k::MODE_BLEND>(dg, dviews, d, ws, s)), "replay");
     prof_mark(3, s);
+    if (sf && sf != s) {
+        // a wait captures the event as recorded at that moment, so one event
+        // per host thread serves every call
+        static thread_local hipEvent_t split_ev = nullptr;
+        if (!split_ev) HIP_TRY(hipEventCreateWithFlags(&split_ev, hipEventDisableTiming), "split event");
+        HIP_TRY(hipEventRecord(split_ev, s), "split record");
+        HIP_TRY(hipStreamWaitEvent(sf, split_ev, 0), "split wait");
+        s = sf;
+    }
     const rvs_k::InpaintWS iws = rvs_k::inpaint_ws_carve(base + P.inpaint, d.npx, n_views);
     HIP_TRY(rvs_k::launch_inpaint(dviews, n_views, P.max_w, P.max_h, iws, nullptr, s), "inpaint");
     prof_mark(4, s);
@@ -633,6 +644,16 @@ int rvs_synthesize_batch_async(const rvs_source* srcs, int n_src, const rvs_targ
                                  (hipStream_t)stream, P, ws);
     if (rc == RVS_OK && g_prof) { g_prof_valid = true; g_prof_ws = ws; g_prof_dims = P.R.d; }
     return rc;
+}
+
+int rvs_synthesize_batch_async_split(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views,
+                                     void* workspace, size_t ws_bytes, uint32_t entry_cap,
+                                     const rvs_view_out* outs, rvs_stream_t stream, rvs_stream_t finish_stream) {
+    SynthPlan P;
+    rvs_k::RasterWS ws;
+    g_prof_valid = false;
+    return enqueue_synth(srcs, n_src, tgts, n_views, workspace, ws_bytes, entry_cap, outs, (hipStream_t)stream, P,
+                         ws, (hipStream_t)finish_stream);
 }
 
 int rvs_synth_check(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views, const void* workspace,

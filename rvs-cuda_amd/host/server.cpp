@@ -264,10 +264,18 @@ bool Server::launch(Slot& s) {
     const double t_launch = now_ms();
     s.cap = batch_cap(s.per_view, tgts.size());
     hipStream_t cs = (hipStream_t)m_compute[s.index % m_compute.size()];
-    int rc = rvs_synthesize_batch_async(S.data(), (int)S.size(), tgts.data(), (int)tgts.size(), s.arena + s.ws_off,
+    int rc;
+    if (m_opt.split_finish) {
+        // binning, raster and replay on the compute stream; inpainting and
+        // encoding on the slot's stream, overlapping the next batch
+        rc = rvs_synthesize_batch_async_split(S.data(), (int)S.size(), tgts.data(), (int)tgts.size(),
+                                              s.arena + s.ws_off, ws, s.cap, s.outs.data(), cs, s.stream);
+    } else {
+        rc = rvs_synthesize_batch_async(S.data(), (int)S.size(), tgts.data(), (int)tgts.size(), s.arena + s.ws_off,
                                         ws, s.cap, s.outs.data(), cs);
-    if (rc == RVS_OK && hipEventRecord(s.computed, cs) != hipSuccess) rc = RVS_E_HIP;
-    if (rc == RVS_OK && hipStreamWaitEvent(s.stream, s.computed, 0) != hipSuccess) rc = RVS_E_HIP;
+        if (rc == RVS_OK && hipEventRecord(s.computed, cs) != hipSuccess) rc = RVS_E_HIP;
+        if (rc == RVS_OK && hipStreamWaitEvent(s.stream, s.computed, 0) != hipSuccess) rc = RVS_E_HIP;
+    }
     if (rc == RVS_OK && s.pinned &&
         hipMemcpyAsync(s.pinned, s.arena, s.yuv_bytes, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
         rc = RVS_E_HIP;

@@ -12,7 +12,7 @@ import rvs_amd as R
 class RvsServeOpts(C.Structure):
     _fields_ = [("memory_budget", C.c_size_t), ("max_batch", C.c_int), ("slots", C.c_int), ("entry_cap", C.c_uint32),
                 ("copy_to_host", C.c_int), ("keep_outputs", C.c_int), ("host_staging_bytes", C.c_size_t),
-                ("compute_streams", C.c_int)]
+                ("compute_streams", C.c_int), ("split_finish", C.c_int)]
 
 
 class RvsRequest(C.Structure):
@@ -54,14 +54,16 @@ class Server:
     """scenes: list of lists of rvs_amd.DeviceSource (kept alive by the server)."""
 
     def __init__(self, scenes, memory_budget, max_batch=64, slots=2, entry_cap=0, copy_to_host=True,
-                 keep_outputs=False, host_staging_bytes=0, compute_streams=1):
+                 keep_outputs=False, host_staging_bytes=0, compute_streams=1,
+                 split_finish=False):
         self._scenes = scenes
         self._arrs = [(R.RvsSource * len(sc))(*[s.struct() for s in sc]) for sc in scenes]
         n = len(scenes)
         ptrs = (C.POINTER(R.RvsSource) * n)(*[C.cast(a, C.POINTER(R.RvsSource)) for a in self._arrs])
         counts = (C.c_int * n)(*[len(sc) for sc in scenes])
         self.opts = RvsServeOpts(int(memory_budget), int(max_batch), int(slots), int(entry_cap), int(copy_to_host),
-                                 int(keep_outputs), int(host_staging_bytes), int(compute_streams))
+                                 int(keep_outputs), int(host_staging_bytes), int(compute_streams),
+                                 int(split_finish))
         self.h = _lib().rvs_server_create(ptrs, counts, n, C.byref(self.opts))
         if not self.h:
             raise RuntimeError(R.host().rvs_host_last_error().decode())

@@ -46,11 +46,13 @@ def _requests(rg_a, rg_b, n):
     return [(0, ta[k]) if k % 3 else (1, tb[k]) for k in range(n)]
 
 
-def test_served_results_equal_alone(torch_cuda):
+@pytest.mark.parametrize("split", [False, True])
+def test_served_results_equal_alone(torch_cuda, split):
     say("case serve open loop, 2 scenes, 18 requests")
     rg_a, src_a, rg_b, src_b = _scenes()
     reqs = _requests(rg_a, rg_b, 18)
-    srv = S.Server([src_a, src_b], memory_budget=6 << 30, max_batch=8, slots=2, keep_outputs=True)
+    srv = S.Server([src_a, src_b], memory_budget=6 << 30, max_batch=8, slots=2, keep_outputs=True,
+                   split_finish=split)
     try:
         st = srv.run_open(reqs, rate=2000.0, seed=4)
         say("  summary", S.summarize(st))
@@ -81,7 +83,8 @@ def test_budget_rejects_request_that_cannot_fit(torch_cuda):
         srv.close()
 
 
-def test_served_erp11_scenes_equal_alone(torch_cuda):
+@pytest.mark.parametrize("split", [False, True])
+def test_served_erp11_scenes_equal_alone(torch_cuda, split):
     say("case serve 2 scenes x 11 ERP sources, ERP + perspective targets, closed loop")
     scale = 16
     rigs = [W.rig("erp11_4096:%d" % seed, scale=scale) for seed in (5, 6)]
@@ -98,7 +101,7 @@ def test_served_erp11_scenes_equal_alone(torch_cuda):
         s0 = rg["sources"][0]
         reqs.append((sc, R.make_target(cam, pos, rot, 10, 1.0, R.scale_of(s0["bd_color"]),
                                        R.scale_of(s0["bd_depth"]))))
-    srv = S.Server(srcs, memory_budget=8 << 30, max_batch=4, slots=2, keep_outputs=True)
+    srv = S.Server(srcs, memory_budget=8 << 30, max_batch=4, slots=2, keep_outputs=True, split_finish=split)
     try:
         st = srv.run_closed(reqs, concurrency=len(reqs))
         say("  summary", S.summarize(st))
