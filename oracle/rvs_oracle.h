@@ -17,8 +17,16 @@
  *   orc_decode_color / orc_decode_depth     pinned: closed forms (exact bilinear
  *                                           weights, near/far end points)
  *   orc_transform_triangles                 pinned: brute-force tiny meshes, identity
- *                                           warp, analytic homography of a plane
- *   orc_blend_step                          pinned: identity / mean / empty colour cases
+ *                                           warp, analytic homography of a plane;
+ *                                           seam triangles (wrap) and the prolongation
+ *                                           branch against a brute force
+ *   orc_blend_step                          pinned: identity / mean / empty colour cases,
+ *                                           exact quality ties, power-sum quality;
+ *                                           x * fl(1/S) (OpenCV Vec/float) unpinned
+ *   orc_synthesize                          pinned: exact identity warp, holes filled from
+ *                                           the L1-nearest covered pixel with the
+ *                                           quality mask, first view's scales, input
+ *                                           order symmetry except the depth rule
  *   orc_inpaint                             pinned: brute-force L1 distance + literal
  *                                           sweep replay on tiny grids
  *   orc_encode_yuv420                       Y pinned (closed form); U,V chroma rule
