@@ -11,9 +11,12 @@ Prints one JSON line:
   value         served views/s: the native request loop (continuous batching,
                 memory-budget admission, batches in flight on several streams,
                 YUV copied back to host memory per request) in a closed loop at
-                the autotuned batch size / slot count; `steps` requests timed
-                after `warmup` requests.  p50/p99: per-request latency
-                (arrival to host copy complete).
+                the autotuned batch size B / slot count; a step is one batch
+                round, so `steps` x B requests are timed after `warmup` x B
+                requests (the timed region holds `steps` batches whatever B
+                is).  p50/p99: per-request latency (arrival to host copy
+                complete).  `vs_plain` = value / plain views/s, beside the
+                device time per view of both.
   plain         the plain per-view path: one view per launch set, synchronous
                 (host call to completion), sources resident.
   plain_stagewise  the same view through the per-stage entry points, in the
@@ -65,6 +68,20 @@ def b_alg(rg, targets):
     return int(tot)
 
 
+def served_alg_bytes(rigs, mix, stats):
+    """Algorithmic bytes moved by a served run: per batch, every source sample
+    of its scene once plus every output sample of its views once."""
+    tot = 0
+    seen = {}
+    for k, b in enumerate(stats["batch"]):
+        scene, t = mix[k]
+        if b not in seen:
+            seen[b] = scene
+            tot += b_alg(rigs[scene], [])
+        tot += b_alg(dict(sources=[]), [t])
+    return tot
+
+
 def log(*a):
     print("#", *a, file=sys.stderr, flush=True)
 
@@ -111,8 +128,9 @@ def stage_profile(run, reps, R):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=512, help="served requests timed")
-    ap.add_argument("--warmup", type=int, default=64, help="served requests before the timed run")
+    ap.add_argument("--steps", type=int, default=32,
+                    help="served batch rounds timed (steps x tuned batch size requests)")
+    ap.add_argument("--warmup", type=int, default=4, help="served batch rounds before the timed run")
     ap.add_argument("--plain-steps", type=int, default=30)
     ap.add_argument("--stagewise-steps", type=int, default=10)
     ap.add_argument("--workload", default="erp4_1080p")
@@ -219,7 +237,7 @@ def main():
     log("plain stagewise %.1f views/s" % stagewise_views_s)
 
     # --------------------------------------------------------------- autotuning
-    walk = W.batch_targets(rg, max(args.steps, 256), seed=4, sizes=((tw, th),))
+    walk = W.batch_targets(rg, 256, seed=4, sizes=((tw, th),))
     cap, cap_counters = A.tune_entry_cap(srcs, walk[:16])
     log("entries per view %d (sample: %d entries, largest bin %d)" % (cap, cap_counters["bin_entries"],
                                                                      cap_counters["max_bin"]))
@@ -234,15 +252,17 @@ def main():
 
     # ------------------------------------------------------------------ serving
     staging = B * A.yuv_bytes(walk[0])
+    n_timed = max(1, args.steps) * B
     srv = S.Server([srcs], memory_budget=budget, max_batch=B, slots=SL, entry_cap=cap, host_staging_bytes=staging)
     try:
-        srv.run_closed(reqs[: args.warmup], concurrency=B * SL)
+        srv.run_closed([reqs[k % len(reqs)] for k in range(max(1, args.warmup) * B)], concurrency=B * SL)
         t_a = time.perf_counter()
-        st = srv.run_closed([reqs[k % len(reqs)] for k in range(args.steps)], concurrency=B * SL)
+        st = srv.run_closed([reqs[k % len(reqs)] for k in range(n_timed)], concurrency=B * SL)
         served_s = time.perf_counter() - t_a
         served = S.summarize(st, wall_s=served_s)
+        served["timed_requests"] = n_timed
         rate = 0.8 * served["views_s"]
-        st_open = srv.run_open([reqs[k % len(reqs)] for k in range(args.steps)], rate=rate, seed=4)
+        st_open = srv.run_open([reqs[k % len(reqs)] for k in range(n_timed)], rate=rate, seed=4)
         open_loop = S.summarize(st_open)
         open_loop["offered_rate"] = round(rate, 1)
     finally:
@@ -387,8 +407,12 @@ def main():
                 srv.run_closed(mix5[:4 * b5], concurrency=2 * b5)
                 a = time.perf_counter()
                 st5 = srv.run_closed(mix5, concurrency=2 * b5)
-                closed = S.summarize(st5, wall_s=time.perf_counter() - a)
+                wall5 = time.perf_counter() - a
+                closed = S.summarize(st5, wall_s=wall5)
                 closed["max_batch"] = b5
+                ab5 = served_alg_bytes(rigs5, mix5, st5)
+                closed["alg_GBs"] = round(ab5 / wall5 / 1e9, 1)
+                closed["alg_frac_of_hbm_peak"] = round(ab5 / wall5 / 1e9 / HBM_PEAK_GBS, 4)
                 trials5.append(closed)
                 log("c5 trial %s" % closed)
             finally:
@@ -396,6 +420,8 @@ def main():
         c5 = {"scenes": "%d x 11 ERP 4096x2048, resident" % len(rigs5),
               "targets": "8192x4096 ERP, 3840x2160 perspective", "requests": len(mix5),
               "resident_source_bytes": int(sum(b_alg(r5, []) for r5 in rigs5)),
+              "hbm_note": "alg_GBs = algorithmic bytes (per batch: its scene's 11 sources once + its outputs once) / "
+                          "wall time, against the 8 TB/s HBM peak",
               "closed_loop": max(trials5, key=lambda t: t["views_s"]), "batch_trials": trials5}
         del srcs5
         torch.cuda.empty_cache()
@@ -408,20 +434,27 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * served_s / args.steps, 4),
+        "ms_per_step": round(1e3 * served_s / max(1, args.steps), 4),
         "p50_ms": served["p50_ms"],
         "p99_ms": served["p99_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "vs_plain": round(served["views_s"] / plain_views_s, 2),
+        "device_ms_per_view": {
+            "plain": round(sum(plain_stage.values()), 4),
+            "batched_%d" % ROOFLINE_VIEWS: round(sum(bstage.values()) / ROOFLINE_VIEWS, 4),
+            "what": "sum of the per-stage device times (HIP events) of one launch set, per view: plain = one "
+                    "view per launch set; batched = the %d-view roofline launch set" % ROOFLINE_VIEWS,
+        },
         "vs_plain_stagewise": round(served["views_s"] / stagewise_views_s, 2),
         "dtype": "f32",
         "data": "synthetic",
         "config": {
             "workload": "%s: 4x ERP 4096x2048 10-bit + 16-bit depth (resident) -> 1920x1080 perspective views at "
                         "random-walk poses, blending 1.0, YUV420 10-bit out" % args.workload,
-            "mode": "served: closed loop, concurrency = batch x slots, continuous batching",
+            "mode": "served: closed loop, concurrency = batch x slots, continuous batching; a step = one batch "
+                    "round (steps x max_batch requests timed)",
             "max_batch": B, "slots": SL, "entries_per_view": cap, "memory_budget_gib": args.budget_gib,
             "p99_bound_ms": P99_BOUND_MS,
         },
@@ -464,21 +497,54 @@ def main():
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_bind as ob
-        t = rg["target"]
-        a = time.perf_counter()
-        ref = ob.synthesize(W.oracle_sources(rg, planes, ob), W.oracle_cam(t["cam"], ob), t["pos"], t["rot"],
-                            t["bitdepth"], t["blending_exp"])
-        cpu_s = time.perf_counter() - a
+        # the oracle is single-threaded: pin it to one core for the timing
+        cpus = sorted(os.sched_getaffinity(0))
+        core = cpus[-1]
+        os.sched_setaffinity(0, {core})
+        try:
+            t = rg["target"]
+            a = time.perf_counter()
+            ref = ob.synthesize(W.oracle_sources(rg, planes, ob), W.oracle_cam(t["cam"], ob), t["pos"], t["rot"],
+                                t["bitdepth"], t["blending_exp"])
+            cpu_s = time.perf_counter() - a
+            # the small perspective case (2 x 640x480 8-bit sources -> 640x480): median of 5
+            rg1 = W.rig("rects_640")
+            pl1 = W.render_sources(rg1)
+            t1 = rg1["target"]
+            o1 = W.oracle_sources(rg1, pl1, ob)
+            c1 = []
+            for _ in range(5):
+                a = time.perf_counter()
+                ob.synthesize(o1, W.oracle_cam(t1["cam"], ob), t1["pos"], t1["rot"], t1["bitdepth"],
+                              t1["blending_exp"])
+                c1.append(time.perf_counter() - a)
+        finally:
+            os.sched_setaffinity(0, set(cpus))
         outp = syn.run()
         torch.cuda.synchronize()
         npdt = np.uint16 if t["bitdepth"] > 8 else np.uint8
         same = all(np.array_equal(outp[k].cpu().numpy().view(npdt), ref[k]) for k in ("y", "u", "v"))
+        model = "unknown"
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
         out["cpu_baseline"] = {
             "value": round(1.0 / cpu_s, 5),
             "unit": "views/s",
             "cores": 1,
             "kind": "port",
             "sample": "one full view of the same workload (single-threaded oracle, %.1f s)" % cpu_s,
+            "cpu_model": model,
+            "nproc": len(cpus),
+            "pinned": "core %d (sched_setaffinity) for the timed oracle runs" % core,
+            "small_perspective_case": {
+                "what": "2 perspective 640x480 8-bit sources -> one 640x480 view, median of 5",
+                "s_per_view": round(float(np.median(c1)), 4),
+            },
         }
         out["parity_full_size"] = {"yuv_bit_exact_vs_oracle": bool(same)}
     print(json.dumps(out), flush=True)

@@ -2,16 +2,21 @@
 // continuous batching of target-view requests onto the batched device path.
 //
 // Requests (scene, target pose and camera) are queued on arrival.  A scheduler
-// thread keeps up to `slots` batches in flight, each on its own HIP stream and
-// in its own share of the memory budget: whenever a slot is free it takes the
-// oldest pending request's scene and admits further pending requests of that
-// scene in arrival order while the batch's workspace plus outputs fit the
-// slot's share (and at most max_batch views).  A batch is one
-// rvs_synthesize_batch_async launch set followed by device-to-host copies of
-// the YUV planes; requests that arrive meanwhile wait for the next free slot,
-// so batches on different slots overlap on the device.  Each request's
-// latency runs from its arrival to the completion of its host copy.  Results
-// equal those of a lone rvs_synthesize call for the same target.
+// thread keeps up to `slots` batches in flight.  Device memory is one arena of
+// `memory_budget` bytes shared by all batches: whenever a slot is free it takes
+// the oldest pending request's scene and admits further pending requests of
+// that scene in arrival order while the batch's workspace plus outputs fit the
+// largest free range of the arena (and at most max_batch views); the batch's
+// memory returns to the arena when it completes, so requests that arrive while
+// others are on the device are admitted as slots and memory free up.  A
+// request that could not fit the whole budget, or whose plan is invalid, fails
+// with the plan's own error code.  A batch is one rvs_synthesize_batch_async
+// launch set followed by device-to-host copies of the YUV planes, so batches
+// on different slots overlap on the device.  A batch whose binned-entry list
+// overflowed is queued again, with a larger list, without blocking the other
+// slots.  Each request's latency runs from its arrival to the completion of
+// its host copy.  Results equal those of a lone rvs_synthesize call for the
+// same target.
 #ifndef RVS_SERVER_HPP
 #define RVS_SERVER_HPP
 
@@ -45,19 +50,29 @@ public:
     const std::vector<uint8_t>& output(int k) const { return m_out[k]; }
     /** Message of the last failure in the current run (empty when none). */
     std::string error() const;
+    /** Batches queued again after their binned-entry list overflowed (all runs). */
+    int requeued() const { return m_requeued; }
 
 private:
     struct Slot;
+    class Pool;
     void reset(const rvs_request* reqs, int n);
     void submit_locked(int k, double now);
+    void complete_locked(int k, int rc, double now, const std::string& msg);
+    size_t batch_bytes(const std::vector<rvs_source>& S, const std::vector<rvs_target>& tgts, uint32_t cap,
+                       size_t* ws_out, int* rc_out) const;
     void loop();
     bool launch(Slot& s);
-    void finish(Slot& s);
+    void finish(Slot& s, int query);
     double now_ms() const;
 
     std::vector<std::vector<rvs_source>> m_scenes;
     rvs_serve_opts m_opt;
     std::vector<Slot*> m_slots;
+    Pool* m_pool = nullptr;         // free ranges of the device arena
+    char* m_base = nullptr;         // the arena (memory_budget bytes)
+    uint32_t m_per_view = 0;        // binned entries per view of a batch (0: library default); grows on overflow
+    int m_requeued = 0;             // batches queued again after a list overflow
     std::vector<void*> m_compute;   // hipStream_t: slot k's launch sets run on m_compute[k % size]
     std::mutex m_mu;
     std::condition_variable m_cv;

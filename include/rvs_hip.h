@@ -220,6 +220,14 @@ int rvs_synthesize(const rvs_source* srcs, int n_src, const rvs_target* tgt, voi
  * 4096 views and 2^31 output pixels per batch. */
 size_t rvs_synth_batch_workspace_size(const rvs_source* srcs, int n_src, const rvs_target* tgts,
                                       int n_views, uint32_t entry_cap);
+/* The same plan with its outcome: RVS_OK and the workspace bytes in *ws_bytes,
+ * or the code a launch with these arguments would fail with (RVS_E_INVALID,
+ * RVS_E_UNSUPPORTED, RVS_E_CAPACITY; rvs_last_error says why).  Touches no
+ * device.  No counterpart in the reference, whose wrappers allocate per call:
+ * for request loops that must tell a malformed request from one that does not
+ * fit their memory budget. */
+int rvs_synth_plan(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views,
+                   uint32_t entry_cap, size_t* ws_bytes);
 int rvs_synthesize_batch(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views,
                          void* workspace, size_t ws_bytes, uint32_t entry_cap,
                          const rvs_view_out* outs, uint32_t* needed_cap, rvs_stream_t stream);

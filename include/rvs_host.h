@@ -54,13 +54,13 @@ int rvs_app_run(const char* config_path, const char* source_path, int stagewise)
  * `split_finish`, the batch's inpainting and encoding run there too).  Latency of a
  * request = arrival to the completion of its YUV copy to host memory. */
 typedef struct {
-    size_t memory_budget;       /* device bytes for in-flight batches (workspaces + outputs), split over slots */
+    size_t memory_budget;       /* device bytes for in-flight batches (workspaces + outputs), one arena shared by the slots */
     int max_batch;              /* views per batch */
     int slots;                  /* batches in flight */
     uint32_t entry_cap;         /* binned entries per view of a batch (0 = library default); grows on overflow */
     int copy_to_host;           /* copy each request's YUV planes to pinned host memory */
     int keep_outputs;           /* keep a host copy per request (rvs_server_output) */
-    size_t host_staging_bytes;  /* pinned staging per slot (0 = a quarter of the slot's share, at most 1 GiB) */
+    size_t host_staging_bytes;  /* pinned staging per slot (0 = budget / (4 * slots), at most 1 GiB) */
     int compute_streams;        /* streams the slots' launch sets run on (0 or 1: one, in order) */
     int split_finish;           /* 1: inpainting + encoding of a batch on its slot's stream, overlapping the
                                    next batch's binning and raster (rvs_synthesize_batch_async_split) */

@@ -136,7 +136,7 @@ uint32_t entry_capacity(uint32_t requested, uint64_t npx, int nsrc) {
 }
 
 void plan_raster(Layout& L, RasterPlan& R, int nsrc, int nviews, uint32_t ntiles, uint32_t npx, uint32_t cap,
-                 int max_sw, int max_sh) {
+                 int max_sw, int max_sh, uint64_t keep_max = 0) {
     rvs_k::BatchDims& d = R.d;
     d.nviews = nviews; d.nsrc = nsrc; d.ntiles = ntiles; d.npx = npx;
     d.cap = entry_capacity(cap, npx, nsrc);
@@ -157,6 +157,8 @@ void plan_raster(Layout& L, RasterPlan& R, int nsrc, int nviews, uint32_t ntiles
         const long long n = std::strtoll(env, nullptr, 10);
         if (n > 0 && (uint64_t)n < keep) keep = (uint64_t)n;
     }
+    // a caller-imposed limit (a fixed workspace) lowers it the same way
+    if (keep_max > 0 && keep_max < keep) keep = keep_max;
     d.range_cap = (uint32_t)(keep < items ? keep : items);
     const size_t nbins = (size_t)nsrc * ntiles;
     R.status = L.take(16 * sizeof(uint32_t));
@@ -224,8 +226,6 @@ int check_raster_status(const rvs_k::RasterWS& ws, const rvs_k::BatchDims& d, ui
             return fail(RVS_E_CAPACITY, "binned entries exceed 2^32 - 1: split the batch");
         return fail(RVS_E_CAPACITY, "binned (quad, tile) entries %u exceed the list capacity %u", st[2], d.cap);
     }
-    if (st[3]) return fail(RVS_E_CAPACITY, "in-order replay of a tile bin of %u quads exceeds %d", st[3],
-                           rvs_k::kFallbackSortCap);
     return RVS_OK;
 }
 
@@ -241,7 +241,7 @@ struct SynthPlan {
 };
 
 int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views, uint32_t cap, SynthPlan& P,
-               std::vector<rvs_k::ViewDesc>* views) {
+               std::vector<rvs_k::ViewDesc>* views, uint64_t keep_max = 0) {
     if (!srcs || n_src < 1 || n_src > RVS_MAX_SOURCES || !tgts || n_views < 1 || n_views > rvs_k::kMaxViews)
         return fail(RVS_E_INVALID, "synthesize: bad source or view count");
     int max_sw = 0, max_sh = 0;
@@ -279,7 +279,7 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
     for (int k = 0; k < n_src; ++k) nb += align256(rvs_k::src_bounds_floats(srcs[k].cam.w, srcs[k].cam.h) * 4);
     P.bounds = L.take(P.any_persp ? nb : 0);
     P.quality = L.take((size_t)px * sizeof(float));
-    plan_raster(L, P.R, n_src, n_views, (uint32_t)tiles, (uint32_t)px, cap, max_sw, max_sh);
+    plan_raster(L, P.R, n_src, n_views, (uint32_t)tiles, (uint32_t)px, cap, max_sw, max_sh, keep_max);
     P.inpaint = L.take(rvs_k::inpaint_ws_bytes((size_t)px, n_views));
     P.total = L.o;
     if (views) {
@@ -321,9 +321,9 @@ void fill_pair(PairGeom& g, const rvs_source& S, const rvs_target& T) {
 // after an event recorded on s behind the in-order replay.
 int enqueue_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views, void* workspace,
                   size_t ws_bytes, uint32_t cap, const rvs_view_out* outs, hipStream_t s, SynthPlan& P,
-                  rvs_k::RasterWS& ws, hipStream_t sf = nullptr) {
+                  rvs_k::RasterWS& ws, hipStream_t sf = nullptr, uint64_t keep_max = 0) {
     std::vector<rvs_k::ViewDesc> views;
-    const int prc = plan_synth(srcs, n_src, tgts, n_views, cap, P, &views);
+    const int prc = plan_synth(srcs, n_src, tgts, n_views, cap, P, &views, keep_max);
     if (prc != RVS_OK) return prc;
     if (!workspace || !outs) return fail(RVS_E_INVALID, "synthesize: null workspace or outputs");
     if (ws_bytes < P.total) return fail(RVS_E_INVALID, "synthesize: workspace %zu < %zu bytes", ws_bytes, P.total);
@@ -628,6 +628,14 @@ size_t rvs_synth_batch_workspace_size(const rvs_source* srcs, int n_src, const r
                                       uint32_t entry_cap) {
     SynthPlan P;
     return plan_synth(srcs, n_src, tgts, n_views, entry_cap, P, nullptr) == RVS_OK ? P.total : 0;
+}
+
+int rvs_synth_plan(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views, uint32_t entry_cap,
+                   size_t* ws_bytes) {
+    SynthPlan P;
+    const int rc = plan_synth(srcs, n_src, tgts, n_views, entry_cap, P, nullptr);
+    if (ws_bytes) *ws_bytes = rc == RVS_OK ? P.total : 0;
+    return rc;
 }
 
 size_t rvs_synth_workspace_size(const rvs_source* srcs, int n_src, const rvs_target* tgt, uint32_t entry_cap) {

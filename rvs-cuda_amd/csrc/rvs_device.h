@@ -119,7 +119,10 @@ __device__ inline float3 unproject_src(const PairGeom& g, int i, int j) {
 }
 
 // Source-frame point -> target u', v' (after the uv rescale) and target depth.
-__device__ inline float3 project_tgt(const PairGeom& g, float3 p) {
+// With `und`, the radius of an equirectangular target takes only the fast
+// float-float path and *und is set when it leaves the radius undecided (the
+// caller then defers what depends on this vertex); without, it is exact.
+__device__ inline float3 project_tgt(const PairGeom& g, float3 p, bool* und = nullptr) {
     float X = 0.f, Y = 0.f, Z = 0.f;
     X += g.R[0] * p.x; X += g.R[1] * p.y; X += g.R[2] * p.z;
     Y += g.R[3] * p.x; Y += g.R[4] * p.y; Y += g.R[5] * p.z;
@@ -129,7 +132,12 @@ __device__ inline float3 project_tgt(const PairGeom& g, float3 p) {
     Z = Z + g.t[2];
     float u, v, vd;
     if (g.dst_erp) {
-        const float radius = norm_sq_sum_d(X, Y, Z);
+        float radius;
+        if (und) {
+            if (!norm_sq_sum_fast(X, Y, Z, radius)) *und = true;
+        } else {
+            radius = norm_sq_sum_d(X, Y, Z);
+        }
         vd = radius;
         const float phi = atan2f(Y, X);
         const float theta = asinf(Z / radius);
@@ -147,10 +155,13 @@ __device__ inline float3 project_tgt(const PairGeom& g, float3 p) {
     return make_float3(g.sx * u + 0.f, g.sy * v + 0.f, vd);
 }
 
-// u', v', target depth of source pixel (i = row, j = column).
-__device__ inline float3 vertex(const PairGeom& g, int i, int j) { return project_tgt(g, unproject_src(g, i, j)); }
+// u', v', target depth of source pixel (i = row, j = column); `und` as in
+// project_tgt.
+__device__ inline float3 vertex(const PairGeom& g, int i, int j, bool* und = nullptr) {
+    return project_tgt(g, unproject_src(g, i, j), und);
+}
 
-__device__ inline float3 vertex(const ArrayGeom& g, int i, int j) {
+__device__ inline float3 vertex(const ArrayGeom& g, int i, int j, bool* = nullptr) {
     const size_t idx = (size_t)i * g.sw + j;
     return make_float3(g.uv[2 * idx], g.uv[2 * idx + 1], g.vd[idx]);
 }
@@ -256,31 +267,21 @@ __device__ inline void tri_load(Tri& T, float3 a, float3 b, float3 c) {
 // valid_tri (transform.cpp:62-75): stretch = (float) max of three double
 // norms, / g_rescale; validity = clamp(10000 - 1000 * stretch, 1, 10000).  The
 // clamp's max(1, q) maps NaN to 1, so the result is never 0 and
-// colorize_triangle's `validity == 0` return is never taken.
-//
-// The three norms share one square root: (float) sqrt(s) of a double s is
-// monotone non-decreasing (correctly rounded sqrt, then rounding to float) and
-// maps NaN to NaN, so applying it after the same (a < b) ? b : a maxima on the
-// squared lengths gives the same bits as the maximum of the three norms; the
-// zero third component adds +0.0 to a non-negative sum, which changes nothing.
-__device__ inline double edge_sq_d(float dx, float dy) {
-    double s = (double)dx * (double)dx;
-    s += (double)dy * (double)dy;
-    return s;
+// colorize_triangle's `validity == 0` return is never taken.  The double norms
+// are computed exactly in float arithmetic (tri_stretch, rvs_math.h): no FP64
+// instruction on the device.
+__device__ inline float tri_validity(const Tri& T, float rescale) {
+    return tri_quality(tri_stretch(T.A0, T.A1, T.B0, T.B1, T.C0, T.C1), rescale);
 }
 
-__device__ inline double dmax_d(double a, double b) { return (a < b) ? b : a; }
-
-__device__ inline float tri_validity(const Tri& T, float rescale) {
-    const double sab = edge_sq_d(T.A0 - T.B0, T.A1 - T.B1);
-    const double sac = edge_sq_d(T.A0 - T.C0, T.A1 - T.C1);
-    const double sbc = edge_sq_d(T.B0 - T.C0, T.B1 - T.C1);
-    float stretch = (float)__builtin_sqrt(dmax_d(sbc, dmax_d(sab, sac)));
-    stretch /= rescale;
-    float quality = 10000.f - 1000.f * stretch;
-    quality = smax(1.f, quality);
-    quality = smin(10000.f, quality);
-    return quality;
+// The same when the stretch is decided by the fast path; false otherwise
+// (non-finite or out-of-range edges, or a square root within 2^-44 of a float
+// rounding midpoint): the caller then defers what depends on it.
+__device__ inline bool tri_validity_fast(const Tri& T, float rescale, float& tv) {
+    float s;
+    if (!tri_stretch_fast(T.A0, T.A1, T.B0, T.B1, T.C0, T.C1, s)) return false;
+    tv = tri_quality(s, rescale);
+    return true;
 }
 
 // colorize_triangle's early returns (transform.cpp:80-96) without the

@@ -21,7 +21,8 @@ constexpr int kTile = 16;            // target tile edge (pixels); one 256-threa
 constexpr int kBinQX = 64;           // binning work item: 64 x 4 source quads
 constexpr int kBinQY = 4;
 constexpr int kFallbackChunk = 128;  // list entries per LDS chunk in the in-order replay
-constexpr int kFallbackSortCap = 8192;  // longest (source, tile) list the replay sorts in LDS
+constexpr int kFallbackSortCap = 8192;  // longest (source, tile) list the replay sorts in LDS (longer
+                                        // ones are sorted in place in the entry list)
 constexpr int kMaxViews = 4096;
 constexpr int kBinVerts = (kBinQY + 1) * (kBinQX + 1);   // vertices of one binning work item
 constexpr int kBinBlocks = 2048;    // persistent binning blocks (work items past the kept range)
@@ -62,8 +63,7 @@ struct RasterWS {
     uint8_t* deferred;       // [global pixel] pixel handled by the in-order replay
     uint32_t* deferred_tiles;// [global tile]
     uint32_t* work;          // binning work items (view, source, block)
-    uint32_t* status;        // [0] n_deferred, [1] list overflow, [2] entries (saturated), [3] replay
-                             // overflow, [4] work items
+    uint32_t* status;        // [0] n_deferred, [1] list overflow, [2] entries (saturated), [4] work items
 };
 
 struct BatchDims {

@@ -7,12 +7,14 @@
 //   * float -> int is cvttss2si: truncation, INT_MIN when out of range or NaN
 //     (transform.cpp:90-93 feeds floor/ceil of projected coordinates through it).
 //   * cv::norm accumulates squares in double (transform.cpp:63-65,
-//     EquirectangularProjector.cpp:80).  norm_sq_sum_d() returns the float
-//     result of that double computation: on the GPU with FP64 itself (gfx950
-//     issues FP64 adds and FMAs at the unpacked FP32 rate, and llvm.sqrt.f64
-//     is correctly rounded), on the host with float-float arithmetic plus an
-//     exact integer path for the rare inputs whose result lies too close to a
-//     float rounding midpoint (norm_sq_sum_ff, checked against double on the CPU).
+//     EquirectangularProjector.cpp:80).  norm_sq_sum_ff() returns the float
+//     result of that double computation without any FP64 arithmetic:
+//     float-float sums of the squares (fma splits every square exactly), one
+//     float square root with an exact fma residual, and an exact integer path
+//     for the rare inputs whose result lies too close to a float rounding
+//     midpoint.  tri_stretch() does the same for valid_tri's maximum of three
+//     edge norms with a single square root.  Host and device run this same
+//     code (checked against double arithmetic on the CPU).
 // All code here must be compiled with -ffp-contract=off.
 #ifndef RVS_MATH_H
 #define RVS_MATH_H
@@ -224,7 +226,15 @@ RVS_HD float xf_to_float(XF X) {
     return u2f(bits);
 }
 
-RVS_HD float norm_slow(float a, float b, float c) {
+// Rare path (inputs near a float rounding midpoint, or outside the
+// float-float range); kept out of line on the device so that its 128-bit
+// arithmetic does not weigh on the callers' registers.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __attribute__((noinline)) float norm_slow(float a, float b, float c);
+__device__ __attribute__((noinline)) float norm_slow(float a, float b, float c) {
+#else
+inline float norm_slow(float a, float b, float c) {
+#endif
     XF A = xf_from_float_sq(a), B = xf_from_float_sq(b), Cc = xf_from_float_sq(c);
     XF s = xf_add_rn53(xf_add_rn53(A, B), Cc);
     return xf_to_float(xf_sqrt_rn53(s));
@@ -239,71 +249,173 @@ RVS_HD void two_sum(float a, float b, float& s, float& e) {
 
 }  // namespace detail
 
-// (float)sqrt(((0.0 + (double)a*a) + (double)b*b) + (double)c*c), exactly,
-// in float-float arithmetic.
-RVS_HD float norm_sq_sum_ff(float a, float b, float c) {
-    using namespace detail;
-    float aa = fabs_(a), ab = fabs_(b), ac = fabs_(c);
-    if (is_nan(a) || is_nan(b) || is_nan(c)) return a + b + c;  // NaN
-    if (!is_finite(aa) || !is_finite(ab) || !is_finite(ac)) return u2f(0x7f800000u);
-    float mx = smax(aa, smax(ab, ac));
-    if (mx == 0.0f) return 0.0f;
+namespace detail {
+
+// Components for which the float-float path is exact: zero or in [2^-40, 2^40)
+// (the squares' fma error terms stay representable, nothing overflows).
+RVS_HD bool ff_range(float ax) {
     const float lo_lim = 9.094947e-13f;   // 2^-40
     const float hi_lim = 1.0995116e12f;   // 2^40
-    bool ok = mx < hi_lim && (aa == 0.0f || aa >= lo_lim) && (ab == 0.0f || ab >= lo_lim) &&
-              (ac == 0.0f || ac >= lo_lim);
-    if (!ok) return norm_slow(a, b, c);
-    // exact squares
-    float pa = aa * aa, ea = ffma(aa, aa, -pa);
-    float pb = ab * ab, eb = ffma(ab, ab, -pb);
-    float pc = ac * ac, ec = ffma(ac, ac, -pc);
-    // float-float sum
+    return ax < hi_lim && (ax == 0.0f || ax >= lo_lim);
+}
+
+// shi + slo = a^2 + b^2 + c^2 (|a|, |b|, |c| in ff_range) within 2^-46
+// relative: every square split exactly by fma, two error-free additions.
+RVS_HD void sq_sum3_ff(float aa, float ab, float ac, float& shi, float& slo) {
+    const float pa = aa * aa, ea = ffma(aa, aa, -pa);
+    const float pb = ab * ab, eb = ffma(ab, ab, -pb);
+    const float pc = ac * ac, ec = ffma(ac, ac, -pc);
     float s1, t1;
     two_sum(pa, pb, s1, t1);
     float lo = t1 + (ea + eb);
     float s2, t2;
     two_sum(s1, pc, s2, t2);
     lo = lo + (t2 + ec);
-    float shi = s2 + lo;
-    float slo = lo - (shi - s2);
-    // float-float sqrt
-    float r = fsqrt(shi);
-    float resid = ffma(-r, r, shi);       // exact
-    float rlo = (resid + slo) / (2.0f * r);
-    // decide RN24(r + rlo)
-    float tol = r * 5.684342e-14f;         // 2^-44 relative
-    uint32_t rb = f2u(r);
-    if (rlo > 0.0f) {
-        float up = u2f(rb + 1u);
-        float half = (up - r) * 0.5f;
-        float d = rlo - half;
-        if (d > tol) return up;
-        if (d < -tol) return r;
-    } else if (rlo < 0.0f) {
-        float dn = u2f(rb - 1u);
-        float half = (r - dn) * 0.5f;
-        float d = -rlo - half;
-        if (d > tol) return dn;
-        if (d < -tol) return r;
-    } else {
-        return r;
-    }
-    return norm_slow(a, b, c);
+    shi = s2 + lo;
+    slo = lo - (shi - s2);
 }
 
-// (float)sqrt(((0.0 + (double)a*a) + (double)b*b) + (double)c*c).  The
-// squares of floats are exact in double, so only the two sums, the square
-// root and the final conversion round, as in the reference.
-RVS_HD float norm_sq_sum_d(float a, float b, float c) {
+// The same for two components (what sq_sum3_ff gives with c = 0: its c terms
+// add exact zeros).
+RVS_HD void sq_sum2_ff(float aa, float ab, float& shi, float& slo) {
+    const float pa = aa * aa, ea = ffma(aa, aa, -pa);
+    const float pb = ab * ab, eb = ffma(ab, ab, -pb);
+    float s1, t1;
+    two_sum(pa, pb, s1, t1);
+    const float lo = t1 + (ea + eb);
+    shi = s1 + lo;
+    slo = lo - (shi - s1);
+}
+
+// RN24(sqrt(shi + slo)) for shi > 0, decided when the root lies more than
+// 2^-44 (relative) from a float rounding midpoint -- far beyond the error of
+// shi + slo and of the reference's two double roundings (sum and root), so
+// the reference's (float) sqrt of the double sum is that same float.  Returns
+// false (undecided) otherwise.
+RVS_HD bool sqrt_ff_rn24(float shi, float slo, float& out) {
+    const float r = fsqrt(shi);
+    const float resid = ffma(-r, r, shi);       // exact
+    const float rlo = (resid + slo) / (2.0f * r);
+    const float tol = r * 5.684342e-14f;        // 2^-44 relative
+    const uint32_t rb = f2u(r);
+    if (rlo > 0.0f) {
+        const float up = u2f(rb + 1u);
+        const float half = (up - r) * 0.5f;
+        const float d = rlo - half;
+        if (d > tol) { out = up; return true; }
+        if (d < -tol) { out = r; return true; }
+    } else if (rlo < 0.0f) {
+        const float dn = u2f(rb - 1u);
+        const float half = (r - dn) * 0.5f;
+        const float d = -rlo - half;
+        if (d > tol) { out = dn; return true; }
+        if (d < -tol) { out = r; return true; }
+    } else {
+        out = r;
+        return true;
+    }
+    return false;
+}
+
+}  // namespace detail
+
+// (float)sqrt(((0.0 + (double)a*a) + (double)b*b) + (double)c*c) without the
+// exact integer path: false (undecided, out = the float root of the float-float
+// sum) for components outside the float-float range or a root within 2^-44 of
+// a float rounding midpoint.
+RVS_HD bool norm_sq_sum_fast(float a, float b, float c, float& out) {
+    using namespace detail;
+    const float aa = fabs_(a), ab = fabs_(b), ac = fabs_(c);
+    if (is_nan(a) || is_nan(b) || is_nan(c)) { out = a + b + c; return true; }  // NaN
+    if (!is_finite(aa) || !is_finite(ab) || !is_finite(ac)) { out = u2f(0x7f800000u); return true; }
+    const float mx = smax(aa, smax(ab, ac));
+    if (mx == 0.0f) { out = 0.0f; return true; }
+    if (!(ff_range(aa) && ff_range(ab) && ff_range(ac))) {
+        out = fsqrt(aa * aa + ab * ab + ac * ac);
+        return false;
+    }
+    float shi, slo;
+    sq_sum3_ff(aa, ab, ac, shi, slo);
+    if (sqrt_ff_rn24(shi, slo, out)) return true;
+    out = fsqrt(shi);
+    return false;
+}
+
+// (float)sqrt(((0.0 + (double)a*a) + (double)b*b) + (double)c*c), exactly,
+// in float arithmetic.
+RVS_HD float norm_sq_sum_ff(float a, float b, float c) {
+    float r;
+    if (norm_sq_sum_fast(a, b, c, r)) return r;
+    return detail::norm_slow(a, b, c);
+}
+
+// The float result of the reference's double norm of a 3-vector (the radius
+// of an equirectangular projection, EquirectangularProjector.cpp:80).
+RVS_HD float norm_sq_sum_d(float a, float b, float c) { return norm_sq_sum_ff(a, b, c); }
+
+// valid_tri's stretch before the rescale (transform.cpp:62-68):
+// (float) std::max(dbc, std::max(dab, dac)) of the double norms |A - B|,
+// |A - C|, |B - C| (differences taken in float).  Fast form: the three squared
+// lengths as float-float sums, the largest chosen with the reference's
+// comparisons (so a NaN length is passed over or kept exactly as the double
+// std::max does), one square root.  Two lengths whose float-float values
+// compare in the wrong order differ by less than 2^-45 relative, so their
+// floats differ only across a rounding midpoint within that distance, which
+// the root's 2^-44 test reports as undecided.  Returns false (undecided) for
+// non-finite or out-of-range components and near midpoints.
+RVS_HD bool tri_stretch_fast(float A0, float A1, float B0, float B1, float C0, float C1, float& out) {
+    using namespace detail;
+    const float x0 = fabs_(A0 - B0), y0 = fabs_(A1 - B1);
+    const float x1 = fabs_(A0 - C0), y1 = fabs_(A1 - C1);
+    const float x2 = fabs_(B0 - C0), y2 = fabs_(B1 - C1);
+    if (!(ff_range(x0) && ff_range(y0) && ff_range(x1) && ff_range(y1) && ff_range(x2) && ff_range(y2)))
+        return false;   // also NaN (every comparison false)
+    float h0, l0, h1, l1, h2, l2;
+    sq_sum2_ff(x0, y0, h0, l0);
+    sq_sum2_ff(x1, y1, h1, l1);
+    sq_sum2_ff(x2, y2, h2, l2);
+    // m = max(dab, dac) = (dab < dac) ? dac : dab; then (dbc < m) ? m : dbc
+    float mh = h0, ml = l0;
+    if (h0 < h1 || (h0 == h1 && l0 < l1)) { mh = h1; ml = l1; }
+    if (!(h2 < mh || (h2 == mh && l2 < ml))) { mh = h2; ml = l2; }
+    if (mh == 0.0f) { out = 0.0f; return true; }
+    return sqrt_ff_rn24(mh, ml, out);
+}
+
+namespace detail {
+// The three exact norms (max of the floats = float of the max of the doubles:
+// the float conversion is monotone and keeps NaN, so the comparisons pick the
+// same value).  Out of line on the device, like norm_slow.
 #if defined(__HIP_DEVICE_COMPILE__)
-    double s = 0.0;
-    s += (double)a * (double)a;
-    s += (double)b * (double)b;
-    s += (double)c * (double)c;
-    return (float)__builtin_sqrt(s);
+__device__ __attribute__((noinline)) float tri_stretch_slow(float A0, float A1, float B0, float B1, float C0,
+                                                            float C1);
+__device__ __attribute__((noinline)) float tri_stretch_slow(float A0, float A1, float B0, float B1, float C0,
+                                                            float C1) {
 #else
-    return norm_sq_sum_ff(a, b, c);
+inline float tri_stretch_slow(float A0, float A1, float B0, float B1, float C0, float C1) {
 #endif
+    const float dab = norm_sq_sum_ff(A0 - B0, A1 - B1, 0.0f);
+    const float dac = norm_sq_sum_ff(A0 - C0, A1 - C1, 0.0f);
+    const float dbc = norm_sq_sum_ff(B0 - C0, B1 - C1, 0.0f);
+    return smax(dbc, smax(dab, dac));
+}
+}  // namespace detail
+
+// The same, always decided: undecided cases take the three exact norms.
+RVS_HD float tri_stretch(float A0, float A1, float B0, float B1, float C0, float C1) {
+    float s;
+    if (tri_stretch_fast(A0, A1, B0, B1, C0, C1, s)) return s;
+    return detail::tri_stretch_slow(A0, A1, B0, B1, C0, C1);
+}
+
+// valid_tri's quality from the stretch (transform.cpp:68-74).  The clamp's
+// max(1, q) maps NaN to 1, so the result is never 0.
+RVS_HD float tri_quality(float stretch, float rescale) {
+    stretch /= rescale;
+    float quality = 10000.f - 1000.f * stretch;
+    quality = smax(1.f, quality);
+    quality = smin(10000.f, quality);
+    return quality;
 }
 
 }  // namespace rvs_math

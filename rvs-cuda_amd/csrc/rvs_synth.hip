@@ -784,24 +784,35 @@ __device__ inline int lane_now() {
 
 // Winner's colour, depth and validity at pixel (x, y) for slot `slot` of
 // source quad (i, j): the state the fold holds after accepting that candidate.
-template <class G>
-__device__ __forceinline__ void resolve(const G& g, int i, int j, int slot, int x, int y, float3& c, float& dep,
+// FAST: vertex radii and the validity take only their fast float-float paths,
+// and the result is false when one of them is undecided (the caller defers the
+// pixel; the coverage pass, which evaluated the same triangle, deferred it
+// already); otherwise they are exact and the result is true.
+template <bool FAST, class G>
+__device__ __forceinline__ bool resolve(const G& g, int i, int j, int slot, int x, int y, float3& c, float& dep,
                                float& shape, int out_w, int out_h, const float* vb) {
     int r[3], cc[3];
     float3 v[3];
+    bool und = false;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         slot_vertex(slot, k, i, j, g.sw, r[k], cc[k]);
         v[k] = (vb && slot < 2) ? kept_vertex(vb, i % kBinQY + (r[k] - i), j % kBinQX + (cc[k] - j))
-                               : vertex(g, r[k], cc[k]);
+                               : vertex(g, r[k], cc[k], FAST ? &und : nullptr);
     }
     Tri T;
-    tri_setup(T, v[0], v[1], v[2], out_w, out_h, g.rescale);
+    if (FAST) {
+        tri_setup_geom(T, v[0], v[1], v[2], out_w, out_h);
+        if (!tri_validity_fast(T, g.rescale, T.tv) || und) return false;
+    } else {
+        tri_setup(T, v[0], v[1], v[2], out_w, out_h, g.rescale);
+    }
     float l1, l2, l3;
     tri_pixel(T, x, y, l1, l2, l3);
     dep = tri_depth(T, l1, l2, l3);
     shape = T.tv;
     c = tri_color(color(g, r[0], cc[0]), color(g, r[1], cc[1]), color(g, r[2], cc[2]), l1, l2, l3);
+    return true;
 }
 
 // The same from the corner vertices and validity the coverage pass cached
@@ -836,12 +847,12 @@ constexpr uint32_t kDeferKey = 0xffffffffu;
 // no pixel is never a winner, so skipping it changes nothing).
 __device__ __forceinline__ float raster_tri(float3 a, float3 b, float3 c, uint32_t id, bool prol_false_possible, int tx0,
                                    int ty0, int out_w, int out_h, float rescale, unsigned long long* key1,
-                                   uint32_t* key2) {
+                                   uint32_t* key2, bool undecided) {
     Tri T;
     if (!tri_setup_geom(T, a, b, c, out_w, out_h)) return 0.f;
     const bool prol = (a.z > 0.f) || (b.z > 0.f) || (c.z > 0.f);
-    const bool special = !finite3(a) || !finite3(b) || !finite3(c) || !is_finite(T.den) ||
-                         (prol_false_possible && !prol);
+    bool special = !finite3(a) || !finite3(b) || !finite3(c) || !is_finite(T.den) ||
+                   (prol_false_possible && !prol) || undecided;
     int xa = max(T.xmin, tx0), xb = min(T.xmax, tx0 + kTile - 1);
     int ya = max(T.ymin, ty0), yb = min(T.ymax, ty0 + kTile - 1);
     if (!special) tri_tight(T, xa, xb, ya, yb);
@@ -878,14 +889,15 @@ __device__ __forceinline__ float raster_tri(float3 a, float3 b, float3 c, uint32
                 if (!tri_surely_outside(T, n1, n2)) mask |= 1ull << ((y - ya) * 8 + (x - xa));
             }
         if (!mask) return 0.f;   // no candidate pixel: the validity is never needed
-        T.tv = tri_validity(T, rescale);
+        // a validity the fast path leaves undecided defers the pixels it covers
+        if (!tri_validity_fast(T, rescale, T.tv)) special = true;
         while (mask) {
             const int bit = __builtin_ctzll(mask);
             mask &= mask - 1ull;
             push(xa + (bit & 7), ya + (bit >> 3));
         }
     } else {
-        T.tv = tri_validity(T, rescale);
+        if (!tri_validity_fast(T, rescale, T.tv)) special = true;
         for (int y = ya; y <= yb; ++y)
             for (int x = xa; x <= xb; ++x) {
                 float n1, n2;
@@ -918,8 +930,11 @@ __device__ __forceinline__ void raster_quad(const G& g, uint32_t ij, uint32_t e,
     unpack_ij(ij, i, j);
     const float* vb = kept_block(ws, d, vs, i, j);
     const int li = i % kBinQY, lj = j % kBinQX;
+    // vertices evaluated here take the fast radius path; an undecided one
+    // defers the pixels of the quad's triangles (kept vertices are exact)
+    bool und = false;
     auto vtx = [&](int di, int dj) -> float3 {
-        return vb ? kept_vertex(vb, li + di, lj + dj) : vertex(g, i + di, j + dj);
+        return vb ? kept_vertex(vb, li + di, lj + dj) : vertex(g, i + di, j + dj, &und);
     };
     // vertices are fetched just before their triangles need them; keys
     // commute, so only the set of candidates matters here
@@ -930,18 +945,18 @@ __device__ __forceinline__ void raster_quad(const G& g, uint32_t ij, uint32_t e,
     {
         const float3 v00 = vtx(0, 0);
         if (gate && v00.z > 0.f && v00.x > 0.f)
-            tv0 = raster_tri(v00, v01, v10, s0, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2);
+            tv0 = raster_tri(v00, v01, v10, s0, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und);
     }
     {
         const float3 v11 = vtx(1, 1);
         if (gate && v11.z > 0.f && v11.x > 0.f)
-            tv1 = raster_tri(v11, v10, v01, s0 | 1u, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2);
+            tv1 = raster_tri(v11, v10, v01, s0 | 1u, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und);
     }
     if (gate && g.wrap && j == 0) {
         const float3 v00 = vtx(0, 0);
-        const float3 l0 = vertex(g, i, g.sw - 1), l1 = vertex(g, i + 1, g.sw - 1);
-        raster_tri(l0, v00, l1, s0 | 2u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2);
-        raster_tri(v10, l1, v00, s0 | 3u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2);
+        const float3 l0 = vertex(g, i, g.sw - 1, &und), l1 = vertex(g, i + 1, g.sw - 1, &und);
+        raster_tri(l0, v00, l1, s0 | 2u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und);
+        raster_tri(v10, l1, v00, s0 | 3u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und);
     }
     if (C) {
         C->tv[0][e] = tv0;
@@ -1060,8 +1075,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI 
                                                    : ws.lists[(size_t)ws.offsets[(size_t)T * d.nsrc + s] +
                                                               (e - pref[k])];
                         unpack_ij(ij, i, j);
-                        resolve(g, i, j, slot, px, py, c, dep, shape, vd.w, vd.h,
-                                kept_block(ws, d, (uint32_t)(v * d.nsrc + s), i, j));
+                        if (!resolve<true>(g, i, j, slot, px, py, c, dep, shape, vd.w, vd.h,
+                                           kept_block(ws, d, (uint32_t)(v * d.nsrc + s), i, j))) {
+                            deferred = true;
+                            break;
+                        }
                     }
                 }
                 if (MODE == MODE_RAW) {
@@ -1109,16 +1127,41 @@ struct Cand {
 constexpr int kCandPool = kFallbackSortCap * 4 / (int)sizeof(Cand);   // records sharing the sort buffer
 static_assert(kCandPool < 0xffff, "candidate links are 16-bit");
 
+// Sorts L[0, n) ascending in place, one block: a bitonic network in its
+// all-ascending form (the first step of each merge compares e with its mirror
+// e ^ (k - 1), the later ones with e ^ jj, always putting the smaller value at
+// the lower index), so the positions from n up to the next power of two act
+// as +inf and their comparisons are skipped.  Global memory written by the
+// block's threads is read back by other threads of the block after each
+// barrier.
+__device__ void block_sort_global(uint32_t* L, uint32_t n) {
+    uint32_t np2 = 1;
+    while (np2 < n) np2 <<= 1;
+    for (uint32_t k = 2; k <= np2; k <<= 1)
+        for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+            const uint32_t mate = jj == (k >> 1) ? k - 1 : jj;
+            for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
+                const uint32_t p = e ^ mate;
+                if (p > e && p < n) {
+                    const uint32_t a = L[e], b = L[p];
+                    if (a > b) { L[e] = b; L[p] = a; }
+                }
+            }
+            __syncthreads();
+        }
+}
+
 // Deferred pixels of one tile and source, in the reference's traversal order.
 // Fast path: the block rasterises the source's bin in parallel and appends
 // every candidate covering a deferred pixel of the current pixel range to
 // that pixel's list in LDS; each pixel then folds its own candidates in
 // ascending (quad, slot) order.  When a range's lists overflow the pool, the
 // range is split (down to a single pixel); only a single pixel with more
-// candidates than the pool falls back to sorting the bin in LDS and scanning
-// it in order.
+// candidates than the pool falls back to sorting the bin (in LDS, or in place
+// in the entry list when it is longer than the LDS buffer; only this block
+// reads this bin after the raster) and scanning it in order.
 template <class G>
-__device__ void replay_source(const G& g, const uint32_t* L, uint32_t n, int tx0, int ty0, int out_w, int out_h,
+__device__ void replay_source(const G& g, uint32_t* L, uint32_t n, int tx0, int ty0, int out_w, int out_h,
                               bool mine, int px, int py, const uint8_t* dmask, uint32_t* sorted, ReplayEntry* rec,
                               uint32_t* pool_n, FoldState& fs) {
     const int tid = threadIdx.x;
@@ -1199,33 +1242,35 @@ __device__ void replay_source(const G& g, const uint32_t* L, uint32_t n, int tx0
         span = min(span * 2, kTile * kTile);
     }
     if (!scan_all) return;
-    if (n > (uint32_t)kFallbackSortCap) {   // cannot be sorted in LDS: reported as a capacity error
-        if (tid == 0) atomicMax(pool_n + 1, n);
-        return;
-    }
     // the remaining deferred pixels (from lo on) scan the whole bin in order
     mine = mine && pme >= lo;
     if (mine) fold_init(fs);
     // overflow: sort the bin (bitonic, ascending quad code = traversal order) and scan it
-    uint32_t np2 = 1;
-    while (np2 < n) np2 <<= 1;
-    for (uint32_t e = tid; e < np2; e += 256) sorted[e] = e < n ? L[e] : 0xffffffffu;
-    __syncthreads();
-    for (uint32_t k = 2; k <= np2; k <<= 1)
-        for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
-            for (uint32_t e = tid; e < np2; e += 256) {
-                const uint32_t p = e ^ jj;
-                if (p > e) {
-                    const uint32_t a = sorted[e], b = sorted[p];
-                    const bool asc = (e & k) == 0;
-                    if ((a > b) == asc) { sorted[e] = b; sorted[p] = a; }
+    const uint32_t* S = sorted;
+    if (n <= (uint32_t)kFallbackSortCap) {
+        uint32_t np2 = 1;
+        while (np2 < n) np2 <<= 1;
+        for (uint32_t e = tid; e < np2; e += 256) sorted[e] = e < n ? L[e] : 0xffffffffu;
+        __syncthreads();
+        for (uint32_t k = 2; k <= np2; k <<= 1)
+            for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+                for (uint32_t e = tid; e < np2; e += 256) {
+                    const uint32_t p = e ^ jj;
+                    if (p > e) {
+                        const uint32_t a = sorted[e], b = sorted[p];
+                        const bool asc = (e & k) == 0;
+                        if ((a > b) == asc) { sorted[e] = b; sorted[p] = a; }
+                    }
                 }
+                __syncthreads();
             }
-            __syncthreads();
-        }
+    } else {
+        block_sort_global(L, n);
+        S = L;
+    }
     for (uint32_t b = 0; b < n; b += kFallbackChunk) {
         if (tid < kFallbackChunk && b + tid < n) {
-            const uint32_t q = sorted[b + tid];
+            const uint32_t q = S[b + tid];
             int i, j;
             unpack_ij(q, i, j);
             Quad Q;
@@ -1260,7 +1305,7 @@ __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, c
     __shared__ uint32_t sorted[kFallbackSortCap];
     __shared__ ReplayEntry rec[kFallbackChunk];
     __shared__ uint8_t dmask[kTile * kTile];
-    __shared__ uint32_t pool_n[2];   // [0] candidate records, [1] bin too large for the sorted scan
+    __shared__ uint32_t pool_n[1];   // candidate records
     const int tid = threadIdx.x;
     const uint32_t ndef = ws.status[1] ? 0u : ws.status[0];
     for (uint32_t ti = blockIdx.x; ti < ndef; ti += gridDim.x) {
@@ -1283,16 +1328,9 @@ __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, c
             const size_t bin = (size_t)T * d.nsrc + s;
             const uint32_t n = ws.counts[bin];
             FoldState fs;
-            if (tid == 0) pool_n[1] = 0u;
             replay_source(g, ws.lists + ws.offsets[bin], n, tx0, ty0, vd.w, vd.h, mine, px, py, dmask, sorted, rec,
                           pool_n, fs);
             __syncthreads();
-            const uint32_t too_big = pool_n[1];
-            __syncthreads();   // read by every thread before thread 0 clears it again
-            if (too_big) {
-                if (tid == 0) atomicMax(&ws.status[3], too_big);
-                continue;
-            }
             if (mine) {
                 float3 c = make_float3(0.f, 0.f, 0.f);
                 float dep = __builtin_inff(), shape = 0.f;
@@ -1300,7 +1338,7 @@ __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, c
                     float dd, sh;
                     int i, j;
                     unpack_ij(fs.ij, i, j);
-                    resolve(g, i, j, fs.slot, px, py, c, dd, sh, vd.w, vd.h, nullptr);
+                    resolve<false>(g, i, j, fs.slot, px, py, c, dd, sh, vd.w, vd.h, nullptr);
                     dep = fs.new_d;
                     shape = fs.shape;
                 }

@@ -6,7 +6,10 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
+#include <iterator>
+#include <map>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -36,31 +39,77 @@ void hip_check(hipError_t e, const char* what) {
 
 }  // namespace
 
+// First-fit allocator over the server's one device arena (offsets, 256-byte
+// granules); neighbouring free ranges merge on release.
+class Server::Pool {
+public:
+    explicit Pool(size_t bytes) : m_total(bytes & ~(size_t)255) {
+        if (m_total) m_free[0] = m_total;
+    }
+    size_t total() const { return m_total; }
+    size_t largest() const {
+        size_t m = 0;
+        for (const auto& f : m_free) m = std::max(m, f.second);
+        return m;
+    }
+    // Returns the offset, or SIZE_MAX when no free range holds `bytes`.
+    size_t take(size_t bytes) {
+        bytes = align256(std::max<size_t>(bytes, 256));
+        for (auto it = m_free.begin(); it != m_free.end(); ++it) {
+            if (it->second < bytes) continue;
+            const size_t off = it->first, len = it->second;
+            m_free.erase(it);
+            if (len > bytes) m_free[off + bytes] = len - bytes;
+            return off;
+        }
+        return SIZE_MAX;
+    }
+    void give(size_t off, size_t bytes) {
+        bytes = align256(std::max<size_t>(bytes, 256));
+        auto it = m_free.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != m_free.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            m_free.erase(nx);
+        }
+        if (it != m_free.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                m_free.erase(it);
+            }
+        }
+    }
+
+private:
+    size_t m_total;
+    std::map<size_t, size_t> m_free;   // offset -> length
+};
+
 // A slot holds one batch from launch until its outputs are on the host: its
 // launch set runs on a compute stream shared with other slots (with one such
 // stream the batches of all slots run in order, so the device is never split
 // between them; several let long tails overlap), and its YUV planes
-// (contiguous at the start of the arena, laid out as in the host staging
-// buffer) are copied back on the slot's own stream, overlapping later
-// batches.
+// (contiguous at the start of the batch's allocation, laid out as in the host
+// staging buffer) are copied back on the slot's own stream, overlapping later
+// batches.  The allocation comes from the server's shared pool at launch and
+// returns to it when the batch completes.
 struct Server::Slot {
-    hipStream_t stream = nullptr;      // host copies (and capacity re-runs)
+    hipStream_t stream = nullptr;      // host copies (and, split, inpainting + encoding)
     hipEvent_t computed = nullptr;     // the batch's launch set has finished
     hipEvent_t done = nullptr;         // ... and its outputs are on the host
-    char* arena = nullptr;
-    size_t arena_bytes = 0;
+    char* arena = nullptr;             // the batch's allocation (pool base + alloc_off)
+    size_t alloc_off = 0, alloc_bytes = 0;
     uint8_t* pinned = nullptr;
     size_t pinned_bytes = 0;
     bool busy = false;
     int index = 0;           // slot number (picks the compute stream)
     int scene = 0;
-    uint32_t per_view = 0;   // binned entries per view (0: library default)
     std::vector<int> reqs;
     std::vector<rvs_target> tgts;
     std::vector<rvs_view_out> outs;
     std::vector<size_t> host_off;
-    size_t ws_bytes = 0;
-    size_t yuv_bytes = 0;    // the batch's encoded planes at the start of the arena
+    size_t yuv_bytes = 0;    // the batch's encoded planes at the start of the allocation
     size_t ws_off = 0;
     uint32_t cap = 0;        // entry capacity the current batch was launched with
 };
@@ -70,7 +119,10 @@ Server::Server(std::vector<std::vector<rvs_source>> scenes, const rvs_serve_opts
     if (m_scenes.empty()) throw std::invalid_argument("server: no scene");
     if (m_opt.slots < 1) m_opt.slots = 1;
     if (m_opt.max_batch < 1) m_opt.max_batch = 1;
-    const size_t share = m_opt.memory_budget / (size_t)m_opt.slots;
+    if (m_opt.memory_budget < 4096) throw std::invalid_argument("server: memory budget too small");
+    m_per_view = m_opt.entry_cap;
+    m_pool = new Pool(m_opt.memory_budget);
+    hip_check(hipMalloc((void**)&m_base, m_pool->total()), "server arena");
     const int ncs = std::max(1, std::min(m_opt.compute_streams, m_opt.slots));
     for (int k = 0; k < ncs; ++k) {
         hipStream_t cs = nullptr;
@@ -84,14 +136,12 @@ Server::Server(std::vector<std::vector<rvs_source>> scenes, const rvs_serve_opts
         hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "stream create");
         hip_check(hipEventCreateWithFlags(&s->computed, hipEventDisableTiming), "event create");
         hip_check(hipEventCreateWithFlags(&s->done, hipEventDisableTiming), "event create");
-        hip_check(hipMalloc((void**)&s->arena, share), "server arena");
-        s->arena_bytes = share;
         if (m_opt.copy_to_host) {
-            s->pinned_bytes = m_opt.host_staging_bytes ? m_opt.host_staging_bytes
-                                                       : std::min(share / 4, (size_t)1 << 30);
+            s->pinned_bytes = m_opt.host_staging_bytes
+                                  ? m_opt.host_staging_bytes
+                                  : std::min(m_pool->total() / (4 * (size_t)m_opt.slots), (size_t)1 << 30);
             hip_check(hipHostMalloc((void**)&s->pinned, s->pinned_bytes, 0), "pinned staging");
         }
-        s->per_view = m_opt.entry_cap;
     }
     m_thread = std::thread([this] { loop(); });
 }
@@ -107,13 +157,14 @@ Server::~Server() {
     for (Slot* s : m_slots) {
         if (s->stream) (void)hipStreamSynchronize(s->stream);
         if (s->pinned) (void)hipHostFree(s->pinned);
-        if (s->arena) (void)hipFree(s->arena);
         if (s->done) (void)hipEventDestroy(s->done);
         if (s->computed) (void)hipEventDestroy(s->computed);
         if (s->stream) (void)hipStreamDestroy(s->stream);
         delete s;
     }
     for (void* cs : m_compute) (void)hipStreamDestroy((hipStream_t)cs);
+    if (m_base) (void)hipFree(m_base);
+    delete m_pool;
 }
 
 double Server::now_ms() const {
@@ -147,6 +198,20 @@ void Server::reset(const rvs_request* reqs, int n) {
 void Server::submit_locked(int k, double now) {
     m_stats[k].arrival_ms = now;
     m_pending.push_back(k);
+}
+
+// Every request ends here, served or failed: its outcome is recorded and, in a
+// closed loop, the next request is submitted in its place, so the number of
+// requests outstanding never shrinks.
+void Server::complete_locked(int k, int rc, double now, const std::string& msg) {
+    m_stats[k].rc = rc;
+    m_stats[k].done_ms = now;
+    if (rc != RVS_OK) {
+        m_error = rc;
+        m_errmsg = msg;
+    }
+    ++m_done;
+    if (m_closed && m_next_closed < m_n) submit_locked(m_next_closed++, now);
 }
 
 int Server::run_open(const rvs_request* reqs, int n, double rate, uint64_t seed) {
@@ -193,56 +258,105 @@ int Server::run_closed(const rvs_request* reqs, int n, int concurrency) {
     return m_error;
 }
 
-// Forms and enqueues the next batch on a free slot (lock held by the caller
-// only around the pending queue).
+// Device bytes of a batch: its YUV planes, colour maps and workspace; 0 with
+// rc set when the plan itself fails.
+size_t Server::batch_bytes(const std::vector<rvs_source>& S, const std::vector<rvs_target>& tgts, uint32_t cap,
+                           size_t* ws_out, int* rc_out) const {
+    size_t ws = 0;
+    const int rc = rvs_synth_plan(S.data(), (int)S.size(), tgts.data(), (int)tgts.size(), cap, &ws);
+    if (rc_out) *rc_out = rc;
+    if (rc != RVS_OK) return 0;
+    size_t h = 0, c = 0;
+    for (const rvs_target& t : tgts) {
+        h += yuv_bytes(t);
+        c += color_bytes(t);
+    }
+    if (ws_out) *ws_out = ws;
+    return align256(h) + c + ws;
+}
+
+// Forms and enqueues the next batch on a free slot.  Admission: the oldest
+// pending request fixes the scene; further pending requests of that scene
+// join in arrival order while the batch fits the largest free range of the
+// pool (and the slot's host staging).  A request that could not fit even the
+// whole budget, or whose plan is invalid, fails at once; one that only waits
+// for memory stays queued until a batch completes.
 bool Server::launch(Slot& s) {
     std::vector<int> batch;
     std::vector<rvs_target> tgts;
-    size_t ws = 0, colb = 0, hostb = 0;
+    size_t ws = 0, need = 0, hostb = 0;
     {
         std::lock_guard<std::mutex> lk(m_mu);
+        while (!m_pending.empty()) {
+            // requests that can never be served fail here, whatever else is in flight
+            const int k = m_pending.front();
+            const rvs_request& r = m_reqs[k];
+            const std::vector<rvs_target> one(1, r.target);
+            int prc = RVS_OK;
+            const size_t b1 = batch_bytes(m_scenes[r.scene], one, batch_cap(m_per_view, 1), nullptr, &prc);
+            std::string why;
+            int code = RVS_OK;
+            if (prc != RVS_OK) {
+                code = prc;
+                why = "request " + std::to_string(k) + ": " + rvs_last_error();
+            } else if (align256(b1) > m_pool->total()) {
+                code = RVS_E_CAPACITY;
+                why = "request " + std::to_string(k) + " needs " + std::to_string(b1) +
+                      " device bytes, more than the memory budget";
+            } else if ((r.target.cam.w & 1) || (r.target.cam.h & 1)) {
+                code = RVS_E_UNSUPPORTED;
+                why = "request " + std::to_string(k) + ": an odd target size cannot be encoded as YUV420";
+            } else if (r.target.bitdepth < 1 || r.target.bitdepth > 16) {
+                code = RVS_E_INVALID;
+                why = "request " + std::to_string(k) + ": output bit depth " + std::to_string(r.target.bitdepth);
+            } else if (s.pinned && yuv_bytes(r.target) > s.pinned_bytes) {
+                code = RVS_E_CAPACITY;
+                why = "request " + std::to_string(k) + " does not fit the host staging buffer";
+            }
+            if (code == RVS_OK) break;
+            m_pending.pop_front();
+            complete_locked(k, code, now_ms(), why);
+            m_cv.notify_all();
+        }
         if (m_pending.empty()) return false;
+        const size_t room = m_pool->largest();
         const int scene = m_reqs[m_pending.front()].scene;
         const std::vector<rvs_source>& S = m_scenes[scene];
         for (auto it = m_pending.begin(); it != m_pending.end() && (int)batch.size() < m_opt.max_batch;) {
             const rvs_request& r = m_reqs[*it];
             if (r.scene != scene) { ++it; continue; }
             tgts.push_back(r.target);
-            const size_t w2 = rvs_synth_batch_workspace_size(S.data(), (int)S.size(), tgts.data(), (int)tgts.size(),
-                                                             batch_cap(s.per_view, tgts.size()));
-            const size_t c2 = colb + color_bytes(r.target);
+            size_t w2 = 0;
+            int prc = RVS_OK;
+            const size_t n2 = batch_bytes(S, tgts, batch_cap(m_per_view, tgts.size()), &w2, &prc);
             const size_t h2 = hostb + yuv_bytes(r.target);
-            const bool fits = w2 > 0 && align256(h2) + c2 + w2 <= s.arena_bytes && (!s.pinned || h2 <= s.pinned_bytes);
-            if (!fits) {
+            if (prc != RVS_OK || n2 > room || (s.pinned && h2 > s.pinned_bytes)) {
+                // a batch too large (in memory, views or pixels) ends here; the
+                // request waits for the next batch
                 tgts.pop_back();
-                if (batch.empty()) {
-                    // cannot be served within one slot's share of the budget
-                    m_stats[*it].rc = RVS_E_CAPACITY;
-                    m_stats[*it].done_ms = now_ms();
-                    m_error = RVS_E_CAPACITY;
-                    m_errmsg = "request " + std::to_string(*it) + " does not fit one slot's share of the memory budget";
-                    ++m_done;
-                    it = m_pending.erase(it);
-                    m_cv.notify_all();
-                    continue;
-                }
                 break;
             }
-            ws = w2; colb = c2; hostb = h2;
+            ws = w2; need = n2; hostb = h2;
             batch.push_back(*it);
             it = m_pending.erase(it);
         }
-        if (batch.empty()) return false;
+        if (batch.empty()) return false;   // the oldest request waits for memory
+        s.alloc_off = m_pool->take(need);
+        if (s.alloc_off == SIZE_MAX) {     // cannot happen: need <= largest free range
+            for (auto it = batch.rbegin(); it != batch.rend(); ++it) m_pending.push_front(*it);
+            return false;
+        }
+        s.alloc_bytes = need;
+        s.arena = m_base + s.alloc_off;
         s.scene = scene;
     }
     const std::vector<rvs_source>& S = m_scenes[s.scene];
     s.reqs = batch;
     s.tgts = tgts;
-    s.ws_bytes = ws;
     s.outs.assign(batch.size(), rvs_view_out{});
     s.host_off.assign(batch.size(), 0);
     // the batch's YUV planes first, contiguous as in the host staging buffer;
-    // then the colour maps; the workspace after them (so it can grow in place)
+    // then the colour maps; the workspace after them
     size_t h = 0;
     for (size_t k = 0; k < batch.size(); ++k) {
         const rvs_target& t = tgts[k];
@@ -262,7 +376,7 @@ bool Server::launch(Slot& s) {
     }
     s.ws_off = o;
     const double t_launch = now_ms();
-    s.cap = batch_cap(s.per_view, tgts.size());
+    s.cap = batch_cap(m_per_view, tgts.size());
     hipStream_t cs = (hipStream_t)m_compute[s.index % m_compute.size()];
     int rc;
     if (m_opt.split_finish) {
@@ -281,6 +395,10 @@ bool Server::launch(Slot& s) {
         rc = RVS_E_HIP;
     if (rc == RVS_OK && hipEventRecord(s.done, s.stream) != hipSuccess) rc = RVS_E_HIP;
     const std::string msg = rc == RVS_OK ? std::string() : std::string(rvs_last_error());
+    if (rc != RVS_OK) {
+        (void)hipStreamSynchronize(cs);
+        (void)hipStreamSynchronize(s.stream);
+    }
     std::lock_guard<std::mutex> lk(m_mu);
     const int id = m_batches++;
     for (int k : batch) {
@@ -289,15 +407,9 @@ bool Server::launch(Slot& s) {
         m_stats[k].batch_views = (int)batch.size();
     }
     if (rc != RVS_OK) {
-        (void)hipStreamSynchronize(cs);
-        (void)hipStreamSynchronize(s.stream);
-        for (int k : batch) {
-            m_stats[k].rc = rc;
-            m_stats[k].done_ms = now_ms();
-            ++m_done;
-        }
-        m_error = rc;
-        m_errmsg = msg.empty() ? "HIP error while enqueueing a batch" : msg;
+        m_pool->give(s.alloc_off, s.alloc_bytes);
+        const double t = now_ms();
+        for (int k : batch) complete_locked(k, rc, t, msg.empty() ? "HIP error while enqueueing a batch" : msg);
         m_cv.notify_all();
         return true;
     }
@@ -305,48 +417,47 @@ bool Server::launch(Slot& s) {
     return true;
 }
 
-void Server::finish(Slot& s) {
+// A completed batch (query = what hipEventQuery returned for its done event).
+void Server::finish(Slot& s, int query_rc) {
+    const hipError_t query = (hipError_t)query_rc;
     const std::vector<rvs_source>& S = m_scenes[s.scene];
     uint32_t needed = 0;
-    int rc = rvs_synth_check(S.data(), (int)S.size(), s.tgts.data(), (int)s.tgts.size(), s.arena + s.ws_off, s.cap,
+    int rc;
+    std::string msg;
+    if (query != hipSuccess) {
+        rc = RVS_E_HIP;
+        msg = std::string("batch failed on the device: ") + hipGetErrorString(query);
+    } else {
+        rc = rvs_synth_check(S.data(), (int)S.size(), s.tgts.data(), (int)s.tgts.size(), s.arena + s.ws_off, s.cap,
                              &needed, s.stream);
-    if (rc == RVS_E_CAPACITY && needed > 0) {
-        // The entry list overflowed: raise this slot's per-view capacity and run
-        // the batch again (the workspace sits after the outputs and grows in place).
-        const uint64_t nv = s.tgts.size();
-        const uint64_t want = ((uint64_t)needed + needed / 8 + 1024 + nv - 1) / nv;
-        s.per_view = (uint32_t)(want > 0xffffffffull ? 0xffffffffull : want);
-        const uint32_t cap = batch_cap(s.per_view, s.tgts.size());
-        const size_t w2 = rvs_synth_batch_workspace_size(S.data(), (int)S.size(), s.tgts.data(),
-                                                         (int)s.tgts.size(), cap);
-        if (w2 > 0 && s.ws_off + w2 <= s.arena_bytes) {
-            rc = rvs_synthesize_batch(S.data(), (int)S.size(), s.tgts.data(), (int)s.tgts.size(), s.arena + s.ws_off,
-                                      w2, cap, s.outs.data(), nullptr, s.stream);
-            if (rc == RVS_OK && s.pinned) {
-                if (hipMemcpyAsync(s.pinned, s.arena, s.yuv_bytes, hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
-                    hipStreamSynchronize(s.stream) != hipSuccess)
-                    rc = RVS_E_HIP;
-            }
-        }
+        if (rc != RVS_OK) msg = rvs_last_error();
     }
-    const std::string msg = rc == RVS_OK ? std::string() : std::string(rvs_last_error());
     const double t = now_ms();
     {
         std::lock_guard<std::mutex> lk(m_mu);
-        if (rc != RVS_OK) m_errmsg = msg.empty() ? "HIP error in a batch" : msg;
-        for (size_t k = 0; k < s.reqs.size(); ++k) {
-            const int r = s.reqs[k];
-            m_stats[r].rc = rc;
-            m_stats[r].done_ms = t;
-            if (rc != RVS_OK) m_error = rc;
-            if (rc == RVS_OK && m_opt.keep_outputs && s.pinned) {
-                const uint8_t* src = s.pinned + s.host_off[k];
-                m_out[r].assign(src, src + yuv_bytes(s.tgts[k]));
-            }
-            ++m_done;
-            if (m_closed && m_next_closed < m_n) submit_locked(m_next_closed++, t);
-        }
+        m_pool->give(s.alloc_off, s.alloc_bytes);
         s.busy = false;
+        if (rc == RVS_E_CAPACITY && needed > 0 && needed != 0xffffffffu) {
+            // The entry list overflowed: raise the per-view capacity and queue the
+            // batch's requests again at the front (arrival times unchanged); their
+            // next launch plans with the larger list, and a request that then no
+            // longer fits the budget fails at admission.
+            const uint64_t nv = s.tgts.size();
+            const uint64_t want = ((uint64_t)needed + needed / 8 + 1024 + nv - 1) / nv;
+            const uint32_t pv = (uint32_t)(want > 0xffffffffull ? 0xffffffffull : want);
+            if (pv > m_per_view) m_per_view = pv;
+            for (auto it = s.reqs.rbegin(); it != s.reqs.rend(); ++it) m_pending.push_front(*it);
+            ++m_requeued;
+        } else {
+            for (size_t k = 0; k < s.reqs.size(); ++k) {
+                const int r = s.reqs[k];
+                if (rc == RVS_OK && m_opt.keep_outputs && s.pinned) {
+                    const uint8_t* src = s.pinned + s.host_off[k];
+                    m_out[r].assign(src, src + yuv_bytes(s.tgts[k]));
+                }
+                complete_locked(r, rc, t, msg.empty() ? "error in a batch" : msg);
+            }
+        }
     }
     m_cv.notify_all();
 }
@@ -356,10 +467,11 @@ void Server::loop() {
     while (true) {
         bool progressed = false;
         for (Slot* s : m_slots) {
-            if (s->busy && hipEventQuery(s->done) == hipSuccess) {
-                finish(*s);
-                progressed = true;
-            }
+            if (!s->busy) continue;
+            const hipError_t q = hipEventQuery(s->done);
+            if (q == hipErrorNotReady) continue;
+            finish(*s, q);
+            progressed = true;
         }
         for (Slot* s : m_slots) {
             if (!s->busy && launch(*s)) progressed = true;

@@ -89,7 +89,9 @@ class Server:
         st = (RvsRequestStats * len(reqs))()
         rc = fn(self.h, arr, len(reqs), *args, st)
         if rc != 0:
-            raise R.RvsError(rc, R.host().rvs_host_last_error().decode())
+            err = R.RvsError(rc, R.host().rvs_host_last_error().decode())
+            err.stats = _stats_dict(st)   # per-request outcomes of the failed run
+            raise err
         return _stats_dict(st)
 
     def run_open(self, reqs, rate, seed=4):

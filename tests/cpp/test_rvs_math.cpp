@@ -24,6 +24,23 @@ static bool same(float x, float y) {
     return a == b;
 }
 
+// valid_tri's stretch as the reference computes it (transform.cpp:62-68):
+// cv::norm of Vec2f differences in double, std::max of the doubles, then float.
+static double ref_norm2(float a0, float a1, float b0, float b1) {
+    double s = 0.0;
+    double v = (double)(a0 - b0);
+    s += v * v;
+    v = (double)(a1 - b1);
+    s += v * v;
+    return std::sqrt(s);
+}
+
+static float ref_stretch(float A0, float A1, float B0, float B1, float C0, float C1) {
+    const double dab = ref_norm2(A0, A1, B0, B1), dac = ref_norm2(A0, A1, C0, C1), dbc = ref_norm2(B0, B1, C0, C1);
+    const double m = (dab < dac) ? dac : dab;
+    return (float)((dbc < m) ? m : dbc);
+}
+
 int main(int argc, char** argv) {
     long n = argc > 1 ? atol(argv[1]) : 2000000;
     std::mt19937_64 rng(12345);
@@ -63,6 +80,53 @@ int main(int argc, char** argv) {
     }
     const float specials[] = {0.f, -0.f, INFINITY, -INFINITY, NAN, 1e-45f, 1e-40f, 3e38f, 1e-30f, 1.f};
     for (float a : specials) for (float b : specials) for (float c : specials) check(a, b, c);
+    // triangle stretch: the fast path (float-float lengths, one root) where it
+    // decides, the three exact norms otherwise, against double arithmetic
+    long sbad = 0, undecided = 0, tris = 0;
+    auto tri = [&](float A0, float A1, float B0, float B1, float C0, float C1) {
+        ++tris;
+        const float r = ref_stretch(A0, A1, B0, B1, C0, C1);
+        float f;
+        if (rvs_math::tri_stretch_fast(A0, A1, B0, B1, C0, C1, f)) {
+            if (!same(r, f)) { if (sbad < 5) printf("stretch fast mismatch %a %a %a %a %a %a: ref %a got %a\n", A0, A1, B0, B1, C0, C1, r, f); ++sbad; }
+        } else {
+            ++undecided;
+        }
+        const float g = rvs_math::tri_stretch(A0, A1, B0, B1, C0, C1);
+        if (!same(r, g)) { if (sbad < 5) printf("stretch mismatch %a %a %a %a %a %a: ref %a got %a\n", A0, A1, B0, B1, C0, C1, r, g); ++sbad; }
+        const float qr = rvs_math::tri_quality(r, 1.0f), qg = rvs_math::tri_quality(g, 1.0f);
+        if (!same(qr, qg)) ++sbad;
+    };
+    std::uniform_real_distribution<float> P(0.f, 2000.f);
+    std::uniform_real_distribution<float> J(-3.f, 3.f);
+    for (long i = 0; i < n; ++i) {
+        // neighbouring projected vertices: small triangles at image coordinates
+        const float x = P(rng), y = P(rng);
+        tri(x, y, x + J(rng), y + J(rng), x + J(rng), y + J(rng));
+        // grid-aligned corners (exact halves and integers: equal edges, ties in the max)
+        const float gx = (float)(rng() % 4096) + 0.5f, gy = (float)(rng() % 2048) + 0.5f;
+        tri(gx, gy, gx + 1.f, gy, gx, gy + 1.f);
+        tri(gx + (float)(rng() % 5), gy, gx, gy + (float)(rng() % 5), gx + (float)(rng() % 3), gy + (float)(rng() % 3));
+        // wide range of scales and equal-length edges in different orders
+        const float e1 = S(rng) * std::ldexp(1.f, E(rng) / 3), e2 = S(rng) * std::ldexp(1.f, E(rng) / 3);
+        tri(0.f, 0.f, e1, e2, e2, e1);
+        tri(e1, e2, 0.f, 0.f, -e2, e1);
+    }
+    // integer legs whose hypotenuse lands near a float midpoint, as an edge of a triangle
+    for (long i = 0; i < n / 4; ++i) {
+        const float a = (float)(1 + rng() % 16777215);
+        const float ulp = std::nextafter(a, 1e30f) - a;
+        const float b = std::sqrt(a * ulp) * (1.0f + (S(rng) * 1e-6f));
+        tri(0.f, 0.f, a, b, a * 0.5f, b * 0.25f);
+        tri(a * 0.5f, b * 0.25f, 0.f, 0.f, a, b);
+    }
+    // non-finite and extreme vertices (NaN lengths are passed over or kept as std::max does)
+    const float sv[] = {0.f, 1.f, -2.5f, 1e-30f, 1e30f, INFINITY, -INFINITY, NAN, 3e38f};
+    for (float a : sv) for (float b : sv) for (float c : sv) for (float d : sv) {
+        tri(a, 1.f, b, 2.f, c, d);
+        tri(1.f, a, 2.f, b, d, c);
+    }
     printf("checked %ld random + %ld steered: fast mismatches %ld, slow mismatches %ld\n", 4 * n, 2 * near, bad, slow_bad);
-    return (bad || slow_bad) ? 1 : 0;
+    printf("stretch: %ld triangles, %ld mismatches, %ld undecided by the fast path\n", tris, sbad, undecided);
+    return (bad || slow_bad || sbad) ? 1 : 0;
 }

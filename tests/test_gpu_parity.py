@@ -221,6 +221,24 @@ def test_raster_random_triangles(torch_cuda):
         _raster_case(torch, col, depth, uv, 80, 64, 1 if case == 3 else 0, "random %d" % case)
 
 
+def test_raster_dense_bin_beyond_the_replay_buffer(torch_cuda):
+    """48,705 source quads whose vertices all land in one 16x16 target tile:
+    the tile's bin holds every quad (far beyond the 8,192 entries the in-order
+    replay sorts in LDS), most pixels have thousands of candidates (up to ~12k,
+    more than the replay's per-pixel lists hold), and the far (2^21) triangles
+    of the first two source rows cover most pixels with depths the raster's
+    score cannot order, so those pixels take the replay's sorted in-order scan
+    of the whole bin.  Bit-exact against the oracle's sequential fold."""
+    torch = torch_cuda
+    rng = np.random.default_rng(21)
+    ih, iw = 192, 256
+    uv = np.stack([rng.uniform(0.01, 16, (ih, iw)), rng.uniform(0, 16, (ih, iw))], -1).astype(np.float32)
+    depth = rng.uniform(1, 4, (ih, iw)).astype(np.float32)
+    depth[:2] = 2.0 ** 21
+    col = rng.random((ih, iw, 3)).astype(np.float32)
+    _raster_case(torch, col, depth, uv, 16, 16, 0, "dense bin, 48705 quads in one tile")
+
+
 # ------------------------------------------------------------------- blend
 
 @pytest.mark.parametrize("exp", [1.0, -1.0, 5.0])

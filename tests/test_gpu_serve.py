@@ -83,6 +83,52 @@ def test_budget_rejects_request_that_cannot_fit(torch_cuda):
         srv.close()
 
 
+def test_closed_loop_keeps_its_window_after_rejections(torch_cuda):
+    """Concurrency 1 with an oversized request and an odd-sized one in the
+    middle: each fails with its own code, and every later request is still
+    submitted and served (a rejected request must not shrink the closed loop's
+    window)."""
+    say("case serve closed loop at concurrency 1 with rejected requests")
+    rg_a, src_a, rg_b, src_b = _scenes()
+    big = W.batch_targets(rg_a, 1, seed=2, scale=1, sizes=((3840, 2160),))[0]
+    odd = W.device_target(rg_b)
+    odd.cam.w -= 1
+    small = [W.device_target(rg_b, p, r) for p, r in W.random_walk(3, seed=13, pos_lim=0.04, yaw_lim=6.0,
+                                                                   pitch_lim=3.0)]
+    reqs = [(1, small[0]), (0, big), (1, small[1]), (1, odd), (1, small[2])]
+    srv = S.Server([src_a, src_b], memory_budget=64 << 20, max_batch=4, slots=1, keep_outputs=True)
+    try:
+        with pytest.raises(R.RvsError) as ei:
+            srv.run_closed(reqs, concurrency=1)
+        rc = list(ei.value.stats["rc"])
+        say("  rc per request", rc)
+        assert rc == [0, R.RVS_E_CAPACITY, 0, R.RVS_E_UNSUPPORTED, 0]
+        for k in (0, 2, 4):
+            assert srv.output(k) == _yuv_alone(src_b, reqs[k][1]), "request %d" % k
+        # the server stays usable
+        st = srv.run_closed([reqs[0], reqs[2]], concurrency=1)
+        assert (st["rc"] == 0).all()
+    finally:
+        srv.close()
+
+
+def test_pool_admits_a_request_larger_than_a_slot_share(torch_cuda):
+    """Slots share one memory arena: a request that needs most of the budget is
+    admitted once the device is free, even with several slots."""
+    say("case serve pooled budget, 3 slots, one large request")
+    rg_a, src_a, rg_b, src_b = _scenes()
+    big = W.batch_targets(rg_a, 1, seed=2, scale=2, sizes=((1920, 1080),))[0]
+    small = W.device_target(rg_b)
+    srv = S.Server([src_a, src_b], memory_budget=160 << 20, max_batch=4, slots=3, keep_outputs=True,
+                   host_staging_bytes=8 << 20)
+    try:
+        st = srv.run_closed([(1, small), (0, big), (1, small)], concurrency=3)
+        assert (st["rc"] == 0).all()
+        assert srv.output(1) == _yuv_alone(src_a, big)
+    finally:
+        srv.close()
+
+
 @pytest.mark.parametrize("split", [False, True])
 def test_served_erp11_scenes_equal_alone(torch_cuda, split):
     say("case serve 2 scenes x 11 ERP sources, ERP + perspective targets, closed loop")

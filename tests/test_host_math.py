@@ -1,6 +1,7 @@
 """CPU: rvs_math.h (shared by the host library and the kernels) emulates the
-reference's double-accumulated cv::norm bit for bit; checked against real
-double arithmetic on random, midpoint-steered and special inputs."""
+reference's double-accumulated cv::norm bit for bit, and valid_tri's maximum of
+three such norms, without FP64; checked against real double arithmetic on
+random, midpoint-steered and special inputs."""
 import os
 import subprocess
 
@@ -14,3 +15,4 @@ def test_norm_emulation_matches_double(tmp_path):
     r = subprocess.run([exe, "300000"], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "fast mismatches 0, slow mismatches 0" in r.stdout
+    assert " 0 mismatches" in r.stdout   # triangle stretch (valid_tri's max of three double norms)
