@@ -254,6 +254,22 @@ int rvs_synth_check(const rvs_source* srcs, int n_src, const rvs_target* tgts, i
                     const void* workspace, uint32_t entry_cap, uint32_t* needed_cap,
                     rvs_stream_t stream);
 
+/* rvs_synthesize without a caller workspace and without allocating: the
+ * workspace is a static device array of the library (RVS_NOALLOC_SCRATCH_BYTES,
+ * 128 MiB, held on every GPU the library runs on until the process ends),
+ * whose device address is looked up once per device on the first call there.
+ * No counterpart in the reference (its wrappers cudaMalloc per call): for
+ * callers that cannot allocate device memory (tight serving loops, harnesses
+ * that forbid it).  Same arguments and results as rvs_synthesize; the binned
+ * vertex windows kept between passes are limited to what fits (the rest are
+ * re-evaluated, with the same results).  A view whose fixed workspace exceeds
+ * the scratch is refused with RVS_E_CAPACITY and a message giving the bytes
+ * it needs.  Synchronises the stream.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (other threads or streams) race. */
+#define RVS_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int rvs_synthesize_noalloc(const rvs_source* srcs, int n_src, const rvs_target* tgt, uint32_t entry_cap,
+                           const rvs_view_out* out, uint32_t* needed_cap, rvs_stream_t stream);
+
 /* Per-stage device times of the last rvs_synthesize* call on the calling
  * thread, measured with HIP events recorded on its stream between the stages
  * when enabled: [0] binning (incl. culling tables built per call, culling),

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/rvs_hip.h"
@@ -407,6 +408,58 @@ int enqueue_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int
     return RVS_OK;
 }
 
+// ------------------------------------------------- static device scratch
+
+// Workspace of rvs_synthesize_noalloc: one static array per GPU, loaded with
+// the library's code object, so that entry point allocates nothing.  In host
+// code `scratch` names a host-side stand-in, never the device memory: every
+// host use goes through the device address from hipGetSymbolAddress, taken
+// once per device with that device current.
+constexpr size_t kNoallocScratchBytes = RVS_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(void** out) {
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(RVS_E_INVALID, "rvs_synthesize_noalloc: device %d", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "scratch address");
+        if (!p) return fail(RVS_E_HIP, "scratch address: null");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return RVS_OK;
+}
+
+// Kept work items fitted to the static scratch: the plan with a single kept
+// item gives the fixed part; what remains holds this many kept items (their
+// tile ranges and vertex windows).  0 when even the fixed part does not fit.
+uint64_t noalloc_keep(const rvs_source* srcs, int n_src, const rvs_target* tgt, uint32_t cap, size_t& need) {
+    SynthPlan P;
+    if (plan_synth(srcs, n_src, tgt, 1, cap, P, nullptr) != RVS_OK) return 0;
+    if (P.total <= kNoallocScratchBytes) { need = P.total; return P.R.d.range_cap ? P.R.d.range_cap : 1; }
+    if (plan_synth(srcs, n_src, tgt, 1, cap, P, nullptr, 1) != RVS_OK) return 0;
+    need = P.total;
+    if (P.total > kNoallocScratchBytes) return 0;
+    const size_t per_item = 256 * sizeof(uint2) + 3 * (size_t)rvs_k::kBinVerts * sizeof(float);
+    const uint64_t keep = 1 + (kNoallocScratchBytes - P.total) / per_item;
+    // alignment of the two kept arrays can cost up to 2 x 255 bytes: step back until it fits
+    for (uint64_t k = keep; k >= 1; k = k > 64 ? k - 64 : k - 1) {
+        if (plan_synth(srcs, n_src, tgt, 1, cap, P, nullptr, k) == RVS_OK && P.total <= kNoallocScratchBytes) {
+            need = P.total;
+            return k;
+        }
+        if (k == 1) break;
+    }
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -691,6 +744,31 @@ int rvs_synthesize_batch(const rvs_source* srcs, int n_src, const rvs_target* tg
 int rvs_synthesize(const rvs_source* srcs, int n_src, const rvs_target* tgt, void* workspace, size_t ws_bytes,
                    uint32_t entry_cap, const rvs_view_out* out, uint32_t* needed_cap, rvs_stream_t stream) {
     return rvs_synthesize_batch(srcs, n_src, tgt, 1, workspace, ws_bytes, entry_cap, out, needed_cap, stream);
+}
+
+int rvs_synthesize_noalloc(const rvs_source* srcs, int n_src, const rvs_target* tgt, uint32_t entry_cap,
+                           const rvs_view_out* out, uint32_t* needed_cap, rvs_stream_t stream) {
+    SynthPlan P;
+    const int prc = plan_synth(srcs, n_src, tgt, 1, entry_cap, P, nullptr);
+    if (prc != RVS_OK) return prc;
+    size_t need = 0;
+    const uint64_t keep = noalloc_keep(srcs, n_src, tgt, entry_cap, need);
+    if (keep == 0)
+        return fail(RVS_E_CAPACITY,
+                    "rvs_synthesize_noalloc: this view needs at least %zu workspace bytes, more than the %zu-byte "
+                    "static scratch (use rvs_synthesize with a caller workspace)",
+                    need, kNoallocScratchBytes);
+    void* ws_ptr = nullptr;
+    const int src = scratch_address(&ws_ptr);
+    if (src != RVS_OK) return src;
+    rvs_k::RasterWS ws;
+    g_prof_valid = false;
+    const int rc = enqueue_synth(srcs, n_src, tgt, 1, ws_ptr, kNoallocScratchBytes, entry_cap, out,
+                                 (hipStream_t)stream, P, ws, nullptr, keep);
+    if (rc != RVS_OK) return rc;
+    const int crc = check_raster_status(ws, P.R.d, needed_cap, (hipStream_t)stream);
+    if (crc == RVS_OK && g_prof) { g_prof_valid = true; g_prof_ws = ws; g_prof_dims = P.R.d; }
+    return crc;
 }
 
 int rvs_profile_counters(uint32_t out[RVS_PROF_COUNTERS]) {

@@ -98,6 +98,7 @@ def hip():
                                      i),
             "rvs_synthesize_batch_async": ([P(RvsSource), i, P(RvsTarget), i, vp, sz, u32, P(RvsViewOut), vp], i),
             "rvs_synth_check": ([P(RvsSource), i, P(RvsTarget), i, vp, u32, P(u32), vp], i),
+            "rvs_synthesize_noalloc": ([P(RvsSource), i, P(RvsTarget), u32, P(RvsViewOut), P(u32), vp], i),
             "rvs_profile_enable": ([i], i),
             "rvs_profile_read": ([vp], i),
             "rvs_profile_counters": ([vp], i),
@@ -427,6 +428,24 @@ class Synthesizer:
             return self.run(stream)
         check(rc)
         return self.out
+
+
+def synthesize_noalloc(sources, target, out, entry_cap=0, stream=None):
+    """rvs_synthesize_noalloc: one view with the library's static device
+    scratch as workspace (no allocation).  out: dict of output tensors as
+    Synthesizer.out (None entries allowed except colour).  Returns the needed
+    entry capacity on RVS_E_CAPACITY (list overflow), else 0."""
+    arr = (RvsSource * len(sources))(*[s.struct() for s in sources])
+    vo = RvsViewOut()
+    for k in ("color", "quality", "depth", "validity", "y", "u", "v"):
+        setattr(vo, k, out[k].data_ptr() if out.get(k) is not None else None)
+    needed = C.c_uint32(0)
+    rc = hip().rvs_synthesize_noalloc(arr, len(sources), C.byref(target), entry_cap, C.byref(vo), C.byref(needed),
+                                      _stream(stream))
+    if rc == RVS_E_CAPACITY and needed.value > 0:
+        return int(needed.value)
+    check(rc)
+    return 0
 
 
 def grow_cap(needed):
