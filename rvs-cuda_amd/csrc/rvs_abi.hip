@@ -125,7 +125,7 @@ struct Layout {
 // deferred-pixel flags, deferred-tile list and binning work list.
 struct RasterPlan {
     rvs_k::BatchDims d;
-    size_t status, counts, single, multi, offsets, fill, lists, ranges, vbuf, imap, bsum, deferred, dtiles, work;
+    size_t status, counts, single, multi, offsets, fill, lists, ranges, vbuf, tvq, imap, bsum, deferred, dtiles, work;
 };
 
 // List capacity: the caller's, or one entry per output pixel and source
@@ -145,7 +145,7 @@ void plan_raster(Layout& L, RasterPlan& R, int nsrc, int nviews, uint32_t ntiles
     d.bins_x = (uint32_t)((max_sw - 1 + rvs_k::kBinQX - 1) / rvs_k::kBinQX);
     d.bins_y = (uint32_t)((max_sh - 1 + rvs_k::kBinQY - 1) / rvs_k::kBinQY);
     const uint64_t items = (uint64_t)d.bins_x * d.bins_y * nsrc * nviews;
-    // Work items that keep their tile ranges and vertex window (5.9 KB each):
+    // Work items that keep their tile ranges, vertex window and quad validities (7.9 KB each):
     // at least an eighth of the items before culling, and enough for the entry
     // capacity (a surviving item bins ~300 entries in the measured scenes, so
     // capacity / 240 items cover a launch set that fills it); the rest
@@ -172,6 +172,7 @@ void plan_raster(Layout& L, RasterPlan& R, int nsrc, int nviews, uint32_t ntiles
     R.lists = L.take((size_t)d.cap * sizeof(uint32_t));
     R.ranges = L.take((size_t)d.range_cap * 256 * sizeof(uint2));
     R.vbuf = L.take((size_t)d.range_cap * 3 * rvs_k::kBinVerts * sizeof(float));
+    R.tvq = L.take((size_t)d.range_cap * 256 * sizeof(float2));
     R.imap = L.take((size_t)items * sizeof(uint32_t));
     R.deferred = L.take(npx);
     R.dtiles = L.take((size_t)ntiles * sizeof(uint32_t));
@@ -190,6 +191,7 @@ rvs_k::RasterWS carve_raster(char* base, const RasterPlan& R) {
     ws.lists = (uint32_t*)(base + R.lists);
     ws.ranges = (uint2*)(base + R.ranges);
     ws.vbuf = (float*)(base + R.vbuf);
+    ws.tvq = (float2*)(base + R.tvq);
     ws.imap = (uint32_t*)(base + R.imap);
     ws.deferred = (uint8_t*)(base + R.deferred);
     ws.deferred_tiles = (uint32_t*)(base + R.dtiles);
@@ -447,9 +449,9 @@ uint64_t noalloc_keep(const rvs_source* srcs, int n_src, const rvs_target* tgt, 
     if (plan_synth(srcs, n_src, tgt, 1, cap, P, nullptr, 1) != RVS_OK) return 0;
     need = P.total;
     if (P.total > kNoallocScratchBytes) return 0;
-    const size_t per_item = 256 * sizeof(uint2) + 3 * (size_t)rvs_k::kBinVerts * sizeof(float);
+    const size_t per_item = 256 * sizeof(uint2) + 3 * (size_t)rvs_k::kBinVerts * sizeof(float) + 256 * sizeof(float2);
     const uint64_t keep = 1 + (kNoallocScratchBytes - P.total) / per_item;
-    // alignment of the two kept arrays can cost up to 2 x 255 bytes: step back until it fits
+    // alignment of the three kept arrays can cost up to 3 x 255 bytes: step back until it fits
     for (uint64_t k = keep; k >= 1; k = k > 64 ? k - 64 : k - 1) {
         if (plan_synth(srcs, n_src, tgt, 1, cap, P, nullptr, k) == RVS_OK && P.total <= kNoallocScratchBytes) {
             need = P.total;

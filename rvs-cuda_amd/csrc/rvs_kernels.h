@@ -58,6 +58,7 @@ struct RasterWS {
     uint32_t* lists;         // [cap] quad codes (row << 16 | column), bins back to back
     uint2* ranges;           // [range_cap work items][256] packed tile range + single-tile slot
     float* vbuf;             // [range_cap work items][kBinQY + 1][kBinQX + 1][3] vertices (u', v', depth)
+    float2* tvq;             // [range_cap work items][256] validity of triangles T1, T2 of each quad
     uint32_t* imap;          // [view][source][item] work index of a kept item, ~0 otherwise
     unsigned long long* bsum;// scan block sums
     uint8_t* deferred;       // [global pixel] pixel handled by the in-order replay

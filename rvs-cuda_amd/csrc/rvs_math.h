@@ -92,6 +92,33 @@ RVS_HD float ffma(float a, float b, float c) {
 }
 RVS_HD float fabs_(float x) { return u2f(f2u(x) & 0x7fffffffu); }
 
+// Square root and reciprocal for the float-float norms below.  The decision
+// there is exact for any root within one ulp (it is corrected from its fma
+// residual; a correction of a whole ulp or more is reported undecided), but
+// the correctly rounded root keeps undecided cases rare: the device's bare
+// v_sqrt_f32 left ~1e-2 of the triangles of a real scene undecided, so both
+// sides use the correctly rounded root.  The reciprocal only scales a
+// correction compared with a tolerance: v_rcp_f32 (one ulp) on the device.  A
+// test build may substitute perturbed versions (RVS_MATH_TEST_ROOT /
+// RVS_MATH_TEST_RCP) to check that any faithful root and reciprocal give the
+// same results.
+RVS_HD float ff_root(float x) {
+#if defined(RVS_MATH_TEST_ROOT)
+    return RVS_MATH_TEST_ROOT(x);
+#else
+    return fsqrt(x);
+#endif
+}
+RVS_HD float ff_rcp(float x) {
+#if defined(RVS_MATH_TEST_RCP)
+    return RVS_MATH_TEST_RCP(x);
+#elif defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
 // float -> uint8/uint16 as MSVC emits it (32-bit cvttss2si then truncation)
 RVS_HD uint32_t quantize_u32(float v, unsigned maxv) { return (uint32_t)cvt_i32(v * (float)maxv); }
 
@@ -287,27 +314,30 @@ RVS_HD void sq_sum2_ff(float aa, float ab, float& shi, float& slo) {
     slo = lo - (shi - s1);
 }
 
-// RN24(sqrt(shi + slo)) for shi > 0, decided when the root lies more than
-// 2^-44 (relative) from a float rounding midpoint -- far beyond the error of
-// shi + slo and of the reference's two double roundings (sum and root), so
-// the reference's (float) sqrt of the double sum is that same float.  Returns
-// false (undecided) otherwise.
+// RN24(sqrt(shi + slo)) for shi > 0 (normal), decided when the root lies more
+// than 2^-44 (relative) from a float rounding midpoint -- far beyond the error
+// of shi + slo and of the reference's two double roundings (sum and root), so
+// the reference's (float) sqrt of the double sum is that same float.  r is any
+// root within one ulp; rlo ~ (sqrt - r) from the exact fma residual.  Returns
+// false (undecided) otherwise, or when the correction reaches a whole ulp.
 RVS_HD bool sqrt_ff_rn24(float shi, float slo, float& out) {
-    const float r = fsqrt(shi);
+    const float r = ff_root(shi);
     const float resid = ffma(-r, r, shi);       // exact
-    const float rlo = (resid + slo) / (2.0f * r);
+    const float rlo = (resid + slo) * (0.5f * ff_rcp(r));
     const float tol = r * 5.684342e-14f;        // 2^-44 relative
     const uint32_t rb = f2u(r);
     if (rlo > 0.0f) {
         const float up = u2f(rb + 1u);
-        const float half = (up - r) * 0.5f;
-        const float d = rlo - half;
+        const float ulp = up - r;
+        const float d = rlo - ulp * 0.5f;
+        if (rlo >= ulp) return false;
         if (d > tol) { out = up; return true; }
         if (d < -tol) { out = r; return true; }
     } else if (rlo < 0.0f) {
         const float dn = u2f(rb - 1u);
-        const float half = (r - dn) * 0.5f;
-        const float d = -rlo - half;
+        const float ulp = r - dn;
+        const float d = -rlo - ulp * 0.5f;
+        if (-rlo >= ulp) return false;
         if (d > tol) { out = dn; return true; }
         if (d < -tol) { out = r; return true; }
     } else {
@@ -411,7 +441,7 @@ RVS_HD float tri_stretch(float A0, float A1, float B0, float B1, float C0, float
 // valid_tri's quality from the stretch (transform.cpp:68-74).  The clamp's
 // max(1, q) maps NaN to 1, so the result is never 0.
 RVS_HD float tri_quality(float stretch, float rescale) {
-    stretch /= rescale;
+    if (rescale != 1.0f) stretch /= rescale;   // x / 1 = x for every x: the division is skipped, not changed
     float quality = 10000.f - 1000.f * stretch;
     quality = smax(1.f, quality);
     quality = smin(10000.f, quality);

@@ -125,6 +125,7 @@ template <class G, class VF>
 __device__ inline void quad_build(Quad& Q, const G& g, int i, int j, VF vtx, int out_w, int out_h,
                                   bool validity = true) {
     Q.alive = 0; Q.prol = 0;
+    Q.t[0].tv = 0.f; Q.t[1].tv = 0.f;
     const float3 v00 = vtx(i, j), v01 = vtx(i, j + 1), v10 = vtx(i + 1, j), v11 = vtx(i + 1, j + 1);
     if (!(v01.z > 0.f && v10.z > 0.f && v01.x > 0.f && v10.x > 0.f)) return;
     if (v00.z > 0.f && v00.x > 0.f) quad_add(Q, 0, v00, v01, v10, out_w, out_h, g.rescale, validity);
@@ -134,6 +135,43 @@ __device__ inline void quad_build(Quad& Q, const G& g, int i, int j, VF vtx, int
         quad_add(Q, 2, l0, v00, l1, out_w, out_h, g.rescale, validity);
         quad_add(Q, 3, v10, l1, v00, out_w, out_h, g.rescale, validity);
     }
+}
+
+// Tile range [tx0, tx1] x [ty0, ty1] of the bounding boxes of the triangles of
+// source quad (i, j) that survive the gating and colorize_triangle's early
+// returns (the union quad_build + quad_tiles compute), one triangle at a time;
+// with tv, also the validities of T1 and T2 (0 when not alive or left
+// undecided by the fast path).  False when none survives.
+template <class G, class VF>
+__device__ inline bool quad_range_tv(const G& g, int i, int j, VF vtx, int out_w, int out_h, int& tx0, int& tx1,
+                                     int& ty0, int& ty1, float2* tv) {
+    if (tv) *tv = make_float2(0.f, 0.f);
+    const float3 v01 = vtx(i, j + 1), v10 = vtx(i + 1, j);
+    if (!(v01.z > 0.f && v10.z > 0.f && v01.x > 0.f && v10.x > 0.f)) return false;
+    int xmin = INT_MAX, xmax = INT_MIN, ymin = INT_MAX, ymax = INT_MIN;
+    auto one = [&](float3 a, float3 b, float3 c, float* tvo) {
+        Tri T;
+        if (!tri_setup_geom(T, a, b, c, out_w, out_h)) return;
+        xmin = min(xmin, T.xmin); xmax = max(xmax, T.xmax);
+        ymin = min(ymin, T.ymin); ymax = max(ymax, T.ymax);
+        // fast path only: an undecided validity stays 0 (unknown), and the
+        // raster then defers the pixels that depend on it
+        if (tvo && !tri_validity_fast(T, g.rescale, *tvo)) *tvo = 0.f;
+    };
+    const float3 v00 = vtx(i, j);
+    if (v00.z > 0.f && v00.x > 0.f) one(v00, v01, v10, tv ? &tv->x : nullptr);
+    {
+        const float3 v11 = vtx(i + 1, j + 1);
+        if (v11.z > 0.f && v11.x > 0.f) one(v11, v10, v01, tv ? &tv->y : nullptr);
+    }
+    if (g.wrap && j == 0) {
+        const float3 l0 = vtx(i, g.sw - 1), l1 = vtx(i + 1, g.sw - 1);
+        one(l0, v00, l1, nullptr);
+        one(v10, l1, v00, nullptr);
+    }
+    if (xmin > xmax) return false;
+    tx0 = xmin / kTile; tx1 = xmax / kTile; ty0 = ymin / kTile; ty1 = ymax / kTile;
+    return true;
 }
 
 // Calls fn(slot, triangle, prol) for every triangle of source quad (i, j)
@@ -344,11 +382,22 @@ __device__ inline float3 kept_vertex(const float* vb, int lr, int lc) {
     return make_float3(p[0], p[1], p[2]);
 }
 
+// Work index of the kept work item holding quad (i, j) of pair vs, or ~0.
+__device__ inline uint32_t kept_index(const RasterWS& ws, const BatchDims& d, uint32_t vs, int i, int j) {
+    const uint32_t item = (uint32_t)(i / kBinQY) * d.bins_x + (uint32_t)(j / kBinQX);
+    return ws.imap[(size_t)vs * (d.bins_x * d.bins_y) + item];
+}
+
 // Vertex block of quad (i, j) of pair vs when its work item was kept, else null.
 __device__ inline const float* kept_block(const RasterWS& ws, const BatchDims& d, uint32_t vs, int i, int j) {
-    const uint32_t item = (uint32_t)(i / kBinQY) * d.bins_x + (uint32_t)(j / kBinQX);
-    const uint32_t wi = ws.imap[(size_t)vs * (d.bins_x * d.bins_y) + item];
+    const uint32_t wi = kept_index(ws, d, vs, i, j);
     return wi == 0xffffffffu ? nullptr : ws.vbuf + (size_t)wi * (3 * kBinVerts);
+}
+
+// Validities of triangles T1, T2 of quad (i, j) of kept item wi (the counting
+// pass evaluated them exactly; only alive triangles' values are meaningful).
+__device__ inline float2 kept_validity(const RasterWS& ws, uint32_t wi, int i, int j) {
+    return ws.tvq[(size_t)wi * 256 + (uint32_t)((i % kBinQY) * kBinQX + (j % kBinQX))];
 }
 
 constexpr int kBinTable = 128;                 // LDS bin table entries per block (power of two)
@@ -482,10 +531,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
         }
         bool more = false;
         int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
+        // a kept item also records its quads' triangle validities for the raster
+        // (exact; this pass waits on memory far more than it computes)
         auto quad_range = [&](auto vtx) {
-            Quad Q;
-            quad_build(Q, g, i, j, vtx, vd.w, vd.h, false);
-            more = quad_tiles(Q, tx0, tx1, ty0, ty1);
+            if (stored) {
+                float2 tv;
+                more = quad_range_tv(g, i, j, vtx, vd.w, vd.h, tx0, tx1, ty0, ty1, &tv);
+                ws.tvq[(size_t)wi * 256 + tid] = tv;
+            } else {
+                more = quad_range_tv(g, i, j, vtx, vd.w, vd.h, tx0, tx1, ty0, ty1, nullptr);
+            }
         };
         {
             float* vb = stored ? ws.vbuf + (size_t)wi * (3 * kBinVerts) : nullptr;
@@ -847,7 +902,7 @@ constexpr uint32_t kDeferKey = 0xffffffffu;
 // no pixel is never a winner, so skipping it changes nothing).
 __device__ __forceinline__ float raster_tri(float3 a, float3 b, float3 c, uint32_t id, bool prol_false_possible, int tx0,
                                    int ty0, int out_w, int out_h, float rescale, unsigned long long* key1,
-                                   uint32_t* key2, bool undecided) {
+                                   uint32_t* key2, bool undecided, float tv_known) {
     Tri T;
     if (!tri_setup_geom(T, a, b, c, out_w, out_h)) return 0.f;
     const bool prol = (a.z > 0.f) || (b.z > 0.f) || (c.z > 0.f);
@@ -889,15 +944,18 @@ __device__ __forceinline__ float raster_tri(float3 a, float3 b, float3 c, uint32
                 if (!tri_surely_outside(T, n1, n2)) mask |= 1ull << ((y - ya) * 8 + (x - xa));
             }
         if (!mask) return 0.f;   // no candidate pixel: the validity is never needed
-        // a validity the fast path leaves undecided defers the pixels it covers
-        if (!tri_validity_fast(T, rescale, T.tv)) special = true;
+        // a validity kept by the counting pass (> 0), else the fast path; one it
+        // leaves undecided defers the pixels the triangle covers
+        if (tv_known > 0.f) T.tv = tv_known;
+        else if (!tri_validity_fast(T, rescale, T.tv)) special = true;
         while (mask) {
             const int bit = __builtin_ctzll(mask);
             mask &= mask - 1ull;
             push(xa + (bit & 7), ya + (bit >> 3));
         }
     } else {
-        if (!tri_validity_fast(T, rescale, T.tv)) special = true;
+        if (tv_known > 0.f) T.tv = tv_known;
+        else if (!tri_validity_fast(T, rescale, T.tv)) special = true;
         for (int y = ya; y <= yb; ++y)
             for (int x = xa; x <= xb; ++x) {
                 float n1, n2;
@@ -928,7 +986,9 @@ __device__ __forceinline__ void raster_quad(const G& g, uint32_t ij, uint32_t e,
                                    const RasterWS& ws, const BatchDims& d, uint32_t vs) {
     int i, j;
     unpack_ij(ij, i, j);
-    const float* vb = kept_block(ws, d, vs, i, j);
+    const uint32_t wi = kept_index(ws, d, vs, i, j);
+    const float* vb = wi == 0xffffffffu ? nullptr : ws.vbuf + (size_t)wi * (3 * kBinVerts);
+    const float2 tvk = vb ? kept_validity(ws, wi, i, j) : make_float2(0.f, 0.f);
     const int li = i % kBinQY, lj = j % kBinQX;
     // vertices evaluated here take the fast radius path; an undecided one
     // defers the pixels of the quad's triangles (kept vertices are exact)
@@ -945,18 +1005,18 @@ __device__ __forceinline__ void raster_quad(const G& g, uint32_t ij, uint32_t e,
     {
         const float3 v00 = vtx(0, 0);
         if (gate && v00.z > 0.f && v00.x > 0.f)
-            tv0 = raster_tri(v00, v01, v10, s0, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und);
+            tv0 = raster_tri(v00, v01, v10, s0, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und, tvk.x);
     }
     {
         const float3 v11 = vtx(1, 1);
         if (gate && v11.z > 0.f && v11.x > 0.f)
-            tv1 = raster_tri(v11, v10, v01, s0 | 1u, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und);
+            tv1 = raster_tri(v11, v10, v01, s0 | 1u, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und, tvk.y);
     }
     if (gate && g.wrap && j == 0) {
         const float3 v00 = vtx(0, 0);
         const float3 l0 = vertex(g, i, g.sw - 1, &und), l1 = vertex(g, i + 1, g.sw - 1, &und);
-        raster_tri(l0, v00, l1, s0 | 2u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und);
-        raster_tri(v10, l1, v00, s0 | 3u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und);
+        raster_tri(l0, v00, l1, s0 | 2u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und, 0.f);
+        raster_tri(v10, l1, v00, s0 | 3u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und, 0.f);
     }
     if (C) {
         C->tv[0][e] = tv0;

@@ -1,6 +1,24 @@
 // CPU check of rvs_math.h: the exact emulation of OpenCV's double-accumulated
 // norm must equal the double computation bit for bit, on random inputs, on
 // inputs steered toward float rounding midpoints, and on special values.
+// With RVS_FAITHFUL defined, the float-float norms run with a square root and
+// a reciprocal rounded to either float neighbour of the exact value (within one
+// ulp, what the device's v_sqrt_f32 / v_rcp_f32 guarantee): the results must
+// stay exact.
+#ifdef RVS_FAITHFUL
+#include <random>
+static std::mt19937_64 g_perturb(99);
+// either float neighbour of the exact value v (a faithful rounding), at random
+static float faithful(double v) {
+    const float r = (float)v;
+    if ((double)r == v || (g_perturb() & 1)) return r;
+    return (double)r > v ? std::nextafter(r, 0.f) : std::nextafter(r, 1e30f);
+}
+static float faithful_sqrt(float x) { return faithful(std::sqrt((double)x)); }
+static float faithful_rcp(float x) { return faithful(1.0 / (double)x); }
+#define RVS_MATH_TEST_ROOT faithful_sqrt
+#define RVS_MATH_TEST_RCP faithful_rcp
+#endif
 #include "../../rvs-cuda_amd/csrc/rvs_math.h"
 
 #include <cmath>

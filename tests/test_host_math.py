@@ -8,10 +8,16 @@ import subprocess
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def test_norm_emulation_matches_double(tmp_path):
+import pytest
+
+
+@pytest.mark.parametrize("faithful", [False, True])
+def test_norm_emulation_matches_double(tmp_path, faithful):
+    """faithful: square root and reciprocal perturbed by up to one ulp, as the
+    device's hardware instructions guarantee."""
     exe = str(tmp_path / "test_rvs_math")
-    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-o", exe,
-                    os.path.join(ROOT, "tests", "cpp", "test_rvs_math.cpp")], check=True)
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off"] + (["-DRVS_FAITHFUL"] if faithful else []) +
+                   ["-o", exe, os.path.join(ROOT, "tests", "cpp", "test_rvs_math.cpp")], check=True)
     r = subprocess.run([exe, "300000"], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "fast mismatches 0, slow mismatches 0" in r.stdout
