@@ -120,6 +120,12 @@ private:
     Config() = default;
 };
 
+namespace detail {
+// Precision of the last configuration loaded (Config.cpp:65,393): the
+// rasteriser's oversampling factor, read by SynthetisedViewTriangle.
+extern float g_rescale;
+}  // namespace detail
+
 }  // namespace rvs
 
 #endif

@@ -133,6 +133,13 @@ size_t rvs_inpaint_workspace_size(int w, int h);
 int rvs_inpaint(float* color, const float* quality, int w, int h, void* workspace,
                 size_t ws_bytes, int32_t* map_out, rvs_stream_t stream);
 
+/* write_color's resize of an oversampled colour image to the output size
+ * (image_writing.cpp:189-190, cv::resize with INTER_LINEAR): an exact 2x
+ * downscale averages each 2x2 block (OpenCV's area path), other sizes use
+ * half-pixel-centred bilinear weights.  OpenCV's operation order is not
+ * pinned (no OpenCV here); the oracle restates the same rule.  float3 images. */
+int rvs_resize_color(const float* src, int sw, int sh, float* dst, int dw, int dh, rvs_stream_t stream);
+
 /* Replaces quantization (process_input_frame.cuh:15) with the CPU semantics of
  * write_color (image_writing.cpp:142-203): Y = (T)(v*max) for every pixel, U,V
  * quantised at full resolution then 2x2 averaged as (a+b+c+d+2)>>2. */
@@ -179,18 +186,22 @@ typedef struct {
     float pos[3], rot[3];
     int bitdepth;             /* output BitDepthColor */
     float blending_exp;       /* BlendingFactor */
-    float rescale;            /* Precision; only 1 is supported by this path */
+    float rescale;            /* Precision: rasterise, blend and inpaint at (int)(0.5f + w * rescale) x
+                                 (int)(0.5f + h * rescale), encode after resizing to w x h
+                                 (SynthesizedView.cpp:326-331, image_writing.cpp:189-190) */
     float color_scale;        /* fl(1/max_level) of the first loaded view (image_loading.cpp:134) */
     float depth_scale;        /* idem for depth (image_loading.cpp:193) */
 } rvs_target;
 
-/* Device outputs of one synthesized view; any pointer may be NULL except color. */
+/* Device outputs of one synthesized view; any pointer may be NULL except color.
+ * The float maps are at the raster size, (int)(0.5f + w * rescale) x
+ * (int)(0.5f + h * rescale) (w x h at Precision 1); the encoded planes at w x h. */
 typedef struct {
-    float* color;             /* w*h*3, inpainted (float3 Y,U,V) */
-    float* quality;           /* w*h, blended quality before inpainting */
-    float* depth;             /* w*h, blended depth */
-    float* validity;          /* w*h, blended validity */
-    void *y, *u, *v;          /* encoded YUV420 planes (u8/u16 by target bitdepth) */
+    float* color;             /* raster size * 3, inpainted (float3 Y,U,V) */
+    float* quality;           /* raster size, blended quality before inpainting */
+    float* depth;             /* raster size, blended depth */
+    float* validity;          /* raster size, blended validity */
+    void *y, *u, *v;          /* encoded YUV420 planes at w x h (u8/u16 by target bitdepth) */
 } rvs_view_out;
 
 /* Workspace bytes for rvs_synthesize over these sources and target (0 when an

@@ -590,6 +590,46 @@ void orc_encode_yuv420(const float* color, int w, int h, int bitdepth, void* y, 
         encode_t(color, w, h, maxv, (uint16_t*)y, (uint16_t*)u, (uint16_t*)v);
 }
 
+/* ------------------------------------------------------------- resize */
+
+namespace {
+/* one axis of the INTER_LINEAR rule (imgproc/resize.cpp, resize_ / HResizeLinear) */
+void lin_axis(int d, int n_src, int n_dst, int& s0, int& s1, float& f) {
+    const double scale = (double)n_src / (double)n_dst;
+    f = (float)((d + 0.5) * scale - 0.5);
+    s0 = (int)std::floor(f);
+    f -= (float)s0;
+    if (s0 < 0) { f = 0.f; s0 = 0; }
+    if (s0 >= n_src - 1) { f = 0.f; s0 = n_src - 1; }
+    s1 = s0 + 1 < n_src ? s0 + 1 : s0;
+}
+}  // namespace
+
+void orc_resize_color(const float* src, int sw, int sh, float* dst, int dw, int dh) {
+    const bool area2 = sw == 2 * dw && sh == 2 * dh;
+    for (int y = 0; y < dh; ++y)
+        for (int x = 0; x < dw; ++x)
+            for (int c = 0; c < 3; ++c) {
+                float v;
+                if (area2) {
+                    const float* r0 = src + 3 * ((size_t)(2 * y) * sw + 2 * x) + c;
+                    const float* r1 = r0 + 3 * (size_t)sw;
+                    v = (((r0[0] + r0[3]) + r1[0]) + r1[3]) * 0.25f;
+                } else {
+                    int x0, x1, y0, y1;
+                    float fx, fy;
+                    lin_axis(x, sw, dw, x0, x1, fx);
+                    lin_axis(y, sh, dh, y0, y1, fy);
+                    const float gx = 1.f - fx, gy = 1.f - fy;
+                    auto at = [&](int r, int q) { return src[3 * ((size_t)r * sw + q) + c]; };
+                    const float h0 = at(y0, x0) * gx + at(y0, x1) * fx;
+                    const float h1 = at(y1, x0) * gx + at(y1, x1) * fx;
+                    v = h0 * gy + h1 * fy;
+                }
+                dst[3 * ((size_t)y * dw + x) + c] = v;
+            }
+}
+
 /* ------------------------------------------------------------- whole frame */
 
 int orc_synthesize(const orc_source* srcs, int n_src, const orc_cam* tgt,
@@ -646,8 +686,14 @@ int orc_synthesize(const orc_source* srcs, int n_src, const orc_cam* tgt,
     tm.t_inpaint = t6 - t5;
     if (out_color) std::memcpy(out_color, bcol.data(), 3 * nout * sizeof(float));
     if (out_y) {
-        if (ow != tgt->w || oh != tgt->h) return -2; /* Precision != 1 resize is not restated */
-        orc_encode_yuv420(bcol.data(), ow, oh, tgt_bitdepth, out_y, out_u, out_v);
+        if (ow != tgt->w || oh != tgt->h) {
+            /* write_color resizes the oversampled colour to the view's size first */
+            std::vector<float> rcol(3 * (size_t)tgt->w * tgt->h);
+            orc_resize_color(bcol.data(), ow, oh, rcol.data(), tgt->w, tgt->h);
+            orc_encode_yuv420(rcol.data(), tgt->w, tgt->h, tgt_bitdepth, out_y, out_u, out_v);
+        } else {
+            orc_encode_yuv420(bcol.data(), ow, oh, tgt_bitdepth, out_y, out_u, out_v);
+        }
     }
     tm.t_encode = now_s() - t6;
     if (timing) *timing = tm;

@@ -32,6 +32,12 @@
  *   orc_encode_yuv420                       Y pinned (closed form); U,V chroma rule
  *                                           restates OpenCV 4.9 resize(0.5) area-fast
  *                                           path: "parity unpinned" (OpenCV absent)
+ *   orc_resize_color                        write_color's cv::resize to the output size
+ *                                           (Precision != 1): pinned by closed forms
+ *                                           (constants, 2x block means of integer
+ *                                           images, centre taps of a 3x downscale,
+ *                                           linear ramps); OpenCV's operation order
+ *                                           "parity unpinned" (OpenCV absent)
  */
 #ifndef RVS_ORACLE_H
 #define RVS_ORACLE_H
@@ -122,6 +128,15 @@ void orc_decode_depth(const void* raw, int bitdepth, int w, int h, float depth_s
 /* write_color (image_writing.cpp:182-203): quantization (:142-180) then
  * write_color_YUV (:72-89).  y: w*h, u/v: (w/2)*(h/2), u8 or u16 by bitdepth. */
 void orc_encode_yuv420(const float* color, int w, int h, int bitdepth, void* y, void* u, void* v);
+
+/* write_color (image_writing.cpp:189-190): cv::resize(color, color, virtualSize)
+ * with the default INTER_LINEAR on a float3 image.  OpenCV 4.9 (imgproc/
+ * resize.cpp) takes its area path for an exact 2x downscale (each 2x2 block,
+ * summed left to right and top to bottom, times 0.25) and otherwise the
+ * half-pixel-centred bilinear rule: f = (float)((d + 0.5) * scale - 0.5),
+ * s = floor(f), f -= s; s < 0 -> (0, f = 0); s >= n - 1 -> (n - 1, f = 0);
+ * weights (1 - f, f) along rows, then along columns. */
+void orc_resize_color(const float* src, int sw, int sh, float* dst, int dw, int dh);
 /* max_level (image_loading.cpp:227-231) */
 unsigned orc_max_level(int bitdepth);
 

@@ -238,10 +238,17 @@ int check_raster_status(const rvs_k::RasterWS& ws, const rvs_k::BatchDims& d, ui
 // in workspace order.
 struct SynthPlan {
     RasterPlan R;
-    size_t views, geoms, bounds, quality, inpaint, total;
-    int max_w, max_h;
-    bool any_persp;
+    size_t views, geoms, bounds, quality, inpaint, resized, total;
+    int max_w, max_h;     // largest raster (oversampled by Precision)
+    int max_ew, max_eh;   // largest view
+    bool any_persp, any_rescaled;
 };
+
+// Raster size of a view with Precision `rescale` (SynthesizedView.cpp:326-329).
+void raster_size(const rvs_cam& c, float rescale, int& w, int& h) {
+    w = (int)(0.5f + c.w * rescale);
+    h = (int)(0.5f + c.h * rescale);
+}
 
 int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_views, uint32_t cap, SynthPlan& P,
                std::vector<rvs_k::ViewDesc>* views, uint64_t keep_max = 0) {
@@ -253,19 +260,29 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
         max_sw = srcs[k].cam.w > max_sw ? srcs[k].cam.w : max_sw;
         max_sh = srcs[k].cam.h > max_sh ? srcs[k].cam.h : max_sh;
     }
-    uint64_t tiles = 0, px = 0;
-    P.max_w = 0; P.max_h = 0; P.any_persp = false;
+    uint64_t tiles = 0, px = 0, rpx = 0;
+    P.max_w = 0; P.max_h = 0; P.max_ew = 0; P.max_eh = 0; P.any_persp = false; P.any_rescaled = false;
     for (int v = 0; v < n_views; ++v) {
         const rvs_target& t = tgts[v];
         if (!cam_ok(&t.cam)) return fail(RVS_E_INVALID, "synthesize: view %d has a bad camera", v);
-        if (t.rescale != 1.0f) return fail(RVS_E_UNSUPPORTED, "synthesize: Precision != 1 is not supported");
-        if (t.cam.w > rvs_k::kMaxSourceSide || t.cam.h > rvs_k::kMaxSourceSide)
+        if (!(t.rescale > 0.f) || !(t.rescale <= 64.f))
+            return fail(RVS_E_INVALID, "synthesize: view %d: Precision %g", v, (double)t.rescale);
+        int rw, rh;
+        raster_size(t.cam, t.rescale, rw, rh);
+        if (rw < 2 || rh < 2) return fail(RVS_E_INVALID, "synthesize: view %d: raster %dx%d", v, rw, rh);
+        if (rw > rvs_k::kMaxSourceSide || rh > rvs_k::kMaxSourceSide)
             return fail(RVS_E_UNSUPPORTED, "synthesize: view %d: side above %d", v, rvs_k::kMaxSourceSide);
-        P.max_w = t.cam.w > P.max_w ? t.cam.w : P.max_w;
-        P.max_h = t.cam.h > P.max_h ? t.cam.h : P.max_h;
+        P.max_w = rw > P.max_w ? rw : P.max_w;
+        P.max_h = rh > P.max_h ? rh : P.max_h;
+        P.max_ew = t.cam.w > P.max_ew ? t.cam.w : P.max_ew;
+        P.max_eh = t.cam.h > P.max_eh ? t.cam.h : P.max_eh;
         P.any_persp = P.any_persp || t.cam.proj == 0;
-        tiles += (uint64_t)((t.cam.w + rvs_k::kTile - 1) / rvs_k::kTile) * ((t.cam.h + rvs_k::kTile - 1) / rvs_k::kTile);
-        px += (uint64_t)t.cam.w * t.cam.h;
+        if (rw != t.cam.w || rh != t.cam.h) {
+            P.any_rescaled = true;
+            rpx += (uint64_t)t.cam.w * t.cam.h;
+        }
+        tiles += (uint64_t)((rw + rvs_k::kTile - 1) / rvs_k::kTile) * ((rh + rvs_k::kTile - 1) / rvs_k::kTile);
+        px += (uint64_t)rw * rh;
     }
     if (max_sw > rvs_k::kMaxSourceSide || max_sh > rvs_k::kMaxSourceSide)
         return fail(RVS_E_UNSUPPORTED, "synthesize: source side above %d", rvs_k::kMaxSourceSide);
@@ -284,6 +301,8 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
     P.quality = L.take((size_t)px * sizeof(float));
     plan_raster(L, P.R, n_src, n_views, (uint32_t)tiles, (uint32_t)px, cap, max_sw, max_sh, keep_max);
     P.inpaint = L.take(rvs_k::inpaint_ws_bytes((size_t)px, n_views));
+    // oversampled views: their colour resized to the view's size, for the encoder
+    P.resized = L.take((size_t)rpx * 3 * sizeof(float));
     P.total = L.o;
     if (views) {
         views->resize(n_views);
@@ -291,7 +310,11 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
         for (int v = 0; v < n_views; ++v) {
             rvs_k::ViewDesc& d = (*views)[v];
             std::memset(&d, 0, sizeof d);
-            view_grid(d, tgts[v].cam.w, tgts[v].cam.h, tb, pb);
+            int rw, rh;
+            raster_size(tgts[v].cam, tgts[v].rescale, rw, rh);
+            view_grid(d, rw, rh, tb, pb);
+            d.ew = tgts[v].cam.w;
+            d.eh = tgts[v].cam.h;
             d.bitdepth = tgts[v].bitdepth;
             d.blend_exp = tgts[v].blending_exp;
             d.blend_inv = 1.0f / tgts[v].blending_exp;
@@ -337,13 +360,20 @@ int enqueue_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int
     }
     char* base = (char*)workspace;
     float* quality = (float*)(base + P.quality);
+    float* resized = (float*)(base + P.resized);
     for (int v = 0; v < n_views; ++v) {
         const rvs_view_out& o = outs[v];
         rvs_k::ViewDesc& d = views[v];
         if (!o.color) return fail(RVS_E_INVALID, "synthesize: view %d has no colour output", v);
         const bool enc = o.y && o.u && o.v;
-        if (enc && ((d.w & 1) || (d.h & 1)))
+        if (enc && ((d.ew & 1) || (d.eh & 1)))
             return fail(RVS_E_UNSUPPORTED, "synthesize: odd target size cannot be encoded (view %d)", v);
+        if (d.w != d.ew || d.h != d.eh) {
+            d.ecol = resized;
+            resized += (size_t)d.ew * d.eh * 3;
+        } else {
+            d.ecol = o.color;
+        }
         if (enc && (d.bitdepth < 1 || d.bitdepth > 16))
             return fail(RVS_E_INVALID, "synthesize: view %d output bit depth %d", v, d.bitdepth);
         d.color = o.color;
@@ -405,7 +435,8 @@ int enqueue_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int
     const rvs_k::InpaintWS iws = rvs_k::inpaint_ws_carve(base + P.inpaint, d.npx, n_views);
     HIP_TRY(rvs_k::launch_inpaint(dviews, n_views, P.max_w, P.max_h, iws, nullptr, s), "inpaint");
     prof_mark(4, s);
-    HIP_TRY(rvs_k::launch_encode_views(dviews, n_views, P.max_w, P.max_h, s), "encode");
+    if (P.any_rescaled) HIP_TRY(rvs_k::launch_resize_views(dviews, n_views, P.max_ew, P.max_eh, s), "resize");
+    HIP_TRY(rvs_k::launch_encode_views(dviews, n_views, P.max_ew, P.max_eh, s), "encode");
     prof_mark(5, s);
     return RVS_OK;
 }
@@ -666,6 +697,13 @@ int rvs_inpaint(float* color, const float* quality, int w, int h, void* workspac
         rvs_k::inpaint_ws_carve((char*)workspace + align256(sizeof(rvs_k::ViewDesc)), (size_t)w * h, 1);
     HIP_TRY(rvs_k::launch_upload(&vd, sizeof vd, dv, s), "view upload");
     HIP_TRY(rvs_k::launch_inpaint(dv, 1, w, h, ws, map_out, s), "inpaint");
+    return RVS_OK;
+}
+
+int rvs_resize_color(const float* src, int sw, int sh, float* dst, int dw, int dh, rvs_stream_t stream) {
+    if (!src || !dst || sw < 1 || sh < 1 || dw < 1 || dh < 1 || (uint64_t)dw * dh >= (1ull << 31))
+        return fail(RVS_E_INVALID, "rvs_resize_color: bad argument");
+    HIP_TRY(rvs_k::launch_resize_color(src, sw, sh, dst, dw, dh, (hipStream_t)stream), "resize_color");
     return RVS_OK;
 }
 

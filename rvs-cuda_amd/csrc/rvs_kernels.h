@@ -46,6 +46,9 @@ struct ViewDesc {
     float* depth;            // w*h
     float* validity;         // w*h (blend: validity, raw: triangle shape)
     void *y, *u, *v;         // encoded planes, may be null
+    int ew, eh;              // size of the encoded planes: the view's own size (w, h are the raster's,
+                             // oversampled by Precision)
+    float* ecol;             // colour the planes encode: color, or its resize to ew x eh
 };
 
 // Device-side state of one launch set.
@@ -80,6 +83,9 @@ struct BatchDims {
 enum { MODE_BLEND = 0, MODE_RAW = 1 };
 
 hipError_t launch_reset_u32(uint32_t* p, size_t n, hipStream_t s);
+// Colour of every oversampled view resized to its encoded size (views[v].ecol);
+// views rasterised at their own size are skipped.
+hipError_t launch_resize_views(const ViewDesc* views, int nviews, int max_ew, int max_eh, hipStream_t s);
 // Scan blocks needed for nbins bins (bsum entries).
 size_t scan_blocks(size_t nbins);
 // Raw bytes into device memory through kernel arguments (stream-ordered, capturable).

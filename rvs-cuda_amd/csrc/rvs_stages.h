@@ -21,8 +21,11 @@ hipError_t launch_blend_step(float* acc_c, float* acc_q, float* acc_d, float* ac
                              hipStream_t s);
 hipError_t launch_encode(const float* color, int w, int h, int bitdepth, void* y, void* u, void* v,
                          hipStream_t s);
-// Encodes every view of a batch that has output planes (ViewDesc y, u, v).
+// Encodes every view of a batch that has output planes (ViewDesc y, u, v) from
+// ecol at the views' own sizes (max_w x max_h: the largest ew x eh).
 hipError_t launch_encode_views(const ViewDesc* views, int nviews, int max_w, int max_h, hipStream_t s);
+// write_color's resize (image_writing.cpp:189-190) of one float3 image.
+hipError_t launch_resize_color(const float* src, int sw, int sh, float* dst, int dw, int dh, hipStream_t s);
 
 }  // namespace rvs_k
 

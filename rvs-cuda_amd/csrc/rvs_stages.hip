@@ -102,16 +102,47 @@ __global__ void k_encode(const float* __restrict__ color, int w, int h, unsigned
 }
 
 // Batch of views (grid.z = view); views without output planes are skipped.
+// The planes encode ecol at the view's size (ew x eh).
 __global__ void k_encode_views(const ViewDesc* __restrict__ views) {
     const ViewDesc& vd = views[blockIdx.z];
     const int bx = blockIdx.x * blockDim.x + threadIdx.x;
     const int by = blockIdx.y;
-    if (!vd.y || bx >= vd.w / 2 || by >= vd.h / 2) return;
+    if (!vd.y || bx >= vd.ew / 2 || by >= vd.eh / 2) return;
     const unsigned maxv = (1u << vd.bitdepth) - 1u;
     if (vd.bitdepth <= 8)
-        encode_block(vd.color, vd.w, bx, by, maxv, (uint8_t*)vd.y, (uint8_t*)vd.u, (uint8_t*)vd.v);
+        encode_block(vd.ecol, vd.ew, bx, by, maxv, (uint8_t*)vd.y, (uint8_t*)vd.u, (uint8_t*)vd.v);
     else
-        encode_block(vd.color, vd.w, bx, by, maxv, (uint16_t*)vd.y, (uint16_t*)vd.u, (uint16_t*)vd.v);
+        encode_block(vd.ecol, vd.ew, bx, by, maxv, (uint16_t*)vd.y, (uint16_t*)vd.u, (uint16_t*)vd.v);
+}
+
+// Oversampled views: colour (w x h) resized to the view's size (ew x eh).
+__global__ void k_resize_views(const ViewDesc* __restrict__ views) {
+    const ViewDesc& vd = views[blockIdx.z];
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (vd.ecol == vd.color || x >= vd.ew || y >= vd.eh) return;
+    const float3 c = resize_linear_px(vd.color, vd.w, vd.h, vd.ew, vd.eh, x, y);
+    float* o = vd.ecol + 3 * ((size_t)y * vd.ew + x);
+    o[0] = c.x; o[1] = c.y; o[2] = c.z;
+}
+
+// The same for one colour image (the stage entry point).
+__global__ void k_resize_color(const float* __restrict__ src, int sw, int sh, float* __restrict__ dst, int dw,
+                               int dh) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= dw || y >= dh) return;
+    const float3 c = resize_linear_px(src, sw, sh, dw, dh, x, y);
+    float* o = dst + 3 * ((size_t)y * dw + x);
+    o[0] = c.x; o[1] = c.y; o[2] = c.z;
+}
+
+hipError_t launch_resize_views(const ViewDesc* views, int nviews, int max_ew, int max_eh, hipStream_t s) {
+    hipLaunchKernelGGL(k_resize_views, dim3((max_ew + 255) / 256, max_eh, nviews), dim3(256), 0, s, views);
+    return hipGetLastError();
+}
+
+hipError_t launch_resize_color(const float* src, int sw, int sh, float* dst, int dw, int dh, hipStream_t s) {
+    hipLaunchKernelGGL(k_resize_color, dim3((dw + 255) / 256, dh), dim3(256), 0, s, src, sw, sh, dst, dw, dh);
+    return hipGetLastError();
 }
 
 hipError_t launch_decode_color(const void* y, const void* u, const void* v, int u8, int w, int h, float cs,

@@ -13,6 +13,11 @@
 
 namespace rvs {
 
+namespace detail {
+float g_rescale = 1.f;
+}  // namespace detail
+
+
 namespace {
 
 template <int N>
@@ -260,6 +265,7 @@ Config Config::loadFromFile(std::string const& filename) {
     if (auto n = root.optional("NumberOfOutputFrames")) c.number_of_output_frames = n.asInt();
     else c.number_of_output_frames = c.number_of_frames;
     if (auto n = root.optional("Precision")) c.precision = (float)n.asDouble();
+    detail::g_rescale = c.precision;
     if (auto n = root.optional("ColorSpace")) {
         if (n.asString() != "YUV" && n.asString() != "RGB") throw std::runtime_error("Unknown color space");
         c.color_space = n.asString();

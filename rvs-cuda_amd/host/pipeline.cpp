@@ -15,6 +15,11 @@
 
 namespace rvs {
 
+// Raster size at Precision `rescale` (SynthesizedView.cpp:326-329).
+static Size raster_size(Size s, float rescale) {
+    return Size{(int)(0.5f + s.width * rescale), (int)(0.5f + s.height * rescale)};
+}
+
 namespace {
 
 void check(int rc, const char* what) {
@@ -278,19 +283,22 @@ void SynthetisedViewTriangle::compute(InputView& input, rvs_stream_t stream) {
     m_uv.resize(2 * nin * sizeof(float));
     m_vdepth.resize(nin * sizeof(float));
     rvs_source s = input.source();
-    check(rvs_reproject(&src, &dst, &x, input.getDevDepth(), s.erp_table, 1.0f, m_uv.as<float>(),
+    // rasterised with the oversampling of detail::g_rescale (SynthesizedView.cpp:325-334)
+    const float rescale = detail::g_rescale;
+    check(rvs_reproject(&src, &dst, &x, input.getDevDepth(), s.erp_table, rescale, m_uv.as<float>(),
                         m_vdepth.as<float>(), stream),
           "rvs_reproject");
-    allocate(vp.getSize(), false);
+    const Size rs = raster_size(vp.getSize(), rescale);
+    allocate(rs, false);
     const int wrap = (dst.proj == 1 && dst.full_hor) ? 1 : 0;
     uint32_t cap = 0;
     for (int attempt = 0; attempt < 8; ++attempt) {
-        const std::size_t need = rvs_raster_workspace_size(src.w, src.h, dst.w, dst.h, cap);
+        const std::size_t need = rvs_raster_workspace_size(src.w, src.h, rs.width, rs.height, cap);
         m_ws.resize(need);
         uint32_t needed = 0;
-        const int rc = rvs_raster(input.getDevColor(), m_vdepth.as<float>(), m_uv.as<float>(), src.w, src.h, dst.w,
-                                  dst.h, wrap, 1.0f, m_ws.get(), need, cap, getDevColor(), getDevDepth(),
-                                  getDevValidity(), &needed, stream);
+        const int rc = rvs_raster(input.getDevColor(), m_vdepth.as<float>(), m_uv.as<float>(), src.w, src.h,
+                                  rs.width, rs.height, wrap, rescale, m_ws.get(), need, cap, getDevColor(),
+                                  getDevDepth(), getDevValidity(), &needed, stream);
         if (rc == RVS_E_CAPACITY && needed > 0) {
             cap = grow_entry_cap(needed);
             continue;
@@ -387,6 +395,57 @@ void Pipeline::saveMaskedDepth(std::vector<float> const&, std::vector<uint8_t> c
 
 namespace {
 
+// OpenCV's INTER_NEAREST resize (imgproc/resize.cpp, resizeNN): source index
+// floor(d * src / dst), clamped.
+std::vector<uint8_t> resize_nearest(std::vector<uint8_t> const& src, Size s, Size d) {
+    std::vector<uint8_t> out((std::size_t)d.area());
+    const double fx = (double)s.width / d.width, fy = (double)s.height / d.height;
+    for (int y = 0; y < d.height; ++y) {
+        const int sy = std::min((int)std::floor(y * fy), s.height - 1);
+        for (int x = 0; x < d.width; ++x) {
+            const int sx = std::min((int)std::floor(x * fx), s.width - 1);
+            out[(std::size_t)y * d.width + x] = src[(std::size_t)sy * s.width + sx];
+        }
+    }
+    return out;
+}
+
+// OpenCV's INTER_LINEAR resize of a float map, the rule of rvs_resize_color
+// (exact 2x downscale: 2x2 block means; else half-pixel-centred bilinear).
+void lin_axis(int d, int n_src, int n_dst, int& s0, int& s1, float& f) {
+    const double scale = (double)n_src / (double)n_dst;
+    f = (float)((d + 0.5) * scale - 0.5);
+    s0 = (int)std::floor(f);
+    f -= (float)s0;
+    if (s0 < 0) { f = 0.f; s0 = 0; }
+    if (s0 >= n_src - 1) { f = 0.f; s0 = n_src - 1; }
+    s1 = s0 + 1 < n_src ? s0 + 1 : s0;
+}
+
+std::vector<float> resize_linear(std::vector<float> const& src, Size s, Size d) {
+    std::vector<float> out((std::size_t)d.area());
+    auto at = [&](int r, int c) { return src[(std::size_t)r * s.width + c]; };
+    const bool area2 = s.width == 2 * d.width && s.height == 2 * d.height;
+    for (int y = 0; y < d.height; ++y)
+        for (int x = 0; x < d.width; ++x) {
+            float v;
+            if (area2) {
+                v = (((at(2 * y, 2 * x) + at(2 * y, 2 * x + 1)) + at(2 * y + 1, 2 * x)) + at(2 * y + 1, 2 * x + 1)) *
+                    0.25f;
+            } else {
+                int x0, x1, y0, y1;
+                float fx, fy;
+                lin_axis(x, s.width, d.width, x0, x1, fx);
+                lin_axis(y, s.height, d.height, y0, y1, fy);
+                const float gx = 1.f - fx, gy = 1.f - fy;
+                const float h0 = at(y0, x0) * gx + at(y0, x1) * fx, h1 = at(y1, x0) * gx + at(y1, x1) * fx;
+                v = h0 * gy + h1 * fy;
+            }
+            out[(std::size_t)y * d.width + x] = v;
+        }
+    return out;
+}
+
 std::vector<uint8_t> concat_planes(DeviceBuffer const& y, DeviceBuffer const& u, DeviceBuffer const& v,
                                    std::size_t ny, std::size_t nuv, rvs_stream_t s) {
     std::vector<uint8_t> out(ny + 2 * nuv);
@@ -400,7 +459,7 @@ std::vector<uint8_t> concat_planes(DeviceBuffer const& y, DeviceBuffer const& u,
 
 void Pipeline::computeView(int inputFrame, int virtualFrame, int virtualView) {
     Config const& c = getConfig();
-    if (c.precision != 1.0f) throw std::runtime_error("Precision != 1 is not supported by this build");
+    detail::g_rescale = c.precision;
     Parameters params_virtual = c.params_virtual[virtualView];
     if (!c.pose_trace.empty()) {   // Pipeline.cpp:173-181
         const Pose& pose = c.pose_trace.at(inputFrame);
@@ -434,10 +493,18 @@ void Pipeline::computeView(int inputFrame, int virtualFrame, int virtualView) {
     }
     const Size vs = params_virtual.getSize();
     const std::size_t npx = (std::size_t)vs.area();
+    // rasteriser, blender and inpainting work at the oversampled size; the
+    // output is resized to the view's size on write (image_writing.cpp:189-190)
+    const Size rs = raster_size(vs, c.precision);
+    const std::size_t nr = (std::size_t)rs.area();
+    const bool rescaled = rs.width != vs.width || rs.height != vs.height;
+    if (rescaled && wantMaskedColor())
+        throw std::runtime_error("MaskedOutputFiles with Precision != 1: the reference masks the oversampled colour "
+                                 "with the resized mask (Pipeline.cpp:307-318), which OpenCV rejects");
     const int bd = params_virtual.getColorBitDepth();
     const std::size_t e = bytes_per(bd);
-    DeviceBuffer color(3 * npx * sizeof(float)), quality(npx * sizeof(float)), depth(npx * sizeof(float)),
-        validity(npx * sizeof(float));
+    DeviceBuffer color(3 * nr * sizeof(float)), quality(nr * sizeof(float)), depth(nr * sizeof(float)),
+        validity(nr * sizeof(float));
     DeviceBuffer y(npx * e), u(npx / 4 * e), v(npx / 4 * e);
     const rvs_cam tcam = cam_of(params_virtual);
     if (fused) {
@@ -475,19 +542,25 @@ void Pipeline::computeView(int inputFrame, int virtualFrame, int virtualView) {
             synths[k]->compute(*inputs[k], m_stream);
             blender->blend(*synths[k], m_stream);
         }
-        hip_check(hipMemcpyAsync(color.get(), blender->getDevColor(), 3 * npx * sizeof(float),
+        hip_check(hipMemcpyAsync(color.get(), blender->getDevColor(), 3 * nr * sizeof(float),
                                  hipMemcpyDeviceToDevice, (hipStream_t)m_stream), "copy");
-        hip_check(hipMemcpyAsync(quality.get(), blender->getDevQuality(), npx * sizeof(float),
+        hip_check(hipMemcpyAsync(quality.get(), blender->getDevQuality(), nr * sizeof(float),
                                  hipMemcpyDeviceToDevice, (hipStream_t)m_stream), "copy");
-        hip_check(hipMemcpyAsync(depth.get(), blender->getDevDepth(), npx * sizeof(float), hipMemcpyDeviceToDevice,
+        hip_check(hipMemcpyAsync(depth.get(), blender->getDevDepth(), nr * sizeof(float), hipMemcpyDeviceToDevice,
                                  (hipStream_t)m_stream), "copy");
-        hip_check(hipMemcpyAsync(validity.get(), blender->getDevValidity(), npx * sizeof(float),
+        hip_check(hipMemcpyAsync(validity.get(), blender->getDevValidity(), nr * sizeof(float),
                                  hipMemcpyDeviceToDevice, (hipStream_t)m_stream), "copy");
-        DeviceBuffer iws(rvs_inpaint_workspace_size(vs.width, vs.height));
-        check(rvs_inpaint(color.as<float>(), quality.as<float>(), vs.width, vs.height, iws.get(), iws.bytes(),
+        DeviceBuffer iws(rvs_inpaint_workspace_size(rs.width, rs.height));
+        check(rvs_inpaint(color.as<float>(), quality.as<float>(), rs.width, rs.height, iws.get(), iws.bytes(),
                           nullptr, m_stream),
               "rvs_inpaint");
-        check(rvs_encode_yuv420(color.as<float>(), vs.width, vs.height, bd, y.get(), u.get(), v.get(), m_stream),
+        DeviceBuffer resized(rescaled ? 3 * npx * sizeof(float) : 0);
+        if (rescaled)
+            check(rvs_resize_color(color.as<float>(), rs.width, rs.height, resized.as<float>(), vs.width, vs.height,
+                                   m_stream),
+                  "rvs_resize_color");
+        check(rvs_encode_yuv420(rescaled ? resized.as<float>() : color.as<float>(), vs.width, vs.height, bd, y.get(),
+                                u.get(), v.get(), m_stream),
               "rvs_encode_yuv420");
     }
     inputs.clear();
@@ -500,11 +573,14 @@ void Pipeline::computeView(int inputFrame, int virtualFrame, int virtualView) {
     }
     // masks and depth maps (Pipeline.cpp:303-336)
     if (wantMask() || wantMaskedColor() || wantMaskedDepth() || wantDepth()) {
-        std::vector<float> val(npx), dep(npx);
-        validity.download(val.data(), npx * sizeof(float), m_stream);
-        depth.download(dep.data(), npx * sizeof(float), m_stream);
-        std::vector<uint8_t> mask(npx);
-        for (std::size_t k = 0; k < npx; ++k) mask[k] = (val[k] > c.validity_threshold) ? 0 : 255;
+        std::vector<float> val(nr), dep(nr);
+        validity.download(val.data(), nr * sizeof(float), m_stream);
+        depth.download(dep.data(), nr * sizeof(float), m_stream);
+        std::vector<uint8_t> rmask(nr);
+        for (std::size_t k = 0; k < nr; ++k) rmask[k] = (val[k] > c.validity_threshold) ? 0 : 255;
+        // mask: resize(INTER_NEAREST); depth: resize() (INTER_LINEAR) to the view's size
+        std::vector<uint8_t> mask = rescaled ? resize_nearest(rmask, rs, vs) : rmask;
+        if (rescaled) dep = resize_linear(dep, rs, vs);
         if (wantMask()) saveMask(mask, virtualFrame, virtualView, params_virtual);
         if (wantMaskedColor()) {
             std::vector<float> col(3 * npx);

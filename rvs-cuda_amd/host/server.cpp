@@ -22,7 +22,12 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 size_t plane_bytes(const rvs_target& t) { return t.bitdepth <= 8 ? 1 : 2; }
 
-size_t color_bytes(const rvs_target& t) { return align256((size_t)t.cam.w * t.cam.h * 3 * sizeof(float)); }
+// Colour map of a request: float3 at the raster size (the view's size scaled by
+// its Precision, SynthesizedView.cpp:326-329).
+size_t color_bytes(const rvs_target& t) {
+    const size_t w = (size_t)(int)(0.5f + t.cam.w * t.rescale), h = (size_t)(int)(0.5f + t.cam.h * t.rescale);
+    return align256(w * h * 3 * sizeof(float));
+}
 
 size_t yuv_bytes(const rvs_target& t) { return (size_t)t.cam.w * t.cam.h * 3 / 2 * plane_bytes(t); }
 

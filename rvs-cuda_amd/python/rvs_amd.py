@@ -91,6 +91,7 @@ def hip():
             "rvs_inpaint_workspace_size": ([i, i], sz),
             "rvs_inpaint": ([vp, vp, i, i, vp, sz, vp, vp], i),
             "rvs_encode_yuv420": ([vp, i, i, i, vp, vp, vp, vp], i),
+            "rvs_resize_color": ([vp, i, i, vp, i, i, vp], i),
             "rvs_synth_workspace_size": ([P(RvsSource), i, P(RvsTarget), u32], sz),
             "rvs_synthesize": ([P(RvsSource), i, P(RvsTarget), vp, sz, u32, P(RvsViewOut), P(u32), vp], i),
             "rvs_synth_batch_workspace_size": ([P(RvsSource), i, P(RvsTarget), i, u32], sz),
@@ -369,6 +370,21 @@ class DeviceSource:
         return s
 
 
+def raster_size(target):
+    """(w, h) at which a target is rasterised, blended and inpainted: its size
+    scaled by its Precision, (int)(0.5f + w * rescale) (SynthesizedView.cpp:326-329)."""
+    r = np.float32(target.rescale)
+    if not (np.isfinite(r) and r > 0):   # refused by the library's plan; size the buffers as the view
+        return (target.cam.w, target.cam.h)
+    return (int(np.float32(0.5) + np.float32(target.cam.w) * r), int(np.float32(0.5) + np.float32(target.cam.h) * r))
+
+
+def resize_color(src, out, stream=None):
+    sh, sw = src.shape[:2]
+    dh, dw = out.shape[:2]
+    check(hip().rvs_resize_color(_ptr(src), sw, sh, _ptr(out), dw, dh, _stream(stream)))
+
+
 def make_target(c, pos, rot, bitdepth, blending_exp, color_scale, depth_scale, rescale=1.0):
     t = RvsTarget()
     t.cam = c
@@ -391,12 +407,13 @@ class Synthesizer:
         self.device = device
         self.arr = (RvsSource * len(sources))(*[s.struct() for s in sources])
         w, h = target.cam.w, target.cam.h
+        rw, rh = raster_size(target)   # float maps at the raster size
         dt = torch.uint8 if target.bitdepth <= 8 else torch.int16
         self.out = {
-            "color": torch.empty((h, w, 3), dtype=torch.float32, device=device),
-            "quality": torch.empty((h, w), dtype=torch.float32, device=device) if want_maps else None,
-            "depth": torch.empty((h, w), dtype=torch.float32, device=device) if want_maps else None,
-            "validity": torch.empty((h, w), dtype=torch.float32, device=device) if want_maps else None,
+            "color": torch.empty((rh, rw, 3), dtype=torch.float32, device=device),
+            "quality": torch.empty((rh, rw), dtype=torch.float32, device=device) if want_maps else None,
+            "depth": torch.empty((rh, rw), dtype=torch.float32, device=device) if want_maps else None,
+            "validity": torch.empty((rh, rw), dtype=torch.float32, device=device) if want_maps else None,
             "y": torch.empty((h, w), dtype=dt, device=device),
             "u": torch.empty((h // 2, w // 2), dtype=dt, device=device),
             "v": torch.empty((h // 2, w // 2), dtype=dt, device=device),
@@ -480,13 +497,14 @@ class BatchSynthesizer:
         vo = (RvsViewOut * self.n)()
         for k, t in enumerate(targets):
             w, h = t.cam.w, t.cam.h
+            rw, rh = raster_size(t)
             dt = torch.uint8 if t.bitdepth <= 8 else torch.int16
-            o = {"color": torch.empty((h, w, 3), dtype=torch.float32, device=self.device),
+            o = {"color": torch.empty((rh, rw, 3), dtype=torch.float32, device=self.device),
                  "y": torch.empty((h, w), dtype=dt, device=self.device),
                  "u": torch.empty((h // 2, w // 2), dtype=dt, device=self.device),
                  "v": torch.empty((h // 2, w // 2), dtype=dt, device=self.device)}
             for key in ("quality", "depth", "validity"):
-                o[key] = torch.empty((h, w), dtype=torch.float32, device=self.device) if want_maps else None
+                o[key] = torch.empty((rh, rw), dtype=torch.float32, device=self.device) if want_maps else None
             self.outs.append(o)
             for key in ("color", "quality", "depth", "validity", "y", "u", "v"):
                 setattr(vo[k], key, o[key].data_ptr() if o[key] is not None else None)
@@ -556,13 +574,16 @@ class StagewiseSynthesizer:
             vd = torch.empty((c.h, c.w), dtype=torch.float32, device=device)
             self.dec.append((col, dep, uv, vd))
         w, h = target.cam.w, target.cam.h
+        rw, rh = raster_size(target)
         f32 = dict(dtype=torch.float32, device=device)
-        self.rc = torch.empty((h, w, 3), **f32)
-        self.rd = torch.empty((h, w), **f32)
-        self.rs = torch.empty((h, w), **f32)
-        self.acc = {"color": torch.empty((h, w, 3), **f32), "quality": torch.empty((h, w), **f32),
-                    "depth": torch.empty((h, w), **f32), "validity": torch.empty((h, w), **f32),
-                    "mask": torch.empty((h, w), dtype=torch.uint8, device=device)}
+        self.rc = torch.empty((rh, rw, 3), **f32)
+        self.rd = torch.empty((rh, rw), **f32)
+        self.rs = torch.empty((rh, rw), **f32)
+        self.acc = {"color": torch.empty((rh, rw, 3), **f32), "quality": torch.empty((rh, rw), **f32),
+                    "depth": torch.empty((rh, rw), **f32), "validity": torch.empty((rh, rw), **f32),
+                    "mask": torch.empty((rh, rw), dtype=torch.uint8, device=device)}
+        # write_color resizes an oversampled colour to the view's size before quantising
+        self.resized = torch.empty((h, w, 3), **f32) if (rw, rh) != (w, h) else None
         dt = torch.uint8 if target.bitdepth <= 8 else torch.int16
         self.out = {"y": torch.empty((h, w), dtype=dt, device=device),
                     "u": torch.empty((h // 2, w // 2), dtype=dt, device=device),
@@ -572,7 +593,7 @@ class StagewiseSynthesizer:
 
     def run(self, stream=None):
         t = self.target
-        w, h = t.cam.w, t.cam.h
+        w, h = raster_size(t)
         wrap = t.cam.proj == 1 and t.cam.full_hor
         for k, s in enumerate(self.sources):
             col, dep, uv, vd = self.dec[k]
@@ -582,6 +603,10 @@ class StagewiseSynthesizer:
                               stream=stream)
             blend_step(self.acc, self.rc, self.rd, self.rs, t.blending_exp, k == 0, stream)
         self.iws = inpaint(self.acc["color"], self.acc["quality"], workspace=self.iws, stream=stream)
-        encode_yuv420(self.acc["color"], t.bitdepth, self.out["y"], self.out["u"], self.out["v"], stream)
+        enc = self.acc["color"]
+        if self.resized is not None:
+            resize_color(enc, self.resized, stream)
+            enc = self.resized
+        encode_yuv420(enc, t.bitdepth, self.out["y"], self.out["u"], self.out["v"], stream)
         return dict(self.out, color=self.acc["color"], quality=self.acc["quality"], depth=self.acc["depth"],
                     validity=self.acc["validity"])

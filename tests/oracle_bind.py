@@ -70,6 +70,7 @@ def lib():
         L.orc_decode_depth.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
                                        C.c_float, C.c_int, fptr]
         L.orc_encode_yuv420.argtypes = [fptr, C.c_int, C.c_int, C.c_int, vp, vp, vp]
+        L.orc_resize_color.argtypes = [fptr, C.c_int, C.c_int, fptr, C.c_int, C.c_int]
         L.orc_max_level.argtypes = [C.c_int]
         L.orc_max_level.restype = C.c_uint
         L.orc_synthesize.argtypes = [C.POINTER(OrcSource), C.c_int, C.POINTER(OrcCam), fptr, fptr,
@@ -233,6 +234,14 @@ def encode_yuv420(color, bitdepth):
     return y, u, v
 
 
+def resize_color(src, dw, dh):
+    src = f32(src)
+    sh, sw = src.shape[:2]
+    dst = np.zeros((dh, dw, 3), np.float32)
+    lib().orc_resize_color(src, sw, sh, dst, dw, dh)
+    return dst
+
+
 def synthesize(sources, tgt_cam, tgt_pos, tgt_rot, tgt_bitdepth, blending_exp, rescale=1.0):
     """sources: list of dicts with keys cam, pos, rot, bd_color, bd_depth, znear, zfar,
     has_invalid, y, u, v, depth (numpy planes).  Returns dict of outputs."""
@@ -253,12 +262,15 @@ def synthesize(sources, tgt_cam, tgt_pos, tgt_rot, tgt_bitdepth, blending_exp, r
         keep += planes
         o.y, o.u, o.v, o.depth = (p.ctypes.data for p in planes)
     w, h = tgt_cam.w, tgt_cam.h
+    # float maps at the raster size of Precision `rescale` (SynthesizedView.cpp:326-329)
+    rw, rh = int(np.float32(0.5) + np.float32(w) * np.float32(rescale)), \
+        int(np.float32(0.5) + np.float32(h) * np.float32(rescale))
     dt = _plane_dtype(tgt_bitdepth)
     out = {
-        "color": np.zeros((h, w, 3), np.float32),
-        "quality": np.zeros((h, w), np.float32),
-        "depth": np.zeros((h, w), np.float32),
-        "validity": np.zeros((h, w), np.float32),
+        "color": np.zeros((rh, rw, 3), np.float32),
+        "quality": np.zeros((rh, rw), np.float32),
+        "depth": np.zeros((rh, rw), np.float32),
+        "validity": np.zeros((rh, rw), np.float32),
         "y": np.zeros((h, w), dt),
         "u": np.zeros((h // 2, w // 2), dt),
         "v": np.zeros((h // 2, w // 2), dt),

@@ -152,3 +152,36 @@ def test_multiframe_outputs_match_the_oracle(tmp_path, torch_cuda, threshold):
     assert "Total time" in r.stdout
     for key, p in out.items():
         assert open(p, "rb").read() == before[key], "CLI output %s differs" % key
+
+
+def test_precision_2_outputs(tmp_path, torch_cuda):
+    """Precision 2: the Application rasterises at twice the view's size; the
+    colour is resized to the view's size before encoding, the mask with
+    nearest sampling (the top-left pixel of each 2x2 block) and the depth map
+    with the linear rule (2x2 block means) before they are written
+    (Pipeline.cpp:305-335)."""
+    say("case app precision 2")
+    rg, frames, trace, cfg, out = _case(tmp_path, 5000.0)
+    c = json.loads(open(cfg).read())
+    c.update(Precision=2.0, NumberOfFrames=1, NumberOfOutputFrames=1)
+    del c["MaskedOutputFiles"]
+    del c["MaskedDepthOutputFiles"]
+    open(cfg, "w").write(json.dumps(c))
+    assert R.app_run(cfg, "") == 1
+    t = rg["target"]
+    w, h = t["cam"].w, t["cam"].h
+    r = ob.synthesize(W.oracle_sources(rg, frames[0], ob), W.oracle_cam(t["cam"], ob), list(t["pos"]), [0.0] * 3,
+                      t["bitdepth"], t["blending_exp"], rescale=2.0)
+    assert r["validity"].shape == (2 * h, 2 * w)
+    assert open(out["out"], "rb").read() == r["y"].tobytes() + r["u"].tobytes() + r["v"].tobytes()
+    m = np.frombuffer(open(out["mask"], "rb").read(), np.uint8)[:w * h].reshape(h, w)
+    rmask = ~(r["validity"] > 5000.0)
+    assert np.array_equal(m, np.where(rmask[0::2, 0::2], 255, 0).astype(np.uint8))
+    d2 = ob.resize_color(np.repeat(r["depth"][:, :, None], 3, axis=2), w, h)[:, :, 0].astype(np.float64)
+    dy = np.frombuffer(open(out["depth"], "rb").read(), np.uint16)[:w * h].reshape(h, w).astype(np.float64)
+    zn, zf = 1.0, 10.0
+    inr = np.isfinite(d2) & (d2 >= zn) & (d2 <= zf)
+    back = zf * zn / (zn + dy / 65535.0 * (zf - zn))
+    assert np.all(np.abs(1 / back[inr] - 1 / d2[inr]) <= 1.01 * (1 / zn - 1 / zf) / 65535.0)
+    assert np.all(dy[~np.isfinite(d2)] == 0)
+

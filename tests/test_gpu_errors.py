@@ -2,7 +2,7 @@
 
 Bad arguments are refused with RVS_E_INVALID / RVS_E_UNSUPPORTED and a
 message (the reference throws std::runtime_error at the same places, e.g.
-odd sizes for YUV420 and unsupported Precision), an entry list that is too
+odd sizes for YUV420 and a Precision that is not a positive factor), an entry list that is too
 small reports the exact capacity it needed, and a failed call leaves the
 library usable: the next valid call gives the same bits as before.
 """
@@ -53,10 +53,11 @@ def test_bad_arguments_are_refused_and_leave_the_library_usable(torch_cuda):
         R.Synthesizer(srcs, _target(rg, 161, 120), want_maps=False).run()
     assert "odd" in str(e.value)
 
-    # Precision != 1
-    with pytest.raises(R.RvsError) as e:
-        R.Synthesizer(srcs, _target(rg, 160, 120, rescale=2.0), want_maps=False)
-    assert e.value.code == R.RVS_E_UNSUPPORTED or "Precision" in str(e.value)
+    # a Precision that is not a positive factor
+    for bad in (0.0, -1.0, float("nan")):
+        with pytest.raises(R.RvsError) as e:
+            R.Synthesizer(srcs, _target(rg, 160, 120, rescale=bad), want_maps=False)
+        assert e.value.code == R.RVS_E_INVALID and "Precision" in str(e.value)
 
     # an empty batch
     with pytest.raises(R.RvsError):

@@ -421,3 +421,38 @@ def test_synthesize_repeatable(torch_cuda):
     torch.cuda.synchronize()
     for k in a:
         assert torch.equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("precision", [2.0, 1.5, 0.5])
+def test_synthesize_precision(torch_cuda, precision):
+    """Precision != 1: rasterise, blend and inpaint at (int)(0.5 + size * p)
+    (SynthesizedView.cpp:326-331; validity stretch divided by p), then resize
+    to the view's size before encoding (image_writing.cpp:189-190).  Maps at
+    the raster size and the encoded planes bit-exact against the oracle (the
+    resize rule is the oracle's restatement of OpenCV's, pinned on the CPU);
+    the per-stage path gives the same bits."""
+    torch = torch_cuda
+    say("case synthesize precision %g" % precision)
+    rg = W.rig("rects_640", scale=2)
+    planes = W.render_sources(rg)
+    tgt = W.device_target(rg, [0.02, -0.01, 0.01], [3.0, -1.0, 0.5])
+    tgt.rescale = precision
+    srcs = W.device_sources(rg, planes)
+    out = R.Synthesizer(srcs, tgt).run()
+    torch.cuda.synchronize()
+    t = rg["target"]
+    ref = ob.synthesize(W.oracle_sources(rg, planes, ob), W.oracle_cam(t["cam"], ob), list(tgt.pos), list(tgt.rot),
+                        t["bitdepth"], t["blending_exp"], rescale=precision)
+    rw, rh = R.raster_size(tgt)
+    assert out["color"].shape[:2] == (rh, rw) and ref["color"].shape[:2] == (rh, rw)
+    for k in ("quality", "depth", "validity", "color"):
+        ok, n = bits_equal(out[k].cpu().numpy(), ref[k])
+        assert ok, "precision %g: %s %d mismatches" % (precision, k, n)
+    for k in ("y", "u", "v"):
+        assert np.array_equal(out[k].cpu().numpy(), ref[k]), "precision %g: plane %s" % (precision, k)
+    sw = R.StagewiseSynthesizer(srcs, tgt).run()
+    torch.cuda.synchronize()
+    for k in ("y", "u", "v", "color", "depth"):
+        assert torch.equal(sw[k].view(torch.int32) if sw[k].dtype == torch.float32 else sw[k],
+                           out[k].view(torch.int32) if out[k].dtype == torch.float32 else out[k]), k
+
