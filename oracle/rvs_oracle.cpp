@@ -630,13 +630,95 @@ void orc_resize_color(const float* src, int sw, int sh, float* dst, int dw, int 
             }
 }
 
+/* ------------------------------------------------------ multispectral */
+
+void orc_blur_masked(const float* y, int stride, const uint8_t* mask, int w, int h, int rad, float* out) {
+    /* IntegralImage2D<float, 1>::computeIntegralImages with a mask
+     * (IntegralImage2D.h:738-770): double sums, finite values under mask == 255,
+     * I[r+1][c+1] = ((I[r][c+1] + I[r+1][c]) - I[r][c]) (+ value) */
+    const size_t W1 = (size_t)w + 1;
+    std::vector<double> I(W1 * (h + 1), 0.0);
+    std::vector<unsigned> N(W1 * (h + 1), 0u);
+    for (int r = 0; r < h; ++r)
+        for (int c = 0; c < w; ++c) {
+            const size_t a = (size_t)r * W1, b = (size_t)(r + 1) * W1;
+            I[b + c + 1] = I[a + c + 1] + I[b + c] - I[a + c];
+            N[b + c + 1] = N[a + c + 1] + N[b + c] - N[a + c];
+            const float v = y[((size_t)r * w + c) * stride];
+            if (mask[(size_t)r * w + c] == 255 && std::isfinite(v)) {
+                I[b + c + 1] += v;
+                N[b + c + 1] += 1;
+            }
+        }
+    /* calcBlurring, one channel (blending.cpp:62-104) */
+    const int rad_1 = 2 * rad + 1;
+    for (int i = 0; i < h; ++i)
+        for (int j = 0; j < w; ++j) {
+            float& o = out[(size_t)i * w + j];
+            if (mask[(size_t)i * w + j] == 0) { o = 0.f; continue; }
+            int sx = j - rad, sy = i - rad, ey = sy + rad_1, ex = sx + rad_1;
+            sx = std::max(0, sx); sy = std::max(0, sy);
+            ex = std::min(ex, w); ey = std::min(ey, h);
+            const size_t ul = (size_t)sy * W1 + sx, ur = (size_t)sy * W1 + ex;
+            const size_t ll = (size_t)ey * W1 + sx, lr = (size_t)ey * W1 + ex;
+            const unsigned cnt = N[lr] + N[ul] - N[ur] - N[ll];
+            const unsigned sum = (unsigned)(int)(I[lr] + I[ul] - I[ur] - I[ll]);
+            /* the reference leaves blr at OpenCV's uninitialised value when cnt == 0;
+             * cnt >= 1 for every pixel under the mask (it counts itself when finite) */
+            o = cnt > 0 ? float(float(sum) / float(cnt)) : 0.f;
+        }
+}
+
+void orc_split_frequencies(const float* color, const float* depth, const float* shape, int w, int h,
+                           float* low, float* high) {
+    /* BlendedViewMultiSpec::blend (BlendedView.cpp:70-88): mask = quality > 0
+     * with quality = shape / depth (SynthesizedView.cpp:440); split_frequencies
+     * (blending.cpp:371-394), YUV: blur Y only, radius max(rows, cols) / 20 */
+    const size_t n = (size_t)w * h;
+    std::vector<uint8_t> mask(n);
+    for (size_t k = 0; k < n; ++k) mask[k] = (shape[k] / depth[k] > 0.f) ? 255 : 0;
+    const int rad = std::max(w, h) / 20;
+    std::vector<float> yb(n);
+    orc_blur_masked(color, 3, mask.data(), w, h, rad, yb.data());
+    for (size_t k = 0; k < n; ++k) {
+        low[3 * k] = yb[k];
+        low[3 * k + 1] = color[3 * k + 1];
+        low[3 * k + 2] = color[3 * k + 2];
+        for (int c = 0; c < 3; ++c) high[3 * k + c] = color[3 * k + c] - low[3 * k + c];
+    }
+}
+
 /* ------------------------------------------------------------- whole frame */
+
+namespace {
+int synthesize_frame(const orc_source* srcs, int n_src, const orc_cam* tgt, const float tgt_pos[3],
+                     const float tgt_rot[3], int tgt_bitdepth, float blending_exp, float rescale, int multispectral,
+                     float exp_low, float exp_high, float* out_color, float* out_quality, float* out_depth,
+                     float* out_validity, void* out_y, void* out_u, void* out_v, orc_timing* timing);
+}  // namespace
 
 int orc_synthesize(const orc_source* srcs, int n_src, const orc_cam* tgt,
                    const float tgt_pos[3], const float tgt_rot[3], int tgt_bitdepth,
                    float blending_exp, float rescale,
                    float* out_color, float* out_quality, float* out_depth, float* out_validity,
                    void* out_y, void* out_u, void* out_v, orc_timing* timing) {
+    return synthesize_frame(srcs, n_src, tgt, tgt_pos, tgt_rot, tgt_bitdepth, blending_exp, rescale, 0, 0.f, 0.f,
+                            out_color, out_quality, out_depth, out_validity, out_y, out_u, out_v, timing);
+}
+
+int orc_synthesize_multispectral(const orc_source* srcs, int n_src, const orc_cam* tgt, const float tgt_pos[3],
+                                 const float tgt_rot[3], int tgt_bitdepth, float exp_low, float exp_high,
+                                 float rescale, float* out_color, float* out_quality, float* out_depth,
+                                 float* out_validity, void* out_y, void* out_u, void* out_v) {
+    return synthesize_frame(srcs, n_src, tgt, tgt_pos, tgt_rot, tgt_bitdepth, 0.f, rescale, 1, exp_low, exp_high,
+                            out_color, out_quality, out_depth, out_validity, out_y, out_u, out_v, nullptr);
+}
+
+namespace {
+int synthesize_frame(const orc_source* srcs, int n_src, const orc_cam* tgt, const float tgt_pos[3],
+                     const float tgt_rot[3], int tgt_bitdepth, float blending_exp, float rescale, int multispectral,
+                     float exp_low, float exp_high, float* out_color, float* out_quality, float* out_depth,
+                     float* out_validity, void* out_y, void* out_u, void* out_v, orc_timing* timing) {
     if (n_src <= 0) return -1;
     orc_timing tm;
     std::memset(&tm, 0, sizeof(tm));
@@ -651,6 +733,17 @@ int orc_synthesize(const orc_source* srcs, int n_src, const orc_cam* tgt,
     std::vector<float> bcol(3 * nout), bdep(nout), bq(nout), bval(nout);
     std::vector<uint8_t> bmask(nout);
     orc_blender bl{ow, oh, 1, blending_exp, bcol.data(), bdep.data(), bq.data(), bval.data(), bmask.data()};
+    /* multispectral: the low band in bl (its depth, quality and validity are the
+     * result's), the high band in bh, each with its own exponent */
+    std::vector<float> hcol, hdep, hq, hval, lowc, highc;
+    std::vector<uint8_t> hmask;
+    orc_blender bh{};
+    if (multispectral) {
+        bl.exp = exp_low;
+        hcol.resize(3 * nout); hdep.resize(nout); hq.resize(nout); hval.resize(nout); hmask.resize(nout);
+        lowc.resize(3 * nout); highc.resize(3 * nout);
+        bh = orc_blender{ow, oh, 1, exp_high, hcol.data(), hdep.data(), hq.data(), hval.data(), hmask.data()};
+    }
     std::vector<float> vcol(3 * nout), vdep(nout), vshape(nout);
 
     for (int s = 0; s < n_src; ++s) {
@@ -670,13 +763,22 @@ int orc_synthesize(const orc_source* srcs, int n_src, const orc_cam* tgt,
         orc_transform_triangles(col.data(), vd.data(), uv.data(), S.cam.w, S.cam.h, ow, oh, wrap, rescale,
                                 vcol.data(), vdep.data(), vshape.data());
         double t3 = now_s();
-        orc_blend_step(&bl, vcol.data(), vdep.data(), vshape.data());
+        if (multispectral) {
+            orc_split_frequencies(vcol.data(), vdep.data(), vshape.data(), ow, oh, lowc.data(), highc.data());
+            orc_blend_step(&bl, lowc.data(), vdep.data(), vshape.data());
+            orc_blend_step(&bh, highc.data(), vdep.data(), vshape.data());
+        } else {
+            orc_blend_step(&bl, vcol.data(), vdep.data(), vshape.data());
+        }
         double t4 = now_s();
         tm.t_decode += t1 - t0;
         tm.t_reproject += t2 - t1;
         tm.t_raster += t3 - t2;
         tm.t_blend += t4 - t3;
     }
+    /* multispectral result: low + high colour, the low band's maps (BlendedView.cpp:86) */
+    if (multispectral)
+        for (size_t k = 0; k < 3 * nout; ++k) bcol[k] = bcol[k] + hcol[k];
     if (out_quality) std::memcpy(out_quality, bq.data(), nout * sizeof(float));
     if (out_depth) std::memcpy(out_depth, bdep.data(), nout * sizeof(float));
     if (out_validity) std::memcpy(out_validity, bval.data(), nout * sizeof(float));
@@ -699,3 +801,4 @@ int orc_synthesize(const orc_source* srcs, int n_src, const orc_cam* tgt,
     if (timing) *timing = tm;
     return 0;
 }
+}  // namespace

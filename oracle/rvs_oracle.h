@@ -32,6 +32,12 @@
  *   orc_encode_yuv420                       Y pinned (closed form); U,V chroma rule
  *                                           restates OpenCV 4.9 resize(0.5) area-fast
  *                                           path: "parity unpinned" (OpenCV absent)
+ *   orc_blur_masked, orc_split_frequencies  pinned: brute-force window sums on tiny
+ *                                           images, masked-out pixels 0, clipped
+ *                                           windows' counts, (int) truncation of the
+ *                                           sum, low + high = input
+ *   orc_synthesize_multispectral            pinned: equal exponents with a radius-0
+ *                                           blur reduce to simple blending
  *   orc_resize_color                        write_color's cv::resize to the output size
  *                                           (Precision != 1): pinned by closed forms
  *                                           (constants, 2x block means of integer
@@ -109,6 +115,20 @@ typedef struct {
  * (SynthesizedView.cpp:440), then BlendedViewSimple::blendBasicMode. */
 void orc_blend_step(orc_blender* b, const float* color, const float* depth, const float* shape);
 
+/* ---- multispectral blending (BlendingMethod "Multispectral") ---- */
+/* detail::calcBlurring of one channel (blending.cpp:62-104) over the masked
+ * integral image of IntegralImage2D<float, 1> (IntegralImage2D.h:738-770):
+ * window (2 rad + 1)^2 clipped to the image, sums in double in the
+ * reference's recurrence order, (int) of the window sum, float division by
+ * the count of finite values under mask == 255; 0 where mask == 0.
+ * y: every `stride`-th float of the image (3: the Y channel of float3). */
+void orc_blur_masked(const float* y, int stride, const uint8_t* mask, int w, int h, int rad, float* out);
+/* BlendedViewMultiSpec::blend's split (BlendedView.cpp:70-80, split_frequencies
+ * blending.cpp:371-394, YUV): mask = shape/depth > 0, Y blurred with radius
+ * max(w, h) / 20; low = (Y blurred, U, V), high = colour - low. */
+void orc_split_frequencies(const float* color, const float* depth, const float* shape, int w, int h,
+                           float* low, float* high);
+
 /* ---- inpaint (stage d) ---- */
 /* detail::inpaintBasicMode (inpainting.cpp:213-223): mask = !(quality > 0)
  * (View.cpp:198-203), compute_nearest sweeps (:104-136), colour copy (:179-198).
@@ -164,6 +184,13 @@ int orc_synthesize(const orc_source* srcs, int n_src, const orc_cam* tgt,
                    float blending_exp, float rescale,
                    float* out_color, float* out_quality, float* out_depth, float* out_validity,
                    void* out_y, void* out_u, void* out_v, orc_timing* timing);
+/* The same with BlendedViewMultiSpec (BlendedView.cpp:62-88): every view is
+ * split into frequency bands, the low band blended with exp_low, the high band
+ * with exp_high; the result is low + high colour with the low band's maps. */
+int orc_synthesize_multispectral(const orc_source* srcs, int n_src, const orc_cam* tgt, const float tgt_pos[3],
+                                 const float tgt_rot[3], int tgt_bitdepth, float exp_low, float exp_high,
+                                 float rescale, float* out_color, float* out_quality, float* out_depth,
+                                 float* out_validity, void* out_y, void* out_u, void* out_v);
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,13 @@ def lib():
                                        C.c_float, C.c_int, fptr]
         L.orc_encode_yuv420.argtypes = [fptr, C.c_int, C.c_int, C.c_int, vp, vp, vp]
         L.orc_resize_color.argtypes = [fptr, C.c_int, C.c_int, fptr, C.c_int, C.c_int]
+        u8ptr = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+        L.orc_blur_masked.argtypes = [fptr, C.c_int, u8ptr, C.c_int, C.c_int, C.c_int, fptr]
+        L.orc_split_frequencies.argtypes = [fptr, fptr, fptr, C.c_int, C.c_int, fptr, fptr]
+        L.orc_synthesize_multispectral.argtypes = [C.POINTER(OrcSource), C.c_int, C.POINTER(OrcCam), fptr, fptr,
+                                                   C.c_int, C.c_float, C.c_float, C.c_float, vp, vp, vp, vp, vp, vp,
+                                                   vp]
+        L.orc_synthesize_multispectral.restype = C.c_int
         L.orc_max_level.argtypes = [C.c_int]
         L.orc_max_level.restype = C.c_uint
         L.orc_synthesize.argtypes = [C.POINTER(OrcSource), C.c_int, C.POINTER(OrcCam), fptr, fptr,
@@ -242,7 +249,27 @@ def resize_color(src, dw, dh):
     return dst
 
 
-def synthesize(sources, tgt_cam, tgt_pos, tgt_rot, tgt_bitdepth, blending_exp, rescale=1.0):
+def blur_masked(y, mask, rad):
+    """One-channel masked box blur of the multispectral split (h x w float, u8 mask)."""
+    y = f32(y)
+    h, w = y.shape
+    out = np.zeros((h, w), np.float32)
+    lib().orc_blur_masked(y, 1, np.ascontiguousarray(mask, np.uint8), w, h, int(rad), out)
+    return out
+
+
+def split_frequencies(color, depth, shape):
+    color, depth, shape = f32(color), f32(depth), f32(shape)
+    h, w = depth.shape
+    low = np.zeros((h, w, 3), np.float32)
+    high = np.zeros((h, w, 3), np.float32)
+    lib().orc_split_frequencies(color, depth, shape, w, h, low, high)
+    return low, high
+
+
+def synthesize(sources, tgt_cam, tgt_pos, tgt_rot, tgt_bitdepth, blending_exp, rescale=1.0, multispectral=None):
+    """multispectral: None (BlendingMethod Simple, exponent blending_exp) or
+    (low exponent, high exponent) for BlendingMethod Multispectral."""
     """sources: list of dicts with keys cam, pos, rot, bd_color, bd_depth, znear, zfar,
     has_invalid, y, u, v, depth (numpy planes).  Returns dict of outputs."""
     n = len(sources)
@@ -276,11 +303,13 @@ def synthesize(sources, tgt_cam, tgt_pos, tgt_rot, tgt_bitdepth, blending_exp, r
         "v": np.zeros((h // 2, w // 2), dt),
     }
     tm = OrcTiming()
-    rc = lib().orc_synthesize(arr, n, C.byref(tgt_cam), f32(tgt_pos), f32(tgt_rot), tgt_bitdepth,
-                              blending_exp, rescale,
-                              *(out[k].ctypes.data for k in ("color", "quality", "depth", "validity",
-                                                              "y", "u", "v")),
-                              C.byref(tm))
+    ptrs = [out[k].ctypes.data for k in ("color", "quality", "depth", "validity", "y", "u", "v")]
+    if multispectral is None:
+        rc = lib().orc_synthesize(arr, n, C.byref(tgt_cam), f32(tgt_pos), f32(tgt_rot), tgt_bitdepth,
+                                  blending_exp, rescale, *ptrs, C.byref(tm))
+    else:
+        rc = lib().orc_synthesize_multispectral(arr, n, C.byref(tgt_cam), f32(tgt_pos), f32(tgt_rot), tgt_bitdepth,
+                                                float(multispectral[0]), float(multispectral[1]), rescale, *ptrs)
     if rc != 0:
         raise RuntimeError("orc_synthesize failed: %d" % rc)
     out["timing"] = {k: getattr(tm, k) for k, _ in OrcTiming._fields_}
