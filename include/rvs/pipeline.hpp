@@ -19,6 +19,8 @@
 #include <string>
 #include <vector>
 
+#include <hip/hip_vector_types.h>
+
 #include "rvs/config.hpp"
 #include "rvs_hip.h"
 
@@ -127,23 +129,47 @@ private:
     DeviceBuffer m_uv, m_vdepth, m_ws;
 };
 
-/** Accumulator of synthesized views (BlendedView.hpp). */
+/** Accumulator of synthesized views (BlendedView.hpp).  blend() reads the
+ * view's colour, depth and validity (the triangle shape of a synthesized
+ * view). */
 class BlendedView : public View {
 public:
-    virtual void blend(SynthesizedView const& view, rvs_stream_t stream) = 0;
+    virtual void blend(View const& view, rvs_stream_t stream) = 0;
 };
 
 /** Weighted blending (BlendedViewSimple, BlendedView.cpp:180-212). */
 class BlendedViewSimple : public BlendedView {
 public:
     explicit BlendedViewSimple(float blending_exp) : m_exp(blending_exp) {}
-    void blend(SynthesizedView const& view, rvs_stream_t stream) override;
+    void blend(View const& view, rvs_stream_t stream) override;
+    /** The same on device maps of size s (float3 colour, depth, shape). */
+    void blend_maps(const float* color, const float* depth, const float* shape, Size s, rvs_stream_t stream);
     float exponent() const { return m_exp; }
 
 private:
     float m_exp;
     bool m_empty = true;
     DeviceBuffer m_mask;
+};
+
+/** Multispectral blending (BlendedViewMultiSpec, BlendedView.cpp:62-88): every
+ * view is split into a low and a high frequency band (rvs_split_frequencies;
+ * ColorSpace RGB blurs all three channels, YUV only Y), each band blended with
+ * its own exponent; the result is the sum of the bands' colours with the low
+ * band's depth, quality and validity. */
+class BlendedViewMultiSpec : public BlendedView {
+public:
+    BlendedViewMultiSpec(float exp_low_freq, float exp_high_freq, bool rgb = false)
+        : m_low_freq(exp_low_freq), m_high_freq(exp_high_freq), m_rgb(rgb) {}
+    void blend(View const& view, rvs_stream_t stream) override;
+    BlendedViewSimple const& low_band() const { return m_low_freq; }
+    BlendedViewSimple const& high_band() const { return m_high_freq; }
+
+private:
+    BlendedViewSimple m_low_freq, m_high_freq;
+    bool m_rgb;
+    bool m_empty = true;
+    DeviceBuffer m_ws, m_low, m_high;
 };
 
 /** The frame loop (Pipeline.hpp / Pipeline.cpp). */
@@ -167,6 +193,15 @@ protected:
     virtual bool wantMask() { return false; }
     virtual bool wantDepth() { return false; }
     virtual bool wantMaskedDepth() { return false; }
+    /** The inpainted colour of the GPU path (Pipeline.hpp:134; Pipeline.cpp:284-296):
+     * device float3 (Y, U, V) per pixel at `size`, the raster size (the view's
+     * size times Precision).  Called on a worker thread once the frame's work
+     * on stream() has completed; the buffer is left untouched until the call
+     * returns, and the next call waits for this one.  The default resizes it to
+     * the view's size, encodes it (write_color, image_writing.cpp:142-203) and
+     * calls saveColor(yuv, ...); for the colour computeView hands over it reuses
+     * the planes the frame's launch sequence already encoded. */
+    virtual void saveColor(float3* color, int virtualFrame, int virtualView, Parameters const& params, Size size);
     /** Encoded YUV420 planes of the inpainted view (Y, then U, V at half size). */
     virtual void saveColor(std::vector<uint8_t> const& yuv, int virtualFrame, int virtualView,
                            Parameters const& params);
@@ -178,6 +213,19 @@ protected:
                            Parameters const& params);
     virtual void saveMaskedDepth(std::vector<float> const& depth, std::vector<uint8_t> const& mask,
                                  int virtualFrame, int virtualView, Parameters const& params);
+    /** Per-view results of the view-by-view path (Pipeline.hpp:161-177): each
+     * input view once synthesized, the blender after each blend, the blender
+     * after the last one (before inpainting).  Called synchronously from
+     * computeView; the views' device maps are complete once stream() is
+     * synchronised.  The fused path (the built-in classes without
+     * setStagewise(true)) forms no per-view results and calls none of them;
+     * the reference declares them without calling them. */
+    virtual void onIntermediateSynthesisResult(int inputFrame, int inputView, int virtualFrame, int virtualView,
+                                               SynthesizedView const& synthesizedView) {}
+    virtual void onIntermediateBlendingResult(int inputFrame, int inputView, int virtualFrame, int virtualView,
+                                              BlendedView const& blendedView) {}
+    virtual void onFinalBlendingResult(int inputFrame, int virtualFrame, int virtualView,
+                                       BlendedView const& blendedView) {}
     /** fl(1/max_level) of the first view loaded by this pipeline
      * (function-static in read_color_YUV / read_depth_YUV, image_loading.cpp:134,193). */
     float colorScale(int bitdepth);
@@ -185,8 +233,21 @@ protected:
     rvs_stream_t stream() const { return m_stream; }
 
 private:
+    /** Device results of one output frame; two alternate so that a frame can
+     * be computed while the previous one is being saved. */
+    struct FrameOut {
+        DeviceBuffer color, quality, depth, validity, y, u, v, resized;
+        Size raster{0, 0}, view{0, 0};
+        int bitdepth = 0;
+        void* done = nullptr;   // hipEvent_t recorded on stream() after the frame's last launch
+    };
     void computeView(int inputFrame, int virtualFrame, int virtualView);
+    FrameOut* frameOf(const float3* color);
     rvs_stream_t m_stream = nullptr;
+    rvs_stream_t m_save_stream = nullptr;   // downloads / encodes of the save thread
+    int m_device = 0;                       // the device current at construction
+    FrameOut m_out[2];
+    unsigned m_frames = 0;
     bool m_stagewise = false;
     float m_color_scale = -1.f, m_depth_scale = -1.f;
     DeviceBuffer m_ws;
@@ -208,6 +269,7 @@ protected:
     bool wantMask() override { return !m_config.outmaskfilenames.empty(); }
     bool wantDepth() override { return !m_config.outdepthfilenames.empty(); }
     bool wantMaskedDepth() override { return !m_config.outmaskdepthfilenames.empty(); }
+    using Pipeline::saveColor;
     void saveColor(std::vector<uint8_t> const& yuv, int vf, int vv, Parameters const& p) override;
     void saveMaskedColor(std::vector<uint8_t> const& yuv, int vf, int vv, Parameters const& p) override;
     void saveMask(std::vector<uint8_t> const& mask, int vf, int vv, Parameters const& p) override;

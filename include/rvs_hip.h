@@ -140,6 +140,27 @@ int rvs_inpaint(float* color, const float* quality, int w, int h, void* workspac
  * pinned (no OpenCV here); the oracle restates the same rule.  float3 images. */
 int rvs_resize_color(const float* src, int sw, int sh, float* dst, int dw, int dh, rvs_stream_t stream);
 
+/* BlendedViewMultiSpec::blend's frequency split (BlendedView.cpp:70-80;
+ * split_frequencies, blending.cpp:371-394; calcBlurring, :62-159) of one
+ * synthesized view (color float3, depth, shape as rvs_raster writes them):
+ * mask = shape / depth > 0; a box blur of radius max(w, h) / 20 over the
+ * finite values under the mask, divided by their count (0 outside the mask);
+ * rgb == 0 (ColorSpace YUV) blurs Y, truncating each window sum to an integer
+ * as the reference does, and keeps U, V in the low band; rgb != 0 (ColorSpace
+ * RGB) blurs all three channels.  low = blurred colour, high = color - low.
+ * Window sums are exact fixed-point sums (2^-40 units) where the reference
+ * sums in double, so a sum lying within rounding distance of an integer (YUV)
+ * or of a float midpoint (RGB) may come out one step apart; colour values
+ * must satisfy |v| < 2^22.  workspace: rvs_split_frequencies_workspace_size
+ * (12 bytes per pixel for YUV, 28 for RGB; 0 for invalid sizes). */
+size_t rvs_split_frequencies_workspace_size(int w, int h, int rgb);
+int rvs_split_frequencies(const float* color, const float* depth, const float* shape, int w, int h, int rgb,
+                          void* workspace, size_t ws_bytes, float* low, float* high, rvs_stream_t stream);
+
+/* out = a + b over w*h float3 pixels: the multispectral result, low band plus
+ * high band colour (BlendedView.cpp:86). */
+int rvs_add_color(const float* a, const float* b, float* out, int w, int h, rvs_stream_t stream);
+
 /* Replaces quantization (process_input_frame.cuh:15) with the CPU semantics of
  * write_color (image_writing.cpp:142-203): Y = (T)(v*max) for every pixel, U,V
  * quantised at full resolution then 2x2 averaged as (a+b+c+d+2)>>2. */

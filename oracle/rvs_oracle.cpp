@@ -669,23 +669,68 @@ void orc_blur_masked(const float* y, int stride, const uint8_t* mask, int w, int
         }
 }
 
-void orc_split_frequencies(const float* color, const float* depth, const float* shape, int w, int h,
+void orc_blur_masked_rgb(const float* color, const uint8_t* mask, int w, int h, int rad, float* out) {
+    /* IntegralImage2D<float, 3> with a mask (IntegralImage2D.h:385-420): one
+     * double integral per channel in the same recurrence, a pixel counted when
+     * mask == 255 and cv::sum of its three channels (a double) is finite */
+    const size_t W1 = (size_t)w + 1;
+    std::vector<double> I(3 * W1 * (h + 1), 0.0);
+    std::vector<unsigned> N(W1 * (h + 1), 0u);
+    for (int r = 0; r < h; ++r)
+        for (int c = 0; c < w; ++c) {
+            const size_t a = (size_t)r * W1, b = (size_t)(r + 1) * W1;
+            for (int ch = 0; ch < 3; ++ch)
+                I[3 * (b + c + 1) + ch] = I[3 * (a + c + 1) + ch] + I[3 * (b + c) + ch] - I[3 * (a + c) + ch];
+            N[b + c + 1] = N[a + c + 1] + N[b + c] - N[a + c];
+            const float* v = color + 3 * ((size_t)r * w + c);
+            const double vs = ((double)v[0] + (double)v[1]) + (double)v[2];
+            if (mask[(size_t)r * w + c] == 255 && std::isfinite(vs)) {
+                for (int ch = 0; ch < 3; ++ch) I[3 * (b + c + 1) + ch] += v[ch];
+                N[b + c + 1] += 1;
+            }
+        }
+    /* calcBlurring, three channels (blending.cpp:109-159): the window's Vec3d
+     * sum converted to Vec3f (rounded, no integer truncation), then divided */
+    const int rad_1 = 2 * rad + 1;
+    for (int i = 0; i < h; ++i)
+        for (int j = 0; j < w; ++j) {
+            float* o = out + 3 * ((size_t)i * w + j);
+            if (mask[(size_t)i * w + j] == 0) { o[0] = o[1] = o[2] = 0.f; continue; }
+            int sx = j - rad, sy = i - rad, ey = sy + rad_1, ex = sx + rad_1;
+            sx = std::max(0, sx); sy = std::max(0, sy);
+            ex = std::min(ex, w); ey = std::min(ey, h);
+            const size_t ul = (size_t)sy * W1 + sx, ur = (size_t)sy * W1 + ex;
+            const size_t ll = (size_t)ey * W1 + sx, lr = (size_t)ey * W1 + ex;
+            const unsigned cnt = N[lr] + N[ul] - N[ur] - N[ll];
+            for (int ch = 0; ch < 3; ++ch) {
+                const float sum = (float)(I[3 * lr + ch] + I[3 * ul + ch] - I[3 * ur + ch] - I[3 * ll + ch]);
+                o[ch] = cnt > 0 ? float(sum / float(cnt)) : 0.f;
+            }
+        }
+}
+
+void orc_split_frequencies(const float* color, const float* depth, const float* shape, int w, int h, int rgb,
                            float* low, float* high) {
     /* BlendedViewMultiSpec::blend (BlendedView.cpp:70-88): mask = quality > 0
      * with quality = shape / depth (SynthesizedView.cpp:440); split_frequencies
-     * (blending.cpp:371-394), YUV: blur Y only, radius max(rows, cols) / 20 */
+     * (blending.cpp:371-394), radius max(rows, cols) / 20; ColorSpace YUV blurs
+     * Y only, RGB blurs all three channels */
     const size_t n = (size_t)w * h;
     std::vector<uint8_t> mask(n);
     for (size_t k = 0; k < n; ++k) mask[k] = (shape[k] / depth[k] > 0.f) ? 255 : 0;
     const int rad = std::max(w, h) / 20;
-    std::vector<float> yb(n);
-    orc_blur_masked(color, 3, mask.data(), w, h, rad, yb.data());
-    for (size_t k = 0; k < n; ++k) {
-        low[3 * k] = yb[k];
-        low[3 * k + 1] = color[3 * k + 1];
-        low[3 * k + 2] = color[3 * k + 2];
-        for (int c = 0; c < 3; ++c) high[3 * k + c] = color[3 * k + c] - low[3 * k + c];
+    if (rgb) {
+        orc_blur_masked_rgb(color, mask.data(), w, h, rad, low);
+    } else {
+        std::vector<float> yb(n);
+        orc_blur_masked(color, 3, mask.data(), w, h, rad, yb.data());
+        for (size_t k = 0; k < n; ++k) {
+            low[3 * k] = yb[k];
+            low[3 * k + 1] = color[3 * k + 1];
+            low[3 * k + 2] = color[3 * k + 2];
+        }
     }
+    for (size_t k = 0; k < 3 * n; ++k) high[k] = color[k] - low[k];
 }
 
 /* ------------------------------------------------------------- whole frame */
@@ -707,10 +752,11 @@ int orc_synthesize(const orc_source* srcs, int n_src, const orc_cam* tgt,
 }
 
 int orc_synthesize_multispectral(const orc_source* srcs, int n_src, const orc_cam* tgt, const float tgt_pos[3],
-                                 const float tgt_rot[3], int tgt_bitdepth, float exp_low, float exp_high,
+                                 const float tgt_rot[3], int tgt_bitdepth, float exp_low, float exp_high, int rgb,
                                  float rescale, float* out_color, float* out_quality, float* out_depth,
                                  float* out_validity, void* out_y, void* out_u, void* out_v) {
-    return synthesize_frame(srcs, n_src, tgt, tgt_pos, tgt_rot, tgt_bitdepth, 0.f, rescale, 1, exp_low, exp_high,
+    return synthesize_frame(srcs, n_src, tgt, tgt_pos, tgt_rot, tgt_bitdepth, 0.f, rescale, rgb ? 2 : 1, exp_low,
+                            exp_high,
                             out_color, out_quality, out_depth, out_validity, out_y, out_u, out_v, nullptr);
 }
 
@@ -764,7 +810,8 @@ int synthesize_frame(const orc_source* srcs, int n_src, const orc_cam* tgt, cons
                                 vcol.data(), vdep.data(), vshape.data());
         double t3 = now_s();
         if (multispectral) {
-            orc_split_frequencies(vcol.data(), vdep.data(), vshape.data(), ow, oh, lowc.data(), highc.data());
+            orc_split_frequencies(vcol.data(), vdep.data(), vshape.data(), ow, oh, multispectral == 2 ? 1 : 0,
+                                  lowc.data(), highc.data());
             orc_blend_step(&bl, lowc.data(), vdep.data(), vshape.data());
             orc_blend_step(&bh, highc.data(), vdep.data(), vshape.data());
         } else {

@@ -32,10 +32,12 @@
  *   orc_encode_yuv420                       Y pinned (closed form); U,V chroma rule
  *                                           restates OpenCV 4.9 resize(0.5) area-fast
  *                                           path: "parity unpinned" (OpenCV absent)
- *   orc_blur_masked, orc_split_frequencies  pinned: brute-force window sums on tiny
- *                                           images, masked-out pixels 0, clipped
+ *   orc_blur_masked, orc_blur_masked_rgb,   pinned: brute-force window sums on tiny
+ *   orc_split_frequencies                   images, masked-out pixels 0, clipped
  *                                           windows' counts, (int) truncation of the
- *                                           sum, low + high = input
+ *                                           Y sum, float rounding of the RGB sums, a
+ *                                           non-finite channel dropping the pixel,
+ *                                           low + high = input
  *   orc_synthesize_multispectral            pinned: equal exponents with a radius-0
  *                                           blur reduce to simple blending
  *   orc_resize_color                        write_color's cv::resize to the output size
@@ -123,10 +125,17 @@ void orc_blend_step(orc_blender* b, const float* color, const float* depth, cons
  * the count of finite values under mask == 255; 0 where mask == 0.
  * y: every `stride`-th float of the image (3: the Y channel of float3). */
 void orc_blur_masked(const float* y, int stride, const uint8_t* mask, int w, int h, int rad, float* out);
+/* The three-channel calcBlurring (blending.cpp:109-159) over
+ * IntegralImage2D<float, 3> (IntegralImage2D.h:385-420): a pixel counts when
+ * mask == 255 and the double sum of its channels is finite; each channel's
+ * double window sum is rounded to float (no integer truncation) and divided by
+ * the count; 0 where mask == 0.  color, out: float3. */
+void orc_blur_masked_rgb(const float* color, const uint8_t* mask, int w, int h, int rad, float* out);
 /* BlendedViewMultiSpec::blend's split (BlendedView.cpp:70-80, split_frequencies
- * blending.cpp:371-394, YUV): mask = shape/depth > 0, Y blurred with radius
- * max(w, h) / 20; low = (Y blurred, U, V), high = colour - low. */
-void orc_split_frequencies(const float* color, const float* depth, const float* shape, int w, int h,
+ * blending.cpp:371-394): mask = shape/depth > 0, radius max(w, h) / 20.
+ * rgb == 0 (ColorSpace YUV): Y blurred, low = (Y blurred, U, V); rgb != 0
+ * (ColorSpace RGB): all three channels blurred.  high = colour - low. */
+void orc_split_frequencies(const float* color, const float* depth, const float* shape, int w, int h, int rgb,
                            float* low, float* high);
 
 /* ---- inpaint (stage d) ---- */
@@ -188,7 +197,7 @@ int orc_synthesize(const orc_source* srcs, int n_src, const orc_cam* tgt,
  * split into frequency bands, the low band blended with exp_low, the high band
  * with exp_high; the result is low + high colour with the low band's maps. */
 int orc_synthesize_multispectral(const orc_source* srcs, int n_src, const orc_cam* tgt, const float tgt_pos[3],
-                                 const float tgt_rot[3], int tgt_bitdepth, float exp_low, float exp_high,
+                                 const float tgt_rot[3], int tgt_bitdepth, float exp_low, float exp_high, int rgb,
                                  float rescale, float* out_color, float* out_quality, float* out_depth,
                                  float* out_validity, void* out_y, void* out_u, void* out_v);
 

@@ -707,6 +707,30 @@ int rvs_resize_color(const float* src, int sw, int sh, float* dst, int dw, int d
     return RVS_OK;
 }
 
+size_t rvs_split_frequencies_workspace_size(int w, int h, int rgb) {
+    if (w < 1 || h < 1 || (uint64_t)w * h >= (1ull << 31)) return 0;
+    return rvs_k::split_ws_bytes((size_t)w * h, rgb);
+}
+
+int rvs_split_frequencies(const float* color, const float* depth, const float* shape, int w, int h, int rgb,
+                          void* workspace, size_t ws_bytes, float* low, float* high, rvs_stream_t stream) {
+    if (!color || !depth || !shape || !low || !high || !workspace || w < 1 || h < 1)
+        return fail(RVS_E_INVALID, "rvs_split_frequencies: bad argument");
+    if ((uint64_t)w * h >= (1ull << 31)) return fail(RVS_E_CAPACITY, "rvs_split_frequencies: image too large");
+    if (ws_bytes < rvs_split_frequencies_workspace_size(w, h, rgb))
+        return fail(RVS_E_INVALID, "rvs_split_frequencies: workspace too small");
+    HIP_TRY(rvs_k::launch_split_frequencies(color, depth, shape, w, h, rgb, workspace, low, high,
+                                            (hipStream_t)stream),
+            "split_frequencies");
+    return RVS_OK;
+}
+
+int rvs_add_color(const float* a, const float* b, float* out, int w, int h, rvs_stream_t stream) {
+    if (!a || !b || !out || w < 1 || h < 1) return fail(RVS_E_INVALID, "rvs_add_color: bad argument");
+    HIP_TRY(rvs_k::launch_add_color(a, b, out, 3 * (size_t)w * h, (hipStream_t)stream), "add_color");
+    return RVS_OK;
+}
+
 int rvs_encode_yuv420(const float* color, int w, int h, int bitdepth, void* y, void* u, void* v,
                       rvs_stream_t stream) {
     if (!color || !y || !u || !v || w < 2 || h < 2 || (w & 1) || (h & 1) || bitdepth < 1 || bitdepth > 16)

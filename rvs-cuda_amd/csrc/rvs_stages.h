@@ -26,6 +26,13 @@ hipError_t launch_encode(const float* color, int w, int h, int bitdepth, void* y
 hipError_t launch_encode_views(const ViewDesc* views, int nviews, int max_w, int max_h, hipStream_t s);
 // write_color's resize (image_writing.cpp:189-190) of one float3 image.
 hipError_t launch_resize_color(const float* src, int sw, int sh, float* dst, int dw, int dh, hipStream_t s);
+// Multispectral blending's frequency split of one float3 view (rgb: blur all
+// three channels, else Y only); ws: split_ws_bytes(w * h, rgb) bytes.
+size_t split_ws_bytes(size_t npx, int rgb);
+hipError_t launch_split_frequencies(const float* color, const float* depth, const float* shape, int w, int h,
+                                    int rgb, void* ws, float* low, float* high, hipStream_t s);
+// out[i] = a[i] + b[i] over n floats.
+hipError_t launch_add_color(const float* a, const float* b, float* out, size_t n, hipStream_t s);
 
 }  // namespace rvs_k
 

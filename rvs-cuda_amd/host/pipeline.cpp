@@ -311,26 +311,82 @@ void SynthetisedViewTriangle::compute(InputView& input, rvs_stream_t stream) {
 
 // ------------------------------------------------------ BlendedViewSimple
 
-void BlendedViewSimple::blend(SynthesizedView const& view, rvs_stream_t stream) {
-    const Size s = view.get_size();
+void BlendedViewSimple::blend(View const& view, rvs_stream_t stream) {
+    blend_maps(view.getDevColor(), view.getDevDepth(), view.getDevValidity(), view.get_size(), stream);
+}
+
+void BlendedViewSimple::blend_maps(const float* color, const float* depth, const float* shape, Size s,
+                                   rvs_stream_t stream) {
     if (m_empty) {
         allocate(s, true);
         m_mask.resize((std::size_t)s.area());
+    } else if (s.width != m_size.width || s.height != m_size.height) {
+        throw std::runtime_error("BlendedViewSimple: views of different sizes");
     }
-    check(rvs_blend_step(getDevColor(), getDevQuality(), getDevDepth(), getDevValidity(), m_mask.as<uint8_t>(),
-                         view.getDevColor(), view.getDevDepth(), view.getDevValidity(), s.width, s.height, m_exp,
-                         m_empty ? 1 : 0, stream),
+    check(rvs_blend_step(getDevColor(), getDevQuality(), getDevDepth(), getDevValidity(), m_mask.as<uint8_t>(), color,
+                         depth, shape, s.width, s.height, m_exp, m_empty ? 1 : 0, stream),
           "rvs_blend_step");
+    m_empty = false;
+}
+
+// --------------------------------------------------- BlendedViewMultiSpec
+
+void BlendedViewMultiSpec::blend(View const& view, rvs_stream_t stream) {
+    const Size s = view.get_size();
+    const std::size_t n = (std::size_t)s.area();
+    const int rgb = m_rgb ? 1 : 0;
+    if (m_empty) allocate(s, true);
+    else if (s.width != m_size.width || s.height != m_size.height)
+        throw std::runtime_error("BlendedViewMultiSpec: views of different sizes");
+    const std::size_t ws = rvs_split_frequencies_workspace_size(s.width, s.height, rgb);
+    if (ws == 0) check(RVS_E_INVALID, "rvs_split_frequencies_workspace_size");
+    m_ws.resize(ws);
+    m_low.resize(3 * n * sizeof(float));
+    m_high.resize(3 * n * sizeof(float));
+    check(rvs_split_frequencies(view.getDevColor(), view.getDevDepth(), view.getDevValidity(), s.width, s.height, rgb,
+                                m_ws.get(), m_ws.bytes(), m_low.as<float>(), m_high.as<float>(), stream),
+          "rvs_split_frequencies");
+    // the bands as views with the synthesized view's depth and validity (BlendedView.cpp:79-84)
+    m_low_freq.blend_maps(m_low.as<float>(), view.getDevDepth(), view.getDevValidity(), s, stream);
+    m_high_freq.blend_maps(m_high.as<float>(), view.getDevDepth(), view.getDevValidity(), s, stream);
+    // assign(low colour + high colour, the low band's maps) (BlendedView.cpp:86)
+    check(rvs_add_color(m_low_freq.getDevColor(), m_high_freq.getDevColor(), getDevColor(), s.width, s.height,
+                        stream),
+          "rvs_add_color");
+    const hipStream_t hs = (hipStream_t)stream;
+    hip_check(hipMemcpyAsync(getDevDepth(), m_low_freq.getDevDepth(), n * sizeof(float), hipMemcpyDeviceToDevice, hs),
+              "copy");
+    hip_check(hipMemcpyAsync(getDevQuality(), m_low_freq.getDevQuality(), n * sizeof(float),
+                             hipMemcpyDeviceToDevice, hs),
+              "copy");
+    hip_check(hipMemcpyAsync(getDevValidity(), m_low_freq.getDevValidity(), n * sizeof(float),
+                             hipMemcpyDeviceToDevice, hs),
+              "copy");
     m_empty = false;
 }
 
 // --------------------------------------------------------------- Pipeline
 
-Pipeline::Pipeline() { hip_check(hipStreamCreate((hipStream_t*)&m_stream), "hipStreamCreate"); }
+Pipeline::Pipeline() {
+    hip_check(hipGetDevice(&m_device), "hipGetDevice");
+    hip_check(hipStreamCreate((hipStream_t*)&m_stream), "hipStreamCreate");
+    hip_check(hipStreamCreateWithFlags((hipStream_t*)&m_save_stream, hipStreamNonBlocking), "hipStreamCreate");
+    for (FrameOut& f : m_out)
+        hip_check(hipEventCreateWithFlags((hipEvent_t*)&f.done, hipEventDisableTiming), "hipEventCreate");
+}
 
 Pipeline::~Pipeline() {
     if (m_save.valid()) m_save.wait();
+    for (FrameOut& f : m_out)
+        if (f.done) (void)hipEventDestroy((hipEvent_t)f.done);
+    if (m_save_stream) (void)hipStreamDestroy((hipStream_t)m_save_stream);
     if (m_stream) (void)hipStreamDestroy((hipStream_t)m_stream);
+}
+
+Pipeline::FrameOut* Pipeline::frameOf(const float3* color) {
+    for (FrameOut& f : m_out)
+        if (f.color.get() == (const void*)color) return &f;
+    return nullptr;
 }
 
 float Pipeline::colorScale(int bitdepth) {
@@ -366,7 +422,9 @@ std::unique_ptr<BlendedView> Pipeline::createBlender(int) {
     if (getConfig().blending_method == BlendingMethod::simple)
         return std::unique_ptr<BlendedView>(new BlendedViewSimple(getConfig().blending_factor));
     if (getConfig().blending_method == BlendingMethod::multispectral)
-        throw std::runtime_error("Multispectral blending is not supported by this build");
+        return std::unique_ptr<BlendedView>(new BlendedViewMultiSpec(getConfig().blending_low_freq_factor,
+                                                                     getConfig().blending_high_freq_factor,
+                                                                     getConfig().color_space == "RGB"));
     std::ostringstream m;
     m << "Unknown view blending method \"" << getConfig().blending_method << "\"";
     throw std::runtime_error(m.str());
@@ -457,6 +515,32 @@ std::vector<uint8_t> concat_planes(DeviceBuffer const& y, DeviceBuffer const& u,
 
 }  // namespace
 
+void Pipeline::saveColor(float3* color, int virtualFrame, int virtualView, Parameters const& params, Size size) {
+    const Size vs = params.getSize();
+    const int bd = params.getColorBitDepth();
+    const std::size_t npx = (std::size_t)vs.area(), e = bytes_per(bd);
+    FrameOut* f = frameOf(color);
+    if (f && f->view == vs && f->raster == size && f->bitdepth == bd) {
+        // planes encoded by the frame's own launch sequence
+        hip_check(hipEventSynchronize((hipEvent_t)f->done), "frame wait");
+        saveColor(concat_planes(f->y, f->u, f->v, npx * e, npx / 4 * e, m_save_stream), virtualFrame, virtualView,
+                  params);
+        return;
+    }
+    // a colour of the caller's: resize to the view's size, encode (write_color)
+    DeviceBuffer resized, y(npx * e), u(npx / 4 * e), v(npx / 4 * e);
+    const float* src = reinterpret_cast<const float*>(color);
+    if (size != vs) {
+        resized.resize(3 * npx * sizeof(float));
+        check(rvs_resize_color(src, size.width, size.height, resized.as<float>(), vs.width, vs.height, m_save_stream),
+              "rvs_resize_color");
+        src = resized.as<float>();
+    }
+    check(rvs_encode_yuv420(src, vs.width, vs.height, bd, y.get(), u.get(), v.get(), m_save_stream),
+          "rvs_encode_yuv420");
+    saveColor(concat_planes(y, u, v, npx * e, npx / 4 * e, m_save_stream), virtualFrame, virtualView, params);
+}
+
 void Pipeline::computeView(int inputFrame, int virtualFrame, int virtualView) {
     Config const& c = getConfig();
     detail::g_rescale = c.precision;
@@ -503,9 +587,21 @@ void Pipeline::computeView(int inputFrame, int virtualFrame, int virtualView) {
                                  "with the resized mask (Pipeline.cpp:307-318), which OpenCV rejects");
     const int bd = params_virtual.getColorBitDepth();
     const std::size_t e = bytes_per(bd);
-    DeviceBuffer color(3 * nr * sizeof(float)), quality(nr * sizeof(float)), depth(nr * sizeof(float)),
-        validity(nr * sizeof(float));
-    DeviceBuffer y(npx * e), u(npx / 4 * e), v(npx / 4 * e);
+    // this frame's results; the save of the frame that used them last has
+    // finished (it is waited for before the next frame's save starts)
+    FrameOut& fo = m_out[m_frames++ & 1];
+    fo.color.resize(3 * nr * sizeof(float));
+    fo.quality.resize(nr * sizeof(float));
+    fo.depth.resize(nr * sizeof(float));
+    fo.validity.resize(nr * sizeof(float));
+    fo.y.resize(npx * e);
+    fo.u.resize(npx / 4 * e);
+    fo.v.resize(npx / 4 * e);
+    fo.raster = rs;
+    fo.view = vs;
+    fo.bitdepth = bd;
+    DeviceBuffer &color = fo.color, &quality = fo.quality, &depth = fo.depth, &validity = fo.validity;
+    DeviceBuffer &y = fo.y, &u = fo.u, &v = fo.v;
     const rvs_cam tcam = cam_of(params_virtual);
     if (fused) {
         std::vector<rvs_source> srcs;
@@ -540,8 +636,11 @@ void Pipeline::computeView(int inputFrame, int virtualFrame, int virtualView) {
             transformer->set_inputPosition(&c.params_real[k]);
             synths[k]->setSpaceTransformer(transformer.get());
             synths[k]->compute(*inputs[k], m_stream);
+            onIntermediateSynthesisResult(inputFrame, k, virtualFrame, virtualView, *synths[k]);
             blender->blend(*synths[k], m_stream);
+            onIntermediateBlendingResult(inputFrame, k, virtualFrame, virtualView, *blender);
         }
+        onFinalBlendingResult(inputFrame, virtualFrame, virtualView, *blender);
         hip_check(hipMemcpyAsync(color.get(), blender->getDevColor(), 3 * nr * sizeof(float),
                                  hipMemcpyDeviceToDevice, (hipStream_t)m_stream), "copy");
         hip_check(hipMemcpyAsync(quality.get(), blender->getDevQuality(), nr * sizeof(float),
@@ -554,21 +653,28 @@ void Pipeline::computeView(int inputFrame, int virtualFrame, int virtualView) {
         check(rvs_inpaint(color.as<float>(), quality.as<float>(), rs.width, rs.height, iws.get(), iws.bytes(),
                           nullptr, m_stream),
               "rvs_inpaint");
-        DeviceBuffer resized(rescaled ? 3 * npx * sizeof(float) : 0);
-        if (rescaled)
-            check(rvs_resize_color(color.as<float>(), rs.width, rs.height, resized.as<float>(), vs.width, vs.height,
-                                   m_stream),
+        if (rescaled) {
+            fo.resized.resize(3 * npx * sizeof(float));
+            check(rvs_resize_color(color.as<float>(), rs.width, rs.height, fo.resized.as<float>(), vs.width,
+                                   vs.height, m_stream),
                   "rvs_resize_color");
-        check(rvs_encode_yuv420(rescaled ? resized.as<float>() : color.as<float>(), vs.width, vs.height, bd, y.get(),
-                                u.get(), v.get(), m_stream),
+        }
+        check(rvs_encode_yuv420(rescaled ? fo.resized.as<float>() : color.as<float>(), vs.width, vs.height, bd,
+                                y.get(), u.get(), v.get(), m_stream),
               "rvs_encode_yuv420");
     }
+    hip_check(hipEventRecord((hipEvent_t)fo.done, (hipStream_t)m_stream), "hipEventRecord");
     inputs.clear();
     if (m_save.valid()) m_save.get();
     if (wantColor()) {
-        auto yuv = concat_planes(y, u, v, npx * e, npx / 4 * e, m_stream);
-        m_save = std::async(std::launch::async, [this, yuv, virtualFrame, virtualView, params_virtual]() {
-            saveColor(yuv, virtualFrame, virtualView, params_virtual);
+        // the reference's GPU path hands the inpainted device colour to saveColor
+        // on a worker thread (Pipeline.cpp:284-296)
+        float3* dev = reinterpret_cast<float3*>(color.get());
+        void* done = fo.done;
+        m_save = std::async(std::launch::async, [this, dev, done, virtualFrame, virtualView, params_virtual, rs]() {
+            hip_check(hipSetDevice(m_device), "hipSetDevice");
+            hip_check(hipEventSynchronize((hipEvent_t)done), "frame wait");
+            saveColor(dev, virtualFrame, virtualView, params_virtual, rs);
         });
     }
     // masks and depth maps (Pipeline.cpp:303-336)

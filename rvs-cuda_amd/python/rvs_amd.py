@@ -92,6 +92,9 @@ def hip():
             "rvs_inpaint": ([vp, vp, i, i, vp, sz, vp, vp], i),
             "rvs_encode_yuv420": ([vp, i, i, i, vp, vp, vp, vp], i),
             "rvs_resize_color": ([vp, i, i, vp, i, i, vp], i),
+            "rvs_split_frequencies_workspace_size": ([i, i, i], sz),
+            "rvs_split_frequencies": ([vp, vp, vp, i, i, i, vp, sz, vp, vp, vp], i),
+            "rvs_add_color": ([vp, vp, vp, i, i, vp], i),
             "rvs_synth_workspace_size": ([P(RvsSource), i, P(RvsTarget), u32], sz),
             "rvs_synthesize": ([P(RvsSource), i, P(RvsTarget), vp, sz, u32, P(RvsViewOut), P(u32), vp], i),
             "rvs_synth_batch_workspace_size": ([P(RvsSource), i, P(RvsTarget), i, u32], sz),
@@ -302,6 +305,26 @@ def inpaint(color, quality, map_out=None, workspace=None, stream=None):
     check(hip().rvs_inpaint(_ptr(color), _ptr(quality), w, h, _ptr(workspace), need, _ptr(map_out),
                             _stream(stream)))
     return workspace
+
+
+def split_frequencies(color, depth, shape, low, high, rgb=False, workspace=None, stream=None):
+    """Multispectral blending's frequency split of one synthesized view
+    (BlendedView.cpp:70-80): low / high bands into `low`, `high` (h x w x 3)."""
+    import torch
+    h, w = depth.shape
+    need = hip().rvs_split_frequencies_workspace_size(w, h, int(bool(rgb)))
+    if need == 0:
+        raise RvsError(RVS_E_INVALID, "split_frequencies: bad image size %dx%d" % (w, h))
+    if workspace is None or workspace.numel() < need:
+        workspace = torch.empty(need, dtype=torch.uint8, device=depth.device)
+    check(hip().rvs_split_frequencies(_ptr(color), _ptr(depth), _ptr(shape), w, h, int(bool(rgb)), _ptr(workspace),
+                                      need, _ptr(low), _ptr(high), _stream(stream)))
+    return workspace
+
+
+def add_color(a, b, out, stream=None):
+    h, w = a.shape[:2]
+    check(hip().rvs_add_color(_ptr(a), _ptr(b), _ptr(out), w, h, _stream(stream)))
 
 
 def encode_yuv420(color, bitdepth, y, u, v, stream=None):
@@ -557,11 +580,16 @@ class StagewiseSynthesizer:
     (synchronous status check), blend; then inpaint and encode.  Same results
     as Synthesizer."""
 
-    def __init__(self, sources, target, device="cuda"):
+    def __init__(self, sources, target, device="cuda", multispectral=None, rgb=False):
+        """multispectral: None (BlendingMethod Simple with target.blending_exp)
+        or (low exponent, high exponent): BlendedViewMultiSpec, ColorSpace RGB
+        when rgb."""
         import torch
         self.sources = sources
         self.target = target
         self.device = device
+        self.multispectral = multispectral
+        self.rgb = rgb
         self.dec = []
         for s in sources:
             c = s.cam
@@ -582,6 +610,11 @@ class StagewiseSynthesizer:
         self.acc = {"color": torch.empty((rh, rw, 3), **f32), "quality": torch.empty((rh, rw), **f32),
                     "depth": torch.empty((rh, rw), **f32), "validity": torch.empty((rh, rw), **f32),
                     "mask": torch.empty((rh, rw), dtype=torch.uint8, device=device)}
+        if multispectral is not None:
+            # the two bands' accumulators and the split of the current view
+            self.bands = [{k: torch.empty_like(v) for k, v in self.acc.items()} for _ in range(2)]
+            self.split = (torch.empty((rh, rw, 3), **f32), torch.empty((rh, rw, 3), **f32))
+            self.sws = None
         # write_color resizes an oversampled colour to the view's size before quantising
         self.resized = torch.empty((h, w, 3), **f32) if (rw, rh) != (w, h) else None
         dt = torch.uint8 if target.bitdepth <= 8 else torch.int16
@@ -601,7 +634,21 @@ class StagewiseSynthesizer:
             reproject(s.cam, t.cam, Rm, tv, dep, s.table, t.rescale, uv, vd, stream)
             self.rws = raster(col, vd, uv, w, h, wrap, t.rescale, self.rc, self.rd, self.rs, workspace=self.rws,
                               stream=stream)
-            blend_step(self.acc, self.rc, self.rd, self.rs, t.blending_exp, k == 0, stream)
+            if self.multispectral is None:
+                blend_step(self.acc, self.rc, self.rd, self.rs, t.blending_exp, k == 0, stream)
+                continue
+            low, high = self.split
+            self.sws = split_frequencies(self.rc, self.rd, self.rs, low, high, self.rgb, self.sws, stream)
+            blend_step(self.bands[0], low, self.rd, self.rs, self.multispectral[0], k == 0, stream)
+            blend_step(self.bands[1], high, self.rd, self.rs, self.multispectral[1], k == 0, stream)
+        if self.multispectral is not None:
+            # low + high colour with the low band's maps (BlendedView.cpp:86)
+            add_color(self.bands[0]["color"], self.bands[1]["color"], self.acc["color"], stream)
+            import torch
+            with torch.cuda.stream(torch.cuda.ExternalStream(stream) if stream is not None else
+                                   torch.cuda.current_stream()):
+                for key in ("quality", "depth", "validity"):
+                    self.acc[key].copy_(self.bands[0][key])
         self.iws = inpaint(self.acc["color"], self.acc["quality"], workspace=self.iws, stream=stream)
         enc = self.acc["color"]
         if self.resized is not None:

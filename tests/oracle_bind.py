@@ -73,10 +73,11 @@ def lib():
         L.orc_resize_color.argtypes = [fptr, C.c_int, C.c_int, fptr, C.c_int, C.c_int]
         u8ptr = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
         L.orc_blur_masked.argtypes = [fptr, C.c_int, u8ptr, C.c_int, C.c_int, C.c_int, fptr]
-        L.orc_split_frequencies.argtypes = [fptr, fptr, fptr, C.c_int, C.c_int, fptr, fptr]
+        L.orc_blur_masked_rgb.argtypes = [fptr, u8ptr, C.c_int, C.c_int, C.c_int, fptr]
+        L.orc_split_frequencies.argtypes = [fptr, fptr, fptr, C.c_int, C.c_int, C.c_int, fptr, fptr]
         L.orc_synthesize_multispectral.argtypes = [C.POINTER(OrcSource), C.c_int, C.POINTER(OrcCam), fptr, fptr,
-                                                   C.c_int, C.c_float, C.c_float, C.c_float, vp, vp, vp, vp, vp, vp,
-                                                   vp]
+                                                   C.c_int, C.c_float, C.c_float, C.c_int, C.c_float, vp, vp, vp, vp,
+                                                   vp, vp, vp]
         L.orc_synthesize_multispectral.restype = C.c_int
         L.orc_max_level.argtypes = [C.c_int]
         L.orc_max_level.restype = C.c_uint
@@ -258,19 +259,30 @@ def blur_masked(y, mask, rad):
     return out
 
 
-def split_frequencies(color, depth, shape):
+def blur_masked_rgb(color, mask, rad):
+    """Three-channel masked box blur (ColorSpace RGB), h x w x 3 float."""
+    color = f32(color)
+    h, w = color.shape[:2]
+    out = np.zeros((h, w, 3), np.float32)
+    lib().orc_blur_masked_rgb(color, np.ascontiguousarray(mask, np.uint8), w, h, int(rad), out)
+    return out
+
+
+def split_frequencies(color, depth, shape, rgb=False):
     color, depth, shape = f32(color), f32(depth), f32(shape)
     h, w = depth.shape
     low = np.zeros((h, w, 3), np.float32)
     high = np.zeros((h, w, 3), np.float32)
-    lib().orc_split_frequencies(color, depth, shape, w, h, low, high)
+    lib().orc_split_frequencies(color, depth, shape, w, h, int(bool(rgb)), low, high)
     return low, high
 
 
-def synthesize(sources, tgt_cam, tgt_pos, tgt_rot, tgt_bitdepth, blending_exp, rescale=1.0, multispectral=None):
+def synthesize(sources, tgt_cam, tgt_pos, tgt_rot, tgt_bitdepth, blending_exp, rescale=1.0, multispectral=None,
+               rgb=False):
     """multispectral: None (BlendingMethod Simple, exponent blending_exp) or
-    (low exponent, high exponent) for BlendingMethod Multispectral."""
-    """sources: list of dicts with keys cam, pos, rot, bd_color, bd_depth, znear, zfar,
+    (low exponent, high exponent) for BlendingMethod Multispectral; rgb: its
+    ColorSpace RGB split (all channels blurred).
+    sources: list of dicts with keys cam, pos, rot, bd_color, bd_depth, znear, zfar,
     has_invalid, y, u, v, depth (numpy planes).  Returns dict of outputs."""
     n = len(sources)
     arr = (OrcSource * n)()
@@ -309,7 +321,8 @@ def synthesize(sources, tgt_cam, tgt_pos, tgt_rot, tgt_bitdepth, blending_exp, r
                                   blending_exp, rescale, *ptrs, C.byref(tm))
     else:
         rc = lib().orc_synthesize_multispectral(arr, n, C.byref(tgt_cam), f32(tgt_pos), f32(tgt_rot), tgt_bitdepth,
-                                                float(multispectral[0]), float(multispectral[1]), rescale, *ptrs)
+                                                float(multispectral[0]), float(multispectral[1]), int(bool(rgb)),
+                                                rescale, *ptrs)
     if rc != 0:
         raise RuntimeError("orc_synthesize failed: %d" % rc)
     out["timing"] = {k: getattr(tm, k) for k, _ in OrcTiming._fields_}
