@@ -181,7 +181,7 @@ def main():
         rg_ls = W.rig(name)
         srcs_ls = W.device_sources(rg_ls, W.render_sources(rg_ls))
         tg = [t for _, t in W.serve_requests([rg_ls], int(n), seed=4, sizes=((w, h, 1 if erp else 0),))]
-        bs = R.BatchSynthesizer(srcs_ls, tg)
+        bs = R.BatchSynthesizer(srcs_ls, tg, want_color=False)
         for _ in range(5):
             bs.run()
         torch.cuda.synchronize()
@@ -200,7 +200,7 @@ def main():
     torch.cuda.synchronize()
     if args.roofline_only > 0:
         walk = roofline_walk(W, rg, args.roofline_only)
-        bs = R.BatchSynthesizer(srcs, walk)
+        bs = R.BatchSynthesizer(srcs, walk, want_color=False)
         for _ in range(5):
             bs.run()
         torch.cuda.synchronize()
@@ -210,7 +210,7 @@ def main():
         return
 
     # ---------------------------------------------------------- plain per-view path
-    syn = R.Synthesizer(srcs, tgt, want_maps=False)
+    syn = R.Synthesizer(srcs, tgt, want_maps=False, want_color=False)
     for _ in range(3):
         syn.run()
     torch.cuda.synchronize()
@@ -271,7 +271,7 @@ def main():
 
     # ------------------- dominant kernel: one launch set of ROOFLINE_VIEWS views
     rwalk = roofline_walk(W, rg)
-    bs = R.BatchSynthesizer(srcs, rwalk)
+    bs = R.BatchSynthesizer(srcs, rwalk, want_color=False)
     for _ in range(5):
         bs.run()
     torch.cuda.synchronize()
@@ -283,7 +283,7 @@ def main():
 
     batched = None
     if args.batch > 0:
-        bb = R.BatchSynthesizer(srcs, walk[: args.batch], entry_cap=cap * args.batch)
+        bb = R.BatchSynthesizer(srcs, walk[: args.batch], entry_cap=cap * args.batch, want_color=False)
         bb.run()
         torch.cuda.synchronize()
         times = []
@@ -301,7 +301,7 @@ def main():
         # one launch set captured in a HIP graph and replayed
         tg = W.batch_targets(rg, 64, seed=3)
         cap3, _ = A.tune_entry_cap(srcs, tg[:15])
-        bs3 = R.BatchSynthesizer(srcs, tg, entry_cap=min(cap3 * len(tg), 0xFFFFFFFF))
+        bs3 = R.BatchSynthesizer(srcs, tg, entry_cap=min(cap3 * len(tg), 0xFFFFFFFF), want_color=False)
         bs3.run()
         cap3 = bs3.entry_cap
         torch.cuda.synchronize()

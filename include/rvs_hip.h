@@ -34,6 +34,7 @@ typedef struct ihipStream_t* rvs_stream_t;   /* hipStream_t */
 #define RVS_E_HIP (-2)       /* HIP runtime error */
 #define RVS_E_CAPACITY (-3)  /* the binned-entry list or a workspace limit was exceeded; see rvs_last_error */
 #define RVS_E_UNSUPPORTED (-4)
+#define RVS_E_TIMEOUT (-5)   /* a bounded device-side wait gave up; the call's results are not valid */
 
 #define RVS_MAX_SOURCES 64
 
@@ -214,9 +215,14 @@ typedef struct {
     float depth_scale;        /* idem for depth (image_loading.cpp:193) */
 } rvs_target;
 
-/* Device outputs of one synthesized view; any pointer may be NULL except color.
- * The float maps are at the raster size, (int)(0.5f + w * rescale) x
- * (int)(0.5f + h * rescale) (w x h at Precision 1); the encoded planes at w x h. */
+/* Device outputs of one synthesized view; any pointer may be NULL, but a view
+ * needs the colour map or the three planes.  The float maps are at the raster
+ * size, (int)(0.5f + w * rescale) x (int)(0.5f + h * rescale) (w x h at
+ * Precision 1); the encoded planes at w x h.  A view with planes and no colour
+ * map keeps its colour in the workspace: tiles with neither holes nor pixels
+ * left to the in-order replay are then encoded by the raster pass directly and
+ * their colour is stored only where inpainting can read it (less traffic, the
+ * same planes). */
 typedef struct {
     float* color;             /* raster size * 3, inpainted (float3 Y,U,V) */
     float* quality;           /* raster size, blended quality before inpainting */

@@ -109,6 +109,13 @@ size_t prep_color_offset(const rvs_cam* cam) {
     return align256(rvs_k::src_bounds_floats(cam->w, cam->h) * sizeof(float));
 }
 
+// The decoded colour plane of a prepared source is packed (8 B per pixel)
+// when its upsampled chroma is a multiple of 1/4 below 2^14: even sizes, bit
+// depth <= 14.
+bool prep_packed(const rvs_source& S) {
+    return S.bitdepth_color <= 14 && ((S.cam.w | S.cam.h) & 1) == 0;
+}
+
 // Sequential sub-allocation of a workspace, 256-byte aligned.
 struct Layout {
     size_t o = 0;
@@ -125,7 +132,8 @@ struct Layout {
 // deferred-pixel flags, deferred-tile list and binning work list.
 struct RasterPlan {
     rvs_k::BatchDims d;
-    size_t status, counts, single, multi, offsets, fill, lists, ranges, vbuf, tvq, imap, bsum, deferred, dtiles, work;
+    size_t status, counts, single, multi, offsets, fill, lists, ranges, vbuf, tvq, imap, bsum, deferred, dtiles, work,
+        tile_enc;
 };
 
 // List capacity: the caller's, or one entry per output pixel and source
@@ -177,6 +185,7 @@ void plan_raster(Layout& L, RasterPlan& R, int nsrc, int nviews, uint32_t ntiles
     R.deferred = L.take(npx);
     R.dtiles = L.take((size_t)ntiles * sizeof(uint32_t));
     R.work = L.take((size_t)items * sizeof(uint32_t));
+    R.tile_enc = L.take((size_t)ntiles);
 }
 
 rvs_k::RasterWS carve_raster(char* base, const RasterPlan& R) {
@@ -196,6 +205,7 @@ rvs_k::RasterWS carve_raster(char* base, const RasterPlan& R) {
     ws.deferred = (uint8_t*)(base + R.deferred);
     ws.deferred_tiles = (uint32_t*)(base + R.dtiles);
     ws.work = (uint32_t*)(base + R.work);
+    ws.tile_enc = (uint8_t*)(base + R.tile_enc);
     return ws;
 }
 
@@ -223,6 +233,9 @@ int check_raster_status(const rvs_k::RasterWS& ws, const rvs_k::BatchDims& d, ui
     uint32_t st[4];
     HIP_TRY(hipMemcpyAsync(st, ws.status, sizeof st, hipMemcpyDeviceToHost, s), "status copy");
     HIP_TRY(hipStreamSynchronize(s), "stream sync");
+    if (st[3])
+        return fail(RVS_E_TIMEOUT, "inpainting: a bounded wait between strip solvers gave up; the launch set's "
+                                   "results are not valid");
     if (st[1]) {
         if (needed_cap) *needed_cap = st[2];
         if (st[2] == 0xffffffffu)
@@ -238,7 +251,7 @@ int check_raster_status(const rvs_k::RasterWS& ws, const rvs_k::BatchDims& d, ui
 // in workspace order.
 struct SynthPlan {
     RasterPlan R;
-    size_t views, geoms, bounds, quality, inpaint, resized, total;
+    size_t views, geoms, bounds, quality, color, inpaint, resized, total;
     int max_w, max_h;     // largest raster (oversampled by Precision)
     int max_ew, max_eh;   // largest view
     bool any_persp, any_rescaled;
@@ -299,6 +312,8 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
     for (int k = 0; k < n_src; ++k) nb += align256(rvs_k::src_bounds_floats(srcs[k].cam.w, srcs[k].cam.h) * 4);
     P.bounds = L.take(P.any_persp ? nb : 0);
     P.quality = L.take((size_t)px * sizeof(float));
+    // the colour map of views whose caller wants only the encoded planes
+    P.color = L.take((size_t)px * 3 * sizeof(float));
     plan_raster(L, P.R, n_src, n_views, (uint32_t)tiles, (uint32_t)px, cap, max_sw, max_sh, keep_max);
     P.inpaint = L.take(rvs_k::inpaint_ws_bytes((size_t)px, n_views));
     // oversampled views: their colour resized to the view's size, for the encoder
@@ -334,7 +349,9 @@ void fill_pair(PairGeom& g, const rvs_source& S, const rvs_target& T) {
     g.py = S.y; g.pu = S.u; g.pv = S.v;
     g.color_u8 = S.bitdepth_color <= 8;
     g.cscale = T.color_scale;
-    g.col = S.prep ? (const float4*)((const char*)S.prep + prep_color_offset(&S.cam)) : nullptr;
+    const void* plane = S.prep ? (const char*)S.prep + prep_color_offset(&S.cam) : nullptr;
+    g.col = plane && !prep_packed(S) ? (const float4*)plane : nullptr;
+    g.colq = plane && prep_packed(S) ? (const ushort4*)plane : nullptr;
     rvs_xform xf;
     rvs_relative_pose(S.pos, S.rot, T.pos, T.rot, &xf);
     std::memcpy(g.R, xf.R, sizeof g.R);
@@ -360,23 +377,29 @@ int enqueue_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int
     }
     char* base = (char*)workspace;
     float* quality = (float*)(base + P.quality);
+    float* icolor = (float*)(base + P.color);
     float* resized = (float*)(base + P.resized);
     for (int v = 0; v < n_views; ++v) {
         const rvs_view_out& o = outs[v];
         rvs_k::ViewDesc& d = views[v];
-        if (!o.color) return fail(RVS_E_INVALID, "synthesize: view %d has no colour output", v);
         const bool enc = o.y && o.u && o.v;
+        if (!o.color && !enc)
+            return fail(RVS_E_INVALID, "synthesize: view %d has neither a colour map nor planes", v);
         if (enc && ((d.ew & 1) || (d.eh & 1)))
             return fail(RVS_E_UNSUPPORTED, "synthesize: odd target size cannot be encoded (view %d)", v);
-        if (d.w != d.ew || d.h != d.eh) {
+        float* color = o.color ? o.color : icolor + (size_t)d.px_base * 3;
+        const bool rescaled = d.w != d.ew || d.h != d.eh;
+        if (rescaled) {
             d.ecol = resized;
             resized += (size_t)d.ew * d.eh * 3;
         } else {
-            d.ecol = o.color;
+            d.ecol = color;
         }
         if (enc && (d.bitdepth < 1 || d.bitdepth > 16))
             return fail(RVS_E_INVALID, "synthesize: view %d output bit depth %d", v, d.bitdepth);
-        d.color = o.color;
+        d.color = color;
+        d.enc_raster = enc && !rescaled;
+        d.partial_color = !o.color && !rescaled;
         d.quality = o.quality ? o.quality : quality + d.px_base;
         d.depth = o.depth;
         d.validity = o.validity;
@@ -432,11 +455,12 @@ int enqueue_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int
         HIP_TRY(hipStreamWaitEvent(sf, split_ev, 0), "split wait");
         s = sf;
     }
-    const rvs_k::InpaintWS iws = rvs_k::inpaint_ws_carve(base + P.inpaint, d.npx, n_views);
+    rvs_k::InpaintWS iws = rvs_k::inpaint_ws_carve(base + P.inpaint, d.npx, n_views);
+    iws.fail = ws.status + 3;   // a bounded wait of the inpainting that gave up
     HIP_TRY(rvs_k::launch_inpaint(dviews, n_views, P.max_w, P.max_h, iws, nullptr, s), "inpaint");
     prof_mark(4, s);
     if (P.any_rescaled) HIP_TRY(rvs_k::launch_resize_views(dviews, n_views, P.max_ew, P.max_eh, s), "resize");
-    HIP_TRY(rvs_k::launch_encode_views(dviews, n_views, P.max_ew, P.max_eh, s), "encode");
+    HIP_TRY(rvs_k::launch_encode_views(dviews, n_views, P.max_ew, P.max_eh, ws.tile_enc, s), "encode");
     prof_mark(5, s);
     return RVS_OK;
 }
@@ -566,7 +590,8 @@ int rvs_source_prep(rvs_source* src, float depth_scale, void* out, size_t bytes,
     g.znear = src->znear; g.zfar = src->zfar; g.has_invalid = src->has_invalid;
     HIP_TRY(rvs_k::launch_src_bounds(g, (float*)out, (hipStream_t)stream), "source bounds");
     HIP_TRY(rvs_k::launch_prep_color(src->y, src->u, src->v, src->bitdepth_color <= 8, src->cam.w, src->cam.h,
-                                     (float4*)((char*)out + prep_color_offset(&src->cam)), (hipStream_t)stream),
+                                     prep_packed(*src) ? 1 : 0, (char*)out + prep_color_offset(&src->cam),
+                                     (hipStream_t)stream),
             "source colour");
     src->prep = (const float*)out;
     src->prep_depth_scale = depth_scale;

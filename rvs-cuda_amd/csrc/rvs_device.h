@@ -59,8 +59,12 @@ struct PairGeom {
     // (6 floats each, bounds_bx blocks per row) for frustum culling; null = no culling
     const float* bounds;
     int bounds_bx;
-    // resident decoded colour per source pixel (color_raw: Y, U, V, 0), or null
+    // resident decoded colour per source pixel (color_raw: Y, U, V), or null:
+    // colq holds Y, 4 U, 4 V as integers (8 B per pixel) for even source sizes
+    // and bit depths <= 14, where the upsampled chroma is a multiple of 1/4;
+    // col holds the floats (16 B per pixel) otherwise.  At most one is set.
     const float4* col;
+    const ushort4* colq;
 };
 
 // Precomputed-array vertex source (the raster entry point on identical inputs).
@@ -221,6 +225,10 @@ __device__ inline float3 color_raw(const PairGeom& g, int i, int j) {
 // Decoded colour times the colour scale (the same product either way, from
 // the resident decoded plane when the source has one).
 __device__ inline float3 color(const PairGeom& g, int i, int j) {
+    if (g.colq) {
+        const ushort4 q = g.colq[(size_t)i * g.sw + j];
+        return make_float3((float)q.x * g.cscale, ((float)q.y * 0.25f) * g.cscale, ((float)q.z * 0.25f) * g.cscale);
+    }
     if (g.col) {
         const float4 r = g.col[(size_t)i * g.sw + j];
         return make_float3(r.x * g.cscale, r.y * g.cscale, r.z * g.cscale);

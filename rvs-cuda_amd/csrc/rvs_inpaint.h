@@ -12,16 +12,24 @@ namespace rvs_k {
 
 constexpr int kInpaintBlocks = 1024;   // persistent component solvers
 constexpr int kInpaintBigBlocks = 256;  // solvers of large components (one 1024-thread block each)
+constexpr int kStripBlocks = 512;       // persistent strip solvers (256 threads each)
 
 // Indexed by global pixel (a view's pixels start at its px_base).
 struct InpaintWS {
     int* label;     // -1 known pixel, else union-find parent (root after flattening)
-    int* dist;      // distance of the map (inpainting.cpp:182-188)
-    int* src;       // global index of the chosen known pixel
+    int* dist;      // distance of the map (inpainting.cpp:182-188); holes only
+    int* src;       // global index of the chosen known pixel; holes only
     int* cid;       // component id of a root
     int* comp;      // 4 per component: x0, x1, y0, y1 (view coordinates)
     int* root;      // root pixel of a component
-    int* counters;  // [0] components, [2] work counter of the small-component pass
+    int* counters;  // [0] components, [2] work counter of the small-component pass, [3] strip tasks,
+                    // [4] / [5] tickets of the forward / backward strip sweeps, [6] components left to
+                    // the one-block solver, [7] strip carry slots in use
+    int4* tasks;    // strip tasks: component, strip | strips << 8 | cols << 16, first carry slot
+    unsigned long long* slots[2];  // per (task, line) scan carries of the forward / backward sweep
+    int* legacy;    // large components the strip plan could not place
+    uint32_t* fail; // set when a bounded wait gives up (may be null)
+    int task_cap, slot_cap;
 };
 
 size_t inpaint_ws_bytes(size_t npx, int nviews);

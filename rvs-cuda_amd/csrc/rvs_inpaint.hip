@@ -90,11 +90,11 @@ __device__ inline void lunite(int* lab, int a, int b) {
     }
 }
 
-// Inpainting state of one 32x32 tile and its labels in LDS: every pixel gets
-// the map's initial (source, distance) -- (itself, 0) when known, (itself,
-// w + h) for a hole (inpaint_color, inpainting.cpp:185-191) -- known pixels
-// the label -1, and each hole pixel the global index of its tile-local root
-// (the smallest local index of its local component).
+// Inpainting state of one 32x32 tile and its labels in LDS: known pixels get
+// the label -1 (their map value (itself, 0) is implied by it and never
+// stored), each hole pixel the map's initial (itself, w + h) (inpaint_color,
+// inpainting.cpp:185-191) and the global index of its tile-local root (the
+// smallest local index of its local component).
 __global__ void __launch_bounds__(256) k_ccl_local(const ViewDesc* __restrict__ views, InpaintWS ws) {
     __shared__ int lab[kCclTile * kCclTile];
     const ViewDesc& vd = views[blockIdx.z];
@@ -108,9 +108,13 @@ __global__ void __launch_bounds__(256) k_ccl_local(const ViewDesc* __restrict__ 
         if (x < w && y < h) {
             const int p = y * w + x, gp = base + p;
             hole = !(vd.quality[p] > 0.f);   // View::get_inpaint_mask (View.cpp:198-203)
-            ws.dist[gp] = hole ? (w + h) : 0;
-            ws.src[gp] = gp;
-            if (!hole) ws.label[gp] = -1;
+            // a known pixel's map value is (itself, 0), implied by its label
+            if (hole) {
+                ws.dist[gp] = w + h;
+                ws.src[gp] = gp;
+            } else {
+                ws.label[gp] = -1;
+            }
         }
         lab[k] = hole ? k : -1;
     }
@@ -706,6 +710,341 @@ __device__ void solve_lines(const Region& R, const InpaintWS& ws, bool cols, Fn*
     }
 }
 
+// ---------------------------------------------------------- strip solver
+//
+// Large components whose lines fit kLineMax are solved with each line split
+// into strips of kStripPx scan positions, one wave per strip (no barrier: the
+// line scan is a wave scan), so that a line step's work is spread over many
+// CUs instead of one.  Strip k
+// of a line takes as its scan input the state after strip k-1's last pixel of
+// the same line (its carry); every other value a strip reads lies in its own
+// columns -- the previous line's values of this sweep are kept in registers,
+// the next line still holds the previous sweep's -- or is the pixel just past
+// its end, which the next strip writes only after taking this strip's carry of
+// that line.  Carries travel through one 64-bit slot per (strip, line), zeroed
+// before the sweep and written once with a single atomic store, so the reader
+// needs no other ordering.  The forward and the backward sweep are separate
+// launches (the backward one reads the forward result).  Within a launch the
+// tasks (component, strip) are handed out by ticket with the strips of a
+// component in scan order, so a strip waits only for a task whose ticket was
+// taken before its own, by a block that is already running: no wait depends on
+// a block that has not started.  Waits are bounded anyway: after kSpinLimit
+// polls a strip gives up, takes an absent carry and sets ws.fail, which the
+// host reports as a failed launch set.  Both sweeps always run (a backward
+// sweep after a forward one that changed nothing changes nothing either).
+
+constexpr int kStripThreads = 64;          // one wave
+constexpr int kStripLanePx = 2;            // consecutive scan positions per lane
+constexpr int kStripPx = kStripThreads * kStripLanePx;
+constexpr int kSpinLimit = 1 << 20;       // polls (each with a short sleep) before a wait gives up
+
+__device__ inline unsigned long long pack_carry(int v, int s) {
+    return ((unsigned long long)(uint32_t)(v | (1 << 30)) << 32) | (uint32_t)(s + 1);
+}
+__device__ inline void unpack_carry(unsigned long long c, int& v, int& s) {
+    v = (int)((uint32_t)(c >> 32) & ((1u << 30) - 1u));
+    s = (int)(uint32_t)c - 1;
+}
+__device__ inline unsigned long long load_slot(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wave-wide inclusive scan of Fn with DPP row shifts and row broadcasts
+// (lanes without a source take the identity, which compose() leaves alone);
+// ex: the composite of the lanes before this one (identity for lane 0).
+__device__ inline Fn fn_dpp_ident() {
+    Fn f;
+    f.cv = 1 << 30; f.cs = -1; f.t = 0; f.k = 0;
+    return f;
+}
+template <int CTRL, int ROWS>
+__device__ inline Fn fn_dpp(const Fn& a) {
+    const Fn id = fn_dpp_ident();
+    Fn r;
+    r.cv = __builtin_amdgcn_update_dpp(id.cv, a.cv, CTRL, ROWS, 0xf, false);
+    r.cs = __builtin_amdgcn_update_dpp(id.cs, a.cs, CTRL, ROWS, 0xf, false);
+    r.t = __builtin_amdgcn_update_dpp(id.t, a.t, CTRL, ROWS, 0xf, false);
+    r.k = __builtin_amdgcn_update_dpp(id.k, a.k, CTRL, ROWS, 0xf, false);
+    return r;
+}
+__device__ inline Fn wave_scan_fn(Fn A, Fn& ex) {
+    A = compose(fn_dpp<0x111, 0xf>(A), A);   // row_shr:1
+    A = compose(fn_dpp<0x112, 0xf>(A), A);   // row_shr:2
+    A = compose(fn_dpp<0x114, 0xf>(A), A);   // row_shr:4
+    A = compose(fn_dpp<0x118, 0xf>(A), A);   // row_shr:8
+    A = compose(fn_dpp<0x142, 0xa>(A), A);   // row_bcast:15 into rows 1, 3
+    A = compose(fn_dpp<0x143, 0xc>(A), A);   // row_bcast:31 into rows 2, 3
+    ex = fn_dpp<0x138, 0xf>(A);              // wave_shr:1
+    return A;
+}
+
+// Strip s of component R in one sweep: scan positions [s P, min(llen,
+// (s+1) P)) of every line, P = kStripPx, kStripLanePx consecutive ones per
+// lane; in_slots / out_slots: the carries of the strip before / this strip,
+// one per line (out_slots null for the last).  Same candidate order and line
+// order semantics as solve_lines.  The raw map words of a line are loaded
+// kRing - 1 steps before their first use (a ring of kRing lines in registers,
+// indexed statically by unrolling the line loop kRing times) and interpreted
+// only when used, so no step waits on a load issued in the same step.
+constexpr int kRing = 4;
+
+struct RawPx {
+    int lab, d, s;
+};
+
+template <bool FIRST>
+__device__ void solve_strip(const Region& R, const InpaintWS& ws, bool cols, bool fwd, int s,
+                            const unsigned long long* in_slots, unsigned long long* out_slots) {
+    constexpr int PX = kStripLanePx;
+    const int lane = threadIdx.x & 63;
+    const int nlines = cols ? R.bw : R.bh;
+    const int llen = cols ? R.bh : R.bw;
+    const int e0 = s * kStripPx, e1 = min(llen, e0 + kStripPx);
+    const int me = e0 + lane * PX;   // this lane's first scan position
+    const int rs = cols ? (fwd ? 2 : 3) : (fwd ? 1 : 4);
+    const int r_prev = cols ? (fwd ? 1 : 4) : (fwd ? 2 : 3);
+    const int r_next = cols ? (fwd ? 4 : 1) : (fwd ? 3 : 2);
+    const int r_along = cols ? (fwd ? 3 : 2) : (fwd ? 4 : 1);
+    auto lcoord = [&](int li) { return fwd ? li : nlines - 1 - li; };
+    // view index of (line coordinate lc, scan index e); false outside the image
+    auto addr = [&](int lc, int e, int& q) {
+        const int pos = fwd ? e : llen - 1 - e;
+        const int x = cols ? R.x0 + lc : R.x0 + pos;
+        const int y = cols ? R.y0 + pos : R.y0 + lc;
+        const bool in = x >= 0 && x < R.w && y >= 0 && y < R.h;
+        q = R.base + (in ? y * R.w + x : 0);
+        return in;
+    };
+    auto fetch = [&](int lc, int e, RawPx& r) {
+        int q;
+        addr(lc, e, q);
+        r.lab = ws.label[q];
+        r.d = ws.dist[q];
+        r.s = ws.src[q];
+    };
+    auto value = [&](int lc, int e, const RawPx& r, int& v, int& sr) {
+        int q;
+        const bool in = addr(lc, e, q);
+        map_value(R, r.lab, r.d, r.s, q, v, sr);
+        return in;
+    };
+    // ring: this lane's pixels, the along-line neighbour after them, (first
+    // strip) the pixel before the line, (later strips) the carry of the strip
+    // before -- zero while not yet written; every lane reads the same slot
+    RawPx rp[kRing][PX], ra[kRing], rl[kRing];
+    unsigned long long rc[kRing];
+#pragma unroll
+    for (int k = 0; k < kRing; ++k) {
+#pragma unroll
+        for (int j = 0; j < PX; ++j) fetch(lcoord(k), me + j, rp[k][j]);
+        fetch(lcoord(k), me + PX, ra[k]);
+        if (FIRST) fetch(lcoord(k), -1, rl[k]);
+        else rc[k] = load_slot(&in_slots[min(k, nlines - 1)]);
+    }
+    int pv[PX], ps[PX];   // the previous line (this sweep) at this lane's positions
+    bool pin[PX];
+#pragma unroll
+    for (int j = 0; j < PX; ++j) {
+        RawPx r;
+        fetch(lcoord(-1), me + j, r);
+        pin[j] = value(lcoord(-1), me + j, r, pv[j], ps[j]);
+    }
+    bool gave_up = false;   // after one wait gave up, later ones do not wait
+    auto step = [&](int li, const int k) {
+        const int k1 = (k + 1) % kRing;
+        const int lc = lcoord(li), lcn = lcoord(li + 1);
+        int sv[PX], ss[PX];
+        bool sin[PX];
+#pragma unroll
+        for (int j = 0; j < PX; ++j) sin[j] = value(lc, me + j, rp[k][j], sv[j], ss[j]);
+        int av, as_;
+        const bool ain = value(lc, me + PX, ra[k], av, as_);
+        // this lane's functions: past the strip (kAbsent, -1, 0), known (0,
+        // itself), a hole of another component (kAbsent, -1, 0), a member: its
+        // best candidate among the neighbours other than the scan input
+        Fn f[PX];
+#pragma unroll
+        for (int j = 0; j < PX; ++j) {
+            int nv, ns;
+            const bool nin = value(lcn, me + j, rp[k1][j], nv, ns);
+            const int alv = j + 1 < PX ? sv[j + 1] : av, als = j + 1 < PX ? ss[j + 1] : as_;
+            const bool alin = j + 1 < PX ? sin[j + 1] : ain;
+            const bool valid = me + j < e1;
+            const bool known = valid & (sv[j] == 0);
+            const bool member = valid & (sv[j] != 0) & (sv[j] != kOther);
+            int bv = sv[j], br = 0, bs = ss[j];
+            cand_if(pin[j] & (pv[j] != kOther), bv, br, bs, pv[j] + 1, r_prev, ps[j]);
+            cand_if(nin & (nv != kOther), bv, br, bs, nv + 1, r_next, ns);
+            cand_if(alin & (alv != kOther), bv, br, bs, alv + 1, r_along, als);
+            f[j].k = 1;
+            f[j].cv = known ? 0 : (member ? bv : kAbsent);
+            f[j].cs = known ? ss[j] : (member ? bs : -1);
+            f[j].t = (member & (rs < br)) ? 1 : 0;
+        }
+        // the scan input of the strip's first position
+        int Sv, Ss;
+        if (FIRST) {
+            int v, sr;
+            const bool in = value(lc, -1, rl[k], v, sr);
+            const bool known = in && v == 0;
+            Sv = known ? 0 : kAbsent;
+            Ss = known ? sr : -1;
+        } else {
+            // the carry of the strip before, loaded kRing steps ahead; polled
+            // again (bounded) while it is not yet written
+            unsigned long long c = rc[k];
+            for (int n = 0; c == 0ull; ++n) {
+                if (gave_up || n >= kSpinLimit) {
+                    if (ws.fail && lane == 0) atomicOr(ws.fail, 1u);
+                    gave_up = true;
+                    c = pack_carry(kAbsent, -1);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                c = load_slot(&in_slots[li]);
+            }
+            unpack_carry(c, Sv, Ss);
+        }
+        // line li + kRing into the slots this line frees
+#pragma unroll
+        for (int j = 0; j < PX; ++j) fetch(lcoord(li + kRing), me + j, rp[k][j]);
+        fetch(lcoord(li + kRing), me + PX, ra[k]);
+        if (FIRST) fetch(lcoord(li + kRing), -1, rl[k]);
+        else rc[k] = load_slot(&in_slots[min(li + kRing, nlines - 1)]);   // past the end: a slot never read
+        Fn A = f[0];
+#pragma unroll
+        for (int j = 1; j < PX; ++j) A = compose(A, f[j]);
+        Fn ex;
+        wave_scan_fn(A, ex);
+        apply_fn(ex, Sv, Ss);   // the identity for lane 0
+#pragma unroll
+        for (int j = 0; j < PX; ++j) {
+            apply_fn(f[j], Sv, Ss);
+            const int e = me + j;
+            const bool member = e < e1 && sv[j] != 0 && sv[j] != kOther;
+            if (member && Sv != sv[j]) {
+                int q;
+                addr(lc, e, q);
+                ws.dist[q] = Sv;
+                ws.src[q] = Ss;
+            }
+            if (out_slots && e == e1 - 1)
+                __hip_atomic_store(&out_slots[li], pack_carry(Sv, Ss), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            pv[j] = member ? Sv : sv[j];
+            ps[j] = member ? Ss : ss[j];
+            pin[j] = sin[j];
+        }
+    };
+    for (int li0 = 0; li0 < nlines; li0 += kRing) {
+#pragma unroll
+        for (int k = 0; k < kRing; ++k)
+            if (li0 + k < nlines) step(li0 + k, k);
+    }
+}
+
+// Which large components the strip solver takes: tasks (component, strip) in
+// component order with the strips of a component consecutive, and their carry
+// slots, while both fit the workspace; the rest go to the one-block solver.
+// One block: a scan over the components in chunks of 1024.
+__global__ void __launch_bounds__(1024) k_inp_plan(InpaintWS ws) {
+    __shared__ int wsum[2][16];
+    __shared__ int s_base[2];
+    __shared__ int s_used[2];
+    __shared__ int s_legacy;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ncomp = ws.counters[0];
+    if (tid == 0) { s_base[0] = s_base[1] = 0; s_used[0] = s_used[1] = 0; s_legacy = 0; }
+    __syncthreads();
+    for (int c0 = 0; c0 < ncomp; c0 += 1024) {
+        const int c = c0 + tid;
+        int S = 0, nslots = 0;
+        bool big = false, cols = false;
+        if (c < ncomp) {
+            const int bw = ws.comp[4 * c + 1] - ws.comp[4 * c + 0] + 1;
+            const int bh = ws.comp[4 * c + 3] - ws.comp[4 * c + 2] + 1;
+            big = !(bw * bh <= kLdsCap || (bw > kLineMax && bh > kLineMax));
+            if (big) {
+                cols = bh <= kLineMax && (bw > kLineMax || bh > bw);   // the orientation of k_inp_big
+                const int nlines = cols ? bw : bh, llen = cols ? bh : bw;
+                S = (llen + kStripPx - 1) / kStripPx;
+                nslots = S * nlines;
+            }
+        }
+        // inclusive block scan of (S, nslots) in component order
+        int a = S, b = nslots;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int oa = __shfl_up(a, off), ob = __shfl_up(b, off);
+            if (lane >= off) { a += oa; b += ob; }
+        }
+        if (lane == 63) { wsum[0][wave] = a; wsum[1][wave] = b; }
+        __syncthreads();
+        for (int w = 0; w < wave; ++w) { a += wsum[0][w]; b += wsum[1][w]; }
+        const int ta = s_base[0] + a, tb = s_base[1] + b;   // inclusive ends
+        // prefix sums only grow, so the accepted components are a prefix of
+        // the candidates and their tasks leave no gaps
+        const bool fits = big && ta <= ws.task_cap && tb <= ws.slot_cap;
+        if (fits) {
+            for (int k = 0; k < S; ++k)
+                ws.tasks[ta - S + k] = make_int4(c, k | (S << 8) | (cols ? 1 << 16 : 0), tb - nslots, 0);
+            atomicMax(&s_used[0], ta);
+            atomicMax(&s_used[1], tb);
+        } else if (big) {
+            ws.legacy[atomicAdd(&s_legacy, 1)] = c;
+        }
+        __syncthreads();
+        // clamped past the capacities: nothing later fits, and the sums cannot overflow
+        if (tid == 1023) { s_base[0] = min(ta, ws.task_cap + 1); s_base[1] = min(tb, ws.slot_cap + 1); }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        ws.counters[3] = s_used[0];
+        ws.counters[6] = s_legacy;
+        ws.counters[7] = s_used[1];
+    }
+}
+
+// Zeroes the carry slots the plan handed out (both sweeps).
+__global__ void k_inp_slot_reset(InpaintWS ws) {
+    const size_t n = (size_t)ws.counters[7];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        ws.slots[0][i] = 0ull;
+        ws.slots[1][i] = 0ull;
+    }
+}
+
+__device__ inline int view_of_pixel(const ViewDesc* views, int nviews, int gp);
+
+template <bool FWD>
+__global__ void __launch_bounds__(kStripThreads) k_inp_strips(const ViewDesc* __restrict__ views, int nviews,
+                                                              InpaintWS ws) {
+    const int ntasks = ws.counters[3];
+    while (true) {
+        // one wave: the ticket is taken by lane 0 and broadcast
+        int t = 0;
+        if (threadIdx.x == 0) t = atomicAdd(&ws.counters[FWD ? 4 : 5], 1);
+        t = __shfl(t, 0);
+        if (t >= ntasks) break;
+        const int4 task = ws.tasks[t];
+        const int c = task.x, s = task.y & 0xff, S = (task.y >> 8) & 0xff;
+        const bool cols = (task.y >> 16) & 1;
+        Region R;
+        R.root = ws.root[c];
+        const ViewDesc& vd = views[view_of_pixel(views, nviews, R.root)];
+        R.base = (int)vd.px_base;
+        R.x0 = ws.comp[4 * c + 0];
+        R.y0 = ws.comp[4 * c + 2];
+        R.bw = ws.comp[4 * c + 1] - R.x0 + 1;
+        R.bh = ws.comp[4 * c + 3] - R.y0 + 1;
+        R.w = vd.w; R.h = vd.h;
+        R.ldist = nullptr; R.lsrc = nullptr;
+        const int nlines = cols ? R.bw : R.bh;
+        unsigned long long* sl = ws.slots[FWD ? 0 : 1] + task.z;
+        unsigned long long* out = s + 1 < S ? sl + (size_t)s * nlines : nullptr;
+        if (s == 0) solve_strip<true>(R, ws, cols, FWD, s, nullptr, out);
+        else solve_strip<false>(R, ws, cols, FWD, s, sl + (size_t)(s - 1) * nlines, out);
+    }
+}
+
 // View owning global pixel gp (views are sorted by px_base).
 __device__ inline int view_of_pixel(const ViewDesc* views, int nviews, int gp) {
     int lo = 0, hi = nviews - 1;
@@ -765,7 +1104,7 @@ __global__ void __launch_bounds__(256) k_inp_solve(const ViewDesc* __restrict__ 
                 const int p = R.base + y * R.w + x;
                 const int lab = ws.label[p];
                 ldist[k] = lab < 0 ? 0 : (lab == R.root ? ws.dist[p] : kOther);
-                lsrc[k] = lab == R.root || lab < 0 ? ws.src[p] : -1;
+                lsrc[k] = lab < 0 ? p : (lab == R.root ? ws.src[p] : -1);
             }
             __syncthreads();
             solve_component<true>(R, ws, wave_tot, s_carry);
@@ -784,14 +1123,15 @@ __global__ void __launch_bounds__(256) k_inp_solve(const ViewDesc* __restrict__ 
 }
 
 // Components too large for the LDS copy whose lines (along one side of the
-// box) fit one step of a 1024-thread block: one block each, static stride.
+// box) fit one step of a 1024-thread block and that the strip plan could not
+// place: one block each, static stride.
 __global__ void __launch_bounds__(kBigThreads) k_inp_big(const ViewDesc* __restrict__ views, int nviews,
                                                          InpaintWS ws) {
     __shared__ Fn wave_tot[2 * kBigThreads / 64];
-    const int ncomp = ws.counters[0];
-    for (int c = blockIdx.x; c < ncomp; c += gridDim.x) {
+    const int nlegacy = ws.counters[6];
+    for (int k = blockIdx.x; k < nlegacy; k += gridDim.x) {
+        const int c = ws.legacy[k];
         const int bw = ws.comp[4 * c + 1] - ws.comp[4 * c + 0] + 1, bh = ws.comp[4 * c + 3] - ws.comp[4 * c + 2] + 1;
-        if (bw * bh <= kLdsCap || (bw > kLineMax && bh > kLineMax)) continue;
         Region R;
         R.root = ws.root[c];
         const ViewDesc& vd = views[view_of_pixel(views, nviews, R.root)];
@@ -812,13 +1152,14 @@ __global__ void __launch_bounds__(kBigThreads) k_inp_big(const ViewDesc* __restr
 // launches only) receives source x, source y and distance per pixel.
 __global__ void k_inp_apply(const ViewDesc* __restrict__ views, InpaintWS ws, int32_t* map_out) {
     RVS_VIEW_PIXEL(vd, p, gp)
-    const int s = ws.src[gp] - (int)vd.px_base;
+    const bool hole = ws.label[gp] >= 0;
+    const int s = hole ? ws.src[gp] - (int)vd.px_base : p;
     if (map_out) {
         map_out[3 * p] = s % vd.w;
         map_out[3 * p + 1] = s / vd.w;
-        map_out[3 * p + 2] = ws.dist[gp];
+        map_out[3 * p + 2] = hole ? ws.dist[gp] : 0;
     }
-    if (ws.label[gp] >= 0 && s != p) {
+    if (hole && s != p) {
         float* color = vd.color;
         color[3 * p] = color[3 * s];
         color[3 * p + 1] = color[3 * s + 1];
@@ -826,15 +1167,35 @@ __global__ void k_inp_apply(const ViewDesc* __restrict__ views, InpaintWS ws, in
     }
 }
 
+namespace {
+// Strip tasks: a large component spans at least its box's longer side in
+// pixels and has a box over kLdsCap pixels, so npx / 64 tasks cover every
+// realistic case; carry slots: one per strip and line.  Components beyond
+// either bound go to the one-block solver.
+size_t task_cap_of(size_t npx, int nviews) { return npx / 64 + 16 * (size_t)nviews + 16; }
+size_t slot_cap_of(size_t npx, int nviews) { return npx / 16 + 4096 * (size_t)nviews; }
+size_t legacy_cap_of(size_t npx, int nviews) { return npx / 128 + 2 * (size_t)nviews + 1; }
+size_t a256(size_t b) { return (b + 255) & ~(size_t)255; }
+}  // namespace
+
 size_t inpaint_ws_bytes(size_t npx, int nviews) {
     const size_t maxc = npx / 2 + (size_t)nviews + 1;
-    return 4 * (4 * npx) + 4 * (4 * maxc) + 4 * maxc + 64;
+    return a256(task_cap_of(npx, nviews) * sizeof(int4)) +
+           2 * a256(slot_cap_of(npx, nviews) * sizeof(unsigned long long)) +
+           a256(4 * legacy_cap_of(npx, nviews)) + 4 * (4 * npx) + 4 * (4 * maxc) + 4 * maxc + 64;
 }
 
 InpaintWS inpaint_ws_carve(void* base, size_t npx, int nviews) {
     const size_t maxc = npx / 2 + (size_t)nviews + 1;
-    int* p = (int*)base;
+    char* b = (char*)base;   // 256-byte aligned
     InpaintWS ws;
+    ws.task_cap = (int)task_cap_of(npx, nviews);
+    ws.slot_cap = (int)slot_cap_of(npx, nviews);
+    ws.tasks = (int4*)b; b += a256(task_cap_of(npx, nviews) * sizeof(int4));
+    ws.slots[0] = (unsigned long long*)b; b += a256(slot_cap_of(npx, nviews) * sizeof(unsigned long long));
+    ws.slots[1] = (unsigned long long*)b; b += a256(slot_cap_of(npx, nviews) * sizeof(unsigned long long));
+    ws.legacy = (int*)b; b += a256(4 * legacy_cap_of(npx, nviews));
+    int* p = (int*)b;
     ws.label = p; p += npx;
     ws.dist = p; p += npx;
     ws.src = p; p += npx;
@@ -842,13 +1203,14 @@ InpaintWS inpaint_ws_carve(void* base, size_t npx, int nviews) {
     ws.comp = p; p += 4 * maxc;
     ws.root = p; p += maxc;
     ws.counters = p;
+    ws.fail = nullptr;
     return ws;
 }
 
 hipError_t launch_inpaint(const ViewDesc* views, int nviews, int max_w, int max_h, const InpaintWS& ws,
                           int32_t* map_out, hipStream_t s) {
     const dim3 b(256), g((unsigned)(((size_t)max_w * max_h + 255) / 256), (unsigned)nviews);
-    hipError_t e = launch_reset_u32((uint32_t*)ws.counters, 3, s);
+    hipError_t e = launch_reset_u32((uint32_t*)ws.counters, 8, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_ccl_local,
                        dim3((max_w + kCclTile - 1) / kCclTile, (max_h + kCclTile - 1) / kCclTile, nviews), b, 0, s,
@@ -858,6 +1220,10 @@ hipError_t launch_inpaint(const ViewDesc* views, int nviews, int max_w, int max_
     const size_t per_block = (size_t)kBboxThreads * kBboxPx;
     hipLaunchKernelGGL(k_comp_bbox, dim3((unsigned)(((size_t)max_w * max_h + per_block - 1) / per_block), (unsigned)nviews),
                        dim3(kBboxThreads), 0, s, views, ws);
+    hipLaunchKernelGGL(k_inp_plan, dim3(1), dim3(1024), 0, s, ws);
+    hipLaunchKernelGGL(k_inp_slot_reset, dim3(256), dim3(256), 0, s, ws);
+    hipLaunchKernelGGL(k_inp_strips<true>, dim3(kStripBlocks), dim3(kStripThreads), 0, s, views, nviews, ws);
+    hipLaunchKernelGGL(k_inp_strips<false>, dim3(kStripBlocks), dim3(kStripThreads), 0, s, views, nviews, ws);
     hipLaunchKernelGGL(k_inp_big, dim3(kInpaintBigBlocks), dim3(kBigThreads), 0, s, views, nviews, ws);
     hipLaunchKernelGGL(k_inp_solve, dim3(kInpaintBlocks), b, 0, s, views, nviews, ws);
     hipLaunchKernelGGL(k_inp_apply, g, b, 0, s, views, ws, map_out);

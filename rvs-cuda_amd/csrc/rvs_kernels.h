@@ -49,6 +49,10 @@ struct ViewDesc {
     int ew, eh;              // size of the encoded planes: the view's own size (w, h are the raster's,
                              // oversampled by Precision)
     float* ecol;             // colour the planes encode: color, or its resize to ew x eh
+    int enc_raster;          // planes wanted, not oversampled: tiles without holes or deferred
+                             // pixels are encoded by k_raster itself
+    int partial_color;       // color is internal: tiles encoded by k_raster write it only on
+                             // their border (all an inpainting next door can read)
 };
 
 // Device-side state of one launch set.
@@ -67,7 +71,9 @@ struct RasterWS {
     uint8_t* deferred;       // [global pixel] pixel handled by the in-order replay
     uint32_t* deferred_tiles;// [global tile]
     uint32_t* work;          // binning work items (view, source, block)
-    uint32_t* status;        // [0] n_deferred, [1] list overflow, [2] entries (saturated), [4] work items
+    uint8_t* tile_enc;       // [global tile] 1: encoded by k_raster (written by every tile)
+    uint32_t* status;        // [0] n_deferred, [1] list overflow, [2] entries (saturated), [3] inpainting
+                             // wait gave up, [4] work items
 };
 
 struct BatchDims {

@@ -9,8 +9,9 @@
 
 namespace rvs_k {
 
-hipError_t launch_prep_color(const void* y, const void* u, const void* v, int u8, int w, int h, float4* out,
-                             hipStream_t s);
+// packed: 16-bit Y, 4 U, 4 V per pixel (even sizes, bit depth <= 14), else float4.
+hipError_t launch_prep_color(const void* y, const void* u, const void* v, int u8, int w, int h, int packed,
+                             void* out, hipStream_t s);
 hipError_t launch_decode_color(const void* y, const void* u, const void* v, int u8, int w, int h, float cs,
                                float* out, hipStream_t s);
 hipError_t launch_decode_depth(const void* raw, int u8, size_t n, float scale, float znear, float zfar,
@@ -22,8 +23,10 @@ hipError_t launch_blend_step(float* acc_c, float* acc_q, float* acc_d, float* ac
 hipError_t launch_encode(const float* color, int w, int h, int bitdepth, void* y, void* u, void* v,
                          hipStream_t s);
 // Encodes every view of a batch that has output planes (ViewDesc y, u, v) from
-// ecol at the views' own sizes (max_w x max_h: the largest ew x eh).
-hipError_t launch_encode_views(const ViewDesc* views, int nviews, int max_w, int max_h, hipStream_t s);
+// ecol at the views' own sizes (max_w x max_h: the largest ew x eh); blocks of
+// tiles that tile_enc (may be null) marks as encoded by k_raster are skipped.
+hipError_t launch_encode_views(const ViewDesc* views, int nviews, int max_w, int max_h, const uint8_t* tile_enc,
+                               hipStream_t s);
 // write_color's resize (image_writing.cpp:189-190) of one float3 image.
 hipError_t launch_resize_color(const float* src, int sw, int sh, float* dst, int dw, int dh, hipStream_t s);
 // Multispectral blending's frequency split of one float3 view (rgb: blur all

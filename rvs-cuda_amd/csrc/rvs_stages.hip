@@ -15,21 +15,29 @@ __global__ void k_decode_color(const void* y, const void* u, const void* v, int 
     if (x >= w || r >= h) return;
     PairGeom g;
     g.sw = w; g.sh = h; g.py = y; g.pu = u; g.pv = v; g.color_u8 = u8; g.cscale = cs; g.col = nullptr;
+    g.colq = nullptr;
     const float3 c = color(g, r, x);
     const size_t o = (size_t)r * w + x;
     out[3 * o] = c.x; out[3 * o + 1] = c.y; out[3 * o + 2] = c.z;
 }
 
-// Resident decoded colour plane of a source (color_raw per pixel).
-__global__ void k_prep_color(const void* y, const void* u, const void* v, int u8, int w, int h,
-                             float4* __restrict__ out) {
+// Resident decoded colour plane of a source (color_raw per pixel): floats, or
+// (packed) Y, 4 U, 4 V as 16-bit integers -- exact for even sizes and codes
+// below 2^14, whose upsampled chroma is a multiple of 1/4.
+__global__ void k_prep_color(const void* y, const void* u, const void* v, int u8, int w, int h, int packed,
+                             void* __restrict__ out) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int r = blockIdx.y;
     if (x >= w || r >= h) return;
     PairGeom g;
-    g.sw = w; g.sh = h; g.py = y; g.pu = u; g.pv = v; g.color_u8 = u8; g.col = nullptr;
+    g.sw = w; g.sh = h; g.py = y; g.pu = u; g.pv = v; g.color_u8 = u8; g.col = nullptr; g.colq = nullptr;
     const float3 c = color_raw(g, r, x);
-    out[(size_t)r * w + x] = make_float4(c.x, c.y, c.z, 0.f);
+    const size_t o = (size_t)r * w + x;
+    if (packed)
+        ((ushort4*)out)[o] = make_ushort4((unsigned short)c.x, (unsigned short)(c.y * 4.f),
+                                          (unsigned short)(c.z * 4.f), 0);
+    else
+        ((float4*)out)[o] = make_float4(c.x, c.y, c.z, 0.f);
 }
 
 // read_depth_YUV -> formatDepth (image_loading.cpp:147-213).
@@ -103,11 +111,15 @@ __global__ void k_encode(const float* __restrict__ color, int w, int h, unsigned
 
 // Batch of views (grid.z = view); views without output planes are skipped.
 // The planes encode ecol at the view's size (ew x eh).
-__global__ void k_encode_views(const ViewDesc* __restrict__ views) {
+__global__ void k_encode_views(const ViewDesc* __restrict__ views, const uint8_t* __restrict__ tile_enc) {
     const ViewDesc& vd = views[blockIdx.z];
     const int bx = blockIdx.x * blockDim.x + threadIdx.x;
     const int by = blockIdx.y;
     if (!vd.y || bx >= vd.ew / 2 || by >= vd.eh / 2) return;
+    // tiles without holes or deferred pixels were encoded by the raster pass
+    if (tile_enc && vd.enc_raster &&
+        tile_enc[vd.tile_base + (uint32_t)((2 * by) / kTile * vd.tiles_x + (2 * bx) / kTile)])
+        return;
     const unsigned maxv = (1u << vd.bitdepth) - 1u;
     if (vd.bitdepth <= 8)
         encode_block(vd.ecol, vd.ew, bx, by, maxv, (uint8_t*)vd.y, (uint8_t*)vd.u, (uint8_t*)vd.v);
@@ -151,9 +163,9 @@ hipError_t launch_decode_color(const void* y, const void* u, const void* v, int 
     return hipGetLastError();
 }
 
-hipError_t launch_prep_color(const void* y, const void* u, const void* v, int u8, int w, int h, float4* out,
-                             hipStream_t s) {
-    hipLaunchKernelGGL(k_prep_color, dim3((w + 255) / 256, h), dim3(256), 0, s, y, u, v, u8, w, h, out);
+hipError_t launch_prep_color(const void* y, const void* u, const void* v, int u8, int w, int h, int packed,
+                             void* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_prep_color, dim3((w + 255) / 256, h), dim3(256), 0, s, y, u, v, u8, w, h, packed, out);
     return hipGetLastError();
 }
 
@@ -189,9 +201,10 @@ hipError_t launch_encode(const float* color, int w, int h, int bitdepth, void* y
     return hipGetLastError();
 }
 
-hipError_t launch_encode_views(const ViewDesc* views, int nviews, int max_w, int max_h, hipStream_t s) {
+hipError_t launch_encode_views(const ViewDesc* views, int nviews, int max_w, int max_h, const uint8_t* tile_enc,
+                               hipStream_t s) {
     const dim3 grid((max_w / 2 + 255) / 256, max_h / 2, nviews);
-    hipLaunchKernelGGL(k_encode_views, grid, dim3(256), 0, s, views);
+    hipLaunchKernelGGL(k_encode_views, grid, dim3(256), 0, s, views, tile_enc);
     return hipGetLastError();
 }
 

@@ -1026,16 +1026,50 @@ __device__ __forceinline__ void raster_quad(const G& g, uint32_t ij, uint32_t e,
 }
 
 template <int MODE>
-__device__ inline void write_pixel(const ViewDesc& vd, size_t o, const BlendState& st, float3 rc, float rd, float rs) {
+__device__ inline void write_pixel(const ViewDesc& vd, size_t o, const BlendState& st, float3 rc, float rd, float rs,
+                                   bool color = true) {
     if (MODE == MODE_RAW) {
         vd.color[3 * o] = rc.x; vd.color[3 * o + 1] = rc.y; vd.color[3 * o + 2] = rc.z;
         vd.depth[o] = rd;
         vd.validity[o] = rs;
     } else {
-        vd.color[3 * o] = st.c.x; vd.color[3 * o + 1] = st.c.y; vd.color[3 * o + 2] = st.c.z;
+        if (color) { vd.color[3 * o] = st.c.x; vd.color[3 * o + 1] = st.c.y; vd.color[3 * o + 2] = st.c.z; }
         if (vd.depth) vd.depth[o] = st.d;
         vd.quality[o] = st.q;
         if (vd.validity) vd.validity[o] = st.v;
+    }
+}
+
+// write_color (image_writing.cpp:142-203) of one pixel of a tile, from the
+// blended colour in registers: Y here, U and V by the top-left pixel of each
+// 2x2 block from its wave neighbours (a 16-pixel tile row is a quarter wave,
+// so the block's four pixels are lanes l, l+1, l+16, l+17 of one wave).  The
+// same quantisation and chroma rule as k_encode_views; tiles start at even
+// coordinates and encoded views have even sizes, so a block is inside or
+// outside as a whole.
+__device__ inline void encode_tile_pixel(const ViewDesc& vd, float3 c, int px, int py, bool inside, int lx, int ly) {
+    const unsigned maxv = (1u << vd.bitdepth) - 1u;
+    const bool u8 = vd.bitdepth <= 8;
+    uint32_t qy = quantize_u32(c.x, maxv), qu = quantize_u32(c.y, maxv), qv = quantize_u32(c.z, maxv);
+    // the plane's element type truncates, as (T) does in k_encode_views
+    if (u8) { qy &= 0xffu; qu &= 0xffu; qv &= 0xffu; } else { qy &= 0xffffu; qu &= 0xffffu; qv &= 0xffffu; }
+    const uint32_t su = qu + (uint32_t)__shfl_down((int)qu, 1) + (uint32_t)__shfl_down((int)qu, 16) +
+                        (uint32_t)__shfl_down((int)qu, 17);
+    const uint32_t sv = qv + (uint32_t)__shfl_down((int)qv, 1) + (uint32_t)__shfl_down((int)qv, 16) +
+                        (uint32_t)__shfl_down((int)qv, 17);
+    if (!inside) return;
+    const size_t o = (size_t)py * vd.ew + px;
+    const bool corner = ((lx | ly) & 1) == 0;
+    const size_t oc = (size_t)(py >> 1) * (vd.ew >> 1) + (px >> 1);
+    if (u8) {
+        ((uint8_t*)vd.y)[o] = (uint8_t)qy;
+        if (corner) { ((uint8_t*)vd.u)[oc] = (uint8_t)((su + 2) >> 2); ((uint8_t*)vd.v)[oc] = (uint8_t)((sv + 2) >> 2); }
+    } else {
+        ((uint16_t*)vd.y)[o] = (uint16_t)qy;
+        if (corner) {
+            ((uint16_t*)vd.u)[oc] = (uint16_t)((su + 2) >> 2);
+            ((uint16_t*)vd.v)[oc] = (uint16_t)((sv + 2) >> 2);
+        }
     }
 }
 
@@ -1049,6 +1083,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI 
     __shared__ uint32_t key2[kRasterGroup][kTile * kTile];
     __shared__ uint32_t pref[kRasterGroup + 1];
     __shared__ int s_deferred;   // any pixel of the tile deferred
+    __shared__ int s_hole;       // any pixel of the tile a hole (quality not > 0)
     // one phase: a smaller cache keeps the block within 20 KB of LDS (8 blocks per CU)
     __shared__ RasterCache<MULTI ? kRasterCache : kRasterCacheOne> cache;
     const uint32_t T = tile_of_block(blockIdx.x, d.ntiles);
@@ -1073,7 +1108,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI 
             key2[k][tid] = 0u;
         }
         if (tid == 0) {
-            if (s0 == 0) s_deferred = 0;
+            if (s0 == 0) { s_deferred = 0; s_hole = 0; }
             uint32_t acc = 0;
             for (int k = 0; k < gs; ++k) {
                 pref[k] = acc;
@@ -1154,12 +1189,25 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI 
         __syncthreads();
     }
     const int t3 = wave_base + lane_now();
-    const int px = tx0 + (t3 % kTile), py = ty0 + (t3 / kTile);
+    const int lx = t3 % kTile, ly = t3 / kTile;
+    const int px = tx0 + lx, py = ty0 + ly;
     const bool inside = px < vd.w && py < vd.h;
     const size_t o = (size_t)py * vd.w + px;
-    if (inside && !deferred) write_pixel<MODE>(vd, o, st, rc, rd, rs);
+    // a tile with a deferred pixel or a hole (quality not > 0: the inpainting
+    // mask) is finished by the replay / inpainting and encoded after them;
+    // any other tile of a view that wants planes is encoded here
     if (deferred) s_deferred = 1;
+    if (MODE == MODE_BLEND && inside && !deferred && !(st.q > 0.f)) s_hole = 1;
     __syncthreads();
+    const bool enc = MODE == MODE_BLEND && vd.enc_raster && !s_deferred && !s_hole;
+    if (inside && !deferred) {
+        // an internal colour map of a tile encoded here is read only on the
+        // tile's border (by the inpainting of a neighbouring tile)
+        const bool border = lx == 0 || ly == 0 || lx == kTile - 1 || ly == kTile - 1;
+        write_pixel<MODE>(vd, o, st, rc, rd, rs, !(enc && vd.partial_color && !border));
+    }
+    if (enc) encode_tile_pixel(vd, st.c, px, py, inside, lx, ly);
+    if (t3 == 0) ws.tile_enc[T] = enc ? 1 : 0;
     // the replay reads the per-pixel flags of the listed tiles only
     if (s_deferred) {
         if (inside) ws.deferred[vd.px_base + o] = deferred ? 1 : 0;

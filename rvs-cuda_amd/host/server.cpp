@@ -22,13 +22,6 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 size_t plane_bytes(const rvs_target& t) { return t.bitdepth <= 8 ? 1 : 2; }
 
-// Colour map of a request: float3 at the raster size (the view's size scaled by
-// its Precision, SynthesizedView.cpp:326-329).
-size_t color_bytes(const rvs_target& t) {
-    const size_t w = (size_t)(int)(0.5f + t.cam.w * t.rescale), h = (size_t)(int)(0.5f + t.cam.h * t.rescale);
-    return align256(w * h * 3 * sizeof(float));
-}
-
 size_t yuv_bytes(const rvs_target& t) { return (size_t)t.cam.w * t.cam.h * 3 / 2 * plane_bytes(t); }
 
 // Entry-list capacity of a batch of n views (0 = the library's default).
@@ -263,21 +256,19 @@ int Server::run_closed(const rvs_request* reqs, int n, int concurrency) {
     return m_error;
 }
 
-// Device bytes of a batch: its YUV planes, colour maps and workspace; 0 with
-// rc set when the plan itself fails.
+// Device bytes of a batch: its YUV planes and workspace (requests return only
+// the planes, so their colour maps live in the workspace); 0 with rc set when
+// the plan itself fails.
 size_t Server::batch_bytes(const std::vector<rvs_source>& S, const std::vector<rvs_target>& tgts, uint32_t cap,
                            size_t* ws_out, int* rc_out) const {
     size_t ws = 0;
     const int rc = rvs_synth_plan(S.data(), (int)S.size(), tgts.data(), (int)tgts.size(), cap, &ws);
     if (rc_out) *rc_out = rc;
     if (rc != RVS_OK) return 0;
-    size_t h = 0, c = 0;
-    for (const rvs_target& t : tgts) {
-        h += yuv_bytes(t);
-        c += color_bytes(t);
-    }
+    size_t h = 0;
+    for (const rvs_target& t : tgts) h += yuv_bytes(t);
     if (ws_out) *ws_out = ws;
-    return align256(h) + c + ws;
+    return align256(h) + ws;
 }
 
 // Forms and enqueues the next batch on a free slot.  Admission: the oldest
@@ -374,12 +365,8 @@ bool Server::launch(Slot& s) {
         h += yuv_bytes(t);
     }
     s.yuv_bytes = h;
-    size_t o = align256(h);
-    for (size_t k = 0; k < batch.size(); ++k) {
-        s.outs[k].color = (float*)(s.arena + o);
-        o += color_bytes(tgts[k]);
-    }
-    s.ws_off = o;
+    for (size_t k = 0; k < batch.size(); ++k) s.outs[k].color = nullptr;   // planes only
+    s.ws_off = align256(h);
     const double t_launch = now_ms();
     s.cap = batch_cap(m_per_view, tgts.size());
     hipStream_t cs = (hipStream_t)m_compute[s.index % m_compute.size()];

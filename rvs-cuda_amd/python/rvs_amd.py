@@ -422,7 +422,8 @@ class Synthesizer:
     """One-view-at-a-time synthesis (the plain per-view path): owns a workspace
     and output buffers for one target size."""
 
-    def __init__(self, sources, target, entry_cap=0, device="cuda", want_maps=True):
+    def __init__(self, sources, target, entry_cap=0, device="cuda", want_maps=True, want_color=True):
+        """want_color False: only the encoded planes (see BatchSynthesizer)."""
         import torch
         self.sources = sources
         self.target = target
@@ -433,7 +434,7 @@ class Synthesizer:
         rw, rh = raster_size(target)   # float maps at the raster size
         dt = torch.uint8 if target.bitdepth <= 8 else torch.int16
         self.out = {
-            "color": torch.empty((rh, rw, 3), dtype=torch.float32, device=device),
+            "color": torch.empty((rh, rw, 3), dtype=torch.float32, device=device) if want_color else None,
             "quality": torch.empty((rh, rw), dtype=torch.float32, device=device) if want_maps else None,
             "depth": torch.empty((rh, rw), dtype=torch.float32, device=device) if want_maps else None,
             "validity": torch.empty((rh, rw), dtype=torch.float32, device=device) if want_maps else None,
@@ -504,12 +505,15 @@ class BatchSynthesizer:
     """Many target views from the same resident sources in one launch set
     (rvs_synthesize_batch).  targets: list of RvsTarget (sizes may differ)."""
 
-    def __init__(self, sources, targets, entry_cap=0, device="cuda", want_maps=False):
+    def __init__(self, sources, targets, entry_cap=0, device="cuda", want_maps=False, want_color=True):
+        """want_color False: only the encoded planes (the colour map stays in
+        the workspace, written only where the inpainting needs it)."""
         import torch
         self.sources = sources
         self.arr = (RvsSource * len(sources))(*[s.struct() for s in sources])
         self.entry_cap = entry_cap
         self.device = device
+        self.want_color = want_color
         self.set_targets(targets, want_maps)
 
     def set_targets(self, targets, want_maps=False):
@@ -522,7 +526,8 @@ class BatchSynthesizer:
             w, h = t.cam.w, t.cam.h
             rw, rh = raster_size(t)
             dt = torch.uint8 if t.bitdepth <= 8 else torch.int16
-            o = {"color": torch.empty((rh, rw, 3), dtype=torch.float32, device=self.device),
+            o = {"color": torch.empty((rh, rw, 3), dtype=torch.float32, device=self.device) if self.want_color
+                 else None,
                  "y": torch.empty((h, w), dtype=dt, device=self.device),
                  "u": torch.empty((h // 2, w // 2), dtype=dt, device=self.device),
                  "v": torch.empty((h // 2, w // 2), dtype=dt, device=self.device)}

@@ -88,6 +88,35 @@ def test_batch_rects_many_views(torch_cuda):
         _check_oracle(outs[k], _oracle(rg, planes, tgts[k]), tgts[k].bitdepth, "rects view %d" % k)
 
 
+@pytest.mark.parametrize("bd", [8, 10])
+def test_planes_only_batch_equals_full(torch_cuda, bd):
+    """Views that want only the encoded planes: tiles without holes or
+    deferred pixels are encoded by the raster pass itself and the colour map
+    (in the workspace) is written only on their borders; the planes equal those
+    of the same batch with full colour maps and the oracle's, at 8 and 10 bits,
+    with holes (the rects scene's uncovered border) and deferred pixels."""
+    torch = torch_cuda
+    say("case planes-only batch, %d bit" % bd)
+    rg = W.rig("rects_640", scale=2)
+    planes = W.render_sources(rg)
+    srcs = W.device_sources(rg, planes)
+    tgts = []
+    for pos, rot in W.random_walk(6, seed=12, pos_lim=0.05, yaw_lim=8.0, pitch_lim=4.0):
+        t = W.device_target(rg, pos, rot)
+        t.bitdepth = bd
+        tgts.append(t)
+    full = R.BatchSynthesizer(srcs, tgts).run()
+    full = [{k: o[k].clone() for k in ("y", "u", "v")} for o in full]
+    part = R.BatchSynthesizer(srcs, tgts, want_color=False).run()
+    torch.cuda.synchronize()
+    for k in range(len(tgts)):
+        assert part[k]["color"] is None
+        for key in ("y", "u", "v"):
+            assert torch.equal(part[k][key], full[k][key]), "view %d plane %s" % (k, key)
+    for k in (0, 3):
+        _check_oracle(part[k], _oracle(rg, planes, tgts[k]), bd, "planes-only view %d" % k)
+
+
 @pytest.mark.parametrize("rot", [[90.0, 0.0, 0.0], [180.0, 0.0, 0.0], [-60.0, 40.0, 0.0], [30.0, -80.0, 10.0]])
 def test_culled_poses_match_oracle_erp(torch_cuda, rot):
     """Targets that see little of each ERP source: binning work items are culled."""

@@ -311,8 +311,9 @@ def test_inpaint_patterns(torch_cuda):
 
 
 def test_inpaint_large_components(torch_cuda):
-    """Components far beyond the LDS box: lines longer than one block step
-    (2048 pixels), along rows (wide box) and columns (tall box), and a wedge."""
+    """Components far beyond the LDS box, solved in strips of 256 pixels per
+    line on separate blocks: lines of 2586 pixels along rows (wide box) and
+    columns (tall box), a wedge and a spiral corridor."""
     torch = torch_cuda
     rng = np.random.default_rng(8)
     for tag, (h, w) in (("wide", (70, 2600)), ("tall", (2600, 70))):
@@ -338,6 +339,21 @@ def test_inpaint_large_components(torch_cuda):
             q[y0 + 2:y1 + 1, x0 + 2] = 1.0
         y0, x0, y1, x1 = y0 + 6, x0 + 6, y1 - 6, x1 - 6
     _inpaint_case(torch, q, "spiral")
+
+
+def test_inpaint_strip_capacity_overflow(torch_cuda):
+    """190 nested L-shaped components whose boxes all span most of a 600 x 4000
+    view: their strip carries exceed the workspace's share (npx / 16 slots),
+    so the plan places a prefix of them on the strip solver and leaves the rest
+    to the one-block solver; every map entry still equals the oracle's."""
+    torch = torch_cuda
+    h, w = 600, 4000
+    q = np.ones((h, w), np.float32)
+    for k in range(190):
+        a = 3 * k + 1
+        q[a, a:w - 2] = 0.0     # the L's row
+        q[a:h - 2, a] = 0.0     # and its column
+    _inpaint_case(torch, q, "nested L overflow")
 
 
 def test_inpaint_component_beyond_line_length(torch_cuda):
