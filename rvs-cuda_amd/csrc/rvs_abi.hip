@@ -132,8 +132,7 @@ struct Layout {
 // deferred-pixel flags, deferred-tile list and binning work list.
 struct RasterPlan {
     rvs_k::BatchDims d;
-    size_t status, counts, single, multi, offsets, fill, lists, ranges, vbuf, tvq, imap, bsum, deferred, dtiles, work,
-        tile_enc;
+    size_t status, counts, offsets, fill, lists, ranges, vbuf, tvq, imap, bsum, deferred, dtiles, work, tile_enc;
 };
 
 // List capacity: the caller's, or one entry per output pixel and source
@@ -153,7 +152,7 @@ void plan_raster(Layout& L, RasterPlan& R, int nsrc, int nviews, uint32_t ntiles
     d.bins_x = (uint32_t)((max_sw - 1 + rvs_k::kBinQX - 1) / rvs_k::kBinQX);
     d.bins_y = (uint32_t)((max_sh - 1 + rvs_k::kBinQY - 1) / rvs_k::kBinQY);
     const uint64_t items = (uint64_t)d.bins_x * d.bins_y * nsrc * nviews;
-    // Work items that keep their tile ranges, vertex window and quad validities (7.9 KB each):
+    // Work items that keep their tile ranges, vertex window and quad validities (6.9 KB each):
     // at least an eighth of the items before culling, and enough for the entry
     // capacity (a surviving item bins ~300 entries in the measured scenes, so
     // capacity / 240 items cover a launch set that fills it); the rest
@@ -172,13 +171,11 @@ void plan_raster(Layout& L, RasterPlan& R, int nsrc, int nviews, uint32_t ntiles
     const size_t nbins = (size_t)nsrc * ntiles;
     R.status = L.take(16 * sizeof(uint32_t));
     R.counts = L.take(nbins * sizeof(uint32_t));
-    R.single = L.take(nbins * sizeof(uint32_t));
-    R.multi = L.take(nbins * sizeof(uint32_t));
     R.offsets = L.take(nbins * sizeof(uint32_t));
     R.fill = L.take(nbins * sizeof(uint32_t));
     R.bsum = L.take(rvs_k::scan_blocks(nbins) * sizeof(unsigned long long));
     R.lists = L.take((size_t)d.cap * sizeof(uint32_t));
-    R.ranges = L.take((size_t)d.range_cap * 256 * sizeof(uint2));
+    R.ranges = L.take((size_t)d.range_cap * 256 * sizeof(uint32_t));
     R.vbuf = L.take((size_t)d.range_cap * 3 * rvs_k::kBinVerts * sizeof(float));
     R.tvq = L.take((size_t)d.range_cap * 256 * sizeof(float2));
     R.imap = L.take((size_t)items * sizeof(uint32_t));
@@ -192,13 +189,11 @@ rvs_k::RasterWS carve_raster(char* base, const RasterPlan& R) {
     rvs_k::RasterWS ws;
     ws.status = (uint32_t*)(base + R.status);
     ws.counts = (uint32_t*)(base + R.counts);
-    ws.single = (uint32_t*)(base + R.single);
-    ws.multi = (uint32_t*)(base + R.multi);
     ws.offsets = (uint32_t*)(base + R.offsets);
     ws.fill = (uint32_t*)(base + R.fill);
     ws.bsum = (unsigned long long*)(base + R.bsum);
     ws.lists = (uint32_t*)(base + R.lists);
-    ws.ranges = (uint2*)(base + R.ranges);
+    ws.ranges = (uint32_t*)(base + R.ranges);
     ws.vbuf = (float*)(base + R.vbuf);
     ws.tvq = (float2*)(base + R.tvq);
     ws.imap = (uint32_t*)(base + R.imap);
@@ -211,8 +206,7 @@ rvs_k::RasterWS carve_raster(char* base, const RasterPlan& R) {
 
 hipError_t reset_raster(const rvs_k::RasterWS& ws, const rvs_k::BatchDims& d, hipStream_t s) {
     hipError_t e = rvs_k::launch_reset_u32(ws.status, 16, s);
-    if (e == hipSuccess) e = rvs_k::launch_reset_u32(ws.single, (size_t)d.nsrc * d.ntiles, s);
-    if (e == hipSuccess) e = rvs_k::launch_reset_u32(ws.multi, (size_t)d.nsrc * d.ntiles, s);
+    if (e == hipSuccess) e = rvs_k::launch_reset_u32(ws.counts, (size_t)d.nsrc * d.ntiles, s);
     if (e == hipSuccess) e = rvs_k::launch_reset_u32(ws.fill, (size_t)d.nsrc * d.ntiles, s);
     return e;
 }
@@ -504,7 +498,7 @@ uint64_t noalloc_keep(const rvs_source* srcs, int n_src, const rvs_target* tgt, 
     if (plan_synth(srcs, n_src, tgt, 1, cap, P, nullptr, 1) != RVS_OK) return 0;
     need = P.total;
     if (P.total > kNoallocScratchBytes) return 0;
-    const size_t per_item = 256 * sizeof(uint2) + 3 * (size_t)rvs_k::kBinVerts * sizeof(float) + 256 * sizeof(float2);
+    const size_t per_item = 256 * sizeof(uint32_t) + 3 * (size_t)rvs_k::kBinVerts * sizeof(float) + 256 * sizeof(float2);
     const uint64_t keep = 1 + (kNoallocScratchBytes - P.total) / per_item;
     // alignment of the three kept arrays can cost up to 3 x 255 bytes: step back until it fits
     for (uint64_t k = keep; k >= 1; k = k > 64 ? k - 64 : k - 1) {

@@ -57,13 +57,11 @@ struct ViewDesc {
 
 // Device-side state of one launch set.
 struct RasterWS {
-    uint32_t* counts;        // [global tile][source] entries per bin (written by the scan)
-    uint32_t* single;        // [global tile][source] single-tile quads of kept ranges (counting pass)
-    uint32_t* multi;         // [global tile][source] all other entries (counting pass)
+    uint32_t* counts;        // [global tile][source] entries per bin (counting pass)
     uint32_t* offsets;       // [global tile][source] first entry of each bin in lists
-    uint32_t* fill;          // [global tile][source] fill cursors of the multi-tile region
+    uint32_t* fill;          // [global tile][source] fill cursors
     uint32_t* lists;         // [cap] quad codes (row << 16 | column), bins back to back
-    uint2* ranges;           // [range_cap work items][256] packed tile range + single-tile slot
+    uint32_t* ranges;        // [range_cap work items][256] packed tile range of each quad
     float* vbuf;             // [range_cap work items][kBinQY + 1][kBinQX + 1][3] vertices (u', v', depth)
     float2* tvq;             // [range_cap work items][256] validity of triangles T1, T2 of each quad
     uint32_t* imap;          // [view][source][item] work index of a kept item, ~0 otherwise

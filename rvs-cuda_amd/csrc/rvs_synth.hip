@@ -345,19 +345,21 @@ __global__ void __launch_bounds__(kCullThreads) k_bin_cull(const G* __restrict__
 // Bins are compact: a counting pass (k_bin<COUNT>) finds every surviving
 // quad's tile range and counts the (quad, tile) entries per (tile, source)
 // bin; an exclusive scan turns the counts into bin offsets; fill passes write
-// the quad codes.  For the first range_cap work items the counting pass keeps
-// each quad's tile range and, for a quad touching a single tile (most of
-// them), its slot in that bin, so their fill pass (k_fill_kept) writes it
-// without atomics; quads spanning several tiles take slots after the
-// single-tile ones through fill cursors.  Later items re-evaluate their
-// vertices in their fill pass (k_bin<FILL>).  When the entries exceed the list
-// capacity the fill passes and the raster are skipped and the needed capacity
-// is reported.
+// the quad codes, taking slots through one fill cursor per bin.  For the first
+// range_cap work items the counting pass keeps each quad's tile range (and the
+// item's vertex window and triangle validities, for the raster), so their fill
+// pass (k_fill_kept) needs no vertex work; later items re-evaluate their
+// vertices in their fill pass (k_bin<FILL>).  The order of a bin's entries is
+// free: the raster's keys commute and the in-order replay sorts them.  When
+// the entries exceed the list capacity the fill passes and the raster are
+// skipped and the needed capacity is reported.
 //
-// k_bin blocks first gather their entries per bin in an LDS table (a few to a
-// few dozen distinct tiles per 64x4 quad block), then take one global atomic
-// per (bin, kind): device-scope atomics are performed at the memory side, so
-// their number and the chains of dependent ones set this kernel's time.
+// Blocks first gather their entries per bin in an LDS table (a few to a few
+// dozen distinct tiles per 64x4 quad block), then take one global atomic per
+// bin: device-scope atomics are performed at the memory side, so their number
+// and the chains of dependent ones set these kernels' time.  The counting
+// pass needs no atomic's return value (its atomics do not hold the block);
+// the fill passes take one returning atomic per bin.
 
 constexpr uint32_t kRangeNone = 0xffffffffu;   // no surviving triangle
 constexpr uint32_t kRangeBig = 0xfffffffeu;    // range too wide to pack: re-evaluated
@@ -502,8 +504,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
                                              BatchDims d, RasterWS ws) {
     __shared__ float3 V[kBinQY + 1][kBinQX + 1];
     __shared__ uint32_t t_key[kBinTable];
-    __shared__ uint32_t t_n[2][kBinTable];      // entries per bin: [0] multi / fill, [1] single-tile
-    __shared__ uint32_t t_base[2][kBinTable];   // [0] fill write cursor, [1] first single-tile slot
+    __shared__ uint32_t t_n[kBinTable];      // entries per bin
+    __shared__ uint32_t t_base[kBinTable];   // fill write cursor per bin (fill pass)
     __shared__ RangeWalk walk;
     const int tid = threadIdx.x;
     const uint32_t per = d.bins_x * d.bins_y;
@@ -527,7 +529,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
         const bool stored = !FILL && wi < d.range_cap;   // same for the whole block
         if (tid < kBinTable) {
             t_key[tid] = kTableEmpty;
-            t_n[0][tid] = 0; t_n[1][tid] = 0;
+            t_n[tid] = 0;
         }
         bool more = false;
         int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
@@ -563,30 +565,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
         }
         auto tile_of = [&](int tx, int ty) { return vd.tile_base + (uint32_t)(ty * vd.tiles_x + tx); };
         auto bin_of = [&](uint32_t t) { return (size_t)t * d.nsrc + s; };
-        // A stored single-tile quad takes a slot of its bin's single-tile
-        // region in the counting pass; every other entry is a multi entry.
-        const bool single = !FILL && stored && more && tx0 == tx1 && ty0 == ty1;
-        int my_slot = -1;
-        uint32_t rank = 0;
-        if (single) {
-            const uint32_t t = tile_of(tx0, ty0);
-            my_slot = table_slot(t_key, t);
-            if (my_slot >= 0) rank = atomicAdd(&t_n[1][my_slot], 1u);
-            else rank = atomicAdd(&ws.single[bin_of(t)], 1u);   // table full: final slot now
-        }
-        // multi entries: narrow ranges by their own lane, wide ones dealt to the block
+        // the range of a kept quad, for its fill pass
+        if (!FILL && stored)
+            ws.ranges[(size_t)wi * 256 + tid] = more ? pack_range(tx0, tx1, ty0, ty1) : kRangeNone;
+        // entries: narrow ranges by their own lane, wide ones dealt to the block
         auto quad_of = [&](int lane) {
             return ((uint32_t)(i0 + lane / kBinQX) << 16) | (uint32_t)(j0 + lane % kBinQX);
         };
-        const bool multi = more && !single;
-        const bool wide = multi && (uint32_t)(tx1 - tx0 + 1) * (uint32_t)(ty1 - ty0 + 1) > kLaneTiles;
+        const bool wide = more && (uint32_t)(tx1 - tx0 + 1) * (uint32_t)(ty1 - ty0 + 1) > kLaneTiles;
         const bool any_wide = __syncthreads_or(wide);
         if (any_wide) {
             range_publish(walk, tid, wide, tx0, tx1, ty0, ty1);
             __syncthreads();   // ranges scanned
         }
         auto each_tile = [&](auto fn) {
-            if (multi && !wide)
+            if (more && !wide)
                 for (int ty = ty0; ty <= ty1; ++ty)
                     for (int tx = tx0; tx <= tx1; ++tx) fn(tid, tx, ty);
             if (any_wide) range_walk(walk, tid, fn);
@@ -595,45 +588,34 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
             const uint32_t t = tile_of(tx, ty);
             const int k = table_slot(t_key, t);
             if (k >= 0) {
-                atomicAdd(&t_n[0][k], 1u);
-            } else if (FILL) {
+                atomicAdd(&t_n[k], 1u);
+            } else if (FILL) {   // table full: a slot of its own
                 const size_t bin = bin_of(t);
-                ws.lists[(size_t)ws.offsets[bin] + ws.single[bin] + atomicAdd(&ws.fill[bin], 1u)] = quad_of(lane);
+                ws.lists[(size_t)ws.offsets[bin] + atomicAdd(&ws.fill[bin], 1u)] = quad_of(lane);
             } else {
-                atomicAdd(&ws.multi[bin_of(t)], 1u);
+                atomicAdd(&ws.counts[bin_of(t)], 1u);
             }
         });
         __syncthreads();   // table complete
-        if (tid < kBinTable && t_key[tid] != kTableEmpty) {
+        if (tid < kBinTable && t_n[tid]) {
             const size_t bin = bin_of(t_key[tid]);
-            if (FILL) {
-                const uint32_t n = t_n[0][tid];
-                if (n) t_base[0][tid] = ws.offsets[bin] + ws.single[bin] + atomicAdd(&ws.fill[bin], n);
-            } else {
-                if (t_n[1][tid]) t_base[1][tid] = atomicAdd(&ws.single[bin], t_n[1][tid]);
-                if (t_n[0][tid]) atomicAdd(&ws.multi[bin], t_n[0][tid]);
-            }
+            if (FILL) t_base[tid] = ws.offsets[bin] + atomicAdd(&ws.fill[bin], t_n[tid]);
+            else atomicAdd(&ws.counts[bin], t_n[tid]);
         }
-        __syncthreads();   // bases known
-        if (FILL)
+        if (FILL) {
+            __syncthreads();   // bases known
             each_tile([&](int lane, int tx, int ty) {
                 const int k = find_slot(t_key, tile_of(tx, ty));
-                if (k >= 0) ws.lists[atomicAdd(&t_base[0][k], 1u)] = quad_of(lane);
+                if (k >= 0) ws.lists[atomicAdd(&t_base[k], 1u)] = quad_of(lane);
             });
-        if (!FILL && stored) {
-            uint2 r;
-            r.x = (tx1 >= tx0) ? pack_range(tx0, tx1, ty0, ty1) : kRangeNone;
-            r.y = (my_slot >= 0) ? t_base[1][my_slot] + rank : rank;
-            ws.ranges[(size_t)wi * 256 + tid] = r;
         }
         __syncthreads();   // LDS free for the next item
     }
 }
 
 // Fill pass over the kept work items, a block per item (striding when there
-// are more items than blocks) and a thread per quad:
-// single-tile quads write their counted slot, the others take slots after the
-// bin's single-tile region through an LDS bin table and the fill cursors.  A quad whose
+// are more items than blocks) and a thread per quad: entries take slots
+// through an LDS bin table and one fill-cursor atomic per bin.  A quad whose
 // range was too wide to pack is re-binned from the kept vertices.
 template <class G>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_fill_kept(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
@@ -657,11 +639,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
         const ViewDesc& vd = views[v];
         const int i0 = (int)(item / d.bins_x) * kBinQY, j0 = (int)(item % d.bins_x) * kBinQX;
         const int i = i0 + tid / kBinQX, j = j0 + tid % kBinQX;
-        const uint32_t q = ((uint32_t)i << 16) | (uint32_t)j;
-        const uint2 r = ws.ranges[(size_t)wi * 256 + tid];
+        const uint32_t r = ws.ranges[(size_t)wi * 256 + tid];
         bool more = false;
         int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
-        if (r.x == kRangeBig) {
+        if (r == kRangeBig) {
             const G& g = geoms[vs];
             const float* vb = ws.vbuf + (size_t)wi * (3 * kBinVerts);
             Quad Q;
@@ -673,17 +654,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
                        },
                        vd.w, vd.h, false);
             more = quad_tiles(Q, tx0, tx1, ty0, ty1);
-        } else if (r.x != kRangeNone) {
-            unpack_range(r.x, tx0, tx1, ty0, ty1);
-            if (tx0 == tx1 && ty0 == ty1) {
-                const size_t bin = (size_t)(vd.tile_base + (uint32_t)(ty0 * vd.tiles_x + tx0)) * d.nsrc + s;
-                ws.lists[(size_t)ws.offsets[bin] + r.y] = q;
-            } else {
-                more = true;
-            }
+        } else if (r != kRangeNone) {
+            unpack_range(r, tx0, tx1, ty0, ty1);
+            more = true;
         }
-        // multi-tile entries (narrow ranges by their own lane, wide ones dealt
-        // to the block): LDS bin table, one fill-cursor atomic per bin
+        // entries (narrow ranges by their own lane, wide ones dealt to the
+        // block): LDS bin table, one fill-cursor atomic per bin
         auto tile_of = [&](int tx, int ty) { return vd.tile_base + (uint32_t)(ty * vd.tiles_x + tx); };
         auto quad_of = [&](int lane) {
             return ((uint32_t)(i0 + lane / kBinQX) << 16) | (uint32_t)(j0 + lane % kBinQX);
@@ -705,15 +681,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
             const int k = table_slot(t_key, t);
             if (k >= 0) {
                 atomicAdd(&t_n[k], 1u);
-            } else {
+            } else {   // table full: a slot of its own
                 const size_t bin = (size_t)t * d.nsrc + s;
-                ws.lists[(size_t)ws.offsets[bin] + ws.single[bin] + atomicAdd(&ws.fill[bin], 1u)] = quad_of(lane);
+                ws.lists[(size_t)ws.offsets[bin] + atomicAdd(&ws.fill[bin], 1u)] = quad_of(lane);
             }
         });
         __syncthreads();   // table complete
         if (tid < kBinTable && t_n[tid]) {
             const size_t bin = (size_t)t_key[tid] * d.nsrc + s;
-            t_base[tid] = ws.offsets[bin] + ws.single[bin] + atomicAdd(&ws.fill[bin], t_n[tid]);
+            t_base[tid] = ws.offsets[bin] + atomicAdd(&ws.fill[bin], t_n[tid]);
         }
         __syncthreads();   // bases known
         each_tile([&](int lane, int tx, int ty) {
@@ -729,17 +705,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
 // it exceeds the list capacity).
 constexpr int kScanPer = 2048;   // bins per scan block (256 threads x 8)
 
-__global__ void __launch_bounds__(256) k_scan_blocks(const uint32_t* __restrict__ single,
-                                                     const uint32_t* __restrict__ multi, uint32_t* counts,
-                                                     uint32_t* offsets, size_t n, unsigned long long* bsum) {
+__global__ void __launch_bounds__(256) k_scan_blocks(const uint32_t* __restrict__ counts, uint32_t* offsets,
+                                                     size_t n, unsigned long long* bsum) {
     __shared__ unsigned long long part[256];
     const size_t base = (size_t)blockIdx.x * kScanPer + (size_t)threadIdx.x * 8;
     uint32_t c[8];
     unsigned long long sum = 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        c[k] = base + k < n ? single[base + k] + multi[base + k] : 0u;
-        if (base + k < n) counts[base + k] = c[k];
+        c[k] = base + k < n ? counts[base + k] : 0u;
         sum += c[k];
     }
     part[threadIdx.x] = sum;
@@ -815,8 +789,7 @@ hipError_t launch_bin(const G* geoms, const ViewDesc* views, const BatchDims& d,
         hipLaunchKernelGGL((k_bin<G, false, false>), dim3((unsigned)blocks), dim3(256), 0, s, geoms, views, d, ws);
     const size_t nbins = (size_t)d.ntiles * d.nsrc;
     const size_t nb = scan_blocks(nbins);
-    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(256), 0, s, ws.single, ws.multi, ws.counts,
-                       ws.offsets, nbins, ws.bsum);
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(256), 0, s, ws.counts, ws.offsets, nbins, ws.bsum);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, ws.bsum, (uint32_t)nb, d.cap, ws.status);
     hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((nbins + 255) / 256)), dim3(256), 0, s, ws.offsets, nbins,
                        ws.bsum);
