@@ -24,7 +24,8 @@ Prints one JSON line:
   open_loop     the same server under Poisson arrivals at 80% of `value`.
   batched       one launch set of `batch` views.
   batched_mixed_graph  64 random-walk poses at 720p / 1080p / 4K in one
-                launch set captured in a HIP graph (entry capacity autotuned).
+                launch set captured in a HIP graph (raster tile edge and entry
+                capacity autotuned on it).
   serving_mix   two resident scenes (the ERP set + 11 perspective 2048x2048
                 views), requests from 720p to 4K perspective and 2048x1024
                 ERP: closed-loop capacity and Poisson arrivals at 70% of it.
@@ -37,6 +38,9 @@ Prints one JSON line:
                 k_raster's HIP-event duration, vs 8 TB/s; `traffic` = its HBM
                 bytes per launch from the committed counter summary.
   cpu_baseline  the single-threaded CPU oracle on one view of the workload.
+The raster tile edge (16 or 32 pixels; same results either way) is autotuned
+on the 8-view launch set and used by every path of the workload
+(config.raster_tile), unless --tile names one.
 """
 import argparse
 import json
@@ -151,6 +155,9 @@ def main():
     ap.add_argument("--launch-set", default="",
                     help="only run one launch set 5 times: RIG,WxH[e],N -- N serving poses of rig RIG at that size "
                          "(suffix e: equirectangular target), e.g. persp11_2048,3840x2160,4 (profiling)")
+    ap.add_argument("--tile", type=int, default=0,
+                    help="raster tile edge, 16 or 32 (0: autotuned; the library's default, 16, in the launch-set "
+                         "and roofline-only modes)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes 5 times (counter calibration)")
     args = ap.parse_args()
@@ -181,13 +188,13 @@ def main():
         rg_ls = W.rig(name)
         srcs_ls = W.device_sources(rg_ls, W.render_sources(rg_ls))
         tg = [t for _, t in W.serve_requests([rg_ls], int(n), seed=4, sizes=((w, h, 1 if erp else 0),))]
-        bs = R.BatchSynthesizer(srcs_ls, tg, want_color=False)
+        bs = R.BatchSynthesizer(srcs_ls, tg, want_color=False, tile=args.tile or None)
         for _ in range(5):
             bs.run()
         torch.cuda.synchronize()
         st, cnt = stage_profile(bs.run, 3, R)
-        print(json.dumps({"launch_set": args.launch_set, "alg_bytes": b_alg(rg_ls, tg), "stage_ms": st,
-                          "counters": cnt}), flush=True)
+        print(json.dumps({"launch_set": args.launch_set, "tile": args.tile or 16, "alg_bytes": b_alg(rg_ls, tg),
+                          "stage_ms": st, "counters": cnt}), flush=True)
         return
 
     rg = W.rig(args.workload)
@@ -200,14 +207,23 @@ def main():
     torch.cuda.synchronize()
     if args.roofline_only > 0:
         walk = roofline_walk(W, rg, args.roofline_only)
-        bs = R.BatchSynthesizer(srcs, walk, want_color=False)
+        bs = R.BatchSynthesizer(srcs, walk, want_color=False, tile=args.tile or None)
         for _ in range(5):
             bs.run()
         torch.cuda.synchronize()
         st, cnt = stage_profile(bs.run, 3, R)
-        print(json.dumps({"roofline_only_views": args.roofline_only, "alg_bytes": b_alg(rg, walk),
+        print(json.dumps({"roofline_only_views": args.roofline_only, "tile": args.tile or 16,
+                          "alg_bytes": b_alg(rg, walk),
                           "stage_ms": st, "counters": cnt}), flush=True)
         return
+
+    # ------------------------------------- raster tile edge (same bits either way)
+    if args.tile:
+        tile, tile_ms = args.tile, None
+    else:
+        tile, tile_ms = A.tune_tile(srcs, roofline_walk(W, rg))
+    tgt.tile = tile
+    log("raster tile %d (%s)" % (tile, tile_ms))
 
     # ---------------------------------------------------------- plain per-view path
     syn = R.Synthesizer(srcs, tgt, want_maps=False, want_color=False)
@@ -238,6 +254,8 @@ def main():
 
     # --------------------------------------------------------------- autotuning
     walk = W.batch_targets(rg, 256, seed=4, sizes=((tw, th),))
+    for t in walk:
+        t.tile = tile
     cap, cap_counters = A.tune_entry_cap(srcs, walk[:16])
     log("entries per view %d (sample: %d entries, largest bin %d)" % (cap, cap_counters["bin_entries"],
                                                                      cap_counters["max_bin"]))
@@ -271,7 +289,7 @@ def main():
 
     # ------------------- dominant kernel: one launch set of ROOFLINE_VIEWS views
     rwalk = roofline_walk(W, rg)
-    bs = R.BatchSynthesizer(srcs, rwalk, want_color=False)
+    bs = R.BatchSynthesizer(srcs, rwalk, want_color=False, tile=tile)
     for _ in range(5):
         bs.run()
     torch.cuda.synchronize()
@@ -300,6 +318,12 @@ def main():
         # 64 random-walk poses, sizes cycling 1280x720 / 1920x1080 / 3840x2160,
         # one launch set captured in a HIP graph and replayed
         tg = W.batch_targets(rg, 64, seed=3)
+        if args.tile:
+            tile3, tile3_ms = args.tile, None
+        else:
+            tile3, tile3_ms = A.tune_tile(srcs, tg, reps=2)
+        for t in tg:
+            t.tile = tile3
         cap3, _ = A.tune_entry_cap(srcs, tg[:15])
         bs3 = R.BatchSynthesizer(srcs, tg, entry_cap=min(cap3 * len(tg), 0xFFFFFFFF), want_color=False)
         bs3.run()
@@ -327,6 +351,7 @@ def main():
         rep_s = (time.perf_counter() - a) / reps
         ok = bs3.check() == 0
         c3 = {"views": 64, "sizes": "1280x720, 1920x1080, 3840x2160 cycling", "entry_cap": cap3,
+              "raster_tile": tile3, "tile_ms": tile3_ms,
               "graph_views_s": round(64 / rep_s, 2), "graph_batch_ms": round(1e3 * rep_s, 3),
               "eager_batch_ms": round(1e3 * eager_s, 3), "bins_ok": bool(ok),
               "alg_bytes": b_alg(rg, tg), "achieved_GBs": round(b_alg(rg, tg) / rep_s / 1e9, 1)}
@@ -455,7 +480,8 @@ def main():
                         "random-walk poses, blending 1.0, YUV420 10-bit out" % args.workload,
             "mode": "served: closed loop, concurrency = batch x slots, continuous batching; a step = one batch "
                     "round (steps x max_batch requests timed)",
-            "max_batch": B, "slots": SL, "entries_per_view": cap, "memory_budget_gib": args.budget_gib,
+            "max_batch": B, "slots": SL, "entries_per_view": cap, "raster_tile": tile,
+            "memory_budget_gib": args.budget_gib,
             "p99_bound_ms": P99_BOUND_MS,
         },
         "served": served,
@@ -477,7 +503,7 @@ def main():
         "batched_mixed_graph": c3,
         "serving_mix": c4,
         "serving_scenes_8k": c5,
-        "autotune": {"trials": trials},
+        "autotune": {"trials": trials, "tile_ms": tile_ms},
         "roofline": {
             "kernel": "k_raster (tile raster + blend), one launch over %d views" % ROOFLINE_VIEWS,
             "bound": "hbm",

@@ -213,6 +213,9 @@ typedef struct {
                                  (SynthesizedView.cpp:326-331, image_writing.cpp:189-190) */
     float color_scale;        /* fl(1/max_level) of the first loaded view (image_loading.cpp:134) */
     float depth_scale;        /* idem for depth (image_loading.cpp:193) */
+    int tile;                 /* raster tile edge: 0 or 16 (16 x 16 tiles), or 32; a launch set uses
+                                 the largest its views ask for (results are the same; the autotuner
+                                 picks the faster per workload) */
 } rvs_target;
 
 /* Device outputs of one synthesized view; any pointer may be NULL, but a view

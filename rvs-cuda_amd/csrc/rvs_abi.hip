@@ -144,9 +144,10 @@ uint32_t entry_capacity(uint32_t requested, uint64_t npx, int nsrc) {
 }
 
 void plan_raster(Layout& L, RasterPlan& R, int nsrc, int nviews, uint32_t ntiles, uint32_t npx, uint32_t cap,
-                 int max_sw, int max_sh, uint64_t keep_max = 0) {
+                 int max_sw, int max_sh, uint64_t keep_max = 0, int shift = 4) {
     rvs_k::BatchDims& d = R.d;
     d.nviews = nviews; d.nsrc = nsrc; d.ntiles = ntiles; d.npx = npx;
+    d.tile_shift = shift;
     d.cap = entry_capacity(cap, npx, nsrc);
     d.max_sw = max_sw; d.max_sh = max_sh;
     d.bins_x = (uint32_t)((max_sw - 1 + rvs_k::kBinQX - 1) / rvs_k::kBinQX);
@@ -212,10 +213,12 @@ hipError_t reset_raster(const rvs_k::RasterWS& ws, const rvs_k::BatchDims& d, hi
 }
 
 // A view's tile grid and its place in the batch's global tile / pixel numbering.
-void view_grid(rvs_k::ViewDesc& v, int w, int h, uint32_t& tiles, uint32_t& px) {
+void view_grid(rvs_k::ViewDesc& v, int w, int h, uint32_t& tiles, uint32_t& px, int shift = 4) {
+    const int edge = 1 << shift;
     v.w = w; v.h = h;
-    v.tiles_x = (w + rvs_k::kTile - 1) / rvs_k::kTile;
-    v.tiles_y = (h + rvs_k::kTile - 1) / rvs_k::kTile;
+    v.tile_shift = shift;
+    v.tiles_x = (w + edge - 1) / edge;
+    v.tiles_y = (h + edge - 1) / edge;
     v.tile_base = tiles;
     v.px_base = px;
     tiles += (uint32_t)(v.tiles_x * v.tiles_y);
@@ -246,6 +249,7 @@ int check_raster_status(const rvs_k::RasterWS& ws, const rvs_k::BatchDims& d, ui
 struct SynthPlan {
     RasterPlan R;
     size_t views, geoms, bounds, quality, color, inpaint, resized, total;
+    int tile_shift;       // log2 of the launch set's tile edge
     int max_w, max_h;     // largest raster (oversampled by Precision)
     int max_ew, max_eh;   // largest view
     bool any_persp, any_rescaled;
@@ -269,6 +273,17 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
     }
     uint64_t tiles = 0, px = 0, rpx = 0;
     P.max_w = 0; P.max_h = 0; P.max_ew = 0; P.max_eh = 0; P.any_persp = false; P.any_rescaled = false;
+    // the launch set's tile edge: the largest any view asks for
+    int shift = 4;
+    for (int v = 0; v < n_views; ++v) {
+        const int t = tgts[v].tile;
+        if (t != 0 && t != rvs_k::kTile && t != rvs_k::kTileBig)
+            return fail(RVS_E_INVALID, "synthesize: view %d: tile edge %d (0, %d or %d)", v, t, rvs_k::kTile,
+                        rvs_k::kTileBig);
+        if (t == rvs_k::kTileBig) shift = 5;
+    }
+    P.tile_shift = shift;
+    const int edge = 1 << shift;
     for (int v = 0; v < n_views; ++v) {
         const rvs_target& t = tgts[v];
         if (!cam_ok(&t.cam)) return fail(RVS_E_INVALID, "synthesize: view %d has a bad camera", v);
@@ -288,7 +303,7 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
             P.any_rescaled = true;
             rpx += (uint64_t)t.cam.w * t.cam.h;
         }
-        tiles += (uint64_t)((rw + rvs_k::kTile - 1) / rvs_k::kTile) * ((rh + rvs_k::kTile - 1) / rvs_k::kTile);
+        tiles += (uint64_t)((rw + edge - 1) / edge) * ((rh + edge - 1) / edge);
         px += (uint64_t)rw * rh;
     }
     if (max_sw > rvs_k::kMaxSourceSide || max_sh > rvs_k::kMaxSourceSide)
@@ -308,7 +323,7 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
     P.quality = L.take((size_t)px * sizeof(float));
     // the colour map of views whose caller wants only the encoded planes
     P.color = L.take((size_t)px * 3 * sizeof(float));
-    plan_raster(L, P.R, n_src, n_views, (uint32_t)tiles, (uint32_t)px, cap, max_sw, max_sh, keep_max);
+    plan_raster(L, P.R, n_src, n_views, (uint32_t)tiles, (uint32_t)px, cap, max_sw, max_sh, keep_max, shift);
     P.inpaint = L.take(rvs_k::inpaint_ws_bytes((size_t)px, n_views));
     // oversampled views: their colour resized to the view's size, for the encoder
     P.resized = L.take((size_t)rpx * 3 * sizeof(float));
@@ -321,7 +336,7 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
             std::memset(&d, 0, sizeof d);
             int rw, rh;
             raster_size(tgts[v].cam, tgts[v].rescale, rw, rh);
-            view_grid(d, rw, rh, tb, pb);
+            view_grid(d, rw, rh, tb, pb, shift);
             d.ew = tgts[v].cam.w;
             d.eh = tgts[v].cam.h;
             d.bitdepth = tgts[v].bitdepth;

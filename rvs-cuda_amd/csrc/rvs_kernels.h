@@ -17,7 +17,9 @@
 
 namespace rvs_k {
 
-constexpr int kTile = 16;            // target tile edge (pixels); one 256-thread block per tile
+constexpr int kTile = 16;            // default target tile edge (pixels), and the edge of the sub-tiles
+                                     // the in-order replay works on
+constexpr int kTileBig = 32;         // the other tile edge a launch set can use (BatchDims::tile_shift)
 constexpr int kBinQX = 64;           // binning work item: 64 x 4 source quads
 constexpr int kBinQY = 4;
 constexpr int kFallbackChunk = 128;  // list entries per LDS chunk in the in-order replay
@@ -49,6 +51,7 @@ struct ViewDesc {
     int ew, eh;              // size of the encoded planes: the view's own size (w, h are the raster's,
                              // oversampled by Precision)
     float* ecol;             // colour the planes encode: color, or its resize to ew x eh
+    int tile_shift;          // log2 of the launch set's tile edge
     int enc_raster;          // planes wanted, not oversampled: tiles without holes or deferred
                              // pixels are encoded by k_raster itself
     int partial_color;       // color is internal: tiles encoded by k_raster write it only on
@@ -82,6 +85,8 @@ struct BatchDims {
     uint32_t range_cap;      // work items whose tile ranges and vertices are kept between the passes
     int max_sw, max_sh;      // largest source grid
     uint32_t bins_x, bins_y; // binning work items per source (for the largest source)
+    int tile_shift;          // log2 of the tile edge: 4 (16 x 16 tiles, 256-thread raster blocks) or
+                             // 5 (32 x 32, 1024-thread blocks)
 };
 
 enum { MODE_BLEND = 0, MODE_RAW = 1 };

@@ -143,8 +143,8 @@ __device__ inline void quad_build(Quad& Q, const G& g, int i, int j, VF vtx, int
 // with tv, also the validities of T1 and T2 (0 when not alive or left
 // undecided by the fast path).  False when none survives.
 template <class G, class VF>
-__device__ inline bool quad_range_tv(const G& g, int i, int j, VF vtx, int out_w, int out_h, int& tx0, int& tx1,
-                                     int& ty0, int& ty1, float2* tv) {
+__device__ inline bool quad_range_tv(const G& g, int i, int j, VF vtx, int out_w, int out_h, int shift, int& tx0,
+                                     int& tx1, int& ty0, int& ty1, float2* tv) {
     if (tv) *tv = make_float2(0.f, 0.f);
     const float3 v01 = vtx(i, j + 1), v10 = vtx(i + 1, j);
     if (!(v01.z > 0.f && v10.z > 0.f && v01.x > 0.f && v10.x > 0.f)) return false;
@@ -170,7 +170,7 @@ __device__ inline bool quad_range_tv(const G& g, int i, int j, VF vtx, int out_w
         one(v10, l1, v00, nullptr);
     }
     if (xmin > xmax) return false;
-    tx0 = xmin / kTile; tx1 = xmax / kTile; ty0 = ymin / kTile; ty1 = ymax / kTile;
+    tx0 = xmin >> shift; tx1 = xmax >> shift; ty0 = ymin >> shift; ty1 = ymax >> shift;   // boxes start at >= 0
     return true;
 }
 
@@ -435,7 +435,7 @@ __device__ inline int find_slot(const uint32_t* keys, uint32_t t) {
 
 // Tile range [tx0, tx1] x [ty0, ty1] touched by the bounding boxes of a
 // quad's surviving triangles; false when none survives.
-__device__ inline bool quad_tiles(const Quad& Q, int& tx0, int& tx1, int& ty0, int& ty1) {
+__device__ inline bool quad_tiles(const Quad& Q, int shift, int& tx0, int& tx1, int& ty0, int& ty1) {
     if (!Q.alive) return false;
     int xmin = INT_MAX, xmax = INT_MIN, ymin = INT_MAX, ymax = INT_MIN;
     for (int k = 0; k < 4; ++k)
@@ -443,7 +443,7 @@ __device__ inline bool quad_tiles(const Quad& Q, int& tx0, int& tx1, int& ty0, i
             xmin = min(xmin, Q.t[k].xmin); xmax = max(xmax, Q.t[k].xmax);
             ymin = min(ymin, Q.t[k].ymin); ymax = max(ymax, Q.t[k].ymax);
         }
-    tx0 = xmin / kTile; tx1 = xmax / kTile; ty0 = ymin / kTile; ty1 = ymax / kTile;
+    tx0 = xmin >> shift; tx1 = xmax >> shift; ty0 = ymin >> shift; ty1 = ymax >> shift;
     return true;
 }
 
@@ -538,10 +538,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
         auto quad_range = [&](auto vtx) {
             if (stored) {
                 float2 tv;
-                more = quad_range_tv(g, i, j, vtx, vd.w, vd.h, tx0, tx1, ty0, ty1, &tv);
+                more = quad_range_tv(g, i, j, vtx, vd.w, vd.h, d.tile_shift, tx0, tx1, ty0, ty1, &tv);
                 ws.tvq[(size_t)wi * 256 + tid] = tv;
             } else {
-                more = quad_range_tv(g, i, j, vtx, vd.w, vd.h, tx0, tx1, ty0, ty1, nullptr);
+                more = quad_range_tv(g, i, j, vtx, vd.w, vd.h, d.tile_shift, tx0, tx1, ty0, ty1, nullptr);
             }
         };
         {
@@ -653,7 +653,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
                            return vertex(g, ii, jj);
                        },
                        vd.w, vd.h, false);
-            more = quad_tiles(Q, tx0, tx1, ty0, ty1);
+            more = quad_tiles(Q, d.tile_shift, tx0, tx1, ty0, ty1);
         } else if (r != kRangeNone) {
             unpack_range(r, tx0, tx1, ty0, ty1);
             more = true;
@@ -873,6 +873,7 @@ constexpr uint32_t kDeferKey = 0xffffffffu;
 // covered (then it is not evaluated: the reference computes it before the
 // bounding box test, but it never causes a return, and a triangle that covers
 // no pixel is never a winner, so skipping it changes nothing).
+template <int TILE>
 __device__ __forceinline__ float raster_tri(float3 a, float3 b, float3 c, uint32_t id, bool prol_false_possible, int tx0,
                                    int ty0, int out_w, int out_h, float rescale, unsigned long long* key1,
                                    uint32_t* key2, bool undecided, float tv_known) {
@@ -881,8 +882,8 @@ __device__ __forceinline__ float raster_tri(float3 a, float3 b, float3 c, uint32
     const bool prol = (a.z > 0.f) || (b.z > 0.f) || (c.z > 0.f);
     bool special = !finite3(a) || !finite3(b) || !finite3(c) || !is_finite(T.den) ||
                    (prol_false_possible && !prol) || undecided;
-    int xa = max(T.xmin, tx0), xb = min(T.xmax, tx0 + kTile - 1);
-    int ya = max(T.ymin, ty0), yb = min(T.ymax, ty0 + kTile - 1);
+    int xa = max(T.xmin, tx0), xb = min(T.xmax, tx0 + TILE - 1);
+    int ya = max(T.ymin, ty0), yb = min(T.ymax, ty0 + TILE - 1);
     if (!special) tri_tight(T, xa, xb, ya, yb);
     if (xa > xb || ya > yb) return 0.f;   // no pixel of this tile can be covered
     auto push = [&](int x, int y) {
@@ -890,7 +891,7 @@ __device__ __forceinline__ float raster_tri(float3 a, float3 b, float3 c, uint32
         tri_num(T, x, y, n1, n2);
         float l1, l2, l3;
         if (!tri_cover(T, n1, n2, l1, l2, l3)) return;
-        const int p = (y - ty0) * kTile + (x - tx0);
+        const int p = (y - ty0) * TILE + (x - tx0);
         const float dd = tri_depth(T, l1, l2, l3);
         if (special || !depth_regular(dd)) {
             atomicMax(&key2[p], kDeferKey);
@@ -953,7 +954,7 @@ struct RasterCache {
 // concatenated bins.  A non-deferred pixel's winner beats every other
 // candidate by more than the float test's error, so any unique id works; equal
 // scores are deferred to the in-order replay.
-template <class G, class Cache>
+template <int TILE, class G, class Cache>
 __device__ __forceinline__ void raster_quad(const G& g, uint32_t ij, uint32_t e, int tx0, int ty0, int out_w, int out_h,
                                    unsigned long long* key1, uint32_t* key2, Cache* C,
                                    const RasterWS& ws, const BatchDims& d, uint32_t vs) {
@@ -978,18 +979,20 @@ __device__ __forceinline__ void raster_quad(const G& g, uint32_t ij, uint32_t e,
     {
         const float3 v00 = vtx(0, 0);
         if (gate && v00.z > 0.f && v00.x > 0.f)
-            tv0 = raster_tri(v00, v01, v10, s0, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und, tvk.x);
+            tv0 = raster_tri<TILE>(v00, v01, v10, s0, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und,
+                                   tvk.x);
     }
     {
         const float3 v11 = vtx(1, 1);
         if (gate && v11.z > 0.f && v11.x > 0.f)
-            tv1 = raster_tri(v11, v10, v01, s0 | 1u, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und, tvk.y);
+            tv1 = raster_tri<TILE>(v11, v10, v01, s0 | 1u, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und,
+                                   tvk.y);
     }
     if (gate && g.wrap && j == 0) {
         const float3 v00 = vtx(0, 0);
         const float3 l0 = vertex(g, i, g.sw - 1, &und), l1 = vertex(g, i + 1, g.sw - 1, &und);
-        raster_tri(l0, v00, l1, s0 | 2u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und, 0.f);
-        raster_tri(v10, l1, v00, s0 | 3u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und, 0.f);
+        raster_tri<TILE>(l0, v00, l1, s0 | 2u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und, 0.f);
+        raster_tri<TILE>(v10, l1, v00, s0 | 3u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und, 0.f);
     }
     if (C) {
         C->tv[0][e] = tv0;
@@ -1015,21 +1018,23 @@ __device__ inline void write_pixel(const ViewDesc& vd, size_t o, const BlendStat
 
 // write_color (image_writing.cpp:142-203) of one pixel of a tile, from the
 // blended colour in registers: Y here, U and V by the top-left pixel of each
-// 2x2 block from its wave neighbours (a 16-pixel tile row is a quarter wave,
-// so the block's four pixels are lanes l, l+1, l+16, l+17 of one wave).  The
+// 2x2 block from its wave neighbours (a tile row is at most half a wave, so
+// the block's four pixels are lanes l, l+1, l+TILE, l+TILE+1 of one wave).  The
 // same quantisation and chroma rule as k_encode_views; tiles start at even
 // coordinates and encoded views have even sizes, so a block is inside or
 // outside as a whole.
+template <int TILE>
 __device__ inline void encode_tile_pixel(const ViewDesc& vd, float3 c, int px, int py, bool inside, int lx, int ly) {
     const unsigned maxv = (1u << vd.bitdepth) - 1u;
     const bool u8 = vd.bitdepth <= 8;
     uint32_t qy = quantize_u32(c.x, maxv), qu = quantize_u32(c.y, maxv), qv = quantize_u32(c.z, maxv);
     // the plane's element type truncates, as (T) does in k_encode_views
     if (u8) { qy &= 0xffu; qu &= 0xffu; qv &= 0xffu; } else { qy &= 0xffffu; qu &= 0xffffu; qv &= 0xffffu; }
-    const uint32_t su = qu + (uint32_t)__shfl_down((int)qu, 1) + (uint32_t)__shfl_down((int)qu, 16) +
-                        (uint32_t)__shfl_down((int)qu, 17);
-    const uint32_t sv = qv + (uint32_t)__shfl_down((int)qv, 1) + (uint32_t)__shfl_down((int)qv, 16) +
-                        (uint32_t)__shfl_down((int)qv, 17);
+    static_assert(2 * TILE <= 64, "a 2x2 block's two rows must lie in one wave");
+    const uint32_t su = qu + (uint32_t)__shfl_down((int)qu, 1) + (uint32_t)__shfl_down((int)qu, TILE) +
+                        (uint32_t)__shfl_down((int)qu, TILE + 1);
+    const uint32_t sv = qv + (uint32_t)__shfl_down((int)qv, 1) + (uint32_t)__shfl_down((int)qv, TILE) +
+                        (uint32_t)__shfl_down((int)qv, TILE + 1);
     if (!inside) return;
     const size_t o = (size_t)py * vd.ew + px;
     const bool corner = ((lx | ly) & 1) == 0;
@@ -1049,23 +1054,27 @@ __device__ inline void encode_tile_pixel(const ViewDesc& vd, float3 c, int px, i
 // MULTI: more sources than one phase holds (the phases loop, and the blend
 // state lives across them); otherwise one phase, which keeps the coverage pass
 // free of the blend state's registers.
-template <class G, int MODE, bool MULTI>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI ? 6 : 8))) k_raster(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
+// TILE: the tile edge; a block has a thread per pixel (256 or 1024 threads),
+// and the caches scale with the tile's area.
+template <class G, int MODE, bool MULTI, int TILE>
+__global__ void __launch_bounds__(TILE * TILE) __attribute__((amdgpu_waves_per_eu(MULTI ? 6 : 8))) k_raster(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
                                                 BatchDims d, RasterWS ws) {
-    __shared__ unsigned long long key1[kRasterGroup][kTile * kTile];
-    __shared__ uint32_t key2[kRasterGroup][kTile * kTile];
+    constexpr int NT = TILE * TILE;
+    constexpr int AREA = NT / (kTile * kTile);
+    __shared__ unsigned long long key1[kRasterGroup][NT];
+    __shared__ uint32_t key2[kRasterGroup][NT];
     __shared__ uint32_t pref[kRasterGroup + 1];
     __shared__ int s_deferred;   // any pixel of the tile deferred
     __shared__ int s_hole;       // any pixel of the tile a hole (quality not > 0)
-    // one phase: a smaller cache keeps the block within 20 KB of LDS (8 blocks per CU)
-    __shared__ RasterCache<MULTI ? kRasterCache : kRasterCacheOne> cache;
+    // one phase: a smaller cache keeps a 16 x 16 block within 20 KB of LDS (8 blocks per CU)
+    __shared__ RasterCache<(MULTI ? kRasterCache : kRasterCacheOne) * AREA> cache;
     const uint32_t T = tile_of_block(blockIdx.x, d.ntiles);
     if (T >= d.ntiles || ws.status[1]) return;   // bins not filled: capacity reported to the host
     const int v = find_view(views, d.nviews, T);
     const ViewDesc& vd = views[v];
     const int tile = (int)(T - vd.tile_base);
     const int tid = threadIdx.x;
-    const int tx0 = (tile % vd.tiles_x) * kTile, ty0 = (tile / vd.tiles_x) * kTile;
+    const int tx0 = (tile % vd.tiles_x) * TILE, ty0 = (tile / vd.tiles_x) * TILE;
     // The thread's index is re-derived after the coverage pass (wave base in
     // an SGPR + lane id) instead of being held in a VGPR through it.
     const int wave_base = __builtin_amdgcn_readfirstlane(tid & ~63);
@@ -1093,19 +1102,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI 
         const uint32_t N = __builtin_amdgcn_readfirstlane(pref[gs]);   // uniform: kept in an SGPR
         const bool cached = N <= (uint32_t)cache.cap;
         // a wave-uniform trip count; the entry index is re-derived each trip
-        const uint32_t trips = (N + 255u) / 256u;
+        const uint32_t trips = (N + (uint32_t)NT - 1u) / (uint32_t)NT;
         for (uint32_t it = 0; it < trips; ++it) {
-            const uint32_t e = it * 256u + (uint32_t)(wave_base + lane_now());
+            const uint32_t e = it * (uint32_t)NT + (uint32_t)(wave_base + lane_now());
             if (e >= N) continue;
             int k = 0;
             while (k + 1 < gs && e >= pref[k + 1]) ++k;
             const uint32_t ij = ws.lists[(size_t)ws.offsets[(size_t)T * d.nsrc + s0 + k] + (e - pref[k])];
-            raster_quad(gv[s0 + k], ij, e, tx0, ty0, vd.w, vd.h, key1[k], key2[k], cached ? &cache : nullptr,
-                        ws, d, (uint32_t)(v * d.nsrc + s0 + k));
+            raster_quad<TILE>(gv[s0 + k], ij, e, tx0, ty0, vd.w, vd.h, key1[k], key2[k], cached ? &cache : nullptr,
+                              ws, d, (uint32_t)(v * d.nsrc + s0 + k));
         }
         __syncthreads();
         const int t2 = wave_base + lane_now();
-        const int px = tx0 + (t2 % kTile), py = ty0 + (t2 / kTile);
+        const int px = tx0 + (t2 % TILE), py = ty0 + (t2 / TILE);
         const bool inside = px < vd.w && py < vd.h;
         if (inside) {
             for (int k = 0; k < gs; ++k) {
@@ -1162,7 +1171,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI 
         __syncthreads();
     }
     const int t3 = wave_base + lane_now();
-    const int lx = t3 % kTile, ly = t3 / kTile;
+    const int lx = t3 % TILE, ly = t3 / TILE;
     const int px = tx0 + lx, py = ty0 + ly;
     const bool inside = px < vd.w && py < vd.h;
     const size_t o = (size_t)py * vd.w + px;
@@ -1176,10 +1185,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MULTI 
     if (inside && !deferred) {
         // an internal colour map of a tile encoded here is read only on the
         // tile's border (by the inpainting of a neighbouring tile)
-        const bool border = lx == 0 || ly == 0 || lx == kTile - 1 || ly == kTile - 1;
+        const bool border = lx == 0 || ly == 0 || lx == TILE - 1 || ly == TILE - 1;
         write_pixel<MODE>(vd, o, st, rc, rd, rs, !(enc && vd.partial_color && !border));
     }
-    if (enc) encode_tile_pixel(vd, st.c, px, py, inside, lx, ly);
+    if (enc) encode_tile_pixel<TILE>(vd, st.c, px, py, inside, lx, ly);
     if (t3 == 0) ws.tile_enc[T] = enc ? 1 : 0;
     // the replay reads the per-pixel flags of the listed tiles only
     if (s_deferred) {
@@ -1395,12 +1404,21 @@ __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, c
         const ViewDesc& vd = views[v];
         const G* gv = geoms + (size_t)v * d.nsrc;
         const int tile = (int)(T - vd.tile_base);
-        const int tx0 = (tile % vd.tiles_x) * kTile, ty0 = (tile / vd.tiles_x) * kTile;
+        const int edge = 1 << d.tile_shift;
+        // the tile in sub-tiles of kTile x kTile pixels, a thread per pixel
+        // (each folds the whole bin of the tile, clipped to its sub-tile)
+        const int nsub = edge / kTile;
+        for (int sub = 0; sub < nsub * nsub; ++sub) {
+        const int tx0 = (tile % vd.tiles_x) * edge + (sub % nsub) * kTile;
+        const int ty0 = (tile / vd.tiles_x) * edge + (sub / nsub) * kTile;
         const int px = tx0 + (tid % kTile), py = ty0 + (tid / kTile);
         const bool inside = px < vd.w && py < vd.h;
         const size_t o = inside ? (size_t)py * vd.w + px : 0;
         const bool mine = inside && ws.deferred[vd.px_base + o];
         dmask[tid] = mine ? 1 : 0;
+        // a sub-tile of a larger tile may hold no deferred pixel (the barrier
+        // also publishes dmask)
+        if (!__syncthreads_or(mine ? 1 : 0)) continue;
         BlendState st;
         float3 rc = make_float3(0.f, 0.f, 0.f);
         float rd = __builtin_inff(), rs = 0.f;
@@ -1434,6 +1452,7 @@ __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, c
         }
         if (mine) write_pixel<MODE>(vd, o, st, rc, rd, rs);
         __syncthreads();
+        }
     }
 }
 
@@ -1441,10 +1460,20 @@ template <class G, int MODE>
 hipError_t launch_raster(const G* geoms, const ViewDesc* views, const BatchDims& d, const RasterWS& ws,
                          hipStream_t s) {
     const unsigned blocks = 8u * ((d.ntiles + 7u) / 8u);
-    if (d.nsrc > kRasterGroup)
-        hipLaunchKernelGGL((k_raster<G, MODE, true>), dim3(blocks), dim3(256), 0, s, geoms, views, d, ws);
-    else
-        hipLaunchKernelGGL((k_raster<G, MODE, false>), dim3(blocks), dim3(256), 0, s, geoms, views, d, ws);
+    const bool multi = d.nsrc > kRasterGroup;
+    if (d.tile_shift == 5) {
+        constexpr int NT = kTileBig * kTileBig;
+        if (multi)
+            hipLaunchKernelGGL((k_raster<G, MODE, true, kTileBig>), dim3(blocks), dim3(NT), 0, s, geoms, views, d, ws);
+        else
+            hipLaunchKernelGGL((k_raster<G, MODE, false, kTileBig>), dim3(blocks), dim3(NT), 0, s, geoms, views, d, ws);
+    } else {
+        constexpr int NT = kTile * kTile;
+        if (multi)
+            hipLaunchKernelGGL((k_raster<G, MODE, true, kTile>), dim3(blocks), dim3(NT), 0, s, geoms, views, d, ws);
+        else
+            hipLaunchKernelGGL((k_raster<G, MODE, false, kTile>), dim3(blocks), dim3(NT), 0, s, geoms, views, d, ws);
+    }
     return hipGetLastError();
 }
 

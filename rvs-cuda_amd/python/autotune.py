@@ -1,6 +1,8 @@
 """Autotuner of the batched and served paths.
 
 Picks, by measurement on the device it runs on:
+  * the raster tile edge (16 or 32 pixels) of a launch set: both are timed on
+    a sample batch, the faster is kept (results are the same bits either way);
   * the binned-entry list capacity per view (entries of (quad, tile) pairs)
     from what the binning pass reports for a sample batch, with headroom -- it
     sizes every workspace and hence how many views fit a memory budget;
@@ -18,6 +20,26 @@ import serving as S
 def yuv_bytes(t):
     """Bytes of a target's encoded YUV420 planes."""
     return t.cam.w * t.cam.h * 3 // 2 * (1 if t.bitdepth <= 8 else 2)
+
+
+def tune_tile(sources, targets, tiles=(16, 32), reps=3):
+    """Raster tile edge for batches like `targets`: each candidate's launch
+    set (encoded planes only) is run once to warm up, then timed `reps` times
+    (wall clock of the synchronous call); returns (fastest tile, {tile: median
+    ms}).  The targets' own tile fields are left as they are."""
+    import statistics
+    ms = {}
+    for t in tiles:
+        bs = R.BatchSynthesizer(sources, targets, want_color=False, tile=t)
+        bs.run()
+        times = []
+        for _ in range(reps):
+            a = time.perf_counter()
+            bs.run()
+            times.append(time.perf_counter() - a)
+        ms[t] = round(1e3 * statistics.median(times), 3)
+        del bs
+    return min(ms, key=ms.get), ms
 
 
 def tune_entry_cap(sources, targets, headroom=1.25):

@@ -43,7 +43,7 @@ class RvsSource(C.Structure):
 class RvsTarget(C.Structure):
     _fields_ = [("cam", RvsCam), ("pos", C.c_float * 3), ("rot", C.c_float * 3),
                 ("bitdepth", C.c_int), ("blending_exp", C.c_float), ("rescale", C.c_float),
-                ("color_scale", C.c_float), ("depth_scale", C.c_float)]
+                ("color_scale", C.c_float), ("depth_scale", C.c_float), ("tile", C.c_int)]
 
 
 class RvsViewOut(C.Structure):
@@ -408,13 +408,15 @@ def resize_color(src, out, stream=None):
     check(hip().rvs_resize_color(_ptr(src), sw, sh, _ptr(out), dw, dh, _stream(stream)))
 
 
-def make_target(c, pos, rot, bitdepth, blending_exp, color_scale, depth_scale, rescale=1.0):
+def make_target(c, pos, rot, bitdepth, blending_exp, color_scale, depth_scale, rescale=1.0, tile=0):
+    """tile: raster tile edge (0 / 16 or 32; a launch set uses its views' largest)."""
     t = RvsTarget()
     t.cam = c
     for k in range(3):
         t.pos[k], t.rot[k] = pos[k], rot[k]
     t.bitdepth, t.blending_exp, t.rescale = bitdepth, blending_exp, rescale
     t.color_scale, t.depth_scale = color_scale, depth_scale
+    t.tile = tile
     return t
 
 
@@ -422,10 +424,14 @@ class Synthesizer:
     """One-view-at-a-time synthesis (the plain per-view path): owns a workspace
     and output buffers for one target size."""
 
-    def __init__(self, sources, target, entry_cap=0, device="cuda", want_maps=True, want_color=True):
-        """want_color False: only the encoded planes (see BatchSynthesizer)."""
+    def __init__(self, sources, target, entry_cap=0, device="cuda", want_maps=True, want_color=True, tile=None):
+        """want_color False: only the encoded planes (see BatchSynthesizer);
+        tile: the raster tile edge (None: as the target says)."""
         import torch
         self.sources = sources
+        if tile is not None:
+            target = RvsTarget.from_buffer_copy(target)
+            target.tile = tile
         self.target = target
         self.entry_cap = entry_cap
         self.device = device
@@ -505,21 +511,26 @@ class BatchSynthesizer:
     """Many target views from the same resident sources in one launch set
     (rvs_synthesize_batch).  targets: list of RvsTarget (sizes may differ)."""
 
-    def __init__(self, sources, targets, entry_cap=0, device="cuda", want_maps=False, want_color=True):
+    def __init__(self, sources, targets, entry_cap=0, device="cuda", want_maps=False, want_color=True, tile=None):
         """want_color False: only the encoded planes (the colour map stays in
-        the workspace, written only where the inpainting needs it)."""
+        the workspace, written only where the inpainting needs it); tile: the
+        raster tile edge for every view (None: as the targets say)."""
         import torch
         self.sources = sources
         self.arr = (RvsSource * len(sources))(*[s.struct() for s in sources])
         self.entry_cap = entry_cap
         self.device = device
         self.want_color = want_color
+        self.tile = tile
         self.set_targets(targets, want_maps)
 
     def set_targets(self, targets, want_maps=False):
         import torch
         self.n = len(targets)
         self.targets = (RvsTarget * self.n)(*targets)
+        if self.tile is not None:
+            for t in self.targets:
+                t.tile = self.tile
         self.outs = []
         vo = (RvsViewOut * self.n)()
         for k, t in enumerate(targets):

@@ -47,14 +47,17 @@ def _check_oracle(out, ref, bitdepth, tag):
         assert np.array_equal(a, ref[k]), "%s: plane %s differs in %d samples" % (tag, k, int((a != ref[k]).sum()))
 
 
-def test_batch_equals_alone_mixed_sizes(torch_cuda):
+@pytest.mark.parametrize("tile", [16, 32])
+def test_batch_equals_alone_mixed_sizes(torch_cuda, tile):
+    """The batch at either raster tile size equals each view alone at the
+    default size (16) and the oracle."""
     torch = torch_cuda
-    say("case batch erp4/4 mixed sizes x6")
+    say("case batch erp4/4 mixed sizes x6, tile %d" % tile)
     rg = W.rig("erp4_1080p", scale=4)
     planes = W.render_sources(rg)
     srcs = W.device_sources(rg, planes)
     tgts = W.batch_targets(rg, 6, seed=3, scale=4)
-    bs = R.BatchSynthesizer(srcs, tgts, want_maps=True)
+    bs = R.BatchSynthesizer(srcs, tgts, want_maps=True, tile=tile)
     outs = bs.run()
     torch.cuda.synchronize()
     for k, t in enumerate(tgts):
@@ -68,16 +71,17 @@ def test_batch_equals_alone_mixed_sizes(torch_cuda):
         _check_oracle(outs[k], _oracle(rg, planes, tgts[k]), tgts[k].bitdepth, "batch view %d" % k)
 
 
-def test_batch_rects_many_views(torch_cuda):
+@pytest.mark.parametrize("tile", [16, 32])
+def test_batch_rects_many_views(torch_cuda, tile):
     torch = torch_cuda
-    say("case batch rects/2 x12")
+    say("case batch rects/2 x12, tile %d" % tile)
     rg = W.rig("rects_640", scale=2)
     planes = W.render_sources(rg)
     srcs = W.device_sources(rg, planes)
     tgts = []
     for k, (pos, rot) in enumerate(W.random_walk(12, seed=7, pos_lim=0.05, yaw_lim=8.0, pitch_lim=4.0)):
         tgts.append(W.device_target(rg, pos, rot))
-    bs = R.BatchSynthesizer(srcs, tgts)
+    bs = R.BatchSynthesizer(srcs, tgts, tile=tile)
     outs = bs.run()
     torch.cuda.synchronize()
     for k in range(len(tgts)):
@@ -88,15 +92,16 @@ def test_batch_rects_many_views(torch_cuda):
         _check_oracle(outs[k], _oracle(rg, planes, tgts[k]), tgts[k].bitdepth, "rects view %d" % k)
 
 
+@pytest.mark.parametrize("tile", [16, 32])
 @pytest.mark.parametrize("bd", [8, 10])
-def test_planes_only_batch_equals_full(torch_cuda, bd):
+def test_planes_only_batch_equals_full(torch_cuda, bd, tile):
     """Views that want only the encoded planes: tiles without holes or
     deferred pixels are encoded by the raster pass itself and the colour map
     (in the workspace) is written only on their borders; the planes equal those
     of the same batch with full colour maps and the oracle's, at 8 and 10 bits,
     with holes (the rects scene's uncovered border) and deferred pixels."""
     torch = torch_cuda
-    say("case planes-only batch, %d bit" % bd)
+    say("case planes-only batch, %d bit, tile %d" % (bd, tile))
     rg = W.rig("rects_640", scale=2)
     planes = W.render_sources(rg)
     srcs = W.device_sources(rg, planes)
@@ -107,7 +112,7 @@ def test_planes_only_batch_equals_full(torch_cuda, bd):
         tgts.append(t)
     full = R.BatchSynthesizer(srcs, tgts).run()
     full = [{k: o[k].clone() for k in ("y", "u", "v")} for o in full]
-    part = R.BatchSynthesizer(srcs, tgts, want_color=False).run()
+    part = R.BatchSynthesizer(srcs, tgts, want_color=False, tile=tile).run()
     torch.cuda.synchronize()
     for k in range(len(tgts)):
         assert part[k]["color"] is None
@@ -152,16 +157,17 @@ def test_culled_poses_match_oracle_perspective(torch_cuda, yaw):
     _check_oracle(out, _oracle(rg, planes, tgt), tgt.bitdepth, "rects yaw %s" % yaw)
 
 
-def test_batch_graph_replay(torch_cuda):
+@pytest.mark.parametrize("tile", [16, 32])
+def test_batch_graph_replay(torch_cuda, tile):
     """A batch captured into a HIP graph replays to the eager result, replay
     after replay, with unrelated eager work in between."""
     torch = torch_cuda
-    say("case graph replay")
+    say("case graph replay, tile %d" % tile)
     rg = W.rig("erp4_1080p", scale=4)
     planes = W.render_sources(rg)
     srcs = W.device_sources(rg, planes)
     tgts = W.batch_targets(rg, 4, seed=5, scale=4)
-    bs = R.BatchSynthesizer(srcs, tgts)
+    bs = R.BatchSynthesizer(srcs, tgts, tile=tile)
     eager = [{k: v.clone() for k, v in o.items() if v is not None} for o in bs.run()]
     torch.cuda.synchronize()
     for o in bs.outs:
@@ -259,12 +265,13 @@ def test_perspective_grid_sources(torch_cuda):
     _check_oracle(out, _oracle(rg, planes, tgt), tgt.bitdepth, "persp11")
 
 
-def test_dense_sources_compact_bins(torch_cuda):
+@pytest.mark.parametrize("tile", [16, 32])
+def test_dense_sources_compact_bins(torch_cuda, tile):
     """Sources far denser than the target (eleven 512x512 views into 128x72):
     thousands of quads per (tile, source) bin, exact against the oracle; the
     entry list grows on demand from a deliberately small capacity."""
     torch = torch_cuda
-    say("case dense bins")
+    say("case dense bins, tile %d" % tile)
     rg = W.rig("persp11_2048", scale=4)
     planes = W.render_sources(rg)
     srcs = W.device_sources(rg, planes)
@@ -273,7 +280,7 @@ def test_dense_sources_compact_bins(torch_cuda):
                         t["blending_exp"], R.scale_of(10), R.scale_of(16))
     R.profile_enable(True)
     try:
-        syn = R.Synthesizer(srcs, tgt, entry_cap=4096, want_maps=False)
+        syn = R.Synthesizer(srcs, tgt, entry_cap=4096, want_maps=False, tile=tile)
         out = syn.run()
         torch.cuda.synchronize()
         cnt = R.profile_counters()
@@ -284,12 +291,14 @@ def test_dense_sources_compact_bins(torch_cuda):
     _check_oracle(out, _oracle(rg, planes, tgt), tgt.bitdepth, "dense")
 
 
-def test_binning_beyond_the_kept_range(torch_cuda, monkeypatch):
+@pytest.mark.parametrize("tile", [16, 32])
+def test_binning_beyond_the_kept_range(torch_cuda, monkeypatch, tile):
     """Work items beyond the kept range (here all but one: RVS_KEEP_ITEMS=1)
     are counted and filled by the re-evaluating binning passes, wide tile
-    ranges included: every map equals the default run's (all items kept)."""
+    ranges included: every map equals the default run's (all items kept,
+    16-pixel tiles)."""
     torch = torch_cuda
-    say("case binning beyond the kept range")
+    say("case binning beyond the kept range, tile %d" % tile)
     cases = []
     rg = W.rig("erp4_1080p", scale=4)
     srcs = W.device_sources(rg, W.render_sources(rg))
@@ -308,7 +317,7 @@ def test_binning_beyond_the_kept_range(torch_cuda, monkeypatch):
         monkeypatch.setenv("RVS_KEEP_ITEMS", "1")
         R.profile_enable(True)
         try:
-            outs = R.BatchSynthesizer(s, tg, want_maps=True).run()
+            outs = R.BatchSynthesizer(s, tg, want_maps=True, tile=tile).run()
             torch.cuda.synchronize()
             cnt = R.profile_counters()
         finally:

@@ -392,12 +392,12 @@ def test_encode(torch_cuda, bd):
 
 # ------------------------------------------------------------ fused view
 
-def _synth_case(torch, rg, pos, rot, tag, exact=True):
-    say("case synthesize %s" % tag)
+def _synth_case(torch, rg, pos, rot, tag, exact=True, tile=None):
+    say("case synthesize %s, tile %s" % (tag, tile))
     planes = W.render_sources(rg)
     srcs = W.device_sources(rg, planes)
     tgt = W.device_target(rg, pos, rot)
-    syn = R.Synthesizer(srcs, tgt)
+    syn = R.Synthesizer(srcs, tgt, tile=tile)
     out = syn.run()
     torch.cuda.synchronize()
     t = rg["target"]
@@ -419,12 +419,16 @@ def test_synthesize_rects_640(torch_cuda):
     _synth_case(torch_cuda, W.rig("rects_640"), [0.0, 0.0, 0.0], [0.0, 0.0, 0.0], "rects_640")
 
 
-def test_synthesize_rects_moved(torch_cuda):
-    _synth_case(torch_cuda, W.rig("rects_640", scale=2), [0.03, 0.01, -0.02], [6.0, 2.0, -1.0], "rects moved")
+@pytest.mark.parametrize("tile", [16, 32])
+def test_synthesize_rects_moved(torch_cuda, tile):
+    _synth_case(torch_cuda, W.rig("rects_640", scale=2), [0.03, 0.01, -0.02], [6.0, 2.0, -1.0], "rects moved",
+                tile=tile)
 
 
-def test_synthesize_erp4_quarter(torch_cuda):
-    _synth_case(torch_cuda, W.rig("erp4_1080p", scale=4), [0.0, 0.0, 0.0], [0.0, 0.0, 0.0], "erp4 quarter")
+@pytest.mark.parametrize("tile", [16, 32])
+def test_synthesize_erp4_quarter(torch_cuda, tile):
+    _synth_case(torch_cuda, W.rig("erp4_1080p", scale=4), [0.0, 0.0, 0.0], [0.0, 0.0, 0.0], "erp4 quarter",
+                tile=tile)
 
 
 def test_synthesize_repeatable(torch_cuda):
@@ -439,8 +443,8 @@ def test_synthesize_repeatable(torch_cuda):
         assert torch.equal(a[k], b[k]), k
 
 
-@pytest.mark.parametrize("precision", [2.0, 1.5, 0.5])
-def test_synthesize_precision(torch_cuda, precision):
+@pytest.mark.parametrize("precision,tile", [(2.0, 16), (1.5, 16), (0.5, 16), (1.5, 32)])
+def test_synthesize_precision(torch_cuda, precision, tile):
     """Precision != 1: rasterise, blend and inpaint at (int)(0.5 + size * p)
     (SynthesizedView.cpp:326-331; validity stretch divided by p), then resize
     to the view's size before encoding (image_writing.cpp:189-190).  Maps at
@@ -448,11 +452,12 @@ def test_synthesize_precision(torch_cuda, precision):
     resize rule is the oracle's restatement of OpenCV's, pinned on the CPU);
     the per-stage path gives the same bits."""
     torch = torch_cuda
-    say("case synthesize precision %g" % precision)
+    say("case synthesize precision %g, tile %d" % (precision, tile))
     rg = W.rig("rects_640", scale=2)
     planes = W.render_sources(rg)
     tgt = W.device_target(rg, [0.02, -0.01, 0.01], [3.0, -1.0, 0.5])
     tgt.rescale = precision
+    tgt.tile = tile
     srcs = W.device_sources(rg, planes)
     out = R.Synthesizer(srcs, tgt).run()
     torch.cuda.synchronize()
