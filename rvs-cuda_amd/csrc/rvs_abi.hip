@@ -324,7 +324,7 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
     // the colour map of views whose caller wants only the encoded planes
     P.color = L.take((size_t)px * 3 * sizeof(float));
     plan_raster(L, P.R, n_src, n_views, (uint32_t)tiles, (uint32_t)px, cap, max_sw, max_sh, keep_max, shift);
-    P.inpaint = L.take(rvs_k::inpaint_ws_bytes((size_t)px, n_views));
+    P.inpaint = L.take(rvs_k::inpaint_ws_bytes((size_t)px, n_views, P.max_w, P.max_h));
     // oversampled views: their colour resized to the view's size, for the encoder
     P.resized = L.take((size_t)rpx * 3 * sizeof(float));
     P.total = L.o;
@@ -410,6 +410,7 @@ int enqueue_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int
         d.enc_raster = enc && !rescaled;
         d.partial_color = !o.color && !rescaled;
         d.quality = o.quality ? o.quality : quality + d.px_base;
+        d.partial_quality = !o.quality;
         d.depth = o.depth;
         d.validity = o.validity;
         d.y = enc ? o.y : nullptr; d.u = enc ? o.u : nullptr; d.v = enc ? o.v : nullptr;
@@ -464,8 +465,9 @@ int enqueue_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int
         HIP_TRY(hipStreamWaitEvent(sf, split_ev, 0), "split wait");
         s = sf;
     }
-    rvs_k::InpaintWS iws = rvs_k::inpaint_ws_carve(base + P.inpaint, d.npx, n_views);
+    rvs_k::InpaintWS iws = rvs_k::inpaint_ws_carve(base + P.inpaint, d.npx, n_views, P.max_w, P.max_h);
     iws.fail = ws.status + 3;   // a bounded wait of the inpainting that gave up
+    iws.rflags = ws.tile_enc;   // tiles the raster left without holes or deferred pixels are skipped
     HIP_TRY(rvs_k::launch_inpaint(dviews, n_views, P.max_w, P.max_h, iws, nullptr, s), "inpaint");
     prof_mark(4, s);
     if (P.any_rescaled) HIP_TRY(rvs_k::launch_resize_views(dviews, n_views, P.max_ew, P.max_eh, s), "resize");
@@ -711,7 +713,7 @@ int rvs_blend_step(float* acc_color, float* acc_quality, float* acc_depth, float
 
 size_t rvs_inpaint_workspace_size(int w, int h) {
     if (w < 1 || h < 1) return 0;
-    return align256(sizeof(rvs_k::ViewDesc)) + align256(rvs_k::inpaint_ws_bytes((size_t)w * h, 1));
+    return align256(sizeof(rvs_k::ViewDesc)) + align256(rvs_k::inpaint_ws_bytes((size_t)w * h, 1, w, h));
 }
 
 int rvs_inpaint(float* color, const float* quality, int w, int h, void* workspace, size_t ws_bytes, int32_t* map_out,
@@ -728,7 +730,7 @@ int rvs_inpaint(float* color, const float* quality, int w, int h, void* workspac
     vd.quality = (float*)quality;
     rvs_k::ViewDesc* dv = (rvs_k::ViewDesc*)workspace;
     const rvs_k::InpaintWS ws =
-        rvs_k::inpaint_ws_carve((char*)workspace + align256(sizeof(rvs_k::ViewDesc)), (size_t)w * h, 1);
+        rvs_k::inpaint_ws_carve((char*)workspace + align256(sizeof(rvs_k::ViewDesc)), (size_t)w * h, 1, w, h);
     HIP_TRY(rvs_k::launch_upload(&vd, sizeof vd, dv, s), "view upload");
     HIP_TRY(rvs_k::launch_inpaint(dv, 1, w, h, ws, map_out, s), "inpaint");
     return RVS_OK;

@@ -12,7 +12,8 @@ namespace rvs_k {
 
 constexpr int kInpaintBlocks = 1024;   // persistent component solvers
 constexpr int kInpaintBigBlocks = 256;  // solvers of large components (one 1024-thread block each)
-constexpr int kStripBlocks = 512;       // persistent strip solvers (256 threads each)
+constexpr int kStripBlocks = 512;       // persistent strip solvers (one wave each)
+constexpr int kHoleTileBlocks = 2048;   // persistent blocks of the passes over the hole tiles
 
 // Indexed by global pixel (a view's pixels start at its px_base).
 struct InpaintWS {
@@ -24,7 +25,13 @@ struct InpaintWS {
     int* root;      // root pixel of a component
     int* counters;  // [0] components, [2] work counter of the small-component pass, [3] strip tasks,
                     // [4] / [5] tickets of the forward / backward strip sweeps, [6] components left to
-                    // the one-block solver, [7] strip carry slots in use
+                    // the one-block solver, [7] strip carry slots in use, [8] tiles with holes
+    uint8_t* thole; // [view][tile row][tile column] of 32 x 32 pixel tiles: 1 when the tile has a hole;
+                    // label, dist and src are stored only in such tiles
+    int* htiles;    // the tiles with holes (index into thole), in no particular order
+    int ttx, tpv;   // tiles per row and per view of the thole grid (sized for the largest view)
+    const uint8_t* rflags;  // raster tile flags of the views (bit 1: a hole or a deferred pixel), or
+                            // null; a 32 x 32 tile whose raster tiles all lack bit 1 has no hole
     int4* tasks;    // strip tasks: component, strip | strips << 8 | cols << 16, first carry slot
     unsigned long long* slots[2];  // per (task, line) scan carries of the forward / backward sweep
     int* legacy;    // large components the strip plan could not place
@@ -32,8 +39,8 @@ struct InpaintWS {
     int task_cap, slot_cap;
 };
 
-size_t inpaint_ws_bytes(size_t npx, int nviews);
-InpaintWS inpaint_ws_carve(void* base, size_t npx, int nviews);
+size_t inpaint_ws_bytes(size_t npx, int nviews, int max_w, int max_h);
+InpaintWS inpaint_ws_carve(void* base, size_t npx, int nviews, int max_w, int max_h);
 // Inpaints every view of the batch in place (views[v].color from views[v].quality).
 hipError_t launch_inpaint(const ViewDesc* views, int nviews, int max_w, int max_h, const InpaintWS& ws,
                           int32_t* map_out, hipStream_t s);

@@ -94,7 +94,9 @@ __device__ inline void lunite(int* lab, int a, int b) {
 // the label -1 (their map value (itself, 0) is implied by it and never
 // stored), each hole pixel the map's initial (itself, w + h) (inpaint_color,
 // inpainting.cpp:185-191) and the global index of its tile-local root (the
-// smallest local index of its local component).
+// smallest local index of its local component).  A tile without holes stores
+// nothing but its flag (0); a tile with holes sets its flag and joins the list
+// of hole tiles that the later passes walk.
 __global__ void __launch_bounds__(256) k_ccl_local(const ViewDesc* __restrict__ views, InpaintWS ws) {
     __shared__ int lab[kCclTile * kCclTile];
     const ViewDesc& vd = views[blockIdx.z];
@@ -102,23 +104,51 @@ __global__ void __launch_bounds__(256) k_ccl_local(const ViewDesc* __restrict__ 
     const int tx0 = blockIdx.x * kCclTile, ty0 = blockIdx.y * kCclTile;
     if (tx0 >= w || ty0 >= h) return;
     const int tid = threadIdx.x;
+    const int t = (int)blockIdx.z * ws.tpv + (int)blockIdx.y * ws.ttx + (int)blockIdx.x;
+    if (ws.rflags) {
+        // the raster tiles covering this tile (one, or up to 2 x 2 of 16 x 16)
+        const int sh = vd.tile_shift;
+        const int rx0 = tx0 >> sh, rx1 = (min(tx0 + kCclTile, w) - 1) >> sh;
+        const int ry0 = ty0 >> sh, ry1 = (min(ty0 + kCclTile, h) - 1) >> sh;
+        int f = 0;
+        for (int ry = ry0; ry <= ry1; ++ry)
+            for (int rx = rx0; rx <= rx1; ++rx) f |= ws.rflags[vd.tile_base + (uint32_t)(ry * vd.tiles_x + rx)];
+        if (!(f & 2)) {   // the same in every thread
+            if (tid == 0) ws.thole[t] = 0;
+            return;
+        }
+    }
+    bool any = false;
     for (int k = tid; k < kCclTile * kCclTile; k += 256) {
         const int x = tx0 + (k % kCclTile), y = ty0 + (k / kCclTile);
         bool hole = false;
-        if (x < w && y < h) {
+        // a pixel of a raster tile without holes is known (its quality may not be stored)
+        if (x < w && y < h &&
+            (!ws.rflags ||
+             (ws.rflags[vd.tile_base + (uint32_t)((y >> vd.tile_shift) * vd.tiles_x + (x >> vd.tile_shift))] & 2))) {
             const int p = y * w + x, gp = base + p;
             hole = !(vd.quality[p] > 0.f);   // View::get_inpaint_mask (View.cpp:198-203)
-            // a known pixel's map value is (itself, 0), implied by its label
             if (hole) {
                 ws.dist[gp] = w + h;
                 ws.src[gp] = gp;
-            } else {
-                ws.label[gp] = -1;
             }
         }
         lab[k] = hole ? k : -1;
+        any |= hole;
     }
-    __syncthreads();
+    if (!__syncthreads_or(any)) {   // also publishes lab
+        if (tid == 0) ws.thole[t] = 0;
+        return;
+    }
+    if (tid == 0) {
+        ws.thole[t] = 1;
+        ws.htiles[atomicAdd(&ws.counters[8], 1)] = t;
+    }
+    // a known pixel's map value is (itself, 0), implied by its label
+    for (int k = tid; k < kCclTile * kCclTile; k += 256) {
+        const int x = tx0 + (k % kCclTile), y = ty0 + (k / kCclTile);
+        if (lab[k] < 0 && x < w && y < h) ws.label[base + y * w + x] = -1;
+    }
     for (int k = tid; k < kCclTile * kCclTile; k += 256) {
         if (lab[k] < 0) continue;
         const int lx = k % kCclTile, ly = k / kCclTile;
@@ -134,59 +164,98 @@ __global__ void __launch_bounds__(256) k_ccl_local(const ViewDesc* __restrict__ 
     }
 }
 
-// Unions across tile borders on the tile-local roots.
-__global__ void k_ccl_merge(const ViewDesc* __restrict__ views, InpaintWS ws) {
-    RVS_VIEW_PIXEL(vd, p, gp)
-    if (ws.label[gp] < 0) return;
-    const int x = p % vd.w, y = p / vd.w;
-    if ((x % kCclTile) == kCclTile - 1 && x + 1 < vd.w && ws.label[gp + 1] >= 0) unite(ws.label, gp, gp + 1);
-    if ((y % kCclTile) == kCclTile - 1 && y + 1 < vd.h && ws.label[gp + vd.w] >= 0) unite(ws.label, gp, gp + vd.w);
+// A hole tile of the list: its view and first pixel.
+struct HoleTile {
+    int v, tx0, ty0;
+};
+
+__device__ inline HoleTile hole_tile(const InpaintWS& ws, int i) {
+    const int t = ws.htiles[i];
+    HoleTile H;
+    H.v = t / ws.tpv;
+    const int r = t - H.v * ws.tpv;
+    H.tx0 = (r % ws.ttx) * kCclTile;
+    H.ty0 = (r / ws.ttx) * kCclTile;
+    return H;
 }
 
-__global__ void k_ccl_flatten(const ViewDesc* __restrict__ views, InpaintWS ws) {
-    RVS_VIEW_PIXEL(vd, p, gp)
-    if (ws.label[gp] < 0) return;
-    const int r = find_root(ws.label, gp);
-    ws.label[gp] = r;
-    if (r == gp) {
-        const int id = atomicAdd(&ws.counters[0], 1);
-        ws.cid[gp] = id;
-        ws.comp[4 * id + 0] = INT_MAX;
-        ws.comp[4 * id + 1] = INT_MIN;
-        ws.comp[4 * id + 2] = INT_MAX;
-        ws.comp[4 * id + 3] = INT_MIN;
-        ws.root[id] = gp;
+// Unions across tile borders on the tile-local roots: a hole tile unites its
+// right column and bottom row with the next tile's pixels (a hole pair across
+// a border has a hole tile on both sides, so every such pair is seen once).
+constexpr int kMergeThreads = 2 * kCclTile;
+
+__global__ void __launch_bounds__(kMergeThreads) k_ccl_merge(const ViewDesc* __restrict__ views, InpaintWS ws) {
+    const int nt = ws.counters[8];
+    const int tid = threadIdx.x;
+    const bool right = tid < kCclTile;
+    for (int i = blockIdx.x; i < nt; i += gridDim.x) {
+        const HoleTile H = hole_tile(ws, i);
+        const ViewDesc& vd = views[H.v];
+        const int x = right ? H.tx0 + kCclTile - 1 : H.tx0 + (tid - kCclTile);
+        const int y = right ? H.ty0 + tid : H.ty0 + kCclTile - 1;
+        const int nx = right ? x + 1 : x, ny = right ? y : y + 1;
+        if (nx >= vd.w || ny >= vd.h) continue;
+        const int gp = (int)vd.px_base + y * vd.w + x, gq = (int)vd.px_base + ny * vd.w + nx;
+        const bool nh = ws.thole[H.v * ws.tpv + (ny / kCclTile) * ws.ttx + nx / kCclTile] != 0;
+        if (nh && ws.label[gp] >= 0 && ws.label[gq] >= 0) unite(ws.label, gp, gq);
     }
 }
 
-// Component bounding boxes.  Same-address device atomics are serialised at
-// the memory side, so a large hole must not cost atomics per pixel: each wave
-// reduces, over its kBboxPx pixels per lane, the box of the component of its
-// first hole (usually the whole wave when a large hole covers it), the block
-// merges the waves' boxes per component, and one thread takes four atomics per
-// distinct component of the block -- a few per 4096 pixels.  Holes of other
-// components use plain atomics.
-constexpr int kBboxThreads = 1024;
-constexpr int kBboxPx = 4;   // pixels per thread, kBboxThreads apart
+__global__ void __launch_bounds__(256) k_ccl_flatten(const ViewDesc* __restrict__ views, InpaintWS ws) {
+    const int nt = ws.counters[8];
+    for (int i = blockIdx.x; i < nt; i += gridDim.x) {
+        const HoleTile H = hole_tile(ws, i);
+        const ViewDesc& vd = views[H.v];
+        for (int k = threadIdx.x; k < kCclTile * kCclTile; k += 256) {
+            const int x = H.tx0 + (k % kCclTile), y = H.ty0 + (k / kCclTile);
+            if (x >= vd.w || y >= vd.h) continue;
+            const int gp = (int)vd.px_base + y * vd.w + x;
+            if (ws.label[gp] < 0) continue;
+            const int r = find_root(ws.label, gp);
+            ws.label[gp] = r;
+            if (r == gp) {
+                const int id = atomicAdd(&ws.counters[0], 1);
+                ws.cid[gp] = id;
+                ws.comp[4 * id + 0] = INT_MAX;
+                ws.comp[4 * id + 1] = INT_MIN;
+                ws.comp[4 * id + 2] = INT_MAX;
+                ws.comp[4 * id + 3] = INT_MIN;
+                ws.root[id] = gp;
+            }
+        }
+    }
+}
+
+// Component bounding boxes, a block per hole tile.  Same-address device
+// atomics are serialised at the memory side, so a large hole must not cost
+// atomics per pixel: each wave reduces, over its kBboxPx pixels per lane, the
+// box of the component of its first hole (usually the whole wave when a large
+// hole covers it), the block merges the waves' boxes per component, and one
+// thread takes four atomics per distinct component of the tile.  Holes of
+// other components use plain atomics.
+constexpr int kBboxThreads = 256;
+constexpr int kBboxPx = kCclTile * kCclTile / kBboxThreads;   // pixels per thread, kBboxThreads apart
 
 __global__ void __launch_bounds__(kBboxThreads) k_comp_bbox(const ViewDesc* __restrict__ views, InpaintWS ws) {
     __shared__ int s_lid[kBboxThreads / 64];
     __shared__ int s_box[kBboxThreads / 64][4];
-    const ViewDesc& vd = views[blockIdx.y];
-    const int n = vd.w * vd.h;
+    const int nt = ws.counters[8];
     const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    for (int i = blockIdx.x; i < nt; i += gridDim.x) {
+    const HoleTile H = hole_tile(ws, i);
+    const ViewDesc& vd = views[H.v];
     int lid = -1;   // wave-uniform: component of the wave's first hole
     int x0 = INT_MAX, x1 = INT_MIN, y0 = INT_MAX, y1 = INT_MIN;
     for (int k = 0; k < kBboxPx; ++k) {
-        const int p = blockIdx.x * (kBboxThreads * kBboxPx) + k * kBboxThreads + (int)threadIdx.x;
-        const bool in = p < n;
-        const int gp = (int)vd.px_base + (in ? p : 0);
+        const int l = k * kBboxThreads + (int)threadIdx.x;
+        const int x = H.tx0 + l % kCclTile, y = H.ty0 + l / kCclTile;
+        const bool in = x < vd.w && y < vd.h;
+        const int gp = (int)vd.px_base + (in ? y * vd.w + x : 0);
         const int r = in ? ws.label[gp] : -1;
         const int id = r >= 0 ? ws.cid[r] : -1;
         const uint64_t act = __ballot(id >= 0);
         if (act == 0ull) continue;
         if (lid < 0) lid = __shfl(id, __ffsll((unsigned long long)act) - 1);
-        const int x = p % vd.w, y = p / vd.w;
         if (id == lid) {
             x0 = min(x0, x); x1 = max(x1, x);
             y0 = min(y0, y); y1 = max(y1, y);
@@ -225,6 +294,8 @@ __global__ void __launch_bounds__(kBboxThreads) k_comp_bbox(const ViewDesc* __re
             atomicMin(&ws.comp[4 * id + 2], b2);
             atomicMax(&ws.comp[4 * id + 3], b3);
         }
+    }
+    __syncthreads();   // s_lid, s_box free for the next tile
     }
 }
 
@@ -271,9 +342,22 @@ constexpr int kOther = -1;      // dist of a hole of another component (inside t
 struct Region {
     int x0, y0, bw, bh, w, h, root;
     int base;     // global index of the view's pixel (0, 0)
+    int tb;       // the view's first thole entry
     int* ldist;   // LDS mode
     int* lsrc;
 };
+
+// The hole flag of the tile of view pixel (x, y).
+__device__ inline bool tile_has_hole(const Region& R, const InpaintWS& ws, int x, int y) {
+    return ws.thole[R.tb + (y / kCclTile) * ws.ttx + x / kCclTile] != 0;
+}
+
+// The label of view pixel (x, y) at global index q: -1 (known) in a tile
+// without holes, which stores none; both words are loaded unconditionally.
+__device__ inline int label_at(const Region& R, const InpaintWS& ws, int x, int y, int q) {
+    const int l = ws.label[q];
+    return tile_has_hole(R, ws, x, y) ? l : -1;
+}
 
 template <bool LDS>
 __device__ inline void region_get(const Region& R, const InpaintWS& ws, int x, int y, int& v, int& s) {
@@ -287,7 +371,7 @@ __device__ inline void region_get(const Region& R, const InpaintWS& ws, int x, i
             v = 0; s = p;                     // a known pixel next to the box
         }
     } else {
-        const int lab = ws.label[p];
+        const int lab = label_at(R, ws, x, y, p);
         if (lab < 0) { v = 0; s = p; }
         else if (lab != R.root) { v = kOther; s = -1; }
         else { v = ws.dist[p]; s = ws.src[p]; }
@@ -465,7 +549,7 @@ __device__ void solve_big(const Region& R, const InpaintWS& ws, Fn* wave_tot, in
                         ok[k][n] = nx >= 0 && nx < R.w && ny >= 0 && ny < R.h;
                         const int q = R.base + (ok[k][n] ? ny * R.w + nx : y * R.w + x);
                         pp[k][n] = q;
-                        lab[k][n] = ws.label[q];
+                        lab[k][n] = label_at(R, ws, ok[k][n] ? nx : x, ok[k][n] ? ny : y, q);
                         dd[k][n] = ws.dist[q];
                         ss[k][n] = ws.src[q];
                     }
@@ -583,8 +667,9 @@ __device__ void solve_lines(const Region& R, const InpaintWS& ws, bool cols, Fn*
         };
         auto inside = [&](int x, int y) { return x >= 0 && x < R.w && y >= 0 && y < R.h; };
         auto load = [&](int x, int y, int& v, int& sr) {
-            const int q = R.base + (inside(x, y) ? y * R.w + x : 0);
-            map_value(R, ws.label[q], ws.dist[q], ws.src[q], q, v, sr);
+            const bool in = inside(x, y);
+            const int q = R.base + (in ? y * R.w + x : 0);
+            map_value(R, label_at(R, ws, in ? x : 0, in ? y : 0, q), ws.dist[q], ws.src[q], q, v, sr);
         };
         bool changed = false;
         // line coordinate of processing step li (-1 / nlines: outside the box)
@@ -789,7 +874,7 @@ __device__ inline Fn wave_scan_fn(Fn A, Fn& ex) {
 constexpr int kRing = 4;
 
 struct RawPx {
-    int lab, d, s;
+    int lab, d, s, th;   // th: the hole flag of the pixel's tile (else lab is not stored)
 };
 
 template <bool FIRST>
@@ -807,25 +892,28 @@ __device__ void solve_strip(const Region& R, const InpaintWS& ws, bool cols, boo
     const int r_along = cols ? (fwd ? 3 : 2) : (fwd ? 4 : 1);
     auto lcoord = [&](int li) { return fwd ? li : nlines - 1 - li; };
     // view index of (line coordinate lc, scan index e); false outside the image
-    auto addr = [&](int lc, int e, int& q) {
+    // (ti: its thole entry)
+    auto addr = [&](int lc, int e, int& q, int& ti) {
         const int pos = fwd ? e : llen - 1 - e;
         const int x = cols ? R.x0 + lc : R.x0 + pos;
         const int y = cols ? R.y0 + pos : R.y0 + lc;
         const bool in = x >= 0 && x < R.w && y >= 0 && y < R.h;
         q = R.base + (in ? y * R.w + x : 0);
+        ti = R.tb + (in ? (y / kCclTile) * ws.ttx + x / kCclTile : 0);
         return in;
     };
     auto fetch = [&](int lc, int e, RawPx& r) {
-        int q;
-        addr(lc, e, q);
+        int q, ti;
+        addr(lc, e, q, ti);
         r.lab = ws.label[q];
         r.d = ws.dist[q];
         r.s = ws.src[q];
+        r.th = ws.thole[ti];
     };
     auto value = [&](int lc, int e, const RawPx& r, int& v, int& sr) {
-        int q;
-        const bool in = addr(lc, e, q);
-        map_value(R, r.lab, r.d, r.s, q, v, sr);
+        int q, ti;
+        const bool in = addr(lc, e, q, ti);
+        map_value(R, r.th ? r.lab : -1, r.d, r.s, q, v, sr);
         return in;
     };
     // ring: this lane's pixels, the along-line neighbour after them, (first
@@ -923,8 +1011,8 @@ __device__ void solve_strip(const Region& R, const InpaintWS& ws, bool cols, boo
             const int e = me + j;
             const bool member = e < e1 && sv[j] != 0 && sv[j] != kOther;
             if (member && Sv != sv[j]) {
-                int q;
-                addr(lc, e, q);
+                int q, ti;
+                addr(lc, e, q, ti);
                 ws.dist[q] = Sv;
                 ws.src[q] = Ss;
             }
@@ -1029,8 +1117,10 @@ __global__ void __launch_bounds__(kStripThreads) k_inp_strips(const ViewDesc* __
         const bool cols = (task.y >> 16) & 1;
         Region R;
         R.root = ws.root[c];
-        const ViewDesc& vd = views[view_of_pixel(views, nviews, R.root)];
+        const int vi = view_of_pixel(views, nviews, R.root);
+        const ViewDesc& vd = views[vi];
         R.base = (int)vd.px_base;
+        R.tb = vi * ws.tpv;
         R.x0 = ws.comp[4 * c + 0];
         R.y0 = ws.comp[4 * c + 2];
         R.bw = ws.comp[4 * c + 1] - R.x0 + 1;
@@ -1089,8 +1179,10 @@ __global__ void __launch_bounds__(256) k_inp_solve(const ViewDesc* __restrict__ 
         }
         Region R;
         R.root = ws.root[c];
-        const ViewDesc& vd = views[view_of_pixel(views, nviews, R.root)];
+        const int vi = view_of_pixel(views, nviews, R.root);
+        const ViewDesc& vd = views[vi];
         R.base = (int)vd.px_base;
+        R.tb = vi * ws.tpv;
         R.x0 = ws.comp[4 * c + 0];
         R.y0 = ws.comp[4 * c + 2];
         R.bw = ws.comp[4 * c + 1] - R.x0 + 1;
@@ -1102,7 +1194,7 @@ __global__ void __launch_bounds__(256) k_inp_solve(const ViewDesc* __restrict__ 
             for (int k = tid; k < area; k += 256) {
                 const int x = R.x0 + k % R.bw, y = R.y0 + k / R.bw;
                 const int p = R.base + y * R.w + x;
-                const int lab = ws.label[p];
+                const int lab = label_at(R, ws, x, y, p);
                 ldist[k] = lab < 0 ? 0 : (lab == R.root ? ws.dist[p] : kOther);
                 lsrc[k] = lab < 0 ? p : (lab == R.root ? ws.src[p] : -1);
             }
@@ -1112,7 +1204,7 @@ __global__ void __launch_bounds__(256) k_inp_solve(const ViewDesc* __restrict__ 
             for (int k = tid; k < area; k += 256) {
                 const int x = R.x0 + k % R.bw, y = R.y0 + k / R.bw;
                 const int p = R.base + y * R.w + x;
-                if (ws.label[p] == R.root) { ws.dist[p] = ldist[k]; ws.src[p] = lsrc[k]; }
+                if (label_at(R, ws, x, y, p) == R.root) { ws.dist[p] = ldist[k]; ws.src[p] = lsrc[k]; }
             }
         } else {
             solve_big(R, ws, wave_tot, s_carry);
@@ -1134,8 +1226,10 @@ __global__ void __launch_bounds__(kBigThreads) k_inp_big(const ViewDesc* __restr
         const int bw = ws.comp[4 * c + 1] - ws.comp[4 * c + 0] + 1, bh = ws.comp[4 * c + 3] - ws.comp[4 * c + 2] + 1;
         Region R;
         R.root = ws.root[c];
-        const ViewDesc& vd = views[view_of_pixel(views, nviews, R.root)];
+        const int vi = view_of_pixel(views, nviews, R.root);
+        const ViewDesc& vd = views[vi];
         R.base = (int)vd.px_base;
+        R.tb = vi * ws.tpv;
         R.x0 = ws.comp[4 * c + 0];
         R.y0 = ws.comp[4 * c + 2];
         R.bw = bw; R.bh = bh;
@@ -1148,23 +1242,39 @@ __global__ void __launch_bounds__(kBigThreads) k_inp_big(const ViewDesc* __restr
     }
 }
 
-// Copies the chosen known pixel's colour into every hole; map_out (single-view
-// launches only) receives source x, source y and distance per pixel.
-__global__ void k_inp_apply(const ViewDesc* __restrict__ views, InpaintWS ws, int32_t* map_out) {
+// Copies the chosen known pixel's colour into every hole of the hole tiles.
+__global__ void __launch_bounds__(256) k_inp_apply(const ViewDesc* __restrict__ views, InpaintWS ws) {
+    const int nt = ws.counters[8];
+    for (int i = blockIdx.x; i < nt; i += gridDim.x) {
+        const HoleTile H = hole_tile(ws, i);
+        const ViewDesc& vd = views[H.v];
+        for (int k = threadIdx.x; k < kCclTile * kCclTile; k += 256) {
+            const int x = H.tx0 + (k % kCclTile), y = H.ty0 + (k / kCclTile);
+            if (x >= vd.w || y >= vd.h) continue;
+            const int p = y * vd.w + x, gp = (int)vd.px_base + p;
+            if (ws.label[gp] < 0) continue;
+            const int s = ws.src[gp] - (int)vd.px_base;
+            if (s != p) {
+                float* color = vd.color;
+                color[3 * p] = color[3 * s];
+                color[3 * p + 1] = color[3 * s + 1];
+                color[3 * p + 2] = color[3 * s + 2];
+            }
+        }
+    }
+}
+
+// The inpainting map of a single-view launch: source x, source y and
+// distance per pixel (a known pixel: itself, 0).
+__global__ void k_inp_map(const ViewDesc* __restrict__ views, InpaintWS ws, int32_t* map_out) {
     RVS_VIEW_PIXEL(vd, p, gp)
-    const bool hole = ws.label[gp] >= 0;
+    const int x = p % vd.w, y = p / vd.w;
+    const bool th = ws.thole[blockIdx.y * ws.tpv + (y / kCclTile) * ws.ttx + x / kCclTile] != 0;
+    const bool hole = th && ws.label[gp] >= 0;
     const int s = hole ? ws.src[gp] - (int)vd.px_base : p;
-    if (map_out) {
-        map_out[3 * p] = s % vd.w;
-        map_out[3 * p + 1] = s / vd.w;
-        map_out[3 * p + 2] = hole ? ws.dist[gp] : 0;
-    }
-    if (hole && s != p) {
-        float* color = vd.color;
-        color[3 * p] = color[3 * s];
-        color[3 * p + 1] = color[3 * s + 1];
-        color[3 * p + 2] = color[3 * s + 2];
-    }
+    map_out[3 * p] = s % vd.w;
+    map_out[3 * p + 1] = s / vd.w;
+    map_out[3 * p + 2] = hole ? ws.dist[gp] : 0;
 }
 
 namespace {
@@ -1176,21 +1286,30 @@ size_t task_cap_of(size_t npx, int nviews) { return npx / 64 + 16 * (size_t)nvie
 size_t slot_cap_of(size_t npx, int nviews) { return npx / 16 + 4096 * (size_t)nviews; }
 size_t legacy_cap_of(size_t npx, int nviews) { return npx / 128 + 2 * (size_t)nviews + 1; }
 size_t a256(size_t b) { return (b + 255) & ~(size_t)255; }
+int tiles_of(int n) { return (n + kCclTile - 1) / kCclTile; }
+size_t grid_tiles(int nviews, int max_w, int max_h) { return (size_t)nviews * tiles_of(max_w) * tiles_of(max_h); }
 }  // namespace
 
-size_t inpaint_ws_bytes(size_t npx, int nviews) {
+size_t inpaint_ws_bytes(size_t npx, int nviews, int max_w, int max_h) {
     const size_t maxc = npx / 2 + (size_t)nviews + 1;
+    const size_t nt = grid_tiles(nviews, max_w, max_h);
     return a256(task_cap_of(npx, nviews) * sizeof(int4)) +
            2 * a256(slot_cap_of(npx, nviews) * sizeof(unsigned long long)) +
-           a256(4 * legacy_cap_of(npx, nviews)) + 4 * (4 * npx) + 4 * (4 * maxc) + 4 * maxc + 64;
+           a256(4 * legacy_cap_of(npx, nviews)) + a256(nt) + a256(4 * nt) + 4 * (4 * npx) + 4 * (4 * maxc) +
+           4 * maxc + 64;
 }
 
-InpaintWS inpaint_ws_carve(void* base, size_t npx, int nviews) {
+InpaintWS inpaint_ws_carve(void* base, size_t npx, int nviews, int max_w, int max_h) {
     const size_t maxc = npx / 2 + (size_t)nviews + 1;
+    const size_t nt = grid_tiles(nviews, max_w, max_h);
     char* b = (char*)base;   // 256-byte aligned
     InpaintWS ws;
     ws.task_cap = (int)task_cap_of(npx, nviews);
     ws.slot_cap = (int)slot_cap_of(npx, nviews);
+    ws.ttx = tiles_of(max_w);
+    ws.tpv = ws.ttx * tiles_of(max_h);
+    ws.thole = (uint8_t*)b; b += a256(nt);
+    ws.htiles = (int*)b; b += a256(4 * nt);
     ws.tasks = (int4*)b; b += a256(task_cap_of(npx, nviews) * sizeof(int4));
     ws.slots[0] = (unsigned long long*)b; b += a256(slot_cap_of(npx, nviews) * sizeof(unsigned long long));
     ws.slots[1] = (unsigned long long*)b; b += a256(slot_cap_of(npx, nviews) * sizeof(unsigned long long));
@@ -1204,29 +1323,32 @@ InpaintWS inpaint_ws_carve(void* base, size_t npx, int nviews) {
     ws.root = p; p += maxc;
     ws.counters = p;
     ws.fail = nullptr;
+    ws.rflags = nullptr;
     return ws;
 }
 
 hipError_t launch_inpaint(const ViewDesc* views, int nviews, int max_w, int max_h, const InpaintWS& ws,
                           int32_t* map_out, hipStream_t s) {
-    const dim3 b(256), g((unsigned)(((size_t)max_w * max_h + 255) / 256), (unsigned)nviews);
-    hipError_t e = launch_reset_u32((uint32_t*)ws.counters, 8, s);
+    const dim3 b(256);
+    hipError_t e = launch_reset_u32((uint32_t*)ws.counters, 16, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_ccl_local,
-                       dim3((max_w + kCclTile - 1) / kCclTile, (max_h + kCclTile - 1) / kCclTile, nviews), b, 0, s,
-                       views, ws);
-    hipLaunchKernelGGL(k_ccl_merge, g, b, 0, s, views, ws);
-    hipLaunchKernelGGL(k_ccl_flatten, g, b, 0, s, views, ws);
-    const size_t per_block = (size_t)kBboxThreads * kBboxPx;
-    hipLaunchKernelGGL(k_comp_bbox, dim3((unsigned)(((size_t)max_w * max_h + per_block - 1) / per_block), (unsigned)nviews),
-                       dim3(kBboxThreads), 0, s, views, ws);
+    hipLaunchKernelGGL(k_ccl_local, dim3(tiles_of(max_w), tiles_of(max_h), nviews), b, 0, s, views, ws);
+    // the passes over the hole tiles: persistent blocks striding over the list
+    const size_t nt = grid_tiles(nviews, max_w, max_h);
+    const dim3 gt((unsigned)(nt < (size_t)kHoleTileBlocks ? nt : (size_t)kHoleTileBlocks));
+    hipLaunchKernelGGL(k_ccl_merge, gt, dim3(kMergeThreads), 0, s, views, ws);
+    hipLaunchKernelGGL(k_ccl_flatten, gt, b, 0, s, views, ws);
+    hipLaunchKernelGGL(k_comp_bbox, gt, dim3(kBboxThreads), 0, s, views, ws);
     hipLaunchKernelGGL(k_inp_plan, dim3(1), dim3(1024), 0, s, ws);
     hipLaunchKernelGGL(k_inp_slot_reset, dim3(256), dim3(256), 0, s, ws);
     hipLaunchKernelGGL(k_inp_strips<true>, dim3(kStripBlocks), dim3(kStripThreads), 0, s, views, nviews, ws);
     hipLaunchKernelGGL(k_inp_strips<false>, dim3(kStripBlocks), dim3(kStripThreads), 0, s, views, nviews, ws);
     hipLaunchKernelGGL(k_inp_big, dim3(kInpaintBigBlocks), dim3(kBigThreads), 0, s, views, nviews, ws);
     hipLaunchKernelGGL(k_inp_solve, dim3(kInpaintBlocks), b, 0, s, views, nviews, ws);
-    hipLaunchKernelGGL(k_inp_apply, g, b, 0, s, views, ws, map_out);
+    hipLaunchKernelGGL(k_inp_apply, gt, b, 0, s, views, ws);
+    if (map_out)
+        hipLaunchKernelGGL(k_inp_map, dim3((unsigned)(((size_t)max_w * max_h + 255) / 256), (unsigned)nviews), b, 0, s,
+                           views, ws, map_out);
     return hipGetLastError();
 }
 

@@ -28,6 +28,7 @@ constexpr int kFallbackSortCap = 8192;  // longest (source, tile) list the repla
 constexpr int kMaxViews = 4096;
 constexpr int kBinVerts = (kBinQY + 1) * (kBinQX + 1);   // vertices of one binning work item
 constexpr int kBinBlocks = 2048;    // persistent binning blocks (work items past the kept range)
+constexpr int kFlatBlocks = 16384;  // persistent blocks of the passes over the kept work items
 constexpr int kRasterGroup = 4;     // sources rasterised together per k_raster phase
 constexpr int kRasterCache = 1024;  // binned quads per phase whose coverage-pass results stay in LDS
 constexpr int kRasterCacheOne = 640; // the same for launch sets of up to kRasterGroup sources (one phase)
@@ -56,6 +57,8 @@ struct ViewDesc {
                              // pixels are encoded by k_raster itself
     int partial_color;       // color is internal: tiles encoded by k_raster write it only on
                              // their border (all an inpainting next door can read)
+    int partial_quality;     // quality is internal: written only in tiles with a hole or a
+                             // deferred pixel (the only ones the inpainting reads)
 };
 
 // Device-side state of one launch set.
@@ -72,7 +75,8 @@ struct RasterWS {
     uint8_t* deferred;       // [global pixel] pixel handled by the in-order replay
     uint32_t* deferred_tiles;// [global tile]
     uint32_t* work;          // binning work items (view, source, block)
-    uint8_t* tile_enc;       // [global tile] 1: encoded by k_raster (written by every tile)
+    uint8_t* tile_enc;       // [global tile] bit 0: encoded by k_raster; bit 1: a hole or a deferred
+                             // pixel (blending launches); written by every tile
     uint32_t* status;        // [0] n_deferred, [1] list overflow, [2] entries (saturated), [3] inpainting
                              // wait gave up, [4] work items
 };

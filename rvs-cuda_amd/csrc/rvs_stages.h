@@ -24,7 +24,7 @@ hipError_t launch_encode(const float* color, int w, int h, int bitdepth, void* y
                          hipStream_t s);
 // Encodes every view of a batch that has output planes (ViewDesc y, u, v) from
 // ecol at the views' own sizes (max_w x max_h: the largest ew x eh); blocks of
-// tiles that tile_enc (may be null) marks as encoded by k_raster are skipped.
+// tiles that tile_enc (may be null) marks as encoded by k_raster (bit 0) are skipped.
 hipError_t launch_encode_views(const ViewDesc* views, int nviews, int max_w, int max_h, const uint8_t* tile_enc,
                                hipStream_t s);
 // write_color's resize (image_writing.cpp:189-190) of one float3 image.

@@ -118,7 +118,7 @@ __global__ void k_encode_views(const ViewDesc* __restrict__ views, const uint8_t
     if (!vd.y || bx >= vd.ew / 2 || by >= vd.eh / 2) return;
     // tiles without holes or deferred pixels were encoded by the raster pass
     if (tile_enc && vd.enc_raster &&
-        tile_enc[vd.tile_base + (uint32_t)(((2 * by) >> vd.tile_shift) * vd.tiles_x + ((2 * bx) >> vd.tile_shift))])
+        (tile_enc[vd.tile_base + (uint32_t)(((2 * by) >> vd.tile_shift) * vd.tiles_x + ((2 * bx) >> vd.tile_shift))] & 1))
         return;
     const unsigned maxv = (1u << vd.bitdepth) - 1u;
     if (vd.bitdepth <= 8)

@@ -778,10 +778,11 @@ hipError_t launch_bin(const G* geoms, const ViewDesc* views, const BatchDims& d,
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t blocks = total < (size_t)kBinBlocks ? total : (size_t)kBinBlocks;
-    // kept items: up to one block each, at most max(total / 8, 65536) blocks
-    // (blocks past the surviving items return at once; a few per item are
-    // cheaper than many idle blocks when most items are culled)
-    const size_t fb = total / 8 > 65536 ? total / 8 : 65536;
+    // kept items: persistent blocks striding over them (blocks past the
+    // surviving items return at once).  A block's set-up, including the few
+    // registers these kernels spill at occupancy 8, is paid once per block, so
+    // a few items per block beat a block per item.
+    const size_t fb = kFlatBlocks;
     const unsigned flat = (unsigned)(d.range_cap < fb ? d.range_cap : fb);
     if (d.range_cap)
         hipLaunchKernelGGL((k_bin<G, false, true>), dim3(flat), dim3(256), 0, s, geoms, views, d, ws);
@@ -1003,7 +1004,7 @@ __device__ __forceinline__ void raster_quad(const G& g, uint32_t ij, uint32_t e,
 
 template <int MODE>
 __device__ inline void write_pixel(const ViewDesc& vd, size_t o, const BlendState& st, float3 rc, float rd, float rs,
-                                   bool color = true) {
+                                   bool color = true, bool quality = true) {
     if (MODE == MODE_RAW) {
         vd.color[3 * o] = rc.x; vd.color[3 * o + 1] = rc.y; vd.color[3 * o + 2] = rc.z;
         vd.depth[o] = rd;
@@ -1011,7 +1012,7 @@ __device__ inline void write_pixel(const ViewDesc& vd, size_t o, const BlendStat
     } else {
         if (color) { vd.color[3 * o] = st.c.x; vd.color[3 * o + 1] = st.c.y; vd.color[3 * o + 2] = st.c.z; }
         if (vd.depth) vd.depth[o] = st.d;
-        vd.quality[o] = st.q;
+        if (quality) vd.quality[o] = st.q;
         if (vd.validity) vd.validity[o] = st.v;
     }
 }
@@ -1057,7 +1058,7 @@ __device__ inline void encode_tile_pixel(const ViewDesc& vd, float3 c, int px, i
 // TILE: the tile edge; a block has a thread per pixel (256 or 1024 threads),
 // and the caches scale with the tile's area.
 template <class G, int MODE, bool MULTI, int TILE>
-__global__ void __launch_bounds__(TILE * TILE) __attribute__((amdgpu_waves_per_eu(MULTI ? 6 : 8))) k_raster(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
+__global__ void __launch_bounds__(TILE * TILE) __attribute__((amdgpu_waves_per_eu(MULTI ? (TILE > kTile ? 4 : 6) : 8))) k_raster(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
                                                 BatchDims d, RasterWS ws) {
     constexpr int NT = TILE * TILE;
     constexpr int AREA = NT / (kTile * kTile);
@@ -1186,10 +1187,11 @@ __global__ void __launch_bounds__(TILE * TILE) __attribute__((amdgpu_waves_per_e
         // an internal colour map of a tile encoded here is read only on the
         // tile's border (by the inpainting of a neighbouring tile)
         const bool border = lx == 0 || ly == 0 || lx == TILE - 1 || ly == TILE - 1;
-        write_pixel<MODE>(vd, o, st, rc, rd, rs, !(enc && vd.partial_color && !border));
+        write_pixel<MODE>(vd, o, st, rc, rd, rs, !(enc && vd.partial_color && !border),
+                          !(vd.partial_quality && !s_hole && !s_deferred));
     }
     if (enc) encode_tile_pixel<TILE>(vd, st.c, px, py, inside, lx, ly);
-    if (t3 == 0) ws.tile_enc[T] = enc ? 1 : 0;
+    if (t3 == 0) ws.tile_enc[T] = (enc ? 1 : 0) | (MODE == MODE_BLEND && (s_hole || s_deferred) ? 2 : 0);
     // the replay reads the per-pixel flags of the listed tiles only
     if (s_deferred) {
         if (inside) ws.deferred[vd.px_base + o] = deferred ? 1 : 0;

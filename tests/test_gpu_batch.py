@@ -122,6 +122,31 @@ def test_planes_only_batch_equals_full(torch_cuda, bd, tile):
         _check_oracle(part[k], _oracle(rg, planes, tgts[k]), bd, "planes-only view %d" % k)
 
 
+@pytest.mark.parametrize("tile", [16, 32])
+def test_internal_maps_ignore_stale_workspace(torch_cuda, tile):
+    """Internal maps (quality, colour of planes-only views) are stored only
+    where the later passes read them: a workspace full of NaN (every unread
+    quality a hole) gives the same colour and planes as a batch that outputs
+    every map."""
+    torch = torch_cuda
+    say("case stale workspace, tile %d" % tile)
+    rg = W.rig("rects_640", scale=2)
+    planes = W.render_sources(rg)
+    srcs = W.device_sources(rg, planes)
+    tgts = [W.device_target(rg, pos, rot)
+            for pos, rot in W.random_walk(4, seed=21, pos_lim=0.05, yaw_lim=8.0, pitch_lim=4.0)]
+    full = [{k: v.clone() for k, v in o.items() if v is not None}
+            for o in R.BatchSynthesizer(srcs, tgts, want_maps=True, tile=tile).run()]
+    for want_color in (True, False):
+        bs = R.BatchSynthesizer(srcs, tgts, want_color=want_color, tile=tile)
+        bs.ws.fill_(0xFF)
+        outs = bs.run()
+        torch.cuda.synchronize()
+        for k in range(len(tgts)):
+            for key in (MAPS if want_color else ("y", "u", "v")):
+                assert torch.equal(outs[k][key], full[k][key]), "colour %s view %d map %s" % (want_color, k, key)
+
+
 @pytest.mark.parametrize("rot", [[90.0, 0.0, 0.0], [180.0, 0.0, 0.0], [-60.0, 40.0, 0.0], [30.0, -80.0, 10.0]])
 def test_culled_poses_match_oracle_erp(torch_cuda, rot):
     """Targets that see little of each ERP source: binning work items are culled."""
