@@ -91,7 +91,9 @@ def log(*a):
 
 
 ROOFLINE_VIEWS = 8
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_raster_erp4_1080p_8views.md")
+def traffic_profile(tile):
+    """The committed counter summary of the roofline launch set at this raster tile edge."""
+    return os.path.join(ROOT, "profiles", "k_raster_erp4_1080p_8views%s.md" % ("" if tile == 16 else "_%d" % tile))
 
 
 def roofline_walk(W, rg, n=ROOFLINE_VIEWS):
@@ -101,17 +103,21 @@ def roofline_walk(W, rg, n=ROOFLINE_VIEWS):
     return W.batch_targets(rg, n, seed=4, sizes=((t.w, t.h),))
 
 
-def committed_traffic():
-    """HBM bytes per k_raster launch of the roofline launch set, from the
-    committed counter summary (rocprofv3 FETCH_SIZE and WRITE_SIZE passes,
-    scaled by the same counters on a copy of known size), or None."""
+def committed_profile(tile):
+    """Figures of the roofline launch set from the committed counter summary:
+    traffic_bytes_per_launch (k_raster's HBM bytes: rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes, scaled by the same counters on a copy of known size),
+    valu_insts_per_launch (SQ_INSTS_VALU) and launch_set_traffic_bytes (every
+    kernel of the launch set); missing ones are absent."""
+    out = {}
     try:
-        for line in open(TRAFFIC_PROFILE):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return int(line.split(":")[1])
-    except OSError:
+        for line in open(traffic_profile(tile)):
+            key, sep, val = line.partition(":")
+            if sep and key in ("traffic_bytes_per_launch", "valu_insts_per_launch", "launch_set_traffic_bytes"):
+                out[key] = int(val)
+    except (OSError, ValueError):
         pass
-    return None
+    return out
 
 
 def stage_profile(run, reps, R):
@@ -218,12 +224,15 @@ def main():
         return
 
     # ------------------------------------- raster tile edge (same bits either way)
+    # (batches of views and the plain one-view launch set are tuned apart: a
+    # lone view pays the replay of its larger tiles without the batch's gains)
     if args.tile:
-        tile, tile_ms = args.tile, None
+        tile, tile_ms, plain_tile, plain_tile_ms = args.tile, None, args.tile, None
     else:
         tile, tile_ms = A.tune_tile(srcs, roofline_walk(W, rg))
-    tgt.tile = tile
-    log("raster tile %d (%s)" % (tile, tile_ms))
+        plain_tile, plain_tile_ms = A.tune_tile(srcs, [tgt])
+    tgt.tile = plain_tile
+    log("raster tile %d (%s), plain %d (%s)" % (tile, tile_ms, plain_tile, plain_tile_ms))
 
     # ---------------------------------------------------------- plain per-view path
     syn = R.Synthesizer(srcs, tgt, want_maps=False, want_color=False)
@@ -296,7 +305,8 @@ def main():
     bstage, bcount = stage_profile(bs.run, args.profile_reps, R)
     balg = b_alg(rg, rwalk)
     achieved = balg / (bstage["raster"] * 1e-3) / 1e9
-    traffic = committed_traffic()
+    prof = committed_profile(tile)
+    traffic = prof.get("traffic_bytes_per_launch")
     del bs
 
     batched = None
@@ -493,6 +503,7 @@ def main():
             "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
             "stage_ms": {k: round(v, 4) for k, v in plain_stage.items()},
             "counters": plain_counters,
+            "raster_tile": plain_tile,
         },
         "plain_stagewise": {
             "views_s": round(stagewise_views_s, 2),
@@ -503,7 +514,7 @@ def main():
         "batched_mixed_graph": c3,
         "serving_mix": c4,
         "serving_scenes_8k": c5,
-        "autotune": {"trials": trials, "tile_ms": tile_ms},
+        "autotune": {"trials": trials, "tile_ms": tile_ms, "plain_tile_ms": plain_tile_ms},
         "roofline": {
             "kernel": "k_raster (tile raster + blend), one launch over %d views" % ROOFLINE_VIEWS,
             "bound": "hbm",
@@ -516,7 +527,16 @@ def main():
             "launch_set_stage_ms": {k: round(v, 4) for k, v in bstage.items()},
             "counters": bcount,
             "traffic": traffic,
-            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic is not None else None,
+            "traffic_source": os.path.relpath(traffic_profile(tile), ROOT) if traffic is not None else None,
+            "raster_tile": tile,
+            # the kernel is priced against HBM (it has no matrix work), but what
+            # limits it is VALU instruction issue: its VALU wave-instructions
+            # (committed counter pass) over this run's kernel time
+            "limiter": "valu-issue",
+            "valu_insts_per_launch": prof.get("valu_insts_per_launch"),
+            "valu_ginst_per_s": (round(prof["valu_insts_per_launch"] / (bstage["raster"] * 1e-3) / 1e9, 1)
+                                 if "valu_insts_per_launch" in prof else None),
+            "launch_set_traffic_bytes": prof.get("launch_set_traffic_bytes"),
         },
     }
 

@@ -144,10 +144,16 @@ uint32_t entry_capacity(uint32_t requested, uint64_t npx, int nsrc) {
 }
 
 void plan_raster(Layout& L, RasterPlan& R, int nsrc, int nviews, uint32_t ntiles, uint32_t npx, uint32_t cap,
-                 int max_sw, int max_sh, uint64_t keep_max = 0, int shift = 4) {
+                 int max_sw, int max_sh, uint64_t keep_max = 0, int shift = 4, bool uneven_items = true) {
     rvs_k::BatchDims& d = R.d;
     d.nviews = nviews; d.nsrc = nsrc; d.ntiles = ntiles; d.npx = npx;
     d.tile_shift = shift;
+    // blocks of the passes over the kept items: items of equirectangular
+    // targets differ widely in cost (those near a pole deal thousands of
+    // tiles) and want many short-lived blocks to even out the tail; culled
+    // perspective items are alike, and fewer blocks each taking several pay
+    // the block set-up (and its spilled registers) fewer times
+    d.flat_blocks = uneven_items ? rvs_k::kFlatBlocks : rvs_k::kFlatBlocks / 4;
     d.cap = entry_capacity(cap, npx, nsrc);
     d.max_sw = max_sw; d.max_sh = max_sh;
     d.bins_x = (uint32_t)((max_sw - 1 + rvs_k::kBinQX - 1) / rvs_k::kBinQX);
@@ -252,7 +258,7 @@ struct SynthPlan {
     int tile_shift;       // log2 of the launch set's tile edge
     int max_w, max_h;     // largest raster (oversampled by Precision)
     int max_ew, max_eh;   // largest view
-    bool any_persp, any_rescaled;
+    bool any_persp, any_erp, any_rescaled;
 };
 
 // Raster size of a view with Precision `rescale` (SynthesizedView.cpp:326-329).
@@ -272,7 +278,8 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
         max_sh = srcs[k].cam.h > max_sh ? srcs[k].cam.h : max_sh;
     }
     uint64_t tiles = 0, px = 0, rpx = 0;
-    P.max_w = 0; P.max_h = 0; P.max_ew = 0; P.max_eh = 0; P.any_persp = false; P.any_rescaled = false;
+    P.max_w = 0; P.max_h = 0; P.max_ew = 0; P.max_eh = 0; P.any_persp = false; P.any_erp = false;
+    P.any_rescaled = false;
     // the launch set's tile edge: the largest any view asks for
     int shift = 4;
     for (int v = 0; v < n_views; ++v) {
@@ -299,6 +306,7 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
         P.max_ew = t.cam.w > P.max_ew ? t.cam.w : P.max_ew;
         P.max_eh = t.cam.h > P.max_eh ? t.cam.h : P.max_eh;
         P.any_persp = P.any_persp || t.cam.proj == 0;
+        P.any_erp = P.any_erp || t.cam.proj != 0;
         if (rw != t.cam.w || rh != t.cam.h) {
             P.any_rescaled = true;
             rpx += (uint64_t)t.cam.w * t.cam.h;
@@ -323,7 +331,8 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
     P.quality = L.take((size_t)px * sizeof(float));
     // the colour map of views whose caller wants only the encoded planes
     P.color = L.take((size_t)px * 3 * sizeof(float));
-    plan_raster(L, P.R, n_src, n_views, (uint32_t)tiles, (uint32_t)px, cap, max_sw, max_sh, keep_max, shift);
+    plan_raster(L, P.R, n_src, n_views, (uint32_t)tiles, (uint32_t)px, cap, max_sw, max_sh, keep_max, shift,
+                P.any_erp);
     P.inpaint = L.take(rvs_k::inpaint_ws_bytes((size_t)px, n_views, P.max_w, P.max_h));
     // oversampled views: their colour resized to the view's size, for the encoder
     P.resized = L.take((size_t)rpx * 3 * sizeof(float));

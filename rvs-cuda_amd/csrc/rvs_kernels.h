@@ -28,7 +28,7 @@ constexpr int kFallbackSortCap = 8192;  // longest (source, tile) list the repla
 constexpr int kMaxViews = 4096;
 constexpr int kBinVerts = (kBinQY + 1) * (kBinQX + 1);   // vertices of one binning work item
 constexpr int kBinBlocks = 2048;    // persistent binning blocks (work items past the kept range)
-constexpr int kFlatBlocks = 16384;  // persistent blocks of the passes over the kept work items
+constexpr int kFlatBlocks = 65536;  // persistent blocks of the passes over the kept work items
 constexpr int kRasterGroup = 4;     // sources rasterised together per k_raster phase
 constexpr int kRasterCache = 1024;  // binned quads per phase whose coverage-pass results stay in LDS
 constexpr int kRasterCacheOne = 640; // the same for launch sets of up to kRasterGroup sources (one phase)
@@ -91,6 +91,7 @@ struct BatchDims {
     uint32_t bins_x, bins_y; // binning work items per source (for the largest source)
     int tile_shift;          // log2 of the tile edge: 4 (16 x 16 tiles, 256-thread raster blocks) or
                              // 5 (32 x 32, 1024-thread blocks)
+    uint32_t flat_blocks;    // blocks of the passes over the kept work items
 };
 
 enum { MODE_BLEND = 0, MODE_RAW = 1 };

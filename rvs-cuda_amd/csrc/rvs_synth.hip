@@ -781,8 +781,12 @@ hipError_t launch_bin(const G* geoms, const ViewDesc* views, const BatchDims& d,
     // kept items: persistent blocks striding over them (blocks past the
     // surviving items return at once).  A block's set-up, including the few
     // registers these kernels spill at occupancy 8, is paid once per block, so
-    // a few items per block beat a block per item.
-    const size_t fb = kFlatBlocks;
+    // a few items per block beat a block per item; fewer, longer-lived blocks
+    // lose to the tail when items differ in cost (equirectangular targets:
+    // items near a pole deal thousands of tiles), so the plan picks the count.
+    // A counter that blocks take items from is slower still: every take is a
+    // same-address atomic.
+    const size_t fb = d.flat_blocks ? d.flat_blocks : (uint32_t)kFlatBlocks;
     const unsigned flat = (unsigned)(d.range_cap < fb ? d.range_cap : fb);
     if (d.range_cap)
         hipLaunchKernelGGL((k_bin<G, false, true>), dim3(flat), dim3(256), 0, s, geoms, views, d, ws);
