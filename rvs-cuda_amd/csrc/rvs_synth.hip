@@ -176,20 +176,22 @@ __device__ inline bool quad_range_tv(const G& g, int i, int j, VF vtx, int out_w
 
 // Calls fn(slot, triangle, prol) for every triangle of source quad (i, j)
 // that gets past the vertex gating and colorize_triangle's early returns, in
-// slot order (the quad_build gating, one triangle at a time).
-template <class G, class F>
-__device__ inline void quad_tris(const G& g, int i, int j, int out_w, int out_h, F fn) {
-    const float3 v00 = vertex(g, i, j), v01 = vertex(g, i, j + 1), v10 = vertex(g, i + 1, j);
-    const float3 v11 = vertex(g, i + 1, j + 1);
+// slot order (the quad_build gating, one triangle at a time); vertices come
+// from vtx (exact), and the triangle's validity is left 0 for fn to evaluate
+// (tri_validity) when it needs it.
+template <class G, class VF, class F>
+__device__ inline void quad_tris(const G& g, int i, int j, VF vtx, int out_w, int out_h, F fn) {
+    const float3 v00 = vtx(i, j), v01 = vtx(i, j + 1), v10 = vtx(i + 1, j);
+    const float3 v11 = vtx(i + 1, j + 1);
     if (!(v01.z > 0.f && v10.z > 0.f && v01.x > 0.f && v10.x > 0.f)) return;
     auto one = [&](int slot, float3 a, float3 b, float3 c) {
         Tri T;
-        if (tri_setup(T, a, b, c, out_w, out_h, g.rescale)) fn(slot, T, (a.z > 0.f) || (b.z > 0.f) || (c.z > 0.f));
+        if (tri_setup_geom(T, a, b, c, out_w, out_h)) fn(slot, T, (a.z > 0.f) || (b.z > 0.f) || (c.z > 0.f));
     };
     if (v00.z > 0.f && v00.x > 0.f) one(0, v00, v01, v10);
     if (v11.z > 0.f && v11.x > 0.f) one(1, v11, v10, v01);
     if (g.wrap && j == 0) {
-        const float3 l0 = vertex(g, i, g.sw - 1), l1 = vertex(g, i + 1, g.sw - 1);
+        const float3 l0 = vtx(i, g.sw - 1), l1 = vtx(i + 1, g.sw - 1);
         one(2, l0, v00, l1);
         one(3, v10, l1, v00);
     }
@@ -1259,8 +1261,16 @@ __device__ void block_sort_global(uint32_t* L, uint32_t n) {
 template <class G>
 __device__ void replay_source(const G& g, uint32_t* L, uint32_t n, int tx0, int ty0, int out_w, int out_h,
                               bool mine, int px, int py, const uint8_t* dmask, uint32_t* sorted, ReplayEntry* rec,
-                              uint32_t* pool_n, FoldState& fs) {
+                              uint32_t* pool_n, FoldState& fs, const RasterWS& ws, const BatchDims& d, uint32_t vs) {
     const int tid = threadIdx.x;
+    // vertex (ii, jj) of the neighbourhood of a quad whose work item keeps
+    // the vertex window vb (null: none) starting at row r0, column c0: from
+    // the window (the same bits as vertex()), else evaluated
+    auto kept_or_eval = [&](const float* vb, int r0, int c0, int ii, int jj) -> float3 {
+        const int rr = ii - r0, cc = jj - c0;
+        if (vb && rr >= 0 && rr <= kBinQY && cc >= 0 && cc <= kBinQX) return kept_vertex(vb, rr, cc);
+        return vertex(g, ii, jj);
+    };
     Cand* pool = reinterpret_cast<Cand*>(sorted);
     int* head = reinterpret_cast<int*>(rec);
     const int pme = (py - ty0) * kTile + (px - tx0);
@@ -1276,7 +1286,11 @@ __device__ void replay_source(const G& g, uint32_t* L, uint32_t n, int tx0, int 
             const uint32_t q = L[e];
             int i, j;
             unpack_ij(q, i, j);
-            quad_tris(g, i, j, out_w, out_h, [&](int k, const Tri& Tr, bool prol) {
+            const float* vb = kept_block(ws, d, vs, i, j);
+            const int r0 = i - i % kBinQY, c0 = j - j % kBinQX;
+            auto vtx = [&](int ii, int jj) { return kept_or_eval(vb, r0, c0, ii, jj); };
+            quad_tris(g, i, j, vtx, out_w, out_h, [&](int k, Tri& Tr, bool prol) {
+                bool have_tv = false;   // the validity, evaluated for the first candidate
                 int xa = max(Tr.xmin, tx0), xb = min(Tr.xmax, tx0 + kTile - 1);
                 int ya = max(Tr.ymin, ty0), yb = min(Tr.ymax, ty0 + kTile - 1);
                 if (is_finite(Tr.A0) && is_finite(Tr.A1) && is_finite(Tr.B0) && is_finite(Tr.B1) &&
@@ -1291,6 +1305,10 @@ __device__ void replay_source(const G& g, uint32_t* L, uint32_t n, int tx0, int 
                         if (tri_surely_outside(Tr, n1, n2)) continue;
                         float l1, l2, l3;
                         if (!tri_cover(Tr, n1, n2, l1, l2, l3)) continue;
+                        if (!have_tv) {
+                            Tr.tv = tri_validity(Tr, g.rescale);
+                            have_tv = true;
+                        }
                         const uint32_t r = atomicAdd(pool_n, 1u);
                         if (r >= (uint32_t)kCandPool) continue;
                         const int nx = atomicExch(&head[p], (int)r);
@@ -1370,7 +1388,9 @@ __device__ void replay_source(const G& g, uint32_t* L, uint32_t n, int tx0, int 
             int i, j;
             unpack_ij(q, i, j);
             Quad Q;
-            auto vtx = [&](int ii, int jj2) -> float3 { return vertex(g, ii, jj2); };
+            const float* vb = kept_block(ws, d, vs, i, j);
+            const int r0 = i - i % kBinQY, c0 = j - j % kBinQX;
+            auto vtx = [&](int ii, int jj) { return kept_or_eval(vb, r0, c0, ii, jj); };
             quad_build(Q, g, i, j, vtx, out_w, out_h);
             ReplayEntry& R = rec[tid];
             for (int k = 0; k < 4; ++k) R.t[k] = Q.t[k];
@@ -1434,7 +1454,7 @@ __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, c
             const uint32_t n = ws.counts[bin];
             FoldState fs;
             replay_source(g, ws.lists + ws.offsets[bin], n, tx0, ty0, vd.w, vd.h, mine, px, py, dmask, sorted, rec,
-                          pool_n, fs);
+                          pool_n, fs, ws, d, (uint32_t)(v * d.nsrc + s));
             __syncthreads();
             if (mine) {
                 float3 c = make_float3(0.f, 0.f, 0.f);
