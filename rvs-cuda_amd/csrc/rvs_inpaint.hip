@@ -819,7 +819,7 @@ __device__ void solve_lines(const Region& R, const InpaintWS& ws, bool cols, Fn*
 // sweep after a forward one that changed nothing changes nothing either).
 
 constexpr int kStripThreads = 64;          // one wave
-constexpr int kStripLanePx = 2;            // consecutive scan positions per lane
+constexpr int kStripLanePx = 1;            // scan positions per lane (1: shortest line steps; 2 is 20 % slower)
 constexpr int kStripPx = kStripThreads * kStripLanePx;
 constexpr int kSpinLimit = 1 << 20;       // polls (each with a short sleep) before a wait gives up
 
