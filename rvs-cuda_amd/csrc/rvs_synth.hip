@@ -1068,9 +1068,10 @@ __global__ void __launch_bounds__(TILE * TILE) __attribute__((amdgpu_waves_per_e
                                                 BatchDims d, RasterWS ws) {
     constexpr int NT = TILE * TILE;
     constexpr int AREA = NT / (kTile * kTile);
-    __shared__ unsigned long long key1[kRasterGroup][NT];
-    __shared__ uint32_t key2[kRasterGroup][NT];
-    __shared__ uint32_t pref[kRasterGroup + 1];
+    constexpr int GRP = MULTI ? kRasterGroupMulti : kRasterGroup;   // sources per phase
+    __shared__ unsigned long long key1[GRP][NT];
+    __shared__ uint32_t key2[GRP][NT];
+    __shared__ uint32_t pref[GRP + 1];
     __shared__ int s_deferred;   // any pixel of the tile deferred
     __shared__ int s_hole;       // any pixel of the tile a hole (quality not > 0)
     // one phase: a smaller cache keeps a 16 x 16 block within 20 KB of LDS (8 blocks per CU)
@@ -1090,8 +1091,8 @@ __global__ void __launch_bounds__(TILE * TILE) __attribute__((amdgpu_waves_per_e
     BlendState st;
     float3 rc = make_float3(0.f, 0.f, 0.f);
     float rd = __builtin_inff(), rs = 0.f;
-    for (int s0 = 0; s0 < (MULTI ? d.nsrc : 1); s0 += kRasterGroup) {
-        const int gs = min(kRasterGroup, d.nsrc - s0);
+    for (int s0 = 0; s0 < (MULTI ? d.nsrc : 1); s0 += GRP) {
+        const int gs = min(GRP, d.nsrc - s0);
         for (int k = 0; k < gs; ++k) {
             key1[k][tid] = 0ull;
             key2[k][tid] = 0u;
