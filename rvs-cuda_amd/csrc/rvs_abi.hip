@@ -161,11 +161,11 @@ void plan_raster(Layout& L, RasterPlan& R, int nsrc, int nviews, uint32_t ntiles
     const uint64_t items = (uint64_t)d.bins_x * d.bins_y * nsrc * nviews;
     // Work items that keep their tile ranges, vertex window and quad validities (6.9 KB each):
     // at least an eighth of the items before culling, and enough for the entry
-    // capacity (a surviving item bins ~300 entries in the measured scenes, so
-    // capacity / 240 items cover a launch set that fills it); the rest
-    // re-evaluate vertices.
+    // capacity (a surviving item bins 165-335 entries in the measured scenes --
+    // fewest for equirectangular targets -- so capacity / 160 items cover a
+    // launch set that fills it); the rest re-evaluate vertices.
     uint64_t keep = items / 8 > 4096 ? items / 8 : 4096;
-    if ((uint64_t)d.cap / 240 > keep) keep = (uint64_t)d.cap / 240;
+    if ((uint64_t)d.cap / 160 > keep) keep = (uint64_t)d.cap / 160;
     // RVS_KEEP_ITEMS=<n> lowers the kept range (less workspace; the items
     // beyond it take the re-evaluating passes, with the same results)
     if (const char* env = std::getenv("RVS_KEEP_ITEMS")) {

@@ -137,11 +137,12 @@ __device__ inline void quad_build(Quad& Q, const G& g, int i, int j, VF vtx, int
     }
 }
 
-// Tile range [tx0, tx1] x [ty0, ty1] of the bounding boxes of the triangles of
-// source quad (i, j) that survive the gating and colorize_triangle's early
-// returns (the union quad_build + quad_tiles compute), one triangle at a time;
-// with tv, also the validities of T1 and T2 (0 when not alive or left
-// undecided by the fast path).  False when none survives.
+// Tile range [tx0, tx1] x [ty0, ty1] of the bounding boxes, narrowed to the
+// pixels each can cover (tri_tight), of the triangles of source quad (i, j)
+// that survive the gating and colorize_triangle's early returns (the union
+// quad_build + quad_tiles compute), one triangle at a time; with tv, also the
+// validities of T1 and T2 (0 when not alive or left undecided by the fast
+// path).  False when no triangle can cover a pixel.
 template <class G, class VF>
 __device__ inline bool quad_range_tv(const G& g, int i, int j, VF vtx, int out_w, int out_h, int shift, int& tx0,
                                      int& tx1, int& ty0, int& ty1, float2* tv) {
@@ -152,8 +153,14 @@ __device__ inline bool quad_range_tv(const G& g, int i, int j, VF vtx, int out_w
     auto one = [&](float3 a, float3 b, float3 c, float* tvo) {
         Tri T;
         if (!tri_setup_geom(T, a, b, c, out_w, out_h)) return;
-        xmin = min(xmin, T.xmin); xmax = max(xmax, T.xmax);
-        ymin = min(ymin, T.ymin); ymax = max(ymax, T.ymax);
+        // tiles only where the triangle can cover a pixel (tri_tight; a triangle
+        // that can cover none is never rasterised, so its validity is unused)
+        int xa = T.xmin, xb = T.xmax, ya = T.ymin, yb = T.ymax;
+        tri_tight(T, xa, xb, ya, yb);
+        if (xa <= xb && ya <= yb) {
+            xmin = min(xmin, xa); xmax = max(xmax, xb);
+            ymin = min(ymin, ya); ymax = max(ymax, yb);
+        }
         // fast path only: an undecided validity stays 0 (unknown), and the
         // raster then defers the pixels that depend on it
         if (tvo && !tri_validity_fast(T, g.rescale, *tvo)) *tvo = 0.f;
@@ -435,16 +442,20 @@ __device__ inline int find_slot(const uint32_t* keys, uint32_t t) {
     return -1;
 }
 
-// Tile range [tx0, tx1] x [ty0, ty1] touched by the bounding boxes of a
-// quad's surviving triangles; false when none survives.
+// Tile range [tx0, tx1] x [ty0, ty1] touched by the narrowed bounding boxes
+// of a quad's surviving triangles; false when none can cover a pixel.
 __device__ inline bool quad_tiles(const Quad& Q, int shift, int& tx0, int& tx1, int& ty0, int& ty1) {
     if (!Q.alive) return false;
     int xmin = INT_MAX, xmax = INT_MIN, ymin = INT_MAX, ymax = INT_MIN;
     for (int k = 0; k < 4; ++k)
         if (Q.alive & (1u << k)) {
-            xmin = min(xmin, Q.t[k].xmin); xmax = max(xmax, Q.t[k].xmax);
-            ymin = min(ymin, Q.t[k].ymin); ymax = max(ymax, Q.t[k].ymax);
+            int xa = Q.t[k].xmin, xb = Q.t[k].xmax, ya = Q.t[k].ymin, yb = Q.t[k].ymax;
+            tri_tight(Q.t[k], xa, xb, ya, yb);   // as quad_range_tv: the pixels it can cover
+            if (xa > xb || ya > yb) continue;
+            xmin = min(xmin, xa); xmax = max(xmax, xb);
+            ymin = min(ymin, ya); ymax = max(ymax, yb);
         }
+    if (xmin > xmax) return false;
     tx0 = xmin >> shift; tx1 = xmax >> shift; ty0 = ymin >> shift; ty1 = ymax >> shift;
     return true;
 }

@@ -292,7 +292,7 @@ def test_perspective_grid_sources(torch_cuda):
 
 @pytest.mark.parametrize("tile", [16, 32])
 def test_dense_sources_compact_bins(torch_cuda, tile):
-    """Sources far denser than the target (eleven 512x512 views into 128x72):
+    """Sources far denser than the target (eleven 512x512 views into 96x54):
     thousands of quads per (tile, source) bin, exact against the oracle; the
     entry list grows on demand from a deliberately small capacity."""
     torch = torch_cuda
@@ -301,7 +301,7 @@ def test_dense_sources_compact_bins(torch_cuda, tile):
     planes = W.render_sources(rg)
     srcs = W.device_sources(rg, planes)
     t = rg["target"]
-    tgt = R.make_target(W.perspective_cam(128, 72), [0.01, 0.02, 0.0], [3.0, 1.0, 0.0], t["bitdepth"],
+    tgt = R.make_target(W.perspective_cam(96, 54), [0.01, 0.02, 0.0], [3.0, 1.0, 0.0], t["bitdepth"],
                         t["blending_exp"], R.scale_of(10), R.scale_of(16))
     R.profile_enable(True)
     try:
