@@ -144,7 +144,8 @@ uint32_t entry_capacity(uint32_t requested, uint64_t npx, int nsrc) {
 }
 
 void plan_raster(Layout& L, RasterPlan& R, int nsrc, int nviews, uint32_t ntiles, uint32_t npx, uint32_t cap,
-                 int max_sw, int max_sh, uint64_t keep_max = 0, int shift = 4, bool uneven_items = true) {
+                 int max_sw, int max_sh, uint64_t keep_max = 0, int shift = 4, bool uneven_items = true,
+                 int n_unculled = 0, double culled_px_frac = 1.0) {
     rvs_k::BatchDims& d = R.d;
     d.nviews = nviews; d.nsrc = nsrc; d.ntiles = ntiles; d.npx = npx;
     d.tile_shift = shift;
@@ -160,12 +161,16 @@ void plan_raster(Layout& L, RasterPlan& R, int nsrc, int nviews, uint32_t ntiles
     d.bins_y = (uint32_t)((max_sh - 1 + rvs_k::kBinQY - 1) / rvs_k::kBinQY);
     const uint64_t items = (uint64_t)d.bins_x * d.bins_y * nsrc * nviews;
     // Work items that keep their tile ranges, vertex window and quad validities (6.9 KB each):
-    // at least an eighth of the items before culling, and enough for the entry
-    // capacity (a surviving item bins 165-335 entries in the measured scenes --
-    // fewest for equirectangular targets -- so capacity / 160 items cover a
-    // launch set that fills it); the rest re-evaluate vertices.
+    // at least an eighth of the items before culling; every item of the
+    // n_unculled views whose items all survive (equirectangular targets are
+    // not culled); and, for the others, enough for their share of the entry
+    // capacity (a surviving item bins 165-335 entries in the measured scenes,
+    // so capacity / 160 items cover a launch set that fills it).  The rest
+    // re-evaluate vertices.
     uint64_t keep = items / 8 > 4096 ? items / 8 : 4096;
-    if ((uint64_t)d.cap / 160 > keep) keep = (uint64_t)d.cap / 160;
+    const uint64_t need = items / (uint64_t)nviews * (uint64_t)n_unculled +
+                          (uint64_t)((double)d.cap * culled_px_frac / 160.0);
+    if (need > keep) keep = need;
     // RVS_KEEP_ITEMS=<n> lowers the kept range (less workspace; the items
     // beyond it take the re-evaluating passes, with the same results)
     if (const char* env = std::getenv("RVS_KEEP_ITEMS")) {
@@ -277,7 +282,8 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
         max_sw = srcs[k].cam.w > max_sw ? srcs[k].cam.w : max_sw;
         max_sh = srcs[k].cam.h > max_sh ? srcs[k].cam.h : max_sh;
     }
-    uint64_t tiles = 0, px = 0, rpx = 0;
+    uint64_t tiles = 0, px = 0, rpx = 0, persp_px = 0;
+    int n_erp = 0;   // equirectangular targets: their work items are not culled
     P.max_w = 0; P.max_h = 0; P.max_ew = 0; P.max_eh = 0; P.any_persp = false; P.any_erp = false;
     P.any_rescaled = false;
     // the launch set's tile edge: the largest any view asks for
@@ -307,6 +313,8 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
         P.max_eh = t.cam.h > P.max_eh ? t.cam.h : P.max_eh;
         P.any_persp = P.any_persp || t.cam.proj == 0;
         P.any_erp = P.any_erp || t.cam.proj != 0;
+        if (t.cam.proj != 0) ++n_erp;
+        else persp_px += (uint64_t)rw * rh;
         if (rw != t.cam.w || rh != t.cam.h) {
             P.any_rescaled = true;
             rpx += (uint64_t)t.cam.w * t.cam.h;
@@ -332,7 +340,7 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
     // the colour map of views whose caller wants only the encoded planes
     P.color = L.take((size_t)px * 3 * sizeof(float));
     plan_raster(L, P.R, n_src, n_views, (uint32_t)tiles, (uint32_t)px, cap, max_sw, max_sh, keep_max, shift,
-                P.any_erp);
+                P.any_erp, n_erp, px ? (double)persp_px / (double)px : 1.0);
     P.inpaint = L.take(rvs_k::inpaint_ws_bytes((size_t)px, n_views, P.max_w, P.max_h));
     // oversampled views: their colour resized to the view's size, for the encoder
     P.resized = L.take((size_t)rpx * 3 * sizeof(float));

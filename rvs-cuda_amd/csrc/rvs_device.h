@@ -313,7 +313,7 @@ __device__ inline bool tri_setup(Tri& T, float3 a, float3 b, float3 c, int out_w
 }
 
 // Narrows [xa, xb] x [ya, yb] to the pixels whose centre lies within
-// dl = 1/8 + 4e-6 s of the triangle's extent, s = max(W, H) + 2; no other
+// dl = 0.07 + 4e-6 s of the triangle's extent, s = max(W, H) + 2; no other
 // pixel can pass tri_cover.  Proof sketch (the box starts at column / row >= 0,
 // so the clamps below never cut it): a centre more than dl outside the
 // x-extent W has an exact barycentric coordinate below -dl/(2W) (its x offset
@@ -321,15 +321,17 @@ __device__ inline bool tri_setup(Tri& T, float3 a, float3 b, float3 c, int out_w
 // of them negative and none below -W); the kernel's l_k differ from the exact
 // ones by at most 2^-19 (W+2)(H+2)/den (rounding of the differences, products,
 // sum and division), which the guard den >= 2^-14 s^3 keeps below 2^-5/s, so
-// the computed one stays below -dl/(2W) + 2^-5/s < -1e-6 (likewise in y).
-// Thinner triangles keep the box.  The binning passes narrow each triangle's
+// the computed one stays below -dl/(2W) + 2^-5/s < -1e-6 (likewise in y):
+// dl > 2W (1e-6 + 2^-5/s) holds with 0.0075 to spare, more than the rounding
+// of the bounds below for coordinates under 2^16.  Thinner triangles keep the
+// box.  The binning passes narrow each triangle's
 // box the same way before picking its tiles.
 __device__ inline void tri_tight(const Tri& T, int& xa, int& xb, int& ya, int& yb) {
     const float minx = smin(smin(T.A0, T.B0), T.C0), maxx = smax(smax(T.A0, T.B0), T.C0);
     const float miny = smin(smin(T.A1, T.B1), T.C1), maxy = smax(smax(T.A1, T.B1), T.C1);
     const float sz = fmaxf(maxx - minx, maxy - miny) + 2.f;
     if (!(T.den >= 6.103515625e-05f * sz * sz * sz)) return;   // 2^-14; NaN or huge extents keep the box
-    const float dl = 0.125f + 4e-6f * sz;
+    const float dl = 0.07f + 4e-6f * sz;
     xa = imax(xa, cvt_i32(__builtin_ceilf(fmaxf(minx - 0.5f - dl, -2.f))));
     xb = imin(xb, cvt_i32(__builtin_floorf(fminf(maxx - 0.5f + dl, 1e9f))));
     ya = imax(ya, cvt_i32(__builtin_ceilf(fmaxf(miny - 0.5f - dl, -2.f))));

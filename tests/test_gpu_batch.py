@@ -292,16 +292,17 @@ def test_perspective_grid_sources(torch_cuda):
 
 @pytest.mark.parametrize("tile", [16, 32])
 def test_dense_sources_compact_bins(torch_cuda, tile):
-    """Sources far denser than the target (eleven 512x512 views into 96x54):
-    thousands of quads per (tile, source) bin, exact against the oracle; the
-    entry list grows on demand from a deliberately small capacity."""
+    """Sources far denser than the target (eleven 512x512 views into 64x36):
+    thousands of quads per (tile, source) bin (beyond the raster's LDS cache
+    of 1024 per phase), exact against the oracle; the entry list grows on
+    demand from a deliberately small capacity."""
     torch = torch_cuda
     say("case dense bins, tile %d" % tile)
     rg = W.rig("persp11_2048", scale=4)
     planes = W.render_sources(rg)
     srcs = W.device_sources(rg, planes)
     t = rg["target"]
-    tgt = R.make_target(W.perspective_cam(96, 54), [0.01, 0.02, 0.0], [3.0, 1.0, 0.0], t["bitdepth"],
+    tgt = R.make_target(W.perspective_cam(64, 36), [0.01, 0.02, 0.0], [3.0, 1.0, 0.0], t["bitdepth"],
                         t["blending_exp"], R.scale_of(10), R.scale_of(16))
     R.profile_enable(True)
     try:
@@ -312,7 +313,7 @@ def test_dense_sources_compact_bins(torch_cuda, tile):
     finally:
         R.profile_enable(False)
     say("  largest bin %d, entries %d, capacity grown to %d" % (cnt["max_bin"], cnt["bin_entries"], syn.entry_cap))
-    assert cnt["max_bin"] > 2048 and syn.entry_cap >= cnt["bin_entries"]
+    assert cnt["max_bin"] > 1024 and syn.entry_cap >= cnt["bin_entries"]
     _check_oracle(out, _oracle(rg, planes, tgt), tgt.bitdepth, "dense")
 
 

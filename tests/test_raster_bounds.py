@@ -3,7 +3,7 @@ drops a pixel that the coverage test (transform.cpp:105-117, float32 as the
 kernel evaluates it) accepts.
 
 The narrowed range keeps the pixels whose centre lies within
-dl = 1/8 + 4e-6 * s of the triangle's extent (s = max(W, H) + 2), when
+dl = 0.07 + 4e-6 * s of the triangle's extent (s = max(W, H) + 2), when
 den >= 2^-14 s^3.  Why that suffices: a centre more than dl outside the
 x-extent W has an exact barycentric coordinate below -dl / (2 W) (the point's
 x offset is the barycentric mean of the corners' offsets, at most two of which
@@ -44,7 +44,7 @@ def _tight(A, B, C, den, zero_margin=False):
     minx, maxx, miny, maxy = xs.min(0), xs.max(0), ys.min(0), ys.max(0)
     sz = np.maximum(maxx - minx, maxy - miny) + f(2)
     guard = den >= f(6.103515625e-05) * sz * sz * sz
-    dl = f(0.125) + f(4e-6) * sz
+    dl = f(0.07) + f(4e-6) * sz
     if zero_margin:
         dl = np.zeros_like(dl)
     xa = np.ceil(np.maximum(minx - f(0.5) - dl, f(-2)))
