@@ -138,8 +138,13 @@ int rvs_inpaint(float* color, const float* quality, int w, int h, void* workspac
  * (image_writing.cpp:189-190, cv::resize with INTER_LINEAR): an exact 2x
  * downscale averages each 2x2 block (OpenCV's area path), other sizes use
  * half-pixel-centred bilinear weights.  OpenCV's operation order is not
- * pinned (no OpenCV here); the oracle restates the same rule.  float3 images. */
-int rvs_resize_color(const float* src, int sw, int sh, float* dst, int dw, int dh, rvs_stream_t stream);
+ * pinned (no OpenCV here); the oracle restates the same rule.  float3 images.
+ * The workspace (rvs_resize_color_workspace_size bytes) receives the per
+ * column and per row taps, computed on the host in double as cv::resize does;
+ * the call returns once they are enqueued (the host copy is not kept). */
+size_t rvs_resize_color_workspace_size(int sw, int sh, int dw, int dh);
+int rvs_resize_color(const float* src, int sw, int sh, float* dst, int dw, int dh, void* workspace,
+                     size_t ws_bytes, rvs_stream_t stream);
 
 /* BlendedViewMultiSpec::blend's frequency split (BlendedView.cpp:70-80;
  * split_frequencies, blending.cpp:371-394; calcBlurring, :62-159) of one

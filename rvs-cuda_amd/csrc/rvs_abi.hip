@@ -224,6 +224,25 @@ hipError_t reset_raster(const rvs_k::RasterWS& ws, const rvs_k::BatchDims& d, hi
 }
 
 // A view's tile grid and its place in the batch's global tile / pixel numbering.
+// cv::resize's INTER_LINEAR taps along one axis (image_writing.cpp:189-190):
+// for each destination index d, source index s = floor(f) and weight f - s of
+// s + 1, f = (float)((d + 0.5) * scale - 0.5) with scale = src / dst in
+// double, clamped to [0, n_src - 1] with weight 0 at the ends; the weight's
+// bits in .y.  Evaluated here, on the host, in double as the reference does.
+void resize_taps(int n_src, int n_dst, int2* out) {
+    const double scale = (double)n_src / (double)n_dst;
+    for (int d = 0; d < n_dst; ++d) {
+        float f = (float)((d + 0.5) * scale - 0.5);
+        int s0 = (int)std::floor(f);
+        f -= (float)s0;
+        if (s0 < 0) { f = 0.f; s0 = 0; }
+        if (s0 >= n_src - 1) { f = 0.f; s0 = n_src - 1; }
+        int fb;
+        std::memcpy(&fb, &f, sizeof fb);
+        out[d] = make_int2(s0, fb);
+    }
+}
+
 void view_grid(rvs_k::ViewDesc& v, int w, int h, uint32_t& tiles, uint32_t& px, int shift = 4) {
     const int edge = 1 << shift;
     v.w = w; v.h = h;
@@ -259,7 +278,7 @@ int check_raster_status(const rvs_k::RasterWS& ws, const rvs_k::BatchDims& d, ui
 // in workspace order.
 struct SynthPlan {
     RasterPlan R;
-    size_t views, geoms, bounds, quality, color, inpaint, resized, total;
+    size_t views, geoms, bounds, quality, color, inpaint, resized, rtab, total;
     int tile_shift;       // log2 of the launch set's tile edge
     int max_w, max_h;     // largest raster (oversampled by Precision)
     int max_ew, max_eh;   // largest view
@@ -282,7 +301,7 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
         max_sw = srcs[k].cam.w > max_sw ? srcs[k].cam.w : max_sw;
         max_sh = srcs[k].cam.h > max_sh ? srcs[k].cam.h : max_sh;
     }
-    uint64_t tiles = 0, px = 0, rpx = 0, persp_px = 0;
+    uint64_t tiles = 0, px = 0, rpx = 0, persp_px = 0, rtaps = 0;
     int n_erp = 0;   // equirectangular targets: their work items are not culled
     P.max_w = 0; P.max_h = 0; P.max_ew = 0; P.max_eh = 0; P.any_persp = false; P.any_erp = false;
     P.any_rescaled = false;
@@ -318,6 +337,7 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
         if (rw != t.cam.w || rh != t.cam.h) {
             P.any_rescaled = true;
             rpx += (uint64_t)t.cam.w * t.cam.h;
+            rtaps += (uint64_t)t.cam.w + t.cam.h;
         }
         tiles += (uint64_t)((rw + edge - 1) / edge) * ((rh + edge - 1) / edge);
         px += (uint64_t)rw * rh;
@@ -344,6 +364,7 @@ int plan_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int n_
     P.inpaint = L.take(rvs_k::inpaint_ws_bytes((size_t)px, n_views, P.max_w, P.max_h));
     // oversampled views: their colour resized to the view's size, for the encoder
     P.resized = L.take((size_t)rpx * 3 * sizeof(float));
+    P.rtab = L.take((size_t)rtaps * sizeof(int2));   // their resize taps
     P.total = L.o;
     if (views) {
         views->resize(n_views);
@@ -405,6 +426,8 @@ int enqueue_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int
     float* quality = (float*)(base + P.quality);
     float* icolor = (float*)(base + P.color);
     float* resized = (float*)(base + P.resized);
+    int2* rtab = (int2*)(base + P.rtab);
+    std::vector<int2> taps;   // host copy of the resize taps, uploaded below
     for (int v = 0; v < n_views; ++v) {
         const rvs_view_out& o = outs[v];
         rvs_k::ViewDesc& d = views[v];
@@ -418,6 +441,10 @@ int enqueue_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int
         if (rescaled) {
             d.ecol = resized;
             resized += (size_t)d.ew * d.eh * 3;
+            d.rtab = rtab + taps.size();
+            taps.resize(taps.size() + (size_t)d.ew + d.eh);
+            resize_taps(d.w, d.ew, taps.data() + taps.size() - d.ew - d.eh);
+            resize_taps(d.h, d.eh, taps.data() + taps.size() - d.eh);
         } else {
             d.ecol = color;
         }
@@ -467,6 +494,8 @@ int enqueue_synth(const rvs_source* srcs, int n_src, const rvs_target* tgts, int
     }
     HIP_TRY(rvs_k::launch_upload(views.data(), views.size() * sizeof(rvs_k::ViewDesc), dviews, s), "view upload");
     HIP_TRY(rvs_k::launch_upload(hg.data(), hg.size() * sizeof(PairGeom), dg, s), "geometry upload");
+    if (!taps.empty())
+        HIP_TRY(rvs_k::launch_upload(taps.data(), taps.size() * sizeof(int2), rtab, s), "resize taps upload");
     HIP_TRY(rvs_k::launch_bin<PairGeom>(dg, dviews, d, ws, s), "bin");
     prof_mark(1, s);
     HIP_TRY((rvs_k::launch_raster<PairGeom, rvs_k::MODE_BLEND>(dg, dviews, d, ws, s)), "raster");
@@ -753,10 +782,24 @@ int rvs_inpaint(float* color, const float* quality, int w, int h, void* workspac
     return RVS_OK;
 }
 
-int rvs_resize_color(const float* src, int sw, int sh, float* dst, int dw, int dh, rvs_stream_t stream) {
-    if (!src || !dst || sw < 1 || sh < 1 || dw < 1 || dh < 1 || (uint64_t)dw * dh >= (1ull << 31))
+size_t rvs_resize_color_workspace_size(int sw, int sh, int dw, int dh) {
+    if (sw < 1 || sh < 1 || dw < 1 || dh < 1 || (uint64_t)dw * dh >= (1ull << 31)) return 0;
+    return align256(((size_t)dw + dh) * sizeof(int2));
+}
+
+int rvs_resize_color(const float* src, int sw, int sh, float* dst, int dw, int dh, void* workspace,
+                     size_t ws_bytes, rvs_stream_t stream) {
+    if (!src || !dst || !workspace || sw < 1 || sh < 1 || dw < 1 || dh < 1 || (uint64_t)dw * dh >= (1ull << 31))
         return fail(RVS_E_INVALID, "rvs_resize_color: bad argument");
-    HIP_TRY(rvs_k::launch_resize_color(src, sw, sh, dst, dw, dh, (hipStream_t)stream), "resize_color");
+    if (ws_bytes < rvs_resize_color_workspace_size(sw, sh, dw, dh))
+        return fail(RVS_E_INVALID, "rvs_resize_color: workspace too small");
+    std::vector<int2> taps((size_t)dw + dh);
+    resize_taps(sw, dw, taps.data());
+    resize_taps(sh, dh, taps.data() + dw);
+    hipStream_t s = (hipStream_t)stream;
+    int2* dtaps = (int2*)workspace;
+    HIP_TRY(rvs_k::launch_upload(taps.data(), taps.size() * sizeof(int2), dtaps, s), "resize taps upload");
+    HIP_TRY(rvs_k::launch_resize_color(src, sw, sh, dst, dw, dh, dtaps, s), "resize_color");
     return RVS_OK;
 }
 

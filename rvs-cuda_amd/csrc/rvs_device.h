@@ -509,20 +509,15 @@ __device__ inline void blend_step(BlendState& s, float3 c, float d, float shape,
 // 189-190), OpenCV's INTER_LINEAR on a float image, at destination pixel
 // (x, y): an exact 2x downscale takes OpenCV's area path, the 2x2 block sum
 // left to right, top to bottom, times 0.25; any other size the half-pixel
-// centred bilinear rule -- f = (float)((d + 0.5) * (src / dst) - 0.5),
-// s = floor(f), f -= s, clamped to [0, n - 1] with f = 0 at the ends, weights
-// (1 - f, f), rows first.  OpenCV's own operation order is not pinned here.
-__device__ inline void resize_axis(int d, int n_src, int n_dst, int& s0, int& s1, float& f) {
-    const double scale = (double)n_src / (double)n_dst;
-    f = (float)((d + 0.5) * scale - 0.5);
-    s0 = (int)__builtin_floorf(f);
-    f -= (float)s0;
-    if (s0 < 0) { f = 0.f; s0 = 0; }
-    if (s0 >= n_src - 1) { f = 0.f; s0 = n_src - 1; }
-    s1 = s0 + 1 < n_src ? s0 + 1 : s0;
-}
-
-__device__ inline float3 resize_linear_px(const float* src, int sw, int sh, int dw, int dh, int x, int y) {
+// centred bilinear rule -- f = (float)((d + 0.5) * (src / dst) - 0.5) in
+// double, s = floor(f), f -= s, clamped to [0, n - 1] with f = 0 at the ends,
+// weights (1 - f, f), rows first.  OpenCV's own operation order is not pinned
+// here.  The taps (s, f bits) of every destination column and row come from
+// tables the host computes (resize_taps): the double expression has no
+// float-only equivalent that is exact in every case (a value on a float
+// rounding midpoint rounds by the double's own error).
+__device__ inline float3 resize_linear_px(const float* src, int sw, int sh, int dw, int dh, int x, int y,
+                                          const int2* tx, const int2* ty) {
     auto at = [&](int r, int c) {
         const float* p = src + 3 * ((size_t)r * sw + c);
         return make_float3(p[0], p[1], p[2]);
@@ -532,10 +527,10 @@ __device__ inline float3 resize_linear_px(const float* src, int sw, int sh, int 
         return make_float3((((a.x + b.x) + c.x) + d.x) * 0.25f, (((a.y + b.y) + c.y) + d.y) * 0.25f,
                            (((a.z + b.z) + c.z) + d.z) * 0.25f);
     }
-    int x0, x1, y0, y1;
-    float fx, fy;
-    resize_axis(x, sw, dw, x0, x1, fx);
-    resize_axis(y, sh, dh, y0, y1, fy);
+    const int2 ax = tx[x], ay = ty[y];
+    const int x0 = ax.x, y0 = ay.x;
+    const int x1 = x0 + 1 < sw ? x0 + 1 : x0, y1 = y0 + 1 < sh ? y0 + 1 : y0;
+    const float fx = __int_as_float(ax.y), fy = __int_as_float(ay.y);
     const float gx = 1.f - fx, gy = 1.f - fy;
     const float3 a = at(y0, x0), b = at(y0, x1), c = at(y1, x0), d = at(y1, x1);
     const float3 r0 = make_float3(a.x * gx + b.x * fx, a.y * gx + b.y * fx, a.z * gx + b.z * fx);

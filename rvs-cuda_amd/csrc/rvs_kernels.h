@@ -53,6 +53,7 @@ struct ViewDesc {
     int ew, eh;              // size of the encoded planes: the view's own size (w, h are the raster's,
                              // oversampled by Precision)
     float* ecol;             // colour the planes encode: color, or its resize to ew x eh
+    const int2* rtab;        // resize taps of an oversampled view: ew columns, then eh rows
     int tile_shift;          // log2 of the launch set's tile edge
     int enc_raster;          // planes wanted, not oversampled: tiles without holes or deferred
                              // pixels are encoded by k_raster itself

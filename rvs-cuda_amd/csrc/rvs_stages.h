@@ -28,7 +28,9 @@ hipError_t launch_encode(const float* color, int w, int h, int bitdepth, void* y
 hipError_t launch_encode_views(const ViewDesc* views, int nviews, int max_w, int max_h, const uint8_t* tile_enc,
                                hipStream_t s);
 // write_color's resize (image_writing.cpp:189-190) of one float3 image.
-hipError_t launch_resize_color(const float* src, int sw, int sh, float* dst, int dw, int dh, hipStream_t s);
+// taps: the resize_taps tables of the columns (dw entries) then the rows (dh)
+hipError_t launch_resize_color(const float* src, int sw, int sh, float* dst, int dw, int dh, const int2* taps,
+                               hipStream_t s);
 // Multispectral blending's frequency split of one float3 view (rgb: blur all
 // three channels, else Y only); ws: split_ws_bytes(w * h, rgb) bytes.
 size_t split_ws_bytes(size_t npx, int rgb);

@@ -132,17 +132,18 @@ __global__ void k_resize_views(const ViewDesc* __restrict__ views) {
     const ViewDesc& vd = views[blockIdx.z];
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (vd.ecol == vd.color || x >= vd.ew || y >= vd.eh) return;
-    const float3 c = resize_linear_px(vd.color, vd.w, vd.h, vd.ew, vd.eh, x, y);
+    const float3 c = resize_linear_px(vd.color, vd.w, vd.h, vd.ew, vd.eh, x, y, vd.rtab, vd.rtab + vd.ew);
     float* o = vd.ecol + 3 * ((size_t)y * vd.ew + x);
     o[0] = c.x; o[1] = c.y; o[2] = c.z;
 }
 
-// The same for one colour image (the stage entry point).
+// The same for one colour image (the stage entry point; taps: dw column
+// entries, then dh row entries).
 __global__ void k_resize_color(const float* __restrict__ src, int sw, int sh, float* __restrict__ dst, int dw,
-                               int dh) {
+                               int dh, const int2* __restrict__ taps) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= dw || y >= dh) return;
-    const float3 c = resize_linear_px(src, sw, sh, dw, dh, x, y);
+    const float3 c = resize_linear_px(src, sw, sh, dw, dh, x, y, taps, taps + dw);
     float* o = dst + 3 * ((size_t)y * dw + x);
     o[0] = c.x; o[1] = c.y; o[2] = c.z;
 }
@@ -152,8 +153,9 @@ hipError_t launch_resize_views(const ViewDesc* views, int nviews, int max_ew, in
     return hipGetLastError();
 }
 
-hipError_t launch_resize_color(const float* src, int sw, int sh, float* dst, int dw, int dh, hipStream_t s) {
-    hipLaunchKernelGGL(k_resize_color, dim3((dw + 255) / 256, dh), dim3(256), 0, s, src, sw, sh, dst, dw, dh);
+hipError_t launch_resize_color(const float* src, int sw, int sh, float* dst, int dw, int dh, const int2* taps,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(k_resize_color, dim3((dw + 255) / 256, dh), dim3(256), 0, s, src, sw, sh, dst, dw, dh, taps);
     return hipGetLastError();
 }
 

@@ -532,7 +532,9 @@ void Pipeline::saveColor(float3* color, int virtualFrame, int virtualView, Param
     const float* src = reinterpret_cast<const float*>(color);
     if (size != vs) {
         resized.resize(3 * npx * sizeof(float));
-        check(rvs_resize_color(src, size.width, size.height, resized.as<float>(), vs.width, vs.height, m_save_stream),
+        DeviceBuffer rws(rvs_resize_color_workspace_size(size.width, size.height, vs.width, vs.height));
+        check(rvs_resize_color(src, size.width, size.height, resized.as<float>(), vs.width, vs.height, rws.get(),
+                               rws.bytes(), m_save_stream),
               "rvs_resize_color");
         src = resized.as<float>();
     }
@@ -655,8 +657,9 @@ void Pipeline::computeView(int inputFrame, int virtualFrame, int virtualView) {
               "rvs_inpaint");
         if (rescaled) {
             fo.resized.resize(3 * npx * sizeof(float));
+            DeviceBuffer rws(rvs_resize_color_workspace_size(rs.width, rs.height, vs.width, vs.height));
             check(rvs_resize_color(color.as<float>(), rs.width, rs.height, fo.resized.as<float>(), vs.width,
-                                   vs.height, m_stream),
+                                   vs.height, rws.get(), rws.bytes(), m_stream),
                   "rvs_resize_color");
         }
         check(rvs_encode_yuv420(rescaled ? fo.resized.as<float>() : color.as<float>(), vs.width, vs.height, bd,

@@ -91,7 +91,8 @@ def hip():
             "rvs_inpaint_workspace_size": ([i, i], sz),
             "rvs_inpaint": ([vp, vp, i, i, vp, sz, vp, vp], i),
             "rvs_encode_yuv420": ([vp, i, i, i, vp, vp, vp, vp], i),
-            "rvs_resize_color": ([vp, i, i, vp, i, i, vp], i),
+            "rvs_resize_color_workspace_size": ([i, i, i, i], sz),
+            "rvs_resize_color": ([vp, i, i, vp, i, i, vp, sz, vp], i),
             "rvs_split_frequencies_workspace_size": ([i, i, i], sz),
             "rvs_split_frequencies": ([vp, vp, vp, i, i, i, vp, sz, vp, vp, vp], i),
             "rvs_add_color": ([vp, vp, vp, i, i, vp], i),
@@ -403,9 +404,14 @@ def raster_size(target):
 
 
 def resize_color(src, out, stream=None):
+    import torch
     sh, sw = src.shape[:2]
     dh, dw = out.shape[:2]
-    check(hip().rvs_resize_color(_ptr(src), sw, sh, _ptr(out), dw, dh, _stream(stream)))
+    nb = hip().rvs_resize_color_workspace_size(sw, sh, dw, dh)
+    if nb == 0:
+        raise RvsError(RVS_E_INVALID, "resize_color: bad sizes")
+    ws = torch.empty(nb, dtype=torch.uint8, device=src.device)   # stream-ordered reuse only after the kernel
+    check(hip().rvs_resize_color(_ptr(src), sw, sh, _ptr(out), dw, dh, _ptr(ws), nb, _stream(stream)))
 
 
 def make_target(c, pos, rot, bitdepth, blending_exp, color_scale, depth_scale, rescale=1.0, tile=0):
