@@ -377,36 +377,39 @@ def main():
         srcs_b = W.device_sources(rg_b, W.render_sources(rg_b))
         mix = W.serve_requests([rg, rg_b], args.c4_requests, seed=4)
         budget4 = max(budget, 96 << 30)   # 4K views over 11 sources need larger workspaces
-        # requests here cost 0.2-15 ms of device time each: batch size and
-        # where a batch's inpainting and encoding run (in order on the compute
-        # stream, or split onto the slot's stream so that the long inpainting
-        # of a large hole overlaps the next batch) are swept (2 slots; every
-        # trial is reported); closed-loop capacity is the best of them, and
-        # the open loop runs at 70 % of it
+        # requests here cost 0.2-5 ms of device time each: batch size, the
+        # batches in flight (slots; the closed loop keeps batch x slots
+        # requests outstanding) and where a batch's inpainting and encoding run
+        # (in order on the compute stream, or split onto the slot's stream so
+        # that the long inpainting of a large hole overlaps the next batch) are
+        # swept (every trial is reported); closed-loop capacity is the best of
+        # them, and the open loop runs at 70 % of it
         trials4 = []
-        for sp4 in (False, True):
-            for b4 in (4, 8, 16, 32):
-                staging4 = b4 * max(A.yuv_bytes(t) for _, t in mix)
-                srv = S.Server([srcs, srcs_b], memory_budget=budget4, max_batch=b4, slots=2, entry_cap=0,
-                               host_staging_bytes=staging4, split_finish=sp4)
-                try:
-                    srv.run_closed(mix[:2 * b4], concurrency=2 * b4)
-                    a = time.perf_counter()
-                    stc = srv.run_closed(mix, concurrency=2 * b4)
-                    cap_s = time.perf_counter() - a
-                    closed = S.summarize(stc, wall_s=cap_s)
-                    closed["max_batch"] = b4
-                    closed["split_finish"] = sp4
-                    trials4.append(closed)
-                    log("c4 trial %s" % closed)
-                finally:
-                    srv.close()
+        for sl4 in (1, 2):
+            for sp4 in (False, True):
+                for b4 in (4, 8, 16, 32):
+                    staging4 = b4 * max(A.yuv_bytes(t) for _, t in mix)
+                    srv = S.Server([srcs, srcs_b], memory_budget=budget4, max_batch=b4, slots=sl4, entry_cap=0,
+                                   host_staging_bytes=staging4, split_finish=sp4)
+                    try:
+                        srv.run_closed(mix[:sl4 * b4], concurrency=sl4 * b4)
+                        a = time.perf_counter()
+                        stc = srv.run_closed(mix, concurrency=sl4 * b4)
+                        cap_s = time.perf_counter() - a
+                        closed = S.summarize(stc, wall_s=cap_s)
+                        closed["max_batch"] = b4
+                        closed["slots"] = sl4
+                        closed["split_finish"] = sp4
+                        trials4.append(closed)
+                        log("c4 trial %s" % closed)
+                    finally:
+                        srv.close()
         closed4 = max(trials4, key=lambda t: t["views_s"])
-        b4, sp4 = closed4["max_batch"], closed4["split_finish"]
-        srv = S.Server([srcs, srcs_b], memory_budget=budget4, max_batch=b4, slots=2, entry_cap=0,
+        b4, sl4, sp4 = closed4["max_batch"], closed4["slots"], closed4["split_finish"]
+        srv = S.Server([srcs, srcs_b], memory_budget=budget4, max_batch=b4, slots=sl4, entry_cap=0,
                        host_staging_bytes=b4 * max(A.yuv_bytes(t) for _, t in mix), split_finish=sp4)
         try:
-            srv.run_closed(mix[:2 * b4], concurrency=2 * b4)
+            srv.run_closed(mix[:sl4 * b4], concurrency=sl4 * b4)
             rate4 = 0.7 * closed4["views_s"]
             sto = srv.run_open(mix, rate=rate4, seed=4)
             open4 = S.summarize(sto)
