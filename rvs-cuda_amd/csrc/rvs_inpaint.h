@@ -17,17 +17,19 @@ constexpr int kHoleTileBlocks = 2048;   // persistent blocks of the passes over 
 
 // Indexed by global pixel (a view's pixels start at its px_base).
 struct InpaintWS {
-    int* label;     // -1 known pixel, else union-find parent (root after flattening)
-    int* dist;      // distance of the map (inpainting.cpp:182-188); holes only
-    int* src;       // global index of the chosen known pixel; holes only
-    int* cid;       // component id of a root
+    // per pixel, one 16-byte record (a line step of the solvers loads all of
+    // it at once): x distance of the map (inpainting.cpp:182-188; holes only),
+    // y global index of the chosen known pixel (holes only), z label (-1 known
+    // pixel, else union-find parent; the root after flattening), w component id
+    // (of a root)
+    int4* px;
     int* comp;      // 4 per component: x0, x1, y0, y1 (view coordinates)
     int* root;      // root pixel of a component
     int* counters;  // [0] components, [2] work counter of the small-component pass, [3] strip tasks,
                     // [4] / [5] tickets of the forward / backward strip sweeps, [6] components left to
                     // the one-block solver, [7] strip carry slots in use, [8] tiles with holes
     uint8_t* thole; // [view][tile row][tile column] of 32 x 32 pixel tiles: 1 when the tile has a hole;
-                    // label, dist and src are stored only in such tiles
+                    // the px records are stored only in such tiles
     int* htiles;    // the tiles with holes (index into thole), in no particular order
     int ttx, tpv;   // tiles per row and per view of the thole grid (sized for the largest view)
     const uint8_t* rflags;  // raster tile flags of the views (bit 1: a hole or a deferred pixel), or

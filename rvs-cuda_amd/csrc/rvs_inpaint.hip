@@ -41,21 +41,24 @@ constexpr int kSweeps = 2;   // forward, backward: the fixed point of the altern
 
 __device__ inline int ld(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
-__device__ inline int find_root(const int* label, int p) {
+// Label (union-find parent) of record p.
+__device__ inline int* label_of(int4* px, int p) { return &px[p].z; }
+
+__device__ inline int find_root(int4* px, int p) {
     while (true) {
-        const int q = ld(&label[p]);
+        const int q = ld(label_of(px, p));
         if (q == p) return p;
         p = q;
     }
 }
 
-__device__ inline void unite(int* label, int a, int b) {
+__device__ inline void unite(int4* px, int a, int b) {
     while (true) {
-        a = find_root(label, a);
-        b = find_root(label, b);
+        a = find_root(px, a);
+        b = find_root(px, b);
         if (a == b) return;
         if (a < b) { const int t = a; a = b; b = t; }
-        const int old = atomicMin(&label[a], b);
+        const int old = atomicMin(label_of(px, a), b);
         if (old == a) return;
         a = old;
     }
@@ -129,8 +132,8 @@ __global__ void __launch_bounds__(256) k_ccl_local(const ViewDesc* __restrict__ 
             const int p = y * w + x, gp = base + p;
             hole = !(vd.quality[p] > 0.f);   // View::get_inpaint_mask (View.cpp:198-203)
             if (hole) {
-                ws.dist[gp] = w + h;
-                ws.src[gp] = gp;
+                ws.px[gp].x = w + h;
+                ws.px[gp].y = gp;
             }
         }
         lab[k] = hole ? k : -1;
@@ -147,7 +150,7 @@ __global__ void __launch_bounds__(256) k_ccl_local(const ViewDesc* __restrict__ 
     // a known pixel's map value is (itself, 0), implied by its label
     for (int k = tid; k < kCclTile * kCclTile; k += 256) {
         const int x = tx0 + (k % kCclTile), y = ty0 + (k / kCclTile);
-        if (lab[k] < 0 && x < w && y < h) ws.label[base + y * w + x] = -1;
+        if (lab[k] < 0 && x < w && y < h) ws.px[base + y * w + x].z = -1;
     }
     for (int k = tid; k < kCclTile * kCclTile; k += 256) {
         if (lab[k] < 0) continue;
@@ -160,7 +163,7 @@ __global__ void __launch_bounds__(256) k_ccl_local(const ViewDesc* __restrict__ 
         if (lab[k] < 0) continue;
         const int r = lfind(lab, k);
         const int x = tx0 + (k % kCclTile), y = ty0 + (k / kCclTile);
-        ws.label[base + y * w + x] = base + (ty0 + r / kCclTile) * w + (tx0 + r % kCclTile);
+        ws.px[base + y * w + x].z = base + (ty0 + r / kCclTile) * w + (tx0 + r % kCclTile);
     }
 }
 
@@ -197,7 +200,7 @@ __global__ void __launch_bounds__(kMergeThreads) k_ccl_merge(const ViewDesc* __r
         if (nx >= vd.w || ny >= vd.h) continue;
         const int gp = (int)vd.px_base + y * vd.w + x, gq = (int)vd.px_base + ny * vd.w + nx;
         const bool nh = ws.thole[H.v * ws.tpv + (ny / kCclTile) * ws.ttx + nx / kCclTile] != 0;
-        if (nh && ws.label[gp] >= 0 && ws.label[gq] >= 0) unite(ws.label, gp, gq);
+        if (nh && ws.px[gp].z >= 0 && ws.px[gq].z >= 0) unite(ws.px, gp, gq);
     }
 }
 
@@ -210,12 +213,12 @@ __global__ void __launch_bounds__(256) k_ccl_flatten(const ViewDesc* __restrict_
             const int x = H.tx0 + (k % kCclTile), y = H.ty0 + (k / kCclTile);
             if (x >= vd.w || y >= vd.h) continue;
             const int gp = (int)vd.px_base + y * vd.w + x;
-            if (ws.label[gp] < 0) continue;
-            const int r = find_root(ws.label, gp);
-            ws.label[gp] = r;
+            if (ws.px[gp].z < 0) continue;
+            const int r = find_root(ws.px, gp);
+            ws.px[gp].z = r;
             if (r == gp) {
                 const int id = atomicAdd(&ws.counters[0], 1);
-                ws.cid[gp] = id;
+                ws.px[gp].w = id;
                 ws.comp[4 * id + 0] = INT_MAX;
                 ws.comp[4 * id + 1] = INT_MIN;
                 ws.comp[4 * id + 2] = INT_MAX;
@@ -251,8 +254,8 @@ __global__ void __launch_bounds__(kBboxThreads) k_comp_bbox(const ViewDesc* __re
         const int x = H.tx0 + l % kCclTile, y = H.ty0 + l / kCclTile;
         const bool in = x < vd.w && y < vd.h;
         const int gp = (int)vd.px_base + (in ? y * vd.w + x : 0);
-        const int r = in ? ws.label[gp] : -1;
-        const int id = r >= 0 ? ws.cid[r] : -1;
+        const int r = in ? ws.px[gp].z : -1;
+        const int id = r >= 0 ? ws.px[r].w : -1;
         const uint64_t act = __ballot(id >= 0);
         if (act == 0ull) continue;
         if (lid < 0) lid = __shfl(id, __ffsll((unsigned long long)act) - 1);
@@ -355,7 +358,7 @@ __device__ inline bool tile_has_hole(const Region& R, const InpaintWS& ws, int x
 // The label of view pixel (x, y) at global index q: -1 (known) in a tile
 // without holes, which stores none; both words are loaded unconditionally.
 __device__ inline int label_at(const Region& R, const InpaintWS& ws, int x, int y, int q) {
-    const int l = ws.label[q];
+    const int l = ws.px[q].z;
     return tile_has_hole(R, ws, x, y) ? l : -1;
 }
 
@@ -374,7 +377,7 @@ __device__ inline void region_get(const Region& R, const InpaintWS& ws, int x, i
         const int lab = label_at(R, ws, x, y, p);
         if (lab < 0) { v = 0; s = p; }
         else if (lab != R.root) { v = kOther; s = -1; }
-        else { v = ws.dist[p]; s = ws.src[p]; }
+        else { const int4 m = ws.px[p]; v = m.x; s = m.y; }
     }
 }
 
@@ -385,7 +388,7 @@ __device__ inline void region_put(const Region& R, const InpaintWS& ws, int x, i
         R.ldist[k] = v; R.lsrc[k] = s;
     } else {
         const int p = R.base + y * R.w + x;
-        ws.dist[p] = v; ws.src[p] = s;
+        *reinterpret_cast<int2*>(&ws.px[p]) = make_int2(v, s);
     }
 }
 
@@ -534,7 +537,7 @@ __device__ void solve_big(const Region& R, const InpaintWS& ws, Fn* wave_tot, in
                 }
             }
             for (int cb = 0; cb < llen; cb += span) {
-                // loads: kBigPx pixels x (itself + 4 neighbours) x (label, dist, src)
+                // loads: kBigPx pixels x (itself + 4 neighbours) x one record each
                 int lab[kBigPx][5], dd[kBigPx][5], ss[kBigPx][5], pp[kBigPx][5];
                 bool ok[kBigPx][5];
 #pragma unroll
@@ -549,9 +552,10 @@ __device__ void solve_big(const Region& R, const InpaintWS& ws, Fn* wave_tot, in
                         ok[k][n] = nx >= 0 && nx < R.w && ny >= 0 && ny < R.h;
                         const int q = R.base + (ok[k][n] ? ny * R.w + nx : y * R.w + x);
                         pp[k][n] = q;
-                        lab[k][n] = label_at(R, ws, ok[k][n] ? nx : x, ok[k][n] ? ny : y, q);
-                        dd[k][n] = ws.dist[q];
-                        ss[k][n] = ws.src[q];
+                        const int4 m = ws.px[q];
+                        lab[k][n] = tile_has_hole(R, ws, ok[k][n] ? nx : x, ok[k][n] ? ny : y) ? m.z : -1;
+                        dd[k][n] = m.x;
+                        ss[k][n] = m.y;
                     }
                 }
                 Fn f[kBigPx];
@@ -618,8 +622,7 @@ __device__ void solve_big(const Region& R, const InpaintWS& ws, Fn* wave_tot, in
                     if (e <= last) {
                         if (cur[k] > 0 && Sv != cur[k]) {
                             changed = true;
-                            ws.dist[pp[k][0]] = Sv;
-                            ws.src[pp[k][0]] = Ss;
+                            *reinterpret_cast<int2*>(&ws.px[pp[k][0]]) = make_int2(Sv, Ss);
                         }
                         if (e == last) { s_carry[0] = Sv; s_carry[1] = Ss; }
                     }
@@ -669,7 +672,8 @@ __device__ void solve_lines(const Region& R, const InpaintWS& ws, bool cols, Fn*
         auto load = [&](int x, int y, int& v, int& sr) {
             const bool in = inside(x, y);
             const int q = R.base + (in ? y * R.w + x : 0);
-            map_value(R, label_at(R, ws, in ? x : 0, in ? y : 0, q), ws.dist[q], ws.src[q], q, v, sr);
+            const int4 m = ws.px[q];
+            map_value(R, tile_has_hole(R, ws, in ? x : 0, in ? y : 0) ? m.z : -1, m.x, m.y, q, v, sr);
         };
         bool changed = false;
         // line coordinate of processing step li (-1 / nlines: outside the box)
@@ -777,8 +781,7 @@ __device__ void solve_lines(const Region& R, const InpaintWS& ws, bool cols, Fn*
                     at(lc, e, x, y);
                     const int q = R.base + y * R.w + x;
                     changed = true;
-                    ws.dist[q] = Sv;
-                    ws.src[q] = Ss;
+                    *reinterpret_cast<int2*>(&ws.px[q]) = make_int2(Sv, Ss);
                 }
                 pv[k] = member ? Sv : sv[k];
                 ps[k] = member ? Ss : ss[k];
@@ -905,9 +908,10 @@ __device__ void solve_strip(const Region& R, const InpaintWS& ws, bool cols, boo
     auto fetch = [&](int lc, int e, RawPx& r) {
         int q, ti;
         addr(lc, e, q, ti);
-        r.lab = ws.label[q];
-        r.d = ws.dist[q];
-        r.s = ws.src[q];
+        const int4 m = ws.px[q];
+        r.lab = m.z;
+        r.d = m.x;
+        r.s = m.y;
         r.th = ws.thole[ti];
     };
     auto value = [&](int lc, int e, const RawPx& r, int& v, int& sr) {
@@ -1013,8 +1017,7 @@ __device__ void solve_strip(const Region& R, const InpaintWS& ws, bool cols, boo
             if (member && Sv != sv[j]) {
                 int q, ti;
                 addr(lc, e, q, ti);
-                ws.dist[q] = Sv;
-                ws.src[q] = Ss;
+                *reinterpret_cast<int2*>(&ws.px[q]) = make_int2(Sv, Ss);
             }
             if (out_slots && e == e1 - 1)
                 __hip_atomic_store(&out_slots[li], pack_carry(Sv, Ss), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1195,8 +1198,9 @@ __global__ void __launch_bounds__(256) k_inp_solve(const ViewDesc* __restrict__ 
                 const int x = R.x0 + k % R.bw, y = R.y0 + k / R.bw;
                 const int p = R.base + y * R.w + x;
                 const int lab = label_at(R, ws, x, y, p);
-                ldist[k] = lab < 0 ? 0 : (lab == R.root ? ws.dist[p] : kOther);
-                lsrc[k] = lab < 0 ? p : (lab == R.root ? ws.src[p] : -1);
+                const int4 m = ws.px[p];
+                ldist[k] = lab < 0 ? 0 : (lab == R.root ? m.x : kOther);
+                lsrc[k] = lab < 0 ? p : (lab == R.root ? m.y : -1);
             }
             __syncthreads();
             solve_component<true>(R, ws, wave_tot, s_carry);
@@ -1204,7 +1208,8 @@ __global__ void __launch_bounds__(256) k_inp_solve(const ViewDesc* __restrict__ 
             for (int k = tid; k < area; k += 256) {
                 const int x = R.x0 + k % R.bw, y = R.y0 + k / R.bw;
                 const int p = R.base + y * R.w + x;
-                if (label_at(R, ws, x, y, p) == R.root) { ws.dist[p] = ldist[k]; ws.src[p] = lsrc[k]; }
+                if (label_at(R, ws, x, y, p) == R.root)
+                    *reinterpret_cast<int2*>(&ws.px[p]) = make_int2(ldist[k], lsrc[k]);
             }
         } else {
             solve_big(R, ws, wave_tot, s_carry);
@@ -1252,8 +1257,9 @@ __global__ void __launch_bounds__(256) k_inp_apply(const ViewDesc* __restrict__ 
             const int x = H.tx0 + (k % kCclTile), y = H.ty0 + (k / kCclTile);
             if (x >= vd.w || y >= vd.h) continue;
             const int p = y * vd.w + x, gp = (int)vd.px_base + p;
-            if (ws.label[gp] < 0) continue;
-            const int s = ws.src[gp] - (int)vd.px_base;
+            const int4 m = ws.px[gp];
+            if (m.z < 0) continue;
+            const int s = m.y - (int)vd.px_base;
             if (s != p) {
                 float* color = vd.color;
                 color[3 * p] = color[3 * s];
@@ -1270,11 +1276,12 @@ __global__ void k_inp_map(const ViewDesc* __restrict__ views, InpaintWS ws, int3
     RVS_VIEW_PIXEL(vd, p, gp)
     const int x = p % vd.w, y = p / vd.w;
     const bool th = ws.thole[blockIdx.y * ws.tpv + (y / kCclTile) * ws.ttx + x / kCclTile] != 0;
-    const bool hole = th && ws.label[gp] >= 0;
-    const int s = hole ? ws.src[gp] - (int)vd.px_base : p;
+    const int4 m = ws.px[gp];
+    const bool hole = th && m.z >= 0;
+    const int s = hole ? m.y - (int)vd.px_base : p;
     map_out[3 * p] = s % vd.w;
     map_out[3 * p + 1] = s / vd.w;
-    map_out[3 * p + 2] = hole ? ws.dist[gp] : 0;
+    map_out[3 * p + 2] = hole ? m.x : 0;
 }
 
 namespace {
@@ -1314,11 +1321,8 @@ InpaintWS inpaint_ws_carve(void* base, size_t npx, int nviews, int max_w, int ma
     ws.slots[0] = (unsigned long long*)b; b += a256(slot_cap_of(npx, nviews) * sizeof(unsigned long long));
     ws.slots[1] = (unsigned long long*)b; b += a256(slot_cap_of(npx, nviews) * sizeof(unsigned long long));
     ws.legacy = (int*)b; b += a256(4 * legacy_cap_of(npx, nviews));
+    ws.px = (int4*)b; b += 16 * npx;
     int* p = (int*)b;
-    ws.label = p; p += npx;
-    ws.dist = p; p += npx;
-    ws.src = p; p += npx;
-    ws.cid = p; p += npx;
     ws.comp = p; p += 4 * maxc;
     ws.root = p; p += maxc;
     ws.counters = p;
