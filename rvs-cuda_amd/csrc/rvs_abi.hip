@@ -132,7 +132,8 @@ struct Layout {
 // deferred-pixel flags, deferred-tile list and binning work list.
 struct RasterPlan {
     rvs_k::BatchDims d;
-    size_t status, counts, offsets, fill, lists, ranges, vbuf, tvq, imap, bsum, deferred, dtiles, work, tile_enc;
+    size_t status, counts, offsets, fill, lists, ranges, vbuf, tvq, imap, bsum, deferred, dtiles, work, tile_enc,
+        rbuf;
 };
 
 // List capacity: the caller's, or one entry per output pixel and source
@@ -195,6 +196,20 @@ void plan_raster(Layout& L, RasterPlan& R, int nsrc, int nviews, uint32_t ntiles
     R.dtiles = L.take((size_t)ntiles * sizeof(uint32_t));
     R.work = L.take((size_t)items * sizeof(uint32_t));
     R.tile_enc = L.take((size_t)ntiles);
+    // the in-order replay's per-source results of its first rbuf_tiles tiles
+    // (tiles beyond them take the one-block-per-tile replay): a sixty-fourth
+    // of the pixels, five floats per source
+    const uint64_t tile_px = (uint64_t)1 << (2 * shift);
+    uint64_t rt = (uint64_t)npx / (64 * tile_px);
+    rt = rt < 16 ? 16 : rt;
+    // RVS_REPLAY_TILES=<n> lowers it (0: every deferred tile in order; the
+    // results are the same)
+    if (const char* env = std::getenv("RVS_REPLAY_TILES")) {
+        const long long n = std::strtoll(env, nullptr, 10);
+        if (n >= 0 && (uint64_t)n < rt) rt = (uint64_t)n;
+    }
+    d.rbuf_tiles = (uint32_t)(rt < ntiles ? rt : ntiles);
+    R.rbuf = L.take((size_t)d.rbuf_tiles * nsrc * tile_px * 5 * sizeof(float));
 }
 
 rvs_k::RasterWS carve_raster(char* base, const RasterPlan& R) {
@@ -213,6 +228,7 @@ rvs_k::RasterWS carve_raster(char* base, const RasterPlan& R) {
     ws.deferred_tiles = (uint32_t*)(base + R.dtiles);
     ws.work = (uint32_t*)(base + R.work);
     ws.tile_enc = (uint8_t*)(base + R.tile_enc);
+    ws.rbuf = (float*)(base + R.rbuf);
     return ws;
 }
 

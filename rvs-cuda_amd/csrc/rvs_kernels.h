@@ -79,6 +79,8 @@ struct RasterWS {
     uint32_t* work;          // binning work items (view, source, block)
     uint8_t* tile_enc;       // [global tile] bit 0: encoded by k_raster; bit 1: a hole or a deferred
                              // pixel (blending launches); written by every tile
+    float* rbuf;             // [rbuf_tiles][source][tile pixel][5] the replay's per-source colour,
+                             // depth and validity of deferred pixels
     uint32_t* status;        // [0] n_deferred, [1] list overflow, [2] entries (saturated), [3] inpainting
                              // wait gave up, [4] work items
 };
@@ -94,6 +96,7 @@ struct BatchDims {
     int tile_shift;          // log2 of the tile edge: 4 (16 x 16 tiles, 256-thread raster blocks) or
                              // 5 (32 x 32, 1024-thread blocks)
     uint32_t flat_blocks;    // blocks of the passes over the kept work items
+    uint32_t rbuf_tiles;     // deferred tiles whose sources are replayed in parallel (RasterWS::rbuf)
 };
 
 enum { MODE_BLEND = 0, MODE_RAW = 1 };

@@ -1427,16 +1427,19 @@ __device__ void replay_source(const G& g, uint32_t* L, uint32_t n, int tx0, int 
     }
 }
 
+// Deferred tiles [first, n_deferred): one block per tile folds every source
+// in order and blends (the tiles before `first` take the per-source pair of
+// kernels below).
 template <class G, int MODE>
 __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
-                                                  BatchDims d, RasterWS ws) {
+                                                  BatchDims d, RasterWS ws, uint32_t first) {
     __shared__ uint32_t sorted[kFallbackSortCap];
     __shared__ ReplayEntry rec[kFallbackChunk];
     __shared__ uint8_t dmask[kTile * kTile];
     __shared__ uint32_t pool_n[1];   // candidate records
     const int tid = threadIdx.x;
     const uint32_t ndef = ws.status[1] ? 0u : ws.status[0];
-    for (uint32_t ti = blockIdx.x; ti < ndef; ti += gridDim.x) {
+    for (uint32_t ti = first + blockIdx.x; ti < ndef; ti += gridDim.x) {
         const uint32_t T = ws.deferred_tiles[ti];
         const int v = find_view(views, d.nviews, T);
         const ViewDesc& vd = views[v];
@@ -1494,6 +1497,95 @@ __global__ void __launch_bounds__(256) k_fallback(const G* __restrict__ geoms, c
     }
 }
 
+// The first d.rbuf_tiles deferred tiles, sources in parallel: a block per
+// (tile, source) folds that source's bin for the tile's deferred pixels (the
+// same replay_source and resolve as k_fallback) and stores each pixel's
+// colour, depth and validity in ws.rbuf; k_fallback_blend then blends them in
+// source order.  A tile's chain of sources is the replay's critical path when
+// few tiles are deferred.
+template <class G>
+__global__ void __launch_bounds__(256) k_fallback_src(const G* __restrict__ geoms,
+                                                      const ViewDesc* __restrict__ views, BatchDims d, RasterWS ws) {
+    __shared__ uint32_t sorted[kFallbackSortCap];
+    __shared__ ReplayEntry rec[kFallbackChunk];
+    __shared__ uint8_t dmask[kTile * kTile];
+    __shared__ uint32_t pool_n[1];
+    const int tid = threadIdx.x;
+    const uint32_t ndef = ws.status[1] ? 0u : min(ws.status[0], d.rbuf_tiles);
+    const int edge = 1 << d.tile_shift;
+    const int nsub = edge / kTile;
+    for (uint32_t job = blockIdx.x; job < ndef * (uint32_t)d.nsrc; job += gridDim.x) {
+        const uint32_t ti = job / (uint32_t)d.nsrc;
+        const int s = (int)(job % (uint32_t)d.nsrc);
+        const uint32_t T = ws.deferred_tiles[ti];
+        const int v = find_view(views, d.nviews, T);
+        const ViewDesc& vd = views[v];
+        const G& g = geoms[(size_t)v * d.nsrc + s];
+        const int tile = (int)(T - vd.tile_base);
+        float* out = ws.rbuf + ((size_t)ti * d.nsrc + s) * ((size_t)edge * edge) * 5;
+        for (int sub = 0; sub < nsub * nsub; ++sub) {
+            const int tx0 = (tile % vd.tiles_x) * edge + (sub % nsub) * kTile;
+            const int ty0 = (tile / vd.tiles_x) * edge + (sub / nsub) * kTile;
+            const int px = tx0 + (tid % kTile), py = ty0 + (tid / kTile);
+            const bool inside = px < vd.w && py < vd.h;
+            const size_t o = inside ? (size_t)py * vd.w + px : 0;
+            const bool mine = inside && ws.deferred[vd.px_base + o];
+            dmask[tid] = mine ? 1 : 0;
+            if (!__syncthreads_or(mine ? 1 : 0)) continue;
+            const size_t bin = (size_t)T * d.nsrc + s;
+            FoldState fs;
+            replay_source(g, ws.lists + ws.offsets[bin], ws.counts[bin], tx0, ty0, vd.w, vd.h, mine, px, py, dmask,
+                          sorted, rec, pool_n, fs, ws, d, (uint32_t)(v * d.nsrc + s));
+            if (mine) {
+                float3 c = make_float3(0.f, 0.f, 0.f);
+                float dep = __builtin_inff(), shape = 0.f;
+                if (fs.slot >= 0) {
+                    float dd, sh;
+                    int i, j;
+                    unpack_ij(fs.ij, i, j);
+                    resolve<false>(g, i, j, fs.slot, px, py, c, dd, sh, vd.w, vd.h, nullptr);
+                    dep = fs.new_d;
+                    shape = fs.shape;
+                }
+                float* r = out + (size_t)(sub * kTile * kTile + tid) * 5;
+                r[0] = c.x; r[1] = c.y; r[2] = c.z; r[3] = dep; r[4] = shape;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <class G>
+__global__ void __launch_bounds__(256) k_fallback_blend(const ViewDesc* __restrict__ views, BatchDims d,
+                                                        RasterWS ws) {
+    const int tid = threadIdx.x;
+    const uint32_t ndef = ws.status[1] ? 0u : min(ws.status[0], d.rbuf_tiles);
+    const int edge = 1 << d.tile_shift;
+    const int nsub = edge / kTile;
+    for (uint32_t ti = blockIdx.x; ti < ndef; ti += gridDim.x) {
+        const uint32_t T = ws.deferred_tiles[ti];
+        const int v = find_view(views, d.nviews, T);
+        const ViewDesc& vd = views[v];
+        const int tile = (int)(T - vd.tile_base);
+        for (int sub = 0; sub < nsub * nsub; ++sub) {
+            const int px = (tile % vd.tiles_x) * edge + (sub % nsub) * kTile + (tid % kTile);
+            const int py = (tile / vd.tiles_x) * edge + (sub / nsub) * kTile + (tid / kTile);
+            if (px >= vd.w || py >= vd.h) continue;
+            const size_t o = (size_t)py * vd.w + px;
+            if (!ws.deferred[vd.px_base + o]) continue;
+            BlendState st;
+            for (int s = 0; s < d.nsrc; ++s) {
+                const float* r = ws.rbuf + (((size_t)ti * d.nsrc + s) * ((size_t)edge * edge) +
+                                            (size_t)(sub * kTile * kTile + tid)) * 5;
+                const float3 c = make_float3(r[0], r[1], r[2]);
+                if (s == 0) blend_first(st, c, r[3], r[4]);
+                else blend_step(st, c, r[3], r[4], vd.blend_exp, vd.blend_inv);
+            }
+            write_pixel<MODE_BLEND>(vd, o, st, make_float3(0.f, 0.f, 0.f), 0.f, 0.f);
+        }
+    }
+}
+
 template <class G, int MODE>
 hipError_t launch_raster(const G* geoms, const ViewDesc* views, const BatchDims& d, const RasterWS& ws,
                          hipStream_t s) {
@@ -1519,7 +1611,17 @@ template <class G, int MODE>
 hipError_t launch_replay(const G* geoms, const ViewDesc* views, const BatchDims& d, const RasterWS& ws,
                          hipStream_t s) {
     const unsigned fb_blocks = d.ntiles < (uint32_t)kFallbackBlocks ? d.ntiles : (uint32_t)kFallbackBlocks;
-    hipLaunchKernelGGL((k_fallback<G, MODE>), dim3(fb_blocks), dim3(256), 0, s, geoms, views, d, ws);
+    uint32_t first = 0;
+    if (MODE == MODE_BLEND && d.nsrc > 1 && d.rbuf_tiles > 0) {
+        // the first rbuf_tiles deferred tiles: sources in parallel, then the blend
+        const uint64_t jobs = (uint64_t)d.rbuf_tiles * (uint64_t)d.nsrc;
+        const unsigned sb = (unsigned)(jobs < (uint64_t)kFallbackBlocks * 4 ? jobs : (uint64_t)kFallbackBlocks * 4);
+        hipLaunchKernelGGL((k_fallback_src<G>), dim3(sb), dim3(256), 0, s, geoms, views, d, ws);
+        const unsigned bb = d.rbuf_tiles < (uint32_t)kFallbackBlocks ? d.rbuf_tiles : (uint32_t)kFallbackBlocks;
+        hipLaunchKernelGGL((k_fallback_blend<G>), dim3(bb), dim3(256), 0, s, views, d, ws);
+        first = d.rbuf_tiles;
+    }
+    hipLaunchKernelGGL((k_fallback<G, MODE>), dim3(fb_blocks), dim3(256), 0, s, geoms, views, d, ws, first);
     return hipGetLastError();
 }
 

@@ -353,3 +353,43 @@ def test_binning_beyond_the_kept_range(torch_cuda, monkeypatch, tile):
         for k in range(len(tg)):
             for key in ("color", "quality", "depth", "validity", "y", "u", "v"):
                 assert torch.equal(outs[k][key], ref[k][key]), "%s view %d map %s" % (tag, k, key)
+
+
+@pytest.mark.parametrize("tile", [16, 32])
+def test_replay_split_equals_in_order(torch_cuda, monkeypatch, tile):
+    """Deferred tiles replay sources in parallel (a block per tile and source,
+    then a blend) up to a workspace limit and one block per tile beyond it:
+    RVS_REPLAY_TILES=0 sends every tile the in-order way, =1 all but the first;
+    every map equals the default run's."""
+    torch = torch_cuda
+    say("case replay split, tile %d" % tile)
+    cases = []
+    rg = W.rig("erp4_1080p", scale=4)
+    srcs = W.device_sources(rg, W.render_sources(rg))
+    cases.append(("erp4/4", srcs, W.batch_targets(rg, 4, seed=5, scale=4)))
+    rg_b = W.rig("persp11_2048", scale=4)
+    srcs_b = W.device_sources(rg_b, W.render_sources(rg_b))
+    cases.append(("persp11/2", srcs_b, [W.device_target(rg_b, [0.01, 0.03, -0.02], [5.0, 2.0, 0.0]),
+                                         W.device_target(rg_b, [-0.02, 0.0, 0.03], [-4.0, 1.0, 0.0])]))
+    for tag, s, tg in cases:
+        monkeypatch.delenv("RVS_REPLAY_TILES", raising=False)
+        R.profile_enable(True)
+        try:
+            ref = [{k: v.clone() for k, v in o.items() if v is not None}
+                   for o in R.BatchSynthesizer(s, tg, want_maps=True, tile=tile).run()]
+            torch.cuda.synchronize()
+            cnt = R.profile_counters()
+        finally:
+            R.profile_enable(False)
+        say("  %s, %d views, %d deferred tiles" % (tag, len(tg), cnt["replay_tiles"]))
+        assert cnt["replay_tiles"] > 1
+        for lim in ("0", "1"):
+            monkeypatch.setenv("RVS_REPLAY_TILES", lim)
+            try:
+                outs = R.BatchSynthesizer(s, tg, want_maps=True, tile=tile).run()
+                torch.cuda.synchronize()
+            finally:
+                monkeypatch.delenv("RVS_REPLAY_TILES", raising=False)
+            for k in range(len(tg)):
+                for key in ("color", "quality", "depth", "validity", "y", "u", "v"):
+                    assert torch.equal(outs[k][key], ref[k][key]), "%s limit %s view %d map %s" % (tag, lim, k, key)
