@@ -189,8 +189,11 @@ void plan_raster(Layout& L, RasterPlan& R, int nsrc, int nviews, uint32_t ntiles
     R.bsum = L.take(rvs_k::scan_blocks(nbins) * sizeof(unsigned long long));
     R.lists = L.take((size_t)d.cap * sizeof(uint32_t));
     R.ranges = L.take((size_t)d.range_cap * 256 * sizeof(uint32_t));
-    R.vbuf = L.take((size_t)d.range_cap * 3 * rvs_k::kBinVerts * sizeof(float));
-    R.tvq = L.take((size_t)d.range_cap * 256 * sizeof(float2));
+    d.kept_geom = 1;
+    if (const char* env = std::getenv("RVS_KEPT_GEOM")) d.kept_geom = std::strtol(env, nullptr, 10) != 0;
+    const size_t kg = d.kept_geom ? d.range_cap : 0;
+    R.vbuf = L.take(kg * 3 * rvs_k::kBinVerts * sizeof(float));
+    R.tvq = L.take(kg * 256 * sizeof(float2));
     R.imap = L.take((size_t)items * sizeof(uint32_t));
     R.deferred = L.take(npx);
     R.dtiles = L.take((size_t)ntiles * sizeof(uint32_t));

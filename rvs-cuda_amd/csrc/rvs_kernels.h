@@ -97,6 +97,7 @@ struct BatchDims {
                              // 5 (32 x 32, 1024-thread blocks)
     uint32_t flat_blocks;    // blocks of the passes over the kept work items
     uint32_t rbuf_tiles;     // deferred tiles whose sources are replayed in parallel (RasterWS::rbuf)
+    uint32_t kept_geom;      // kept work items also store vertex windows and validities (vbuf, tvq)
 };
 
 enum { MODE_BLEND = 0, MODE_RAW = 1 };

@@ -394,7 +394,9 @@ __device__ inline float3 kept_vertex(const float* vb, int lr, int lc) {
 }
 
 // Work index of the kept work item holding quad (i, j) of pair vs, or ~0.
+// (always ~0 when kept items store no vertex windows)
 __device__ inline uint32_t kept_index(const RasterWS& ws, const BatchDims& d, uint32_t vs, int i, int j) {
+    if (!d.kept_geom) return 0xffffffffu;
     const uint32_t item = (uint32_t)(i / kBinQY) * d.bins_x + (uint32_t)(j / kBinQX);
     return ws.imap[(size_t)vs * (d.bins_x * d.bins_y) + item];
 }
@@ -540,6 +542,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
         const int i0 = (int)(item / d.bins_x) * kBinQY, j0 = (int)(item % d.bins_x) * kBinQX;
         const int i = i0 + tid / kBinQX, j = j0 + tid % kBinQX;
         const bool stored = !FILL && wi < d.range_cap;   // same for the whole block
+        const bool stored_geom = stored && d.kept_geom;
         if (tid < kBinTable) {
             t_key[tid] = kTableEmpty;
             t_n[tid] = 0;
@@ -549,7 +552,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
         // a kept item also records its quads' triangle validities for the raster
         // (exact; this pass waits on memory far more than it computes)
         auto quad_range = [&](auto vtx) {
-            if (stored) {
+            if (stored_geom) {
                 float2 tv;
                 more = quad_range_tv(g, i, j, vtx, vd.w, vd.h, d.tile_shift, tx0, tx1, ty0, ty1, &tv);
                 ws.tvq[(size_t)wi * 256 + tid] = tv;
@@ -558,7 +561,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
             }
         };
         {
-            float* vb = stored ? ws.vbuf + (size_t)wi * (3 * kBinVerts) : nullptr;
+            float* vb = stored_geom ? ws.vbuf + (size_t)wi * (3 * kBinVerts) : nullptr;
             for (int k = tid; k < kBinVerts; k += 256) {
                 const int rr = k / (kBinQX + 1), c = k % (kBinQX + 1);
                 const int ii = i0 + rr, jj = j0 + c;
@@ -657,12 +660,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
         int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
         if (r == kRangeBig) {
             const G& g = geoms[vs];
-            const float* vb = ws.vbuf + (size_t)wi * (3 * kBinVerts);
+            const float* vb = d.kept_geom ? ws.vbuf + (size_t)wi * (3 * kBinVerts) : nullptr;
             Quad Q;
             quad_build(Q, g, i, j,
                        [&](int ii, int jj) -> float3 {
                            const int rr = ii - i0, c = jj - j0;
-                           if (rr >= 0 && rr <= kBinQY && c >= 0 && c <= kBinQX) return kept_vertex(vb, rr, c);
+                           if (vb && rr >= 0 && rr <= kBinQY && c >= 0 && c <= kBinQX) return kept_vertex(vb, rr, c);
                            return vertex(g, ii, jj);
                        },
                        vd.w, vd.h, false);

@@ -393,3 +393,30 @@ def test_replay_split_equals_in_order(torch_cuda, monkeypatch, tile):
             for k in range(len(tg)):
                 for key in ("color", "quality", "depth", "validity", "y", "u", "v"):
                     assert torch.equal(outs[k][key], ref[k][key]), "%s limit %s view %d map %s" % (tag, lim, k, key)
+
+
+def test_without_kept_geometry(torch_cuda, monkeypatch):
+    """RVS_KEPT_GEOM=0: kept work items store only their tile ranges (no
+    vertex windows or validities), and the raster, the fill pass of wide
+    ranges and the replay evaluate vertices themselves: every map equals the
+    default run's."""
+    torch = torch_cuda
+    say("case without kept geometry")
+    rg = W.rig("persp11_2048", scale=4)
+    s = W.device_sources(rg, W.render_sources(rg))
+    tg = [W.device_target(rg, [0.01, 0.03, -0.02], [5.0, 2.0, 0.0]),
+          R.make_target(R.cam(1, 512, 256), [0.05, 0.02, 0.0], [20.0, 0.0, 0.0], 10, 1.0,
+                        R.scale_of(10), R.scale_of(16))]
+    monkeypatch.delenv("RVS_KEPT_GEOM", raising=False)
+    ref = [{k: v.clone() for k, v in o.items() if v is not None}
+           for o in R.BatchSynthesizer(s, tg, want_maps=True).run()]
+    torch.cuda.synchronize()
+    monkeypatch.setenv("RVS_KEPT_GEOM", "0")
+    try:
+        outs = R.BatchSynthesizer(s, tg, want_maps=True).run()
+        torch.cuda.synchronize()
+    finally:
+        monkeypatch.delenv("RVS_KEPT_GEOM", raising=False)
+    for k in range(len(tg)):
+        for key in ("color", "quality", "depth", "validity", "y", "u", "v"):
+            assert torch.equal(outs[k][key], ref[k][key]), "view %d map %s" % (k, key)
