@@ -31,8 +31,8 @@ constexpr int kBinBlocks = 2048;    // persistent binning blocks (work items pas
 constexpr int kFlatBlocks = 65536;  // persistent blocks of the passes over the kept work items
 constexpr int kRasterGroup = 4;     // sources rasterised together in k_raster's one-phase instantiation
 constexpr int kRasterGroupMulti = 4; // sources per phase of the looping instantiation (more sources)
-constexpr int kRasterCache = 1024;  // binned quads per phase whose coverage-pass results stay in LDS
-constexpr int kRasterCacheOne = 640; // the same for launch sets of up to kRasterGroup sources (one phase)
+constexpr int kRasterCache = 1536;  // binned quads per phase whose coverage-pass results stay in LDS
+constexpr int kRasterCacheOne = 960; // the same for launch sets of up to kRasterGroup sources (one phase)
 constexpr int kMaxSourceSide = 65536;  // bins hold quads as (row << 16) | column; also the
                                        // largest target side (tile coordinates fit 16 bits)
 constexpr int kFallbackBlocks = 1024;

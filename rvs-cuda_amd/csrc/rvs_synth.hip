@@ -961,14 +961,15 @@ __device__ __forceinline__ float raster_tri(float3 a, float3 b, float3 c, uint32
     return T.tv;
 }
 
-// Per-phase LDS copy of what the coverage pass computed for each binned quad:
-// the validity of triangles T1 and T2 and the quad code (the corner vertices
-// are read again from the kept vertex blocks).
+// Per-phase LDS copy of what the coverage pass looked up for each binned
+// quad: the quad code and its kept work item (~0: none), so that the winners'
+// corner vertices and validities are read from the kept blocks without the
+// work-item map.
 template <int CAP>
 struct RasterCache {
     static constexpr int cap = CAP;
-    float tv[2][CAP];
     uint32_t ij[CAP];
+    uint32_t wi[CAP];
 };
 
 // Candidate ids: (entry << 2) | slot, entry = index of the quad in the phase's
@@ -996,18 +997,17 @@ __device__ __forceinline__ void raster_quad(const G& g, uint32_t ij, uint32_t e,
     const float3 v01 = vtx(0, 1), v10 = vtx(1, 0);
     const bool gate = v01.z > 0.f && v10.z > 0.f && v01.x > 0.f && v10.x > 0.f;
     const uint32_t s0 = e << 2;
-    float tv0 = 0.f, tv1 = 0.f;
     {
         const float3 v00 = vtx(0, 0);
         if (gate && v00.z > 0.f && v00.x > 0.f)
-            tv0 = raster_tri<TILE>(v00, v01, v10, s0, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und,
-                                   tvk.x);
+            raster_tri<TILE>(v00, v01, v10, s0, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und,
+                             tvk.x);
     }
     {
         const float3 v11 = vtx(1, 1);
         if (gate && v11.z > 0.f && v11.x > 0.f)
-            tv1 = raster_tri<TILE>(v11, v10, v01, s0 | 1u, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und,
-                                   tvk.y);
+            raster_tri<TILE>(v11, v10, v01, s0 | 1u, false, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und,
+                             tvk.y);
     }
     if (gate && g.wrap && j == 0) {
         const float3 v00 = vtx(0, 0);
@@ -1016,9 +1016,8 @@ __device__ __forceinline__ void raster_quad(const G& g, uint32_t ij, uint32_t e,
         raster_tri<TILE>(v10, l1, v00, s0 | 3u, true, tx0, ty0, out_w, out_h, g.rescale, key1, key2, und, 0.f);
     }
     if (C) {
-        C->tv[0][e] = tv0;
-        C->tv[1][e] = tv1;
         C->ij[e] = ij;
+        C->wi[e] = wi;
     }
 }
 
@@ -1158,17 +1157,21 @@ __global__ void __launch_bounds__(TILE * TILE) __attribute__((amdgpu_waves_per_e
                     const G& g = gv[s];
                     int i, j;
                     const float* vb = nullptr;
+                    uint32_t wk = 0xffffffffu;
                     if (cached) {
                         unpack_ij(cache.ij[e], i, j);
-                        if (slot < 2) vb = kept_block(ws, d, (uint32_t)(v * d.nsrc + s), i, j);
+                        if (slot < 2) wk = cache.wi[e];
+                        if (wk != 0xffffffffu) vb = ws.vbuf + (size_t)wk * (3 * kBinVerts);
                     }
                     if (vb) {
-                        // T1: v00 v01 v10; T2: v11 v10 v01 (corner offsets of slot_vertex)
+                        // T1: v00 v01 v10; T2: v11 v10 v01 (corner offsets of slot_vertex);
+                        // the validity the counting pass kept (the coverage pass used it)
                         const int li = i % kBinQY, lj = j % kBinQX;
                         const float3 a = kept_vertex(vb, li + slot, lj + slot);
                         const float3 b = kept_vertex(vb, li + slot, lj + 1 - slot);
                         const float3 cc = kept_vertex(vb, li + 1 - slot, lj + slot);
-                        resolve_cached(g, i, j, slot, a, b, cc, cache.tv[slot][e], px, py, c, dep, shape);
+                        const float2 tvk = kept_validity(ws, wk, i, j);
+                        resolve_cached(g, i, j, slot, a, b, cc, slot ? tvk.y : tvk.x, px, py, c, dep, shape);
                     } else {
                         const uint32_t ij = cached ? cache.ij[e]
                                                    : ws.lists[(size_t)ws.offsets[(size_t)T * d.nsrc + s] +

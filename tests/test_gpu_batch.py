@@ -291,18 +291,20 @@ def test_perspective_grid_sources(torch_cuda):
 
 
 @pytest.mark.parametrize("tile", [16, 32])
-def test_dense_sources_compact_bins(torch_cuda, tile):
-    """Sources far denser than the target (eleven 512x512 views into 64x36):
-    thousands of quads per (tile, source) bin (beyond the raster's LDS cache
-    of 1024 per phase), exact against the oracle; the entry list grows on
-    demand from a deliberately small capacity."""
+@pytest.mark.parametrize("size", [(64, 36), (32, 18)])
+def test_dense_sources_compact_bins(torch_cuda, tile, size):
+    """Sources far denser than the target (eleven 512x512 views into 64x36 or
+    32x18): thousands of quads per (tile, source) bin, at 32x18 one bin alone
+    beyond the raster's LDS cache of a phase (1536 quads per 16x16 tile area),
+    exact against the oracle; the entry list grows on demand from a
+    deliberately small capacity."""
     torch = torch_cuda
-    say("case dense bins, tile %d" % tile)
+    say("case dense bins, tile %d, %dx%d" % (tile, size[0], size[1]))
     rg = W.rig("persp11_2048", scale=4)
     planes = W.render_sources(rg)
     srcs = W.device_sources(rg, planes)
     t = rg["target"]
-    tgt = R.make_target(W.perspective_cam(64, 36), [0.01, 0.02, 0.0], [3.0, 1.0, 0.0], t["bitdepth"],
+    tgt = R.make_target(W.perspective_cam(*size), [0.01, 0.02, 0.0], [3.0, 1.0, 0.0], t["bitdepth"],
                         t["blending_exp"], R.scale_of(10), R.scale_of(16))
     R.profile_enable(True)
     try:
@@ -314,6 +316,8 @@ def test_dense_sources_compact_bins(torch_cuda, tile):
         R.profile_enable(False)
     say("  largest bin %d, entries %d, capacity grown to %d" % (cnt["max_bin"], cnt["bin_entries"], syn.entry_cap))
     assert cnt["max_bin"] > 1024 and syn.entry_cap >= cnt["bin_entries"]
+    if size == (32, 18):
+        assert cnt["max_bin"] > 1536 * (tile // 16) ** 2
     _check_oracle(out, _oracle(rg, planes, tgt), tgt.bitdepth, "dense")
 
 
