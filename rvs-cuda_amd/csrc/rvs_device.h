@@ -237,9 +237,32 @@ __device__ inline float3 color(const PairGeom& g, int i, int j) {
     return make_float3(r.x * g.cscale, r.y * g.cscale, r.z * g.cscale);
 }
 
+// Colours of three source pixels (a triangle's corners), the same values as
+// three color() calls: the resident plane's three loads are issued together
+// and then converted, so that a resolve waits on memory once, not per corner.
+__device__ inline void color3(const PairGeom& g, const int r[3], const int c[3], float3 out[3]) {
+    if (g.colq) {
+        ushort4 q[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) q[k] = g.colq[(size_t)r[k] * g.sw + c[k]];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            out[k] = make_float3((float)q[k].x * g.cscale, ((float)q[k].y * 0.25f) * g.cscale,
+                                 ((float)q[k].z * 0.25f) * g.cscale);
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) out[k] = color(g, r[k], c[k]);
+}
+
 __device__ inline float3 color(const ArrayGeom& g, int i, int j) {
     const size_t idx = (size_t)i * g.sw + j;
     return make_float3(g.color[3 * idx], g.color[3 * idx + 1], g.color[3 * idx + 2]);
+}
+
+__device__ inline void color3(const ArrayGeom& g, const int r[3], const int c[3], float3 out[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) out[k] = color(g, r[k], c[k]);
 }
 
 // ------------------------------------------------------------ triangles

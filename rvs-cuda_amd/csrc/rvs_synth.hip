@@ -841,14 +841,15 @@ template <bool FAST, class G>
 __device__ __forceinline__ bool resolve(const G& g, int i, int j, int slot, int x, int y, float3& c, float& dep,
                                float& shape, int out_w, int out_h, const float* vb) {
     int r[3], cc[3];
-    float3 v[3];
+    float3 v[3], col[3];
     bool und = false;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        slot_vertex(slot, k, i, j, g.sw, r[k], cc[k]);
+    for (int k = 0; k < 3; ++k) slot_vertex(slot, k, i, j, g.sw, r[k], cc[k]);
+    color3(g, r, cc, col);   // issued first: its loads overlap the vertices'
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
         v[k] = (vb && slot < 2) ? kept_vertex(vb, i % kBinQY + (r[k] - i), j % kBinQX + (cc[k] - j))
                                : vertex(g, r[k], cc[k], FAST ? &und : nullptr);
-    }
     Tri T;
     if (FAST) {
         tri_setup_geom(T, v[0], v[1], v[2], out_w, out_h);
@@ -860,7 +861,7 @@ __device__ __forceinline__ bool resolve(const G& g, int i, int j, int slot, int 
     tri_pixel(T, x, y, l1, l2, l3);
     dep = tri_depth(T, l1, l2, l3);
     shape = T.tv;
-    c = tri_color(color(g, r[0], cc[0]), color(g, r[1], cc[1]), color(g, r[2], cc[2]), l1, l2, l3);
+    c = tri_color(col[0], col[1], col[2], l1, l2, l3);
     return true;
 }
 
@@ -869,6 +870,11 @@ __device__ __forceinline__ bool resolve(const G& g, int i, int j, int slot, int 
 template <class G>
 __device__ __forceinline__ void resolve_cached(const G& g, int i, int j, int slot, float3 a, float3 b, float3 cv, float tv,
                                       int x, int y, float3& c, float& dep, float& shape) {
+    int r[3], cc[3];
+    float3 col[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) slot_vertex(slot, k, i, j, g.sw, r[k], cc[k]);
+    color3(g, r, cc, col);
     Tri T;
     tri_load(T, a, b, cv);
     T.tv = tv;
@@ -876,10 +882,7 @@ __device__ __forceinline__ void resolve_cached(const G& g, int i, int j, int slo
     tri_pixel(T, x, y, l1, l2, l3);
     dep = tri_depth(T, l1, l2, l3);
     shape = tv;
-    int r[3], cc[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) slot_vertex(slot, k, i, j, g.sw, r[k], cc[k]);
-    c = tri_color(color(g, r[0], cc[0]), color(g, r[1], cc[1]), color(g, r[2], cc[2]), l1, l2, l3);
+    c = tri_color(col[0], col[1], col[2], l1, l2, l3);
 }
 
 // key2 value of a pixel that must take the in-order replay (above every
