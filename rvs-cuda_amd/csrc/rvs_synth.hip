@@ -1088,6 +1088,7 @@ __global__ void __launch_bounds__(TILE * TILE) __attribute__((amdgpu_waves_per_e
     __shared__ unsigned long long key1[GRP][NT];
     __shared__ uint32_t key2[GRP][NT];
     __shared__ uint32_t pref[GRP + 1];
+    __shared__ uint32_t boff[GRP];   // list offsets of the phase's bins
     __shared__ int s_deferred;   // any pixel of the tile deferred
     __shared__ int s_hole;       // any pixel of the tile a hole (quality not > 0)
     // one phase: a smaller cache keeps a 16 x 16 block within 20 KB of LDS (8 blocks per CU)
@@ -1118,6 +1119,7 @@ __global__ void __launch_bounds__(TILE * TILE) __attribute__((amdgpu_waves_per_e
             uint32_t acc = 0;
             for (int k = 0; k < gs; ++k) {
                 pref[k] = acc;
+                boff[k] = ws.offsets[(size_t)T * d.nsrc + s0 + k];
                 acc += ws.counts[(size_t)T * d.nsrc + s0 + k];
             }
             pref[gs] = acc;
@@ -1132,7 +1134,7 @@ __global__ void __launch_bounds__(TILE * TILE) __attribute__((amdgpu_waves_per_e
             if (e >= N) continue;
             int k = 0;
             while (k + 1 < gs && e >= pref[k + 1]) ++k;
-            const uint32_t ij = ws.lists[(size_t)ws.offsets[(size_t)T * d.nsrc + s0 + k] + (e - pref[k])];
+            const uint32_t ij = ws.lists[(size_t)boff[k] + (e - pref[k])];
             raster_quad<TILE>(gv[s0 + k], ij, e, tx0, ty0, vd.w, vd.h, key1[k], key2[k], cached ? &cache : nullptr,
                               ws, d, (uint32_t)(v * d.nsrc + s0 + k));
         }
@@ -1176,9 +1178,7 @@ __global__ void __launch_bounds__(TILE * TILE) __attribute__((amdgpu_waves_per_e
                         const float2 tvk = kept_validity(ws, wk, i, j);
                         resolve_cached(g, i, j, slot, a, b, cc, slot ? tvk.y : tvk.x, px, py, c, dep, shape);
                     } else {
-                        const uint32_t ij = cached ? cache.ij[e]
-                                                   : ws.lists[(size_t)ws.offsets[(size_t)T * d.nsrc + s] +
-                                                              (e - pref[k])];
+                        const uint32_t ij = cached ? cache.ij[e] : ws.lists[(size_t)boff[k] + (e - pref[k])];
                         unpack_ij(ij, i, j);
                         if (!resolve<true>(g, i, j, slot, px, py, c, dep, shape, vd.w, vd.h,
                                            kept_block(ws, d, (uint32_t)(v * d.nsrc + s), i, j))) {
