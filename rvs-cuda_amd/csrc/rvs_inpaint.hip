@@ -837,33 +837,54 @@ __device__ inline unsigned long long load_slot(const unsigned long long* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Wave-wide inclusive scan of Fn with DPP row shifts and row broadcasts
-// (lanes without a source take the identity, which compose() leaves alone);
-// ex: the composite of the lanes before this one (identity for lane 0).
-__device__ inline Fn fn_dpp_ident() {
-    Fn f;
-    f.cv = 1 << 30; f.cs = -1; f.t = 0; f.k = 0;
-    return f;
+// The strip solver's wave scan of the line functions (Fn), in two words per
+// lane: p = 2 cv + t (t is 0 or 1) and cs; the k of every composite is its
+// number of scan positions (each lane's function covers kStripLanePx of them,
+// each of k 1), known from the lane, so k is not shuffled.  compose_p is
+// compose() on that form: cv1 + k2 < cv2 + t2 holds iff 2 (cv1 + k2) <
+// p2 + t2 (the left side is even), and the composite's p is then
+// 2 (cv1 + k2) + t1, else p2; apply_p is apply_fn() likewise.  Distances stay
+// below kAbsent + 2^20 (line lengths and counts are far below 2^20), so the
+// doubled values stay below 2^30 + 2^21, under the identity's p (1.5 * 2^30,
+// which leaves room for 2 k below 2^31).  Lanes whose DPP source lies outside
+// the shift take the identity, which compose_p leaves alone.
+constexpr int kPackIdent = 0x60000000;
+
+__device__ inline void compose_p(int p1, int cs1, int& p2, int& cs2, int k2) {
+    const int lhs = (p1 & ~1) + 2 * k2;
+    const bool a = lhs < p2 + (p2 & 1);
+    p2 = a ? lhs + (p1 & 1) : p2;
+    cs2 = a ? cs1 : cs2;
 }
+
+__device__ inline void apply_p(int p, int cs, int k, int& v, int& s) {
+    const bool a = 2 * (v + k) < p + (p & 1);
+    s = a ? s : cs;
+    v = a ? v + k : (p >> 1);
+}
+
 template <int CTRL, int ROWS>
-__device__ inline Fn fn_dpp(const Fn& a) {
-    const Fn id = fn_dpp_ident();
-    Fn r;
-    r.cv = __builtin_amdgcn_update_dpp(id.cv, a.cv, CTRL, ROWS, 0xf, false);
-    r.cs = __builtin_amdgcn_update_dpp(id.cs, a.cs, CTRL, ROWS, 0xf, false);
-    r.t = __builtin_amdgcn_update_dpp(id.t, a.t, CTRL, ROWS, 0xf, false);
-    r.k = __builtin_amdgcn_update_dpp(id.k, a.k, CTRL, ROWS, 0xf, false);
-    return r;
+__device__ inline void scan_level_p(int& p, int& cs, int k2) {
+    const int sp = __builtin_amdgcn_update_dpp(kPackIdent, p, CTRL, ROWS, 0xf, false);
+    const int scs = __builtin_amdgcn_update_dpp(-1, cs, CTRL, ROWS, 0xf, false);
+    compose_p(sp, scs, p, cs, k2);
 }
-__device__ inline Fn wave_scan_fn(Fn A, Fn& ex) {
-    A = compose(fn_dpp<0x111, 0xf>(A), A);   // row_shr:1
-    A = compose(fn_dpp<0x112, 0xf>(A), A);   // row_shr:2
-    A = compose(fn_dpp<0x114, 0xf>(A), A);   // row_shr:4
-    A = compose(fn_dpp<0x118, 0xf>(A), A);   // row_shr:8
-    A = compose(fn_dpp<0x142, 0xa>(A), A);   // row_bcast:15 into rows 1, 3
-    A = compose(fn_dpp<0x143, 0xc>(A), A);   // row_bcast:31 into rows 2, 3
-    ex = fn_dpp<0x138, 0xf>(A);              // wave_shr:1
-    return A;
+
+// Inclusive scan of (p, cs) over the wave: DPP row shifts by 1, 2, 4, 8, then
+// row broadcasts, each level composing with the number of positions this
+// lane's composite covers before it; ep, ecs: the composite of the lanes
+// before this one (the identity for lane 0), which covers PX * lane positions.
+template <int PX>
+__device__ inline void wave_scan_p(int& p, int& cs, int& ep, int& ecs) {
+    const int lane = __lane_id(), r = lane & 15, row = lane >> 4;
+    scan_level_p<0x111, 0xf>(p, cs, PX);                           // row_shr:1
+    scan_level_p<0x112, 0xf>(p, cs, PX * min(2, r + 1));           // row_shr:2
+    scan_level_p<0x114, 0xf>(p, cs, PX * min(4, r + 1));           // row_shr:4
+    scan_level_p<0x118, 0xf>(p, cs, PX * min(8, r + 1));           // row_shr:8
+    scan_level_p<0x142, 0xa>(p, cs, PX * (r + 1));                 // row_bcast:15 into rows 1, 3
+    scan_level_p<0x143, 0xc>(p, cs, PX * (r + 1 + 16 * (row & 1)));   // row_bcast:31 into rows 2, 3
+    ep = __builtin_amdgcn_update_dpp(kPackIdent, p, 0x138, 0xf, 0xf, false);   // wave_shr:1
+    ecs = __builtin_amdgcn_update_dpp(-1, cs, 0x138, 0xf, 0xf, false);
 }
 
 // Strip s of component R in one sweep: scan positions [s P, min(llen,
@@ -1006,9 +1027,9 @@ __device__ void solve_strip(const Region& R, const InpaintWS& ws, bool cols, boo
         Fn A = f[0];
 #pragma unroll
         for (int j = 1; j < PX; ++j) A = compose(A, f[j]);
-        Fn ex;
-        wave_scan_fn(A, ex);
-        apply_fn(ex, Sv, Ss);   // the identity for lane 0
+        int ap = 2 * A.cv + A.t, acs = A.cs, ep, ecs;
+        wave_scan_p<PX>(ap, acs, ep, ecs);
+        apply_p(ep, ecs, PX * lane, Sv, Ss);   // the identity for lane 0
 #pragma unroll
         for (int j = 0; j < PX; ++j) {
             apply_fn(f[j], Sv, Ss);
