@@ -1116,12 +1116,22 @@ __global__ void __launch_bounds__(TILE * TILE) __attribute__((amdgpu_waves_per_e
         }
         if (tid == 0) {
             if (s0 == 0) { s_deferred = 0; s_hole = 0; }
-            uint32_t acc = 0;
-            for (int k = 0; k < gs; ++k) {
-                pref[k] = acc;
-                boff[k] = ws.offsets[(size_t)T * d.nsrc + s0 + k];
-                acc += ws.counts[(size_t)T * d.nsrc + s0 + k];
+            // the phase's counts and offsets loaded together, then summed
+            uint32_t cnt[GRP], off[GRP];
+#pragma unroll
+            for (int k = 0; k < GRP; ++k) {
+                const size_t b = (size_t)T * d.nsrc + s0 + (k < gs ? k : 0);
+                cnt[k] = ws.counts[b];
+                off[k] = ws.offsets[b];
             }
+            uint32_t acc = 0;
+#pragma unroll
+            for (int k = 0; k < GRP; ++k)
+                if (k < gs) {
+                    pref[k] = acc;
+                    boff[k] = off[k];
+                    acc += cnt[k];
+                }
             pref[gs] = acc;
         }
         __syncthreads();
