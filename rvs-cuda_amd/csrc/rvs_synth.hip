@@ -14,10 +14,10 @@
 //      from k_src_bounds and are tested with error margins); the rest form a
 //      work list.
 //   1. k_bin (counting pass, scan, fill pass): every surviving source quad
-//      is appended to the 16x16 target tiles its triangles' (reference-exact)
-//      bounding boxes touch; the counting pass keeps each work item's vertex
-//      block for the raster.
-//   2. k_raster: one 256-thread block per target tile, four sources per
+//      is appended to the 16x16 or 32x32 target tiles its triangles' narrowed
+//      (reference-exact) bounding boxes touch; the counting pass keeps each
+//      work item's vertex block and validities for the raster.
+//   2. k_raster: one 256- or 1024-thread block per target tile, four sources per
 //      phase (one phase, 8 waves per SIMD, for launch sets of up to four
 //      sources).  Every covered pixel takes an LDS 64-bit atomicMax, key1 = max
 //      of (score bits << 32 | candidate id), and a 32-bit one, key2 = max
@@ -27,11 +27,14 @@
 //      or extreme values (key2 = all ones), is deferred.  Every other pixel
 //      provably ends the sequential fold on the argmax (the float test errs
 //      by less than 2^-21 relative), so its winner is re-evaluated -- from
-//      the kept vertices and the validity the coverage pass left in LDS --
-//      and blended.
-//   3. k_fallback: deferred pixels replay the reference fold literally, in
-//      traversal order (per-pixel candidate lists gathered in LDS, or a sort
-//      of the tile's bins when a list overflows).
+//      the kept vertices and validity of the work item the coverage pass
+//      left in LDS -- and blended.
+//   3. k_fallback_src / k_fallback_blend / k_fallback: deferred pixels replay
+//      the reference fold literally, in traversal order (per-pixel candidate
+//      lists gathered in LDS, or a sort of the tile's bins when a list
+//      overflows), each source of a tile in its own block and then blended,
+//      or (beyond the workspace for that) a block per tile folding its
+//      sources in turn.
 // Blending across sources (BlendedViewSimple::blendBasicMode) runs per pixel in
 // registers inside the same block, in InputCameraNames order.
 #include <hip/hip_runtime.h>
