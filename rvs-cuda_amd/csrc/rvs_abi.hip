@@ -132,7 +132,7 @@ struct Layout {
 // deferred-pixel flags, deferred-tile list and binning work list.
 struct RasterPlan {
     rvs_k::BatchDims d;
-    size_t status, counts, offsets, fill, lists, ranges, vbuf, tvq, imap, bsum, deferred, dtiles, work, tile_enc,
+    size_t status, counts, offsets, fill, lists, ranges, ktab, vbuf, tvq, imap, bsum, deferred, dtiles, work, tile_enc,
         rbuf;
 };
 
@@ -189,6 +189,7 @@ void plan_raster(Layout& L, RasterPlan& R, int nsrc, int nviews, uint32_t ntiles
     R.bsum = L.take(rvs_k::scan_blocks(nbins) * sizeof(unsigned long long));
     R.lists = L.take((size_t)d.cap * sizeof(uint32_t));
     R.ranges = L.take((size_t)d.range_cap * 256 * sizeof(uint32_t));
+    R.ktab = L.take((size_t)d.range_cap * (1 + rvs_k::kBinTable) * sizeof(uint2));
     d.kept_geom = 1;
     if (const char* env = std::getenv("RVS_KEPT_GEOM")) d.kept_geom = std::strtol(env, nullptr, 10) != 0;
     const size_t kg = d.kept_geom ? d.range_cap : 0;
@@ -224,6 +225,7 @@ rvs_k::RasterWS carve_raster(char* base, const RasterPlan& R) {
     ws.bsum = (unsigned long long*)(base + R.bsum);
     ws.lists = (uint32_t*)(base + R.lists);
     ws.ranges = (uint32_t*)(base + R.ranges);
+    ws.ktab = (uint2*)(base + R.ktab);
     ws.vbuf = (float*)(base + R.vbuf);
     ws.tvq = (float2*)(base + R.tvq);
     ws.imap = (uint32_t*)(base + R.imap);

@@ -22,6 +22,7 @@ constexpr int kTile = 16;            // default target tile edge (pixels), and t
 constexpr int kTileBig = 32;         // the other tile edge a launch set can use (BatchDims::tile_shift)
 constexpr int kBinQX = 64;           // binning work item: 64 x 4 source quads
 constexpr int kBinQY = 4;
+constexpr int kBinTable = 128;       // LDS bin table entries per binning block (power of two)
 constexpr int kFallbackChunk = 128;  // list entries per LDS chunk in the in-order replay
 constexpr int kFallbackSortCap = 8192;  // longest (source, tile) list the replay sorts in LDS (longer
                                         // ones are sorted in place in the entry list)
@@ -70,6 +71,8 @@ struct RasterWS {
     uint32_t* fill;          // [global tile][source] fill cursors
     uint32_t* lists;         // [cap] quad codes (row << 16 | column), bins back to back
     uint32_t* ranges;        // [range_cap work items][256] packed tile range of each quad
+    uint2* ktab;             // [range_cap work items][1 + kBinTable]: the counting pass's LDS bin
+                             // table of the item, [0].x = n used slots, then n of (slot | count << 8, tile)
     float* vbuf;             // [range_cap work items][kBinQY + 1][kBinQX + 1][3] vertices (u', v', depth)
     float2* tvq;             // [range_cap work items][256] validity of triangles T1, T2 of each quad
     uint32_t* imap;          // [view][source][item] work index of a kept item, ~0 otherwise

@@ -416,7 +416,6 @@ __device__ inline float2 kept_validity(const RasterWS& ws, uint32_t wi, int i, i
     return ws.tvq[(size_t)wi * 256 + (uint32_t)((i % kBinQY) * kBinQX + (j % kBinQX))];
 }
 
-constexpr int kBinTable = 128;                 // LDS bin table entries per block (power of two)
 constexpr uint32_t kTableEmpty = 0xffffffffu;
 
 // Slot of global tile t in the block's LDS table, inserting it; -1 when the
@@ -525,6 +524,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
     __shared__ uint32_t t_n[kBinTable];      // entries per bin
     __shared__ uint32_t t_base[kBinTable];   // fill write cursor per bin (fill pass)
     __shared__ RangeWalk walk;
+    __shared__ uint32_t t_used;              // used table slots (kept items)
     const int tid = threadIdx.x;
     const uint32_t per = d.bins_x * d.bins_y;
     if (FILL && ws.status[1]) return;   // list capacity exceeded: nothing to fill
@@ -621,6 +621,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
             if (FILL) t_base[tid] = ws.offsets[bin] + atomicAdd(&ws.fill[bin], t_n[tid]);
             else atomicAdd(&ws.counts[bin], t_n[tid]);
         }
+        if (stored) {
+            // the item's table for its fill pass: the used slots (slot, count,
+            // tile) in any order, their number first
+            if (tid == 0) t_used = 0u;
+            __syncthreads();
+            uint2* kt = ws.ktab + (size_t)wi * (1 + kBinTable);
+            if (tid < kBinTable && t_n[tid])
+                kt[1 + atomicAdd(&t_used, 1u)] = make_uint2((uint32_t)tid | (t_n[tid] << 8), t_key[tid]);
+            __syncthreads();
+            if (tid == 0) kt[0] = make_uint2(t_used, 0u);
+        }
         if (FILL) {
             __syncthreads();   // bases known
             each_tile([&](int lane, int tx, int ty) {
@@ -634,13 +645,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
 
 // Fill pass over the kept work items, a block per item (striding when there
 // are more items than blocks) and a thread per quad: entries take slots
-// through an LDS bin table and one fill-cursor atomic per bin.  A quad whose
-// range was too wide to pack is re-binned from the kept vertices.
+// through the item's LDS bin table, rebuilt from what the counting pass
+// stored, and one fill-cursor atomic per bin.  A quad whose range was too
+// wide to pack is re-binned from the kept vertices.
 template <class G>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_fill_kept(const G* __restrict__ geoms, const ViewDesc* __restrict__ views,
                                                    BatchDims d, RasterWS ws) {
     __shared__ uint32_t t_key[kBinTable];
-    __shared__ uint32_t t_n[kBinTable];
     __shared__ uint32_t t_base[kBinTable];
     __shared__ RangeWalk walk;
     if (ws.status[1]) return;   // list capacity exceeded: nothing to fill
@@ -648,10 +659,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
     const int tid = threadIdx.x;
     const uint32_t per = d.bins_x * d.bins_y;
     for (uint32_t wi = blockIdx.x; wi < lim; wi += gridDim.x) {
-        if (tid < kBinTable) {
-            t_key[tid] = kTableEmpty;
-            t_n[tid] = 0;
-        }
+        if (tid < kBinTable) t_key[tid] = kTableEmpty;
         const uint32_t code = ws.work[wi];
         const uint32_t item = code % per, vs = code / per;
         const int s = (int)(vs % (uint32_t)d.nsrc), v = (int)(vs / (uint32_t)d.nsrc);
@@ -695,25 +703,31 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
                     for (int tx = tx0; tx <= tx1; ++tx) fn(tid, tx, ty);
             if (any_wide) range_walk(walk, tid, fn);
         };
+        // the counting pass's table of this item, at the same slots (so
+        // find_slot sees the same probe sequences), with one fill-cursor
+        // atomic per used slot; a tile that found the table full there is
+        // absent here too and takes a slot of its own per entry
+        {
+            const uint2* kt = ws.ktab + (size_t)wi * (1 + kBinTable);
+            const uint32_t nused = kt[0].x;
+            if (tid < (int)nused) {
+                const uint2 e = kt[1 + tid];
+                const uint32_t slot = e.x & 0xffu, n = e.x >> 8;
+                t_key[slot] = e.y;
+                const size_t bin = (size_t)e.y * d.nsrc + s;
+                t_base[slot] = ws.offsets[bin] + atomicAdd(&ws.fill[bin], n);
+            }
+        }
+        __syncthreads();   // table and bases known
         each_tile([&](int lane, int tx, int ty) {
             const uint32_t t = tile_of(tx, ty);
-            const int k = table_slot(t_key, t);
+            const int k = find_slot(t_key, t);
             if (k >= 0) {
-                atomicAdd(&t_n[k], 1u);
-            } else {   // table full: a slot of its own
+                ws.lists[atomicAdd(&t_base[k], 1u)] = quad_of(lane);
+            } else {
                 const size_t bin = (size_t)t * d.nsrc + s;
                 ws.lists[(size_t)ws.offsets[bin] + atomicAdd(&ws.fill[bin], 1u)] = quad_of(lane);
             }
-        });
-        __syncthreads();   // table complete
-        if (tid < kBinTable && t_n[tid]) {
-            const size_t bin = (size_t)t_key[tid] * d.nsrc + s;
-            t_base[tid] = ws.offsets[bin] + atomicAdd(&ws.fill[bin], t_n[tid]);
-        }
-        __syncthreads();   // bases known
-        each_tile([&](int lane, int tx, int ty) {
-            const int k = find_slot(t_key, tile_of(tx, ty));
-            if (k >= 0) ws.lists[atomicAdd(&t_base[k], 1u)] = quad_of(lane);
         });
         __syncthreads();   // table free for the next item
     }
