@@ -574,7 +574,8 @@ int scratch_address(void** out) {
 
 // Kept work items fitted to the static scratch: the plan with a single kept
 // item gives the fixed part; what remains holds this many kept items (their
-// tile ranges and vertex windows).  0 when even the fixed part does not fit.
+// tile ranges, bin tables, vertex windows and validities).  0 when even the
+// fixed part does not fit.
 uint64_t noalloc_keep(const rvs_source* srcs, int n_src, const rvs_target* tgt, uint32_t cap, size_t& need) {
     SynthPlan P;
     if (plan_synth(srcs, n_src, tgt, 1, cap, P, nullptr) != RVS_OK) return 0;
@@ -582,9 +583,10 @@ uint64_t noalloc_keep(const rvs_source* srcs, int n_src, const rvs_target* tgt, 
     if (plan_synth(srcs, n_src, tgt, 1, cap, P, nullptr, 1) != RVS_OK) return 0;
     need = P.total;
     if (P.total > kNoallocScratchBytes) return 0;
-    const size_t per_item = 256 * sizeof(uint32_t) + 3 * (size_t)rvs_k::kBinVerts * sizeof(float) + 256 * sizeof(float2);
+    const size_t per_item = 256 * sizeof(uint32_t) + (1 + rvs_k::kBinTable) * sizeof(uint2) +
+                            3 * (size_t)rvs_k::kBinVerts * sizeof(float) + 256 * sizeof(float2);
     const uint64_t keep = 1 + (kNoallocScratchBytes - P.total) / per_item;
-    // alignment of the three kept arrays can cost up to 3 x 255 bytes: step back until it fits
+    // alignment of the four kept arrays can cost up to 4 x 255 bytes: step back until it fits
     for (uint64_t k = keep; k >= 1; k = k > 64 ? k - 64 : k - 1) {
         if (plan_synth(srcs, n_src, tgt, 1, cap, P, nullptr, k) == RVS_OK && P.total <= kNoallocScratchBytes) {
             need = P.total;
